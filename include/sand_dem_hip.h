@@ -1,0 +1,125 @@
+/*
+ * sand_dem_hip.h -- C ABI of the MI355X `hip` backend for the surface-mesh DEM step.
+ *
+ * The boundary mirrors the reference's native CPU backend ABI
+ * (src/sand_atlas_dem/cpu.py:288-293, generated `cpu_dispatch`):
+ *
+ *     int cpu_dispatch(uint32_t count, void** buffers, uint64_t* bytes,
+ *                      double* values[, void* scratch]);
+ *
+ * Buffers are bound by position in the reference's `Resources` field order
+ * (src/sand_atlas_dem/shaders/solver.slang:20-100, minus the `c` config block);
+ * `values` carries the `Config` scalars with vector fields expanded per axis
+ * (solver.slang:12-19; cpu.py:217-228) followed by a thread-count slot.  All
+ * pointers here are device pointers (HBM) owned by the caller; the library keeps
+ * only its scratch workspace.  Every entry point returns 0 on success and a
+ * nonzero SAD_E_* code on a contract or HIP runtime failure; physics failures are
+ * reported, as in the reference, by bits OR-ed into control[0] (gpu.py:11-26).
+ * No C++ exception crosses this boundary.
+ */
+#ifndef SAND_DEM_HIP_H
+#define SAND_DEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Buffer slots: position = field index in solver.slang `Resources` (lines 22-98). */
+enum sad_buffer {
+    SAD_CONTROL = 0, SAD_CELL, SAD_POSITION, SAD_POSITION_REMAINDER,
+    SAD_VELOCITY_REMAINDER, SAD_MOMENTUM_REMAINDER, SAD_ROTATION_REMAINDER,
+    SAD_REFERENCE_POSITION_REMAINDER, SAD_CELL_LOG, SAD_ACCUMULATED_REMAINDER,
+    SAD_VELOCITY, SAD_ROTATION, SAD_MOMENTUM, SAD_FORCE, SAD_TORQUE,
+    SAD_REFERENCE_POSITION, SAD_REFERENCE_CELL, SAD_SHAPE_ID, SAD_BODY, SAD_INV_I,
+    SAD_GRIDS, SAD_SAMPLE_INFO, SAD_GRID_ORIGIN, SAD_DISTANCE, SAD_SAMPLE,
+    SAD_MESH_ROOTS, SAD_MESH_HULL_INFO, SAD_MESH_PLANES, SAD_MESH_MIN, SAD_MESH_MAX,
+    SAD_MESH_LINKS, SAD_MESH_TRIANGLES, SAD_MESH_NORMALS, SAD_MESH_FRONTIER,
+    SAD_MESH_PAIR_MASK, SAD_MESH_PAIR_TRANSFORM, SAD_MESH_CLUSTER_INFO,
+    SAD_MESH_CLUSTER_RANGES, SAD_MESH_CLUSTER_MIN, SAD_MESH_CLUSTER_MAX,
+    SAD_MESH_SAMPLE_INDICES, SAD_BIN_COUNTS, SAD_BIN_PARTICLES, SAD_PAIR_COUNTS,
+    SAD_PAIR_OFFSETS, SAD_PAIRS, SAD_CONTACT_COUNTS, SAD_CONTACT_OFFSETS, SAD_META,
+    SAD_POINT_DEPTH, SAD_NORMAL_AREA, SAD_SLIP, SAD_CONTACT_FORCE,
+    SAD_NORMAL_LOSS_POWER, SAD_NORMAL_DAMPING, SAD_FORCE_VELOCITY, SAD_FORCE_OMEGA,
+    SAD_OLD_META, SAD_OLD_SLIP, SAD_OLD_NORMAL, SAD_OLD_HASH, SAD_HASH, SAD_HEADS,
+    SAD_NEXT, SAD_PARTICLE_STRESS, SAD_TIMESTEP_ROWS, SAD_PARTICLE_DIAGNOSTIC,
+    SAD_DIAGNOSTIC_ROWS, SAD_ACCUMULATED, SAD_FABRIC_ROWS, SAD_WALL_PLANE,
+    SAD_WALL_REMAINDER, SAD_WALL_VELOCITY, SAD_WALL_MATERIAL, SAD_WALL_BOX_IDS,
+    SAD_WALL_ROWS, SAD_WALL_ACCUMULATED,
+    SAD_BUFFER_COUNT /* = 77 */
+};
+
+/* values[] slots: Config fields (solver.slang:12-19) expanded per axis. */
+enum sad_value {
+    SAD_V_ACTIVE_TIMESTEP = 0, SAD_V_N, SAD_V_SHAPES, SAD_V_PAIR_CAPACITY,
+    SAD_V_CONTACT_CAPACITY, SAD_V_BIN_CAPACITY, SAD_V_BIN_COUNT, SAD_V_BINS_X,
+    SAD_V_BINS_Y, SAD_V_BINS_Z, SAD_V_HASH_SIZE, SAD_V_DT, SAD_V_KN, SAD_V_KT,
+    SAD_V_MU, SAD_V_NORMAL_VISCOSITY, SAD_V_SKIN, SAD_V_DRAG, SAD_V_MAX_OVERLAP,
+    SAD_V_HISTORY_STEP, SAD_V_MESH_GEOMETRY, SAD_V_PERIODIC, SAD_V_WALL_COUNT,
+    SAD_V_WALL_BOX_ENABLED, SAD_V_SOURCE_SURFACE_NORMAL,
+    SAD_V_THREADS, /* reference cpu.py:419 appends the worker count here */
+    SAD_VALUE_COUNT /* = 26 */
+};
+
+/* History side slots for sad_hip_advance (gpu.py:305-316 double buffer). */
+enum sad_side { SAD_SIDE_META = 0, SAD_SIDE_SLIP, SAD_SIDE_NORMAL_AREA, SAD_SIDE_HASH,
+                SAD_SIDE_COUNT };
+
+/* Return codes. */
+#define SAD_OK 0
+#define SAD_E_HIP 1          /* HIP runtime error; sad_hip_last_error() names it */
+#define SAD_E_ARGUMENT 2     /* contract violation: size, null pointer, unknown kernel */
+#define SAD_E_UNSUPPORTED 3  /* configuration outside the mesh path (SDF, walls, ...) */
+
+/* Device workspace owned by the library (replaces the reference's collector
+ * scratch object, cpu_parallel.hpp:3-25 / cpu.py:364-376).  Holds staging,
+ * per-task counts, scan partials and the per-grain pair adjacency; grows on
+ * demand, never shrinks.  Returns NULL on failure. */
+void* sad_hip_scratch_create(void);
+void sad_hip_scratch_destroy(void* scratch);
+/* stats[0..7]: tasks, queried nodes, staged contacts, scratch bytes, rebuilds,
+ * culled nodes (learned cull), audited nodes, audit misses. */
+int sad_hip_scratch_stats(void* scratch, uint64_t* stats);
+
+/* One reference kernel by name, `count` threads as the reference dispatches it
+ * (gpu.py:529-565).  Replaces cpu_dispatch (cpu.py:391-428) / Metal dispatch
+ * (backend.py:62-83).  `stream` is a hipStream_t (NULL = default stream).
+ * Supported names: beginStep advanceCell checkRebuild clearBins fillBins
+ * countPairs pairScan emitPairs resetMeshContacts prepareMeshPairs
+ * collectMeshFrontiers clearContactStorage resolveContacts buildHistoryHash
+ * gather finishStep captureContactKinematics checkContactTimestep. */
+int sad_hip_dispatch(const char* kernel, uint32_t count, void* const* buffers,
+                     const uint64_t* bytes, const double* values, void* scratch,
+                     void* stream);
+
+/* `steps` explicit timesteps, the reference's per-step encoder sequence
+ * (gpu.py:714-726) on device, with the history sides swapped every step:
+ * step k writes side[(active + k) % 2] and reads the other.  `sides` holds
+ * 2*SAD_SIDE_COUNT device pointers (side 0 then side 1); `buffers` entries for
+ * meta/slip/normalArea/hash and old* are ignored.  values[SAD_V_HISTORY_STEP]
+ * applies to the first step; later steps always carry history (gpu.py:726).
+ * `graph` != 0 replays a captured hipGraph of the step pair when possible. */
+int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers,
+                    const uint64_t* bytes, const double* values,
+                    void* const* sides, const uint64_t* side_bytes,
+                    void* scratch, void* stream, int graph);
+
+/* Probes over the reference's query functions (solver.slang:253-257, 306-318,
+ * 418-424): meshProbe (float4 points -> float4 normal,signed distance),
+ * meshCullProbe (float4 points -> uint), meshClusterProbe (src,dst -> uint per
+ * cluster), meshPairBoundsProbe (direction, first -> 2 float4 per cluster).
+ * uniforms[] = {count, shape|src|direction, dst|first}. */
+int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buffers,
+                  const uint64_t* bytes, const double* values, const void* input,
+                  void* output, void* stream);
+
+/* Name of the last HIP error seen by this library (static string). */
+const char* sad_hip_last_error(void);
+/* ABI version, currently 1. */
+int sad_hip_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SAND_DEM_HIP_H */

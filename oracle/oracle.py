@@ -1,0 +1,177 @@
+"""ORACLE adapter -- test infrastructure only.
+
+Only tests/, `__graft_entry__.smoke()` and bench.py's `cpu_baseline` leg import this
+module, as the checker.  It exposes the C++ CPU restatement (`cpu_backend.cpp`) and,
+when built, the reference's own generated CPU collector (`_ref/ref_collector.so`)
+through the same duck-typed adapter protocol as the reference CPU backend
+(`src/sand_atlas_dem/cpu.py:431-689`): host NumPy buffers, encoders that record
+operations, serial execution at submit.
+"""
+
+import ctypes as ct
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIBRARY = HERE / "liboracle.so"
+REF_COLLECTOR = HERE / "_ref" / "ref_collector.so"
+
+
+def build(reference=True):
+    """Compile the oracle (and the reference collector when the reference exists)."""
+    targets = ["liboracle.so"] + (["ref"] if reference else [])
+    subprocess.run(["make", "-s", "-C", str(HERE), *targets], check=True)
+
+
+def _load(path, symbol, argtypes):
+    lib = ct.CDLL(str(path))
+    fn = getattr(lib, symbol)
+    fn.argtypes, fn.restype = argtypes, ct.c_int
+    return lib, fn
+
+
+class Buffer:
+    def __init__(self, array=None, *, count=None, stride=4):
+        self.array = (
+            np.ascontiguousarray(array).copy().view(np.uint8).reshape(-1)
+            if array is not None
+            else np.zeros(max(4, count * stride), np.uint8)
+        )
+        self.size = self.array.nbytes
+
+    def copy_from_numpy(self, array):
+        data = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
+        if data.size != self.size:
+            raise ValueError("CPU buffer upload size mismatch")
+        self.array[:] = data
+
+    def to_numpy(self):
+        return self.array.copy()
+
+
+class Encoder:
+    def __init__(self):
+        self.operations = []
+
+    def copy_buffer(self, dest, dest_offset, source, source_offset, size):
+        if min(dest_offset, source_offset, size) < 0 or dest_offset + size > dest.size or source_offset + size > source.size:
+            raise ValueError("CPU buffer copy out of bounds")
+        self.operations.append(
+            lambda: np.copyto(dest.array[dest_offset : dest_offset + size], source.array[source_offset : source_offset + size])
+        )
+
+
+def pointer_table(bindings, order):
+    arrays = [bindings[name].array for name in order]
+    pointers = (ct.c_void_p * len(arrays))(*(a.ctypes.data for a in arrays))
+    sizes = (ct.c_uint64 * len(arrays))(*(a.nbytes for a in arrays))
+    return pointers, sizes
+
+
+def make_oracle(package, threads=1, collector="oracle"):
+    """Create the oracle adapter bound to the package's ABI tables.
+
+    collector="reference" routes collectMeshFrontiers through the reference's own
+    generated collector (`_ref/ref_collector.so`, cpu_dispatch ABI), everything else
+    through the restatement.
+    """
+    order = package.engine.BUFFER_ORDER
+    config_values = package.engine.config_values
+    if not LIBRARY.exists():
+        build(reference=False)
+    lib, dispatch_fn = _load(
+        LIBRARY, "oracle_dispatch",
+        [ct.c_char_p, ct.c_uint32, ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_double)],
+    )
+    probe_fn = lib.oracle_probe
+    probe_fn.argtypes = [
+        ct.c_char_p, ct.POINTER(ct.c_uint32), ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_uint64),
+        ct.POINTER(ct.c_double), ct.c_void_p, ct.c_void_p,
+    ]
+    probe_fn.restype = ct.c_int
+    ref_fn = None
+    if collector == "reference":
+        if not REF_COLLECTOR.exists():
+            raise FileNotFoundError("reference collector not built (oracle/_ref)")
+        ref_lib, ref_fn = _load(
+            REF_COLLECTOR, "cpu_dispatch",
+            [ct.c_uint32, ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_double), ct.c_void_p],
+        )
+        create = ref_lib.cpu_scratch_create
+        create.restype = ct.c_void_p
+        ref_scratch = create()
+
+    class Oracle:
+        name = "cpu"
+        buffer_type = Buffer
+
+        def __init__(self):
+            self.threads = threads
+            self.device = self
+            self.calls = 0
+
+        def buffer(self, array=None, *, count=None, stride=4):
+            return Buffer(array, count=count, stride=stride)
+
+        def create_command_encoder(self):
+            return Encoder()
+
+        def dispatch(self, name, n, encoder=None, *, r):
+            def execute():
+                pointers, sizes = pointer_table(r, order)
+                values = config_values(r["c"], self.threads)
+                vals = values.ctypes.data_as(ct.POINTER(ct.c_double))
+                if name == "collectMeshFrontiers" and ref_fn is not None:
+                    control = r["control"].array.view(np.uint32)
+                    count = min(int(control[1]), r["c"]["pairCapacity"])
+                    status = ref_fn(count, pointers, sizes, vals, ref_scratch)
+                else:
+                    status = dispatch_fn(name.encode(), int(n), pointers, sizes, vals)
+                self.calls += 1
+                if status:
+                    raise RuntimeError(f"oracle dispatch {name} failed with status {status}")
+
+            if encoder is None:
+                execute()
+            else:
+                encoder.operations.append(execute)
+
+        def probe(self, name, uniforms, bindings, data, out):
+            pointers, sizes = pointer_table(bindings, order)
+            values = config_values(bindings["c"], self.threads)
+            u = (ct.c_uint32 * len(uniforms))(*uniforms)
+            status = probe_fn(
+                name.encode(), u, pointers, sizes, values.ctypes.data_as(ct.POINTER(ct.c_double)),
+                None if data is None else data.ctypes.data, out.ctypes.data,
+            )
+            if status:
+                raise RuntimeError(f"oracle probe {name} failed with status {status}")
+            return out
+
+        def submit(self, encoder):
+            for operation in encoder.operations:
+                operation()
+            encoder.operations.clear()
+
+        def wait(self):
+            pass
+
+        def read(self, buffer, dtype=np.float32, shape=None):
+            array = buffer.array.view(dtype).copy()
+            return array if shape is None else array.reshape(shape)
+
+        def read_prefix(self, buffer, count, dtype=np.float32, shape=None):
+            if type(count) is not int or count < 0 or count * np.dtype(dtype).itemsize > buffer.size:
+                raise ValueError("CPU buffer prefix out of bounds")
+            array = buffer.array[: count * np.dtype(dtype).itemsize].view(dtype).copy()
+            return array if shape is None else array.reshape(shape)
+
+        def write_prefix(self, buffer, array):
+            data = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
+            if data.size > buffer.size:
+                raise ValueError("CPU buffer prefix out of bounds")
+            buffer.array[: data.size] = data
+
+    return Oracle()

@@ -1,0 +1,210 @@
+"""Seeded synthetic workloads for the mesh step: template libraries and packings.
+
+Everything derives from one integer seed through `numpy.random.default_rng`
+streams split per purpose (templates, sizes, positions, orientations, velocities),
+so a workload is reproducible on any host.  The reference has no generator for
+these libraries or packings; its own assemblies come from `initial_assembly`
+(`simulation.py:26-56`) and `benchmark.py:15-79`.  Orientations are normalized
+4-D Gaussians as in `simulation.py:75-78`.
+"""
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import geometry
+from .spec import Experiment
+
+
+def streams(seed):
+    root = np.random.SeedSequence(seed)
+    names = ("templates", "sizes", "positions", "orientations", "velocities", "shapes", "spare")
+    return {name: np.random.default_rng(s) for name, s in zip(names, root.spawn(len(names)))}
+
+
+def icosphere_library(count, *, subdivisions, sigma, diameter, nodes, rng, seed):
+    """`count` perturbed icospheres; `subdivisions` is an int or a sequence to draw from."""
+    bases = {}
+    shapes = []
+    for k in range(count):
+        s = int(subdivisions if np.isscalar(subdivisions) else rng.choice(subdivisions))
+        if s not in bases:
+            bases[s] = geometry.icosphere(s)
+        v, f = bases[s]
+        shapes.append(
+            geometry.perturbed_template(
+                f"ico{s}-{k}", v, f, sigma=sigma, diameter=diameter, nodes=nodes, rng=rng, seed=seed + k
+            )
+        )
+    return shapes
+
+
+def uneven_library(count, *, tri_range=(320, 5120), node_range=(256, 4096), sigma, diameter, rng, seed):
+    """Templates with log-uniform triangle counts and node counts proportional to area."""
+    tris = np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]), count)).astype(int)
+    shapes = []
+    for k, t in enumerate(tris):
+        v, f = geometry.hull_sphere(t)
+        shapes.append(
+            geometry.perturbed_template(f"hull{t}-{k}", v, f, sigma=sigma, diameter=diameter, nodes=16, rng=rng, seed=seed + k)
+        )
+    areas = np.array([s.areas.sum() for s in shapes])
+    lo, hi = areas.min(), areas.max()
+    counts = node_range[0] + (node_range[1] - node_range[0]) * (areas - lo) / max(hi - lo, 1e-30)
+    out = []
+    for k, (s, n) in enumerate(zip(shapes, counts)):
+        out.append(geometry.make_shape(s.id, s.vertices, s.faces, nodes=int(round(n)), seed=seed + 7919 + k, provenance=s.provenance))
+    return out
+
+
+def sphere_packing(radii, box, rng, *, iterations=400, tolerance=1e-3):
+    """Periodic overlap-relaxed sphere packing (growth from half size).
+
+    Radii grow from 0.5x to 1x over the first half of the iterations while pairwise
+    overlaps are pushed apart symmetrically; stops once the largest overlap is below
+    `tolerance` times the mean diameter.
+    """
+    from scipy.spatial import cKDTree
+
+    n = len(radii)
+    x = rng.uniform(0, box, (n, 3))
+    mean = 2 * radii.mean()
+    grow = max(1, iterations // 2)
+    for it in range(iterations):
+        scale = min(1.0, 0.5 + 0.5 * (it + 1) / grow)
+        r = radii * scale
+        tree = cKDTree(x, boxsize=box)
+        pairs = tree.query_pairs(2 * r.max(), output_type="ndarray")
+        if len(pairs) == 0:
+            if scale >= 1:
+                break
+            continue
+        d = x[pairs[:, 1]] - x[pairs[:, 0]]
+        d -= np.round(d / box) * box
+        dist = np.linalg.norm(d, axis=1)
+        overlap = r[pairs[:, 0]] + r[pairs[:, 1]] - dist
+        hit = overlap > 0
+        if scale >= 1 and (not hit.any() or overlap[hit].max() < tolerance * mean):
+            break
+        pairs, d, dist, overlap = pairs[hit], d[hit], dist[hit], overlap[hit]
+        u = d / np.maximum(dist, 1e-12)[:, None]
+        push = 0.5 * overlap[:, None] * u
+        move = np.zeros_like(x)
+        np.add.at(move, pairs[:, 0], -push)
+        np.add.at(move, pairs[:, 1], push)
+        x = np.mod(x + move, box)
+    return x
+
+
+@dataclass
+class Workload:
+    name: str
+    shapes: list
+    spec: Experiment
+    positions: np.ndarray  # metres
+    cell: np.ndarray  # metres
+    ids: np.ndarray
+    scales: np.ndarray
+    rotations: np.ndarray  # xyzw, body to world
+    velocities: np.ndarray  # m/s
+    rate: tuple  # imposed diagonal log strain rate, 1/s
+    steps: int
+
+    def engine(self, backend):
+        from .engine import Engine
+
+        e = Engine(self.shapes, self.spec, self.positions, self.cell, self.ids, self.scales, backend=backend)
+        e.set_kinematics(rotation=self.rotations, velocity=self.velocities * e.time_unit / e.length_unit)
+        return e
+
+
+def random_rotations(n, rng):
+    q = rng.normal(size=(n, 4))
+    return q / np.linalg.norm(q, axis=1)[:, None]
+
+
+def packed_workload(name, shapes, n, phi, *, seed, sphere_fraction=None, size_range=(0.8, 1.2),
+                    velocity_sigma=0.0, rate=(0, 0, 0), steps=1000, spec_overrides=None, rng=None,
+                    ids=None, scales=None):
+    """Grains on an overlap-relaxed sphere packing at solids fraction `phi`.
+
+    Contact spheres have the grains' equivalent diameters scaled so the sphere
+    packing reaches `sphere_fraction` (default phi, i.e. equivalent spheres), which
+    sets how closely neighbouring irregular grains approach.
+    """
+    rng = rng or streams(seed)
+    if ids is None:
+        ids = rng["shapes"].integers(0, len(shapes), n).astype(np.uint32)
+    if scales is None:
+        scales = rng["sizes"].uniform(*size_range, n)
+    d = np.array([shapes[i].equivalent_diameter for i in ids]) * scales
+    volume = float(np.sum(np.pi / 6 * d**3))
+    box = (volume / phi) ** (1 / 3)
+    alpha = ((sphere_fraction or phi) / phi) ** (1 / 3)
+    x = sphere_packing(0.5 * alpha * d, box, rng["positions"])
+    q = random_rotations(n, rng["orientations"])
+    v = rng["velocities"].normal(0, velocity_sigma, (n, 3)) if velocity_sigma else np.zeros((n, 3))
+    spec = Experiment(contact_geometry="mesh", particles=n, **(spec_overrides or {}))
+    return Workload(name, shapes, spec, x, np.full(3, box), ids, scales, q, v, tuple(rate), steps)
+
+
+def conservative_timestep(shapes, ids, scales, spec):
+    """Largest dt the conservative preflight accepts (`gpu.py:370-413`), rounded down."""
+    limits = []
+    for i in np.unique(ids):
+        sh = shapes[int(i)]
+        for s in (scales[ids == i].min(), scales[ids == i].max()):
+            mass = spec.density * sh.volume * s**3
+            stiffness = (spec.normal_stiffness + spec.tangential_stiffness) * sh.areas.sum() * s * s
+            rotation = np.linalg.eigvalsh(sh.inertia * spec.density * s**5).min() / (stiffness * (sh.radius * s) ** 2)
+            limits.append(0.1 * np.sqrt(min(mass / stiffness, rotation)))
+    dt = min(limits)
+    exponent = np.floor(np.log10(dt))
+    return float(np.floor(dt / 10**exponent * 100) / 100 * 10**exponent)
+
+
+def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, steps=None):
+    """The named benchmark configurations (sizes overridable for tests).
+
+    small  : 1,000 icospheres (subdiv 2, sigma 0.1), 512 nodes, phi 0.30, v ~ N(0, 0.01 m/s)
+    dense  : 10,000 grains from 64 perturbed icospheres (subdiv 2-4, sigma 0.15), 2,048
+             nodes, phi 0.60, isotropic compression at 0.5/s
+    uneven : 50,000 grains from 424 templates (320-5,120 triangles log-uniform,
+             256-4,096 nodes by area), phi 0.60
+    """
+    rng = streams(seed)
+    d0 = 3e-4
+    base = dict(friction=0.5, density=2650.0, tangential_stiffness=0.5)
+    # kn* = E / ((1 - nu^2) d) with E = 70 GPa, nu = 0.2 (traction per unit penetration).
+    kn = 70e9 / ((1 - 0.2**2) * d0)
+    base.update(normal_stiffness=kn, tangential_stiffness=0.5 * kn)
+    if config == "small":
+        n = n or 1000
+        shapes = icosphere_library(templates or 16, subdivisions=2, sigma=0.1, diameter=d0,
+                                   nodes=nodes or 512, rng=rng["templates"], seed=seed)
+        phi, sphere_fraction, vsig, rate = 0.30, 0.30, 0.01, (0, 0, 0)
+        steps = steps or 1000
+        cpp = 256
+    elif config == "dense":
+        n = n or 10_000
+        shapes = icosphere_library(templates or 64, subdivisions=(2, 3, 4), sigma=0.15, diameter=d0,
+                                   nodes=nodes or 2048, rng=rng["templates"], seed=seed)
+        phi, sphere_fraction, vsig, rate = 0.60, 0.62, 0.0, (-0.5, -0.5, -0.5)
+        steps = steps or 2000
+        cpp = 256
+    elif config == "uneven":
+        n = n or 50_000
+        shapes = uneven_library(templates or 424, sigma=0.15, diameter=d0, rng=rng["templates"], seed=seed)
+        phi, sphere_fraction, vsig, rate = 0.60, 0.62, 0.0, (0, 0, 0)
+        steps = steps or 1000
+        cpp = 256
+    else:
+        raise ValueError(f"unknown workload {config!r}")
+    ids = rng["shapes"].integers(0, len(shapes), n).astype(np.uint32)
+    probe = Experiment(contact_geometry="mesh", particles=n, **base)
+    scales = rng["sizes"].uniform(0.8, 1.2, n)
+    dt = conservative_timestep(shapes, ids, scales, probe)
+    overrides = dict(base, timestep=dt, max_overlap_fraction=0.5, contacts_per_particle=cpp)
+    return packed_workload(config, shapes, n, phi, seed=seed, sphere_fraction=sphere_fraction,
+                           velocity_sigma=vsig, rate=rate, steps=steps, spec_overrides=overrides, rng=rng,
+                           ids=ids, scales=scales)

@@ -1,0 +1,562 @@
+"""Host step driver for the surface-mesh DEM step.
+
+Restates the reference `Engine` (`src/sand_atlas_dem/gpu.py:94-1092`) for the mesh
+path: SI -> normalized units, capacities, the buffer inventory in the kernels'
+`Resources` layout, the per-step encoder sequence, the device error word with
+transactional rollback, capacity growth, the double-buffered friction history, and
+state/restore.  It is adapter-agnostic exactly as the reference is: it talks to a
+backend object through the duck-typed adapter protocol of `gpu.py` (buffer,
+dispatch, create_command_encoder/copy_buffer, submit, wait, read, read_prefix,
+write_prefix).  The product adapter is `HipBackend` (`hip.py`); when an adapter
+offers `encode_steps`, whole timesteps are encoded natively instead of kernel by
+kernel.
+"""
+
+import math
+from dataclasses import replace
+
+import numpy as np
+
+from .geometry import pack_meshes, sample_arrays
+from .spec import finite_array
+
+ERRORS = {  # gpu.py:11-26
+    2: "degenerate contact distance gradient",
+    4: "excessive penetration",
+    8: "bin capacity",
+    16: "pair capacity",
+    32: "contact capacity",
+    64: "history hash full",
+    128: "contact-list cycle",
+    256: "periodic image ambiguity",
+    512: "nonfinite state",
+    1024: "mesh traversal stack capacity",
+    2048: "nonperiodic domain escape",
+    4096: "invalid boundary box",
+    8192: "active-contact timestep bound",
+    16384: "coincident centres unsupported by experimental source-normal contacts",
+}
+CAPACITY_FLAGS = 8 | 16 | 32 | 64
+
+DAMPING_STATE = ("normalDamping", "forceVelocity", "forceOmega")
+RESIDUAL_STATE = (
+    "positionRemainder",
+    "velocityRemainder",
+    "momentumRemainder",
+    "rotationRemainder",
+    "accumulatedRemainder",
+    "cellLog",
+)
+STATE = ("position", "velocity", "rotation", "momentum", "cell", "force", "torque", "accumulated", *RESIDUAL_STATE)
+
+# Field order of the kernels' Resources block (solver.slang:20-100) = C ABI slots.
+BUFFER_ORDER = (
+    "control", "cell", "position", "positionRemainder", "velocityRemainder",
+    "momentumRemainder", "rotationRemainder", "referencePositionRemainder", "cellLog",
+    "accumulatedRemainder", "velocity", "rotation", "momentum", "force", "torque",
+    "referencePosition", "referenceCell", "shapeId", "body", "invI", "grids", "sampleInfo",
+    "gridOrigin", "distance", "sample", "meshRoots", "meshHullInfo", "meshPlanes", "meshMin",
+    "meshMax", "meshLinks", "meshTriangles", "meshNormals", "meshFrontier", "meshPairMask",
+    "meshPairTransform", "meshClusterInfo", "meshClusterRanges", "meshClusterMin",
+    "meshClusterMax", "meshSampleIndices", "binCounts", "binParticles", "pairCounts",
+    "pairOffsets", "pairs", "contactCounts", "contactOffsets", "meta", "pointDepth",
+    "normalArea", "slip", "contactForce", "normalLossPower", "normalDamping", "forceVelocity",
+    "forceOmega", "oldMeta", "oldSlip", "oldNormal", "oldHash", "hash", "heads", "next",
+    "particleStress", "timestepRows", "particleDiagnostic", "diagnosticRows", "accumulated",
+    "fabricRows", "wallPlane", "wallRemainder", "wallVelocity", "wallMaterial", "wallBoxIds",
+    "wallRows", "wallAccumulated",
+)
+# Config scalars (solver.slang:12-19) with vector fields expanded per axis.
+VALUE_ORDER = (
+    ("activeTimestep", None), ("N", None), ("shapes", None), ("pairCapacity", None),
+    ("contactCapacity", None), ("binCapacity", None), ("binCount", None), ("bins", 0),
+    ("bins", 1), ("bins", 2), ("hashSize", None), ("dt", None), ("kn", None), ("kt", None),
+    ("mu", None), ("normalViscosity", None), ("skin", None), ("drag", None),
+    ("maxOverlap", None), ("historyStep", None), ("meshGeometry", None), ("periodic", None),
+    ("wallCount", None), ("wallBoxEnabled", None), ("sourceSurfaceNormal", None),
+)
+
+
+def config_values(c, threads=1):
+    """Flat `double values[]` of the C ABI (cpu.py:411-420)."""
+    out = [float(c[name] if axis is None else c[name][axis]) for name, axis in VALUE_ORDER]
+    out.append(float(threads))
+    return np.asarray(out, np.float64)
+
+
+class RejectedStep(RuntimeError):
+    """Device error word was nonzero after a chunk (gpu.py:51-57)."""
+
+    def __init__(self, flags):
+        self.flags = int(flags)
+        super().__init__("GPU step rejected: " + ", ".join(v for k, v in ERRORS.items() if self.flags & k))
+
+
+def f4(x):
+    a = np.asarray(x, np.float32)
+    return np.pad(a, ((0, 0), (0, 4 - a.shape[1])))
+
+
+class Engine:
+    """Mesh-path engine over an explicit backend adapter (gpu.py:94-368)."""
+
+    def __init__(self, shapes, spec, positions, cell, shape_ids, scales, *, backend):
+        spec.require_mesh_path()
+        positions = finite_array(positions)
+        cell = finite_array(cell, (3,), positive=True)
+        if positions.ndim != 2 or positions.shape[1] != 3 or len(positions) < 1:
+            raise ValueError("At least one three-dimensional position required")
+        if not spec.periodic and (np.any(positions < 0) or np.any(positions >= cell)):
+            raise ValueError("Nonperiodic particle centres must lie in [0, cell)")
+        self.m = backend
+        self.spec = spec
+        self.shapes = list(shapes)
+        self.n = len(positions)
+        self.viscous = spec.viscous
+        self.step = 0
+        self.history = False
+        self.active = 0
+        self.capacity_retries = []
+        self.length_unit = float(np.median([s.equivalent_diameter for s in self.shapes]))
+        self.time_unit = math.sqrt(spec.density * self.length_unit / spec.normal_stiffness)
+        self.mass_unit = spec.density * self.length_unit**3
+        self.force_unit = spec.normal_stiffness * self.length_unit**3
+        self.stress_unit = spec.normal_stiffness * self.length_unit
+        self.energy_unit = self.force_unit * self.length_unit
+        self.ids = np.asarray(shape_ids, np.uint32)
+        self.scales = np.asarray(scales, float)
+        if len(self.ids) != self.n or len(self.scales) != self.n:
+            raise ValueError("One shape id and scale per particle required")
+        if np.any(self.ids >= len(self.shapes)):
+            raise ValueError("Shape id outside the library")
+        l = self.length_unit
+        self.body = np.array(
+            [
+                [
+                    s,
+                    spec.density * self.shapes[int(i)].volume * s**3 / self.mass_unit,
+                    self.shapes[int(i)].radius * s / l,
+                    self.shapes[int(i)].equivalent_diameter * s / l,
+                ]
+                for i, s in zip(self.ids, self.scales)
+            ],
+            np.float32,
+        )
+        inertia = np.array(
+            [
+                np.linalg.inv(self.shapes[int(i)].inertia * spec.density * s**5 / (self.mass_unit * l * l))
+                for i, s in zip(self.ids, self.scales)
+            ]
+        )
+        self.skin = float(spec.skin * 2 * self.body[:, 2].max())
+        self.cutoff = float(2 * self.body[:, 2].max() + self.skin)
+        box = np.asarray(cell, float) / l
+        if spec.periodic and self.cutoff >= 0.5 * box.min():
+            raise ValueError("Cell too small for unique periodic images")
+        self.bins = np.maximum(1, np.minimum(64, np.floor(box / self.cutoff))).astype(np.uint32)
+        self.bin_count = int(self.bins.prod())
+        self.pair_capacity = max(1, self.n * spec.pairs_per_particle)
+        self.contact_capacity = max(16, self.n * spec.contacts_per_particle)
+        self.hash_size = 1 << math.ceil(math.log2(max(32, 2 * self.contact_capacity)))
+        self.bin_capacity = spec.bin_capacity
+        self._preflight_dt()
+        estimated = (
+            244 * self.contact_capacity
+            + 8 * self.hash_size
+            + 196 * self.pair_capacity
+            + 8 * self.bin_count * self.bin_capacity
+            + 1184 * self.n
+        )
+        estimated += sum(224 * len(s.faces) + 736 + 8 * len(s.samples) for s in self.shapes)
+        if estimated > spec.working_memory_gib * 1024**3:
+            raise MemoryError(f"Estimated working set {estimated / 1024**3:.2f} GiB exceeds budget")
+        self.estimated_bytes = estimated
+        self.buffers = {}
+        self._static_geometry()
+        self.allocate("meshPairMask", self.pair_capacity)
+        self.allocate("meshPairTransform", 8 * self.pair_capacity, 16)
+        self.add("shapeId", self.ids)
+        self.add("body", self.body)
+        self.add("invI", f4(inertia.reshape(-1, 3)))
+        normalized = np.asarray(positions, float) / l
+        self.add("position", f4(normalized))
+        self.add("velocity", np.zeros((self.n, 4), np.float32))
+        self.add("rotation", np.tile(np.array([0, 0, 0, 1], np.float32), (self.n, 1)))
+        for name in ("momentum", "force", "torque", "referencePosition", "particleDiagnostic", "timestepRows", "accumulated"):
+            self.add(name, np.zeros((self.n, 4), np.float32))
+        self.add("particleStress", np.zeros((3 * self.n, 4), np.float32))
+        self.add("diagnosticRows", np.zeros((8 * self.n, 4), np.float32))
+        for name in DAMPING_STATE:
+            self.add(name, np.zeros((self.n, 4), np.float32))
+        for name in RESIDUAL_STATE:
+            residual = np.zeros((2 if name == "cellLog" else self.n, 4), np.float32)
+            if name == "positionRemainder":
+                residual[:, :3] = normalized - normalized.astype(np.float32).astype(float)
+            self.add(name, residual)
+        self.add("referencePositionRemainder", np.zeros((self.n, 4), np.float32))
+        self.allocate("fabricRows", 2 * self.pair_capacity, 16)
+        self.add("cell", np.array([list(box) + [0], [0, 0, 0, 0], list(box) + [0]], np.float32))
+        self.initial_cell = np.asarray(cell, float)
+        for name in ("wallPlane", "wallRemainder", "wallVelocity", "wallMaterial", "wallAccumulated", "wallRows"):
+            self.add(name, np.zeros((1, 4), np.float32))
+        self.add("wallBoxIds", np.zeros(6, np.uint32))
+        self.add("referenceCell", np.array([list(box) + [0], [0, 0, 0, self.cutoff]], np.float32))
+        ctrl = np.zeros(16, np.uint32)
+        ctrl[4] = 1
+        self.add("control", ctrl)
+        for name, count in (
+            ("binCounts", self.bin_count),
+            ("binParticles", self.bin_count * self.bin_capacity),
+            ("pairCounts", self.n),
+            ("pairOffsets", self.n),
+            ("contactCounts", self.pair_capacity),
+            ("contactOffsets", self.pair_capacity),
+            ("heads", self.n),
+            ("next", 2 * self.contact_capacity),
+        ):
+            self.allocate(name, count)
+        self.allocate("pairs", self.pair_capacity, 8)
+        self.history_buffers = []
+        for _ in (0, 1):
+            self.history_buffers.append(
+                {
+                    "meta": self.m.buffer(count=self.contact_capacity, stride=16),
+                    "slip": self.m.buffer(count=self.contact_capacity, stride=16),
+                    "normalArea": self.m.buffer(count=self.contact_capacity, stride=16),
+                    "hash": self.m.buffer(np.full(self.hash_size, 0xFFFFFFFF, np.uint32)),
+                }
+            )
+        self.allocate("pointDepth", self.contact_capacity, 16)
+        self.allocate("contactForce", self.contact_capacity, 16)
+        self.allocate("normalLossPower", self.contact_capacity)
+        names = (
+            "position", "velocity", "rotation", "momentum", "cell", "force", "torque",
+            "referencePosition", "referenceCell", "control", "pairs", "pairCounts", "pairOffsets",
+            "particleStress", "particleDiagnostic", "timestepRows", "accumulated",
+            "referencePositionRemainder", *RESIDUAL_STATE, *DAMPING_STATE,
+        )
+        self.backups = {name: self.m.buffer(count=self.buffers[name].size // 4) for name in names}
+        self.history_backup = {
+            name: self.m.buffer(count=self.hash_size if name == "hash" else 4)
+            for name in ("meta", "slip", "normalArea", "hash")
+        }
+
+    def _preflight_dt(self):  # gpu.py:370-413
+        if self.spec.timestep_policy == "active-contact":
+            return
+        limits = []
+        for i, s in zip(self.ids, self.scales):
+            sh = self.shapes[int(i)]
+            mass = self.spec.density * sh.volume * s**3
+            stiffness = (self.spec.normal_stiffness + self.spec.tangential_stiffness) * sh.areas.sum() * s * s
+            rotation = np.linalg.eigvalsh(sh.inertia * self.spec.density * s**5).min() / (
+                stiffness * (sh.radius * s) ** 2
+            )
+            elastic_limit = math.sqrt(min(mass / stiffness, rotation))
+            viscous_limit = math.inf
+            viscosity = self.spec.normal_viscosity * sh.areas.sum() * s**2
+            if viscosity > 0:
+                viscous_limit = min(
+                    mass / viscosity,
+                    np.linalg.eigvalsh(sh.inertia * self.spec.density * s**5).min() / (viscosity * (sh.radius * s) ** 2),
+                )
+            limits.append(0.1 * min(elastic_limit, viscous_limit))
+        if self.spec.timestep > min(limits):
+            raise ValueError(f"Timestep exceeds conservative estimate {min(limits):.6g} s")
+
+    def add(self, name, array):
+        self.buffers[name] = self.m.buffer(array)
+
+    def allocate(self, name, count, stride=4):
+        self.buffers[name] = self.m.buffer(count=count, stride=stride)
+
+    def _static_geometry(self):  # gpu.py:421-488, mesh branch
+        mesh = pack_meshes(self.shapes, self.length_unit)
+        for name, data in mesh.items():
+            self.add(name, data)
+        info, samples = sample_arrays(self.shapes, self.length_unit)
+        self.add("sampleInfo", info)
+        self.add("sample", samples)
+        # SDF-path inputs are unused by the mesh step; bound as one-row placeholders.
+        self.add("grids", np.zeros((1, 4), np.uint32))
+        self.add("gridOrigin", np.zeros((1, 4), np.float32))
+        self.add("distance", np.zeros(4, np.float32))
+
+    def config(self, dt=None, drag=0):  # gpu.py:493-518
+        spec = self.spec
+        return {
+            "activeTimestep": int(spec.timestep_policy == "active-contact"),
+            "N": self.n,
+            "shapes": len(self.shapes),
+            "pairCapacity": self.pair_capacity,
+            "contactCapacity": self.contact_capacity,
+            "binCapacity": self.bin_capacity,
+            "binCount": self.bin_count,
+            "bins": self.bins.tolist(),
+            "hashSize": self.hash_size,
+            "dt": (spec.timestep if dt is None else dt) / self.time_unit,
+            "kn": 1.0,
+            "kt": spec.tangential_stiffness / spec.normal_stiffness,
+            "mu": spec.friction,
+            "normalViscosity": spec.normal_viscosity / (spec.normal_stiffness * self.time_unit),
+            "skin": self.skin,
+            "drag": drag * self.time_unit,
+            "maxOverlap": spec.max_overlap_fraction,
+            "historyStep": int(self.history),
+            "meshGeometry": 1,
+            "sourceSurfaceNormal": 0,
+            "periodic": int(spec.periodic),
+            "wallCount": 0,
+            "wallBoxEnabled": 0,
+        }
+
+    def bindings(self, dt=None, drag=0):  # gpu.py:490-527
+        new = self.history_buffers[self.active]
+        old = self.history_buffers[1 - self.active]
+        return dict(
+            self.buffers,
+            c=self.config(dt, drag),
+            **new,
+            oldMeta=old["meta"],
+            oldSlip=old["slip"],
+            oldNormal=old["normalArea"],
+            oldHash=old["hash"],
+        )
+
+    def dispatch(self, name, n, e, b):
+        self.m.dispatch(name, n, e, r=b)
+
+    def forces(self, e, b, *, capture_kinematics=True):  # gpu.py:547-565, mesh path
+        if self.viscous and capture_kinematics:
+            self.dispatch("captureContactKinematics", self.n, e, b)
+        self.dispatch("clearBins", self.bin_count, e, b)
+        self.dispatch("fillBins", self.n, e, b)
+        self.dispatch("countPairs", self.n, e, b)
+        self.dispatch("pairScan", self.n, e, b)
+        self.dispatch("emitPairs", self.n, e, b)
+        self.dispatch("resetMeshContacts", self.pair_capacity, e, b)
+        self.dispatch("prepareMeshPairs", self.pair_capacity, e, b)
+        self.dispatch("collectMeshFrontiers", 64 * 4096, e, b)
+        self.dispatch("clearContactStorage", max(self.n, self.hash_size), e, b)
+        self.dispatch("resolveContacts", self.contact_capacity, e, b)
+        self.dispatch("buildHistoryHash", self.contact_capacity, e, b)
+        self.dispatch("gather", self.n, e, b)
+        if self.spec.timestep_policy == "active-contact":
+            self.dispatch("checkContactTimestep", self.n, e, b)
+
+    def initialize(self):  # gpu.py:580-587
+        e = self.m.device.create_command_encoder()
+        self.forces(e, self.bindings(dt=0))
+        self.m.submit(e)
+        self.m.device.wait()
+        self.check_errors()
+        self.history = True
+        self.active = 1 - self.active
+
+    def check_errors(self):  # gpu.py:589-595
+        ctrl = self.m.read(self.buffers["control"], np.uint32)
+        if ctrl[15] != 0:
+            raise RuntimeError("Arithmetic materialization mask must remain zero")
+        if ctrl[0]:
+            raise RejectedStep(ctrl[0])
+        return ctrl
+
+    def advance(self, steps, *, rate=(0, 0, 0), drag=0):  # gpu.py:597-647
+        if type(steps) is not int or steps < 0:
+            raise ValueError("Nonnegative integer step count required")
+        rate = finite_array(rate, (3,))
+        if not self.spec.periodic and np.any(rate != 0):
+            raise ValueError("Affine cell rates require a periodic domain")
+        if not np.isfinite(drag) or drag < 0:
+            raise ValueError("Finite nonnegative drag required")
+        for attempt in range(5):
+            try:
+                return self._advance_once(steps, rate=rate, drag=drag)
+            except RejectedStep as exc:
+                capacity = exc.flags & CAPACITY_FLAGS
+                if not capacity or exc.flags & ~CAPACITY_FLAGS or attempt == 4:
+                    raise
+                state = self.state()
+                old = self.spec
+                settings = {}
+                if capacity & 8:
+                    settings["bin_capacity"] = old.bin_capacity * 2
+                if capacity & 16:
+                    settings["pairs_per_particle"] = old.pairs_per_particle * 2
+                if capacity & (32 | 64):
+                    settings["contacts_per_particle"] = max(
+                        old.contacts_per_particle * 2, math.ceil(2 * self.contact_capacity / self.n)
+                    )
+                log = self.capacity_retries + [{"step": self.step, "flags": exc.flags, "new_capacities": settings}]
+                replacement = Engine(
+                    self.shapes,
+                    replace(old, **settings),
+                    state["position"][:, :3] * self.length_unit,
+                    state["cell"][0, :3] * self.length_unit,
+                    self.ids,
+                    self.scales,
+                    backend=self.m,
+                )
+                replacement.initial_cell = self.initial_cell.copy()
+                replacement.restore(state, self.step)
+                self.__dict__.update(replacement.__dict__)
+                self.capacity_retries = log
+
+    def _reserve_history_backup(self):  # gpu.py:649-661
+        count = int(self.m.read(self.buffers["control"], np.uint32)[2])
+        if count < 0 or count > self.contact_capacity:
+            raise ValueError("Committed contact count is outside capacity")
+        size = max(16, count * 16)
+        self.history_backup.update(
+            {
+                name: self.m.buffer(count=size // 4)
+                for name in ("meta", "slip", "normalArea")
+                if self.history_backup[name].size < size
+            }
+        )
+        return count
+
+    def _copy_history_backup(self, encoder, committed, count, *, restore=False):  # gpu.py:663-670
+        for name, backup in self.history_backup.items():
+            size = committed[name].size if name == "hash" else count * 16
+            if size:
+                src, dst = (backup, committed[name]) if restore else (committed[name], backup)
+                encoder.copy_buffer(dst, 0, src, 0, size)
+
+    def _encode_steps(self, e, steps, drag):
+        """Per-step sequence (gpu.py:714-726); natively encoded when offered."""
+        if hasattr(self.m, "encode_steps"):
+            self.m.encode_steps(e, steps, self.active, self.bindings(drag=drag), self.history_buffers)
+            if steps:
+                self.active = (self.active + steps) % 2
+                self.history = True
+            return
+        for _ in range(steps):
+            b = self.bindings(drag=drag)
+            if self.spec.timestep_policy == "active-contact":
+                self.dispatch("checkContactTimestep", self.n, e, b)
+            self.dispatch("beginStep", self.n, e, b)
+            self.dispatch("advanceCell", 1, e, b)
+            self.dispatch("checkRebuild", self.n, e, b)
+            self.forces(e, b)
+            self.dispatch("finishStep", self.n, e, b)
+            self.active = 1 - self.active
+            self.history = True
+
+    def _advance_once(self, steps, *, rate=(0, 0, 0), drag=0):  # gpu.py:672-744
+        if getattr(self, "invalid", False):
+            raise RuntimeError("Engine is invalid; restore a checkpoint")
+        if steps < 1:
+            return
+        history_count = self._reserve_history_backup()
+        cell = self.m.read(self.buffers["cell"], shape=(3, 4))
+        original_cell = cell.copy()
+        cell[1, :3] = np.asarray(rate) * self.time_unit
+        self.buffers["cell"].copy_from_numpy(cell)
+        e = self.m.device.create_command_encoder()
+        for name, backup in self.backups.items():
+            e.copy_buffer(backup, 0, self.buffers[name], 0, self.buffers[name].size)
+        original_active, original_history = self.active, self.history
+        committed = self.history_buffers[1 - self.active]
+        self._copy_history_backup(e, committed, history_count)
+        self._encode_steps(e, steps, drag)
+        self.m.submit(e)
+        self.m.device.wait()
+        try:
+            self.check_errors()
+        except RuntimeError:
+            e = self.m.device.create_command_encoder()
+            for name, backup in self.backups.items():
+                e.copy_buffer(self.buffers[name], 0, backup, 0, backup.size)
+            self._copy_history_backup(e, committed, history_count, restore=True)
+            self.m.submit(e)
+            self.m.device.wait()
+            self.buffers["cell"].copy_from_numpy(original_cell)
+            self.active, self.history = original_active, original_history
+            raise
+        self.step += steps
+
+    def state(self):  # gpu.py:893-936
+        out = {
+            name: self.m.read(self.buffers[name], shape=(3 if name == "cell" else 2 if name == "cellLog" else self.n, 4))
+            for name in STATE
+        }
+        if self.viscous:
+            for name in DAMPING_STATE:
+                out[name] = self.m.read(self.buffers[name], shape=(self.n, 4))
+        ctrl = self.check_errors()
+        h = self.history_buffers[1 - self.active]
+        count = int(ctrl[2])
+        out["contacts"] = count
+        for name in ("meta", "slip", "normalArea"):
+            out[name] = self.m.read_prefix(
+                h[name], 4 * count, np.uint32 if name == "meta" else np.float32, shape=(count, 4)
+            )
+        return out
+
+    def restore(self, state, step):  # gpu.py:1016-1092
+        if self.viscous:
+            for name in DAMPING_STATE:
+                finite_array(state[name], (self.n, 4))
+        for name in DAMPING_STATE:
+            self.buffers[name].copy_from_numpy(state[name] if self.viscous else np.zeros((self.n, 4), np.float32))
+        for name in STATE:
+            self.buffers[name].copy_from_numpy(np.asarray(state[name], np.float32))
+        count = int(state["contacts"])
+        if count < 0 or count > self.contact_capacity:
+            raise ValueError("Checkpoint contact count is outside capacity")
+        h = self.history_buffers[1]
+        for name in ("meta", "slip", "normalArea"):
+            values = np.asarray(state[name], dtype=np.uint32 if name == "meta" else np.float32)
+            if values.shape != (count, 4):
+                raise ValueError("Checkpoint contact history shape mismatch")
+            self.m.write_prefix(h[name], values)
+        ctrl = np.zeros(16, np.uint32)
+        ctrl[2] = count
+        ctrl[4] = 1
+        self.buffers["control"].copy_from_numpy(ctrl)
+        self.active = 1
+        self.history = True
+        e = self.m.device.create_command_encoder()
+        b = self.bindings(dt=0)
+        self.dispatch("clearContactStorage", max(self.n, self.hash_size), e, b)
+        self.dispatch("buildHistoryHash", self.contact_capacity, e, b)
+        self.m.submit(e)
+        self.m.device.wait()
+        self.check_errors()
+        self.active = 0
+        ctrl[4] = 1
+        self.buffers["control"].copy_from_numpy(ctrl)
+        e = self.m.device.create_command_encoder()
+        self.forces(e, self.bindings(dt=0), capture_kinematics=False)
+        self.m.submit(e)
+        self.m.device.wait()
+        self.check_errors()
+        self.active = 1
+        self.step = step
+
+    # ---- convenience, not in the reference: direct state upload for generated workloads
+    def set_kinematics(self, *, rotation=None, velocity=None, momentum=None):
+        """Overwrite body rotations (xyzw), velocities or momenta in normalized units."""
+        if rotation is not None:
+            self.buffers["rotation"].copy_from_numpy(f4(rotation))
+        if velocity is not None:
+            self.buffers["velocity"].copy_from_numpy(f4(velocity))
+        if momentum is not None:
+            self.buffers["momentum"].copy_from_numpy(f4(momentum))
+
+    def read(self, name, dtype=np.float32, rows=4):
+        """Whole buffer as an array of `rows`-wide records."""
+        a = self.m.read(self.buffers[name], dtype)
+        return a.reshape(-1, rows) if rows > 1 else a
+
+    def contacts(self):
+        """Committed contact rows of the last completed step (meta, pointDepth, normalArea)."""
+        ctrl = self.m.read(self.buffers["control"], np.uint32)
+        count = int(ctrl[2])
+        h = self.history_buffers[1 - self.active]
+        return (
+            self.m.read_prefix(h["meta"], 4 * count, np.uint32, shape=(count, 4)),
+            self.m.read_prefix(self.buffers["pointDepth"], 4 * count, np.float32, shape=(count, 4)),
+            self.m.read_prefix(h["normalArea"], 4 * count, np.float32, shape=(count, 4)),
+        )

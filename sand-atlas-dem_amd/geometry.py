@@ -1,0 +1,451 @@
+"""Grain templates, surface quadrature nodes and the packed static mesh layout.
+
+The packed layout is the one the reference's mesh path binds
+(`src/sand_atlas_dem/mesh.py:10-222`, bound by `gpu.py:421-488`): a median-split
+triangle BVH with stackless escape links, feature pseudonormals (face, three edges,
+three angle-weighted vertices; Baerentzen & Aanaes 2005), outward-rounded convex-hull
+planes in six dominant-axis groups, a preorder sample-cluster tree (<=64 nodes per
+leaf) and a 32-subtree frontier.  Every array keeps the reference's row format so a
+packed library is interchangeable with the reference's own buffers.
+
+Template generation (icospheres, Fibonacci-hull spheres, lognormal radial noise) and
+mass properties are this package's own: the reference obtains them from trimesh
+(`geometry.py:166-259`), which is not a dependency here.
+"""
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+F32 = np.float32
+
+
+@dataclass
+class Shape:
+    """One grain template in metres, centred at its centre of mass.
+
+    Mirrors the mesh-path fields of the reference `Shape` (`geometry.py:17-31`);
+    the SDF grid of the reference's other geometry path is not carried.
+    """
+
+    id: str
+    vertices: np.ndarray
+    faces: np.ndarray
+    samples: np.ndarray
+    areas: np.ndarray
+    volume: float
+    inertia: np.ndarray  # unit density; multiply by kg/m^3
+    radius: float
+    provenance: dict = field(default_factory=dict)
+
+    def validate(self):
+        for a in (self.vertices, self.samples, self.areas, self.inertia):
+            if not np.isfinite(a).all():
+                raise ValueError("Nonfinite shape data")
+        if self.volume <= 0 or self.radius <= 0 or np.any(self.areas <= 0):
+            raise ValueError("Invalid shape measure")
+        if len(self.samples) != len(self.areas) or self.samples.shape[1:] != (3,):
+            raise ValueError("Invalid surface samples")
+        if np.linalg.eigvalsh(self.inertia).min() <= 0:
+            raise ValueError("Inertia is not positive definite")
+
+    @property
+    def equivalent_diameter(self):
+        return (6 * self.volume / np.pi) ** (1 / 3)
+
+
+def mass_properties(vertices, faces):
+    """Volume, centroid and unit-density inertia of a closed outward-wound mesh.
+
+    Each face spans a tetrahedron with the origin.  For a tetrahedron with edge
+    matrix A = [a b c] the second moment is det(A)/120 * A (I + 11^T) A^T
+    (canonical covariance of the unit simplex), summed and shifted to the centroid
+    with the parallel-axis rule.  Inertia = tr(C) I - C.
+    """
+    v = np.asarray(vertices, float)
+    f = np.asarray(faces, np.int64)
+    a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    det = np.einsum("ij,ij->i", a, np.cross(b, c))
+    volume = det.sum() / 6
+    if volume <= 0:
+        raise ValueError("Mesh must enclose positive volume with outward winding")
+    centroid = (det[:, None] * (a + b + c)).sum(0) / 24 / volume
+    s = a + b + c
+    cov = np.einsum("k,ki,kj->ij", det, a, a)
+    cov += np.einsum("k,ki,kj->ij", det, b, b)
+    cov += np.einsum("k,ki,kj->ij", det, c, c)
+    cov += np.einsum("k,ki,kj->ij", det, s, s)
+    cov /= 120
+    cov -= volume * np.outer(centroid, centroid)
+    inertia = np.trace(cov) * np.eye(3) - cov
+    return float(volume), centroid, inertia
+
+
+def face_areas(vertices, faces):
+    v = np.asarray(vertices, float)
+    tri = v[np.asarray(faces)]
+    return 0.5 * np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=1)
+
+
+def icosphere(subdivisions):
+    """Unit icosphere by midpoint subdivision (10*4^s + 2 vertices, 20*4^s faces)."""
+    t = (1 + 5**0.5) / 2
+    v = [
+        [-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+        [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+        [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1],
+    ]
+    f = [
+        [0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+        [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+        [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+        [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1],
+    ]
+    vertices = [np.asarray(p, float) / np.linalg.norm(p) for p in v]
+    faces = np.asarray(f, np.int64)
+    for _ in range(subdivisions):
+        midpoint = {}
+
+        def mid(i, j):
+            key = (min(i, j), max(i, j))
+            if key not in midpoint:
+                p = vertices[i] + vertices[j]
+                vertices.append(p / np.linalg.norm(p))
+                midpoint[key] = len(vertices) - 1
+            return midpoint[key]
+
+        out = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            out += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        faces = np.asarray(out, np.int64)
+    return np.asarray(vertices), faces
+
+
+def hull_sphere(triangles):
+    """Unit sphere with an arbitrary even triangle count (Fibonacci points + hull).
+
+    A convex hull of V points on a sphere is a closed triangulation with 2V-4 faces,
+    so any target count of at least 4 is reachable; faces are oriented outward.
+    """
+    from scipy.spatial import ConvexHull
+
+    count = max(4, int(triangles) // 2 + 2)
+    k = np.arange(count) + 0.5
+    z = 1 - 2 * k / count
+    angle = np.pi * (1 + 5**0.5) * k
+    r = np.sqrt(1 - z * z)
+    points = np.column_stack([r * np.cos(angle), r * np.sin(angle), z])
+    hull = ConvexHull(points)
+    faces = hull.simplices.astype(np.int64).copy()
+    tri = points[faces]
+    outward = np.einsum("ij,ij->i", np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), tri.mean(1))
+    faces[outward < 0] = faces[outward < 0][:, ::-1]
+    return points, faces
+
+
+def sobol_area_samples(vertices, faces, count, seed):
+    """Area-weighted scrambled-Sobol surface nodes with equal weights A/count.
+
+    Same construction as the reference's `from_mesh` (`geometry.py:216-232`):
+    first Sobol coordinate selects a face by cumulative area, the other two map to
+    barycentric coordinates with the square-root warp.
+    """
+    from scipy.stats import qmc
+
+    area = face_areas(vertices, faces)
+    u = qmc.Sobol(3, scramble=True, seed=seed).random_base2(int(np.ceil(np.log2(count))))[:count]
+    face = np.searchsorted(np.cumsum(area) / area.sum(), u[:, 0], side="right")
+    face = np.minimum(face, len(faces) - 1)
+    tri = np.asarray(vertices, float)[np.asarray(faces)[face]]
+    root = np.sqrt(u[:, 1])
+    v = u[:, 2]
+    points = (
+        (1 - root[:, None]) * tri[:, 0]
+        + (root * (1 - v))[:, None] * tri[:, 1]
+        + (root * v)[:, None] * tri[:, 2]
+    )
+    return points, np.full(count, area.sum() / count)
+
+
+def make_shape(shape_id, vertices, faces, *, nodes, seed, scale=1.0, provenance=None):
+    """Centre a closed mesh at its centre of mass and attach surface nodes."""
+    v = np.asarray(vertices, float) * scale
+    faces = np.asarray(faces, np.int64)
+    volume, centroid, _ = mass_properties(v, faces)
+    v = v - centroid
+    volume, _, inertia = mass_properties(v, faces)
+    samples, areas = sobol_area_samples(v, faces, nodes, seed)
+    shape = Shape(
+        shape_id,
+        v,
+        faces,
+        samples,
+        areas,
+        volume,
+        inertia,
+        float(np.linalg.norm(v, axis=1).max()),
+        dict(provenance or {}, sample_count=nodes, seed=seed, sampling="scrambled_sobol_area"),
+    )
+    shape.validate()
+    return shape
+
+
+def perturbed_template(shape_id, base_vertices, faces, *, sigma, diameter, nodes, rng, seed):
+    """Radially perturbed closed mesh (per-vertex lognormal(0, sigma) radii).
+
+    The result is rescaled so its equivalent diameter equals `diameter` metres.
+    """
+    radii = rng.lognormal(0.0, sigma, len(base_vertices))
+    v = base_vertices * radii[:, None]
+    volume, _, _ = mass_properties(v, faces)
+    v = v * (diameter / ((6 * volume / np.pi) ** (1 / 3)))
+    return make_shape(
+        shape_id,
+        v,
+        faces,
+        nodes=nodes,
+        seed=seed,
+        provenance={"generator": "radially_perturbed_sphere", "sigma": sigma},
+    )
+
+
+def box_shape(extents, *, nodes=32, seed=0, shape_id="box"):
+    """Axis-aligned box with the vertex/face layout of trimesh's `creation.box`."""
+    v = np.array(
+        [[0, 0, 0], [0, 0, 1], [0, 1, 0], [0, 1, 1], [1, 0, 0], [1, 0, 1], [1, 1, 0], [1, 1, 1]],
+        float,
+    )
+    v = (v - 0.5) * np.asarray(extents, float)
+    f = np.array(
+        [[1, 3, 0], [4, 1, 0], [0, 3, 2], [2, 4, 0], [1, 7, 3], [5, 1, 4],
+         [5, 7, 1], [3, 7, 2], [6, 4, 2], [2, 7, 6], [6, 5, 4], [7, 5, 6]],
+        np.int64,
+    )
+    return make_shape(shape_id, v, f, nodes=nodes, seed=seed)
+
+
+def concave_prism_mesh():
+    """The reference's extruded-L fixture mesh in metres (`examples/create_fixtures.py:16-29`)."""
+    polygon = np.array([[0, 0], [1, 0], [1, 0.5], [0.5, 0.5], [0.5, 1], [0, 1]], float)
+    v = np.vstack(
+        [np.column_stack([polygon, np.full(6, -0.3)]), np.column_stack([polygon, np.full(6, 0.3)])]
+    ) * 0.001
+    triangles = np.array([[0, 1, 3], [1, 2, 3], [0, 3, 5], [3, 4, 5]])
+    faces = [*triangles[:, ::-1], *(triangles + 6)]
+    for i in range(6):
+        j = (i + 1) % 6
+        faces.extend([[i, j, j + 6], [i, j + 6, i + 6]])
+    return v, np.asarray(faces, np.int64)
+
+
+# --------------------------------------------------------------------------------------
+# Packed static geometry (reference `mesh.py:10-222`)
+# --------------------------------------------------------------------------------------
+
+_UINT_ARRAYS = (
+    "meshRoots",
+    "meshLinks",
+    "meshHullInfo",
+    "meshFrontier",
+    "meshClusterInfo",
+    "meshClusterRanges",
+    "meshSampleIndices",
+)
+
+
+def _round_out(values, direction):
+    return np.nextafter(values.astype(F32), F32(direction))
+
+
+def _check_manifold(faces):
+    """Two faces per edge, opposite orientation, one fan per vertex (`mesh.py:111-144`)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    edges = faces[:, [[0, 1], [1, 2], [2, 0]]].reshape(-1, 2)
+    unique, inverse, count = np.unique(np.sort(edges, axis=1), axis=0, return_inverse=True, return_counts=True)
+    inverse = inverse.reshape(-1)
+    if np.any(count != 2):
+        raise ValueError("Mesh query requires exactly two faces per edge")
+    direction = np.where(edges[:, 0] < edges[:, 1], 1, -1)
+    oriented = np.zeros(len(unique), int)
+    np.add.at(oriented, inverse, direction)
+    if np.any(oriented):
+        raise ValueError("Inconsistent source winding")
+    corner = np.arange(3 * len(faces))
+    a = 3 * (corner // 3) + corner % 3
+    b = 3 * (corner // 3) + (corner % 3 + 1) % 3
+    corners = np.stack([a, b], axis=1)
+    corners[direction < 0] = corners[direction < 0, ::-1]
+    paired = corners[np.argsort(inverse, kind="stable")].reshape(-1, 2, 2)
+    graph = coo_matrix(
+        (np.ones(2 * len(paired)), (paired[:, 0].ravel(), paired[:, 1].ravel())),
+        shape=(3 * len(faces),) * 2,
+    )
+    components, _ = connected_components(graph, directed=False)
+    if components != len(np.unique(faces)):
+        raise ValueError("Disconnected vertex fans in source mesh")
+    return unique, inverse
+
+
+def _pseudonormals(vertices, faces, inverse):
+    tri = vertices[faces]
+    cross = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    norm = np.linalg.norm(cross, axis=1)
+    if np.any(norm <= 0):
+        raise ValueError("Degenerate source triangle")
+    face_normal = cross / norm[:, None]
+    angle = np.empty((len(faces), 3))
+    for k in range(3):
+        u = tri[:, (k + 1) % 3] - tri[:, k]
+        w = tri[:, (k + 2) % 3] - tri[:, k]
+        angle[:, k] = np.arctan2(np.linalg.norm(np.cross(u, w), axis=1), (u * w).sum(1))
+    vertex_normal = np.zeros_like(vertices)
+    np.add.at(vertex_normal, faces.ravel(), (face_normal[:, None, :] * angle[:, :, None]).reshape(-1, 3))
+    edge_normal = np.zeros((inverse.max() + 1, 3))
+    np.add.at(edge_normal, inverse, np.repeat(face_normal, 3, axis=0))
+    feature = np.concatenate(
+        [face_normal[:, None, :], edge_normal[inverse.reshape(-1, 3)], vertex_normal[faces]], axis=1
+    )
+    lengths = np.linalg.norm(feature, axis=2)
+    if np.any(lengths < 1e-12):
+        raise ValueError("Undefined mesh feature pseudonormal")
+    return feature / lengths[:, :, None]
+
+
+def _hull_groups(vertices):
+    """Outward-rounded hull planes grouped by dominant normal axis (`mesh.py:74-90`)."""
+    from scipy.spatial import ConvexHull
+
+    rounded = vertices.astype(F32).astype(float)
+    hull = ConvexHull(rounded)
+    margin = 16 * np.finfo(F32).eps * (1 + abs(rounded).max())
+    planes = hull.equations.astype(F32)
+    planes[:, 3] = _round_out(hull.equations[:, 3] - margin, -np.inf)
+    planes = np.unique(planes, axis=0)
+    dominant = np.abs(planes[:, :3]).argmax(axis=1)
+    group = 2 * dominant + (planes[np.arange(len(planes)), dominant] >= 0)
+    return [planes[group == g] for g in range(6)]
+
+
+class _Packer:
+    def __init__(self):
+        self.a = {name: [] for name in (
+            "meshFrontier", "meshClusterInfo", "meshClusterRanges", "meshClusterMin",
+            "meshClusterMax", "meshSampleIndices", "meshHullInfo", "meshPlanes",
+            "meshRoots", "meshMin", "meshMax", "meshLinks", "meshTriangles", "meshNormals",
+        )}
+
+    def clusters(self, points):
+        """Preorder median-split sample tree; leaves hold <=64 node keys."""
+        ranges, lo, hi, keys = (self.a[k] for k in (
+            "meshClusterRanges", "meshClusterMin", "meshClusterMax", "meshSampleIndices"))
+        first = len(ranges)
+
+        def split(ids):
+            node = len(ranges)
+            ranges.append([0, 0, 0, 0])
+            lo.append([*_round_out(points[ids].min(0), -np.inf), 0])
+            hi.append([*_round_out(points[ids].max(0), np.inf), 0])
+            if len(ids) > 64:
+                axis = np.ptp(points[ids], axis=0).argmax()
+                order = ids[np.argsort(points[ids, axis], kind="stable")]
+                half = len(order) // 2
+                split(order[:half])
+                split(order[half:])
+            else:
+                ranges[node][0:2] = [len(keys), len(ids)]
+                keys.extend(ids.tolist())
+            ranges[node][2] = len(ranges)
+
+        split(np.arange(len(points)))
+        self.a["meshClusterInfo"].append([first, len(ranges) - first, 0, 0])
+        # Up to 32 disjoint complete subtrees; split the largest internal one first.
+        pending = [first]
+        while len(pending) < 32:
+            internal = [n for n in pending if ranges[n][1] == 0]
+            if not internal:
+                break
+            node = max(internal, key=lambda n: ranges[n][2] - n)
+            at = pending.index(node)
+            pending[at : at + 1] = [node + 1, ranges[node + 1][2]]
+        self.a["meshFrontier"].extend([[n, ranges[n][2], 0, 0] for n in pending])
+        self.a["meshFrontier"].extend([[0, 0, 0, 0]] * (32 - len(pending)))
+
+    def bvh(self, tri, feature):
+        """Median-split triangle BVH, <=4 triangles per leaf, escape links."""
+        links, lo, hi = self.a["meshLinks"], self.a["meshMin"], self.a["meshMax"]
+        root = len(links)
+        first_triangle = len(self.a["meshTriangles"]) // 3
+        order, depths = [], []
+
+        def build(ids, depth):
+            node = len(links)
+            depths.append(depth)
+            lo.append([*_round_out(tri[ids].min(axis=(0, 1)), -np.inf), 0])
+            hi.append([*_round_out(tri[ids].max(axis=(0, 1)), np.inf), 0])
+            links.append([0, 0, 0, 0])
+            if len(ids) <= 4:
+                links[node][1:3] = [first_triangle + len(order), len(ids)]
+                order.extend(ids.tolist())
+            else:
+                centres = tri[ids].mean(axis=1)
+                axis = np.ptp(centres, axis=0).argmax()
+                ids = ids[np.argsort(centres[:, axis], kind="stable")]
+                half = len(ids) // 2
+                build(ids[:half], depth + 1)
+                build(ids[half:], depth + 1)
+            links[node][0] = len(links)
+
+        build(np.arange(len(tri)), 0)
+        if max(depths) > 30:
+            raise ValueError("Mesh hierarchy exceeds GPU traversal stack depth")
+        self.a["meshRoots"].append([root, len(links), max(depths), 0])
+        self.a["meshTriangles"].extend(tri[order].reshape(-1, 3))
+        self.a["meshNormals"].extend(feature[order].reshape(-1, 3))
+
+    def add(self, shape, length_unit):
+        vertices = np.asarray(shape.vertices, float) / length_unit
+        faces = np.asarray(shape.faces, np.int64)
+        self.clusters((np.asarray(shape.samples) / length_unit).astype(F32))
+        for group in _hull_groups(vertices):
+            self.a["meshHullInfo"].append([len(self.a["meshPlanes"]), len(group), 0, 0])
+            self.a["meshPlanes"].extend(group)
+        _, inverse = _check_manifold(faces)
+        feature = _pseudonormals(vertices, faces, inverse)
+        tri = vertices[faces].astype(F32)
+        rounded = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+        if np.any(np.linalg.norm(rounded, axis=1) == 0):
+            raise ValueError("Triangle degenerates in normalized FP32")
+        self.bvh(tri, feature)
+
+    def arrays(self):
+        out = {}
+        for name, rows in self.a.items():
+            if name in _UINT_ARRAYS:
+                out[name] = np.asarray(rows, np.uint32)
+            elif name in ("meshTriangles", "meshNormals"):
+                a = np.asarray(rows, F32).reshape(-1, 3)
+                out[name] = np.pad(a, ((0, 0), (0, 1)))
+            else:
+                out[name] = np.asarray(rows, F32).reshape(-1, 4) if rows else np.zeros((0, 4), F32)
+        return out
+
+
+def pack_meshes(shapes, length_unit):
+    """Pack a template library into the reference's 14 mesh arrays (normalized units)."""
+    packer = _Packer()
+    for shape in shapes:
+        packer.add(shape, length_unit)
+    return packer.arrays()
+
+
+def sample_arrays(shapes, length_unit):
+    """`sampleInfo` (offset, count) and `sample` (xyz, area) rows (`gpu.py:464-488`)."""
+    info, samples, offset = [], [], 0
+    for s in shapes:
+        info.append([offset, len(s.samples), 0, 0])
+        samples.append(np.column_stack([s.samples / length_unit, s.areas / length_unit**2]).astype(F32))
+        offset += len(s.samples)
+    return np.asarray(info, np.uint32), np.concatenate(samples)

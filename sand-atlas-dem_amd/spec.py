@@ -1,0 +1,111 @@
+"""Experiment contract for the surface-mesh step (SI units).
+
+Restates the fields and validation of the reference `Experiment`
+(`src/sand_atlas_dem/spec.py:12-166`) that the mesh timestep reads through
+`Engine` (`gpu.py:95-368, 490-527`).  Field names, defaults and error behaviour are
+kept, so a reference case JSON loads unchanged; stage/servo fields are accepted and
+validated but driving stages is the run layer's job, outside this backend.
+"""
+
+from dataclasses import asdict, dataclass, field
+
+import numpy as np
+
+
+def finite_array(value, shape=None, positive=False):
+    """Reference `util.finite_array` (`util.py:49-59`)."""
+    a = np.asarray(value, dtype=np.float64)
+    if (
+        (shape is not None and a.shape != shape)
+        or not np.isfinite(a).all()
+        or (positive and np.any(a <= 0))
+    ):
+        raise ValueError(f"Expected finite {'positive ' if positive else ''}array with shape {shape}")
+    return a
+
+
+@dataclass
+class Experiment:
+    format: str = "sand-dem-experiment/1"
+    shape_library: str = ""
+    contact_geometry: str = "mesh"
+    contact_normal: str = "target-gradient"
+    periodic: bool = True
+    planes: list = field(default_factory=list)
+    boundary_box: list | None = None
+    particles: int = 1000
+    shape_sampling: str = "random"
+    seed: int = 42
+    density: float = 2650.0
+    normal_stiffness: float = 1e12  # traction / penetration, Pa/m
+    tangential_stiffness: float = 5e11
+    friction: float = 0.5
+    normal_viscosity: float = 0.0  # N s / m^3
+    timestep_policy: str = "conservative"
+    timestep: float = 1e-7
+    initial_solid_fraction: float = 0.15
+    scale: float = 1.0
+    cell: list | None = None
+    skin: float = 0.05  # fraction of the maximum particle diameter
+    working_memory_gib: float = 12.0
+    contacts_per_particle: int = 256
+    pairs_per_particle: int = 32
+    bin_capacity: int = 128
+    stages: list = field(default_factory=lambda: [{"kind": "hold", "steps": 1000, "drag": 0.0}])
+    diagnostic_interval: int = 50
+    snapshot_interval: int = 1000
+    max_overlap_fraction: float = 0.01
+
+    def __post_init__(self):
+        if self.format != "sand-dem-experiment/1":
+            raise ValueError("Unsupported experiment format")
+        if self.contact_geometry not in {"sdf", "mesh"}:
+            raise ValueError("Unsupported contact geometry")
+        if self.contact_normal not in {"target-gradient", "source-surface-experimental"}:
+            raise ValueError("Unsupported contact normal")
+        if self.timestep_policy not in {"conservative", "active-contact"}:
+            raise ValueError("Unsupported timestep policy")
+        if self.shape_sampling not in {"random", "balanced"}:
+            raise ValueError("shape_sampling must be random or balanced")
+        if type(self.periodic) is not bool:
+            raise ValueError("periodic must be an explicit boolean")
+        for name in (
+            "particles", "seed", "contacts_per_particle", "pairs_per_particle",
+            "bin_capacity", "diagnostic_interval", "snapshot_interval",
+        ):
+            value = getattr(self, name)
+            if isinstance(value, bool) or not isinstance(value, int) or value < (0 if name == "seed" else 1):
+                raise ValueError(f"Invalid integer {name}")
+        for name in (
+            "density", "normal_stiffness", "tangential_stiffness", "timestep", "scale",
+            "skin", "working_memory_gib", "max_overlap_fraction",
+        ):
+            value = getattr(self, name)
+            if not np.isfinite(value) or value <= 0:
+                raise ValueError(f"Invalid positive {name}")
+        if (
+            not np.isfinite(self.friction)
+            or self.friction < 0
+            or not np.isfinite(self.normal_viscosity)
+            or self.normal_viscosity < 0
+            or not 0 < self.initial_solid_fraction < 0.5
+        ):
+            raise ValueError("Invalid friction or initial solid fraction")
+        if self.cell is not None:
+            finite_array(self.cell, (3,), positive=True)
+
+    def to_dict(self):
+        return asdict(self)
+
+    @property
+    def viscous(self):
+        return self.normal_viscosity > 0 or any(p.get("normal_viscosity", 0) > 0 for p in self.planes)
+
+    def require_mesh_path(self):
+        """The `hip` backend implements the surface-mesh step without planes."""
+        if self.contact_geometry != "mesh":
+            raise NotImplementedError("The hip backend implements contact_geometry='mesh' only")
+        if self.contact_normal != "target-gradient":
+            raise NotImplementedError("The hip backend implements target-gradient contact normals only")
+        if self.planes or self.boundary_box is not None:
+            raise NotImplementedError("Translating planes are not implemented by the hip backend")
