@@ -80,6 +80,14 @@ inline float3 fmax3(float3 a, float3 b) { return {std::fmax(a.x, b.x), std::fmax
 inline float3 fmin3(float3 a, float3 b) { return {std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)}; }
 inline float3 floor3(float3 a) { return {std::floor(a.x), std::floor(a.y), std::floor(a.z)}; }
 inline bool finite3(float3 a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
+// FP32 transcendentals: the reference calls the platform's ::sinf/::cosf/::expf
+// (instrumented-collector.cpp:877-943), whose last bit differs between libms
+// (glibc is off by one ulp on ~1% of arguments).  The oracle fixes them to the
+// correctly rounded value, obtained from the FP64 libm functions.
+inline float sin32(float x) { return float(std::sin(double(x))); }
+inline float cos32(float x) { return float(std::cos(double(x))); }
+inline float exp32(float x) { return float(std::exp(double(x))); }
+
 inline float4 normalize4(float4 a) {
     float s = 1.0f / std::sqrt(dot4(a, a));
     return a * s;
@@ -262,14 +270,14 @@ inline void compensated4(float4& high, float4& low, float4 inc) {  // :115-118
 inline float expm1Stable(float x) {  // :119-122
     if (std::fabs(x) < 0.01f)
         return x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f + x / 120.0f))));
-    return std::exp(x) - 1.0f;
+    return exp32(x) - 1.0f;
 }
 inline float3 expm1Stable3(float3 x) { return {expm1Stable(x.x), expm1Stable(x.y), expm1Stable(x.z)}; }
 inline float4 rotationIncrement(float3 w, float dt, float4 q) {  // :124-129
     float half = 0.5f * length(w) * dt;
     float coefficient = std::fabs(half) < 0.001f ? 0.5f * dt * (1.0f - half * half / 6.0f)
-                                                 : std::sin(half) / length(w);
-    float cm1 = -2.0f * std::sin(0.5f * half) * std::sin(0.5f * half);
+                                                 : sin32(half) / length(w);
+    float cm1 = -2.0f * sin32(0.5f * half) * sin32(0.5f * half);
     return qmul(f4(coefficient * w, cm1), q);
 }
 inline void accumulate(const R& r, uint32_t id, float4 inc) {  // :130-134
@@ -537,7 +545,7 @@ void beginStep(const R& r, uint32_t id) {  // solver.slang:859-888
     r.rotationRemainder[id] = ql;
     float3 logarithm = xyz(r.cellLog[0]), residual = xyz(r.cellLog[1]);
     compensated3(logarithm, residual, rate * dt);
-    float3 box = xyz(r.cell[2]) * f3(std::exp(logarithm.x), std::exp(logarithm.y), std::exp(logarithm.z));
+    float3 box = xyz(r.cell[2]) * f3(exp32(logarithm.x), exp32(logarithm.y), exp32(logarithm.z));
     float3 p = xyz(r.position[id]), pl = xyz(r.positionRemainder[id]);
     compensated3(p, pl, expm1Stable3(rate * dt) * (p + pl) + dt * (v + vl));
     if (!finite3(p) || !finite3(pl)) { fail(r, 512); return; }
@@ -558,7 +566,7 @@ void advanceCell(const R& r) {  // :890-897
     compensated3(logarithm, residual, xyz(r.cell[1]) * r.c.dt);
     r.cellLog[0] = f4(logarithm, 0);
     r.cellLog[1] = f4(residual, 0);
-    r.cell[0] = f4(xyz(r.cell[2]) * f3(std::exp(logarithm.x), std::exp(logarithm.y), std::exp(logarithm.z)), 0);
+    r.cell[0] = f4(xyz(r.cell[2]) * f3(exp32(logarithm.x), exp32(logarithm.y), exp32(logarithm.z)), 0);
     r.control[4] = 0;
     if (r.c.periodic != 0 && r.referenceCell[1].w >= 0.5f * std::fmin(r.cell[0].x, std::fmin(r.cell[0].y, r.cell[0].z)))
         fail(r, 256);
@@ -735,7 +743,7 @@ inline float3 transported(float3 x, float3 a, float3 b, float spin) {  // :691-6
     if (c < -0.999999f) return {0, 0, 0};
     float3 v = cross(a, b);
     float3 out = x + cross(v, x) + cross(v, cross(v, x)) / (1.0f + c);
-    out = out * std::cos(spin) + cross(b, out) * std::sin(spin) + b * dot(b, out) * (1.0f - std::cos(spin));
+    out = out * cos32(spin) + cross(b, out) * sin32(spin) + b * dot(b, out) * (1.0f - cos32(spin));
     return out - b * dot(out, b);
 }
 void resolveContacts(const R& r, uint32_t id) {  // :703-749 (grain-grain contacts)

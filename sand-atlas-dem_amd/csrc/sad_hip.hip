@@ -1,0 +1,1387 @@
+// sad_hip.hip -- MI355X (gfx950) kernels of the surface-mesh DEM step and the C ABI
+// declared in include/sand_dem_hip.h.
+//
+// Semantics follow the reference CPU backend (solver.slang + cpu_scheduling.slang as
+// spliced by cpu.py:46-103), so neighbour lists, contact sets and order, history
+// lookups and per-grain sums are reproduced exactly; the schedule is the GPU's own:
+//   * neighbour search: atomic bin fill + per-bin id sort (= serial fill order),
+//     27-cell stencil count / scan / emit, per-grain pair adjacency for the gather;
+//   * contact collection: persistent wave64 work queue over directed-pair tasks;
+//     cluster culls lane-parallel, ballot-compacted query queue in LDS, per-lane
+//     BVH stack in LDS; hits ranked by wave prefix count, staged, then placed by a
+//     task-count scan (= the CPU's pair/pass/preorder/lane order);
+//   * friction history: per-contact resolve, CAS linear-probe hash;
+//   * force/torque gather: one lane per grain walking its pairs in descending
+//     contact id with the compensated FP32 sums of the reference;
+//   * integrator: per-grain compensated kicks, quaternion drift, affine streaming.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/sand_dem_hip.h"
+#include "dem_math.h"
+
+namespace {
+
+struct Cfg {
+    uint32_t activeTimestep, N, shapes, pairCapacity, contactCapacity, binCapacity, binCount;
+    uint32_t bins[3];
+    uint32_t hashSize;
+    float dt, kn, kt, mu, normalViscosity, skin, drag, maxOverlap;
+    uint32_t historyStep, meshGeometry, periodic, wallCount, wallBoxEnabled, sourceSurfaceNormal;
+};
+
+// Device pointers of one binding set (reference Resources order) plus config.
+struct P {
+    Cfg c;
+    uint32_t* control;
+    float4 *cell, *position, *positionRemainder, *velocityRemainder, *momentumRemainder;
+    float4 *rotationRemainder, *referencePositionRemainder, *cellLog, *accumulatedRemainder;
+    float4 *velocity, *rotation, *momentum, *force, *torque, *referencePosition, *referenceCell;
+    const uint32_t* shapeId;
+    const float4 *body, *invI;
+    const uint4* sampleInfo;
+    const float4* sample;
+    const uint4 *meshRoots, *meshHullInfo;
+    const float4 *meshPlanes, *meshMin, *meshMax;
+    const uint4* meshLinks;
+    const float4 *meshTriangles, *meshNormals;
+    uint32_t* meshPairMask;
+    float4* meshPairTransform;
+    const uint4 *meshClusterInfo, *meshClusterRanges;
+    const float4 *meshClusterMin, *meshClusterMax;
+    const uint32_t* meshSampleIndices;
+    uint32_t *binCounts, *binParticles, *pairCounts, *pairOffsets;
+    uint2* pairs;
+    uint32_t* contactCounts;
+    uint4* meta;
+    float4 *pointDepth, *normalArea, *slip, *contactForce;
+    float* normalLossPower;
+    float4 *normalDamping, *forceVelocity, *forceOmega;
+    const uint4* oldMeta;
+    const float4 *oldSlip, *oldNormal;
+    const uint32_t* oldHash;
+    uint32_t* hash;
+    float4 *particleStress, *timestepRows, *particleDiagnostic, *accumulated;
+};
+
+// Library workspace (device arrays), grown on demand.
+struct W {
+    uint32_t* counters;   // [0] task cursor, [1] staged count, [2] queried nodes, [3] rebuilds
+    uint32_t* taskCount;  // 2P
+    uint32_t* taskOffset; // 2P + 1
+    uint32_t* adjCount;   // N
+    uint32_t* adjOffset;  // N
+    uint32_t* adjCursor;  // N
+    uint32_t* adjList;    // 2P
+    uint32_t* sums;       // scan tile sums
+    uint2* stageKey;      // C
+    uint4* stageMeta;     // C
+    float4* stagePd;      // C
+    float4* stageNa;      // C
+    uint32_t stageCap, taskCap, nCap, sumsCap;
+};
+
+constexpr int WAVE = 64;
+constexpr int COLLECT_WAVES = 4;
+constexpr int QCAP = 256;             // per-wave query queue (node keys)
+constexpr int MAX_CLUSTERS = 1024;    // per source template
+constexpr int STACK = 32;             // BVH traversal stack (reference depth limit)
+constexpr int SCAN_TILE = 1024;       // 256 threads x 4
+
+__device__ __forceinline__ void fail(const P& p, uint32_t bit) { atomicOr(p.control, bit); }
+__device__ __forceinline__ uint32_t ld_volatile(const uint32_t* a) { return __atomic_load_n(a, __ATOMIC_RELAXED); }
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// ---------------------------------------------------------------- shared helpers (solver.slang:130-186)
+__device__ v3 centerBranch(const P& p, uint32_t i, uint32_t j) {  // FP64 (cpu_scheduling.slang:10-15)
+    float4 c = p.cell[0], pj = p.position[j], pi = p.position[i];
+    float4 rj = p.positionRemainder[j], ri = p.positionRemainder[i];
+    double bx = c.x, by = c.y, bz = c.z;
+    double dx = (double(pj.x) - double(pi.x)) + (double(rj.x) - double(ri.x));
+    double dy = (double(pj.y) - double(pi.y)) + (double(rj.y) - double(ri.y));
+    double dz = (double(pj.z) - double(pi.z)) + (double(rj.z) - double(ri.z));
+    if (p.c.periodic != 0) {
+        dx = dx - floor(dx / bx + 0.5) * bx;
+        dy = dy - floor(dy / by + 0.5) * by;
+        dz = dz - floor(dz / bz + 0.5) * bz;
+    }
+    return {float(dx), float(dy), float(dz)};
+}
+__device__ v3 omega(const P& p, uint32_t i) {
+    v4 q = v4of(p.rotation[i]);
+    v3 b = qinv(q, xyz(p.momentum[i]) + xyz(p.momentumRemainder[i]));
+    v3 w = {dot(xyz(p.invI[3 * i]), b), dot(xyz(p.invI[3 * i + 1]), b), dot(xyz(p.invI[3 * i + 2]), b)};
+    return qrot(q, w);
+}
+__device__ float rotationalEnergy(const P& p, uint32_t i, v3 l) {
+    v3 b = qinv(v4of(p.rotation[i]), l);
+    v3 w = {dot(xyz(p.invI[3 * i]), b), dot(xyz(p.invI[3 * i + 1]), b), dot(xyz(p.invI[3 * i + 2]), b)};
+    return 0.5f * dot(b, w);
+}
+__device__ void accumulateRow(const P& p, uint32_t id, v4 inc) {
+    v4 high = v4of(p.accumulated[id]), low = v4of(p.accumulatedRemainder[id]);
+    compensated4(high, low, inc);
+    p.accumulated[id] = f4(high);
+    p.accumulatedRemainder[id] = f4(low);
+}
+__device__ void accumulateNormalLoss(const P& p, uint32_t id, float dt) {
+    float4 value = p.normalDamping[id];
+    v3 high = {value.x, 0, 0}, low = {value.y, 0, 0};
+    compensated3(high, low, {value.z * dt, 0, 0});
+    value.x = high.x;
+    value.y = low.x;
+    p.normalDamping[id] = value;
+}
+__device__ float cellPower(const P& p, uint32_t i) {
+    v3 diagonal = {p.particleStress[3 * i].x, p.particleStress[3 * i + 1].y, p.particleStress[3 * i + 2].z};
+    v3 v = xyz(p.velocity[i]);
+    return -dot(diagonal + p.body[i].y * v * v, xyz(p.cell[1]));
+}
+__device__ void binIndex(const P& p, uint32_t i, uint32_t b[3]) {
+    v3 g = floor3(xyz(p.position[i]) / xyz(p.cell[0]) * mk3(float(p.c.bins[0]), float(p.c.bins[1]), float(p.c.bins[2])));
+    b[0] = min(uint32_t(g.x), p.c.bins[0] - 1);
+    b[1] = min(uint32_t(g.y), p.c.bins[1] - 1);
+    b[2] = min(uint32_t(g.z), p.c.bins[2] - 1);
+}
+__device__ __forceinline__ uint32_t binFlat(const P& p, const uint32_t b[3]) { return (b[0] * p.c.bins[1] + b[1]) * p.c.bins[2] + b[2]; }
+__device__ __forceinline__ uint32_t modIndex(int a, uint32_t n) { return uint32_t((a % int(n) + int(n)) % int(n)); }
+
+// ---------------------------------------------------------------- mesh predicates (solver.slang:259-305)
+__device__ bool meshMayContain(const P& p, uint32_t shape, v3 q) {
+    uint32_t root = p.meshRoots[shape].x;
+    v3 low = xyz(p.meshMin[root]), high = xyz(p.meshMax[root]);
+    if (q.x < low.x || q.y < low.y || q.z < low.z || q.x > high.x || q.y > high.y || q.z > high.z) return false;
+    v3 direction = (q - 0.5f * (low + high)) / (high - low);
+    uint4 info = p.meshHullInfo[6 * shape + dominantGroup(direction)];
+    for (uint32_t k = 0; k < info.y; k++) {
+        float4 plane = p.meshPlanes[info.x + k];
+        if (dot(xyz(plane), q) + plane.w > 0) return false;
+    }
+    return true;
+}
+__device__ bool meshBoxMayContact(const P& p, uint32_t shape, v3 low, v3 high) {
+    uint32_t root = p.meshRoots[shape].x;
+    v3 tl = xyz(p.meshMin[root]), th = xyz(p.meshMax[root]);
+    if (high.x < tl.x || high.y < tl.y || high.z < tl.z || low.x > th.x || low.y > th.y || low.z > th.z) return false;
+    v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
+    v3 direction = (center - 0.5f * (tl + th)) / (th - tl);
+    uint4 info = p.meshHullInfo[6 * shape + dominantGroup(direction)];
+    for (uint32_t k = 0; k < info.y; k++) {
+        float4 plane = p.meshPlanes[info.x + k];
+        if (dot(xyz(plane), center) + plane.w - dot(fabs3(xyz(plane)), half) > 0) return false;
+    }
+    return true;
+}
+__device__ bool meshClusterMayContact(const P& p, v4 qs, v4 qd, float ss, float sd, v3 branch, uint32_t dstShape,
+                                      uint32_t cluster) {
+    v3 sl = xyz(p.meshClusterMin[cluster]), sh = xyz(p.meshClusterMax[cluster]);
+    v3 low = {1e30f, 1e30f, 1e30f}, high = {-1e30f, -1e30f, -1e30f};
+    for (uint32_t corner = 0; corner < 8; corner++) {
+        v3 q = {(corner & 1) ? sh.x : sl.x, (corner & 2) ? sh.y : sl.y, (corner & 4) ? sh.z : sl.z};
+        q = qinv(qd, qrot(qs, q * ss) - branch) / sd;
+        low = fmin3(low, q);
+        high = fmax3(high, q);
+    }
+    float pad = 64.0f * DEM_EPS * (1.0f + maxabs(low) + maxabs(high) + length(branch) / sd);
+    low = low - mk3(pad, pad, pad);
+    high = high + mk3(pad, pad, pad);
+    return meshBoxMayContact(p, dstShape, low, high);
+}
+
+// Nearest-feature BVH query (cpu_scheduling.slang:18-48).  The pruning distance
+// of a popped node is recomputed (identical to the CPU's cached push-time value);
+// the stack holds node offsets from the template root, lane-strided in LDS.
+__device__ v4 meshDistance(const P& p, uint32_t shape, v3 q, uint16_t* stack) {
+    const uint32_t root = p.meshRoots[shape].x;
+    uint32_t top = 1;
+    stack[0] = 0;
+    float best = 1e30f;
+    v3 nearest = {0, 0, 0};
+    uint32_t nearestNormal = 0;
+    bool found = false;
+    while (top > 0) {
+        --top;
+        uint32_t node = root + stack[top * WAVE];
+        if (boxDistance(xyz(p.meshMin[node]), xyz(p.meshMax[node]), q) > best) continue;
+        uint4 link = p.meshLinks[node];
+        if (link.z == 0) {
+            uint32_t a = node + 1, b = p.meshLinks[a].x;
+            float da = boxDistance(xyz(p.meshMin[a]), xyz(p.meshMax[a]), q);
+            float db = boxDistance(xyz(p.meshMin[b]), xyz(p.meshMax[b]), q);
+            if (da < db) { uint32_t s = a; a = b; b = s; float t = da; da = db; db = t; }
+            if (top + 2 > STACK) { fail(p, 1024); return {0, 0, 0, 1e20f}; }
+            if (da <= best) stack[(top++) * WAVE] = uint16_t(a - root);
+            if (db <= best) stack[(top++) * WAVE] = uint16_t(b - root);
+            continue;
+        }
+        for (uint32_t k = 0; k < link.z; k++) {
+            uint32_t triangle = link.y + k, feature;
+            v3 c = closestTriangle(q, xyz(p.meshTriangles[3 * triangle]), xyz(p.meshTriangles[3 * triangle + 1]),
+                                   xyz(p.meshTriangles[3 * triangle + 2]), feature);
+            float distance = dot(q - c, q - c);
+            if (distance < best) { best = distance; nearest = c; nearestNormal = 7 * triangle + feature; found = true; }
+        }
+    }
+    v3 pseudo = found ? xyz(p.meshNormals[nearestNormal]) : mk3(0, 0, 0);
+    float distance = sqrtf(best);
+    float side = dot(q - nearest, pseudo) < 0 ? -1.0f : 1.0f;
+    v3 normal = distance > 1e-12f ? side * (q - nearest) / distance : pseudo;
+    return mk4(normal, side * distance);
+}
+
+// ---------------------------------------------------------------- integrator (solver.slang:859-920)
+__global__ void k_begin_step(P p) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N || ld_volatile(p.control) != 0) return;
+    float dt = p.c.dt;
+    v3 rate = xyz(p.cell[1]);
+    accumulateNormalLoss(p, id, 0.5f * dt);
+    float4 body = p.body[id];
+    v3 v = xyz(p.velocity[id]), vl = xyz(p.velocityRemainder[id]);
+    v3 l = xyz(p.momentum[id]), ll = xyz(p.momentumRemainder[id]);
+    compensated3(v, vl, 0.5f * dt * xyz(p.force[id]) / body.y);
+    compensated3(l, ll, 0.5f * dt * xyz(p.torque[id]));
+    float loss = -expm1Stable(-p.c.drag * dt) * (0.5f * body.y * dot(v + vl, v + vl) + rotationalEnergy(p, id, l + ll));
+    accumulateRow(p, id, {0, loss, 0.5f * dt * cellPower(p, id), 0.0f});
+    float damping = expm1Stable(-0.5f * p.c.drag * dt);
+    compensated3(v, vl, damping * (v + vl));
+    compensated3(l, ll, damping * (l + ll));
+    p.momentum[id] = f4(l, 0);
+    p.momentumRemainder[id] = f4(ll, 0);
+    v4 q = v4of(p.rotation[id]), ql = v4of(p.rotationRemainder[id]);
+    v3 w = omega(p, id);
+    p.rotation[id] = f4(normalize4(q + rotationIncrement(w, 0.5f * dt, q)));
+    w = omega(p, id);
+    compensated4(q, ql, rotationIncrement(w, dt, q));
+    if (fabsf(dot4(q, q) - 1.0f) > 2e-6f) { q = normalize4(q); ql = {0, 0, 0, 0}; }
+    p.rotation[id] = f4(q);
+    p.rotationRemainder[id] = f4(ql);
+    v3 logarithm = xyz(p.cellLog[0]), residual = xyz(p.cellLog[1]);
+    compensated3(logarithm, residual, rate * dt);
+    v3 box = xyz(p.cell[2]) * mk3(dem_expf(logarithm.x), dem_expf(logarithm.y), dem_expf(logarithm.z));
+    v3 x = xyz(p.position[id]), xl = xyz(p.positionRemainder[id]);
+    compensated3(x, xl, expm1Stable3(rate * dt) * (x + xl) + dt * (v + vl));
+    if (!finite3(x) || !finite3(xl)) { fail(p, 512); return; }
+    if (p.c.periodic != 0) x = x - floor3(x / box) * box;
+    else {
+        v3 s = x + xl, t = (x - box) + xl;
+        if (s.x < 0 || s.y < 0 || s.z < 0 || t.x >= 0 || t.y >= 0 || t.z >= 0) { fail(p, 2048); return; }
+    }
+    p.position[id] = f4(x, 0);
+    p.positionRemainder[id] = f4(xl, 0);
+    compensated3(v, vl, expm1Stable3(-rate * dt) * (v + vl));
+    p.velocity[id] = f4(v, 0);
+    p.velocityRemainder[id] = f4(vl, 0);
+}
+__global__ void k_advance_cell(P p) {
+    if (ld_volatile(p.control) != 0) return;
+    v3 logarithm = xyz(p.cellLog[0]), residual = xyz(p.cellLog[1]);
+    compensated3(logarithm, residual, xyz(p.cell[1]) * p.c.dt);
+    p.cellLog[0] = f4(logarithm, 0);
+    p.cellLog[1] = f4(residual, 0);
+    v3 box = xyz(p.cell[2]) * mk3(dem_expf(logarithm.x), dem_expf(logarithm.y), dem_expf(logarithm.z));
+    p.cell[0] = f4(box, 0);
+    p.control[4] = 0;
+    if (p.c.periodic != 0 && p.referenceCell[1].w >= 0.5f * fminf(box.x, fminf(box.y, box.z))) fail(p, 256);
+}
+__global__ void k_check_rebuild(P p) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N) return;
+    v3 d = (xyz(p.position[id]) - xyz(p.referencePosition[id])) +
+           (xyz(p.positionRemainder[id]) - xyz(p.referencePositionRemainder[id]));
+    v3 box = xyz(p.cell[0]);
+    if (p.c.periodic != 0) d = d - floor3(d / box + mk3(0.5f, 0.5f, 0.5f)) * box;
+    float deformation = length(box - xyz(p.referenceCell[0]));
+    if (length(d) + deformation > 0.5f * p.c.skin) atomicOr(p.control + 4, 1u);
+}
+__global__ void k_finish_step(P p) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N || ld_volatile(p.control) != 0) return;
+    accumulateNormalLoss(p, id, 0.5f * p.c.dt);
+    float4 body = p.body[id];
+    v3 v = xyz(p.velocity[id]), vl = xyz(p.velocityRemainder[id]);
+    v3 l = xyz(p.momentum[id]), ll = xyz(p.momentumRemainder[id]);
+    compensated3(v, vl, 0.5f * p.c.dt * xyz(p.force[id]) / body.y);
+    compensated3(l, ll, 0.5f * p.c.dt * xyz(p.torque[id]));
+    float loss = -expm1Stable(-p.c.drag * p.c.dt) * (0.5f * body.y * dot(v + vl, v + vl) + rotationalEnergy(p, id, l + ll));
+    accumulateRow(p, id, {0, loss, 0.5f * p.c.dt * cellPower(p, id), 0.0f});
+    float damping = expm1Stable(-0.5f * p.c.drag * p.c.dt);
+    compensated3(v, vl, damping * (v + vl));
+    compensated3(l, ll, damping * (l + ll));
+    if (!finite3(v) || !finite3(l)) { fail(p, 512); return; }
+    p.velocity[id] = f4(v, 0);
+    p.velocityRemainder[id] = f4(vl, 0);
+    p.momentum[id] = f4(l, 0);
+    p.momentumRemainder[id] = f4(ll, 0);
+}
+__global__ void k_capture_kinematics(P p) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N || ld_volatile(p.control) != 0) return;
+    p.forceVelocity[id] = f4(xyz(p.velocity[id]) + xyz(p.velocityRemainder[id]), 0);
+    p.forceOmega[id] = f4(omega(p, id), 0);
+}
+__device__ float inverseInertiaBound(const P& p, uint32_t id) {
+    v3 one = {1, 1, 1};
+    return fmaxf(fmaxf(dot(fabs3(xyz(p.invI[3 * id])), one), dot(fabs3(xyz(p.invI[3 * id + 1])), one)),
+                 dot(fabs3(xyz(p.invI[3 * id + 2])), one));
+}
+__global__ void k_check_timestep(P p) {  // solver.slang:816-832 without planes
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N || ld_volatile(p.control) != 0 || p.c.dt == 0) return;
+    float dt = p.c.dt;
+    float rx = p.timestepRows[id].x, ry = p.timestepRows[id].y;
+    v3 v = xyz(p.velocity[id]) + xyz(p.velocityRemainder[id]);
+    float speed = length(v) + dt * length(xyz(p.force[id])) / p.body[id].y;
+    float spin = inverseInertiaBound(p, id) * (length(xyz(p.momentum[id]) + xyz(p.momentumRemainder[id])) + dt * length(xyz(p.torque[id])));
+    float motion = dt * (speed + p.body[id].z * spin + 0.0f);
+    if (!(isfinite(rx) && isfinite(ry) && isfinite(motion) && isfinite(spin))) { fail(p, 512); return; }
+    v3 rate = fabs3(xyz(p.cell[1])) * dt;
+    if (dt * sqrtf(rx) > 0.1f || dt * ry > 0.1f || dt * spin > 0.01f || motion > 0.25f * p.c.skin ||
+        rate.x > 0.01f || rate.y > 0.01f || rate.z > 0.01f)
+        fail(p, 8192);
+}
+
+// ---------------------------------------------------------------- neighbour search (solver.slang:346-392)
+__global__ void k_clear_bins(P p) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p.control[4] == 0) return;
+    if (id < p.c.binCount) p.binCounts[id] = 0;
+}
+__global__ void k_fill_bins(P p) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N || p.control[4] == 0) return;
+    v3 x = xyz(p.position[id]), xl = xyz(p.positionRemainder[id]);
+    if (!finite3(x) || !finite3(xl)) { fail(p, 512); return; }
+    if (p.c.periodic == 0) {
+        v3 s = x + xl, t = (x - xyz(p.cell[0])) + xl;
+        if (s.x < 0 || s.y < 0 || s.z < 0 || t.x >= 0 || t.y >= 0 || t.z >= 0) { fail(p, 2048); return; }
+    }
+    uint32_t b[3];
+    binIndex(p, id, b);
+    uint32_t bin = binFlat(p, b);
+    uint32_t slot = atomicAdd(p.binCounts + bin, 1u);
+    if (slot < p.c.binCapacity) p.binParticles[bin * p.c.binCapacity + slot] = id;
+    else fail(p, 8);
+}
+// Ascending ids within each bin: the order the serial fill produces.
+__global__ void k_sort_bins(P p) {
+    uint32_t bin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bin >= p.c.binCount || p.control[4] == 0) return;
+    uint32_t n = min(p.binCounts[bin], p.c.binCapacity);
+    uint32_t* a = p.binParticles + size_t(bin) * p.c.binCapacity;
+    for (uint32_t k = 1; k < n; k++) {
+        uint32_t v = a[k];
+        uint32_t m = k;
+        while (m > 0 && a[m - 1] > v) { a[m] = a[m - 1]; m--; }
+        a[m] = v;
+    }
+}
+__device__ uint32_t findPairs(const P& p, uint32_t id, bool emit) {
+    uint32_t b[3];
+    binIndex(p, id, b);
+    float cutoff = p.referenceCell[1].w;
+    v3 g = mk3(cutoff, cutoff, cutoff) / xyz(p.cell[0]) * mk3(float(p.c.bins[0]), float(p.c.bins[1]), float(p.c.bins[2]));
+    int reach[3] = {int(ceilf(g.x)), int(ceilf(g.y)), int(ceilf(g.z))};
+    int low[3], width[3];
+    for (int a = 0; a < 3; a++) {
+        low[a] = max(int(b[a]) - reach[a], 0);
+        int high = min(int(b[a]) + reach[a], int(p.c.bins[a]) - 1);
+        width[a] = p.c.periodic != 0 ? int(min(uint32_t(2 * reach[a] + 1), p.c.bins[a])) : high - low[a] + 1;
+    }
+    float4 bi = p.body[id];
+    uint32_t count = 0;
+    uint32_t base = emit ? p.pairOffsets[id] : 0;
+    for (int x = 0; x < width[0]; x++)
+        for (int y = 0; y < width[1]; y++)
+            for (int z = 0; z < width[2]; z++) {
+                uint32_t c[3];
+                if (p.c.periodic != 0) {
+                    c[0] = modIndex(int(b[0]) - reach[0] + x, p.c.bins[0]);
+                    c[1] = modIndex(int(b[1]) - reach[1] + y, p.c.bins[1]);
+                    c[2] = modIndex(int(b[2]) - reach[2] + z, p.c.bins[2]);
+                } else {
+                    c[0] = uint32_t(low[0] + x); c[1] = uint32_t(low[1] + y); c[2] = uint32_t(low[2] + z);
+                }
+                uint32_t flat = binFlat(p, c);
+                uint32_t occupancy = min(p.binCounts[flat], p.c.binCapacity);
+                const uint32_t* slots = p.binParticles + size_t(flat) * p.c.binCapacity;
+                for (uint32_t s = 0; s < occupancy; s++) {
+                    uint32_t j = slots[s];
+                    if (j <= id) continue;
+                    v3 d = centerBranch(p, id, j);
+                    float range = bi.z + p.body[j].z + p.c.skin;
+                    if (dot(d, d) > range * range) continue;
+                    if (emit) {
+                        uint32_t index = base + count;
+                        if (index < p.c.pairCapacity) p.pairs[index] = make_uint2(id, j);
+                        else fail(p, 16);
+                    }
+                    count++;
+                }
+            }
+    return count;
+}
+__global__ void k_count_pairs(P p) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < p.c.N && p.control[4] != 0) p.pairCounts[id] = findPairs(p, id, false);
+}
+__global__ void k_emit_pairs(P p) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N || p.control[4] == 0) return;
+    findPairs(p, id, true);
+    p.referencePosition[id] = p.position[id];
+    p.referencePositionRemainder[id] = p.positionRemainder[id];
+    if (id == 0) {
+        p.control[1] = p.pairOffsets[p.c.N - 1] + p.pairCounts[p.c.N - 1];
+        p.referenceCell[0] = p.cell[0];
+    }
+}
+
+// ---------------------------------------------------------------- generic exclusive scan
+// Length = min(cap, lenPtr ? *lenPtr * mul : cap); gated on control[4] when `gate`.
+struct ScanArgs {
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t* sums;
+    uint32_t cap;
+    const uint32_t* lenPtr;
+    uint32_t mul;
+    const uint32_t* gate;  // skip unless *gate != 0 (null = always)
+    uint32_t* total;       // optional total
+};
+__device__ __forceinline__ uint32_t scan_len(const ScanArgs& a) {
+    if (!a.lenPtr) return a.cap;
+    uint64_t n = uint64_t(*a.lenPtr) * a.mul;
+    return uint32_t(min<uint64_t>(n, a.cap));
+}
+__device__ __forceinline__ uint32_t wave_inclusive(uint32_t v, int lane) {
+    for (int d = 1; d < WAVE; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, WAVE);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+// Block (256) exclusive scan; returns exclusive prefix, writes block total.
+__device__ uint32_t block_exclusive(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t waveSums[4];
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t inc = wave_inclusive(v, lane);
+    if (lane == 63) waveSums[wv] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int k = 0; k < wv; k++) before += waveSums[k];
+    total = waveSums[0] + waveSums[1] + waveSums[2] + waveSums[3];
+    __syncthreads();
+    return before + inc - v;
+}
+__global__ __launch_bounds__(256) void k_scan_reduce(ScanArgs a) {
+    if (a.gate && *a.gate == 0) return;
+    uint32_t n = scan_len(a);
+    uint32_t base = blockIdx.x * SCAN_TILE;
+    if (base >= n) return;
+    uint32_t s = 0;
+    for (int k = 0; k < 4; k++) {
+        uint32_t idx = base + threadIdx.x * 4 + k;
+        if (idx < n) s += a.in[idx];
+    }
+    uint32_t total;
+    block_exclusive(s, total);
+    if (threadIdx.x == 0) a.sums[blockIdx.x] = total;
+}
+__global__ __launch_bounds__(256) void k_scan_sums(ScanArgs a) {
+    if (a.gate && *a.gate == 0) return;
+    uint32_t n = scan_len(a);
+    uint32_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    uint32_t carry = 0;
+    for (uint32_t start = 0; start < tiles; start += 256) {
+        uint32_t idx = start + threadIdx.x;
+        uint32_t v = idx < tiles ? a.sums[idx] : 0;
+        uint32_t total;
+        uint32_t ex = block_exclusive(v, total);
+        if (idx < tiles) a.sums[idx] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && a.total) *a.total = carry;
+}
+__global__ __launch_bounds__(256) void k_scan_apply(ScanArgs a) {
+    if (a.gate && *a.gate == 0) return;
+    uint32_t n = scan_len(a);
+    uint32_t base = blockIdx.x * SCAN_TILE;
+    if (base >= n) return;
+    uint32_t v[4], s = 0;
+    for (int k = 0; k < 4; k++) {
+        uint32_t idx = base + threadIdx.x * 4 + k;
+        v[k] = idx < n ? a.in[idx] : 0;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t ex = block_exclusive(s, total) + a.sums[blockIdx.x];
+    for (int k = 0; k < 4; k++) {
+        uint32_t idx = base + threadIdx.x * 4 + k;
+        if (idx < n) a.out[idx] = ex;
+        ex += v[k];
+    }
+}
+
+// ---------------------------------------------------------------- per-grain pair adjacency
+// Pairs touching each grain in ascending pair index; rebuilt every step so the
+// workspace carries no state between steps.
+__global__ void k_adj_clear(P p, W w) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N) return;
+    w.adjCount[id] = 0;
+    w.adjCursor[id] = 0;
+}
+__global__ void k_adj_count(P p, W w) {
+    uint32_t pairs = min(p.control[1], p.c.pairCapacity);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < pairs; k += gridDim.x * blockDim.x) {
+        uint2 ij = p.pairs[k];
+        atomicAdd(w.adjCount + ij.x, 1u);
+        atomicAdd(w.adjCount + ij.y, 1u);
+    }
+}
+__global__ void k_adj_fill(P p, W w) {
+    uint32_t pairs = min(p.control[1], p.c.pairCapacity);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < pairs; k += gridDim.x * blockDim.x) {
+        uint2 ij = p.pairs[k];
+        w.adjList[w.adjOffset[ij.x] + atomicAdd(w.adjCursor + ij.x, 1u)] = k;
+        w.adjList[w.adjOffset[ij.y] + atomicAdd(w.adjCursor + ij.y, 1u)] = k;
+    }
+}
+__global__ void k_adj_sort(P p, W w) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N) return;
+    uint32_t* a = w.adjList + w.adjOffset[id];
+    uint32_t n = w.adjCount[id];
+    for (uint32_t k = 1; k < n; k++) {
+        uint32_t v = a[k];
+        uint32_t m = k;
+        while (m > 0 && a[m - 1] > v) { a[m] = a[m - 1]; m--; }
+        a[m] = v;
+    }
+}
+
+// ---------------------------------------------------------------- contact collection
+__global__ void k_reset_mesh(P p, W w) {  // resetMeshContacts (solver.slang:397-403) + cursors
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id == 0) {
+        w.counters[0] = 0;
+        w.counters[1] = 0;
+    }
+    if (ld_volatile(p.control) != 0) return;
+    if (id < p.c.pairCapacity) p.contactCounts[id] = 0;
+    if (id == 0) { p.control[2] = 0; p.control[6] = 0; }
+}
+__global__ void k_prepare_pairs(P p) {  // solver.slang:469-489 (Metal frontier transforms)
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= min(p.control[1], p.c.pairCapacity) || ld_volatile(p.control) != 0) return;
+    uint2 ij = p.pairs[id];
+    uint32_t mask = 0;
+    for (uint32_t pass = 0; pass < 2; pass++) {
+        uint32_t i = pass == 0 ? ij.x : ij.y, j = pass == 0 ? ij.y : ij.x;
+        v3 branch = centerBranch(p, i, j);
+        if (meshClusterMayContact(p, v4of(p.rotation[i]), v4of(p.rotation[j]), p.body[i].x, p.body[j].x, branch,
+                                  p.shapeId[j], p.meshClusterInfo[p.shapeId[i]].x))
+            mask |= 1u << pass;
+    }
+    for (uint32_t pass = 0; pass < 2; pass++) {
+        if ((mask & (1u << pass)) == 0) continue;
+        uint32_t i = pass == 0 ? ij.x : ij.y, j = pass == 0 ? ij.y : ij.x;
+        v4 qi = v4of(p.rotation[i]), qj = v4of(p.rotation[j]);
+        float ratio = p.body[i].x / p.body[j].x;
+        v3 a = qinv(qj, qrot(qi, mk3(ratio, 0, 0)));
+        v3 b = qinv(qj, qrot(qi, mk3(0, ratio, 0)));
+        v3 c = qinv(qj, qrot(qi, mk3(0, 0, ratio)));
+        v3 branch = centerBranch(p, i, j), offset = qinv(qj, -branch) / p.body[j].x;
+        uint32_t base = 4 * (2 * id + pass);
+        p.meshPairTransform[base] = make_float4(a.x, b.x, c.x, offset.x);
+        p.meshPairTransform[base + 1] = make_float4(a.y, b.y, c.y, offset.y);
+        p.meshPairTransform[base + 2] = make_float4(a.z, b.z, c.z, offset.z);
+        p.meshPairTransform[base + 3] = make_float4(length(branch) / p.body[j].x, 0, 0, 0);
+    }
+    p.meshPairMask[id] = mask;
+}
+
+struct TaskFrame {
+    uint32_t i, j, si, sj, sampleBase;
+    v4 qi, qj;
+    float s_i, s_j, wi, wj;  // body scale and equivalent diameter
+    v3 branch;
+    float4 plane;
+};
+
+// Exact part of nodeContact (solver.slang:319-343) for one queued node.
+__device__ bool queryNode(const P& p, const TaskFrame& f, uint32_t node, uint16_t* stack, float4& pd, float4& na) {
+    float4 sp = p.sample[f.sampleBase + node];
+    v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
+    v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
+    v4 r = meshDistance(p, f.sj, local, stack);
+    float depth = -r.w * f.s_j;
+    pd = f4(arm, depth);
+    if (depth <= 0) return false;
+    v3 grad = qrot(f.qj, xyz(r));
+    float len = length(grad);
+    if (len < 1e-7f || !isfinite(depth)) { fail(p, 2); return false; }
+    na = f4(grad / len, sp.w * f.s_i * f.s_i);
+    if (depth > p.c.maxOverlap * fminf(f.wi, f.wj)) fail(p, 4);
+    return true;
+}
+
+// Rank-ordered staging of one batch of hits.
+__device__ void stageHits(const P& p, const W& w, const TaskFrame& f, uint32_t task, bool hit, uint32_t node,
+                          float4 pd, float4 na, uint32_t& running, int lane) {
+    uint64_t mask = __ballot(hit);
+    uint32_t count = __popcll(mask);
+    if (count == 0) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(w.counters + 1, count);
+    base = __shfl(base, 0);
+    if (hit) {
+        uint32_t rank = __popcll(mask & lanes_below(lane));
+        uint32_t pos = base + rank;
+        if (pos < w.stageCap) {
+            w.stageKey[pos] = make_uint2(task, running + rank);
+            w.stageMeta[pos] = make_uint4(f.i, f.j, node, 0);
+            w.stagePd[pos] = pd;
+            w.stageNa[pos] = na;
+        } else {
+            fail(p, 32);
+        }
+    }
+    running += count;
+}
+
+__device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, const uint32_t* queue,
+                           uint32_t qn, uint16_t* stack, uint32_t& running, int lane) {
+    for (uint32_t q = 0; q < qn; q += WAVE) {
+        uint32_t k = q + lane;
+        bool hit = false;
+        uint32_t node = 0;
+        float4 pd = make_float4(0, 0, 0, 0), na = make_float4(0, 0, 0, 0);
+        if (k < qn) {
+            node = queue[k];
+            hit = queryNode(p, f, node, stack, pd, na);
+        }
+        stageHits(p, w, f, task, hit, node, pd, na, running, lane);
+    }
+}
+
+// One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
+__global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
+    __shared__ uint16_t stacks[COLLECT_WAVES][STACK * WAVE];
+    __shared__ uint32_t queues[COLLECT_WAVES][QCAP];
+    __shared__ uint64_t candidates[COLLECT_WAVES][MAX_CLUSTERS / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint16_t* stack = &stacks[wv][lane];
+    uint32_t* queue = queues[wv];
+    uint64_t* cand = candidates[wv];
+    const uint32_t pairs = min(p.control[1], p.c.pairCapacity);
+    const uint32_t tasks = 2 * pairs;
+    uint32_t queried = 0;
+    for (;;) {
+        uint32_t task = 0;
+        if (lane == 0) task = atomicAdd(w.counters, 1u);
+        task = __shfl(task, 0);
+        if (task >= tasks) break;
+        uint32_t running = 0;
+        if (ld_volatile(p.control) == 0) {
+            TaskFrame f;
+            uint2 ij = p.pairs[task >> 1];
+            uint32_t pass = task & 1;
+            f.i = pass == 0 ? ij.x : ij.y;
+            f.j = pass == 0 ? ij.y : ij.x;
+            f.si = p.shapeId[f.i];
+            f.sj = p.shapeId[f.j];
+            f.sampleBase = p.sampleInfo[f.si].x;
+            f.qi = v4of(p.rotation[f.i]);
+            f.qj = v4of(p.rotation[f.j]);
+            float4 bi = p.body[f.i], bj = p.body[f.j];
+            f.s_i = bi.x; f.s_j = bj.x; f.wi = bi.w; f.wj = bj.w;
+            f.branch = centerBranch(p, f.i, f.j);
+            // Rejection plane: largest violation at the source centre, first in
+            // plane order on ties (strict >, initial -1e30), as a wave reduction
+            // over (value, sequence) pairs.
+            v3 centre = -qinv(f.qj, f.branch) / f.s_j;
+            {
+                float v0 = -1e30f;
+                uint32_t s0 = 0xffffffffu, i0 = 0xffffffffu;
+                uint32_t sq = 0;
+                for (uint32_t g = 0; g < 6; g++) {
+                    uint4 hull = p.meshHullInfo[6 * f.sj + g];
+                    for (uint32_t k = lane; k < hull.y; k += WAVE) {
+                        float4 e = p.meshPlanes[hull.x + k];
+                        float v = dot(xyz(e), centre) + e.w;
+                        if (v > v0) { v0 = v; s0 = sq + k; i0 = hull.x + k; }
+                    }
+                    sq += hull.y;
+                }
+                for (int d = 1; d < WAVE; d <<= 1) {
+                    float ov = __shfl_xor(v0, d);
+                    uint32_t os = __shfl_xor(s0, d), oi = __shfl_xor(i0, d);
+                    if (ov > v0 || (ov == v0 && os < s0)) { v0 = ov; s0 = os; i0 = oi; }
+                }
+                f.plane = i0 == 0xffffffffu ? make_float4(0, 0, 0, 0) : p.meshPlanes[i0];
+            }
+            // Cluster culls for every cluster of the source template, lane-parallel.
+            uint4 info = p.meshClusterInfo[f.si];
+            for (uint32_t base = 0; base < info.y; base += WAVE) {
+                uint32_t c = base + lane;
+                bool ok = c < info.y && meshClusterMayContact(p, f.qi, f.qj, f.s_i, f.s_j, f.branch, f.sj, info.x + c);
+                uint64_t m = __ballot(ok);
+                if (lane == 0) cand[base / WAVE] = m;
+            }
+            wave_sync();
+            // Stackless preorder walk; candidate leaves feed the query queue.
+            uint32_t qn = 0;
+            uint32_t cluster = info.x, end = info.x + info.y;
+            while (cluster < end) {
+                uint32_t rel = cluster - info.x;
+                bool candidate = (cand[rel / WAVE] >> (rel % WAVE)) & 1ull;
+                uint4 range = p.meshClusterRanges[cluster];
+                if (candidate && range.y > 0) {
+                    bool pass_ = false;
+                    uint32_t node = 0;
+                    if (uint32_t(lane) < range.y) {
+                        node = p.meshSampleIndices[range.x + lane];
+                        float4 sp = p.sample[f.sampleBase + node];
+                        v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
+                        v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
+                        float planePad = 64.0f * DEM_EPS * (1.0f + maxabs(local));
+                        pass_ = !(dot(xyz(f.plane), local) + f.plane.w > planePad) && meshMayContain(p, f.sj, local);
+                    }
+                    uint64_t m = __ballot(pass_);
+                    uint32_t n = __popcll(m);
+                    if (qn + n > QCAP) {
+                        wave_sync();
+                        flushQueue(p, w, f, task, queue, qn, stack, running, lane);
+                        queried += qn;
+                        qn = 0;
+                        wave_sync();
+                    }
+                    if (pass_) queue[qn + __popcll(m & lanes_below(lane))] = node;
+                    qn += n;
+                }
+                cluster = candidate ? cluster + 1 : range.z;
+            }
+            wave_sync();
+            flushQueue(p, w, f, task, queue, qn, stack, running, lane);
+            queried += qn;
+            wave_sync();
+        }
+        if (lane == 0) w.taskCount[task] = running;
+    }
+    if (lane == 0 && queried) atomicAdd(w.counters + 2, queried);
+}
+
+// control[2] = total contacts, per-pair counts (solver.slang:397-403 semantics).
+__global__ void k_contact_total(P p, W w) {
+    uint32_t pairs = min(p.control[1], p.c.pairCapacity);
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ld_volatile(p.control) != 0) return;
+    if (id < pairs) p.contactCounts[id] = w.taskCount[2 * id] + w.taskCount[2 * id + 1];
+    if (id == 0) {
+        uint32_t total = w.counters[4];
+        if (total > p.c.contactCapacity) { fail(p, 32); total = p.c.contactCapacity; }
+        p.control[2] = total;
+    }
+}
+__global__ void k_scatter(P p, W w) {
+    if (ld_volatile(p.control) != 0) return;
+    uint32_t staged = min(w.counters[1], w.stageCap);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < staged; k += gridDim.x * blockDim.x) {
+        uint2 key = w.stageKey[k];
+        uint32_t id = w.taskOffset[key.x] + key.y;
+        if (id < p.c.contactCapacity) {
+            p.meta[id] = w.stageMeta[k];
+            p.pointDepth[id] = w.stagePd[k];
+            p.normalArea[id] = w.stageNa[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- friction history (solver.slang:689-759)
+__device__ __forceinline__ uint32_t hashKey(uint32_t size, uint4 key) {
+    uint32_t h = key.x * 73856093u ^ key.y * 19349663u ^ key.z * 83492791u;
+    h ^= h >> 16;
+    return h & (size - 1);
+}
+__device__ v3 transported(v3 x, v3 a, v3 b, float spin) {
+    float c = fminf(fmaxf(dot(a, b), -1.0f), 1.0f);
+    if (c < -0.999999f) return {0, 0, 0};
+    v3 v = cross(a, b);
+    v3 out = x + cross(v, x) + cross(v, cross(v, x)) / (1.0f + c);
+    float cs = dem_cosf(spin), sn = dem_sinf(spin);
+    out = out * cs + cross(b, out) * sn + b * dot(b, out) * (1.0f - cs);
+    return out - b * dot(out, b);
+}
+__global__ void k_clear_storage(P p) {
+    uint32_t n = max(p.c.hashSize, p.c.N);
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x)
+        if (id < p.c.hashSize) p.hash[id] = DEM_EMPTY;
+}
+__global__ void k_resolve(P p) {
+    if (ld_volatile(p.control) != 0) return;
+    uint32_t count = min(p.control[2], p.c.contactCapacity);
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < count; id += gridDim.x * blockDim.x) {
+        uint4 key = p.meta[id];
+        uint32_t i = key.x, j = key.y;
+        float4 na = p.normalArea[id];
+        v3 n = xyz(na);
+        float viscosity = p.c.normalViscosity;
+        v3 wi = viscosity > 0 ? xyz(p.forceOmega[i]) : omega(p, i);
+        v3 wj = viscosity > 0 ? xyz(p.forceOmega[j]) : omega(p, j);
+        v3 elastic = {0, 0, 0};
+        if (p.c.historyStep != 0) {
+            uint32_t slot = hashKey(p.c.hashSize, key);
+            for (uint32_t probe = 0; probe < p.c.hashSize; probe++) {
+                uint32_t old = p.oldHash[slot];
+                if (old == DEM_EMPTY) break;
+                uint4 om = p.oldMeta[old];
+                if (om.x == key.x && om.y == key.y && om.z == key.z) {
+                    elastic = transported(xyz(p.oldSlip[old]), xyz(p.oldNormal[old]), n, 0.5f * dot(wi + wj, n) * p.c.dt);
+                    break;
+                }
+                slot = (slot + 1) & (p.c.hashSize - 1);
+            }
+        }
+        float4 pdv = p.pointDepth[id];
+        v3 arm = xyz(pdv);
+        v3 branch = centerBranch(p, i, j);
+        v3 translation = xyz(p.velocity[i]) - xyz(p.velocity[j]);
+        if (viscosity > 0) translation = xyz(p.forceVelocity[i]) - xyz(p.forceVelocity[j]);
+        v3 rv = translation - xyz(p.cell[1]) * branch + cross(wi, arm) - cross(wj, arm - branch);
+        elastic = elastic + (rv - n * dot(rv, n)) * p.c.dt;
+        v3 trial = elastic;
+        float area = na.w, depth = pdv.w;
+        float fn = 0, lossPower = 0;
+        float normalVelocity = dot(rv, n);
+        if (depth > 0) {  // normalResponse (normal.slang:3-11), weight 0.5
+            float el = 0.5f * p.c.kn * area * depth;
+            fn = viscosity == 0 ? el : fmaxf(0.0f, el - 0.5f * viscosity * area * normalVelocity);
+            lossPower = fmaxf(0.0f, (el - fn) * normalVelocity);
+        }
+        p.normalLossPower[id] = lossPower;
+        v3 ft = -0.5f * p.c.kt * area * elastic;
+        float magnitude = length(ft);
+        if (magnitude > p.c.mu * fn && magnitude > 0) {
+            ft = ft * (p.c.mu * fn / magnitude);
+            elastic = -ft / (0.5f * p.c.kt * area);
+            atomicAdd(p.control + 6, 1u);
+        }
+        p.slip[id] = f4(elastic, fmaxf(0.0f, -dot(ft, trial - elastic)));
+        float4 cf = f4(fn * n + ft, 0.25f * area * (p.c.kn * depth * depth + p.c.kt * dot(elastic, elastic)));
+        p.contactForce[id] = cf;
+        if (!isfinite(cf.x) || !isfinite(cf.y) || !isfinite(cf.z) || !isfinite(cf.w) || !isfinite(lossPower)) fail(p, 512);
+    }
+}
+__global__ void k_build_hash(P p) {
+    if (ld_volatile(p.control) != 0) return;
+    uint32_t count = min(p.control[2], p.c.contactCapacity);
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < count; id += gridDim.x * blockDim.x) {
+        uint32_t slot = hashKey(p.c.hashSize, p.meta[id]);
+        bool placed = false;
+        for (uint32_t probe = 0; probe < p.c.hashSize; probe++) {
+            if (atomicCAS(p.hash + slot, DEM_EMPTY, id) == DEM_EMPTY) { placed = true; break; }
+            slot = (slot + 1) & (p.c.hashSize - 1);
+        }
+        if (!placed) fail(p, 64);
+    }
+}
+
+// ---------------------------------------------------------------- force / torque gather (solver.slang:761-811)
+__global__ void k_gather(P p, W w) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N || ld_volatile(p.control) != 0) return;
+    v3 f = {0, 0, 0}, t = {0, 0, 0}, fLow = {0, 0, 0}, tLow = {0, 0, 0};
+    v3 stress0 = {0, 0, 0}, stress1 = {0, 0, 0}, stress2 = {0, 0, 0};
+    float energy = 0, denom = 0, maxDepth = 0, dissipation = 0, normalPower = 0;
+    float stiffnessRate = 0, viscousRate = 0;
+    float inverseInertia = p.c.activeTimestep ? inverseInertiaBound(p, id) : 0.0f;
+    float mass = p.body[id].y;
+    uint32_t first = w.adjOffset[id];
+    for (uint32_t k = w.adjCount[id]; k-- > 0;) {
+        uint32_t pair = w.adjList[first + k];
+        uint2 ij = p.pairs[pair];
+        for (int pass = 1; pass >= 0; pass--) {
+            uint32_t task = 2 * pair + pass;
+            uint32_t cnt = w.taskCount[task];
+            if (cnt == 0) continue;
+            uint32_t src = pass == 0 ? ij.x : ij.y, dst = pass == 0 ? ij.y : ij.x;
+            bool source = id == src;
+            v3 branch = centerBranch(p, src, dst);
+            float minW = fminf(p.body[src].w, p.body[dst].w);
+            uint32_t off = w.taskOffset[task];
+            for (uint32_t contact = off + cnt; contact-- > off;) {
+                float4 cf = p.contactForce[contact];
+                v3 force = xyz(cf);
+                float4 pdv = p.pointDepth[contact];
+                v3 arm = xyz(pdv);
+                if (p.c.activeTimestep != 0) {
+                    v3 localArm = source ? arm : arm - branch;
+                    float mobility = 1.0f / mass + dot(localArm, localArm) * inverseInertia;
+                    float area = p.normalArea[contact].w * 0.5f;
+                    float coupling = 2.0f;
+                    stiffnessRate += coupling * area * (p.c.kn * (1.0f + p.c.mu) + p.c.kt) * mobility;
+                    viscousRate += coupling * area * p.c.normalViscosity * (1.0f + p.c.mu) * mobility;
+                }
+                compensated3(f, fLow, source ? force : -force);
+                compensated3(t, tLow, source ? cross(arm, force) : cross(arm - branch, -force));
+                stress0 = stress0 - 0.5f * force.x * branch;
+                stress1 = stress1 - 0.5f * force.y * branch;
+                stress2 = stress2 - 0.5f * force.z * branch;
+                energy += 0.5f * cf.w;
+                denom += length(force);
+                maxDepth = fmaxf(maxDepth, pdv.w / minW);
+                dissipation += 0.5f * p.slip[contact].w;
+                normalPower += 0.5f * p.normalLossPower[contact];
+            }
+        }
+    }
+    f = f + fLow;
+    t = t + tLow;
+    p.timestepRows[id] = make_float4(stiffnessRate, viscousRate, 0, 0);
+    float4 normalState = p.normalDamping[id];
+    normalState.z = normalPower;
+    p.normalDamping[id] = normalState;
+    p.force[id] = f4(f, maxDepth);
+    p.torque[id] = f4(t, dissipation);
+    p.particleStress[3 * id] = f4(stress0, 0);
+    p.particleStress[3 * id + 1] = f4(stress1, 0);
+    p.particleStress[3 * id + 2] = f4(stress2, 0);
+    v3 vel = xyz(p.velocity[id]);
+    float kinetic = 0.5f * (mass * dot(vel, vel) + dot(xyz(p.momentum[id]), omega(p, id)));
+    p.particleDiagnostic[id] = make_float4(kinetic, energy, length(f), denom);
+    accumulateRow(p, id, {dissipation, 0, 0, 0});
+}
+
+// ---------------------------------------------------------------- probes
+__global__ void k_mesh_probe(P p, uint32_t count, uint32_t shape, const float4* points, float4* out) {
+    __shared__ uint16_t stacks[4][STACK * WAVE];
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    uint16_t* stack = &stacks[threadIdx.x >> 6][threadIdx.x & 63];
+    if (id < count) out[id] = f4(meshDistance(p, shape, xyz(points[id]), stack));
+}
+__global__ void k_cull_probe(P p, uint32_t count, uint32_t shape, const float4* points, uint32_t* out) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < count) out[id] = meshMayContain(p, shape, xyz(points[id])) ? 1 : 0;
+}
+__global__ void k_cluster_probe(P p, uint32_t count, uint32_t src, uint32_t dst, uint32_t* out) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count) return;
+    v3 branch = centerBranch(p, src, dst);
+    out[id] = meshClusterMayContact(p, v4of(p.rotation[src]), v4of(p.rotation[dst]), p.body[src].x, p.body[dst].x, branch,
+                                    p.shapeId[dst], p.meshClusterInfo[p.shapeId[src]].x + id) ? 1 : 0;
+}
+__global__ void k_pair_bounds_probe(P p, uint32_t count, uint32_t direction, uint32_t first, float4* out) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count) return;
+    uint32_t cluster = first + id;
+    v3 center = 0.5f * (xyz(p.meshClusterMin[cluster]) + xyz(p.meshClusterMax[cluster]));
+    v3 half = 0.5f * (xyz(p.meshClusterMax[cluster]) - xyz(p.meshClusterMin[cluster]));
+    float c[3], e[3];
+    for (int axis = 0; axis < 3; axis++) {
+        float4 row = p.meshPairTransform[4 * direction + axis];
+        c[axis] = dot(xyz(row), center) + row.w;
+        e[axis] = dot(fabs3(xyz(row)), half);
+    }
+    v3 low = mk3(c[0], c[1], c[2]) - mk3(e[0], e[1], e[2]), high = mk3(c[0], c[1], c[2]) + mk3(e[0], e[1], e[2]);
+    float pad = 256.0f * DEM_EPS * (1.0f + maxabs(low) + maxabs(high) + p.meshPairTransform[4 * direction + 3].x);
+    out[2 * id] = f4(low - mk3(pad, pad, pad), 0);
+    out[2 * id + 1] = f4(high + mk3(pad, pad, pad), 0);
+}
+
+// ================================================================= host side
+thread_local char g_error[256] = "no error";
+
+int hip_fail(hipError_t e, const char* where) {
+    std::snprintf(g_error, sizeof(g_error), "%s: %s", where, hipGetErrorString(e));
+    return SAD_E_HIP;
+}
+#define HIP_OK(call)                                          \
+    do {                                                      \
+        hipError_t _e = (call);                               \
+        if (_e != hipSuccess) return hip_fail(_e, #call);     \
+    } while (0)
+
+struct Scratch {
+    W w{};
+    uint64_t stats[8] = {};
+    size_t bytes = 0;
+    int device = -1;
+    int cus = 0;
+};
+
+template <class T>
+int grow(T*& ptr, uint32_t& cap, size_t need, size_t& bytes) {
+    if (ptr && cap >= need) return SAD_OK;
+    if (ptr) { (void)hipFree(ptr); bytes -= size_t(cap) * sizeof(T); ptr = nullptr; cap = 0; }
+    size_t n = std::max<size_t>(need, 1);
+    HIP_OK(hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T)));
+    cap = uint32_t(n);
+    bytes += n * sizeof(T);
+    return SAD_OK;
+}
+
+struct Caps {
+    uint32_t counters = 0, taskCount = 0, taskOffset = 0, adjCount = 0, adjOffset = 0, adjCursor = 0, adjList = 0;
+    uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0;
+};
+
+struct ScratchImpl : Scratch {
+    Caps caps;
+    int ensure(const Cfg& c) {
+        size_t tasks = size_t(2) * c.pairCapacity;
+        int rc;
+        if ((rc = grow(w.counters, caps.counters, 16, bytes))) return rc;
+        if ((rc = grow(w.taskCount, caps.taskCount, tasks, bytes))) return rc;
+        if ((rc = grow(w.taskOffset, caps.taskOffset, tasks + 1, bytes))) return rc;
+        if ((rc = grow(w.adjCount, caps.adjCount, c.N, bytes))) return rc;
+        if ((rc = grow(w.adjOffset, caps.adjOffset, c.N, bytes))) return rc;
+        if ((rc = grow(w.adjCursor, caps.adjCursor, c.N, bytes))) return rc;
+        if ((rc = grow(w.adjList, caps.adjList, tasks, bytes))) return rc;
+        size_t sums = (std::max<size_t>(tasks + 1, c.N) + SCAN_TILE - 1) / SCAN_TILE + 1;
+        if ((rc = grow(w.sums, caps.sums, sums, bytes))) return rc;
+        if ((rc = grow(w.stageKey, caps.stageKey, c.contactCapacity, bytes))) return rc;
+        if ((rc = grow(w.stageMeta, caps.stageMeta, c.contactCapacity, bytes))) return rc;
+        if ((rc = grow(w.stagePd, caps.stagePd, c.contactCapacity, bytes))) return rc;
+        if ((rc = grow(w.stageNa, caps.stageNa, c.contactCapacity, bytes))) return rc;
+        w.stageCap = c.contactCapacity;
+        w.taskCap = uint32_t(tasks);
+        w.nCap = c.N;
+        w.sumsCap = caps.sums;
+        if (device < 0) {
+            HIP_OK(hipGetDevice(&device));
+            HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        }
+        return SAD_OK;
+    }
+    ~ScratchImpl() {
+        (void)hipFree(w.counters); (void)hipFree(w.taskCount); (void)hipFree(w.taskOffset); (void)hipFree(w.adjCount);
+        (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
+        (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
+    }
+};
+
+template <class T>
+T* slot(void* const* b, int i) { return reinterpret_cast<T*>(b[i]); }
+
+int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
+    if (!b || !bytes || !v) { std::snprintf(g_error, sizeof(g_error), "null binding table"); return SAD_E_ARGUMENT; }
+    Cfg& c = p.c;
+    c.activeTimestep = uint32_t(v[SAD_V_ACTIVE_TIMESTEP]);
+    c.N = uint32_t(v[SAD_V_N]);
+    c.shapes = uint32_t(v[SAD_V_SHAPES]);
+    c.pairCapacity = uint32_t(v[SAD_V_PAIR_CAPACITY]);
+    c.contactCapacity = uint32_t(v[SAD_V_CONTACT_CAPACITY]);
+    c.binCapacity = uint32_t(v[SAD_V_BIN_CAPACITY]);
+    c.binCount = uint32_t(v[SAD_V_BIN_COUNT]);
+    c.bins[0] = uint32_t(v[SAD_V_BINS_X]);
+    c.bins[1] = uint32_t(v[SAD_V_BINS_Y]);
+    c.bins[2] = uint32_t(v[SAD_V_BINS_Z]);
+    c.hashSize = uint32_t(v[SAD_V_HASH_SIZE]);
+    c.dt = float(v[SAD_V_DT]);
+    c.kn = float(v[SAD_V_KN]);
+    c.kt = float(v[SAD_V_KT]);
+    c.mu = float(v[SAD_V_MU]);
+    c.normalViscosity = float(v[SAD_V_NORMAL_VISCOSITY]);
+    c.skin = float(v[SAD_V_SKIN]);
+    c.drag = float(v[SAD_V_DRAG]);
+    c.maxOverlap = float(v[SAD_V_MAX_OVERLAP]);
+    c.historyStep = uint32_t(v[SAD_V_HISTORY_STEP]);
+    c.meshGeometry = uint32_t(v[SAD_V_MESH_GEOMETRY]);
+    c.periodic = uint32_t(v[SAD_V_PERIODIC]);
+    c.wallCount = uint32_t(v[SAD_V_WALL_COUNT]);
+    c.wallBoxEnabled = uint32_t(v[SAD_V_WALL_BOX_ENABLED]);
+    c.sourceSurfaceNormal = uint32_t(v[SAD_V_SOURCE_SURFACE_NORMAL]);
+    if (c.meshGeometry == 0 || c.wallCount != 0 || c.sourceSurfaceNormal != 0) {
+        std::snprintf(g_error, sizeof(g_error), "only the surface-mesh step without planes is implemented");
+        return SAD_E_UNSUPPORTED;
+    }
+    if (c.N == 0 || c.hashSize == 0 || (c.hashSize & (c.hashSize - 1)) != 0) {
+        std::snprintf(g_error, sizeof(g_error), "invalid N or hash size");
+        return SAD_E_ARGUMENT;
+    }
+    struct Need { int slot; uint64_t bytes; const char* name; };
+    const Need needs[] = {
+        {SAD_CONTROL, 64, "control"}, {SAD_CELL, 48, "cell"}, {SAD_POSITION, 16ull * c.N, "position"},
+        {SAD_POSITION_REMAINDER, 16ull * c.N, "positionRemainder"}, {SAD_VELOCITY, 16ull * c.N, "velocity"},
+        {SAD_ROTATION, 16ull * c.N, "rotation"}, {SAD_MOMENTUM, 16ull * c.N, "momentum"},
+        {SAD_INV_I, 48ull * c.N, "invI"}, {SAD_BODY, 16ull * c.N, "body"}, {SAD_SHAPE_ID, 4ull * c.N, "shapeId"},
+        {SAD_PAIRS, 8ull * c.pairCapacity, "pairs"}, {SAD_CONTACT_COUNTS, 4ull * c.pairCapacity, "contactCounts"},
+        {SAD_META, 16ull * c.contactCapacity, "meta"}, {SAD_POINT_DEPTH, 16ull * c.contactCapacity, "pointDepth"},
+        {SAD_NORMAL_AREA, 16ull * c.contactCapacity, "normalArea"}, {SAD_SLIP, 16ull * c.contactCapacity, "slip"},
+        {SAD_CONTACT_FORCE, 16ull * c.contactCapacity, "contactForce"},
+        {SAD_NORMAL_LOSS_POWER, 4ull * c.contactCapacity, "normalLossPower"},
+        {SAD_HASH, 4ull * c.hashSize, "hash"}, {SAD_OLD_HASH, 4ull * c.hashSize, "oldHash"},
+        {SAD_BIN_COUNTS, 4ull * c.binCount, "binCounts"},
+        {SAD_BIN_PARTICLES, 4ull * c.binCount * c.binCapacity, "binParticles"},
+        {SAD_PARTICLE_STRESS, 48ull * c.N, "particleStress"}, {SAD_MESH_PAIR_MASK, 4ull * c.pairCapacity, "meshPairMask"},
+        {SAD_MESH_PAIR_TRANSFORM, 128ull * c.pairCapacity, "meshPairTransform"},
+    };
+    for (const Need& n : needs) {
+        if (!b[n.slot] || bytes[n.slot] < n.bytes) {
+            std::snprintf(g_error, sizeof(g_error), "buffer %s too small (%llu < %llu bytes)", n.name,
+                          (unsigned long long)bytes[n.slot], (unsigned long long)n.bytes);
+            return SAD_E_ARGUMENT;
+        }
+    }
+    p.control = slot<uint32_t>(b, SAD_CONTROL);
+    p.cell = slot<float4>(b, SAD_CELL);
+    p.position = slot<float4>(b, SAD_POSITION);
+    p.positionRemainder = slot<float4>(b, SAD_POSITION_REMAINDER);
+    p.velocityRemainder = slot<float4>(b, SAD_VELOCITY_REMAINDER);
+    p.momentumRemainder = slot<float4>(b, SAD_MOMENTUM_REMAINDER);
+    p.rotationRemainder = slot<float4>(b, SAD_ROTATION_REMAINDER);
+    p.referencePositionRemainder = slot<float4>(b, SAD_REFERENCE_POSITION_REMAINDER);
+    p.cellLog = slot<float4>(b, SAD_CELL_LOG);
+    p.accumulatedRemainder = slot<float4>(b, SAD_ACCUMULATED_REMAINDER);
+    p.velocity = slot<float4>(b, SAD_VELOCITY);
+    p.rotation = slot<float4>(b, SAD_ROTATION);
+    p.momentum = slot<float4>(b, SAD_MOMENTUM);
+    p.force = slot<float4>(b, SAD_FORCE);
+    p.torque = slot<float4>(b, SAD_TORQUE);
+    p.referencePosition = slot<float4>(b, SAD_REFERENCE_POSITION);
+    p.referenceCell = slot<float4>(b, SAD_REFERENCE_CELL);
+    p.shapeId = slot<const uint32_t>(b, SAD_SHAPE_ID);
+    p.body = slot<const float4>(b, SAD_BODY);
+    p.invI = slot<const float4>(b, SAD_INV_I);
+    p.sampleInfo = slot<const uint4>(b, SAD_SAMPLE_INFO);
+    p.sample = slot<const float4>(b, SAD_SAMPLE);
+    p.meshRoots = slot<const uint4>(b, SAD_MESH_ROOTS);
+    p.meshHullInfo = slot<const uint4>(b, SAD_MESH_HULL_INFO);
+    p.meshPlanes = slot<const float4>(b, SAD_MESH_PLANES);
+    p.meshMin = slot<const float4>(b, SAD_MESH_MIN);
+    p.meshMax = slot<const float4>(b, SAD_MESH_MAX);
+    p.meshLinks = slot<const uint4>(b, SAD_MESH_LINKS);
+    p.meshTriangles = slot<const float4>(b, SAD_MESH_TRIANGLES);
+    p.meshNormals = slot<const float4>(b, SAD_MESH_NORMALS);
+    p.meshPairMask = slot<uint32_t>(b, SAD_MESH_PAIR_MASK);
+    p.meshPairTransform = slot<float4>(b, SAD_MESH_PAIR_TRANSFORM);
+    p.meshClusterInfo = slot<const uint4>(b, SAD_MESH_CLUSTER_INFO);
+    p.meshClusterRanges = slot<const uint4>(b, SAD_MESH_CLUSTER_RANGES);
+    p.meshClusterMin = slot<const float4>(b, SAD_MESH_CLUSTER_MIN);
+    p.meshClusterMax = slot<const float4>(b, SAD_MESH_CLUSTER_MAX);
+    p.meshSampleIndices = slot<const uint32_t>(b, SAD_MESH_SAMPLE_INDICES);
+    p.binCounts = slot<uint32_t>(b, SAD_BIN_COUNTS);
+    p.binParticles = slot<uint32_t>(b, SAD_BIN_PARTICLES);
+    p.pairCounts = slot<uint32_t>(b, SAD_PAIR_COUNTS);
+    p.pairOffsets = slot<uint32_t>(b, SAD_PAIR_OFFSETS);
+    p.pairs = slot<uint2>(b, SAD_PAIRS);
+    p.contactCounts = slot<uint32_t>(b, SAD_CONTACT_COUNTS);
+    p.meta = slot<uint4>(b, SAD_META);
+    p.pointDepth = slot<float4>(b, SAD_POINT_DEPTH);
+    p.normalArea = slot<float4>(b, SAD_NORMAL_AREA);
+    p.slip = slot<float4>(b, SAD_SLIP);
+    p.contactForce = slot<float4>(b, SAD_CONTACT_FORCE);
+    p.normalLossPower = slot<float>(b, SAD_NORMAL_LOSS_POWER);
+    p.normalDamping = slot<float4>(b, SAD_NORMAL_DAMPING);
+    p.forceVelocity = slot<float4>(b, SAD_FORCE_VELOCITY);
+    p.forceOmega = slot<float4>(b, SAD_FORCE_OMEGA);
+    p.oldMeta = slot<const uint4>(b, SAD_OLD_META);
+    p.oldSlip = slot<const float4>(b, SAD_OLD_SLIP);
+    p.oldNormal = slot<const float4>(b, SAD_OLD_NORMAL);
+    p.oldHash = slot<const uint32_t>(b, SAD_OLD_HASH);
+    p.hash = slot<uint32_t>(b, SAD_HASH);
+    p.particleStress = slot<float4>(b, SAD_PARTICLE_STRESS);
+    p.timestepRows = slot<float4>(b, SAD_TIMESTEP_ROWS);
+    p.particleDiagnostic = slot<float4>(b, SAD_PARTICLE_DIAGNOSTIC);
+    p.accumulated = slot<float4>(b, SAD_ACCUMULATED);
+    return SAD_OK;
+}
+
+inline unsigned blocks(uint64_t n, unsigned t = 256) { return unsigned(std::max<uint64_t>(1, (n + t - 1) / t)); }
+
+int launch_scan(const ScanArgs& a, uint32_t cap, hipStream_t s) {
+    unsigned tiles = blocks(cap, SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(256), 0, s, a);
+    return SAD_OK;
+}
+
+// Reference kernels by name (gpu.py:547-565 / solver.slang entry points).
+int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
+    const Cfg& c = p.c;
+    W& w = sc.w;
+    unsigned gridStride = std::max(1, sc.cus) * 8;
+    if (name == "beginStep") hipLaunchKernelGGL(k_begin_step, dim3(blocks(c.N)), dim3(256), 0, s, p);
+    else if (name == "advanceCell") hipLaunchKernelGGL(k_advance_cell, dim3(1), dim3(1), 0, s, p);
+    else if (name == "checkRebuild") hipLaunchKernelGGL(k_check_rebuild, dim3(blocks(c.N)), dim3(256), 0, s, p);
+    else if (name == "clearBins") hipLaunchKernelGGL(k_clear_bins, dim3(blocks(c.binCount)), dim3(256), 0, s, p);
+    else if (name == "fillBins") {
+        hipLaunchKernelGGL(k_fill_bins, dim3(blocks(c.N)), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(k_sort_bins, dim3(blocks(c.binCount)), dim3(256), 0, s, p);
+    } else if (name == "countPairs") hipLaunchKernelGGL(k_count_pairs, dim3(blocks(c.N)), dim3(256), 0, s, p);
+    else if (name == "pairScan") {
+        ScanArgs a{p.pairCounts, p.pairOffsets, w.sums, c.N, nullptr, 1, p.control + 4, nullptr};
+        launch_scan(a, c.N, s);
+    } else if (name == "emitPairs") {
+        hipLaunchKernelGGL(k_emit_pairs, dim3(blocks(c.N)), dim3(256), 0, s, p);
+    } else if (name == "resetMeshContacts") {
+        hipLaunchKernelGGL(k_reset_mesh, dim3(blocks(c.pairCapacity)), dim3(256), 0, s, p, w);
+    } else if (name == "prepareMeshPairs") {
+        hipLaunchKernelGGL(k_prepare_pairs, dim3(blocks(c.pairCapacity)), dim3(256), 0, s, p);
+    } else if (name == "collectMeshFrontiers") {
+        hipLaunchKernelGGL(k_adj_clear, dim3(blocks(c.N)), dim3(256), 0, s, p, w);
+        hipLaunchKernelGGL(k_adj_count, dim3(gridStride), dim3(256), 0, s, p, w);
+        ScanArgs adj{w.adjCount, w.adjOffset, w.sums, c.N, nullptr, 1, nullptr, nullptr};
+        launch_scan(adj, c.N, s);
+        hipLaunchKernelGGL(k_adj_fill, dim3(gridStride), dim3(256), 0, s, p, w);
+        hipLaunchKernelGGL(k_adj_sort, dim3(blocks(c.N)), dim3(256), 0, s, p, w);
+        unsigned persistent = std::max(1, sc.cus) * 6;
+        hipLaunchKernelGGL(k_collect, dim3(persistent), dim3(COLLECT_WAVES * WAVE), 0, s, p, w);
+        ScanArgs a{w.taskCount, w.taskOffset, w.sums, 2 * c.pairCapacity, p.control + 1, 2, nullptr, w.counters + 4};
+        launch_scan(a, 2 * c.pairCapacity, s);
+        hipLaunchKernelGGL(k_contact_total, dim3(blocks(c.pairCapacity)), dim3(256), 0, s, p, w);
+        hipLaunchKernelGGL(k_scatter, dim3(gridStride), dim3(256), 0, s, p, w);
+    } else if (name == "clearContactStorage") {
+        hipLaunchKernelGGL(k_clear_storage, dim3(gridStride), dim3(256), 0, s, p);
+    } else if (name == "resolveContacts") hipLaunchKernelGGL(k_resolve, dim3(gridStride), dim3(256), 0, s, p);
+    else if (name == "buildHistoryHash") hipLaunchKernelGGL(k_build_hash, dim3(gridStride), dim3(256), 0, s, p);
+    else if (name == "gather") hipLaunchKernelGGL(k_gather, dim3(blocks(c.N)), dim3(256), 0, s, p, w);
+    else if (name == "finishStep") hipLaunchKernelGGL(k_finish_step, dim3(blocks(c.N)), dim3(256), 0, s, p);
+    else if (name == "captureContactKinematics") hipLaunchKernelGGL(k_capture_kinematics, dim3(blocks(c.N)), dim3(256), 0, s, p);
+    else if (name == "checkContactTimestep") hipLaunchKernelGGL(k_check_timestep, dim3(blocks(c.N)), dim3(256), 0, s, p);
+    else {
+        std::snprintf(g_error, sizeof(g_error), "unknown kernel %s", name.c_str());
+        return SAD_E_ARGUMENT;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, name.c_str());
+    return SAD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* sad_hip_scratch_create(void) {
+    try { return new ScratchImpl(); } catch (...) { return nullptr; }
+}
+void sad_hip_scratch_destroy(void* scratch) { delete static_cast<ScratchImpl*>(scratch); }
+int sad_hip_scratch_stats(void* scratch, uint64_t* stats) {
+    if (!scratch || !stats) return SAD_E_ARGUMENT;
+    auto* sc = static_cast<ScratchImpl*>(scratch);
+    uint32_t counters[4] = {0, 0, 0, 0};
+    if (sc->w.counters) {
+        hipError_t e = hipMemcpy(counters, sc->w.counters, sizeof(counters), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "stats");
+    }
+    stats[0] = counters[0];
+    stats[1] = counters[2];
+    stats[2] = counters[1];
+    stats[3] = sc->bytes;
+    for (int k = 4; k < 8; k++) stats[k] = sc->stats[k];
+    return SAD_OK;
+}
+
+int sad_hip_dispatch(const char* kernel, uint32_t count, void* const* buffers, const uint64_t* bytes,
+                     const double* values, void* scratch, void* stream) {
+    (void)count;
+    if (!kernel || !scratch) return SAD_E_ARGUMENT;
+    try {
+        P p{};
+        int rc = bind(p, buffers, bytes, values);
+        if (rc) return rc;
+        auto* sc = static_cast<ScratchImpl*>(scratch);
+        if ((rc = sc->ensure(p.c))) return rc;
+        return run_kernel(kernel, p, *sc, static_cast<hipStream_t>(stream));
+    } catch (...) {
+        std::snprintf(g_error, sizeof(g_error), "exception in dispatch");
+        return SAD_E_HIP;
+    }
+}
+
+int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const uint64_t* bytes,
+                    const double* values, void* const* sides, const uint64_t* side_bytes, void* scratch,
+                    void* stream, int graph) {
+    (void)graph;
+    if (!scratch || !sides || !side_bytes) return SAD_E_ARGUMENT;
+    try {
+        void* table[SAD_BUFFER_COUNT];
+        uint64_t sizes[SAD_BUFFER_COUNT];
+        std::memcpy(table, buffers, sizeof(table));
+        std::memcpy(sizes, bytes, sizeof(sizes));
+        double vals[SAD_VALUE_COUNT];
+        std::memcpy(vals, values, sizeof(vals));
+        auto* sc = static_cast<ScratchImpl*>(scratch);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        static const char* sequence[] = {"beginStep", "advanceCell", "checkRebuild", "clearBins", "fillBins",
+                                         "countPairs", "pairScan", "emitPairs", "resetMeshContacts",
+                                         "collectMeshFrontiers", "clearContactStorage", "resolveContacts",
+                                         "buildHistoryHash", "gather"};
+        for (uint32_t k = 0; k < steps; k++) {
+            uint32_t side = (active + k) & 1, old = 1 - side;
+            const int slotsNew[4] = {SAD_META, SAD_SLIP, SAD_NORMAL_AREA, SAD_HASH};
+            const int slotsOld[4] = {SAD_OLD_META, SAD_OLD_SLIP, SAD_OLD_NORMAL, SAD_OLD_HASH};
+            for (int q = 0; q < SAD_SIDE_COUNT; q++) {
+                table[slotsNew[q]] = sides[side * SAD_SIDE_COUNT + q];
+                sizes[slotsNew[q]] = side_bytes[side * SAD_SIDE_COUNT + q];
+                table[slotsOld[q]] = sides[old * SAD_SIDE_COUNT + q];
+                sizes[slotsOld[q]] = side_bytes[old * SAD_SIDE_COUNT + q];
+            }
+            if (k > 0) vals[SAD_V_HISTORY_STEP] = 1;
+            P p{};
+            int rc = bind(p, table, sizes, vals);
+            if (rc) return rc;
+            if ((rc = sc->ensure(p.c))) return rc;
+            if (p.c.activeTimestep && (rc = run_kernel("checkContactTimestep", p, *sc, s))) return rc;
+            for (const char* name : sequence) {
+                if (std::strcmp(name, "clearBins") == 0 && p.c.normalViscosity > 0 &&
+                    (rc = run_kernel("captureContactKinematics", p, *sc, s)))
+                    return rc;
+                if ((rc = run_kernel(name, p, *sc, s))) return rc;
+            }
+            if (p.c.activeTimestep && (rc = run_kernel("checkContactTimestep", p, *sc, s))) return rc;
+            if ((rc = run_kernel("finishStep", p, *sc, s))) return rc;
+        }
+        return SAD_OK;
+    } catch (...) {
+        std::snprintf(g_error, sizeof(g_error), "exception in advance");
+        return SAD_E_HIP;
+    }
+}
+
+int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buffers, const uint64_t* bytes,
+                  const double* values, const void* input, void* output, void* stream) {
+    if (!kernel || !uniforms || !output) return SAD_E_ARGUMENT;
+    P p{};
+    int rc = bind(p, buffers, bytes, values);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::string name(kernel);
+    uint32_t count = uniforms[0];
+    if (count == 0) return SAD_OK;
+    if (name == "meshProbe")
+        hipLaunchKernelGGL(k_mesh_probe, dim3(blocks(count)), dim3(256), 0, s, p, count, uniforms[1],
+                           static_cast<const float4*>(input), static_cast<float4*>(output));
+    else if (name == "meshCullProbe")
+        hipLaunchKernelGGL(k_cull_probe, dim3(blocks(count)), dim3(256), 0, s, p, count, uniforms[1],
+                           static_cast<const float4*>(input), static_cast<uint32_t*>(output));
+    else if (name == "meshClusterProbe")
+        hipLaunchKernelGGL(k_cluster_probe, dim3(blocks(count)), dim3(256), 0, s, p, count, uniforms[1], uniforms[2],
+                           static_cast<uint32_t*>(output));
+    else if (name == "meshPairBoundsProbe")
+        hipLaunchKernelGGL(k_pair_bounds_probe, dim3(blocks(count)), dim3(256), 0, s, p, count, uniforms[1],
+                           uniforms[2], static_cast<float4*>(output));
+    else {
+        std::snprintf(g_error, sizeof(g_error), "unknown probe %s", kernel);
+        return SAD_E_ARGUMENT;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, kernel);
+    return SAD_OK;
+}
+
+const char* sad_hip_last_error(void) { return g_error; }
+int sad_hip_abi_version(void) { return 1; }
+
+}  // extern "C"

@@ -272,6 +272,11 @@ class Engine:
 
     def _static_geometry(self):  # gpu.py:421-488, mesh branch
         mesh = pack_meshes(self.shapes, self.length_unit)
+        roots = mesh["meshRoots"]
+        if np.any(roots[:, 1] - roots[:, 0] > 65535):
+            raise ValueError("A template BVH exceeds 65,535 nodes (device traversal stack holds 16-bit offsets)")
+        if np.any(mesh["meshClusterInfo"][:, 1] > 1024):
+            raise ValueError("A template has more than 1,024 sample clusters (about 32k nodes)")
         for name, data in mesh.items():
             self.add(name, data)
         info, samples = sample_arrays(self.shapes, self.length_unit)

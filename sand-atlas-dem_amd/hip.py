@@ -1,0 +1,213 @@
+"""The `hip` backend adapter: device buffers in HBM, kernels through the C ABI.
+
+Implements the duck-typed adapter protocol the reference `Engine` drives
+(`src/sand_atlas_dem/backend.py:9-121` for Metal, `cpu.py:540-689` for CPU):
+buffer / kernel dispatch / command encoder with copy_buffer / submit / wait / read /
+read_prefix / write_prefix / doctor.  Device memory and the stream come from
+PyTorch-ROCm; every kernel is the hand-written gfx950 code in `csrc/`, reached via
+ctypes.  There is no CPU fallback: a missing library or device raises.
+"""
+
+import ctypes as ct
+import platform
+
+import numpy as np
+
+from . import native
+from .engine import BUFFER_ORDER, config_values
+
+SIDE_ORDER = ("meta", "slip", "normalArea", "hash")
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("The hip backend needs a ROCm GPU; none is visible")
+    return torch
+
+
+class Buffer:
+    """A device allocation (uint8 tensor) with the reference Buffer interface."""
+
+    def __init__(self, backend, array=None, *, count=None, stride=4):
+        torch = backend.torch
+        if array is not None:
+            host = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
+            self.tensor = torch.from_numpy(host.copy()).to(backend.torch_device)
+        else:
+            self.tensor = torch.zeros(max(4, count * stride), dtype=torch.uint8, device=backend.torch_device)
+        self.size = int(self.tensor.numel())
+        self.ptr = int(self.tensor.data_ptr())
+
+    def copy_from_numpy(self, array):
+        data = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
+        if data.size != self.size:
+            raise ValueError("HIP buffer upload size mismatch")
+        import torch
+
+        self.tensor.copy_(torch.from_numpy(data.copy()))
+
+    def to_numpy(self):
+        return self.tensor.cpu().numpy().copy()
+
+
+class Encoder:
+    """Records copies and kernel launches; `submit` issues them in order."""
+
+    def __init__(self):
+        self.operations = []
+
+    def copy_buffer(self, dest, dest_offset, source, source_offset, size):
+        if min(dest_offset, source_offset, size) < 0 or dest_offset + size > dest.size or source_offset + size > source.size:
+            raise ValueError("HIP buffer copy out of bounds")
+        self.operations.append(
+            lambda: dest.tensor[dest_offset : dest_offset + size].copy_(source.tensor[source_offset : source_offset + size])
+        )
+
+
+def _table(bindings):
+    ptrs = (ct.c_void_p * len(BUFFER_ORDER))(*(bindings[name].ptr for name in BUFFER_ORDER))
+    sizes = (ct.c_uint64 * len(BUFFER_ORDER))(*(bindings[name].size for name in BUFFER_ORDER))
+    return ptrs, sizes
+
+
+class HipBackend:
+    name = "hip"
+
+    def __init__(self, device=0, graph=False):
+        self.torch = _torch()
+        self.torch_device = self.torch.device("cuda", device)
+        self.torch.cuda.set_device(self.torch_device)
+        self.device = self  # the Engine reaches encoders and wait() via m.device (gpu.py:581, 727)
+        self.lib = native.library()
+        self.scratch = self.lib.sad_hip_scratch_create()
+        if not self.scratch:
+            raise MemoryError("HIP scratch creation failed")
+        self.graph = graph
+        self.buffer_type = Buffer
+
+    def __del__(self):
+        lib, scratch = getattr(self, "lib", None), getattr(self, "scratch", None)
+        if lib is not None and scratch:
+            try:
+                self.torch.cuda.synchronize()
+            except Exception:
+                pass
+            lib.sad_hip_scratch_destroy(scratch)
+            self.scratch = None
+
+    # ---- adapter protocol
+    def buffer(self, array=None, *, count=None, stride=4):
+        return Buffer(self, array, count=count, stride=stride)
+
+    def create_command_encoder(self):
+        return Encoder()
+
+    @property
+    def stream(self):
+        return ct.c_void_p(self.torch.cuda.current_stream(self.torch_device).cuda_stream)
+
+    def _check(self, status, what):
+        if status:
+            raise RuntimeError(f"{what} failed ({status}): {self.lib.sad_hip_last_error().decode()}")
+
+    def dispatch(self, name, n, encoder=None, *, r):
+        def execute():
+            ptrs, sizes = _table(r)
+            values = config_values(r["c"])
+            self._check(
+                self.lib.sad_hip_dispatch(
+                    name.encode(), int(n), ptrs, sizes, values.ctypes.data_as(ct.POINTER(ct.c_double)),
+                    self.scratch, self.stream,
+                ),
+                f"dispatch {name}",
+            )
+
+        if encoder is None:
+            execute()
+        else:
+            encoder.operations.append(execute)
+
+    def encode_steps(self, encoder, steps, active, bindings, history_buffers):
+        """Whole timesteps in one native call (per-step order of gpu.py:714-726)."""
+
+        def execute():
+            ptrs, sizes = _table(bindings)
+            values = config_values(bindings["c"])
+            sides = [history_buffers[s][name] for s in (0, 1) for name in SIDE_ORDER]
+            sp = (ct.c_void_p * 8)(*(b.ptr for b in sides))
+            ss = (ct.c_uint64 * 8)(*(b.size for b in sides))
+            self._check(
+                self.lib.sad_hip_advance(
+                    steps, active, ptrs, sizes, values.ctypes.data_as(ct.POINTER(ct.c_double)), sp, ss,
+                    self.scratch, self.stream, int(self.graph),
+                ),
+                "advance",
+            )
+
+        encoder.operations.append(execute)
+
+    def probe(self, name, uniforms, bindings, data, out):
+        """Run a query probe on device; `data`/`out` are host arrays."""
+        torch = self.torch
+        src = None if data is None else torch.from_numpy(np.ascontiguousarray(data).copy()).to(self.torch_device)
+        dst = torch.zeros(out.nbytes, dtype=torch.uint8, device=self.torch_device)
+        ptrs, sizes = _table(bindings)
+        values = config_values(bindings["c"])
+        u = (ct.c_uint32 * 4)(*(list(uniforms) + [0] * (4 - len(uniforms))))
+        self._check(
+            self.lib.sad_hip_probe(
+                name.encode(), u, ptrs, sizes, values.ctypes.data_as(ct.POINTER(ct.c_double)),
+                None if src is None else ct.c_void_p(src.data_ptr()), ct.c_void_p(dst.data_ptr()), self.stream,
+            ),
+            f"probe {name}",
+        )
+        out.view(np.uint8).reshape(-1)[:] = dst.cpu().numpy()
+        return out
+
+    def submit(self, encoder):
+        for operation in encoder.operations:
+            operation()
+        encoder.operations.clear()
+
+    def wait(self):
+        self.torch.cuda.synchronize(self.torch_device)
+
+    def read(self, buffer, dtype=np.float32, shape=None):
+        array = buffer.tensor.cpu().numpy().view(dtype).copy()
+        return array if shape is None else array.reshape(shape)
+
+    def read_prefix(self, buffer, count, dtype=np.float32, shape=None):
+        if type(count) is not int or count < 0 or count * np.dtype(dtype).itemsize > buffer.size:
+            raise ValueError("HIP buffer prefix out of bounds")
+        n = count * np.dtype(dtype).itemsize
+        array = buffer.tensor[:n].cpu().numpy().view(dtype).copy()
+        return array if shape is None else array.reshape(shape)
+
+    def write_prefix(self, buffer, array):
+        data = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
+        if data.size > buffer.size:
+            raise ValueError("HIP buffer prefix out of bounds")
+        if data.size:
+            buffer.tensor[: data.size].copy_(self.torch.from_numpy(data.copy()))
+
+    def stats(self):
+        out = (ct.c_uint64 * 8)()
+        self._check(self.lib.sad_hip_scratch_stats(self.scratch, out), "stats")
+        keys = ("tasks", "queried_nodes", "staged_contacts", "scratch_bytes", "rebuilds", "culled", "audited", "misses")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def doctor(self):
+        props = self.torch.cuda.get_device_properties(self.torch_device)
+        return dict(
+            format="sand-dem-doctor/1",
+            backend="hip",
+            fallback=False,
+            adapter=props.name,
+            gcn_arch=getattr(props, "gcnArchName", ""),
+            compute_units=props.multi_processor_count,
+            abi_version=int(self.lib.sad_hip_abi_version()),
+            numerics="hand-written gfx950 HIP, FP32 compensated state, CPU-order deterministic collection",
+            host=platform.processor() or platform.machine(),
+        )

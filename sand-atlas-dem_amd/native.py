@@ -1,0 +1,81 @@
+"""Build and load the in-tree HIP library (`libsad_hip.so`, gfx950).
+
+The library is compiled with hipcc directly (no JIT cache), in-tree, so the same
+file is what tests, the smoke check and the benchmark load.  FP32 semantics are
+pinned by the flags: no contraction, correctly rounded divide/sqrt, denormals kept,
+packed-FP32 instructions disabled.
+"""
+
+import ctypes as ct
+import os
+import subprocess
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+SOURCE = HERE / "csrc" / "sad_hip.hip"
+LIBRARY = HERE / "libsad_hip.so"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def compile_library(output=LIBRARY):
+    """Compile csrc/ into `output` (always rebuilds)."""
+    tmp = Path(str(output) + ".tmp")
+    cmd = [HIPCC, *FLAGS, "-o", str(tmp), str(SOURCE)]
+    result = subprocess.run(cmd, capture_output=True, text=True)
+    if result.returncode:
+        raise RuntimeError(f"hipcc failed ({result.returncode}):\n{result.stderr[-4000:]}")
+    tmp.replace(output)
+    return output
+
+
+_LIB = None
+
+
+def library():
+    """The loaded HIP library; raises if it has not been built (no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not LIBRARY.exists():
+            raise FileNotFoundError(f"{LIBRARY} is missing: build it with __graft_entry__.build() first")
+        lib = ct.CDLL(str(LIBRARY))
+        vp, u32, u64p, dp = ct.c_void_p, ct.c_uint32, ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_double)
+        vpp = ct.POINTER(ct.c_void_p)
+        signatures = {
+            "sad_hip_scratch_create": (vp, []),
+            "sad_hip_scratch_destroy": (None, [vp]),
+            "sad_hip_scratch_stats": (ct.c_int, [vp, u64p]),
+            "sad_hip_dispatch": (ct.c_int, [ct.c_char_p, u32, vpp, u64p, dp, vp, vp]),
+            "sad_hip_advance": (ct.c_int, [u32, u32, vpp, u64p, dp, vpp, u64p, vp, vp, ct.c_int]),
+            "sad_hip_probe": (ct.c_int, [ct.c_char_p, ct.POINTER(u32), vpp, u64p, dp, vp, vp, vp]),
+            "sad_hip_last_error": (ct.c_char_p, []),
+            "sad_hip_abi_version": (ct.c_int, []),
+        }
+        for name, (res, args) in signatures.items():
+            fn = getattr(lib, name)
+            fn.restype, fn.argtypes = res, args
+        _LIB = lib
+    return _LIB
+
+
+EXPORTED = (
+    "sad_hip_scratch_create",
+    "sad_hip_scratch_destroy",
+    "sad_hip_scratch_stats",
+    "sad_hip_dispatch",
+    "sad_hip_advance",
+    "sad_hip_probe",
+    "sad_hip_last_error",
+    "sad_hip_abi_version",
+)
