@@ -78,9 +78,18 @@ enum sad_side { SAD_SIDE_META = 0, SAD_SIDE_SLIP, SAD_SIDE_NORMAL_AREA, SAD_SIDE
  * demand, never shrinks.  Returns NULL on failure. */
 void* sad_hip_scratch_create(void);
 void sad_hip_scratch_destroy(void* scratch);
-/* stats[0..7]: tasks, queried nodes, staged contacts, scratch bytes, rebuilds,
- * culled nodes (learned cull), audited nodes, audit misses. */
+/* stats[0..7]: cumulative steps (collections), directed candidate pairs, exactly
+ * queried nodes and contacts since creation; workspace bytes; three reserved. */
 int sad_hip_scratch_stats(void* scratch, uint64_t* stats);
+
+/* Per-kernel HIP-event timing on the launch stream (no reference counterpart;
+ * the reference's Engine.profile_step, gpu.py:938-1014, times stage groups).
+ * enable != 0 clears the accumulators and brackets every launch with events. */
+int sad_hip_profile(void* scratch, int enable);
+/* Copies accumulated milliseconds and launch counts per kernel id (up to
+ * `capacity`); returns the number of kernel ids or a negative error. */
+int sad_hip_profile_read(void* scratch, double* ms, uint64_t* launches, uint32_t capacity);
+const char* sad_hip_kernel_name(uint32_t kernel);
 
 /* One reference kernel by name, `count` threads as the reference dispatches it
  * (gpu.py:529-565).  Replaces cpu_dispatch (cpu.py:391-428) / Metal dispatch

@@ -191,13 +191,13 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
                                    nodes=nodes or 2048, rng=rng["templates"], seed=seed)
         phi, sphere_fraction, vsig, rate = 0.60, 0.62, 0.0, (-0.5, -0.5, -0.5)
         steps = steps or 2000
-        cpp = 256
+        cpp = 512  # ~190 node contacts per grain at phi 0.60: headroom over the default 256
     elif config == "uneven":
         n = n or 50_000
         shapes = uneven_library(templates or 424, sigma=0.15, diameter=d0, rng=rng["templates"], seed=seed)
         phi, sphere_fraction, vsig, rate = 0.60, 0.62, 0.0, (0, 0, 0)
         steps = steps or 1000
-        cpp = 256
+        cpp = 512
     else:
         raise ValueError(f"unknown workload {config!r}")
     ids = rng["shapes"].integers(0, len(shapes), n).astype(np.uint32)

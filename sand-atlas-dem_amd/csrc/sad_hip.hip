@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/sand_dem_hip.h"
 #include "dem_math.h"
@@ -82,6 +83,7 @@ struct W {
     uint4* stageMeta;     // C
     float4* stagePd;      // C
     float4* stageNa;      // C
+    unsigned long long* totals;  // steps, directed pairs, queried nodes, contacts (cumulative)
     uint32_t stageCap, taskCap, nCap, sumsCap;
 };
 
@@ -575,6 +577,7 @@ __global__ void k_reset_mesh(P p, W w) {  // resetMeshContacts (solver.slang:397
     if (id == 0) {
         w.counters[0] = 0;
         w.counters[1] = 0;
+        w.counters[2] = 0;
     }
     if (ld_volatile(p.control) != 0) return;
     if (id < p.c.pairCapacity) p.contactCounts[id] = 0;
@@ -791,6 +794,10 @@ __global__ void k_contact_total(P p, W w) {
         uint32_t total = w.counters[4];
         if (total > p.c.contactCapacity) { fail(p, 32); total = p.c.contactCapacity; }
         p.control[2] = total;
+        atomicAdd(w.totals + 0, 1ull);
+        atomicAdd(w.totals + 1, 2ull * pairs);
+        atomicAdd(w.totals + 2, (unsigned long long)w.counters[2]);
+        atomicAdd(w.totals + 3, (unsigned long long)total);
     }
 }
 __global__ void k_scatter(P p, W w) {
@@ -1001,6 +1008,23 @@ __global__ void k_pair_bounds_probe(P p, uint32_t count, uint32_t direction, uin
 // ================================================================= host side
 thread_local char g_error[256] = "no error";
 
+// Kernel identities for the per-kernel HIP-event profile.
+#define SAD_KERNELS(X) X(begin_step) X(advance_cell) X(check_rebuild) X(clear_bins) X(fill_bins) X(sort_bins) \
+    X(count_pairs) X(scan_reduce) X(scan_sums) X(scan_apply) X(emit_pairs) X(adj_clear) X(adj_count) X(adj_fill) \
+    X(adj_sort) X(reset_mesh) X(prepare_pairs) X(collect) X(contact_total) X(scatter) X(clear_storage) X(resolve) \
+    X(build_hash) X(gather) X(finish_step) X(capture_kinematics) X(check_timestep)
+enum KernelId {
+#define SAD_ID(name) KID_##name,
+    SAD_KERNELS(SAD_ID)
+#undef SAD_ID
+    KID_COUNT
+};
+const char* const kKernelNames[] = {
+#define SAD_NAME(name) "k_" #name,
+    SAD_KERNELS(SAD_NAME)
+#undef SAD_NAME
+};
+
 int hip_fail(hipError_t e, const char* where) {
     std::snprintf(g_error, sizeof(g_error), "%s: %s", where, hipGetErrorString(e));
     return SAD_E_HIP;
@@ -1017,6 +1041,14 @@ struct Scratch {
     size_t bytes = 0;
     int device = -1;
     int cus = 0;
+    unsigned long long* totals = nullptr;  // device: steps, directed pairs, queried nodes, contacts
+    // HIP-event profile: pending (kernel, start, stop) triples resolved on read.
+    bool profiling = false;
+    struct Mark { int kernel; hipEvent_t start, stop; };
+    std::vector<Mark> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[KID_COUNT] = {};
+    uint64_t launches[KID_COUNT] = {};
 };
 
 template <class T>
@@ -1041,6 +1073,11 @@ struct ScratchImpl : Scratch {
         size_t tasks = size_t(2) * c.pairCapacity;
         int rc;
         if ((rc = grow(w.counters, caps.counters, 16, bytes))) return rc;
+        if (!w.totals) {
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.totals), 4 * sizeof(unsigned long long)));
+            HIP_OK(hipMemset(w.totals, 0, 4 * sizeof(unsigned long long)));
+            totals = w.totals;
+        }
         if ((rc = grow(w.taskCount, caps.taskCount, tasks, bytes))) return rc;
         if ((rc = grow(w.taskOffset, caps.taskOffset, tasks + 1, bytes))) return rc;
         if ((rc = grow(w.adjCount, caps.adjCount, c.N, bytes))) return rc;
@@ -1064,6 +1101,9 @@ struct ScratchImpl : Scratch {
         return SAD_OK;
     }
     ~ScratchImpl() {
+        for (auto& m : pending) { pool.push_back(m.start); pool.push_back(m.stop); }
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+        (void)hipFree(w.totals);
         (void)hipFree(w.counters); (void)hipFree(w.taskCount); (void)hipFree(w.taskOffset); (void)hipFree(w.adjCount);
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
@@ -1199,11 +1239,27 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
 
 inline unsigned blocks(uint64_t n, unsigned t = 256) { return unsigned(std::max<uint64_t>(1, (n + t - 1) / t)); }
 
-int launch_scan(const ScanArgs& a, uint32_t cap, hipStream_t s) {
+hipEvent_t take_event(Scratch& sc) {
+    if (!sc.pool.empty()) { hipEvent_t e = sc.pool.back(); sc.pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+// Launch with optional start/stop events on the same stream.
+#define LAUNCH(SC, ID, KERNEL, GRID, BLOCK, STREAM, ...)                                        \
+    do {                                                                                        \
+        hipEvent_t _a = nullptr, _b = nullptr;                                                  \
+        if ((SC).profiling) { _a = take_event(SC); _b = take_event(SC); }                       \
+        if (_a) (void)hipEventRecord(_a, STREAM);                                               \
+        hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, __VA_ARGS__);                        \
+        if (_b) { (void)hipEventRecord(_b, STREAM); (SC).pending.push_back({ID, _a, _b}); }      \
+    } while (0)
+
+int launch_scan(Scratch& sc, const ScanArgs& a, uint32_t cap, hipStream_t s) {
     unsigned tiles = blocks(cap, SCAN_TILE);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(256), 0, s, a);
+    LAUNCH(sc, KID_scan_reduce, k_scan_reduce, dim3(tiles), dim3(256), s, a);
+    LAUNCH(sc, KID_scan_sums, k_scan_sums, dim3(1), dim3(256), s, a);
+    LAUNCH(sc, KID_scan_apply, k_scan_apply, dim3(tiles), dim3(256), s, a);
     return SAD_OK;
 }
 
@@ -1211,51 +1267,66 @@ int launch_scan(const ScanArgs& a, uint32_t cap, hipStream_t s) {
 int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
     const Cfg& c = p.c;
     W& w = sc.w;
-    unsigned gridStride = std::max(1, sc.cus) * 8;
-    if (name == "beginStep") hipLaunchKernelGGL(k_begin_step, dim3(blocks(c.N)), dim3(256), 0, s, p);
-    else if (name == "advanceCell") hipLaunchKernelGGL(k_advance_cell, dim3(1), dim3(1), 0, s, p);
-    else if (name == "checkRebuild") hipLaunchKernelGGL(k_check_rebuild, dim3(blocks(c.N)), dim3(256), 0, s, p);
-    else if (name == "clearBins") hipLaunchKernelGGL(k_clear_bins, dim3(blocks(c.binCount)), dim3(256), 0, s, p);
+    const dim3 T(256);
+    const dim3 perGrain(blocks(c.N)), gridStride(std::max(1, sc.cus) * 8);
+    if (name == "beginStep") LAUNCH(sc, KID_begin_step, k_begin_step, perGrain, T, s, p);
+    else if (name == "advanceCell") LAUNCH(sc, KID_advance_cell, k_advance_cell, dim3(1), dim3(1), s, p);
+    else if (name == "checkRebuild") LAUNCH(sc, KID_check_rebuild, k_check_rebuild, perGrain, T, s, p);
+    else if (name == "clearBins") LAUNCH(sc, KID_clear_bins, k_clear_bins, dim3(blocks(c.binCount)), T, s, p);
     else if (name == "fillBins") {
-        hipLaunchKernelGGL(k_fill_bins, dim3(blocks(c.N)), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(k_sort_bins, dim3(blocks(c.binCount)), dim3(256), 0, s, p);
-    } else if (name == "countPairs") hipLaunchKernelGGL(k_count_pairs, dim3(blocks(c.N)), dim3(256), 0, s, p);
+        LAUNCH(sc, KID_fill_bins, k_fill_bins, perGrain, T, s, p);
+        LAUNCH(sc, KID_sort_bins, k_sort_bins, dim3(blocks(c.binCount)), T, s, p);
+    } else if (name == "countPairs") LAUNCH(sc, KID_count_pairs, k_count_pairs, perGrain, T, s, p);
     else if (name == "pairScan") {
         ScanArgs a{p.pairCounts, p.pairOffsets, w.sums, c.N, nullptr, 1, p.control + 4, nullptr};
-        launch_scan(a, c.N, s);
+        launch_scan(sc, a, c.N, s);
     } else if (name == "emitPairs") {
-        hipLaunchKernelGGL(k_emit_pairs, dim3(blocks(c.N)), dim3(256), 0, s, p);
+        LAUNCH(sc, KID_emit_pairs, k_emit_pairs, perGrain, T, s, p);
     } else if (name == "resetMeshContacts") {
-        hipLaunchKernelGGL(k_reset_mesh, dim3(blocks(c.pairCapacity)), dim3(256), 0, s, p, w);
+        LAUNCH(sc, KID_reset_mesh, k_reset_mesh, dim3(blocks(c.pairCapacity)), T, s, p, w);
     } else if (name == "prepareMeshPairs") {
-        hipLaunchKernelGGL(k_prepare_pairs, dim3(blocks(c.pairCapacity)), dim3(256), 0, s, p);
+        LAUNCH(sc, KID_prepare_pairs, k_prepare_pairs, dim3(blocks(c.pairCapacity)), T, s, p);
     } else if (name == "collectMeshFrontiers") {
-        hipLaunchKernelGGL(k_adj_clear, dim3(blocks(c.N)), dim3(256), 0, s, p, w);
-        hipLaunchKernelGGL(k_adj_count, dim3(gridStride), dim3(256), 0, s, p, w);
+        LAUNCH(sc, KID_adj_clear, k_adj_clear, perGrain, T, s, p, w);
+        LAUNCH(sc, KID_adj_count, k_adj_count, gridStride, T, s, p, w);
         ScanArgs adj{w.adjCount, w.adjOffset, w.sums, c.N, nullptr, 1, nullptr, nullptr};
-        launch_scan(adj, c.N, s);
-        hipLaunchKernelGGL(k_adj_fill, dim3(gridStride), dim3(256), 0, s, p, w);
-        hipLaunchKernelGGL(k_adj_sort, dim3(blocks(c.N)), dim3(256), 0, s, p, w);
-        unsigned persistent = std::max(1, sc.cus) * 6;
-        hipLaunchKernelGGL(k_collect, dim3(persistent), dim3(COLLECT_WAVES * WAVE), 0, s, p, w);
+        launch_scan(sc, adj, c.N, s);
+        LAUNCH(sc, KID_adj_fill, k_adj_fill, gridStride, T, s, p, w);
+        LAUNCH(sc, KID_adj_sort, k_adj_sort, perGrain, T, s, p, w);
+        const dim3 persistent(std::max(1, sc.cus) * 6);
+        LAUNCH(sc, KID_collect, k_collect, persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
         ScanArgs a{w.taskCount, w.taskOffset, w.sums, 2 * c.pairCapacity, p.control + 1, 2, nullptr, w.counters + 4};
-        launch_scan(a, 2 * c.pairCapacity, s);
-        hipLaunchKernelGGL(k_contact_total, dim3(blocks(c.pairCapacity)), dim3(256), 0, s, p, w);
-        hipLaunchKernelGGL(k_scatter, dim3(gridStride), dim3(256), 0, s, p, w);
-    } else if (name == "clearContactStorage") {
-        hipLaunchKernelGGL(k_clear_storage, dim3(gridStride), dim3(256), 0, s, p);
-    } else if (name == "resolveContacts") hipLaunchKernelGGL(k_resolve, dim3(gridStride), dim3(256), 0, s, p);
-    else if (name == "buildHistoryHash") hipLaunchKernelGGL(k_build_hash, dim3(gridStride), dim3(256), 0, s, p);
-    else if (name == "gather") hipLaunchKernelGGL(k_gather, dim3(blocks(c.N)), dim3(256), 0, s, p, w);
-    else if (name == "finishStep") hipLaunchKernelGGL(k_finish_step, dim3(blocks(c.N)), dim3(256), 0, s, p);
-    else if (name == "captureContactKinematics") hipLaunchKernelGGL(k_capture_kinematics, dim3(blocks(c.N)), dim3(256), 0, s, p);
-    else if (name == "checkContactTimestep") hipLaunchKernelGGL(k_check_timestep, dim3(blocks(c.N)), dim3(256), 0, s, p);
+        launch_scan(sc, a, 2 * c.pairCapacity, s);
+        LAUNCH(sc, KID_contact_total, k_contact_total, dim3(blocks(c.pairCapacity)), T, s, p, w);
+        LAUNCH(sc, KID_scatter, k_scatter, gridStride, T, s, p, w);
+    } else if (name == "clearContactStorage") LAUNCH(sc, KID_clear_storage, k_clear_storage, gridStride, T, s, p);
+    else if (name == "resolveContacts") LAUNCH(sc, KID_resolve, k_resolve, gridStride, T, s, p);
+    else if (name == "buildHistoryHash") LAUNCH(sc, KID_build_hash, k_build_hash, gridStride, T, s, p);
+    else if (name == "gather") LAUNCH(sc, KID_gather, k_gather, perGrain, T, s, p, w);
+    else if (name == "finishStep") LAUNCH(sc, KID_finish_step, k_finish_step, perGrain, T, s, p);
+    else if (name == "captureContactKinematics") LAUNCH(sc, KID_capture_kinematics, k_capture_kinematics, perGrain, T, s, p);
+    else if (name == "checkContactTimestep") LAUNCH(sc, KID_check_timestep, k_check_timestep, perGrain, T, s, p);
     else {
         std::snprintf(g_error, sizeof(g_error), "unknown kernel %s", name.c_str());
         return SAD_E_ARGUMENT;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, name.c_str());
+    return SAD_OK;
+}
+
+// Resolve pending event pairs into per-kernel totals (synchronizes the events).
+int resolve_profile(Scratch& sc) {
+    for (auto& m : sc.pending) {
+        HIP_OK(hipEventSynchronize(m.stop));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, m.start, m.stop));
+        sc.ms[m.kernel] += ms;
+        sc.launches[m.kernel] += 1;
+        sc.pool.push_back(m.start);
+        sc.pool.push_back(m.stop);
+    }
+    sc.pending.clear();
     return SAD_OK;
 }
 
@@ -1270,18 +1341,43 @@ void sad_hip_scratch_destroy(void* scratch) { delete static_cast<ScratchImpl*>(s
 int sad_hip_scratch_stats(void* scratch, uint64_t* stats) {
     if (!scratch || !stats) return SAD_E_ARGUMENT;
     auto* sc = static_cast<ScratchImpl*>(scratch);
-    uint32_t counters[4] = {0, 0, 0, 0};
-    if (sc->w.counters) {
-        hipError_t e = hipMemcpy(counters, sc->w.counters, sizeof(counters), hipMemcpyDeviceToHost);
+    unsigned long long totals[4] = {0, 0, 0, 0};
+    if (sc->w.totals) {
+        hipError_t e = hipMemcpy(totals, sc->w.totals, sizeof(totals), hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_fail(e, "stats");
     }
-    stats[0] = counters[0];
-    stats[1] = counters[2];
-    stats[2] = counters[1];
-    stats[3] = sc->bytes;
-    for (int k = 4; k < 8; k++) stats[k] = sc->stats[k];
+    for (int k = 0; k < 4; k++) stats[k] = totals[k];
+    stats[4] = sc->bytes;
+    for (int k = 5; k < 8; k++) stats[k] = sc->stats[k];
     return SAD_OK;
 }
+
+int sad_hip_profile(void* scratch, int enable) {
+    if (!scratch) return SAD_E_ARGUMENT;
+    auto* sc = static_cast<ScratchImpl*>(scratch);
+    int rc = resolve_profile(*sc);
+    if (rc) return rc;
+    if (enable) {
+        std::memset(sc->ms, 0, sizeof(sc->ms));
+        std::memset(sc->launches, 0, sizeof(sc->launches));
+    }
+    sc->profiling = enable != 0;
+    return SAD_OK;
+}
+
+int sad_hip_profile_read(void* scratch, double* ms, uint64_t* launches, uint32_t capacity) {
+    if (!scratch) return -SAD_E_ARGUMENT;
+    auto* sc = static_cast<ScratchImpl*>(scratch);
+    if (resolve_profile(*sc)) return -SAD_E_HIP;
+    uint32_t n = std::min<uint32_t>(capacity, KID_COUNT);
+    for (uint32_t k = 0; k < n; k++) {
+        if (ms) ms[k] = sc->ms[k];
+        if (launches) launches[k] = sc->launches[k];
+    }
+    return int(KID_COUNT);
+}
+
+const char* sad_hip_kernel_name(uint32_t kernel) { return kernel < KID_COUNT ? kKernelNames[kernel] : ""; }
 
 int sad_hip_dispatch(const char* kernel, uint32_t count, void* const* buffers, const uint64_t* bytes,
                      const double* values, void* scratch, void* stream) {

@@ -193,10 +193,24 @@ class HipBackend:
             buffer.tensor[: data.size].copy_(self.torch.from_numpy(data.copy()))
 
     def stats(self):
+        """Cumulative device work counters of this backend's workspace."""
         out = (ct.c_uint64 * 8)()
         self._check(self.lib.sad_hip_scratch_stats(self.scratch, out), "stats")
-        keys = ("tasks", "queried_nodes", "staged_contacts", "scratch_bytes", "rebuilds", "culled", "audited", "misses")
+        keys = ("collections", "directed_pairs", "queried_nodes", "contacts", "scratch_bytes")
         return dict(zip(keys, (int(v) for v in out)))
+
+    def profile(self, enable=True):
+        """Start (clearing) or stop per-kernel HIP-event timing."""
+        self._check(self.lib.sad_hip_profile(self.scratch, int(enable)), "profile")
+
+    def profile_read(self):
+        """{kernel: (total ms, launches)} accumulated since profiling was enabled."""
+        ms = (ct.c_double * 64)()
+        launches = (ct.c_uint64 * 64)()
+        n = self.lib.sad_hip_profile_read(self.scratch, ms, launches, 64)
+        if n < 0:
+            raise RuntimeError(self.lib.sad_hip_last_error().decode())
+        return {self.lib.sad_hip_kernel_name(k).decode(): (ms[k], int(launches[k])) for k in range(n) if launches[k]}
 
     def doctor(self):
         props = self.torch.cuda.get_device_properties(self.torch_device)

@@ -59,6 +59,9 @@ def library():
             "sad_hip_dispatch": (ct.c_int, [ct.c_char_p, u32, vpp, u64p, dp, vp, vp]),
             "sad_hip_advance": (ct.c_int, [u32, u32, vpp, u64p, dp, vpp, u64p, vp, vp, ct.c_int]),
             "sad_hip_probe": (ct.c_int, [ct.c_char_p, ct.POINTER(u32), vpp, u64p, dp, vp, vp, vp]),
+            "sad_hip_profile": (ct.c_int, [vp, ct.c_int]),
+            "sad_hip_profile_read": (ct.c_int, [vp, dp, u64p, u32]),
+            "sad_hip_kernel_name": (ct.c_char_p, [u32]),
             "sad_hip_last_error": (ct.c_char_p, []),
             "sad_hip_abi_version": (ct.c_int, []),
         }
@@ -76,6 +79,9 @@ EXPORTED = (
     "sad_hip_dispatch",
     "sad_hip_advance",
     "sad_hip_probe",
+    "sad_hip_profile",
+    "sad_hip_profile_read",
+    "sad_hip_kernel_name",
     "sad_hip_last_error",
     "sad_hip_abi_version",
 )
