@@ -38,6 +38,7 @@ struct Cfg {
 // Device pointers of one binding set (reference Resources order) plus config.
 struct P {
     Cfg c;
+    uint32_t meshNodes;  // BVH nodes over all templates (rows of meshLinks)
     uint32_t* control;
     float4 *cell, *position, *positionRemainder, *velocityRemainder, *momentumRemainder;
     float4 *rotationRemainder, *referencePositionRemainder, *cellLog, *accumulatedRemainder;
@@ -84,6 +85,9 @@ struct W {
     float4* stagePd;      // C
     float4* stageNa;      // C
     unsigned long long* totals;  // steps, directed pairs, queried nodes, contacts (cumulative)
+    float4* wide;         // 4 rows per BVH node: both child boxes + child references
+    uint32_t* xcd;        // [0,8) task count per XCD queue, [8,16) fill cursors, [16,24) pop cursors
+    uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue
     uint32_t stageCap, taskCap, nCap, sumsCap;
 };
 
@@ -228,6 +232,72 @@ __device__ v4 meshDistance(const P& p, uint32_t shape, v3 q, uint16_t* stack) {
             uint32_t triangle = link.y + k, feature;
             v3 c = closestTriangle(q, xyz(p.meshTriangles[3 * triangle]), xyz(p.meshTriangles[3 * triangle + 1]),
                                    xyz(p.meshTriangles[3 * triangle + 2]), feature);
+            float distance = dot(q - c, q - c);
+            if (distance < best) { best = distance; nearest = c; nearestNormal = 7 * triangle + feature; found = true; }
+        }
+    }
+    v3 pseudo = found ? xyz(p.meshNormals[nearestNormal]) : mk3(0, 0, 0);
+    float distance = sqrtf(best);
+    float side = dot(q - nearest, pseudo) < 0 ? -1.0f : 1.0f;
+    v3 normal = distance > 1e-12f ? side * (q - nearest) / distance : pseudo;
+    return mk4(normal, side * distance);
+}
+
+// Wide node rows derived from the reference BVH arrays (meshMin/meshMax/meshLinks):
+// row 0 = (left child min, left ref), row 1 = (left child max, right ref),
+// row 2 = (right child min, -), row 3 = (right child max, -).  A ref is a node index,
+// or LEAF | (count - 1) << 28 | first triangle for a leaf.  The traversal below
+// visits nodes in the reference order and compares the same FP32 box distances,
+// cached at push time as the CPU variant does (cpu_scheduling.slang:18-48).
+constexpr uint32_t LEAF = 0x80000000u;
+constexpr int WSTACK = 16;  // max depth 14 => at most 15 pending entries (host-checked)
+
+__device__ __forceinline__ uint32_t nodeRef(const P& p, uint32_t node) {
+    uint4 l = p.meshLinks[node];
+    return l.z == 0 ? node : (LEAF | ((l.z - 1) << 28) | l.y);
+}
+__global__ void k_wide_nodes(P p, W w, uint32_t nodes) {
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
+        uint4 link = p.meshLinks[n];
+        if (link.z != 0) continue;
+        uint32_t a = n + 1, b = p.meshLinks[a].x;
+        float4 la = p.meshMin[a], ha = p.meshMax[a], lb = p.meshMin[b], hb = p.meshMax[b];
+        w.wide[4 * size_t(n)] = make_float4(la.x, la.y, la.z, __uint_as_float(nodeRef(p, a)));
+        w.wide[4 * size_t(n) + 1] = make_float4(ha.x, ha.y, ha.z, __uint_as_float(nodeRef(p, b)));
+        w.wide[4 * size_t(n) + 2] = make_float4(lb.x, lb.y, lb.z, 0.0f);
+        w.wide[4 * size_t(n) + 3] = make_float4(hb.x, hb.y, hb.z, 0.0f);
+    }
+}
+__device__ v4 meshDistanceWide(const P& p, const float4* __restrict__ wide, uint32_t shape, v3 q, uint32_t* sref,
+                               float* sdist) {
+    const uint32_t root = p.meshRoots[shape].x;
+    uint32_t top = 1;
+    sref[0] = nodeRef(p, root);
+    sdist[0] = boxDistance(xyz(p.meshMin[root]), xyz(p.meshMax[root]), q);
+    float best = 1e30f;
+    v3 nearest = {0, 0, 0};
+    uint32_t nearestNormal = 0;
+    bool found = false;
+    while (top > 0) {
+        --top;
+        uint32_t ref = sref[top * WAVE];
+        if (sdist[top * WAVE] > best) continue;
+        if ((ref & LEAF) == 0) {
+            const float4* node = wide + 4 * size_t(ref);
+            float4 w0 = node[0], w1 = node[1], w2 = node[2], w3 = node[3];
+            float da = boxDistance(xyz(w0), xyz(w1), q), db = boxDistance(xyz(w2), xyz(w3), q);
+            uint32_t a = __float_as_uint(w0.w), b = __float_as_uint(w1.w);
+            if (da < db) { uint32_t s = a; a = b; b = s; float t = da; da = db; db = t; }
+            if (top + 2 > WSTACK) { fail(p, 1024); return {0, 0, 0, 1e20f}; }
+            if (da <= best) { sref[top * WAVE] = a; sdist[top * WAVE] = da; top++; }
+            if (db <= best) { sref[top * WAVE] = b; sdist[top * WAVE] = db; top++; }
+            continue;
+        }
+        uint32_t first = ref & 0x0fffffffu, count = ((ref >> 28) & 3u) + 1;
+        for (uint32_t k = 0; k < count; k++) {
+            uint32_t triangle = first + k, feature;
+            const float4* t = p.meshTriangles + 3 * size_t(triangle);
+            v3 c = closestTriangle(q, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
             float distance = dot(q - c, q - c);
             if (distance < best) { best = distance; nearest = c; nearestNormal = 7 * triangle + feature; found = true; }
         }
@@ -613,6 +683,50 @@ __global__ void k_prepare_pairs(P p) {  // solver.slang:469-489 (Metal frontier 
     p.meshPairMask[id] = mask;
 }
 
+// Task queues per XCD, keyed by the target template (the BVH a task reads).
+__global__ void k_bucket_reset(W w) {
+    if (threadIdx.x < 24) w.xcd[threadIdx.x] = 0;
+}
+__device__ __forceinline__ uint32_t taskQueue(const P& p, uint32_t task) {
+    uint2 ij = p.pairs[task >> 1];
+    uint32_t target = (task & 1) ? ij.x : ij.y;
+    return p.shapeId[target] & 7u;
+}
+// Wave-aggregated: one atomic per queue per wave (8 hot counters otherwise serialize).
+__global__ void k_bucket_count(P p, W w) {
+    const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
+    const int lane = threadIdx.x & 63;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < tasks; base += stride) {
+        uint32_t t = base + lane;
+        uint32_t qi = t < tasks ? taskQueue(p, t) : 8u;
+        for (uint32_t q = 0; q < 8; q++) {
+            uint64_t m = __ballot(qi == q);
+            if (lane == 0 && m) atomicAdd(w.xcd + q, uint32_t(__popcll(m)));
+        }
+    }
+}
+__global__ void k_bucket_fill(P p, W w) {
+    uint32_t off[8], acc = 0;
+    for (int k = 0; k < 8; k++) { off[k] = acc; acc += w.xcd[k]; }
+    const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
+    const int lane = threadIdx.x & 63;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < tasks; base += stride) {
+        uint32_t t = base + lane;
+        uint32_t qi = t < tasks ? taskQueue(p, t) : 8u;
+        uint32_t slot = 0;
+        for (uint32_t q = 0; q < 8; q++) {
+            uint64_t m = __ballot(qi == q);
+            uint32_t start = 0;
+            if (lane == 0 && m) start = atomicAdd(w.xcd + 8 + q, uint32_t(__popcll(m)));
+            start = __shfl(start, 0);
+            if (qi == q) slot = start + uint32_t(__popcll(m & lanes_below(lane)));
+        }
+        if (qi < 8) w.xcdTasks[off[qi] + slot] = t;
+    }
+}
+
 struct TaskFrame {
     uint32_t i, j, si, sj, sampleBase;
     v4 qi, qj;
@@ -622,11 +736,12 @@ struct TaskFrame {
 };
 
 // Exact part of nodeContact (solver.slang:319-343) for one queued node.
-__device__ bool queryNode(const P& p, const TaskFrame& f, uint32_t node, uint16_t* stack, float4& pd, float4& na) {
+__device__ bool queryNode(const P& p, const W& w, const TaskFrame& f, uint32_t node, uint32_t* sref, float* sdist,
+                          float4& pd, float4& na) {
     float4 sp = p.sample[f.sampleBase + node];
     v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
     v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
-    v4 r = meshDistance(p, f.sj, local, stack);
+    v4 r = meshDistanceWide(p, w.wide, f.sj, local, sref, sdist);
     float depth = -r.w * f.s_j;
     pd = f4(arm, depth);
     if (depth <= 0) return false;
@@ -663,7 +778,7 @@ __device__ void stageHits(const P& p, const W& w, const TaskFrame& f, uint32_t t
 }
 
 __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, const uint32_t* queue,
-                           uint32_t qn, uint16_t* stack, uint32_t& running, int lane) {
+                           uint32_t qn, uint32_t* sref, float* sdist, uint32_t& running, int lane) {
     for (uint32_t q = 0; q < qn; q += WAVE) {
         uint32_t k = q + lane;
         bool hit = false;
@@ -671,7 +786,7 @@ __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
         float4 pd = make_float4(0, 0, 0, 0), na = make_float4(0, 0, 0, 0);
         if (k < qn) {
             node = queue[k];
-            hit = queryNode(p, f, node, stack, pd, na);
+            hit = queryNode(p, w, f, node, sref, sdist, pd, na);
         }
         stageHits(p, w, f, task, hit, node, pd, na, running, lane);
     }
@@ -679,21 +794,36 @@ __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
 
 // One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
 __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
-    __shared__ uint16_t stacks[COLLECT_WAVES][STACK * WAVE];
+    __shared__ uint32_t stackRef[COLLECT_WAVES][WSTACK * WAVE];
+    __shared__ float stackDist[COLLECT_WAVES][WSTACK * WAVE];
     __shared__ uint32_t queues[COLLECT_WAVES][QCAP];
     __shared__ uint64_t candidates[COLLECT_WAVES][MAX_CLUSTERS / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint16_t* stack = &stacks[wv][lane];
+    uint32_t* sref = &stackRef[wv][lane];
+    float* sdist = &stackDist[wv][lane];
     uint32_t* queue = queues[wv];
     uint64_t* cand = candidates[wv];
-    const uint32_t pairs = min(p.control[1], p.c.pairCapacity);
-    const uint32_t tasks = 2 * pairs;
     uint32_t queried = 0;
+    // XCD-affine queues: tasks are grouped by target template; a wave drains the
+    // queue of its own XCD (workgroups are dispatched round-robin over 8 XCDs)
+    // and then steals from the others.
+    uint32_t qcount[8], qoff[8];
+    {
+        uint32_t acc = 0;
+        for (int k = 0; k < 8; k++) { qcount[k] = w.xcd[k]; qoff[k] = acc; acc += qcount[k]; }
+    }
+    uint32_t home = blockIdx.x & 7, tried = 0;
     for (;;) {
-        uint32_t task = 0;
-        if (lane == 0) task = atomicAdd(w.counters, 1u);
-        task = __shfl(task, 0);
-        if (task >= tasks) break;
+        uint32_t task = 0xffffffffu;
+        while (tried < 8) {
+            uint32_t qi = (home + tried) & 7;
+            uint32_t k = 0;
+            if (lane == 0) k = atomicAdd(w.xcd + 16 + qi, 1u);
+            k = __shfl(k, 0);
+            if (k < qcount[qi]) { task = w.xcdTasks[qoff[qi] + k]; break; }
+            tried++;
+        }
+        if (task == 0xffffffffu) break;
         uint32_t running = 0;
         if (ld_volatile(p.control) == 0) {
             TaskFrame f;
@@ -764,7 +894,7 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
                     uint32_t n = __popcll(m);
                     if (qn + n > QCAP) {
                         wave_sync();
-                        flushQueue(p, w, f, task, queue, qn, stack, running, lane);
+                        flushQueue(p, w, f, task, queue, qn, sref, sdist, running, lane);
                         queried += qn;
                         qn = 0;
                         wave_sync();
@@ -775,7 +905,7 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
                 cluster = candidate ? cluster + 1 : range.z;
             }
             wave_sync();
-            flushQueue(p, w, f, task, queue, qn, stack, running, lane);
+            flushQueue(p, w, f, task, queue, qn, sref, sdist, running, lane);
             queried += qn;
             wave_sync();
         }
@@ -904,16 +1034,78 @@ __global__ void k_build_hash(P p) {
 }
 
 // ---------------------------------------------------------------- force / torque gather (solver.slang:761-811)
-__global__ void k_gather(P p, W w) {
-    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per grain.  The grain's contact endpoints are visited in the order of
+// the CPU's linked list (descending contact id: pairs descending, pass 1 then 0,
+// contacts descending), staged in LDS 64 at a time, and each of the reference's
+// sequential sums runs on its own lane with the reference's exact FP32 operations:
+// lanes 0-2 force (compensated), 3-5 torque (compensated), 6-14 stress rows,
+// 15 energy, 16 |F| sum, 17 max depth, 18 dissipation, 19 normal power,
+// 20-21 frozen-contact rates.  Sums over different quantities are independent, so
+// splitting them across lanes leaves every sum's operation sequence unchanged.
+constexpr int GCHUNK = 64;
+constexpr int GFIELDS = 16;
+constexpr int GSTRIDE = GCHUNK + 1;  // padded rows: lanes reading different fields hit different banks
+enum GField { G_FX, G_FY, G_FZ, G_E, G_AX, G_AY, G_AZ, G_DEPTH, G_SLIP, G_NLP, G_AREA, G_SRC, G_BX, G_BY, G_BZ, G_MINW };
+
+__device__ __forceinline__ float gget(const float* st, int field, int e) { return st[field * GSTRIDE + e]; }
+
+__global__ __launch_bounds__(256) void k_gather(P p, W w) {
+    __shared__ float stage[4][GFIELDS * GSTRIDE];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t id = blockIdx.x * 4 + wv;
     if (id >= p.c.N || ld_volatile(p.control) != 0) return;
-    v3 f = {0, 0, 0}, t = {0, 0, 0}, fLow = {0, 0, 0}, tLow = {0, 0, 0};
-    v3 stress0 = {0, 0, 0}, stress1 = {0, 0, 0}, stress2 = {0, 0, 0};
-    float energy = 0, denom = 0, maxDepth = 0, dissipation = 0, normalPower = 0;
-    float stiffnessRate = 0, viscousRate = 0;
-    float inverseInertia = p.c.activeTimestep ? inverseInertiaBound(p, id) : 0.0f;
-    float mass = p.body[id].y;
-    uint32_t first = w.adjOffset[id];
+    float* st = stage[wv];
+    const float mass = p.body[id].y;
+    const float inverseInertia = p.c.activeTimestep ? inverseInertiaBound(p, id) : 0.0f;
+    float hi = 0.0f, lo = 0.0f;  // compensated chains (lanes 0-5) or plain sums
+    int fill = 0;
+    auto process = [&](int count) {
+        wave_sync();
+        for (int e = 0; e < count; e++) {
+            bool source = gget(st, G_SRC, e) != 0.0f;
+            v3 force = {gget(st, G_FX, e), gget(st, G_FY, e), gget(st, G_FZ, e)};
+            if (lane < 3) {
+                float fk = lane == 0 ? force.x : (lane == 1 ? force.y : force.z);
+                v3 h = {hi, 0, 0}, l = {lo, 0, 0};
+                compensated3(h, l, {source ? fk : -fk, 0, 0});
+                hi = h.x; lo = l.x;
+            } else if (lane < 6) {
+                v3 arm = {gget(st, G_AX, e), gget(st, G_AY, e), gget(st, G_AZ, e)};
+                v3 branch = {gget(st, G_BX, e), gget(st, G_BY, e), gget(st, G_BZ, e)};
+                v3 c = source ? cross(arm, force) : cross(arm - branch, -force);
+                float ck = lane == 3 ? c.x : (lane == 4 ? c.y : c.z);
+                v3 h = {hi, 0, 0}, l = {lo, 0, 0};
+                compensated3(h, l, {ck, 0, 0});
+                hi = h.x; lo = l.x;
+            } else if (lane < 15) {
+                int r = (lane - 6) / 3, c = (lane - 6) % 3;
+                float fr = r == 0 ? force.x : (r == 1 ? force.y : force.z);
+                float bc = gget(st, G_BX + c, e);
+                hi = hi - 0.5f * fr * bc;
+            } else if (lane == 15) {
+                hi += 0.5f * gget(st, G_E, e);
+            } else if (lane == 16) {
+                hi += length(force);
+            } else if (lane == 17) {
+                hi = fmaxf(hi, gget(st, G_DEPTH, e) / gget(st, G_MINW, e));
+            } else if (lane == 18) {
+                hi += 0.5f * gget(st, G_SLIP, e);
+            } else if (lane == 19) {
+                hi += 0.5f * gget(st, G_NLP, e);
+            } else if (lane < 22 && p.c.activeTimestep != 0) {
+                v3 arm = {gget(st, G_AX, e), gget(st, G_AY, e), gget(st, G_AZ, e)};
+                v3 branch = {gget(st, G_BX, e), gget(st, G_BY, e), gget(st, G_BZ, e)};
+                v3 localArm = source ? arm : arm - branch;
+                float mobility = 1.0f / mass + dot(localArm, localArm) * inverseInertia;
+                float area = gget(st, G_AREA, e) * 0.5f;
+                float coupling = 2.0f;
+                if (lane == 20) hi += coupling * area * (p.c.kn * (1.0f + p.c.mu) + p.c.kt) * mobility;
+                else hi += coupling * area * p.c.normalViscosity * (1.0f + p.c.mu) * mobility;
+            }
+        }
+        wave_sync();
+    };
+    const uint32_t first = w.adjOffset[id];
     for (uint32_t k = w.adjCount[id]; k-- > 0;) {
         uint32_t pair = w.adjList[first + k];
         uint2 ij = p.pairs[pair];
@@ -922,47 +1114,55 @@ __global__ void k_gather(P p, W w) {
             uint32_t cnt = w.taskCount[task];
             if (cnt == 0) continue;
             uint32_t src = pass == 0 ? ij.x : ij.y, dst = pass == 0 ? ij.y : ij.x;
-            bool source = id == src;
+            float source = id == src ? 1.0f : 0.0f;
             v3 branch = centerBranch(p, src, dst);
             float minW = fminf(p.body[src].w, p.body[dst].w);
             uint32_t off = w.taskOffset[task];
-            for (uint32_t contact = off + cnt; contact-- > off;) {
-                float4 cf = p.contactForce[contact];
-                v3 force = xyz(cf);
-                float4 pdv = p.pointDepth[contact];
-                v3 arm = xyz(pdv);
-                if (p.c.activeTimestep != 0) {
-                    v3 localArm = source ? arm : arm - branch;
-                    float mobility = 1.0f / mass + dot(localArm, localArm) * inverseInertia;
-                    float area = p.normalArea[contact].w * 0.5f;
-                    float coupling = 2.0f;
-                    stiffnessRate += coupling * area * (p.c.kn * (1.0f + p.c.mu) + p.c.kt) * mobility;
-                    viscousRate += coupling * area * p.c.normalViscosity * (1.0f + p.c.mu) * mobility;
+            for (uint32_t done = 0; done < cnt;) {
+                uint32_t m = min(cnt - done, uint32_t(GCHUNK - fill));
+                if (uint32_t(lane) < m) {
+                    uint32_t contact = off + cnt - 1 - (done + lane);
+                    float4 cf = p.contactForce[contact];
+                    float4 pdv = p.pointDepth[contact];
+                    int e = fill + lane;
+                    st[G_FX * GSTRIDE + e] = cf.x; st[G_FY * GSTRIDE + e] = cf.y; st[G_FZ * GSTRIDE + e] = cf.z;
+                    st[G_E * GSTRIDE + e] = cf.w;
+                    st[G_AX * GSTRIDE + e] = pdv.x; st[G_AY * GSTRIDE + e] = pdv.y; st[G_AZ * GSTRIDE + e] = pdv.z;
+                    st[G_DEPTH * GSTRIDE + e] = pdv.w;
+                    st[G_SLIP * GSTRIDE + e] = p.slip[contact].w;
+                    st[G_NLP * GSTRIDE + e] = p.normalLossPower[contact];
+                    st[G_AREA * GSTRIDE + e] = p.c.activeTimestep ? p.normalArea[contact].w : 0.0f;
+                    st[G_SRC * GSTRIDE + e] = source;
+                    st[G_BX * GSTRIDE + e] = branch.x; st[G_BY * GSTRIDE + e] = branch.y; st[G_BZ * GSTRIDE + e] = branch.z;
+                    st[G_MINW * GSTRIDE + e] = minW;
                 }
-                compensated3(f, fLow, source ? force : -force);
-                compensated3(t, tLow, source ? cross(arm, force) : cross(arm - branch, -force));
-                stress0 = stress0 - 0.5f * force.x * branch;
-                stress1 = stress1 - 0.5f * force.y * branch;
-                stress2 = stress2 - 0.5f * force.z * branch;
-                energy += 0.5f * cf.w;
-                denom += length(force);
-                maxDepth = fmaxf(maxDepth, pdv.w / minW);
-                dissipation += 0.5f * p.slip[contact].w;
-                normalPower += 0.5f * p.normalLossPower[contact];
+                fill += int(m);
+                done += m;
+                if (fill == GCHUNK) { process(fill); fill = 0; }
             }
         }
     }
-    f = f + fLow;
-    t = t + tLow;
-    p.timestepRows[id] = make_float4(stiffnessRate, viscousRate, 0, 0);
+    if (fill) process(fill);
+    // Collect the chain results (compensated chains round hi + lo once, as the reference).
+    float total = lane < 6 ? hi + lo : hi;
+    float fx = __shfl(total, 0), fy = __shfl(total, 1), fz = __shfl(total, 2);
+    float tx = __shfl(total, 3), ty = __shfl(total, 4), tz = __shfl(total, 5);
+    float sv[9];
+    for (int k = 0; k < 9; k++) sv[k] = __shfl(total, 6 + k);
+    float energy = __shfl(total, 15), denom = __shfl(total, 16), maxDepth = __shfl(total, 17);
+    float dissipation = __shfl(total, 18), normalPower = __shfl(total, 19);
+    float stiffnessRate = __shfl(total, 20), viscousRate = __shfl(total, 21);
+    if (lane != 0) return;
+    v3 f = {fx, fy, fz}, t = {tx, ty, tz};
+    p.timestepRows[id] = make_float4(p.c.activeTimestep ? stiffnessRate : 0.0f, p.c.activeTimestep ? viscousRate : 0.0f, 0, 0);
     float4 normalState = p.normalDamping[id];
     normalState.z = normalPower;
     p.normalDamping[id] = normalState;
     p.force[id] = f4(f, maxDepth);
     p.torque[id] = f4(t, dissipation);
-    p.particleStress[3 * id] = f4(stress0, 0);
-    p.particleStress[3 * id + 1] = f4(stress1, 0);
-    p.particleStress[3 * id + 2] = f4(stress2, 0);
+    p.particleStress[3 * id] = make_float4(sv[0], sv[1], sv[2], 0);
+    p.particleStress[3 * id + 1] = make_float4(sv[3], sv[4], sv[5], 0);
+    p.particleStress[3 * id + 2] = make_float4(sv[6], sv[7], sv[8], 0);
     v3 vel = xyz(p.velocity[id]);
     float kinetic = 0.5f * (mass * dot(vel, vel) + dot(xyz(p.momentum[id]), omega(p, id)));
     p.particleDiagnostic[id] = make_float4(kinetic, energy, length(f), denom);
@@ -1012,7 +1212,8 @@ thread_local char g_error[256] = "no error";
 #define SAD_KERNELS(X) X(begin_step) X(advance_cell) X(check_rebuild) X(clear_bins) X(fill_bins) X(sort_bins) \
     X(count_pairs) X(scan_reduce) X(scan_sums) X(scan_apply) X(emit_pairs) X(adj_clear) X(adj_count) X(adj_fill) \
     X(adj_sort) X(reset_mesh) X(prepare_pairs) X(collect) X(contact_total) X(scatter) X(clear_storage) X(resolve) \
-    X(build_hash) X(gather) X(finish_step) X(capture_kinematics) X(check_timestep)
+    X(build_hash) X(gather) X(finish_step) X(capture_kinematics) X(check_timestep) \
+    X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_fill)
 enum KernelId {
 #define SAD_ID(name) KID_##name,
     SAD_KERNELS(SAD_ID)
@@ -1064,11 +1265,12 @@ int grow(T*& ptr, uint32_t& cap, size_t need, size_t& bytes) {
 
 struct Caps {
     uint32_t counters = 0, taskCount = 0, taskOffset = 0, adjCount = 0, adjOffset = 0, adjCursor = 0, adjList = 0;
-    uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0;
+    uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
 };
 
 struct ScratchImpl : Scratch {
     Caps caps;
+    int ensure_nodes(uint32_t nodes) { return grow(w.wide, caps.wide, 4 * size_t(std::max(nodes, 1u)), bytes); }
     int ensure(const Cfg& c) {
         size_t tasks = size_t(2) * c.pairCapacity;
         int rc;
@@ -1090,6 +1292,8 @@ struct ScratchImpl : Scratch {
         if ((rc = grow(w.stageMeta, caps.stageMeta, c.contactCapacity, bytes))) return rc;
         if ((rc = grow(w.stagePd, caps.stagePd, c.contactCapacity, bytes))) return rc;
         if ((rc = grow(w.stageNa, caps.stageNa, c.contactCapacity, bytes))) return rc;
+        if ((rc = grow(w.xcd, caps.xcd, 24, bytes))) return rc;
+        if ((rc = grow(w.xcdTasks, caps.xcdTasks, tasks, bytes))) return rc;
         w.stageCap = c.contactCapacity;
         w.taskCap = uint32_t(tasks);
         w.nCap = c.N;
@@ -1106,6 +1310,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.totals);
         (void)hipFree(w.counters); (void)hipFree(w.taskCount); (void)hipFree(w.taskOffset); (void)hipFree(w.adjCount);
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
+        (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
 };
@@ -1173,6 +1378,7 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
             return SAD_E_ARGUMENT;
         }
     }
+    p.meshNodes = uint32_t(bytes[SAD_MESH_LINKS] / 16);
     p.control = slot<uint32_t>(b, SAD_CONTROL);
     p.cell = slot<float4>(b, SAD_CELL);
     p.position = slot<float4>(b, SAD_POSITION);
@@ -1293,6 +1499,12 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         launch_scan(sc, adj, c.N, s);
         LAUNCH(sc, KID_adj_fill, k_adj_fill, gridStride, T, s, p, w);
         LAUNCH(sc, KID_adj_sort, k_adj_sort, perGrain, T, s, p, w);
+        int rc = sc.ensure_nodes(p.meshNodes);
+        if (rc) return rc;
+        LAUNCH(sc, KID_wide_nodes, k_wide_nodes, dim3(blocks(p.meshNodes)), T, s, p, w, p.meshNodes);
+        LAUNCH(sc, KID_bucket_reset, k_bucket_reset, dim3(1), dim3(64), s, w);
+        LAUNCH(sc, KID_bucket_count, k_bucket_count, gridStride, T, s, p, w);
+        LAUNCH(sc, KID_bucket_fill, k_bucket_fill, gridStride, T, s, p, w);
         const dim3 persistent(std::max(1, sc.cus) * 6);
         LAUNCH(sc, KID_collect, k_collect, persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
         ScanArgs a{w.taskCount, w.taskOffset, w.sums, 2 * c.pairCapacity, p.control + 1, 2, nullptr, w.counters + 4};
@@ -1302,7 +1514,7 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
     } else if (name == "clearContactStorage") LAUNCH(sc, KID_clear_storage, k_clear_storage, gridStride, T, s, p);
     else if (name == "resolveContacts") LAUNCH(sc, KID_resolve, k_resolve, gridStride, T, s, p);
     else if (name == "buildHistoryHash") LAUNCH(sc, KID_build_hash, k_build_hash, gridStride, T, s, p);
-    else if (name == "gather") LAUNCH(sc, KID_gather, k_gather, perGrain, T, s, p, w);
+    else if (name == "gather") LAUNCH(sc, KID_gather, k_gather, dim3(blocks(c.N, 4)), T, s, p, w);
     else if (name == "finishStep") LAUNCH(sc, KID_finish_step, k_finish_step, perGrain, T, s, p);
     else if (name == "captureContactKinematics") LAUNCH(sc, KID_capture_kinematics, k_capture_kinematics, perGrain, T, s, p);
     else if (name == "checkContactTimestep") LAUNCH(sc, KID_check_timestep, k_check_timestep, perGrain, T, s, p);

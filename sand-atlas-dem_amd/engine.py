@@ -275,6 +275,10 @@ class Engine:
         roots = mesh["meshRoots"]
         if np.any(roots[:, 1] - roots[:, 0] > 65535):
             raise ValueError("A template BVH exceeds 65,535 nodes (device traversal stack holds 16-bit offsets)")
+        if np.any(roots[:, 2] > 14):
+            raise ValueError("A template BVH is deeper than 14 levels (device traversal stack holds 16 entries)")
+        if len(mesh["meshTriangles"]) // 3 >= 1 << 28:
+            raise ValueError("More than 2^28 triangles in the template library")
         if np.any(mesh["meshClusterInfo"][:, 1] > 1024):
             raise ValueError("A template has more than 1,024 sample clusters (about 32k nodes)")
         for name, data in mesh.items():
