@@ -86,7 +86,8 @@ struct W {
     float4* stageNa;      // C
     unsigned long long* totals;  // steps, directed pairs, queried nodes, contacts (cumulative)
     float4* wide;         // 4 rows per BVH node: both child boxes + child references
-    uint32_t* xcd;        // [0,8) task count per XCD queue, [8,16) fill cursors, [16,24) pop cursors
+    uint32_t* xcd;        // per queue q: count at [q*LINE], fill cursor at [(8+q)*LINE], pop cursor at [(16+q)*LINE]
+    uint32_t* stageCount; // staged hits per queue at [q*LINE]; queue q stages into [q*stageCap, (q+1)*stageCap)
     uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue
     uint32_t stageCap, taskCap, nCap, sumsCap;
 };
@@ -97,6 +98,16 @@ constexpr int QCAP = 256;             // per-wave query queue (node keys)
 constexpr int MAX_CLUSTERS = 1024;    // per source template
 constexpr int STACK = 32;             // BVH traversal stack (reference depth limit)
 constexpr int SCAN_TILE = 1024;       // 256 threads x 4
+// Device counters that many waves update atomically each sit on their own 256-byte
+// line: same-line atomics serialize in one L2 channel (~10 ns each).
+constexpr int LINE = 64;
+#ifndef SAD_POP
+#define SAD_POP 1
+#endif
+#ifndef SAD_COLLECT_BLOCKS_PER_CU
+#define SAD_COLLECT_BLOCKS_PER_CU 6
+#endif
+constexpr int POP = SAD_POP;          // tasks taken per queue atomic
 
 __device__ __forceinline__ void fail(const P& p, uint32_t bit) { atomicOr(p.control, bit); }
 __device__ __forceinline__ uint32_t ld_volatile(const uint32_t* a) { return __atomic_load_n(a, __ATOMIC_RELAXED); }
@@ -645,9 +656,8 @@ __global__ void k_adj_sort(P p, W w) {
 __global__ void k_reset_mesh(P p, W w) {  // resetMeshContacts (solver.slang:397-403) + cursors
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id == 0) {
-        w.counters[0] = 0;
-        w.counters[1] = 0;
-        w.counters[2] = 0;
+        w.counters[2 * LINE] = 0;
+        for (int q = 0; q < 8; q++) w.stageCount[q * LINE] = 0;
     }
     if (ld_volatile(p.control) != 0) return;
     if (id < p.c.pairCapacity) p.contactCounts[id] = 0;
@@ -685,45 +695,41 @@ __global__ void k_prepare_pairs(P p) {  // solver.slang:469-489 (Metal frontier 
 
 // Task queues per XCD, keyed by the target template (the BVH a task reads).
 __global__ void k_bucket_reset(W w) {
-    if (threadIdx.x < 24) w.xcd[threadIdx.x] = 0;
+    if (threadIdx.x < 24) w.xcd[threadIdx.x * LINE] = 0;
 }
 __device__ __forceinline__ uint32_t taskQueue(const P& p, uint32_t task) {
     uint2 ij = p.pairs[task >> 1];
     uint32_t target = (task & 1) ? ij.x : ij.y;
     return p.shapeId[target] & 7u;
 }
-// Wave-aggregated: one atomic per queue per wave (8 hot counters otherwise serialize).
+// Block-aggregated: LDS counts, then one global atomic per queue per block.
 __global__ void k_bucket_count(P p, W w) {
+    __shared__ uint32_t local[8];
+    if (threadIdx.x < 8) local[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
-    const int lane = threadIdx.x & 63;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < tasks; base += stride) {
-        uint32_t t = base + lane;
-        uint32_t qi = t < tasks ? taskQueue(p, t) : 8u;
-        for (uint32_t q = 0; q < 8; q++) {
-            uint64_t m = __ballot(qi == q);
-            if (lane == 0 && m) atomicAdd(w.xcd + q, uint32_t(__popcll(m)));
-        }
-    }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks; t += gridDim.x * blockDim.x)
+        atomicAdd(local + taskQueue(p, t), 1u);
+    __syncthreads();
+    if (threadIdx.x < 8 && local[threadIdx.x]) atomicAdd(w.xcd + threadIdx.x * LINE, local[threadIdx.x]);
 }
 __global__ void k_bucket_fill(P p, W w) {
+    __shared__ uint32_t local[8], base[8], cursor[8];
     uint32_t off[8], acc = 0;
-    for (int k = 0; k < 8; k++) { off[k] = acc; acc += w.xcd[k]; }
+    for (int k = 0; k < 8; k++) { off[k] = acc; acc += w.xcd[k * LINE]; }
     const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
-    const int lane = threadIdx.x & 63;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < tasks; base += stride) {
-        uint32_t t = base + lane;
+    for (uint32_t start = blockIdx.x * blockDim.x; start < tasks; start += stride) {
+        if (threadIdx.x < 8) { local[threadIdx.x] = 0; cursor[threadIdx.x] = 0; }
+        __syncthreads();
+        uint32_t t = start + threadIdx.x;
         uint32_t qi = t < tasks ? taskQueue(p, t) : 8u;
-        uint32_t slot = 0;
-        for (uint32_t q = 0; q < 8; q++) {
-            uint64_t m = __ballot(qi == q);
-            uint32_t start = 0;
-            if (lane == 0 && m) start = atomicAdd(w.xcd + 8 + q, uint32_t(__popcll(m)));
-            start = __shfl(start, 0);
-            if (qi == q) slot = start + uint32_t(__popcll(m & lanes_below(lane)));
-        }
-        if (qi < 8) w.xcdTasks[off[qi] + slot] = t;
+        if (qi < 8) atomicAdd(local + qi, 1u);
+        __syncthreads();
+        if (threadIdx.x < 8) base[threadIdx.x] = local[threadIdx.x] ? atomicAdd(w.xcd + (8 + threadIdx.x) * LINE, local[threadIdx.x]) : 0;
+        __syncthreads();
+        if (qi < 8) w.xcdTasks[off[qi] + base[qi] + atomicAdd(cursor + qi, 1u)] = t;
+        __syncthreads();
     }
 }
 
@@ -754,18 +760,19 @@ __device__ bool queryNode(const P& p, const W& w, const TaskFrame& f, uint32_t n
 }
 
 // Rank-ordered staging of one batch of hits.
-__device__ void stageHits(const P& p, const W& w, const TaskFrame& f, uint32_t task, bool hit, uint32_t node,
-                          float4 pd, float4 na, uint32_t& running, int lane) {
+__device__ void stageHits(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t queue, bool hit,
+                          uint32_t node, float4 pd, float4 na, uint32_t& running, int lane) {
     uint64_t mask = __ballot(hit);
     uint32_t count = __popcll(mask);
     if (count == 0) return;
     uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(w.counters + 1, count);
+    if (lane == 0) base = atomicAdd(w.stageCount + queue * LINE, count);
     base = __shfl(base, 0);
     if (hit) {
         uint32_t rank = __popcll(mask & lanes_below(lane));
-        uint32_t pos = base + rank;
-        if (pos < w.stageCap) {
+        uint32_t local = base + rank;
+        size_t pos = size_t(queue) * w.stageCap + local;
+        if (local < w.stageCap) {
             w.stageKey[pos] = make_uint2(task, running + rank);
             w.stageMeta[pos] = make_uint4(f.i, f.j, node, 0);
             w.stagePd[pos] = pd;
@@ -777,8 +784,9 @@ __device__ void stageHits(const P& p, const W& w, const TaskFrame& f, uint32_t t
     running += count;
 }
 
-__device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, const uint32_t* queue,
-                           uint32_t qn, uint32_t* sref, float* sdist, uint32_t& running, int lane) {
+__device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
+                           const uint32_t* queue, uint32_t qn, uint32_t* sref, float* sdist, uint32_t& running,
+                           int lane) {
     for (uint32_t q = 0; q < qn; q += WAVE) {
         uint32_t k = q + lane;
         bool hit = false;
@@ -788,7 +796,7 @@ __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
             node = queue[k];
             hit = queryNode(p, w, f, node, sref, sdist, pd, na);
         }
-        stageHits(p, w, f, task, hit, node, pd, na, running, lane);
+        stageHits(p, w, f, task, stageQueue, hit, node, pd, na, running, lane);
     }
 }
 
@@ -810,20 +818,26 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
     uint32_t qcount[8], qoff[8];
     {
         uint32_t acc = 0;
-        for (int k = 0; k < 8; k++) { qcount[k] = w.xcd[k]; qoff[k] = acc; acc += qcount[k]; }
+        for (int k = 0; k < 8; k++) { qcount[k] = w.xcd[k * LINE]; qoff[k] = acc; acc += qcount[k]; }
     }
-    uint32_t home = blockIdx.x & 7, tried = 0;
+    const uint32_t home = blockIdx.x & 7;
+    uint32_t tried = 0, next = 0, left = 0, from = 0;
     for (;;) {
         uint32_t task = 0xffffffffu;
-        while (tried < 8) {
-            uint32_t qi = (home + tried) & 7;
-            uint32_t k = 0;
-            if (lane == 0) k = atomicAdd(w.xcd + 16 + qi, 1u);
-            k = __shfl(k, 0);
-            if (k < qcount[qi]) { task = w.xcdTasks[qoff[qi] + k]; break; }
-            tried++;
+        if (left == 0) {
+            while (tried < 8) {
+                uint32_t qi = (home + tried) & 7;
+                uint32_t k = 0;
+                if (lane == 0) k = atomicAdd(w.xcd + (16 + qi) * LINE, uint32_t(POP));
+                k = __shfl(k, 0);
+                if (k < qcount[qi]) { from = qi; next = k; left = min(uint32_t(POP), qcount[qi] - k); break; }
+                tried++;
+            }
         }
-        if (task == 0xffffffffu) break;
+        if (left == 0) break;
+        task = w.xcdTasks[qoff[from] + next];
+        next++;
+        left--;
         uint32_t running = 0;
         if (ld_volatile(p.control) == 0) {
             TaskFrame f;
@@ -894,7 +908,7 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
                     uint32_t n = __popcll(m);
                     if (qn + n > QCAP) {
                         wave_sync();
-                        flushQueue(p, w, f, task, queue, qn, sref, sdist, running, lane);
+                        flushQueue(p, w, f, task, home, queue, qn, sref, sdist, running, lane);
                         queried += qn;
                         qn = 0;
                         wave_sync();
@@ -905,13 +919,13 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
                 cluster = candidate ? cluster + 1 : range.z;
             }
             wave_sync();
-            flushQueue(p, w, f, task, queue, qn, sref, sdist, running, lane);
+            flushQueue(p, w, f, task, home, queue, qn, sref, sdist, running, lane);
             queried += qn;
             wave_sync();
         }
         if (lane == 0) w.taskCount[task] = running;
     }
-    if (lane == 0 && queried) atomicAdd(w.counters + 2, queried);
+    if (lane == 0 && queried) atomicAdd(w.counters + 2 * LINE, queried);
 }
 
 // control[2] = total contacts, per-pair counts (solver.slang:397-403 semantics).
@@ -921,19 +935,21 @@ __global__ void k_contact_total(P p, W w) {
     if (ld_volatile(p.control) != 0) return;
     if (id < pairs) p.contactCounts[id] = w.taskCount[2 * id] + w.taskCount[2 * id + 1];
     if (id == 0) {
-        uint32_t total = w.counters[4];
+        uint32_t total = w.counters[4 * LINE];
         if (total > p.c.contactCapacity) { fail(p, 32); total = p.c.contactCapacity; }
         p.control[2] = total;
         atomicAdd(w.totals + 0, 1ull);
         atomicAdd(w.totals + 1, 2ull * pairs);
-        atomicAdd(w.totals + 2, (unsigned long long)w.counters[2]);
+        atomicAdd(w.totals + 2, (unsigned long long)w.counters[2 * LINE]);
         atomicAdd(w.totals + 3, (unsigned long long)total);
     }
 }
 __global__ void k_scatter(P p, W w) {
     if (ld_volatile(p.control) != 0) return;
-    uint32_t staged = min(w.counters[1], w.stageCap);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < staged; k += gridDim.x * blockDim.x) {
+    const uint32_t queue = blockIdx.y;
+    uint32_t staged = min(w.stageCount[queue * LINE], w.stageCap);
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < staged; r += gridDim.x * blockDim.x) {
+        size_t k = size_t(queue) * w.stageCap + r;
         uint2 key = w.stageKey[k];
         uint32_t id = w.taskOffset[key.x] + key.y;
         if (id < p.c.contactCapacity) {
@@ -1266,6 +1282,7 @@ int grow(T*& ptr, uint32_t& cap, size_t need, size_t& bytes) {
 struct Caps {
     uint32_t counters = 0, taskCount = 0, taskOffset = 0, adjCount = 0, adjOffset = 0, adjCursor = 0, adjList = 0;
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
+    uint32_t stageCount = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -1274,7 +1291,8 @@ struct ScratchImpl : Scratch {
     int ensure(const Cfg& c) {
         size_t tasks = size_t(2) * c.pairCapacity;
         int rc;
-        if ((rc = grow(w.counters, caps.counters, 16, bytes))) return rc;
+        if ((rc = grow(w.counters, caps.counters, 16 * LINE, bytes))) return rc;
+        if ((rc = grow(w.stageCount, caps.stageCount, 8 * LINE, bytes))) return rc;
         if (!w.totals) {
             HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.totals), 4 * sizeof(unsigned long long)));
             HIP_OK(hipMemset(w.totals, 0, 4 * sizeof(unsigned long long)));
@@ -1288,11 +1306,11 @@ struct ScratchImpl : Scratch {
         if ((rc = grow(w.adjList, caps.adjList, tasks, bytes))) return rc;
         size_t sums = (std::max<size_t>(tasks + 1, c.N) + SCAN_TILE - 1) / SCAN_TILE + 1;
         if ((rc = grow(w.sums, caps.sums, sums, bytes))) return rc;
-        if ((rc = grow(w.stageKey, caps.stageKey, c.contactCapacity, bytes))) return rc;
-        if ((rc = grow(w.stageMeta, caps.stageMeta, c.contactCapacity, bytes))) return rc;
-        if ((rc = grow(w.stagePd, caps.stagePd, c.contactCapacity, bytes))) return rc;
-        if ((rc = grow(w.stageNa, caps.stageNa, c.contactCapacity, bytes))) return rc;
-        if ((rc = grow(w.xcd, caps.xcd, 24, bytes))) return rc;
+        if ((rc = grow(w.stageKey, caps.stageKey, size_t(8) * c.contactCapacity, bytes))) return rc;
+        if ((rc = grow(w.stageMeta, caps.stageMeta, size_t(8) * c.contactCapacity, bytes))) return rc;
+        if ((rc = grow(w.stagePd, caps.stagePd, size_t(8) * c.contactCapacity, bytes))) return rc;
+        if ((rc = grow(w.stageNa, caps.stageNa, size_t(8) * c.contactCapacity, bytes))) return rc;
+        if ((rc = grow(w.xcd, caps.xcd, 24 * LINE, bytes))) return rc;
         if ((rc = grow(w.xcdTasks, caps.xcdTasks, tasks, bytes))) return rc;
         w.stageCap = c.contactCapacity;
         w.taskCap = uint32_t(tasks);
@@ -1310,7 +1328,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.totals);
         (void)hipFree(w.counters); (void)hipFree(w.taskCount); (void)hipFree(w.taskOffset); (void)hipFree(w.adjCount);
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
-        (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide);
+        (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
 };
@@ -1505,12 +1523,12 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         LAUNCH(sc, KID_bucket_reset, k_bucket_reset, dim3(1), dim3(64), s, w);
         LAUNCH(sc, KID_bucket_count, k_bucket_count, gridStride, T, s, p, w);
         LAUNCH(sc, KID_bucket_fill, k_bucket_fill, gridStride, T, s, p, w);
-        const dim3 persistent(std::max(1, sc.cus) * 6);
+        const dim3 persistent(std::max(1, sc.cus) * SAD_COLLECT_BLOCKS_PER_CU);
         LAUNCH(sc, KID_collect, k_collect, persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
-        ScanArgs a{w.taskCount, w.taskOffset, w.sums, 2 * c.pairCapacity, p.control + 1, 2, nullptr, w.counters + 4};
+        ScanArgs a{w.taskCount, w.taskOffset, w.sums, 2 * c.pairCapacity, p.control + 1, 2, nullptr, w.counters + 4 * LINE};
         launch_scan(sc, a, 2 * c.pairCapacity, s);
         LAUNCH(sc, KID_contact_total, k_contact_total, dim3(blocks(c.pairCapacity)), T, s, p, w);
-        LAUNCH(sc, KID_scatter, k_scatter, gridStride, T, s, p, w);
+        LAUNCH(sc, KID_scatter, k_scatter, dim3(gridStride.x / 8 + 1, 8), T, s, p, w);
     } else if (name == "clearContactStorage") LAUNCH(sc, KID_clear_storage, k_clear_storage, gridStride, T, s, p);
     else if (name == "resolveContacts") LAUNCH(sc, KID_resolve, k_resolve, gridStride, T, s, p);
     else if (name == "buildHistoryHash") LAUNCH(sc, KID_build_hash, k_build_hash, gridStride, T, s, p);

@@ -13,7 +13,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 SOURCE = HERE / "csrc" / "sad_hip.hip"
-LIBRARY = HERE / "libsad_hip.so"
+LIBRARY = Path(os.environ.get("SAND_DEM_HIP_LIBRARY", HERE / "libsad_hip.so"))
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",
@@ -29,10 +29,10 @@ FLAGS = [
 ]
 
 
-def compile_library(output=LIBRARY):
-    """Compile csrc/ into `output` (always rebuilds)."""
+def compile_library(output=HERE / "libsad_hip.so", defines=()):
+    """Compile csrc/ into `output` (always rebuilds); `defines` are -D tunables."""
     tmp = Path(str(output) + ".tmp")
-    cmd = [HIPCC, *FLAGS, "-o", str(tmp), str(SOURCE)]
+    cmd = [HIPCC, *FLAGS, *(f"-D{d}" for d in defines), "-o", str(tmp), str(SOURCE)]
     result = subprocess.run(cmd, capture_output=True, text=True)
     if result.returncode:
         raise RuntimeError(f"hipcc failed ({result.returncode}):\n{result.stderr[-4000:]}")
