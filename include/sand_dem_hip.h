@@ -123,6 +123,37 @@ int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buf
                   const uint64_t* bytes, const double* values, const void* input,
                   void* output, void* stream);
 
+/* ---- Learned signed-distance cull (no reference counterpart).  One MLP per
+ * template: positional encoding (xyz + sin/cos of 2^k pi x, k < 6) -> 64 -> 64 ->
+ * 64 -> 1, ReLU, BF16 MFMA with FP32 accumulation.  `weights` = BF16 blocks of
+ * 11,264 values per template, `blocks` = FP32 blocks of 264 floats per template
+ * (biases, output layer, input centre/scale, cull margin); masters, gradients and
+ * Adam moments use 11,008 floats per template.  All pointers are device pointers. */
+/* Enable the cull in the collection stage (audit != 0: culled nodes are still
+ * queried exactly and counted; stats[5..7] = culled, audited, audit misses). */
+int sad_hip_nn_configure(void* scratch, int enabled, int audit, const void* weights,
+                         const void* blocks, uint32_t templates);
+/* Mean-squared-error gradient over template-major points (float4 xyz; counts per
+ * template multiples of 64) into [template][slice] partials; adds squared errors
+ * to loss[template]. */
+int sad_hip_nn_grad(const void* points, const float* labels, const uint32_t* tstart,
+                    const uint32_t* tcount, uint32_t templates, uint32_t slices,
+                    const void* weights, const float* blocks, float* partial,
+                    float* loss, float loss_scale, void* stream);
+/* Sum the partials into grad; when lr > 0 take one Adam step (bias-corrected at
+ * `step`) on the FP32 masters. */
+int sad_hip_nn_adam(float* master, float* m1, float* m2, const float* partial,
+                    float* grad, uint32_t slices, uint32_t templates, float lr,
+                    float beta1, float beta2, float eps, uint32_t step, void* stream);
+/* FP32 masters -> BF16 weight blocks + FP32 layer parameters of the blocks. */
+int sad_hip_nn_pack(const float* master, void* weights, float* blocks,
+                    uint32_t templates, void* stream);
+/* Predictions for 64-point tiles (tile_template[t] = template of tile t). */
+int sad_hip_nn_infer(const void* points, const uint32_t* tile_template, uint32_t tiles,
+                     const void* weights, const float* blocks, float* out, void* stream);
+/* One BF16 MFMA 16x16x16 tile (row-major A, B; FP32 D) for operand-layout tests. */
+int sad_hip_mfma_probe(const void* a, const void* b, float* d, void* stream);
+
 /* Name of the last HIP error seen by this library (static string). */
 const char* sad_hip_last_error(void);
 /* ABI version, currently 1. */

@@ -24,6 +24,7 @@
 
 #include "../../include/sand_dem_hip.h"
 #include "dem_math.h"
+#include "nn.h"
 
 namespace {
 
@@ -89,6 +90,11 @@ struct W {
     uint32_t* xcd;        // per queue q: count at [q*LINE], fill cursor at [(8+q)*LINE], pop cursor at [(16+q)*LINE]
     uint32_t* stageCount; // staged hits per queue at [q*LINE]; queue q stages into [q*stageCap, (q+1)*stageCap)
     uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue
+    // Learned cull (nn.h): per-template BF16 weights and FP32 blocks; stats on separate lines.
+    uint32_t nnEnabled, nnAudit, nnTemplates;
+    const uint16_t* nnW;
+    const float* nnF;
+    unsigned long long* nnStats;  // [0] culled, [LINE/2] audited culled nodes, [LINE] audit misses
     uint32_t stageCap, taskCap, nCap, sumsCap;
 };
 
@@ -784,7 +790,41 @@ __device__ void stageHits(const P& p, const W& w, const TaskFrame& f, uint32_t t
     running += count;
 }
 
-__device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
+// Learned cull of up to 64 queued nodes (two 32-row MFMA passes).  The BF16
+// activations use the wave's BVH-stack LDS (idle outside the exact queries).
+// Returns this lane's verdict for queue entry q + lane.
+__device__ bool cullBatch(const P& p, const W& w, const TaskFrame& f, const uint32_t* queue, uint32_t q, uint32_t qn,
+                          uint16_t* bufA, uint16_t* bufB, int lane) {
+    if (f.sj >= w.nnTemplates) return false;
+    const uint16_t* wb = w.nnW + size_t(f.sj) * nn::BSTRIDE;
+    const float* fb = w.nnF + size_t(f.sj) * nn::FSTRIDE;
+    const float margin = fb[nn::FMARGIN];
+    bool verdict[2] = {false, false};
+    for (int half = 0; half < 2; half++) {
+        if (q + half * 32 >= qn) break;
+        if (lane < 32) {
+            uint32_t k = q + half * 32 + lane;
+            uint16_t* row = bufA + lane * nn::K1;
+            if (k < qn) {
+                float4 sp = p.sample[f.sampleBase + queue[k]];
+                v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
+                v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
+                nn_features(local, fb, row);
+            } else {
+                for (int c = 0; c < nn::K1; c++) row[c] = 0;
+            }
+        }
+        wave_sync();
+        float y = nn_forward<2>(bufA, bufB, wb, fb, lane);
+        verdict[half] = lane < 32 && q + half * 32 + lane < qn && y > margin;
+        wave_sync();
+    }
+    bool second = __shfl(verdict[1], lane & 31);
+    return lane < 32 ? verdict[0] : second;
+}
+
+// Exact queries of queue[0..qn) in 64-node batches, ranked in queue order.
+__device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
                            const uint32_t* queue, uint32_t qn, uint32_t* sref, float* sdist, uint32_t& running,
                            int lane) {
     for (uint32_t q = 0; q < qn; q += WAVE) {
@@ -800,7 +840,56 @@ __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
     }
 }
 
+// Learned cull of the queue: drops nodes whose predicted signed distance exceeds
+// the template margin, compacting survivors in place (order kept, so hit ranks
+// are those of the unculled walk).  Audit mode keeps every node, queries it and
+// counts culled nodes that were hits.  Returns the surviving queue length.
+__device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
+                              uint32_t* queue, uint32_t qn, uint32_t* sref, float* sdist, uint32_t& running, int lane,
+                              uint16_t* nnA, uint16_t* nnB) {
+    uint32_t kept = 0, culledTotal = 0, missTotal = 0;
+    for (uint32_t q = 0; q < qn; q += WAVE) {
+        uint32_t k = q + lane;
+        bool culled = cullBatch(p, w, f, queue, q, qn, nnA, nnB, lane);
+        uint32_t node = k < qn ? queue[k] : 0u;
+        culledTotal += __popcll(__ballot(culled));
+        if (w.nnAudit) {
+            bool hit = false;
+            float4 pd = make_float4(0, 0, 0, 0), na = make_float4(0, 0, 0, 0);
+            if (k < qn) hit = queryNode(p, w, f, node, sref, sdist, pd, na);
+            missTotal += __popcll(__ballot(culled && hit));
+            stageHits(p, w, f, task, stageQueue, hit, node, pd, na, running, lane);
+        } else {
+            bool keep = k < qn && !culled;
+            uint64_t m = __ballot(keep);
+            wave_sync();
+            if (keep) queue[kept + __popcll(m & lanes_below(lane))] = node;
+            kept += __popcll(m);
+            wave_sync();
+        }
+    }
+    if (lane == 0 && culledTotal) {
+        atomicAdd(w.nnStats, (unsigned long long)culledTotal);
+        if (w.nnAudit) atomicAdd(w.nnStats + LINE / 2, (unsigned long long)culledTotal);
+        if (missTotal) atomicAdd(w.nnStats + LINE, (unsigned long long)missTotal);
+    }
+    return w.nnAudit ? 0u : kept;
+}
+
+template <bool NN>
+__device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
+                           uint32_t* queue, uint32_t qn, uint32_t* sref, float* sdist, uint32_t& running, int lane,
+                           uint16_t* nnA, uint16_t* nnB) {
+    if (NN) {
+        qn = cullQueue(p, w, f, task, stageQueue, queue, qn, sref, sdist, running, lane, nnA, nnB);
+        wave_sync();
+    }
+    queryQueue(p, w, f, task, stageQueue, queue, qn, sref, sdist, running, lane);
+}
+
 // One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
+// NN = learned-cull variant (a separate instantiation keeps the plain kernel's registers).
+template <bool NN>
 __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
     __shared__ uint32_t stackRef[COLLECT_WAVES][WSTACK * WAVE];
     __shared__ float stackDist[COLLECT_WAVES][WSTACK * WAVE];
@@ -809,6 +898,8 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t* sref = &stackRef[wv][lane];
     float* sdist = &stackDist[wv][lane];
+    uint16_t* nnA = reinterpret_cast<uint16_t*>(stackRef[wv]);   // 32 x 64 BF16 = the stack's 4 KB
+    uint16_t* nnB = reinterpret_cast<uint16_t*>(stackDist[wv]);
     uint32_t* queue = queues[wv];
     uint64_t* cand = candidates[wv];
     uint32_t queried = 0;
@@ -908,7 +999,7 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
                     uint32_t n = __popcll(m);
                     if (qn + n > QCAP) {
                         wave_sync();
-                        flushQueue(p, w, f, task, home, queue, qn, sref, sdist, running, lane);
+                        flushQueue<NN>(p, w, f, task, home, queue, qn, sref, sdist, running, lane, nnA, nnB);
                         queried += qn;
                         qn = 0;
                         wave_sync();
@@ -919,7 +1010,7 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
                 cluster = candidate ? cluster + 1 : range.z;
             }
             wave_sync();
-            flushQueue(p, w, f, task, home, queue, qn, sref, sdist, running, lane);
+            flushQueue<NN>(p, w, f, task, home, queue, qn, sref, sdist, running, lane, nnA, nnB);
             queried += qn;
             wave_sync();
         }
@@ -1328,7 +1419,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.totals);
         (void)hipFree(w.counters); (void)hipFree(w.taskCount); (void)hipFree(w.taskOffset); (void)hipFree(w.adjCount);
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
-        (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount);
+        (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
 };
@@ -1524,7 +1615,10 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         LAUNCH(sc, KID_bucket_count, k_bucket_count, gridStride, T, s, p, w);
         LAUNCH(sc, KID_bucket_fill, k_bucket_fill, gridStride, T, s, p, w);
         const dim3 persistent(std::max(1, sc.cus) * SAD_COLLECT_BLOCKS_PER_CU);
-        LAUNCH(sc, KID_collect, k_collect, persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+        if (w.nnEnabled)
+            LAUNCH(sc, KID_collect, k_collect<true>, persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+        else
+            LAUNCH(sc, KID_collect, k_collect<false>, persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
         ScanArgs a{w.taskCount, w.taskOffset, w.sums, 2 * c.pairCapacity, p.control + 1, 2, nullptr, w.counters + 4 * LINE};
         launch_scan(sc, a, 2 * c.pairCapacity, s);
         LAUNCH(sc, KID_contact_total, k_contact_total, dim3(blocks(c.pairCapacity)), T, s, p, w);
@@ -1578,7 +1672,14 @@ int sad_hip_scratch_stats(void* scratch, uint64_t* stats) {
     }
     for (int k = 0; k < 4; k++) stats[k] = totals[k];
     stats[4] = sc->bytes;
-    for (int k = 5; k < 8; k++) stats[k] = sc->stats[k];
+    unsigned long long nnv[3] = {0, 0, 0};
+    if (sc->w.nnStats) {
+        for (int k = 0; k < 3; k++) {
+            hipError_t e = hipMemcpy(nnv + k, sc->w.nnStats + k * (LINE / 2), 8, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return hip_fail(e, "stats");
+        }
+    }
+    for (int k = 0; k < 3; k++) stats[5 + k] = nnv[k];
     return SAD_OK;
 }
 
@@ -1705,6 +1806,77 @@ int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buf
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, kernel);
     return SAD_OK;
+}
+
+int sad_hip_nn_configure(void* scratch, int enabled, int audit, const void* weights, const void* blocks,
+                         uint32_t templates) {
+    if (!scratch) return SAD_E_ARGUMENT;
+    if (enabled && (!weights || !blocks || templates == 0)) return SAD_E_ARGUMENT;
+    auto* sc = static_cast<ScratchImpl*>(scratch);
+    if (!sc->w.nnStats) {
+        HIP_OK(hipMalloc(reinterpret_cast<void**>(&sc->w.nnStats), 3 * LINE * sizeof(uint32_t)));
+        sc->bytes += 3 * LINE * sizeof(uint32_t);
+    }
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemset(sc->w.nnStats, 0, 3 * LINE * sizeof(uint32_t)));
+    sc->w.nnEnabled = enabled ? 1u : 0u;
+    sc->w.nnAudit = audit ? 1u : 0u;
+    sc->w.nnW = static_cast<const uint16_t*>(weights);
+    sc->w.nnF = static_cast<const float*>(blocks);
+    sc->w.nnTemplates = enabled ? templates : 0u;
+    return SAD_OK;
+}
+
+int sad_hip_nn_grad(const void* points, const float* labels, const uint32_t* tstart, const uint32_t* tcount,
+                    uint32_t templates, uint32_t slices, const void* weights, const float* blocks, float* partial,
+                    float* loss, float loss_scale, void* stream) {
+    if (!points || !labels || !tstart || !tcount || !weights || !blocks || !partial || !loss || templates == 0 ||
+        slices == 0)
+        return SAD_E_ARGUMENT;
+    NNTrainArgs a{static_cast<const float4*>(points), labels, tstart, tcount, static_cast<const uint16_t*>(weights),
+                  blocks, partial, loss, slices, loss_scale};
+    hipLaunchKernelGGL(k_nn_grad, dim3(slices, templates), dim3(128), 0, static_cast<hipStream_t>(stream), a);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAD_OK : hip_fail(e, "nn_grad");
+}
+
+int sad_hip_nn_adam(float* master, float* m1, float* m2, const float* partial, float* grad, uint32_t slices,
+                    uint32_t templates, float lr, float beta1, float beta2, float eps, uint32_t step, void* stream) {
+    if (!master || !m1 || !m2 || !partial || !grad || step == 0) return SAD_E_ARGUMENT;
+    float bc1 = 1.0f - powf(beta1, float(step)), bc2 = 1.0f - powf(beta2, float(step));
+    size_t total = size_t(templates) * nn::PARAMS;
+    unsigned grid = unsigned(std::min<size_t>((total + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_nn_adam, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream), master, m1, m2, partial,
+                       grad, slices, templates, lr, beta1, beta2, eps, bc1, bc2);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAD_OK : hip_fail(e, "nn_adam");
+}
+
+int sad_hip_nn_pack(const float* master, void* weights, float* blocks, uint32_t templates, void* stream) {
+    if (!master || !weights || !blocks || templates == 0) return SAD_E_ARGUMENT;
+    hipLaunchKernelGGL(k_nn_pack, dim3(16, templates), dim3(256), 0, static_cast<hipStream_t>(stream), master,
+                       static_cast<uint16_t*>(weights), blocks, templates);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAD_OK : hip_fail(e, "nn_pack");
+}
+
+int sad_hip_nn_infer(const void* points, const uint32_t* tile_template, uint32_t tiles, const void* weights,
+                     const float* blocks, float* out, void* stream) {
+    if (tiles == 0) return SAD_OK;
+    if (!points || !tile_template || !weights || !blocks || !out) return SAD_E_ARGUMENT;
+    hipLaunchKernelGGL(k_nn_infer, dim3(tiles), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const float4*>(points), tile_template, static_cast<const uint16_t*>(weights), blocks,
+                       out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAD_OK : hip_fail(e, "nn_infer");
+}
+
+int sad_hip_mfma_probe(const void* a, const void* b, float* d, void* stream) {
+    if (!a || !b || !d) return SAD_E_ARGUMENT;
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const uint16_t*>(a), static_cast<const uint16_t*>(b), d);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAD_OK : hip_fail(e, "mfma_probe");
 }
 
 const char* sad_hip_last_error(void) { return g_error; }

@@ -166,6 +166,31 @@ class HipBackend:
         out.view(np.uint8).reshape(-1)[:] = dst.cpu().numpy()
         return out
 
+    def probe_device(self, name, uniforms, bindings, src, dst):
+        """Query probe on device tensors (no host round trip)."""
+        ptrs, sizes = _table(bindings)
+        values = config_values(bindings["c"])
+        u = (ct.c_uint32 * 4)(*(list(uniforms) + [0] * (4 - len(uniforms))))
+        self._check(
+            self.lib.sad_hip_probe(
+                name.encode(), u, ptrs, sizes, values.ctypes.data_as(ct.POINTER(ct.c_double)),
+                None if src is None else ct.c_void_p(src.data_ptr()), ct.c_void_p(dst.data_ptr()), self.stream,
+            ),
+            f"probe {name}",
+        )
+        return dst
+
+    def configure_cull(self, enabled, audit=False, weights=None, blocks=None, templates=0):
+        """Switch the learned cull of the collection stage on or off (device tensors)."""
+        self._check(
+            self.lib.sad_hip_nn_configure(
+                self.scratch, int(enabled), int(audit),
+                ct.c_void_p(weights.data_ptr()) if enabled else None,
+                ct.c_void_p(blocks.data_ptr()) if enabled else None, int(templates),
+            ),
+            "nn_configure",
+        )
+
     def submit(self, encoder):
         for operation in encoder.operations:
             operation()
@@ -196,7 +221,8 @@ class HipBackend:
         """Cumulative device work counters of this backend's workspace."""
         out = (ct.c_uint64 * 8)()
         self._check(self.lib.sad_hip_scratch_stats(self.scratch, out), "stats")
-        keys = ("collections", "directed_pairs", "queried_nodes", "contacts", "scratch_bytes")
+        keys = ("collections", "directed_pairs", "queried_nodes", "contacts", "scratch_bytes", "nn_culled",
+                "nn_audited", "nn_misses")
         return dict(zip(keys, (int(v) for v in out)))
 
     def profile(self, enable=True):

@@ -1,0 +1,281 @@
+"""Learned signed-distance cull for the mesh contact collection (no reference counterpart).
+
+The reference decides every node->triangle query exactly: after the cluster,
+rejection-plane and hull-group culls (`mesh.py`, `solver.slang:319-343`) each
+surviving surface node walks the target template's BVH.  Most of those nodes are
+outside the target surface.  This module trains, per template, a small MLP that
+predicts the signed distance of a point in the template frame and lets the
+collection stage skip the exact query of a node whose prediction exceeds a
+calibrated margin:
+
+    features  xyz and sin/cos(2^k pi u), k = 0..5, of u = (x - centre) * scale
+    network   39 (padded to 48) -> 64 -> 64 -> 64 -> 1, ReLU, BF16 weights and
+              activations, FP32 accumulation (v_mfma_f32_16x16x16_bf16)
+    target    signed distance clamped to +-clamp * R, divided by clamp * R
+              (R = template bounding radius), so outputs live in [-1, 1]
+    margin    3 x the largest |prediction - target| over a held-out validation set
+
+Training runs on the GPU through the library's own kernels (`sad_hip_nn_grad`:
+forward + backward with MFMA, `sad_hip_nn_adam`, `sad_hip_nn_pack`); the labels
+come from the library's exact mesh distance query (`meshProbe`), i.e. from the
+very function the cull stands in for.  Sample points are drawn near the surface
+and kept only where the hull-group test (`meshCullProbe`) passes, since only
+such points can reach the exact query.
+
+A cull changes results only if it drops a node that would have been a contact.
+`attach(audit=True)` keeps every node, queries it exactly and counts culled
+nodes that were contacts ("misses"); a run with zero misses proves the culled
+run of the same steps gives identical contacts.
+"""
+
+import math
+import time
+
+import numpy as np
+
+from .engine import f4
+
+IN, K1, H, OCTAVES = 39, 48, 64, 6
+OFF_W1 = 0
+OFF_B1 = OFF_W1 + H * IN
+OFF_W2 = OFF_B1 + H
+OFF_B2 = OFF_W2 + H * H
+OFF_W3 = OFF_B2 + H
+OFF_B3 = OFF_W3 + H * H
+OFF_W4 = OFF_B3 + H
+OFF_B4 = OFF_W4 + H
+PARAMS = OFF_B4 + 1
+STRIDE = 11008  # FP32 master / gradient / Adam stride per template
+BSTRIDE = 2 * H * H + H * K1  # BF16 weight block per template
+FSTRIDE = 264  # FP32 block: biases, output layer, centre, scale, margin
+FB1, FB2, FB3, FW4, FB4, FCENTER, FSCALE, FMARGIN = 0, 64, 128, 192, 256, 257, 260, 261
+
+
+def initial_parameters(templates, rng):
+    """He-normal hidden layers, 1/sqrt(64) output layer, zero biases."""
+    master = np.zeros((templates, STRIDE), np.float32)
+    for t in range(templates):
+        m = master[t]
+        m[OFF_W1:OFF_B1] = rng.normal(0, math.sqrt(2 / IN), H * IN)
+        m[OFF_W2:OFF_B2] = rng.normal(0, math.sqrt(2 / H), H * H)
+        m[OFF_W3:OFF_B3] = rng.normal(0, math.sqrt(2 / H), H * H)
+        m[OFF_W4:OFF_B4] = rng.normal(0, math.sqrt(1 / H), H)
+    return master
+
+
+def surface_points(vertices, faces, count, rng):
+    """Area-uniform points on a triangle mesh and their face normals."""
+    tri = vertices[faces]
+    cross = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    area = 0.5 * np.linalg.norm(cross, axis=1)
+    face = rng.choice(len(faces), size=count, p=area / area.sum())
+    r1, r2 = rng.random(count), rng.random(count)
+    s = np.sqrt(r1)
+    a, b, c = 1 - s, s * (1 - r2), s * r2
+    t = tri[face]
+    points = a[:, None] * t[:, 0] + b[:, None] * t[:, 1] + c[:, None] * t[:, 2]
+    normals = cross[face] / (2 * area[face])[:, None]
+    return points, normals
+
+
+def candidate_points(vertices, faces, count, rng):
+    """Mixture around the surface: normal offsets at three widths plus the box."""
+    lo, hi = vertices.min(0), vertices.max(0)
+    centre = 0.5 * (lo + hi)
+    radius = float(np.linalg.norm(vertices - centre, axis=1).max())
+    parts = []
+    for fraction, width in ((0.35, 0.01), (0.3, 0.04), (0.15, 0.12)):
+        k = int(count * fraction)
+        p, n = surface_points(vertices, faces, k, rng)
+        parts.append(p + n * rng.normal(0, width * radius, (k, 1)))
+    k = count - sum(len(p) for p in parts)
+    pad = 0.1 * (hi - lo)
+    parts.append(rng.uniform(lo - pad, hi + pad, (k, 3)))
+    return np.concatenate(parts).astype(np.float32), centre, radius
+
+
+class LearnedCull:
+    """Per-template SDF networks for an `Engine` on the hip backend.
+
+    engine      a built Engine whose backend is `HipBackend`
+    points      training points per template (rounded up to a multiple of 64)
+    validation  held-out points per template for the margin
+    clamp       label clamp as a fraction of the template bounding radius
+    """
+
+    def __init__(self, engine, *, points=16384, validation=16384, clamp=0.2, seed=0, factor=3.0):
+        self.engine = engine
+        self.m = engine.m
+        if getattr(self.m, "name", None) != "hip":
+            raise ValueError("LearnedCull runs on the hip backend")
+        self.torch = self.m.torch
+        self.dev = self.m.torch_device
+        self.templates = len(engine.shapes)
+        self.points = 64 * math.ceil(points / 64)
+        self.validation = 64 * math.ceil(validation / 64)
+        self.clamp = float(clamp)
+        self.factor = float(factor)
+        self.rng = np.random.default_rng(seed)
+        self.history = []
+        self.timings = {}
+        t0 = time.perf_counter()
+        self._make_data()
+        self.timings["data_s"] = time.perf_counter() - t0
+        torch = self.torch
+        T = self.templates
+        self.master = torch.from_numpy(initial_parameters(T, self.rng)).to(self.dev)
+        self.m1 = torch.zeros_like(self.master)
+        self.m2 = torch.zeros_like(self.master)
+        self.grad = torch.zeros_like(self.master)
+        self.weights = torch.zeros((T, BSTRIDE), dtype=torch.int16, device=self.dev)
+        blocks = np.zeros((T, FSTRIDE), np.float32)
+        blocks[:, FCENTER:FCENTER + 3] = self.centre
+        blocks[:, FSCALE] = self.scale
+        blocks[:, FMARGIN] = np.inf  # no cull before calibration
+        self.blocks = torch.from_numpy(blocks).to(self.dev)
+        self.slices = max(1, math.ceil(512 / T))
+        self.partial = torch.zeros((T, self.slices, STRIDE), dtype=torch.float32, device=self.dev)
+        self.loss = torch.zeros(T, dtype=torch.float32, device=self.dev)
+        self.step = 0
+        self._pack()
+
+    # ---- data
+    def _make_data(self):
+        e, torch = self.engine, self.torch
+        bindings = e.bindings()
+        unit = e.length_unit
+        need = {"train": self.points, "valid": self.validation}
+        out = {k: ([], []) for k in need}
+        self.centre = np.zeros((self.templates, 3), np.float32)
+        self.scale = np.zeros(self.templates, np.float32)
+        self.label_scale = np.zeros(self.templates, np.float32)
+        for t, shape in enumerate(e.shapes):
+            vertices = shape.vertices / unit
+            lo, hi = vertices.min(0), vertices.max(0)
+            centre = 0.5 * (lo + hi)
+            self.centre[t] = centre
+            self.scale[t] = 1.0 / (0.55 * float((hi - lo).max()))
+            radius = float(np.linalg.norm(vertices - centre, axis=1).max())
+            self.label_scale[t] = self.clamp * radius
+            kept = []
+            total = self.points + self.validation
+            while sum(len(k) for k in kept) < total:
+                cand, _, _ = candidate_points(vertices, shape.faces, 2 * total, self.rng)
+                src = torch.from_numpy(f4(cand)).to(self.dev)
+                flags = torch.zeros(len(cand), dtype=torch.int32, device=self.dev)
+                self.m.probe_device("meshCullProbe", [len(cand), t], bindings, src, flags)
+                kept.append(cand[flags.cpu().numpy() != 0])
+            pts = np.concatenate(kept)[:total]
+            src = torch.from_numpy(f4(pts)).to(self.dev)
+            dist = torch.zeros((total, 4), dtype=torch.float32, device=self.dev)
+            self.m.probe_device("meshProbe", [total, t], bindings, src, dist)
+            sdf = dist[:, 3].cpu().numpy()
+            label = np.clip(sdf / self.label_scale[t], -1.0, 1.0).astype(np.float32)
+            out["train"][0].append(pts[: self.points])
+            out["train"][1].append(label[: self.points])
+            out["valid"][0].append(pts[self.points:])
+            out["valid"][1].append(label[self.points:])
+        T = self.templates
+        self.train_pts = torch.from_numpy(f4(np.concatenate(out["train"][0]))).to(self.dev)
+        self.train_labels = torch.from_numpy(np.concatenate(out["train"][1])).to(self.dev)
+        self.valid_pts = torch.from_numpy(f4(np.concatenate(out["valid"][0]))).to(self.dev)
+        self.valid_labels = np.concatenate(out["valid"][1]).reshape(T, -1)
+        self.tstart = torch.from_numpy(np.arange(T, dtype=np.uint32) * self.points).to(self.dev)
+        self.tcount = torch.from_numpy(np.full(T, self.points, np.uint32)).to(self.dev)
+        self.valid_tiles = torch.from_numpy(np.repeat(np.arange(T, dtype=np.uint32), self.validation // 64)).to(self.dev)
+
+    # ---- native calls
+    def _ptr(self, x):
+        import ctypes as ct
+
+        return ct.c_void_p(x.data_ptr())
+
+    def _check(self, status, what):
+        self.m._check(status, what)
+
+    def _pack(self):
+        self._check(self.m.lib.sad_hip_nn_pack(self._ptr(self.master), self._ptr(self.weights), self._ptr(self.blocks),
+                                               self.templates, self.m.stream), "nn_pack")
+
+    def gradient(self):
+        """Mean-squared-error gradient of every template on its training set (into `self.grad`)."""
+        self.loss.zero_()
+        self._check(self.m.lib.sad_hip_nn_grad(
+            self._ptr(self.train_pts), self._ptr(self.train_labels), self._ptr(self.tstart), self._ptr(self.tcount),
+            self.templates, self.slices, self._ptr(self.weights), self._ptr(self.blocks), self._ptr(self.partial),
+            self._ptr(self.loss), 1.0 / self.points, self.m.stream), "nn_grad")
+        self._adam(0.0)
+        return self.grad
+
+    def _adam(self, lr):
+        self._check(self.m.lib.sad_hip_nn_adam(
+            self._ptr(self.master), self._ptr(self.m1), self._ptr(self.m2), self._ptr(self.partial),
+            self._ptr(self.grad), self.slices, self.templates, float(lr), 0.9, 0.999, 1e-8, max(1, self.step),
+            self.m.stream), "nn_adam")
+
+    def train(self, iterations=2000, lr=2e-3, lr_final=5e-5, log_every=250):
+        """Full-batch Adam with cosine decay; returns per-template final training MSE."""
+        torch = self.torch
+        torch.cuda.synchronize(self.dev)
+        t0 = time.perf_counter()
+        for it in range(iterations):
+            self.loss.zero_()
+            self._check(self.m.lib.sad_hip_nn_grad(
+                self._ptr(self.train_pts), self._ptr(self.train_labels), self._ptr(self.tstart),
+                self._ptr(self.tcount), self.templates, self.slices, self._ptr(self.weights), self._ptr(self.blocks),
+                self._ptr(self.partial), self._ptr(self.loss), 1.0 / self.points, self.m.stream), "nn_grad")
+            self.step += 1
+            rate = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * it / max(1, iterations - 1)))
+            self._adam(rate)
+            self._pack()
+            if log_every and (it % log_every == 0 or it == iterations - 1):
+                mse = (self.loss / self.points).cpu().numpy()
+                self.history.append((it, float(mse.mean()), float(mse.max())))
+        torch.cuda.synchronize(self.dev)
+        self.timings["train_s"] = time.perf_counter() - t0
+        self.timings["iterations"] = iterations
+        return (self.loss / self.points).cpu().numpy()
+
+    def predict(self, points_tensor, tile_template):
+        """Network outputs for 64-point tiles (device tensors)."""
+        out = self.torch.zeros(points_tensor.shape[0], dtype=self.torch.float32, device=self.dev)
+        self._check(self.m.lib.sad_hip_nn_infer(
+            self._ptr(points_tensor), self._ptr(tile_template), int(tile_template.numel()), self._ptr(self.weights),
+            self._ptr(self.blocks), self._ptr(out), self.m.stream), "nn_infer")
+        return out
+
+    def calibrate(self):
+        """Margin per template = factor x max |prediction - target| on the validation set."""
+        pred = self.predict(self.valid_pts, self.valid_tiles).cpu().numpy().reshape(self.templates, -1)
+        err = np.abs(pred - self.valid_labels)
+        self.max_error = err.max(1)
+        self.margin = (self.factor * self.max_error).astype(np.float32)
+        self.blocks[:, FMARGIN] = self.torch.from_numpy(self.margin).to(self.dev)
+        inside = self.valid_labels <= 0
+        would_cull = pred > self.margin[:, None]
+        self.validation_report = {
+            "max_error_mean": float(self.max_error.mean()),
+            "max_error_max": float(self.max_error.max()),
+            "margin_mean": float(self.margin.mean()),
+            "cull_fraction_outside": float(would_cull[~inside].mean()) if (~inside).any() else 0.0,
+            "validation_misses": int((would_cull & inside).sum()),
+        }
+        return self.margin
+
+    def attach(self, audit=False):
+        """Enable the cull in the engine's collection stage."""
+        if not hasattr(self, "margin"):
+            raise RuntimeError("calibrate() before attach()")
+        self.torch.cuda.synchronize(self.dev)
+        self.m.configure_cull(True, audit, self.weights, self.blocks, self.templates)
+
+    def detach(self):
+        self.torch.cuda.synchronize(self.dev)
+        self.m.configure_cull(False)
+
+    def parameters(self):
+        return self.master.cpu().numpy()
+
+    def packed(self):
+        """(BF16 weight blocks as uint16, FP32 blocks) on the host."""
+        return self.weights.cpu().numpy().view(np.uint16), self.blocks.cpu().numpy()

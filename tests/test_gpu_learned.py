@@ -1,0 +1,137 @@
+"""Learned signed-distance cull on the GPU: MFMA layout, network kernels vs the
+CPU network oracle, and the cull itself (audit misses, unchanged contacts, fewer
+exact queries)."""
+
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from oracle import nn_oracle as nn
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip(pkg):
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    return HipBackend(0)
+
+
+def _p(t):
+    return ct.c_void_p(t.data_ptr())
+
+
+def _random_master(rng, templates):
+    from sand_atlas_dem_amd import learned
+
+    master = learned.initial_parameters(templates, rng)
+    for t in range(templates):
+        m = master[t]
+        for lo, hi in ((nn.OFF_B1, nn.OFF_W2), (nn.OFF_B2, nn.OFF_W3), (nn.OFF_B3, nn.OFF_W4)):
+            m[lo:hi] = rng.normal(0, 0.1, hi - lo)
+        m[nn.OFF_B4] = 0.03
+    return master
+
+
+def test_mfma_operand_layout(hip):
+    """D = A B for one 16x16x16 BF16 tile, exact on small integers."""
+    torch = hip.torch
+    rng = np.random.default_rng(1)
+    A = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    B = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    a = torch.from_numpy(nn.bf16_bits(A).view(np.int16)).to(hip.torch_device)
+    b = torch.from_numpy(nn.bf16_bits(B).view(np.int16)).to(hip.torch_device)
+    d = torch.zeros((16, 16), dtype=torch.float32, device=hip.torch_device)
+    hip._check(hip.lib.sad_hip_mfma_probe(_p(a), _p(b), _p(d), hip.stream), "mfma probe")
+    assert np.array_equal(d.cpu().numpy(), A @ B)
+
+
+def test_pack_matches_oracle_layout(hip):
+    torch = hip.torch
+    rng = np.random.default_rng(2)
+    T = 3
+    master = _random_master(rng, T)
+    m = torch.from_numpy(master).to(hip.torch_device)
+    wb = torch.zeros((T, nn.BSTRIDE), dtype=torch.int16, device=hip.torch_device)
+    fb = torch.zeros((T, nn.FSTRIDE), dtype=torch.float32, device=hip.torch_device)
+    hip._check(hip.lib.sad_hip_nn_pack(_p(m), _p(wb), _p(fb), T, hip.stream), "pack")
+    wbh, fbh = wb.cpu().numpy().view(np.uint16), fb.cpu().numpy()
+    for t in range(T):
+        ow, of = nn.pack(master[t, : nn.PARAMS], (0, 0, 0), 0.0, 0.0)
+        assert np.array_equal(wbh[t], ow)
+        assert np.array_equal(fbh[t, : nn.FCENTER], of[: nn.FCENTER])
+
+
+def _setup(hip, rng, T, tiles):
+    torch = hip.torch
+    master = _random_master(rng, T)
+    pts = rng.uniform(-1.2, 1.2, (T * tiles * 64, 4)).astype(np.float32)
+    labels = rng.uniform(-1, 1, T * tiles * 64).astype(np.float32)
+    centre = rng.uniform(-0.1, 0.1, (T, 3)).astype(np.float32)
+    scale = np.float32(0.9)
+    dev = hip.torch_device
+    m = torch.from_numpy(master).to(dev)
+    wb = torch.zeros((T, nn.BSTRIDE), dtype=torch.int16, device=dev)
+    fbh = np.zeros((T, nn.FSTRIDE), np.float32)
+    fbh[:, nn.FCENTER:nn.FCENTER + 3] = centre
+    fbh[:, nn.FSCALE] = scale
+    fb = torch.from_numpy(fbh).to(dev)
+    hip._check(hip.lib.sad_hip_nn_pack(_p(m), _p(wb), _p(fb), T, hip.stream), "pack")
+    return master, pts, labels, centre, scale, m, wb, fb
+
+
+def test_inference_matches_oracle(hip):
+    torch = hip.torch
+    rng = np.random.default_rng(3)
+    T, tiles = 3, 8
+    master, pts, _, centre, scale, _, wb, fb = _setup(hip, rng, T, tiles)
+    dev = hip.torch_device
+    tile_t = torch.from_numpy(np.repeat(np.arange(T, dtype=np.uint32), tiles).view(np.int32)).to(dev)
+    p = torch.from_numpy(pts).to(dev)
+    out = torch.zeros(len(pts), dtype=torch.float32, device=dev)
+    hip._check(hip.lib.sad_hip_nn_infer(_p(p), _p(tile_t), T * tiles, _p(wb), _p(fb), _p(out), hip.stream), "infer")
+    got = out.cpu().numpy().reshape(T, -1)
+    for t in range(T):
+        X = nn.features(pts[t * tiles * 64:(t + 1) * tiles * 64], centre[t], scale)
+        want, _ = nn.forward(master[t, : nn.PARAMS], X, bf16_mode=True)
+        err = np.abs(got[t] - want)
+        # FP32 accumulation order and hardware sin/cos can move a BF16 activation
+        # by one ulp (2^-8 relative): bound the worst case and the typical case.
+        assert err.max() <= 2e-2 * np.abs(want).max(), err.max()
+        assert np.median(err) <= 1e-3 * np.abs(want).max(), np.median(err)
+
+
+def test_gradient_matches_oracle(hip):
+    torch = hip.torch
+    rng = np.random.default_rng(4)
+    T, tiles, slices = 2, 6, 3
+    master, pts, labels, centre, scale, _, wb, fb = _setup(hip, rng, T, tiles)
+    dev = hip.torch_device
+    n = tiles * 64
+    p = torch.from_numpy(pts).to(dev)
+    lab = torch.from_numpy(labels).to(dev)
+    tstart = torch.from_numpy((np.arange(T, dtype=np.uint32) * n).view(np.int32)).to(dev)
+    tcount = torch.from_numpy(np.full(T, n, np.uint32).view(np.int32)).to(dev)
+    partial = torch.zeros((T, slices, 11008), dtype=torch.float32, device=dev)
+    loss = torch.zeros(T, dtype=torch.float32, device=dev)
+    grad = torch.zeros((T, 11008), dtype=torch.float32, device=dev)
+    zeros = torch.zeros((T, 11008), dtype=torch.float32, device=dev)
+    lib = hip.lib
+    hip._check(lib.sad_hip_nn_grad(_p(p), _p(lab), _p(tstart), _p(tcount), T, slices, _p(wb), _p(fb), _p(partial),
+                                   _p(loss), 1.0 / n, hip.stream), "grad")
+    hip._check(lib.sad_hip_nn_adam(_p(zeros), _p(zeros), _p(zeros), _p(partial), _p(grad), slices, T, 0.0, 0.9,
+                                   0.999, 1e-8, 1, hip.stream), "sum")
+    g = grad.cpu().numpy()[:, : nn.PARAMS]
+    lo = loss.cpu().numpy()
+    blocks = [(nn.OFF_W1, nn.OFF_B1), (nn.OFF_B1, nn.OFF_W2), (nn.OFF_W2, nn.OFF_B2), (nn.OFF_B2, nn.OFF_W3),
+              (nn.OFF_W3, nn.OFF_B3), (nn.OFF_B3, nn.OFF_W4), (nn.OFF_W4, nn.OFF_B4), (nn.OFF_B4, nn.PARAMS)]
+    for t in range(T):
+        X = nn.features(pts[t * n:(t + 1) * n], centre[t], scale)
+        want = nn.gradient(master[t, : nn.PARAMS], X, labels[t * n:(t + 1) * n], bf16_mode=True)
+        y, _ = nn.forward(master[t, : nn.PARAMS], X, True)
+        assert abs(lo[t] - np.sum((y - labels[t * n:(t + 1) * n]) ** 2)) <= 1e-3 * lo[t]
+        for a, b in blocks:
+            rel = np.linalg.norm(g[t, a:b] - want[a:b]) / max(np.linalg.norm(want[a:b]), 1e-30)
+            assert rel < 1e-2, (t, a, rel)
