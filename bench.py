@@ -28,7 +28,9 @@ METRIC = "grain-steps/s at 10k & 50k grains, plain vs NN-culled HIP; force rel e
 WORKLOADS = {
     "dense": "10,000 grains, 64 perturbed icospheres (subdiv 2-4, sigma 0.15), 2,048 nodes/grain, phi 0.60, "
              "isotropic compression 0.5/s, mu 0.5",
-    "uneven": "50,000 grains, 424 templates (320-5,120 triangles log-uniform, 256-4,096 nodes by area), phi 0.60, mu 0.5",
+    "uneven": "50,000 grains, 424 sand-like spherical-harmonic templates (320-5,120 triangles log-uniform, "
+              "256-4,096 nodes by area), phi 0.60, mu 0.5",
+    "concave": "1,000 grains, 8 strongly non-convex harmonic templates (1,280 triangles, 1,024 nodes), phi 0.55, mu 0.5",
     "small": "1,000 icospheres (subdiv 2, sigma 0.1), 512 nodes/grain, phi 0.30, v ~ N(0, 0.01 m/s), mu 0.5",
 }
 
@@ -59,6 +61,66 @@ def kernel_bytes(name, n, pd, q, c, hash_size):
     return table.get(name)
 
 
+def nn_leg(args, w, e, hip, start, step0, plain_state, log_):
+    """Train the learned cull (untimed), audit it, then time the culled path from the
+    same saved state as the plain timed run."""
+    import numpy as np
+    import torch
+    from sand_atlas_dem_amd import learned
+
+    t0 = time.perf_counter()
+    lc = learned.LearnedCull(e, points=args.nn_points, batch=args.nn_batch, validation=16384, seed=args.seed)
+    data_s = time.perf_counter() - t0
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    lc.train(iterations=args.nn_iters, lr=3e-3, log_every=max(1, args.nn_iters // 4))
+    ev1.record()
+    torch.cuda.synchronize()
+    train_ms = ev0.elapsed_time(ev1)
+    lc.calibrate()
+    rep = lc.validation_report
+    points = args.nn_iters * lc.batch * lc.templates
+    flops = 61440.0 * points  # MFMA flops per point per Adam step: forward 22,528 + weight grads 22,528 + backprop 16,384
+    log_(f"[bench] nn: data {data_s:.1f} s, {args.nn_iters} Adam steps in {train_ms / 1e3:.1f} s "
+         f"({flops / (train_ms * 1e-3) / 1e12:.2f} TFLOP/s MFMA), enabled {rep['templates_enabled']}/{rep['templates']} "
+         f"templates, max error {rep['max_error_mean']:.3f} (label units)")
+    out = {
+        "templates_enabled": rep["templates_enabled"],
+        "templates": rep["templates"],
+        "train": {"points_per_template": lc.points, "batch": lc.batch, "iterations": args.nn_iters,
+                  "data_s": data_s, "train_s": train_ms / 1e3, "mfma_tflops": flops / (train_ms * 1e-3) / 1e12,
+                  "mfma_peak_tflops": 2500.0, "final_mse": lc.history[-1][1] if lc.history else None},
+        "validation": rep,
+    }
+    if not lc.attach(audit=True):
+        out.update(value=None, note="no template admits a safe cull (3 x max error >= label clamp); "
+                                    "the NN-culled path is the plain path")
+        return out
+    e.restore(start, step0)
+    before = hip.stats()
+    e.advance(args.audit_steps, rate=w.rate)
+    hip.wait()
+    after = hip.stats()
+    queued = after["queried_nodes"] - before["queried_nodes"]
+    out["audit"] = {"steps": args.audit_steps, "queued_nodes": queued, "culled": after["nn_culled"],
+                    "misses": after["nn_misses"], "cull_fraction": after["nn_culled"] / max(queued, 1)}
+    lc.attach(audit=False)
+    e.restore(start, step0)
+    hip.wait()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.advance(args.steps, rate=w.rate)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lc.detach()
+    st = e.state()
+    same = st["contacts"] == plain_state["contacts"] and all(
+        np.array_equal(st[k], plain_state[k]) for k in ("meta", "force", "torque", "position", "rotation"))
+    out.update(value=e.n * args.steps / elapsed, ms_per_step=1e3 * elapsed / args.steps, identical_to_plain=bool(same))
+    log_(f"[bench] nn-culled: {out['ms_per_step']:.3f} ms/step, audit {out['audit']}, identical {same}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +132,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps in the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one instead of a captured step")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-nn", action="store_true", help="skip the learned-cull leg")
+    ap.add_argument("--nn-points", type=int, default=1 << 20)
+    ap.add_argument("--nn-batch", type=int, default=16384)
+    ap.add_argument("--nn-iters", type=int, default=4000)
+    ap.add_argument("--audit-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -77,7 +146,7 @@ def main():
         sys.exit(2)
 
     pkg = load_package()
-    from sand_atlas_dem_amd import assembly
+    from sand_atlas_dem_amd import assembly, autotune
     from sand_atlas_dem_amd.hip import HipBackend
     import numpy as np
     import torch
@@ -86,7 +155,7 @@ def main():
     w = assembly.config_workload(args.config, seed=args.seed)
     log(f"[bench] workload {args.config}: {len(w.ids)} grains, {len(w.shapes)} templates, "
         f"dt {w.spec.timestep:.3g} s, generated in {time.perf_counter() - t0:.1f} s")
-    hip = HipBackend(0)
+    hip = HipBackend(0, graph=not args.no_graph)
     t0 = time.perf_counter()
     e = w.engine(hip)
     e.initialize()
@@ -94,6 +163,14 @@ def main():
     e.advance(args.warmup, rate=w.rate)
     hip.wait()
     log(f"[bench] warmup {args.warmup} steps done")
+    tuned = None
+    if not args.no_autotune:
+        tuned = autotune.autotune(e, rate=w.rate, steps=3, log=log)
+        log(f"[bench] autotuned launch shape {tuned['best']} ({tuned['ms_per_step']:.3f} ms/step)")
+    start, step0 = e.state(), e.step
+    e.restore(start, step0)
+    e.advance(1, rate=w.rate)  # capture the graphs outside the timed region
+    e.restore(start, step0)
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -104,6 +181,7 @@ def main():
     value = n * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} grain-steps/s")
+    plain_state = e.state()
 
     # Per-kernel HIP-event profile over a separate window (same stream as the launches).
     before = hip.stats()
@@ -139,11 +217,11 @@ def main():
         from oracle import oracle
 
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        state = e.state()
+        state = plain_state
         cpu = w.engine(oracle.make_oracle(pkg, threads=threads))
-        cpu.restore(state, e.step)
+        cpu.restore(state, step0 + args.steps)
         ref = w.engine(hip)
-        ref.restore(state, e.step)
+        ref.restore(state, step0 + args.steps)
         t0 = time.perf_counter()
         cpu.advance(args.cpu_steps, rate=w.rate)
         cpu_s = time.perf_counter() - t0
@@ -168,6 +246,10 @@ def main():
         }
         log(f"[bench] cpu baseline {cpu_baseline['value']:.4g} grain-steps/s on {threads} threads; parity {parity}")
 
+    nn = None if args.no_nn else nn_leg(args, w, e, hip, start, step0, plain_state, log)
+    if nn and nn.get("value"):
+        nn["speedup_vs_plain"] = nn["value"] / value
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -181,8 +263,9 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded templates and packing)",
-        "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "grains": n, "path": "plain HIP",
-                   "seed": args.seed},
+        "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "grains": n,
+                   "path": "plain HIP" + (" (hipGraph-captured steps)" if not args.no_graph else ""),
+                   "launch_shape": tuned["best"] if tuned else None, "seed": args.seed},
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,
@@ -201,6 +284,7 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu_baseline,
         "parity": parity,
+        "nn_culled": nn,
     }
     print(json.dumps(out), flush=True)
 

@@ -123,6 +123,13 @@ int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buf
                   const uint64_t* bytes, const double* values, const void* input,
                   void* output, void* stream);
 
+/* Launch-shape tunables of the collection stage (no reference counterpart; none
+ * changes any result): "collect_blocks_per_cu" (1..32 workgroups per CU in the
+ * persistent launch), "collect_pop" (1..16 tasks taken per queue atomic),
+ * "collect_queue" (128 or 256 queued nodes per wavefront).  value < 0 only reads;
+ * *previous (if non-null) receives the value before the call. */
+int sad_hip_tune(void* scratch, const char* name, int64_t value, int64_t* previous);
+
 /* ---- Learned signed-distance cull (no reference counterpart).  One MLP per
  * template: positional encoding (xyz + sin/cos of 2^k pi x, k < 6) -> 64 -> 64 ->
  * 64 -> 1, ReLU, BF16 MFMA with FP32 accumulation.  `weights` = BF16 blocks of
@@ -151,6 +158,16 @@ int sad_hip_nn_pack(const float* master, void* weights, float* blocks,
 /* Predictions for 64-point tiles (tile_template[t] = template of tile t). */
 int sad_hip_nn_infer(const void* points, const uint32_t* tile_template, uint32_t tiles,
                      const void* weights, const float* blocks, float* out, void* stream);
+/* Training points for every template: uniforms = (templates, points per template,
+ * seed high, seed low); `buffers`/`values` as for sad_hip_dispatch (the packed mesh
+ * arrays).  triangles: 3 float4 rows per triangle (normalized units) with per-
+ * template ranges tri_start/tri_count and inclusive area prefix `cdf`; box: per
+ * template (lo, grain diameter), (hi, label scale).  Writes float4 points and
+ * labels = exact signed distance / label scale clamped to [-1, 1]. */
+int sad_hip_nn_sample(const uint32_t* uniforms, void* const* buffers, const uint64_t* bytes,
+                      const double* values, const void* triangles, const float* cdf,
+                      const uint32_t* tri_start, const uint32_t* tri_count, const void* box,
+                      void* points, float* labels, void* stream);
 /* One BF16 MFMA 16x16x16 tile (row-major A, B; FP32 D) for operand-layout tests. */
 int sad_hip_mfma_probe(const void* a, const void* b, float* d, void* stream);
 

@@ -4,7 +4,7 @@ Import name: `sand_atlas_dem_amd` (the directory is `sand-atlas-dem_amd/`; the r
 root's `load_package()` helpers register it under that name).
 """
 
-from . import assembly, engine, geometry, native, spec  # noqa: F401
+from . import assembly, autotune, engine, geometry, learned, native, spec  # noqa: F401
 from .engine import Engine, RejectedStep  # noqa: F401
 from .spec import Experiment  # noqa: F401
 

@@ -39,15 +39,26 @@ def icosphere_library(count, *, subdivisions, sigma, diameter, nodes, rng, seed)
     return shapes
 
 
-def uneven_library(count, *, tri_range=(320, 5120), node_range=(256, 4096), sigma, diameter, rng, seed):
-    """Templates with log-uniform triangle counts and node counts proportional to area."""
+def uneven_library(count, *, tri_range=(320, 5120), node_range=(256, 4096), surface="harmonic", sigma=0.15,
+                   diameter, rng, seed):
+    """Templates with log-uniform triangle counts and node counts proportional to area.
+
+    surface "harmonic": sand-like spherical-harmonic grains (resolution-independent
+    shape); "lognormal": per-vertex lognormal(0, sigma) radii.
+    """
     tris = np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]), count)).astype(int)
     shapes = []
+    descriptors = geometry.harmonic_descriptors()
     for k, t in enumerate(tris):
         v, f = geometry.hull_sphere(t)
-        shapes.append(
-            geometry.perturbed_template(f"hull{t}-{k}", v, f, sigma=sigma, diameter=diameter, nodes=16, rng=rng, seed=seed + k)
-        )
+        if surface == "harmonic":
+            shapes.append(geometry.harmonic_template(f"sh{t}-{k}", v, f, descriptors=descriptors, diameter=diameter,
+                                                     nodes=16, rng=rng, seed=seed + k))
+        elif surface == "lognormal":
+            shapes.append(geometry.perturbed_template(f"hull{t}-{k}", v, f, sigma=sigma, diameter=diameter, nodes=16,
+                                                      rng=rng, seed=seed + k))
+        else:
+            raise ValueError(f"unknown surface {surface!r}")
     areas = np.array([s.areas.sum() for s in shapes])
     lo, hi = areas.min(), areas.max()
     counts = node_range[0] + (node_range[1] - node_range[0]) * (areas - lo) / max(hi - lo, 1e-30)
@@ -169,8 +180,10 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
     small  : 1,000 icospheres (subdiv 2, sigma 0.1), 512 nodes, phi 0.30, v ~ N(0, 0.01 m/s)
     dense  : 10,000 grains from 64 perturbed icospheres (subdiv 2-4, sigma 0.15), 2,048
              nodes, phi 0.60, isotropic compression at 0.5/s
-    uneven : 50,000 grains from 424 templates (320-5,120 triangles log-uniform,
-             256-4,096 nodes by area), phi 0.60
+    uneven : 50,000 grains from 424 sand-like spherical-harmonic templates (320-5,120
+             triangles log-uniform, 256-4,096 nodes by area), phi 0.60
+    concave: 1,000 grains from 8 strongly non-convex low-degree harmonic templates
+             (1,280 triangles, 1,024 nodes), phi 0.55 -- loose hull bounds
     """
     rng = streams(seed)
     d0 = 3e-4
@@ -194,8 +207,20 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
         cpp = 512  # ~190 node contacts per grain at phi 0.60: headroom over the default 256
     elif config == "uneven":
         n = n or 50_000
-        shapes = uneven_library(templates or 424, sigma=0.15, diameter=d0, rng=rng["templates"], seed=seed)
+        shapes = uneven_library(templates or 424, diameter=d0, rng=rng["templates"], seed=seed)
         phi, sphere_fraction, vsig, rate = 0.60, 0.62, 0.0, (0, 0, 0)
+        steps = steps or 1000
+        cpp = 512
+    elif config == "concave":
+        n = n or 1000
+        rngt = rng["templates"]
+        descriptors = geometry.harmonic_descriptors(d2=0.3, top=6, decay=1.0)
+        shapes = []
+        for k in range(templates or 8):
+            v, f = geometry.icosphere(3)
+            shapes.append(geometry.harmonic_template(f"concave-{k}", v, f, descriptors=descriptors, diameter=d0,
+                                                     nodes=nodes or 1024, rng=rngt, seed=seed + k))
+        phi, sphere_fraction, vsig, rate = 0.55, 0.57, 0.0, (0, 0, 0)
         steps = steps or 1000
         cpp = 512
     else:

@@ -96,6 +96,7 @@ struct W {
     const float* nnF;
     unsigned long long* nnStats;  // [0] culled, [LINE/2] audited culled nodes, [LINE] audit misses
     uint32_t stageCap, taskCap, nCap, sumsCap;
+    uint32_t pop;         // tasks taken per queue atomic (tunable)
 };
 
 constexpr int WAVE = 64;
@@ -113,7 +114,7 @@ constexpr int LINE = 64;
 #ifndef SAD_COLLECT_BLOCKS_PER_CU
 #define SAD_COLLECT_BLOCKS_PER_CU 6
 #endif
-constexpr int POP = SAD_POP;          // tasks taken per queue atomic
+constexpr int POP = SAD_POP;          // default tasks taken per queue atomic
 
 __device__ __forceinline__ void fail(const P& p, uint32_t bit) { atomicOr(p.control, bit); }
 __device__ __forceinline__ uint32_t ld_volatile(const uint32_t* a) { return __atomic_load_n(a, __ATOMIC_RELAXED); }
@@ -880,7 +881,8 @@ template <bool NN>
 __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
                            uint32_t* queue, uint32_t qn, uint32_t* sref, float* sdist, uint32_t& running, int lane,
                            uint16_t* nnA, uint16_t* nnB) {
-    if (NN) {
+    // Templates whose margin is infinite (calibration found no safe cull) skip the network.
+    if (NN && f.sj < w.nnTemplates && w.nnF[size_t(f.sj) * nn::FSTRIDE + nn::FMARGIN] < 1e30f) {
         qn = cullQueue(p, w, f, task, stageQueue, queue, qn, sref, sdist, running, lane, nnA, nnB);
         wave_sync();
     }
@@ -888,12 +890,14 @@ __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
 }
 
 // One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
-// NN = learned-cull variant (a separate instantiation keeps the plain kernel's registers).
-template <bool NN>
+// NN = learned-cull variant (a separate instantiation keeps the plain kernel's registers);
+// QN = query-queue capacity per wave (tunable: results do not depend on it, since hits
+// are ranked by their running count within the task, not by flush boundaries).
+template <bool NN, int QN>
 __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
     __shared__ uint32_t stackRef[COLLECT_WAVES][WSTACK * WAVE];
     __shared__ float stackDist[COLLECT_WAVES][WSTACK * WAVE];
-    __shared__ uint32_t queues[COLLECT_WAVES][QCAP];
+    __shared__ uint32_t queues[COLLECT_WAVES][QN];
     __shared__ uint64_t candidates[COLLECT_WAVES][MAX_CLUSTERS / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t* sref = &stackRef[wv][lane];
@@ -919,9 +923,9 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
             while (tried < 8) {
                 uint32_t qi = (home + tried) & 7;
                 uint32_t k = 0;
-                if (lane == 0) k = atomicAdd(w.xcd + (16 + qi) * LINE, uint32_t(POP));
+                if (lane == 0) k = atomicAdd(w.xcd + (16 + qi) * LINE, w.pop);
                 k = __shfl(k, 0);
-                if (k < qcount[qi]) { from = qi; next = k; left = min(uint32_t(POP), qcount[qi] - k); break; }
+                if (k < qcount[qi]) { from = qi; next = k; left = min(w.pop, qcount[qi] - k); break; }
                 tried++;
             }
         }
@@ -997,7 +1001,7 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
                     }
                     uint64_t m = __ballot(pass_);
                     uint32_t n = __popcll(m);
-                    if (qn + n > QCAP) {
+                    if (qn + n > QN) {
                         wave_sync();
                         flushQueue<NN>(p, w, f, task, home, queue, qn, sref, sdist, running, lane, nnA, nnB);
                         queried += qn;
@@ -1276,6 +1280,71 @@ __global__ __launch_bounds__(256) void k_gather(P p, W w) {
     accumulateRow(p, id, {dissipation, 0, 0, 0});
 }
 
+// ---------------------------------------------------------------- learned-cull training data
+// Counter-based uniforms (splitmix64 finaliser): reproducible per (seed, template, point, draw).
+__device__ __forceinline__ uint64_t nn_mix(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float nn_unit(uint64_t bits) { return float(uint32_t(bits >> 40)) * (1.0f / 16777216.0f); }
+
+struct NNSampleArgs {
+    const float4* triangles;   // per template: 3 rows per triangle, normalized units
+    const float* cdf;          // inclusive area prefix per triangle, last = 1 per template
+    const uint32_t* triStart;  // first triangle of each template
+    const uint32_t* triCount;
+    const float4* box;         // per template: (lo, diameter), (hi, label scale)
+    float4* points;            // [template][count]
+    float* labels;
+    uint32_t count;
+    uint64_t seed;
+};
+// One candidate per point, redrawn until it passes the hull-group test the
+// collection applies before any exact query (at most 64 draws): even points lie
+// within one grain diameter of the surface (offset d s^3 along the face normal,
+// s ~ U(-1, 1)), odd points are uniform in the bounding box padded by 10%.  The
+// label is the exact BVH signed distance, clamped to +-1 label units.
+__global__ void k_nn_sample(P p, NNSampleArgs a) {
+    __shared__ uint16_t stacks[4][STACK * WAVE];
+    const uint32_t t = blockIdx.y, id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= a.count) return;
+    uint16_t* stack = &stacks[threadIdx.x >> 6][threadIdx.x & 63];
+    const float4 b0 = a.box[2 * t], b1 = a.box[2 * t + 1];
+    const v3 lo = xyz(b0), hi = xyz(b1), pad = 0.1f * (hi - lo);
+    const uint32_t first = a.triStart[t], n = a.triCount[t];
+    v3 q = {0, 0, 0};
+    for (uint32_t draw = 0; draw < 64; draw++) {
+        uint64_t base = nn_mix(a.seed ^ nn_mix((uint64_t(t) << 40) ^ (uint64_t(id) << 8) ^ draw));
+        float u0 = nn_unit(base), u1 = nn_unit(nn_mix(base + 1)), u2 = nn_unit(nn_mix(base + 2)),
+              u3 = nn_unit(nn_mix(base + 3));
+        if ((id & 1) == 0) {
+            uint32_t l = 0, r = n - 1;  // first triangle whose inclusive prefix exceeds u0
+            while (l < r) {
+                uint32_t m = (l + r) >> 1;
+                if (a.cdf[first + m] > u0) r = m; else l = m + 1;
+            }
+            const float4* tri = a.triangles + 3 * size_t(first + l);
+            v3 A = xyz(tri[0]), B = xyz(tri[1]), C = xyz(tri[2]);
+            float sq = sqrtf(u1);
+            v3 x = (1.0f - sq) * A + (sq * (1.0f - u2)) * B + (sq * u2) * C;
+            v3 nrm = cross(B - A, C - A);
+            nrm = nrm / length(nrm);
+            float s = 2.0f * u3 - 1.0f;
+            q = x + (b0.w * s * s * s) * nrm;
+        } else {
+            q = mk3(lo.x - pad.x + u0 * (hi.x - lo.x + 2 * pad.x), lo.y - pad.y + u1 * (hi.y - lo.y + 2 * pad.y),
+                    lo.z - pad.z + u2 * (hi.z - lo.z + 2 * pad.z));
+        }
+        if (meshMayContain(p, t, q)) break;
+    }
+    const size_t out = size_t(t) * a.count + id;
+    a.points[out] = f4(q, 0.0f);
+    float d = meshDistance(p, t, q, stack).w / b1.w;
+    a.labels[out] = fminf(fmaxf(d, -1.0f), 1.0f);
+}
+
 // ---------------------------------------------------------------- probes
 __global__ void k_mesh_probe(P p, uint32_t count, uint32_t shape, const float4* points, float4* out) {
     __shared__ uint16_t stacks[4][STACK * WAVE];
@@ -1378,6 +1447,18 @@ struct Caps {
 
 struct ScratchImpl : Scratch {
     Caps caps;
+    // Tunables of the collection launch (sad_hip_tune); none changes results.
+    int blocksPerCU = SAD_COLLECT_BLOCKS_PER_CU;
+    uint32_t pop = POP;
+    int queue = QCAP;
+    // Captured timesteps keyed by everything a step's launches bake in.
+    struct Graph { std::vector<uint64_t> key; hipGraphExec_t exec; };
+    std::vector<Graph> graphs;
+    hipStream_t captureStream = nullptr;
+    void drop_graphs() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
     int ensure_nodes(uint32_t nodes) { return grow(w.wide, caps.wide, 4 * size_t(std::max(nodes, 1u)), bytes); }
     int ensure(const Cfg& c) {
         size_t tasks = size_t(2) * c.pairCapacity;
@@ -1414,6 +1495,8 @@ struct ScratchImpl : Scratch {
         return SAD_OK;
     }
     ~ScratchImpl() {
+        drop_graphs();
+        if (captureStream) (void)hipStreamDestroy(captureStream);
         for (auto& m : pending) { pool.push_back(m.start); pool.push_back(m.stop); }
         for (hipEvent_t e : pool) (void)hipEventDestroy(e);
         (void)hipFree(w.totals);
@@ -1614,11 +1697,19 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         LAUNCH(sc, KID_bucket_reset, k_bucket_reset, dim3(1), dim3(64), s, w);
         LAUNCH(sc, KID_bucket_count, k_bucket_count, gridStride, T, s, p, w);
         LAUNCH(sc, KID_bucket_fill, k_bucket_fill, gridStride, T, s, p, w);
-        const dim3 persistent(std::max(1, sc.cus) * SAD_COLLECT_BLOCKS_PER_CU);
-        if (w.nnEnabled)
-            LAUNCH(sc, KID_collect, k_collect<true>, persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
-        else
-            LAUNCH(sc, KID_collect, k_collect<false>, persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+        const dim3 persistent(std::max(1, sc.cus) * sc.blocksPerCU);
+        w.pop = sc.pop;
+        if (w.nnEnabled) {
+            if (sc.queue == 128)
+                LAUNCH(sc, KID_collect, (k_collect<true, 128>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+            else
+                LAUNCH(sc, KID_collect, (k_collect<true, 256>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+        } else {
+            if (sc.queue == 128)
+                LAUNCH(sc, KID_collect, (k_collect<false, 128>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+            else
+                LAUNCH(sc, KID_collect, (k_collect<false, 256>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+        }
         ScanArgs a{w.taskCount, w.taskOffset, w.sums, 2 * c.pairCapacity, p.control + 1, 2, nullptr, w.counters + 4 * LINE};
         launch_scan(sc, a, 2 * c.pairCapacity, s);
         LAUNCH(sc, KID_contact_total, k_contact_total, dim3(blocks(c.pairCapacity)), T, s, p, w);
@@ -1727,10 +1818,81 @@ int sad_hip_dispatch(const char* kernel, uint32_t count, void* const* buffers, c
     }
 }
 
+// One timestep's launches (per-step order of gpu.py:714-726).
+int launch_step(P& p, ScratchImpl& sc, hipStream_t s) {
+    static const char* sequence[] = {"beginStep", "advanceCell", "checkRebuild", "clearBins", "fillBins",
+                                     "countPairs", "pairScan", "emitPairs", "resetMeshContacts",
+                                     "collectMeshFrontiers", "clearContactStorage", "resolveContacts",
+                                     "buildHistoryHash", "gather"};
+    int rc;
+    if (p.c.activeTimestep && (rc = run_kernel("checkContactTimestep", p, sc, s))) return rc;
+    for (const char* name : sequence) {
+        if (std::strcmp(name, "clearBins") == 0 && p.c.normalViscosity > 0 &&
+            (rc = run_kernel("captureContactKinematics", p, sc, s)))
+            return rc;
+        if ((rc = run_kernel(name, p, sc, s))) return rc;
+    }
+    if (p.c.activeTimestep && (rc = run_kernel("checkContactTimestep", p, sc, s))) return rc;
+    return run_kernel("finishStep", p, sc, s);
+}
+
+// Graph key: every value a captured step bakes into its launches.
+std::vector<uint64_t> graph_key(void* const* table, const uint64_t* sizes, const double* vals, const ScratchImpl& sc) {
+    std::vector<uint64_t> key;
+    key.reserve(2 * SAD_BUFFER_COUNT + SAD_VALUE_COUNT + sizeof(W) / 8 + 4);
+    for (int k = 0; k < SAD_BUFFER_COUNT; k++) key.push_back(reinterpret_cast<uint64_t>(table[k]));
+    for (int k = 0; k < SAD_BUFFER_COUNT; k++) key.push_back(sizes[k]);
+    for (int k = 0; k < SAD_VALUE_COUNT; k++) {
+        uint64_t bits;
+        std::memcpy(&bits, vals + k, 8);
+        key.push_back(bits);
+    }
+    uint64_t words[(sizeof(W) + 7) / 8] = {};
+    std::memcpy(words, &sc.w, sizeof(W));
+    for (uint64_t v : words) key.push_back(v);
+    key.push_back(uint64_t(sc.blocksPerCU));
+    key.push_back(uint64_t(sc.pop));
+    key.push_back(uint64_t(sc.queue));
+    key.push_back(uint64_t(sc.cus));
+    return key;
+}
+
+// Replay (capturing on first use) the step for this key on stream s.
+int graph_step(P& p, ScratchImpl& sc, const std::vector<uint64_t>& key, hipStream_t s) {
+    hipGraphExec_t exec = nullptr;
+    for (size_t k = 0; k < sc.graphs.size(); k++)
+        if (sc.graphs[k].key == key) {
+            exec = sc.graphs[k].exec;
+            std::rotate(sc.graphs.begin(), sc.graphs.begin() + k, sc.graphs.begin() + k + 1);  // most recent first
+            break;
+        }
+    if (!exec) {
+        if (!sc.captureStream) HIP_OK(hipStreamCreateWithFlags(&sc.captureStream, hipStreamNonBlocking));
+        HIP_OK(hipStreamBeginCapture(sc.captureStream, hipStreamCaptureModeRelaxed));
+        int rc = launch_step(p, sc, sc.captureStream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(sc.captureStream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+        if (sc.graphs.size() >= 4) {
+            (void)hipGraphExecDestroy(sc.graphs.back().exec);
+            sc.graphs.pop_back();
+        }
+        sc.graphs.insert(sc.graphs.begin(), ScratchImpl::Graph{key, exec});
+    }
+    HIP_OK(hipGraphLaunch(exec, s));
+    return SAD_OK;
+}
+
 int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const uint64_t* bytes,
                     const double* values, void* const* sides, const uint64_t* side_bytes, void* scratch,
                     void* stream, int graph) {
-    (void)graph;
     if (!scratch || !sides || !side_bytes) return SAD_E_ARGUMENT;
     try {
         void* table[SAD_BUFFER_COUNT];
@@ -1741,10 +1903,6 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const
         std::memcpy(vals, values, sizeof(vals));
         auto* sc = static_cast<ScratchImpl*>(scratch);
         hipStream_t s = static_cast<hipStream_t>(stream);
-        static const char* sequence[] = {"beginStep", "advanceCell", "checkRebuild", "clearBins", "fillBins",
-                                         "countPairs", "pairScan", "emitPairs", "resetMeshContacts",
-                                         "collectMeshFrontiers", "clearContactStorage", "resolveContacts",
-                                         "buildHistoryHash", "gather"};
         for (uint32_t k = 0; k < steps; k++) {
             uint32_t side = (active + k) & 1, old = 1 - side;
             const int slotsNew[4] = {SAD_META, SAD_SLIP, SAD_NORMAL_AREA, SAD_HASH};
@@ -1760,21 +1918,51 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const
             int rc = bind(p, table, sizes, vals);
             if (rc) return rc;
             if ((rc = sc->ensure(p.c))) return rc;
-            if (p.c.activeTimestep && (rc = run_kernel("checkContactTimestep", p, *sc, s))) return rc;
-            for (const char* name : sequence) {
-                if (std::strcmp(name, "clearBins") == 0 && p.c.normalViscosity > 0 &&
-                    (rc = run_kernel("captureContactKinematics", p, *sc, s)))
-                    return rc;
-                if ((rc = run_kernel(name, p, *sc, s))) return rc;
+            if ((rc = sc->ensure_nodes(p.meshNodes))) return rc;
+            if (graph && !sc->profiling) {
+                sc->w.pop = sc->pop;
+                rc = graph_step(p, *sc, graph_key(table, sizes, vals, *sc), s);
+            } else {
+                rc = launch_step(p, *sc, s);
             }
-            if (p.c.activeTimestep && (rc = run_kernel("checkContactTimestep", p, *sc, s))) return rc;
-            if ((rc = run_kernel("finishStep", p, *sc, s))) return rc;
+            if (rc) return rc;
         }
         return SAD_OK;
     } catch (...) {
         std::snprintf(g_error, sizeof(g_error), "exception in advance");
         return SAD_E_HIP;
     }
+}
+
+int sad_hip_tune(void* scratch, const char* name, int64_t value, int64_t* previous) {
+    if (!scratch || !name) return SAD_E_ARGUMENT;
+    auto* sc = static_cast<ScratchImpl*>(scratch);
+    std::string key(name);
+    int64_t old;
+    if (key == "collect_blocks_per_cu") {
+        old = sc->blocksPerCU;
+        if (value >= 0) {
+            if (value < 1 || value > 32) return SAD_E_ARGUMENT;
+            sc->blocksPerCU = int(value);
+        }
+    } else if (key == "collect_pop") {
+        old = sc->pop;
+        if (value >= 0) {
+            if (value < 1 || value > 16) return SAD_E_ARGUMENT;
+            sc->pop = uint32_t(value);
+        }
+    } else if (key == "collect_queue") {
+        old = sc->queue;
+        if (value >= 0) {
+            if (value != 128 && value != 256) return SAD_E_ARGUMENT;
+            sc->queue = int(value);
+        }
+    } else {
+        std::snprintf(g_error, sizeof(g_error), "unknown tunable %s", name);
+        return SAD_E_ARGUMENT;
+    }
+    if (previous) *previous = old;
+    return SAD_OK;
 }
 
 int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buffers, const uint64_t* bytes,
@@ -1869,6 +2057,24 @@ int sad_hip_nn_infer(const void* points, const uint32_t* tile_template, uint32_t
                        out);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SAD_OK : hip_fail(e, "nn_infer");
+}
+
+int sad_hip_nn_sample(const uint32_t* uniforms, void* const* buffers, const uint64_t* bytes, const double* values,
+                      const void* triangles, const float* cdf, const uint32_t* tri_start, const uint32_t* tri_count,
+                      const void* box, void* points, float* labels, void* stream) {
+    if (!uniforms || !triangles || !cdf || !tri_start || !tri_count || !box || !points || !labels)
+        return SAD_E_ARGUMENT;
+    P p{};
+    int rc = bind(p, buffers, bytes, values);
+    if (rc) return rc;
+    const uint32_t templates = uniforms[0], count = uniforms[1];
+    if (templates == 0 || count == 0) return SAD_OK;
+    NNSampleArgs a{static_cast<const float4*>(triangles), cdf, tri_start, tri_count, static_cast<const float4*>(box),
+                   static_cast<float4*>(points), labels, count,
+                   (uint64_t(uniforms[2]) << 32) | uniforms[3]};
+    hipLaunchKernelGGL(k_nn_sample, dim3(blocks(count), templates), dim3(256), 0, static_cast<hipStream_t>(stream), p, a);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAD_OK : hip_fail(e, "nn_sample");
 }
 
 int sad_hip_mfma_probe(const void* a, const void* b, float* d, void* stream) {

@@ -180,6 +180,12 @@ class HipBackend:
         )
         return dst
 
+    def tune(self, name, value=-1):
+        """Read (value < 0) or set a collection launch tunable; returns the previous value."""
+        prev = ct.c_int64(0)
+        self._check(self.lib.sad_hip_tune(self.scratch, name.encode(), int(value), ct.byref(prev)), f"tune {name}")
+        return int(prev.value)
+
     def configure_cull(self, enabled, audit=False, weights=None, blocks=None, templates=0):
         """Switch the learned cull of the collection stage on or off (device tensors)."""
         self._check(

@@ -17,10 +17,11 @@ calibrated margin:
 
 Training runs on the GPU through the library's own kernels (`sad_hip_nn_grad`:
 forward + backward with MFMA, `sad_hip_nn_adam`, `sad_hip_nn_pack`); the labels
-come from the library's exact mesh distance query (`meshProbe`), i.e. from the
-very function the cull stands in for.  Sample points are drawn near the surface
-and kept only where the hull-group test (`meshCullProbe`) passes, since only
-such points can reach the exact query.
+come from the library's exact mesh distance query, i.e. from the very function
+the cull stands in for.  `sad_hip_nn_sample` draws the points on the GPU, half
+within one grain diameter of the surface and half uniform in the bounding box,
+redrawing each until it passes the hull-group test the collection applies first
+(only such points can reach the exact query).
 
 A cull changes results only if it drops a node that would have been a contact.
 `attach(audit=True)` keeps every node, queries it exactly and counts culled
@@ -33,7 +34,6 @@ import time
 
 import numpy as np
 
-from .engine import f4
 
 IN, K1, H, OCTAVES = 39, 48, 64, 6
 OFF_W1 = 0
@@ -63,47 +63,18 @@ def initial_parameters(templates, rng):
     return master
 
 
-def surface_points(vertices, faces, count, rng):
-    """Area-uniform points on a triangle mesh and their face normals."""
-    tri = vertices[faces]
-    cross = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
-    area = 0.5 * np.linalg.norm(cross, axis=1)
-    face = rng.choice(len(faces), size=count, p=area / area.sum())
-    r1, r2 = rng.random(count), rng.random(count)
-    s = np.sqrt(r1)
-    a, b, c = 1 - s, s * (1 - r2), s * r2
-    t = tri[face]
-    points = a[:, None] * t[:, 0] + b[:, None] * t[:, 1] + c[:, None] * t[:, 2]
-    normals = cross[face] / (2 * area[face])[:, None]
-    return points, normals
-
-
-def candidate_points(vertices, faces, count, rng):
-    """Mixture around the surface: normal offsets at three widths plus the box."""
-    lo, hi = vertices.min(0), vertices.max(0)
-    centre = 0.5 * (lo + hi)
-    radius = float(np.linalg.norm(vertices - centre, axis=1).max())
-    parts = []
-    for fraction, width in ((0.35, 0.01), (0.3, 0.04), (0.15, 0.12)):
-        k = int(count * fraction)
-        p, n = surface_points(vertices, faces, k, rng)
-        parts.append(p + n * rng.normal(0, width * radius, (k, 1)))
-    k = count - sum(len(p) for p in parts)
-    pad = 0.1 * (hi - lo)
-    parts.append(rng.uniform(lo - pad, hi + pad, (k, 3)))
-    return np.concatenate(parts).astype(np.float32), centre, radius
-
-
 class LearnedCull:
     """Per-template SDF networks for an `Engine` on the hip backend.
 
     engine      a built Engine whose backend is `HipBackend`
-    points      training points per template (rounded up to a multiple of 64)
+    points      training points per template (a multiple of `batch`)
+    batch       points per template per Adam step (a multiple of 64)
     validation  held-out points per template for the margin
     clamp       label clamp as a fraction of the template bounding radius
+    factor      margin = factor x max validation error
     """
 
-    def __init__(self, engine, *, points=16384, validation=16384, clamp=0.2, seed=0, factor=3.0):
+    def __init__(self, engine, *, points=1 << 20, batch=16384, validation=16384, clamp=0.2, seed=0, factor=3.0):
         self.engine = engine
         self.m = engine.m
         if getattr(self.m, "name", None) != "hip":
@@ -111,7 +82,8 @@ class LearnedCull:
         self.torch = self.m.torch
         self.dev = self.m.torch_device
         self.templates = len(engine.shapes)
-        self.points = 64 * math.ceil(points / 64)
+        self.batch = 64 * math.ceil(min(batch, points) / 64)
+        self.points = self.batch * max(1, points // self.batch)
         self.validation = 64 * math.ceil(validation / 64)
         self.clamp = float(clamp)
         self.factor = float(factor)
@@ -141,14 +113,22 @@ class LearnedCull:
 
     # ---- data
     def _make_data(self):
+        """Sample and label every template's points on the GPU (`sad_hip_nn_sample`)."""
+        import ctypes as ct
+
+        from .engine import config_values
+        from .hip import _table
+
         e, torch = self.engine, self.torch
         bindings = e.bindings()
         unit = e.length_unit
-        need = {"train": self.points, "valid": self.validation}
-        out = {k: ([], []) for k in need}
-        self.centre = np.zeros((self.templates, 3), np.float32)
-        self.scale = np.zeros(self.templates, np.float32)
-        self.label_scale = np.zeros(self.templates, np.float32)
+        T = self.templates
+        self.centre = np.zeros((T, 3), np.float32)
+        self.scale = np.zeros(T, np.float32)
+        self.label_scale = np.zeros(T, np.float32)
+        tris, cdfs, starts, counts = [], [], [], []
+        box = np.zeros((T, 2, 4), np.float32)
+        first = 0
         for t, shape in enumerate(e.shapes):
             vertices = shape.vertices / unit
             lo, hi = vertices.min(0), vertices.max(0)
@@ -157,32 +137,45 @@ class LearnedCull:
             self.scale[t] = 1.0 / (0.55 * float((hi - lo).max()))
             radius = float(np.linalg.norm(vertices - centre, axis=1).max())
             self.label_scale[t] = self.clamp * radius
-            kept = []
-            total = self.points + self.validation
-            while sum(len(k) for k in kept) < total:
-                cand, _, _ = candidate_points(vertices, shape.faces, 2 * total, self.rng)
-                src = torch.from_numpy(f4(cand)).to(self.dev)
-                flags = torch.zeros(len(cand), dtype=torch.int32, device=self.dev)
-                self.m.probe_device("meshCullProbe", [len(cand), t], bindings, src, flags)
-                kept.append(cand[flags.cpu().numpy() != 0])
-            pts = np.concatenate(kept)[:total]
-            src = torch.from_numpy(f4(pts)).to(self.dev)
-            dist = torch.zeros((total, 4), dtype=torch.float32, device=self.dev)
-            self.m.probe_device("meshProbe", [total, t], bindings, src, dist)
-            sdf = dist[:, 3].cpu().numpy()
-            label = np.clip(sdf / self.label_scale[t], -1.0, 1.0).astype(np.float32)
-            out["train"][0].append(pts[: self.points])
-            out["train"][1].append(label[: self.points])
-            out["valid"][0].append(pts[self.points:])
-            out["valid"][1].append(label[self.points:])
-        T = self.templates
-        self.train_pts = torch.from_numpy(f4(np.concatenate(out["train"][0]))).to(self.dev)
-        self.train_labels = torch.from_numpy(np.concatenate(out["train"][1])).to(self.dev)
-        self.valid_pts = torch.from_numpy(f4(np.concatenate(out["valid"][0]))).to(self.dev)
-        self.valid_labels = np.concatenate(out["valid"][1]).reshape(T, -1)
-        self.tstart = torch.from_numpy(np.arange(T, dtype=np.uint32) * self.points).to(self.dev)
-        self.tcount = torch.from_numpy(np.full(T, self.points, np.uint32)).to(self.dev)
-        self.valid_tiles = torch.from_numpy(np.repeat(np.arange(T, dtype=np.uint32), self.validation // 64)).to(self.dev)
+            tri = vertices[shape.faces]
+            area = 0.5 * np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=1)
+            cdf = np.cumsum(area) / area.sum()
+            cdf[-1] = 1.0
+            tris.append(np.pad(tri, ((0, 0), (0, 0), (0, 1))).reshape(-1, 4))
+            cdfs.append(cdf)
+            starts.append(first)
+            counts.append(len(tri))
+            first += len(tri)
+            box[t, 0, :3], box[t, 0, 3] = lo, shape.equivalent_diameter / unit
+            box[t, 1, :3], box[t, 1, 3] = hi, self.label_scale[t]
+        dev = self.dev
+        tri_d = torch.from_numpy(np.concatenate(tris).astype(np.float32)).to(dev)
+        cdf_d = torch.from_numpy(np.concatenate(cdfs).astype(np.float32)).to(dev)
+        start_d = torch.from_numpy(np.array(starts, np.uint32).view(np.int32)).to(dev)
+        count_d = torch.from_numpy(np.array(counts, np.uint32).view(np.int32)).to(dev)
+        box_d = torch.from_numpy(box).to(dev)
+        per = self.points + self.validation
+        pts = torch.zeros((T * per, 4), dtype=torch.float32, device=dev)
+        labels = torch.zeros(T * per, dtype=torch.float32, device=dev)
+        seed = int(self.rng.integers(0, 2**63))
+        u = (ct.c_uint32 * 4)(T, per, (seed >> 32) & 0xFFFFFFFF, seed & 0xFFFFFFFF)
+        ptrs, sizes = _table(bindings)
+        values = config_values(bindings["c"])
+        self._check(self.m.lib.sad_hip_nn_sample(
+            u, ptrs, sizes, values.ctypes.data_as(ct.POINTER(ct.c_double)), self._ptr(tri_d), self._ptr(cdf_d),
+            self._ptr(start_d), self._ptr(count_d), self._ptr(box_d), self._ptr(pts), self._ptr(labels),
+            self.m.stream), "nn_sample")
+        self.train_pts, self.train_labels = pts, labels  # training rows: [t * per, t * per + points)
+        self.valid_pts = pts.view(T, per, 4)[:, self.points:].reshape(-1, 4).contiguous()
+        self.valid_labels = labels.view(T, per)[:, self.points:].cpu().numpy()
+        batches = self.points // self.batch
+        rows = (np.arange(T, dtype=np.int64)[None, :] * per + np.arange(batches, dtype=np.int64)[:, None] * self.batch)
+        if rows.max() + self.batch > 2**32:
+            raise ValueError("training set exceeds 32-bit point indices")
+        self.tstarts = torch.from_numpy(rows.astype(np.uint32).view(np.int32)).to(dev)
+        self.tcount = torch.from_numpy(np.full(T, self.batch, np.uint32).view(np.int32)).to(dev)
+        self.valid_tiles = torch.from_numpy(
+            np.repeat(np.arange(T, dtype=np.uint32), self.validation // 64).view(np.int32)).to(dev)
 
     # ---- native calls
     def _ptr(self, x):
@@ -197,13 +190,16 @@ class LearnedCull:
         self._check(self.m.lib.sad_hip_nn_pack(self._ptr(self.master), self._ptr(self.weights), self._ptr(self.blocks),
                                                self.templates, self.m.stream), "nn_pack")
 
-    def gradient(self):
-        """Mean-squared-error gradient of every template on its training set (into `self.grad`)."""
+    def _grad(self, batch_index):
         self.loss.zero_()
         self._check(self.m.lib.sad_hip_nn_grad(
-            self._ptr(self.train_pts), self._ptr(self.train_labels), self._ptr(self.tstart), self._ptr(self.tcount),
-            self.templates, self.slices, self._ptr(self.weights), self._ptr(self.blocks), self._ptr(self.partial),
-            self._ptr(self.loss), 1.0 / self.points, self.m.stream), "nn_grad")
+            self._ptr(self.train_pts), self._ptr(self.train_labels), self._ptr(self.tstarts[batch_index]),
+            self._ptr(self.tcount), self.templates, self.slices, self._ptr(self.weights), self._ptr(self.blocks),
+            self._ptr(self.partial), self._ptr(self.loss), 1.0 / self.batch, self.m.stream), "nn_grad")
+
+    def gradient(self, batch_index=0):
+        """Mean-squared-error gradient of every template on one training batch (into `self.grad`)."""
+        self._grad(batch_index)
         self._adam(0.0)
         return self.grad
 
@@ -214,27 +210,24 @@ class LearnedCull:
             self.m.stream), "nn_adam")
 
     def train(self, iterations=2000, lr=2e-3, lr_final=5e-5, log_every=250):
-        """Full-batch Adam with cosine decay; returns per-template final training MSE."""
+        """Minibatch Adam with cosine decay; returns per-template MSE of the last batch."""
         torch = self.torch
         torch.cuda.synchronize(self.dev)
         t0 = time.perf_counter()
+        batches = self.tstarts.shape[0]
         for it in range(iterations):
-            self.loss.zero_()
-            self._check(self.m.lib.sad_hip_nn_grad(
-                self._ptr(self.train_pts), self._ptr(self.train_labels), self._ptr(self.tstart),
-                self._ptr(self.tcount), self.templates, self.slices, self._ptr(self.weights), self._ptr(self.blocks),
-                self._ptr(self.partial), self._ptr(self.loss), 1.0 / self.points, self.m.stream), "nn_grad")
+            self._grad(it % batches)
             self.step += 1
             rate = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * it / max(1, iterations - 1)))
             self._adam(rate)
             self._pack()
             if log_every and (it % log_every == 0 or it == iterations - 1):
-                mse = (self.loss / self.points).cpu().numpy()
+                mse = (self.loss / self.batch).cpu().numpy()
                 self.history.append((it, float(mse.mean()), float(mse.max())))
         torch.cuda.synchronize(self.dev)
         self.timings["train_s"] = time.perf_counter() - t0
         self.timings["iterations"] = iterations
-        return (self.loss / self.points).cpu().numpy()
+        return (self.loss / self.batch).cpu().numpy()
 
     def predict(self, points_tensor, tile_template):
         """Network outputs for 64-point tiles (device tensors)."""
@@ -250,15 +243,21 @@ class LearnedCull:
         err = np.abs(pred - self.valid_labels)
         self.max_error = err.max(1)
         self.margin = (self.factor * self.max_error).astype(np.float32)
-        self.blocks[:, FMARGIN] = self.torch.from_numpy(self.margin).to(self.dev)
+        # A margin at or above the label clamp can never cull (targets saturate at 1):
+        # such templates skip the network in the collection stage.
+        self.enabled = self.margin < 1.0
+        device_margin = np.where(self.enabled, self.margin, np.float32(np.inf)).astype(np.float32)
+        self.blocks[:, FMARGIN] = self.torch.from_numpy(device_margin).to(self.dev)
         inside = self.valid_labels <= 0
-        would_cull = pred > self.margin[:, None]
+        would_cull = pred > np.where(self.margin < 1.0, self.margin, np.inf)[:, None]
         self.validation_report = {
             "max_error_mean": float(self.max_error.mean()),
             "max_error_max": float(self.max_error.max()),
             "margin_mean": float(self.margin.mean()),
             "cull_fraction_outside": float(would_cull[~inside].mean()) if (~inside).any() else 0.0,
             "validation_misses": int((would_cull & inside).sum()),
+            "templates_enabled": int(self.enabled.sum()),
+            "templates": int(self.templates),
         }
         return self.margin
 
@@ -267,7 +266,11 @@ class LearnedCull:
         if not hasattr(self, "margin"):
             raise RuntimeError("calibrate() before attach()")
         self.torch.cuda.synchronize(self.dev)
+        if not self.enabled.any():  # nothing can be culled: keep the plain collection kernel
+            self.m.configure_cull(False)
+            return False
         self.m.configure_cull(True, audit, self.weights, self.blocks, self.templates)
+        return True
 
     def detach(self):
         self.torch.cuda.synchronize(self.dev)

@@ -62,12 +62,14 @@ def library():
             "sad_hip_profile": (ct.c_int, [vp, ct.c_int]),
             "sad_hip_profile_read": (ct.c_int, [vp, dp, u64p, u32]),
             "sad_hip_kernel_name": (ct.c_char_p, [u32]),
+            "sad_hip_tune": (ct.c_int, [vp, ct.c_char_p, ct.c_int64, ct.POINTER(ct.c_int64)]),
             "sad_hip_nn_configure": (ct.c_int, [vp, ct.c_int, ct.c_int, vp, vp, u32]),
             "sad_hip_nn_grad": (ct.c_int, [vp, vp, vp, vp, u32, u32, vp, vp, vp, vp, ct.c_float, vp]),
             "sad_hip_nn_adam": (ct.c_int, [vp, vp, vp, vp, vp, u32, u32, ct.c_float, ct.c_float, ct.c_float,
                                            ct.c_float, u32, vp]),
             "sad_hip_nn_pack": (ct.c_int, [vp, vp, vp, u32, vp]),
             "sad_hip_nn_infer": (ct.c_int, [vp, vp, u32, vp, vp, vp, vp]),
+            "sad_hip_nn_sample": (ct.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
             "sad_hip_mfma_probe": (ct.c_int, [vp, vp, vp, vp]),
             "sad_hip_last_error": (ct.c_char_p, []),
             "sad_hip_abi_version": (ct.c_int, []),
@@ -89,11 +91,13 @@ EXPORTED = (
     "sad_hip_profile",
     "sad_hip_profile_read",
     "sad_hip_kernel_name",
+    "sad_hip_tune",
     "sad_hip_nn_configure",
     "sad_hip_nn_grad",
     "sad_hip_nn_adam",
     "sad_hip_nn_pack",
     "sad_hip_nn_infer",
+    "sad_hip_nn_sample",
     "sad_hip_mfma_probe",
     "sad_hip_last_error",
     "sad_hip_abi_version",

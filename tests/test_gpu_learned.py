@@ -135,3 +135,70 @@ def test_gradient_matches_oracle(hip):
         for a, b in blocks:
             rel = np.linalg.norm(g[t, a:b] - want[a:b]) / max(np.linalg.norm(want[a:b]), 1e-30)
             assert rel < 1e-2, (t, a, rel)
+
+
+def test_sampler_points_pass_hull_test_and_labels_are_exact_distances(pkg, hip):
+    """Every sampled point passes the hull-group test (64 redraws suffice here) and
+    its label is the exact query's signed distance / label scale, clamped."""
+    from sand_atlas_dem_amd import assembly, learned
+
+    w = assembly.config_workload("concave", n=200, templates=2, nodes=256)
+    e = w.engine(hip)
+    lc = learned.LearnedCull(e, points=4096, batch=1024, validation=1024, seed=3)
+    T, per = lc.templates, lc.points + lc.validation
+    pts = lc.train_pts.cpu().numpy().reshape(T, per, 4)
+    labels = lc.train_labels.cpu().numpy().reshape(T, per)
+    b = e.bindings()
+    for t in range(T):
+        inside = hip.probe("meshCullProbe", [per, t], b, pts[t], np.zeros(per, np.uint32))
+        assert inside.mean() > 0.999
+        d = hip.probe("meshProbe", [per, t], b, pts[t], np.zeros((per, 4), np.float32))[:, 3]
+        want = np.clip(d / lc.label_scale[t], -1, 1).astype(np.float32)
+        assert np.array_equal(labels[t], want)
+        near = np.abs(d[0::2]) < 1.0  # even points: within one grain diameter of the surface
+        assert near.mean() > 0.99
+        assert (labels[t] < 0).mean() > 0.1 and (labels[t] > 0).mean() > 0.1
+
+
+def test_learned_cull_gain_on_concave_grains(pkg):
+    """On grains whose hull bounds are loose the trained network removes exact
+    queries; the audit finds no culled contact and the culled run is identical to
+    the plain run (contacts, forces, torques, poses)."""
+    from sand_atlas_dem_amd import assembly, learned
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    m = HipBackend(0)
+    w = assembly.config_workload("concave", n=400, templates=4, nodes=512)
+    e = w.engine(m)
+    e.initialize()
+    lc = learned.LearnedCull(e, points=1 << 20, batch=16384, validation=16384, seed=1)
+    lc.train(iterations=4000, lr=3e-3, log_every=0)
+    lc.calibrate()
+    assert lc.validation_report["validation_misses"] == 0
+    state, step = e.state(), e.step
+
+    def run(mode):
+        e.restore(state, step)
+        if mode == "plain":
+            lc.detach()
+        else:
+            assert lc.attach(audit=(mode == "audit"))
+        before = m.stats()
+        e.advance(3, rate=w.rate)
+        after = m.stats()
+        lc.detach()
+        queried = after["queried_nodes"] - before["queried_nodes"]
+        return e.state(), queried, after
+
+    plain, queried, _ = run("plain")
+    audit, _, s = run("audit")
+    assert s["nn_audited"] == s["nn_culled"] > 0
+    assert s["nn_misses"] == 0
+    culled, queued, s = run("culled")
+    exact = queued - s["nn_culled"]
+    assert queued == queried
+    assert exact <= 0.9 * queried, (exact, queried)
+    for other in (audit, culled):
+        assert other["contacts"] == plain["contacts"]
+        for name in ("meta", "force", "torque", "position", "rotation", "slip", "normalArea"):
+            assert np.array_equal(other[name], plain[name]), name

@@ -1,0 +1,91 @@
+"""Runtime paths that must not change results: hipGraph-captured steps, every
+collection launch shape the tuner may pick, and the tuner itself."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("position", "positionRemainder", "velocity", "velocityRemainder", "rotation", "rotationRemainder",
+         "momentum", "force", "torque", "accumulated", "cell", "slip", "normalArea", "meta")
+
+
+@pytest.fixture(scope="module")
+def dense(pkg):
+    from sand_atlas_dem_amd import assembly
+
+    return assembly.config_workload("dense", n=300, templates=8)
+
+
+def _equal(a, b):
+    assert a["contacts"] == b["contacts"]
+    for name in STATE:
+        assert np.array_equal(a[name], b[name], equal_nan=True), name
+
+
+def _run(workload, backend, chunks=(2, 3)):
+    e = workload.engine(backend)
+    e.initialize()
+    for k in chunks:
+        e.advance(k, rate=workload.rate)
+    return e
+
+
+def test_graph_captured_steps_match_eager(dense):
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    eager = _run(dense, HipBackend(0, graph=False)).state()
+    g = HipBackend(0, graph=True)
+    captured = _run(dense, g, chunks=(2, 3, 1))
+    ref = _run(dense, HipBackend(0, graph=False), chunks=(2, 3, 1)).state()
+    _equal(captured.state(), ref)
+    # replays after eager work on the same stream stay exact
+    captured.advance(2, rate=dense.rate)
+    ref_e = _run(dense, HipBackend(0, graph=False), chunks=(2, 3, 1, 2)).state()
+    _equal(captured.state(), ref_e)
+    assert eager["contacts"] > 0
+
+
+@pytest.mark.parametrize("setting", [
+    {"collect_blocks_per_cu": 2, "collect_pop": 4, "collect_queue": 128},
+    {"collect_blocks_per_cu": 8, "collect_pop": 2, "collect_queue": 256},
+])
+def test_launch_shapes_do_not_change_results(dense, setting):
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    ref = _run(dense, HipBackend(0)).state()
+    m = HipBackend(0)
+    for name, value in setting.items():
+        m.tune(name, value)
+    assert {k: m.tune(k) for k in setting} == setting
+    _equal(_run(dense, m).state(), ref)
+
+
+def test_tune_rejects_bad_values(pkg):
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    m = HipBackend(0)
+    for name, value in (("collect_queue", 64), ("collect_pop", 0), ("collect_blocks_per_cu", 99), ("nope", 1)):
+        with pytest.raises(RuntimeError):
+            m.tune(name, value)
+
+
+def test_autotune_restores_state_and_applies_best(dense, tmp_path):
+    from sand_atlas_dem_amd import autotune
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    m = HipBackend(0, graph=True)
+    e = _run(dense, m)
+    before, step = e.state(), e.step
+    space = {"collect_blocks_per_cu": (2, 6), "collect_queue": (128, 256)}
+    result = autotune.autotune(e, rate=dense.rate, steps=1, space=space, cache=tmp_path / "tune.json")
+    assert not result["cached"] and len(result["trials"]) == 3
+    assert {k: m.tune(k) for k in space} == result["best"]
+    # the engine is left as restore(before) leaves it (forces recomputed at dt = 0,
+    # gpu.py:1016-1092), whatever the trials did
+    ref = dense.engine(HipBackend(0))
+    ref.restore(before, step)
+    _equal(e.state(), ref.state())
+    assert e.step == step
+    again = autotune.autotune(e, rate=dense.rate, steps=1, space=space, cache=tmp_path / "tune.json")
+    assert again["cached"] and again["best"] == result["best"]
