@@ -123,6 +123,13 @@ int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buf
                   const uint64_t* bytes, const double* values, const void* input,
                   void* output, void* stream);
 
+/* The reference Engine's hierarchical scan (scan.slang, driven by the Scan class
+ * at gpu.py:65-91): "blockScan" writes the exclusive scan of each 256-element
+ * block of input[0..n) to output and the block totals to aux; "addBlocks" adds
+ * aux[id / 256] to output[id].  Unsigned wrap-around, as the GPU kernels. */
+int sad_hip_scan_kernel(const char* kernel, uint32_t n, const uint32_t* input,
+                        uint32_t* output, uint32_t* aux, void* stream);
+
 /* Launch-shape tunables of the collection stage (no reference counterpart; none
  * changes any result): "collect_blocks_per_cu" (1..32 workgroups per CU in the
  * persistent launch), "collect_pop" (1..16 tasks taken per queue atomic),

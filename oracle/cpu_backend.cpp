@@ -9,8 +9,8 @@
 // pair-order collector cpuCollectMesh, non-atomic atomics from
 // shaders/cpu_scheduling.slang:3-99).  FP32 arithmetic follows the generated C++:
 // dot() accumulates from 0.0f left to right, length = sqrtf(dot), normalize =
-// x * (1/sqrtf(dot)), min/max = fminf/fmaxf, transcendentals from libm
-// (evidence/.../instrumented-collector.cpp:869-1027).  Build with
+// x * (1/sqrtf(dot)), min/max = fminf/fmaxf, sinf/cosf/expf as the correctly rounded
+// FP32 value (below; evidence/.../instrumented-collector.cpp:869-1027).  Build with
 // -O3 -fno-fast-math -ffp-contract=off (cpu.py:323-333).
 //
 // Exported C ABI (binding order of include/sand_dem_hip.h, same as the

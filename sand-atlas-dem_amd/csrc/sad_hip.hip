@@ -1280,6 +1280,33 @@ __global__ __launch_bounds__(256) void k_gather(P p, W w) {
     accumulateRow(p, id, {dissipation, 0, 0, 0});
 }
 
+// ---------------------------------------------------------------- scan.slang (the reference Engine's Scan)
+// blockScan: exclusive uint scan of each 256-element block, block totals out
+// (scan.slang:1-21, unsigned wrap-around as on the GPU); addBlocks: add the scanned
+// block totals (scan.slang:22-28).  Wave64 prefix via DPP-free shuffles, then the
+// four wave totals through LDS.
+__global__ __launch_bounds__(256) void k_block_scan(uint32_t n, const uint32_t* in, uint32_t* out, uint32_t* totals) {
+    __shared__ uint32_t waves[4];
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t v = id < n ? in[id] : 0u;
+    uint32_t x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d);
+        if (lane >= uint32_t(d)) x += y;
+    }
+    if (lane == 63) waves[wv] = x;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < wv; k++) base += waves[k];
+    x += base;
+    if (id < n) out[id] = x - v;
+    if (threadIdx.x == 255) totals[blockIdx.x] = x;
+}
+__global__ void k_add_blocks(uint32_t n, uint32_t* out, const uint32_t* offsets) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) out[id] += offsets[id / 256];
+}
+
 // ---------------------------------------------------------------- learned-cull training data
 // Counter-based uniforms (splitmix64 finaliser): reproducible per (seed, template, point, draw).
 __device__ __forceinline__ uint64_t nn_mix(uint64_t z) {
@@ -1722,6 +1749,16 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
     else if (name == "captureContactKinematics") LAUNCH(sc, KID_capture_kinematics, k_capture_kinematics, perGrain, T, s, p);
     else if (name == "checkContactTimestep") LAUNCH(sc, KID_check_timestep, k_check_timestep, perGrain, T, s, p);
     else {
+        // Reference entry points outside the mesh path (walls, SDF grid, diagnostics,
+        // the serial verification collector) are known but not provided here.
+        static const char* unsupported[] = {"advanceWalls", "collectWallContacts", "gatherWallRows",
+                                            "refreshWallPower", "countContacts", "emitContacts", "contactTotal",
+                                            "collectMeshContacts", "diagnosticRows", "fabricRows", "smoke"};
+        for (const char* u : unsupported)
+            if (name == u) {
+                std::snprintf(g_error, sizeof(g_error), "kernel %s is not part of the hip mesh path", u);
+                return SAD_E_UNSUPPORTED;
+            }
         std::snprintf(g_error, sizeof(g_error), "unknown kernel %s", name.c_str());
         return SAD_E_ARGUMENT;
     }
@@ -2057,6 +2094,24 @@ int sad_hip_nn_infer(const void* points, const uint32_t* tile_template, uint32_t
                        out);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SAD_OK : hip_fail(e, "nn_infer");
+}
+
+int sad_hip_scan_kernel(const char* kernel, uint32_t n, const uint32_t* input, uint32_t* output, uint32_t* aux,
+                        void* stream) {
+    if (!kernel || !output || !aux) return SAD_E_ARGUMENT;
+    if (n == 0) return SAD_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (std::strcmp(kernel, "blockScan") == 0) {
+        if (!input) return SAD_E_ARGUMENT;
+        hipLaunchKernelGGL(k_block_scan, dim3(blocks(n)), dim3(256), 0, s, n, input, output, aux);
+    } else if (std::strcmp(kernel, "addBlocks") == 0) {
+        hipLaunchKernelGGL(k_add_blocks, dim3(blocks(n)), dim3(256), 0, s, n, output, aux);
+    } else {
+        std::snprintf(g_error, sizeof(g_error), "unknown scan kernel %s", kernel);
+        return SAD_E_ARGUMENT;
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAD_OK : hip_fail(e, kernel);
 }
 
 int sad_hip_nn_sample(const uint32_t* uniforms, void* const* buffers, const uint64_t* bytes, const double* values,

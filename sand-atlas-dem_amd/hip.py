@@ -72,6 +72,43 @@ def _table(bindings):
     return ptrs, sizes
 
 
+class Kernel:
+    """`m.kernel(name, module)` of the adapter protocol (gpu.py:75-90 Scan, 529-530).
+
+    solver.slang kernels go through the library's dispatch table; scan.slang's
+    blockScan/addBlocks through `sad_hip_scan_kernel`.  diagnostic.slang reductions
+    are not part of this backend.
+    """
+
+    def __init__(self, backend, name, module):
+        if module not in {"solver.slang", "scan.slang"}:
+            raise NotImplementedError(f"{module} kernels are not provided by the hip backend")
+        self.m, self.name, self.module = backend, name, module
+
+    def dispatch(self, *, thread_count, command_encoder=None, vars=None, **kwargs):
+        if self.module == "solver.slang":
+            r = (vars or {}).get("r", kwargs.get("r"))
+            return self.m.dispatch(self.name, int(thread_count[0]), command_encoder, r=r)
+        n = int(kwargs["N"])
+        if self.name == "blockScan":
+            args = (kwargs["input"], kwargs["output"], kwargs["totals"])
+        elif self.name == "addBlocks":
+            args = (None, kwargs["output"], kwargs["offsets"])
+        else:
+            raise NotImplementedError(f"scan kernel {self.name}")
+
+        def execute():
+            src, dst, aux = args
+            self.m._check(self.m.lib.sad_hip_scan_kernel(
+                self.name.encode(), n, None if src is None else ct.c_void_p(src.ptr), ct.c_void_p(dst.ptr),
+                ct.c_void_p(aux.ptr), self.m.stream), self.name)
+
+        if command_encoder is None:
+            execute()
+        else:
+            command_encoder.operations.append(execute)
+
+
 class HipBackend:
     name = "hip"
 
@@ -86,6 +123,7 @@ class HipBackend:
             raise MemoryError("HIP scratch creation failed")
         self.graph = graph
         self.buffer_type = Buffer
+        self.kernels = {}
 
     def __del__(self):
         lib, scratch = getattr(self, "lib", None), getattr(self, "scratch", None)
@@ -103,6 +141,12 @@ class HipBackend:
 
     def create_command_encoder(self):
         return Encoder()
+
+    def kernel(self, name, module="solver.slang"):
+        key = (module, name)
+        if key not in self.kernels:
+            self.kernels[key] = Kernel(self, name, module)
+        return self.kernels[key]
 
     @property
     def stream(self):

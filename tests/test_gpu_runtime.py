@@ -89,3 +89,39 @@ def test_autotune_restores_state_and_applies_best(dense, tmp_path):
     assert e.step == step
     again = autotune.autotune(e, rate=dense.rate, steps=1, space=space, cache=tmp_path / "tune.json")
     assert again["cached"] and again["best"] == result["best"]
+
+
+class _Scan:
+    """The reference Engine's hierarchical scan driver (gpu.py:65-91), restated."""
+
+    def __init__(self, m, n):
+        self.m, self.n = m, n
+        blocks = (n + 255) // 256
+        self.totals = m.buffer(count=blocks)
+        self.offsets = m.buffer(count=blocks)
+        self.child = _Scan(m, blocks) if blocks > 1 else None
+
+    def encode(self, e, source, dest):
+        self.m.kernel("blockScan", "scan.slang").dispatch(
+            thread_count=[((self.n + 255) // 256) * 256, 1, 1], command_encoder=e, N=self.n, input=source,
+            output=dest, totals=self.totals)
+        if self.child:
+            self.child.encode(e, self.totals, self.offsets)
+            self.m.kernel("addBlocks", "scan.slang").dispatch(
+                thread_count=[self.n, 1, 1], command_encoder=e, N=self.n, output=dest, offsets=self.offsets)
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 257, 65536, 70001, 1 << 20])
+def test_reference_scan_kernels_through_the_adapter(n):
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    m = HipBackend(0)
+    rng = np.random.default_rng(n)
+    values = rng.integers(0, 1000, n).astype(np.uint32)
+    src, dst = m.buffer(values), m.buffer(count=n)
+    e = m.create_command_encoder()
+    _Scan(m, n).encode(e, src, dst)
+    m.submit(e)
+    m.wait()
+    want = (np.cumsum(values.astype(np.uint64)) - values).astype(np.uint32)
+    assert np.array_equal(m.read(dst, np.uint32)[:n], want)
