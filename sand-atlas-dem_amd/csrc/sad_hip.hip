@@ -40,6 +40,7 @@ struct Cfg {
 struct P {
     Cfg c;
     uint32_t meshNodes;  // BVH nodes over all templates (rows of meshLinks)
+    uint32_t templates;  // rows of meshRoots
     uint32_t* control;
     float4 *cell, *position, *positionRemainder, *velocityRemainder, *momentumRemainder;
     float4 *rotationRemainder, *referencePositionRemainder, *cellLog, *accumulatedRemainder;
@@ -89,7 +90,10 @@ struct W {
     float4* wide;         // 4 rows per BVH node: both child boxes + child references
     uint32_t* xcd;        // per queue q: count at [q*LINE], fill cursor at [(8+q)*LINE], pop cursor at [(16+q)*LINE]
     uint32_t* stageCount; // staged hits per queue at [q*LINE]; queue q stages into [q*stageCap, (q+1)*stageCap)
-    uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue
+    uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue, ascending target template within a queue
+    uint32_t* tmplCount;  // [MAX_TEMPLATES] tasks per target template
+    uint32_t* tmplStart;  // first slot of each template's tasks in xcdTasks
+    uint32_t* tmplCursor; // fill cursor per template
     // Learned cull (nn.h): per-template BF16 weights and FP32 blocks; stats on separate lines.
     uint32_t nnEnabled, nnAudit, nnTemplates;
     const uint16_t* nnW;
@@ -700,44 +704,58 @@ __global__ void k_prepare_pairs(P p) {  // solver.slang:469-489 (Metal frontier 
     p.meshPairMask[id] = mask;
 }
 
-// Task queues per XCD, keyed by the target template (the BVH a task reads).
-__global__ void k_bucket_reset(W w) {
+// Task queues per XCD, keyed by the target template (the BVH a task reads): a
+// counting sort by template, queue q = template & 7 holding its templates in
+// ascending order, so the waves of one XCD sweep through one template's BVH at a
+// time and its L2 holds the working set.  Each block owns one contiguous chunk of
+// tasks; LDS counts are reserved with one global atomic per (block, template).
+constexpr uint32_t MAX_TEMPLATES = 4096;
+__device__ __forceinline__ uint32_t taskTemplate(const P& p, uint32_t task) {
+    uint2 ij = p.pairs[task >> 1];
+    return p.shapeId[(task & 1) ? ij.x : ij.y];
+}
+__global__ void k_bucket_reset(P p, W w) {
+    for (uint32_t t = threadIdx.x; t < p.templates; t += blockDim.x) { w.tmplCount[t] = 0; w.tmplCursor[t] = 0; }
     if (threadIdx.x < 24) w.xcd[threadIdx.x * LINE] = 0;
 }
-__device__ __forceinline__ uint32_t taskQueue(const P& p, uint32_t task) {
-    uint2 ij = p.pairs[task >> 1];
-    uint32_t target = (task & 1) ? ij.x : ij.y;
-    return p.shapeId[target] & 7u;
+__device__ __forceinline__ void chunkOf(uint32_t tasks, uint32_t& lo, uint32_t& hi) {
+    uint32_t per = (tasks + gridDim.x - 1) / gridDim.x;
+    lo = min(tasks, blockIdx.x * per);
+    hi = min(tasks, lo + per);
 }
-// Block-aggregated: LDS counts, then one global atomic per queue per block.
-__global__ void k_bucket_count(P p, W w) {
-    __shared__ uint32_t local[8];
-    if (threadIdx.x < 8) local[threadIdx.x] = 0;
+__global__ __launch_bounds__(256) void k_bucket_count(P p, W w) {
+    __shared__ uint32_t local[MAX_TEMPLATES];
+    for (uint32_t t = threadIdx.x; t < p.templates; t += blockDim.x) local[t] = 0;
     __syncthreads();
-    const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks; t += gridDim.x * blockDim.x)
-        atomicAdd(local + taskQueue(p, t), 1u);
+    uint32_t lo, hi;
+    chunkOf(2 * min(p.control[1], p.c.pairCapacity), lo, hi);
+    for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) atomicAdd(local + taskTemplate(p, k), 1u);
     __syncthreads();
-    if (threadIdx.x < 8 && local[threadIdx.x]) atomicAdd(w.xcd + threadIdx.x * LINE, local[threadIdx.x]);
+    for (uint32_t t = threadIdx.x; t < p.templates; t += blockDim.x)
+        if (local[t]) atomicAdd(w.tmplCount + t, local[t]);
 }
-__global__ void k_bucket_fill(P p, W w) {
-    __shared__ uint32_t local[8], base[8], cursor[8];
-    uint32_t off[8], acc = 0;
-    for (int k = 0; k < 8; k++) { off[k] = acc; acc += w.xcd[k * LINE]; }
-    const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t start = blockIdx.x * blockDim.x; start < tasks; start += stride) {
-        if (threadIdx.x < 8) { local[threadIdx.x] = 0; cursor[threadIdx.x] = 0; }
-        __syncthreads();
-        uint32_t t = start + threadIdx.x;
-        uint32_t qi = t < tasks ? taskQueue(p, t) : 8u;
-        if (qi < 8) atomicAdd(local + qi, 1u);
-        __syncthreads();
-        if (threadIdx.x < 8) base[threadIdx.x] = local[threadIdx.x] ? atomicAdd(w.xcd + (8 + threadIdx.x) * LINE, local[threadIdx.x]) : 0;
-        __syncthreads();
-        if (qi < 8) w.xcdTasks[off[qi] + base[qi] + atomicAdd(cursor + qi, 1u)] = t;
-        __syncthreads();
+// Queue totals at w.xcd[q * LINE]; absolute template starts in queue order.
+__global__ void k_bucket_offsets(P p, W w) {
+    if (threadIdx.x != 0) return;
+    uint32_t base = 0;
+    for (uint32_t q = 0; q < 8; q++) {
+        uint32_t start = base;
+        for (uint32_t t = q; t < p.templates; t += 8) { w.tmplStart[t] = base; base += w.tmplCount[t]; }
+        w.xcd[q * LINE] = base - start;
     }
+}
+__global__ __launch_bounds__(256) void k_bucket_fill(P p, W w) {
+    __shared__ uint32_t local[MAX_TEMPLATES];
+    for (uint32_t t = threadIdx.x; t < p.templates; t += blockDim.x) local[t] = 0;
+    __syncthreads();
+    uint32_t lo, hi;
+    chunkOf(2 * min(p.control[1], p.c.pairCapacity), lo, hi);
+    for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) atomicAdd(local + taskTemplate(p, k), 1u);
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < p.templates; t += blockDim.x)
+        if (local[t]) local[t] = w.tmplStart[t] + atomicAdd(w.tmplCursor + t, local[t]);
+    __syncthreads();
+    for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) w.xcdTasks[atomicAdd(local + taskTemplate(p, k), 1u)] = k;
 }
 
 struct TaskFrame {
@@ -1416,7 +1434,7 @@ thread_local char g_error[256] = "no error";
     X(count_pairs) X(scan_reduce) X(scan_sums) X(scan_apply) X(emit_pairs) X(adj_clear) X(adj_count) X(adj_fill) \
     X(adj_sort) X(reset_mesh) X(prepare_pairs) X(collect) X(contact_total) X(scatter) X(clear_storage) X(resolve) \
     X(build_hash) X(gather) X(finish_step) X(capture_kinematics) X(check_timestep) \
-    X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_fill)
+    X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_offsets) X(bucket_fill)
 enum KernelId {
 #define SAD_ID(name) KID_##name,
     SAD_KERNELS(SAD_ID)
@@ -1469,7 +1487,7 @@ int grow(T*& ptr, uint32_t& cap, size_t need, size_t& bytes) {
 struct Caps {
     uint32_t counters = 0, taskCount = 0, taskOffset = 0, adjCount = 0, adjOffset = 0, adjCursor = 0, adjList = 0;
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
-    uint32_t stageCount = 0;
+    uint32_t stageCount = 0, tmplCount = 0, tmplStart = 0, tmplCursor = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -1511,6 +1529,9 @@ struct ScratchImpl : Scratch {
         if ((rc = grow(w.stageNa, caps.stageNa, size_t(8) * c.contactCapacity, bytes))) return rc;
         if ((rc = grow(w.xcd, caps.xcd, 24 * LINE, bytes))) return rc;
         if ((rc = grow(w.xcdTasks, caps.xcdTasks, tasks, bytes))) return rc;
+        if ((rc = grow(w.tmplCount, caps.tmplCount, MAX_TEMPLATES, bytes))) return rc;
+        if ((rc = grow(w.tmplStart, caps.tmplStart, MAX_TEMPLATES, bytes))) return rc;
+        if ((rc = grow(w.tmplCursor, caps.tmplCursor, MAX_TEMPLATES, bytes))) return rc;
         w.stageCap = c.contactCapacity;
         w.taskCap = uint32_t(tasks);
         w.nCap = c.N;
@@ -1530,6 +1551,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.counters); (void)hipFree(w.taskCount); (void)hipFree(w.taskOffset); (void)hipFree(w.adjCount);
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
+        (void)hipFree(w.tmplCount); (void)hipFree(w.tmplStart); (void)hipFree(w.tmplCursor);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
 };
@@ -1598,6 +1620,7 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
         }
     }
     p.meshNodes = uint32_t(bytes[SAD_MESH_LINKS] / 16);
+    p.templates = uint32_t(bytes[SAD_MESH_ROOTS] / 16);
     p.control = slot<uint32_t>(b, SAD_CONTROL);
     p.cell = slot<float4>(b, SAD_CELL);
     p.position = slot<float4>(b, SAD_POSITION);
@@ -1721,9 +1744,15 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         int rc = sc.ensure_nodes(p.meshNodes);
         if (rc) return rc;
         LAUNCH(sc, KID_wide_nodes, k_wide_nodes, dim3(blocks(p.meshNodes)), T, s, p, w, p.meshNodes);
-        LAUNCH(sc, KID_bucket_reset, k_bucket_reset, dim3(1), dim3(64), s, w);
-        LAUNCH(sc, KID_bucket_count, k_bucket_count, gridStride, T, s, p, w);
-        LAUNCH(sc, KID_bucket_fill, k_bucket_fill, gridStride, T, s, p, w);
+        if (p.templates > MAX_TEMPLATES) {
+            std::snprintf(g_error, sizeof(g_error), "more than %u templates", MAX_TEMPLATES);
+            return SAD_E_UNSUPPORTED;
+        }
+        const dim3 chunks(std::max(1, sc.cus) * 2);
+        LAUNCH(sc, KID_bucket_reset, k_bucket_reset, dim3(1), T, s, p, w);
+        LAUNCH(sc, KID_bucket_count, k_bucket_count, chunks, T, s, p, w);
+        LAUNCH(sc, KID_bucket_offsets, k_bucket_offsets, dim3(1), dim3(64), s, p, w);
+        LAUNCH(sc, KID_bucket_fill, k_bucket_fill, chunks, T, s, p, w);
         const dim3 persistent(std::max(1, sc.cus) * sc.blocksPerCU);
         w.pop = sc.pop;
         if (w.nnEnabled) {
