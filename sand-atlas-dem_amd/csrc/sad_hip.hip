@@ -41,6 +41,7 @@ struct P {
     Cfg c;
     uint32_t meshNodes;  // BVH nodes over all templates (rows of meshLinks)
     uint32_t templates;  // rows of meshRoots
+    uint32_t planes;     // rows of meshPlanes
     uint32_t* control;
     float4 *cell, *position, *positionRemainder, *velocityRemainder, *momentumRemainder;
     float4 *rotationRemainder, *referencePositionRemainder, *cellLog, *accumulatedRemainder;
@@ -222,6 +223,59 @@ __device__ bool meshClusterMayContact(const P& p, v4 qs, v4 qd, float ss, float 
     low = low - mk3(pad, pad, pad);
     high = high + mk3(pad, pad, pad);
     return meshBoxMayContact(p, dstShape, low, high);
+}
+
+// ---------------------------------------------------------------- hull-group tests, batched loads
+// meshMayContain / meshBoxMayContact (solver.slang:259-288) are conjunctions over
+// the planes of one group, so any evaluation order gives the reference's answer.
+// The reference loop waits for each plane's load before testing it; smooth grains
+// have 100-500 planes per group, so here four planes are loaded together (indices
+// clamped into the group) and tested with the reference's exact FP32 expressions.
+constexpr uint32_t PLANE_BATCH = 4;
+__device__ __forceinline__ bool planesPass(const float4* __restrict__ planes, uint32_t first, uint32_t count, v3 x,
+                                           v3 half, bool box) {
+    for (uint32_t k = 0; k < count; k += PLANE_BATCH) {
+        float4 pl[PLANE_BATCH];
+#pragma unroll
+        for (uint32_t j = 0; j < PLANE_BATCH; j++) pl[j] = planes[first + min(k + j, count - 1)];
+        bool bad = false;
+#pragma unroll
+        for (uint32_t j = 0; j < PLANE_BATCH; j++) {
+            float v = box ? dot(xyz(pl[j]), x) + pl[j].w - dot(fabs3(xyz(pl[j])), half) : dot(xyz(pl[j]), x) + pl[j].w;
+            bad |= v > 0;
+        }
+        if (bad) return false;
+    }
+    return true;
+}
+__device__ bool meshMayContainFast(const P& p, const W& w, uint32_t shape, v3 q) {
+    uint32_t root = p.meshRoots[shape].x;
+    v3 low = xyz(p.meshMin[root]), high = xyz(p.meshMax[root]);
+    if (q.x < low.x || q.y < low.y || q.z < low.z || q.x > high.x || q.y > high.y || q.z > high.z) return false;
+    v3 direction = (q - 0.5f * (low + high)) / (high - low);
+    uint4 info = p.meshHullInfo[6 * shape + dominantGroup(direction)];
+    return info.y == 0 || planesPass(p.meshPlanes, info.x, info.y, q, mk3(0, 0, 0), false);
+}
+__device__ bool meshClusterMayContactFast(const P& p, const W& w, v4 qs, v4 qd, float ss, float sd, v3 branch,
+                                          uint32_t dstShape, uint32_t cluster) {
+    v3 sl = xyz(p.meshClusterMin[cluster]), sh = xyz(p.meshClusterMax[cluster]);
+    v3 low = {1e30f, 1e30f, 1e30f}, high = {-1e30f, -1e30f, -1e30f};
+    for (uint32_t corner = 0; corner < 8; corner++) {
+        v3 q = {(corner & 1) ? sh.x : sl.x, (corner & 2) ? sh.y : sl.y, (corner & 4) ? sh.z : sl.z};
+        q = qinv(qd, qrot(qs, q * ss) - branch) / sd;
+        low = fmin3(low, q);
+        high = fmax3(high, q);
+    }
+    float pad = 64.0f * DEM_EPS * (1.0f + maxabs(low) + maxabs(high) + length(branch) / sd);
+    low = low - mk3(pad, pad, pad);
+    high = high + mk3(pad, pad, pad);
+    uint32_t root = p.meshRoots[dstShape].x;
+    v3 tl = xyz(p.meshMin[root]), th = xyz(p.meshMax[root]);
+    if (high.x < tl.x || high.y < tl.y || high.z < tl.z || low.x > th.x || low.y > th.y || low.z > th.z) return false;
+    v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
+    v3 direction = (center - 0.5f * (tl + th)) / (th - tl);
+    uint4 info = p.meshHullInfo[6 * dstShape + dominantGroup(direction)];
+    return info.y == 0 || planesPass(p.meshPlanes, info.x, info.y, center, half, true);
 }
 
 // Nearest-feature BVH query (cpu_scheduling.slang:18-48).  The pruning distance
@@ -994,7 +1048,7 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
             uint4 info = p.meshClusterInfo[f.si];
             for (uint32_t base = 0; base < info.y; base += WAVE) {
                 uint32_t c = base + lane;
-                bool ok = c < info.y && meshClusterMayContact(p, f.qi, f.qj, f.s_i, f.s_j, f.branch, f.sj, info.x + c);
+                bool ok = c < info.y && meshClusterMayContactFast(p, w, f.qi, f.qj, f.s_i, f.s_j, f.branch, f.sj, info.x + c);
                 uint64_t m = __ballot(ok);
                 if (lane == 0) cand[base / WAVE] = m;
             }
@@ -1015,7 +1069,7 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
                         v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
                         v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
                         float planePad = 64.0f * DEM_EPS * (1.0f + maxabs(local));
-                        pass_ = !(dot(xyz(f.plane), local) + f.plane.w > planePad) && meshMayContain(p, f.sj, local);
+                        pass_ = !(dot(xyz(f.plane), local) + f.plane.w > planePad) && meshMayContainFast(p, w, f.sj, local);
                     }
                     uint64_t m = __ballot(pass_);
                     uint32_t n = __popcll(m);
@@ -1401,6 +1455,18 @@ __global__ void k_cull_probe(P p, uint32_t count, uint32_t shape, const float4* 
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id < count) out[id] = meshMayContain(p, shape, xyz(points[id])) ? 1 : 0;
 }
+__global__ void k_cull_fast_probe(P p, W w, uint32_t count, uint32_t shape, const float4* points, uint32_t* out) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < count) out[id] = meshMayContainFast(p, w, shape, xyz(points[id])) ? 1 : 0;
+}
+__global__ void k_cluster_fast_probe(P p, W w, uint32_t count, uint32_t src, uint32_t dst, uint32_t* out) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count) return;
+    v3 branch = centerBranch(p, src, dst);
+    out[id] = meshClusterMayContactFast(p, w, v4of(p.rotation[src]), v4of(p.rotation[dst]), p.body[src].x,
+                                        p.body[dst].x, branch, p.shapeId[dst],
+                                        p.meshClusterInfo[p.shapeId[src]].x + id) ? 1 : 0;
+}
 __global__ void k_cluster_probe(P p, uint32_t count, uint32_t src, uint32_t dst, uint32_t* out) {
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= count) return;
@@ -1621,6 +1687,7 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
     }
     p.meshNodes = uint32_t(bytes[SAD_MESH_LINKS] / 16);
     p.templates = uint32_t(bytes[SAD_MESH_ROOTS] / 16);
+    p.planes = uint32_t(bytes[SAD_MESH_PLANES] / 16);
     p.control = slot<uint32_t>(b, SAD_CONTROL);
     p.cell = slot<float4>(b, SAD_CELL);
     p.position = slot<float4>(b, SAD_POSITION);
@@ -2053,6 +2120,12 @@ int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buf
     else if (name == "meshPairBoundsProbe")
         hipLaunchKernelGGL(k_pair_bounds_probe, dim3(blocks(count)), dim3(256), 0, s, p, count, uniforms[1],
                            uniforms[2], static_cast<float4*>(output));
+    else if (name == "meshCullFastProbe")  // the collection stage's batched hull tests
+        hipLaunchKernelGGL(k_cull_fast_probe, dim3(blocks(count)), dim3(256), 0, s, p, W{}, count, uniforms[1],
+                           static_cast<const float4*>(input), static_cast<uint32_t*>(output));
+    else if (name == "meshClusterFastProbe")
+        hipLaunchKernelGGL(k_cluster_fast_probe, dim3(blocks(count)), dim3(256), 0, s, p, W{}, count, uniforms[1],
+                           uniforms[2], static_cast<uint32_t*>(output));
     else {
         std::snprintf(g_error, sizeof(g_error), "unknown probe %s", kernel);
         return SAD_E_ARGUMENT;

@@ -143,3 +143,38 @@ def test_collector_matches_reference_generated_collector(pkg, dense, hip, oracle
     ma, pa, na = a.contacts()
     mb, pb, nb = b.contacts()
     assert np.array_equal(ma, mb) and same(pa, pb) and same(na, nb)
+
+
+def test_batched_hull_tests_equal_reference(pkg, hip):
+    """The collection stage's batched hull-group tests return exactly the reference
+    meshMayContain / meshClusterMayContact answers (smooth grains: 100-500 planes
+    per group, several load batches per test)."""
+    from sand_atlas_dem_amd import assembly
+
+    w = assembly.config_workload("uneven", n=300, templates=6)
+    e = w.engine(hip)
+    e.initialize()
+    b = e.bindings()
+    info = hip.read(e.buffers["meshHullInfo"], np.uint32, (-1, 4))
+    assert info[:, 1].max() > 32
+    rng = np.random.default_rng(9)
+    for t in range(len(w.shapes)):
+        v = w.shapes[t].vertices / e.length_unit
+        near = v[rng.integers(0, len(v), 20000)] * rng.uniform(0.9, 1.1, (20000, 1))
+        pts = np.concatenate([rng.uniform(v.min(0) - 0.1, v.max(0) + 0.1, (20000, 3)), near])
+        pts = np.pad(pts, ((0, 0), (0, 1))).astype(np.float32)
+        ref = hip.probe("meshCullProbe", [len(pts), t], b, pts, np.zeros(len(pts), np.uint32))
+        fast = hip.probe("meshCullFastProbe", [len(pts), t], b, pts, np.zeros(len(pts), np.uint32))
+        assert np.array_equal(ref, fast)
+        assert 0.05 < ref.mean() < 0.95
+    pairs = e.read("pairs", np.uint32, 2)[: int(e.read("control", np.uint32, 1)[1])]
+    clusters = hip.read(e.buffers["meshClusterInfo"], np.uint32, (-1, 4))[:, 1]
+    checked = 0
+    for i, j in pairs[rng.choice(len(pairs), 200, replace=False)]:
+        for src, dst in ((i, j), (j, i)):
+            n = int(clusters[w.ids[src]])
+            ref = hip.probe("meshClusterProbe", [n, src, dst], b, None, np.zeros(n, np.uint32))
+            fast = hip.probe("meshClusterFastProbe", [n, src, dst], b, None, np.zeros(n, np.uint32))
+            assert np.array_equal(ref, fast), (src, dst)
+            checked += int(ref.sum())
+    assert checked > 0
