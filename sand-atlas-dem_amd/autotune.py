@@ -69,13 +69,16 @@ def autotune(engine, *, rate=(0, 0, 0), steps=3, space=SPACE, cache=None, log=No
 
     best_t = measure(current)
     trials = [(dict(current), best_t)]
+    seen = {tuple(sorted(current.items()))}
     if log:
         log(f"[autotune] {current}: {1e3 * best_t:.3f} ms/step")
     for name, values in space.items():
         for value in values:
-            if value == current[name]:
-                continue
             trial = dict(current, **{name: value})
+            tag = tuple(sorted(trial.items()))
+            if tag in seen:
+                continue
+            seen.add(tag)
             t = measure(trial)
             trials.append((trial, t))
             if log:

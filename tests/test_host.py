@@ -198,3 +198,58 @@ def test_capacity_growth_replays_to_large_capacity_result(pkg, oracle_module):
     tight = replace(w, spec=replace(w.spec, contacts_per_particle=4)).engine(cpu)
     with pytest.raises(pkg.RejectedStep):
         tight.initialize()
+
+
+def test_autotune_coordinate_descent_and_cache(pkg, tmp_path):
+    """Tuner logic on a stand-in engine whose step time depends on the knobs."""
+    import time
+    from types import SimpleNamespace
+
+    from sand_atlas_dem_amd import autotune
+
+    class Backend:
+        def __init__(self):
+            self.knobs = {"collect_blocks_per_cu": 6, "collect_pop": 1, "collect_queue": 128}
+            self.torch = SimpleNamespace(cuda=SimpleNamespace(get_device_name=lambda d: "stand-in"))
+            self.torch_device = None
+
+        def tune(self, name, value=-1):
+            old = self.knobs[name]
+            if value >= 0:
+                self.knobs[name] = value
+            return old
+
+        def wait(self):
+            pass
+
+    class Engine:
+        def __init__(self):
+            self.m = Backend()
+            self.n, self.step = 4, 7
+            self.shapes = [SimpleNamespace(faces=np.zeros((8, 3)), samples=np.zeros((16, 3)))]
+            self.ids = np.zeros(4, np.uint32)
+            self.restores = 0
+
+        def state(self):
+            return {"x": 1}
+
+        def restore(self, state, step):
+            self.restores += 1
+            self.step = step
+
+        def advance(self, k, rate=None):
+            best = {"collect_blocks_per_cu": 4, "collect_pop": 2, "collect_queue": 256}
+            miss = sum(self.m.knobs[n] != v for n, v in best.items())
+            time.sleep(0.002 * (1 + 3 * miss) * k)
+            self.step += k
+
+    e = Engine()
+    space = {"collect_blocks_per_cu": (2, 4, 6), "collect_pop": (1, 2), "collect_queue": (128, 256)}
+    r = autotune.autotune(e, steps=1, space=space, cache=tmp_path / "t.json")
+    assert r["best"] == {"collect_blocks_per_cu": 4, "collect_pop": 2, "collect_queue": 256}
+    assert e.m.knobs == r["best"] and e.step == 7 and not r["cached"]
+    settings = [tuple(sorted(t["setting"].items())) for t in r["trials"]]
+    assert len(settings) == len(set(settings)) == 5  # base + 2 + 1 + 1, no repeats
+    e2 = Engine()
+    r2 = autotune.autotune(e2, steps=1, space=space, cache=tmp_path / "t.json")
+    assert r2["cached"] and e2.m.knobs == r["best"] and e2.restores == 0
