@@ -92,9 +92,8 @@ struct W {
     uint32_t* xcd;        // per queue q: count at [q*LINE], fill cursor at [(8+q)*LINE], pop cursor at [(16+q)*LINE]
     uint32_t* stageCount; // staged hits per queue at [q*LINE]; queue q stages into [q*stageCap, (q+1)*stageCap)
     uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue, ascending target template within a queue
-    uint32_t* tmplCount;  // [MAX_TEMPLATES] tasks per target template
-    uint32_t* tmplStart;  // first slot of each template's tasks in xcdTasks
-    uint32_t* tmplCursor; // fill cursor per template
+    uint32_t* keyCount;   // tasks per (target rank, source) key
+    uint32_t* keyStart;   // exclusive scan of keyCount; consumed as fill cursors
     // Learned cull (nn.h): per-template BF16 weights and FP32 blocks; stats on separate lines.
     uint32_t nnEnabled, nnAudit, nnTemplates;
     const uint16_t* nnW;
@@ -758,58 +757,56 @@ __global__ void k_prepare_pairs(P p) {  // solver.slang:469-489 (Metal frontier 
     p.meshPairMask[id] = mask;
 }
 
-// Task queues per XCD, keyed by the target template (the BVH a task reads): a
-// counting sort by template, queue q = template & 7 holding its templates in
-// ascending order, so the waves of one XCD sweep through one template's BVH at a
-// time and its L2 holds the working set.  Each block owns one contiguous chunk of
-// tasks; LDS counts are reserved with one global atomic per (block, template).
+// Task queues per XCD: a counting sort of the directed-pair tasks by (target
+// template, source template).  Queue q = target & 7 holds its targets in ascending
+// order, each target's tasks grouped by source template, so the waves of one XCD
+// sweep one target BVH and, within it, one source's cluster tree and samples at a
+// time, and the XCD's L2 holds both.  Libraries above SORT_PAIRS_MAX templates are
+// sorted by target only.  Order inside a bucket is irrelevant to the results (a
+// task's hits are ranked within the task, tasks by id).
 constexpr uint32_t MAX_TEMPLATES = 4096;
-__device__ __forceinline__ uint32_t taskTemplate(const P& p, uint32_t task) {
+constexpr uint32_t SORT_PAIRS_MAX = 1024;
+struct BucketShape {
+    uint32_t perQueue;  // target ranks per queue = ceil(T / 8)
+    uint32_t sources;   // secondary key range (T, or 1)
+    __device__ __host__ uint32_t keys() const { return 8 * perQueue * sources; }
+};
+__device__ __host__ inline BucketShape bucketShape(uint32_t templates) {
+    return {(templates + 7) / 8, templates <= SORT_PAIRS_MAX ? templates : 1u};
+}
+__device__ __forceinline__ uint32_t taskKey(const P& p, uint32_t task, BucketShape b) {
     uint2 ij = p.pairs[task >> 1];
-    return p.shapeId[(task & 1) ? ij.x : ij.y];
+    uint32_t target = p.shapeId[(task & 1) ? ij.x : ij.y], source = p.shapeId[(task & 1) ? ij.y : ij.x];
+    uint32_t rank = (target & 7) * b.perQueue + (target >> 3);
+    return rank * b.sources + (b.sources > 1 ? source : 0u);
 }
 __global__ void k_bucket_reset(P p, W w) {
-    for (uint32_t t = threadIdx.x; t < p.templates; t += blockDim.x) { w.tmplCount[t] = 0; w.tmplCursor[t] = 0; }
-    if (threadIdx.x < 24) w.xcd[threadIdx.x * LINE] = 0;
+    const BucketShape b = bucketShape(p.templates);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < b.keys(); k += gridDim.x * blockDim.x)
+        w.keyCount[k] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 24) w.xcd[threadIdx.x * LINE] = 0;
 }
-__device__ __forceinline__ void chunkOf(uint32_t tasks, uint32_t& lo, uint32_t& hi) {
-    uint32_t per = (tasks + gridDim.x - 1) / gridDim.x;
-    lo = min(tasks, blockIdx.x * per);
-    hi = min(tasks, lo + per);
+__global__ void k_bucket_count(P p, W w) {
+    const BucketShape b = bucketShape(p.templates);
+    const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks; t += gridDim.x * blockDim.x)
+        atomicAdd(w.keyCount + taskKey(p, t, b), 1u);
 }
-__global__ __launch_bounds__(256) void k_bucket_count(P p, W w) {
-    __shared__ uint32_t local[MAX_TEMPLATES];
-    for (uint32_t t = threadIdx.x; t < p.templates; t += blockDim.x) local[t] = 0;
-    __syncthreads();
-    uint32_t lo, hi;
-    chunkOf(2 * min(p.control[1], p.c.pairCapacity), lo, hi);
-    for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) atomicAdd(local + taskTemplate(p, k), 1u);
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < p.templates; t += blockDim.x)
-        if (local[t]) atomicAdd(w.tmplCount + t, local[t]);
-}
-// Queue totals at w.xcd[q * LINE]; absolute template starts in queue order.
+// Queue totals from the scanned key offsets (w.keyStart, exclusive).
 __global__ void k_bucket_offsets(P p, W w) {
-    if (threadIdx.x != 0) return;
-    uint32_t base = 0;
-    for (uint32_t q = 0; q < 8; q++) {
-        uint32_t start = base;
-        for (uint32_t t = q; t < p.templates; t += 8) { w.tmplStart[t] = base; base += w.tmplCount[t]; }
-        w.xcd[q * LINE] = base - start;
+    const BucketShape b = bucketShape(p.templates);
+    if (threadIdx.x < 8) {
+        const uint32_t q = threadIdx.x, span = b.perQueue * b.sources;
+        const uint32_t lo = w.keyStart[q * span];
+        const uint32_t hi = q == 7 ? w.counters[6 * LINE] : w.keyStart[(q + 1) * span];
+        w.xcd[q * LINE] = hi - lo;
     }
 }
-__global__ __launch_bounds__(256) void k_bucket_fill(P p, W w) {
-    __shared__ uint32_t local[MAX_TEMPLATES];
-    for (uint32_t t = threadIdx.x; t < p.templates; t += blockDim.x) local[t] = 0;
-    __syncthreads();
-    uint32_t lo, hi;
-    chunkOf(2 * min(p.control[1], p.c.pairCapacity), lo, hi);
-    for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) atomicAdd(local + taskTemplate(p, k), 1u);
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < p.templates; t += blockDim.x)
-        if (local[t]) local[t] = w.tmplStart[t] + atomicAdd(w.tmplCursor + t, local[t]);
-    __syncthreads();
-    for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) w.xcdTasks[atomicAdd(local + taskTemplate(p, k), 1u)] = k;
+__global__ void k_bucket_fill(P p, W w) {
+    const BucketShape b = bucketShape(p.templates);
+    const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks; t += gridDim.x * blockDim.x)
+        w.xcdTasks[atomicAdd(w.keyStart + taskKey(p, t, b), 1u)] = t;
 }
 
 struct TaskFrame {
@@ -1553,7 +1550,7 @@ int grow(T*& ptr, uint32_t& cap, size_t need, size_t& bytes) {
 struct Caps {
     uint32_t counters = 0, taskCount = 0, taskOffset = 0, adjCount = 0, adjOffset = 0, adjCursor = 0, adjList = 0;
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
-    uint32_t stageCount = 0, tmplCount = 0, tmplStart = 0, tmplCursor = 0;
+    uint32_t stageCount = 0, keyCount = 0, keyStart = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -1571,6 +1568,15 @@ struct ScratchImpl : Scratch {
         graphs.clear();
     }
     int ensure_nodes(uint32_t nodes) { return grow(w.wide, caps.wide, 4 * size_t(std::max(nodes, 1u)), bytes); }
+    int ensure_keys(uint32_t templates) {
+        const uint32_t keys = bucketShape(templates).keys();
+        int rc = grow(w.keyCount, caps.keyCount, keys, bytes);
+        if (rc || (rc = grow(w.keyStart, caps.keyStart, keys, bytes))) return rc;
+        const size_t sums = (size_t(keys) + SCAN_TILE - 1) / SCAN_TILE + 1;
+        if (sums > caps.sums && (rc = grow(w.sums, caps.sums, sums, bytes))) return rc;
+        w.sumsCap = caps.sums;
+        return SAD_OK;
+    }
     int ensure(const Cfg& c) {
         size_t tasks = size_t(2) * c.pairCapacity;
         int rc;
@@ -1595,9 +1601,7 @@ struct ScratchImpl : Scratch {
         if ((rc = grow(w.stageNa, caps.stageNa, size_t(8) * c.contactCapacity, bytes))) return rc;
         if ((rc = grow(w.xcd, caps.xcd, 24 * LINE, bytes))) return rc;
         if ((rc = grow(w.xcdTasks, caps.xcdTasks, tasks, bytes))) return rc;
-        if ((rc = grow(w.tmplCount, caps.tmplCount, MAX_TEMPLATES, bytes))) return rc;
-        if ((rc = grow(w.tmplStart, caps.tmplStart, MAX_TEMPLATES, bytes))) return rc;
-        if ((rc = grow(w.tmplCursor, caps.tmplCursor, MAX_TEMPLATES, bytes))) return rc;
+
         w.stageCap = c.contactCapacity;
         w.taskCap = uint32_t(tasks);
         w.nCap = c.N;
@@ -1617,7 +1621,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.counters); (void)hipFree(w.taskCount); (void)hipFree(w.taskOffset); (void)hipFree(w.adjCount);
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
-        (void)hipFree(w.tmplCount); (void)hipFree(w.tmplStart); (void)hipFree(w.tmplCursor);
+        (void)hipFree(w.keyCount); (void)hipFree(w.keyStart);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
 };
@@ -1815,11 +1819,14 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
             std::snprintf(g_error, sizeof(g_error), "more than %u templates", MAX_TEMPLATES);
             return SAD_E_UNSUPPORTED;
         }
-        const dim3 chunks(std::max(1, sc.cus) * 2);
-        LAUNCH(sc, KID_bucket_reset, k_bucket_reset, dim3(1), T, s, p, w);
-        LAUNCH(sc, KID_bucket_count, k_bucket_count, chunks, T, s, p, w);
+        if ((rc = sc.ensure_keys(p.templates))) return rc;
+        const uint32_t keys = bucketShape(p.templates).keys();
+        LAUNCH(sc, KID_bucket_reset, k_bucket_reset, dim3(std::min(blocks(keys), 1024u)), T, s, p, w);
+        LAUNCH(sc, KID_bucket_count, k_bucket_count, gridStride, T, s, p, w);
+        ScanArgs keyScan{w.keyCount, w.keyStart, w.sums, keys, nullptr, 1, nullptr, w.counters + 6 * LINE};
+        launch_scan(sc, keyScan, keys, s);
         LAUNCH(sc, KID_bucket_offsets, k_bucket_offsets, dim3(1), dim3(64), s, p, w);
-        LAUNCH(sc, KID_bucket_fill, k_bucket_fill, chunks, T, s, p, w);
+        LAUNCH(sc, KID_bucket_fill, k_bucket_fill, gridStride, T, s, p, w);
         const dim3 persistent(std::max(1, sc.cus) * sc.blocksPerCU);
         w.pop = sc.pop;
         if (w.nnEnabled) {
@@ -2052,6 +2059,7 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const
             if (rc) return rc;
             if ((rc = sc->ensure(p.c))) return rc;
             if ((rc = sc->ensure_nodes(p.meshNodes))) return rc;
+            if (p.templates <= MAX_TEMPLATES && (rc = sc->ensure_keys(p.templates))) return rc;
             if (graph && !sc->profiling) {
                 sc->w.pop = sc->pop;
                 rc = graph_step(p, *sc, graph_key(table, sizes, vals, *sc), s);
