@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/sand_dem_hip.h"
@@ -114,6 +115,9 @@ constexpr int SCAN_TILE = 1024;       // 256 threads x 4
 constexpr int LINE = 64;
 #ifndef SAD_POP
 #define SAD_POP 1
+#endif
+#ifndef SAD_COLLECT_WAVES_PER_EU
+#define SAD_COLLECT_WAVES_PER_EU 5
 #endif
 #ifndef SAD_COLLECT_BLOCKS_PER_CU
 #define SAD_COLLECT_BLOCKS_PER_CU 6
@@ -343,12 +347,25 @@ __global__ void k_wide_nodes(P p, W w, uint32_t nodes) {
         w.wide[4 * size_t(n) + 3] = make_float4(hb.x, hb.y, hb.z, 0.0f);
     }
 }
+// Cached push distances on the LDS stack: exact FP32, or BF16 truncated toward
+// zero (every distance is >= 0, so the stored value never exceeds the true one).
+// A rounded-down distance can only make the pop-time test `dist > best` fail more
+// often, i.e. visit nodes the exact test would skip; such a node's triangles are
+// all farther than `best`, so they cannot replace it under the strict `<`, and the
+// traversal order (decided at push time from the exact values) is unchanged: the
+// result is identical, with half the stack LDS.
+__device__ __forceinline__ void dstore(float* p, float v) { *p = v; }
+__device__ __forceinline__ float dload(const float* p) { return *p; }
+__device__ __forceinline__ void dstore(uint16_t* p, float v) { *p = uint16_t(__float_as_uint(v) >> 16); }
+__device__ __forceinline__ float dload(const uint16_t* p) { return __uint_as_float(uint32_t(*p) << 16); }
+
+template <typename D>
 __device__ v4 meshDistanceWide(const P& p, const float4* __restrict__ wide, uint32_t shape, v3 q, uint32_t* sref,
-                               float* sdist) {
+                               D* sdist) {
     const uint32_t root = p.meshRoots[shape].x;
     uint32_t top = 1;
     sref[0] = nodeRef(p, root);
-    sdist[0] = boxDistance(xyz(p.meshMin[root]), xyz(p.meshMax[root]), q);
+    dstore(sdist, boxDistance(xyz(p.meshMin[root]), xyz(p.meshMax[root]), q));
     float best = 1e30f;
     v3 nearest = {0, 0, 0};
     uint32_t nearestNormal = 0;
@@ -356,7 +373,7 @@ __device__ v4 meshDistanceWide(const P& p, const float4* __restrict__ wide, uint
     while (top > 0) {
         --top;
         uint32_t ref = sref[top * WAVE];
-        if (sdist[top * WAVE] > best) continue;
+        if (dload(sdist + top * WAVE) > best) continue;
         if ((ref & LEAF) == 0) {
             const float4* node = wide + 4 * size_t(ref);
             float4 w0 = node[0], w1 = node[1], w2 = node[2], w3 = node[3];
@@ -364,8 +381,8 @@ __device__ v4 meshDistanceWide(const P& p, const float4* __restrict__ wide, uint
             uint32_t a = __float_as_uint(w0.w), b = __float_as_uint(w1.w);
             if (da < db) { uint32_t s = a; a = b; b = s; float t = da; da = db; db = t; }
             if (top + 2 > WSTACK) { fail(p, 1024); return {0, 0, 0, 1e20f}; }
-            if (da <= best) { sref[top * WAVE] = a; sdist[top * WAVE] = da; top++; }
-            if (db <= best) { sref[top * WAVE] = b; sdist[top * WAVE] = db; top++; }
+            if (da <= best) { sref[top * WAVE] = a; dstore(sdist + top * WAVE, da); top++; }
+            if (db <= best) { sref[top * WAVE] = b; dstore(sdist + top * WAVE, db); top++; }
             continue;
         }
         uint32_t first = ref & 0x0fffffffu, count = ((ref >> 28) & 3u) + 1;
@@ -818,7 +835,8 @@ struct TaskFrame {
 };
 
 // Exact part of nodeContact (solver.slang:319-343) for one queued node.
-__device__ bool queryNode(const P& p, const W& w, const TaskFrame& f, uint32_t node, uint32_t* sref, float* sdist,
+template <typename D>
+__device__ bool queryNode(const P& p, const W& w, const TaskFrame& f, uint32_t node, uint32_t* sref, D* sdist,
                           float4& pd, float4& na) {
     float4 sp = p.sample[f.sampleBase + node];
     v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
@@ -894,8 +912,9 @@ __device__ bool cullBatch(const P& p, const W& w, const TaskFrame& f, const uint
 }
 
 // Exact queries of queue[0..qn) in 64-node batches, ranked in queue order.
+template <typename D>
 __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
-                           const uint32_t* queue, uint32_t qn, uint32_t* sref, float* sdist, uint32_t& running,
+                           const uint32_t* queue, uint32_t qn, uint32_t* sref, D* sdist, uint32_t& running,
                            int lane) {
     for (uint32_t q = 0; q < qn; q += WAVE) {
         uint32_t k = q + lane;
@@ -946,14 +965,16 @@ __device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32
     return w.nnAudit ? 0u : kept;
 }
 
-template <bool NN>
+template <bool NN, typename D>
 __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
-                           uint32_t* queue, uint32_t qn, uint32_t* sref, float* sdist, uint32_t& running, int lane,
+                           uint32_t* queue, uint32_t qn, uint32_t* sref, D* sdist, uint32_t& running, int lane,
                            uint16_t* nnA, uint16_t* nnB) {
     // Templates whose margin is infinite (calibration found no safe cull) skip the network.
-    if (NN && f.sj < w.nnTemplates && w.nnF[size_t(f.sj) * nn::FSTRIDE + nn::FMARGIN] < 1e30f) {
-        qn = cullQueue(p, w, f, task, stageQueue, queue, qn, sref, sdist, running, lane, nnA, nnB);
-        wave_sync();
+    if constexpr (NN) {
+        if (f.sj < w.nnTemplates && w.nnF[size_t(f.sj) * nn::FSTRIDE + nn::FMARGIN] < 1e30f) {
+            qn = cullQueue(p, w, f, task, stageQueue, queue, qn, sref, sdist, running, lane, nnA, nnB);
+            wave_sync();
+        }
     }
     queryQueue(p, w, f, task, stageQueue, queue, qn, sref, sdist, running, lane);
 }
@@ -963,14 +984,17 @@ __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
 // QN = query-queue capacity per wave (tunable: results do not depend on it, since hits
 // are ranked by their running count within the task, not by flush boundaries).
 template <bool NN, int QN>
-__global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
+__device__ __forceinline__ void collectBody(P p, W w) {
+    // The learned-cull variant needs the FP32 distance stack: its 4 KB doubles as a
+    // BF16 activation buffer of the network.
+    using D = typename std::conditional<NN, float, uint16_t>::type;
     __shared__ uint32_t stackRef[COLLECT_WAVES][WSTACK * WAVE];
-    __shared__ float stackDist[COLLECT_WAVES][WSTACK * WAVE];
+    __shared__ D stackDist[COLLECT_WAVES][WSTACK * WAVE];
     __shared__ uint32_t queues[COLLECT_WAVES][QN];
     __shared__ uint64_t candidates[COLLECT_WAVES][MAX_CLUSTERS / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t* sref = &stackRef[wv][lane];
-    float* sdist = &stackDist[wv][lane];
+    D* sdist = &stackDist[wv][lane];
     uint16_t* nnA = reinterpret_cast<uint16_t*>(stackRef[wv]);   // 32 x 64 BF16 = the stack's 4 KB
     uint16_t* nnB = reinterpret_cast<uint16_t*>(stackDist[wv]);
     uint32_t* queue = queues[wv];
@@ -1072,7 +1096,7 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
                     uint32_t n = __popcll(m);
                     if (qn + n > QN) {
                         wave_sync();
-                        flushQueue<NN>(p, w, f, task, home, queue, qn, sref, sdist, running, lane, nnA, nnB);
+                        flushQueue<NN, D>(p, w, f, task, home, queue, qn, sref, sdist, running, lane, nnA, nnB);
                         queried += qn;
                         qn = 0;
                         wave_sync();
@@ -1083,7 +1107,7 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
                 cluster = candidate ? cluster + 1 : range.z;
             }
             wave_sync();
-            flushQueue<NN>(p, w, f, task, home, queue, qn, sref, sdist, running, lane, nnA, nnB);
+            flushQueue<NN, D>(p, w, f, task, home, queue, qn, sref, sdist, running, lane, nnA, nnB);
             queried += qn;
             wave_sync();
         }
@@ -1091,6 +1115,14 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect(P p, W w) {
     }
     if (lane == 0 && queried) atomicAdd(w.counters + 2 * LINE, queried);
 }
+
+// The plain kernel keeps <= 96 VGPRs and 29 KB of LDS per workgroup so five
+// workgroups (20 waves) share a CU: the exact queries are L2-latency bound.
+template <int QN>
+__global__ __launch_bounds__(COLLECT_WAVES * WAVE) __attribute__((amdgpu_waves_per_eu(SAD_COLLECT_WAVES_PER_EU)))
+void k_collect(P p, W w) { collectBody<false, QN>(p, w); }
+template <int QN>
+__global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect_nn(P p, W w) { collectBody<true, QN>(p, w); }
 
 // control[2] = total contacts, per-pair counts (solver.slang:397-403 semantics).
 __global__ void k_contact_total(P p, W w) {
@@ -1831,14 +1863,14 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         w.pop = sc.pop;
         if (w.nnEnabled) {
             if (sc.queue == 128)
-                LAUNCH(sc, KID_collect, (k_collect<true, 128>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+                LAUNCH(sc, KID_collect, (k_collect_nn<128>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
             else
-                LAUNCH(sc, KID_collect, (k_collect<true, 256>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+                LAUNCH(sc, KID_collect, (k_collect_nn<256>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
         } else {
             if (sc.queue == 128)
-                LAUNCH(sc, KID_collect, (k_collect<false, 128>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+                LAUNCH(sc, KID_collect, (k_collect<128>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
             else
-                LAUNCH(sc, KID_collect, (k_collect<false, 256>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
+                LAUNCH(sc, KID_collect, (k_collect<256>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
         }
         ScanArgs a{w.taskCount, w.taskOffset, w.sums, 2 * c.pairCapacity, p.control + 1, 2, nullptr, w.counters + 4 * LINE};
         launch_scan(sc, a, 2 * c.pairCapacity, s);
