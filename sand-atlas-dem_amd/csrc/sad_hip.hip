@@ -1257,12 +1257,14 @@ __global__ void k_build_hash(P p) {
 constexpr int GCHUNK = 64;
 constexpr int GFIELDS = 16;
 constexpr int GSTRIDE = GCHUNK + 1;  // padded rows: lanes reading different fields hit different banks
+constexpr int GSUMS = 22;            // reference sums per grain, one per lane
 enum GField { G_FX, G_FY, G_FZ, G_E, G_AX, G_AY, G_AZ, G_DEPTH, G_SLIP, G_NLP, G_AREA, G_SRC, G_BX, G_BY, G_BZ, G_MINW };
 
 __device__ __forceinline__ float gget(const float* st, int field, int e) { return st[field * GSTRIDE + e]; }
 
 __global__ __launch_bounds__(256) void k_gather(P p, W w) {
     __shared__ float stage[4][GFIELDS * GSTRIDE];
+    __shared__ float contrib[4][GSUMS * GSTRIDE];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t id = blockIdx.x * 4 + wv;
     if (id >= p.c.N || ld_volatile(p.control) != 0) return;
@@ -1271,49 +1273,57 @@ __global__ __launch_bounds__(256) void k_gather(P p, W w) {
     const float inverseInertia = p.c.activeTimestep ? inverseInertiaBound(p, id) : 0.0f;
     float hi = 0.0f, lo = 0.0f;  // compensated chains (lanes 0-5) or plain sums
     int fill = 0;
+    // Two passes per chunk of endpoints: lane e evaluates every contribution of
+    // endpoint e (the reference's expressions, term by term), then lane k folds
+    // quantity k over the chunk in order -- compensated (force, torque), plain
+    // (stress, energy, |F|, dissipation, power, timestep rates) or max (depth).
+    float* ct = contrib[wv];
     auto process = [&](int count) {
         wave_sync();
-        for (int e = 0; e < count; e++) {
-            bool source = gget(st, G_SRC, e) != 0.0f;
-            v3 force = {gget(st, G_FX, e), gget(st, G_FY, e), gget(st, G_FZ, e)};
-            if (lane < 3) {
-                float fk = lane == 0 ? force.x : (lane == 1 ? force.y : force.z);
-                v3 h = {hi, 0, 0}, l = {lo, 0, 0};
-                compensated3(h, l, {source ? fk : -fk, 0, 0});
-                hi = h.x; lo = l.x;
-            } else if (lane < 6) {
-                v3 arm = {gget(st, G_AX, e), gget(st, G_AY, e), gget(st, G_AZ, e)};
-                v3 branch = {gget(st, G_BX, e), gget(st, G_BY, e), gget(st, G_BZ, e)};
-                v3 c = source ? cross(arm, force) : cross(arm - branch, -force);
-                float ck = lane == 3 ? c.x : (lane == 4 ? c.y : c.z);
-                v3 h = {hi, 0, 0}, l = {lo, 0, 0};
-                compensated3(h, l, {ck, 0, 0});
-                hi = h.x; lo = l.x;
-            } else if (lane < 15) {
-                int r = (lane - 6) / 3, c = (lane - 6) % 3;
-                float fr = r == 0 ? force.x : (r == 1 ? force.y : force.z);
-                float bc = gget(st, G_BX + c, e);
-                hi = hi - 0.5f * fr * bc;
-            } else if (lane == 15) {
-                hi += 0.5f * gget(st, G_E, e);
-            } else if (lane == 16) {
-                hi += length(force);
-            } else if (lane == 17) {
-                hi = fmaxf(hi, gget(st, G_DEPTH, e) / gget(st, G_MINW, e));
-            } else if (lane == 18) {
-                hi += 0.5f * gget(st, G_SLIP, e);
-            } else if (lane == 19) {
-                hi += 0.5f * gget(st, G_NLP, e);
-            } else if (lane < 22 && p.c.activeTimestep != 0) {
-                v3 arm = {gget(st, G_AX, e), gget(st, G_AY, e), gget(st, G_AZ, e)};
-                v3 branch = {gget(st, G_BX, e), gget(st, G_BY, e), gget(st, G_BZ, e)};
-                v3 localArm = source ? arm : arm - branch;
-                float mobility = 1.0f / mass + dot(localArm, localArm) * inverseInertia;
-                float area = gget(st, G_AREA, e) * 0.5f;
-                float coupling = 2.0f;
-                if (lane == 20) hi += coupling * area * (p.c.kn * (1.0f + p.c.mu) + p.c.kt) * mobility;
-                else hi += coupling * area * p.c.normalViscosity * (1.0f + p.c.mu) * mobility;
+        if (lane < count) {
+            const int e = lane;
+            const bool source = gget(st, G_SRC, e) != 0.0f;
+            const v3 force = {gget(st, G_FX, e), gget(st, G_FY, e), gget(st, G_FZ, e)};
+            const v3 arm = {gget(st, G_AX, e), gget(st, G_AY, e), gget(st, G_AZ, e)};
+            const v3 branch = {gget(st, G_BX, e), gget(st, G_BY, e), gget(st, G_BZ, e)};
+            ct[0 * GSTRIDE + e] = source ? force.x : -force.x;
+            ct[1 * GSTRIDE + e] = source ? force.y : -force.y;
+            ct[2 * GSTRIDE + e] = source ? force.z : -force.z;
+            const v3 c = source ? cross(arm, force) : cross(arm - branch, -force);
+            ct[3 * GSTRIDE + e] = c.x;
+            ct[4 * GSTRIDE + e] = c.y;
+            ct[5 * GSTRIDE + e] = c.z;
+            const float fr[3] = {force.x, force.y, force.z}, bc[3] = {branch.x, branch.y, branch.z};
+            for (int r = 0; r < 3; r++)
+                for (int k = 0; k < 3; k++) ct[(6 + 3 * r + k) * GSTRIDE + e] = 0.5f * fr[r] * bc[k];
+            ct[15 * GSTRIDE + e] = 0.5f * gget(st, G_E, e);
+            ct[16 * GSTRIDE + e] = length(force);
+            ct[17 * GSTRIDE + e] = gget(st, G_DEPTH, e) / gget(st, G_MINW, e);
+            ct[18 * GSTRIDE + e] = 0.5f * gget(st, G_SLIP, e);
+            ct[19 * GSTRIDE + e] = 0.5f * gget(st, G_NLP, e);
+            if (p.c.activeTimestep != 0) {
+                const v3 localArm = source ? arm : arm - branch;
+                const float mobility = 1.0f / mass + dot(localArm, localArm) * inverseInertia;
+                const float area = gget(st, G_AREA, e) * 0.5f;
+                const float coupling = 2.0f;
+                ct[20 * GSTRIDE + e] = coupling * area * (p.c.kn * (1.0f + p.c.mu) + p.c.kt) * mobility;
+                ct[21 * GSTRIDE + e] = coupling * area * p.c.normalViscosity * (1.0f + p.c.mu) * mobility;
             }
+        }
+        wave_sync();
+        const float* row = ct + lane * GSTRIDE;
+        if (lane < 6) {
+            for (int e = 0; e < count; e++) {
+                v3 h = {hi, 0, 0}, l = {lo, 0, 0};
+                compensated3(h, l, {row[e], 0, 0});
+                hi = h.x; lo = l.x;
+            }
+        } else if (lane < 15) {
+            for (int e = 0; e < count; e++) hi = hi - row[e];
+        } else if (lane == 17) {
+            for (int e = 0; e < count; e++) hi = fmaxf(hi, row[e]);
+        } else if (lane < 20 || (lane < 22 && p.c.activeTimestep != 0)) {
+            for (int e = 0; e < count; e++) hi += row[e];
         }
         wave_sync();
     };
