@@ -121,6 +121,17 @@ def nn_leg(args, w, e, hip, start, step0, plain_state, log_):
     return out
 
 
+def measured_traffic(kernel, config):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by its 1 GiB copy calibration), if one exists."""
+    names = {"dense": "dense10k", "uneven": "uneven50k"}
+    path = ROOT / "profiles" / f"{kernel}_{names.get(config, config)}_traffic.json"
+    if not path.exists():
+        return None
+    data = json.loads(path.read_text())
+    return data["traffic_bytes"] if data.get("kernel") == kernel and data.get("workload") == config else None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,7 +296,7 @@ def main():
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",
             "frac": achieved / (PEAK_HBM / 1e9),
-            "traffic": None,
+            "traffic": measured_traffic(dominant, args.config),
         },
         "step_roofline": {
             "bytes_per_step": step_bytes,
