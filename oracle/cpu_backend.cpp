@@ -136,7 +136,8 @@ struct R {
     float4 *oldSlip, *oldNormal;
     uint32_t *oldHash, *hash, *heads, *next;
     float4 *particleStress, *timestepRows, *particleDiagnostic, *accumulated;
-    float4* wallVelocity;
+    float4 *wallPlane, *wallRemainder, *wallVelocity, *wallMaterial, *wallRows, *wallAccumulated;
+    uint32_t* wallBoxIds;
     uint64_t metaCount;  // capacity of the meta/pointDepth/normalArea views
 };
 
@@ -237,7 +238,13 @@ R bind(void** b, const uint64_t* bytes, const double* v) {
     r.timestepRows = at<float4>(b, SAD_TIMESTEP_ROWS);
     r.particleDiagnostic = at<float4>(b, SAD_PARTICLE_DIAGNOSTIC);
     r.accumulated = at<float4>(b, SAD_ACCUMULATED);
+    r.wallPlane = at<float4>(b, SAD_WALL_PLANE);
+    r.wallRemainder = at<float4>(b, SAD_WALL_REMAINDER);
     r.wallVelocity = at<float4>(b, SAD_WALL_VELOCITY);
+    r.wallMaterial = at<float4>(b, SAD_WALL_MATERIAL);
+    r.wallBoxIds = at<uint32_t>(b, SAD_WALL_BOX_IDS);
+    r.wallRows = at<float4>(b, SAD_WALL_ROWS);
+    r.wallAccumulated = at<float4>(b, SAD_WALL_ACCUMULATED);
     r.metaCount = bytes[SAD_META] / 16;
     return r;
 }
@@ -518,6 +525,181 @@ void cpuCollectMesh(const R& r, uint32_t pair) {
     }
 }
 
+// ---------------------------------------------------------------- translating planes (solver.slang:135-145, 578-667)
+float wallStepWork(const R& r, uint32_t id, float dt) {  // :135-145
+    float total = 0;
+    for (uint32_t w = 0; w < r.c.wallCount; w++) {
+        uint32_t index = w * r.c.N + id;
+        float increment = dt * r.wallRows[3 * index + 1].w;
+        float4 state = r.wallAccumulated[index];
+        float3 high = {state.x, 0, 0}, low = {state.y, 0, 0};
+        compensated3(high, low, {increment, 0, 0});
+        r.wallAccumulated[index] = {high.x, low.x, 0, 0};
+        r.wallRows[3 * index + 2] = {high.x + low.x, 0, 0, 0};
+        total += increment;
+    }
+    return total;
+}
+struct float2p { float x, y; };
+inline float2p wallTwoSum(float a, float b) {  // :578-582 (materialization mask 0)
+    float high = a + b, v = high - a;
+    float low = (a - (high - v)) + (b - v);
+    return {high, low};
+}
+inline float2p wallAdd(float2p a, float b) {  // :583-586
+    float2p sum = wallTwoSum(a.x, b);
+    return wallTwoSum(sum.x, sum.y + a.y);
+}
+inline float2p wallAddDot(float2p sum, float3 a, float3 b) {  // :587-594
+    float product[3] = {a.x * b.x, a.y * b.y, a.z * b.z};
+    float error[3] = {std::fmaf(a.x, b.x, -product[0]), std::fmaf(a.y, b.y, -product[1]), std::fmaf(a.z, b.z, -product[2])};
+    for (int axis = 0; axis < 3; axis++) {
+        sum = wallAdd(sum, product[axis]);
+        sum = wallAdd(sum, error[axis]);
+    }
+    return sum;
+}
+float2p wallCentreDistance(const R& r, uint32_t body, uint32_t wall) {  // :596-604
+    float4 plane = r.wallPlane[wall], low = r.wallRemainder[wall];
+    float3 p = xyz(r.position[body]), pl = xyz(r.positionRemainder[body]);
+    float2p sum = {-plane.w, -low.w};
+    sum = wallAddDot(sum, xyz(plane), p);
+    sum = wallAddDot(sum, xyz(plane), pl);
+    sum = wallAddDot(sum, xyz(low), p);
+    return wallAddDot(sum, xyz(low), pl);
+}
+float localWallDistance(const R& r, uint32_t body, uint32_t wall, float3 arm) {  // :605-610
+    float2p sum = wallCentreDistance(r, body, wall);
+    sum = wallAddDot(sum, xyz(r.wallPlane[wall]), arm);
+    sum = wallAddDot(sum, xyz(r.wallRemainder[wall]), arm);
+    return sum.x + sum.y;
+}
+bool wallClusterMayContact(const R& r, uint32_t body, uint32_t wall, uint32_t cluster) {  // :613-623
+    float3 low = xyz(r.meshClusterMin[cluster]), high = xyz(r.meshClusterMax[cluster]);
+    float4 plane = r.wallPlane[wall];
+    float3 normal = qinv(r.rotation[body], xyz(plane)) * r.body[body].x;
+    float3 center = 0.5f * (low + high), half = 0.5f * (high - low);
+    float2p centreDistance = wallCentreDistance(r, body, wall);
+    float base = centreDistance.x + centreDistance.y;
+    float support = dot(normal, center) - dot(fabs3(normal), half);
+    float pad = 64.0f * 1.1920928955078125e-7f *
+                (1.0f + std::fabs(plane.w) + length(xyz(r.position[body])) + length(normal) * (length(center) + length(half)));
+    return base + support <= pad;
+}
+void appendWallContact(const R& r, uint32_t body, uint32_t wall, uint32_t node) {  // :625-638
+    float4 point = r.sample[r.sampleInfo[r.shapeId[body]].x + node];
+    float3 arm = qrot(r.rotation[body], xyz(point) * r.body[body].x);
+    float4 plane = r.wallPlane[wall];
+    float phi = localWallDistance(r, body, wall, arm);
+    if (!std::isfinite(phi)) { fail(r, 512); return; }
+    if (phi >= 0) return;
+    if (-phi > r.c.maxOverlap * r.body[body].w) fail(r, 4);
+    uint32_t offset = r.control[2]++;
+    if (offset >= r.c.contactCapacity) { fail(r, 32); return; }
+    r.meta[offset] = {body, r.c.N + wall, node, 1};
+    r.pointDepth[offset] = f4(arm, -phi);
+    r.normalArea[offset] = f4(xyz(plane), point.w * r.body[body].x * r.body[body].x);
+}
+void collectWallContacts(const R& r, uint32_t id) {  // CPU serial order, cpu_scheduling.slang:133-154
+    uint32_t body = id % r.c.N, wall = id / r.c.N;
+    if (wall >= r.c.wallCount || r.control[0] != 0) return;
+    float4 plane = r.wallPlane[wall];
+    float2p centreDistance = wallCentreDistance(r, body, wall);
+    float phi = centreDistance.x + centreDistance.y;
+    float pad = 64.0f * 1.1920928955078125e-7f * (1.0f + length(xyz(r.position[body])) + std::fabs(plane.w));
+    if (phi > r.body[body].z + pad) return;
+    uint4 info = r.meshClusterInfo[r.shapeId[body]];
+    uint32_t cluster = info.x;
+    while (cluster < info.x + info.y) {
+        bool candidate = wallClusterMayContact(r, body, wall, cluster);
+        uint4 range = r.meshClusterRanges[cluster];
+        if (candidate)
+            for (uint32_t lane = 0; lane < range.y; lane++) appendWallContact(r, body, wall, r.meshSampleIndices[range.x + lane]);
+        cluster = candidate ? cluster + 1 : range.z;
+    }
+}
+void advanceWalls(const R& r) {  // :669-687
+    if (r.control[0] != 0) return;
+    for (uint32_t w = 0; w < r.c.wallCount; w++) {
+        float4 plane = r.wallPlane[w], residual = r.wallRemainder[w];
+        float3 high = {plane.w, 0, 0}, low = {residual.w, 0, 0};
+        compensated3(high, low, {dot(xyz(plane), xyz(r.wallVelocity[w])) * r.c.dt, 0, 0});
+        plane.w = high.x;
+        residual.w = low.x;
+        if (!std::isfinite(high.x) || !std::isfinite(low.x)) { fail(r, 512); return; }
+        r.wallPlane[w] = plane;
+        r.wallRemainder[w] = residual;
+    }
+    if (r.c.wallBoxEnabled != 0) {
+        for (uint32_t axis = 0; axis < 3; axis++) {
+            uint32_t lo = r.wallBoxIds[2 * axis], hi = r.wallBoxIds[2 * axis + 1];
+            float low = r.wallPlane[lo].w + r.wallRemainder[lo].w;
+            float high = -(r.wallPlane[hi].w + r.wallRemainder[hi].w);
+            if (low < 0 || high > get(xyz(r.cell[0]), int(axis)) || high <= low) fail(r, 4096);
+        }
+    }
+}
+// planeViscousResponse (plane.slang:13-44) with normalResponse weight 1 (normal.slang:3-11)
+struct PlaneResponse { float4 forceEnergy, slipLoss; bool sliding; float normalLossPower; };
+PlaneResponse planeViscousResponse(float3 normal, float signedDistance, float area, float3 rv, float3 oldSlip, float kn,
+                                   float kt, float mu, float viscosity, float dt) {
+    PlaneResponse value{{0, 0, 0, 0}, {0, 0, 0, 0}, false, 0};
+    if (signedDistance >= 0) return value;
+    float depth = -signedDistance;
+    float3 tangent = rv - normal * dot(rv, normal);
+    float3 trial = oldSlip - normal * dot(oldSlip, normal) + dt * tangent;
+    float3 slip = trial;
+    float normalVelocity = dot(rv, normal);
+    float fn = 0;
+    {
+        float elastic = 1.0f * kn * area * depth;
+        fn = viscosity == 0 ? elastic : std::fmax(0.0f, elastic - 1.0f * viscosity * area * normalVelocity);
+        value.normalLossPower = std::fmax(0.0f, (elastic - fn) * normalVelocity);
+    }
+    float3 ft = -kt * area * slip;
+    float magnitude = length(ft);
+    if (magnitude > mu * fn && magnitude > 0) {
+        value.sliding = true;
+        ft = ft * (mu * fn / magnitude);
+        slip = -ft / (kt * area);
+    }
+    value.forceEnergy = f4(fn * normal + ft, 0.5f * area * (kn * depth * depth + kt * dot(slip, slip)));
+    value.slipLoss = f4(slip, std::fmax(0.0f, -dot(ft, trial - slip)));
+    return value;
+}
+void gatherWallRows(const R& r, uint32_t index) {  // :837-856
+    if (index >= r.c.N * r.c.wallCount || r.control[0] != 0) return;
+    uint32_t body = index % r.c.N, wall = index / r.c.N, count = 0, visited = 0;
+    float3 reaction = {0, 0, 0}, moment = {0, 0, 0};
+    float power = 0;
+    uint32_t entry = r.heads[body];
+    while (entry != EMPTY) {
+        if (visited++ > 2 * r.c.contactCapacity) { fail(r, 128); return; }
+        uint32_t contact = entry / 2;
+        if (r.meta[contact].y == r.c.N + wall) {
+            float3 force = xyz(r.contactForce[contact]);
+            float3 point = xyz(r.position[body]) + xyz(r.positionRemainder[body]) + xyz(r.pointDepth[contact]);
+            reaction = reaction - force;
+            moment = moment - cross(point, force);
+            power += dot(force, xyz(r.wallVelocity[wall]));
+            count++;
+        }
+        entry = r.next[entry];
+    }
+    float4 work = r.wallAccumulated[index];
+    float countBits;
+    std::memcpy(&countBits, &count, 4);
+    r.wallRows[3 * index] = f4(reaction, countBits);
+    r.wallRows[3 * index + 1] = f4(moment, power);
+    r.wallRows[3 * index + 2] = {work.x + work.y, 0, 0, 0};
+}
+void refreshWallPower(const R& r, uint32_t index) {  // :950-955
+    if (index >= r.c.N * r.c.wallCount) return;
+    float4 value = r.wallRows[3 * index + 1];
+    value.w = -dot(xyz(r.wallRows[3 * index]), xyz(r.wallVelocity[index / r.c.N]));
+    r.wallRows[3 * index + 1] = value;
+}
+
 // ---------------------------------------------------------------- kernels
 void beginStep(const R& r, uint32_t id) {  // solver.slang:859-888
     if (id >= r.c.N || r.control[0] != 0) return;
@@ -529,7 +711,7 @@ void beginStep(const R& r, uint32_t id) {  // solver.slang:859-888
     compensated3(v, vl, 0.5f * dt * xyz(r.force[id]) / r.body[id].y);
     compensated3(l, ll, 0.5f * dt * xyz(r.torque[id]));
     float loss = -expm1Stable(-r.c.drag * dt) * (0.5f * r.body[id].y * dot(v + vl, v + vl) + rotationalEnergy(r, id, l + ll));
-    accumulate(r, id, {0, loss, 0.5f * dt * cellPower(r, id), 0.0f});  // wallStepWork = 0 without walls
+    accumulate(r, id, {0, loss, 0.5f * dt * cellPower(r, id), wallStepWork(r, id, 0.5f * dt)});
     float damping = expm1Stable(-0.5f * r.c.drag * dt);
     compensated3(v, vl, damping * (v + vl));
     compensated3(l, ll, damping * (l + ll));
@@ -586,7 +768,7 @@ void finishStep(const R& r, uint32_t id) {  // :906-920
     compensated3(v, vl, 0.5f * r.c.dt * xyz(r.force[id]) / r.body[id].y);
     compensated3(l, ll, 0.5f * r.c.dt * xyz(r.torque[id]));
     float loss = -expm1Stable(-r.c.drag * r.c.dt) * (0.5f * r.body[id].y * dot(v + vl, v + vl) + rotationalEnergy(r, id, l + ll));
-    accumulate(r, id, {0, loss, 0.5f * r.c.dt * cellPower(r, id), 0.0f});
+    accumulate(r, id, {0, loss, 0.5f * r.c.dt * cellPower(r, id), wallStepWork(r, id, 0.5f * r.c.dt)});
     float damping = expm1Stable(-0.5f * r.c.drag * r.c.dt);
     compensated3(v, vl, damping * (v + vl));
     compensated3(l, ll, damping * (l + ll));
@@ -746,14 +928,16 @@ inline float3 transported(float3 x, float3 a, float3 b, float spin) {  // :691-6
     out = out * cos32(spin) + cross(b, out) * sin32(spin) + b * dot(b, out) * (1.0f - cos32(spin));
     return out - b * dot(out, b);
 }
-void resolveContacts(const R& r, uint32_t id) {  // :703-749 (grain-grain contacts)
+void resolveContacts(const R& r, uint32_t id) {  // :703-749
     if (id >= std::min(r.control[2], r.c.contactCapacity) || r.control[0] != 0) return;
     uint4 key = r.meta[id];
     uint32_t i = key.x, j = key.y;
     float3 n = xyz(r.normalArea[id]);
+    bool wall = j >= r.c.N;
     float viscosity = r.c.normalViscosity;
-    float3 wi = viscosity > 0 ? xyz(r.forceOmega[i]) : omega(r, i);
-    float3 wj = viscosity > 0 ? xyz(r.forceOmega[j]) : omega(r, j);
+    if (wall) viscosity = r.wallMaterial[j - r.c.N].w;
+    float3 wi = viscosity > 0 ? xyz(r.forceOmega[i]) : omega(r, i), wj = {0, 0, 0};
+    if (!wall) wj = viscosity > 0 ? xyz(r.forceOmega[j]) : omega(r, j);
     float3 elastic = {0, 0, 0};
     uint32_t slot = hashKey(r, key);
     if (r.c.historyStep != 0) {
@@ -769,6 +953,26 @@ void resolveContacts(const R& r, uint32_t id) {  // :703-749 (grain-grain contac
         }
     }
     float3 arm = xyz(r.pointDepth[id]);
+    if (wall) {
+        uint32_t w = j - r.c.N;
+        float4 material = r.wallMaterial[w];
+        float3 v = viscosity > 0 ? xyz(r.forceVelocity[i]) : xyz(r.velocity[i]) + xyz(r.velocityRemainder[i]);
+        float3 rv = v + cross(wi, arm) - xyz(r.wallVelocity[w]);
+        PlaneResponse response = planeViscousResponse(n, -r.pointDepth[id].w, r.normalArea[id].w, rv, elastic, material.x,
+                                                      material.y, material.z, material.w, r.c.dt);
+        r.slip[id] = response.slipLoss;
+        r.contactForce[id] = response.forceEnergy;
+        r.normalLossPower[id] = response.normalLossPower;
+        float4 fe = response.forceEnergy;
+        if (!std::isfinite(fe.x) || !std::isfinite(fe.y) || !std::isfinite(fe.z) || !std::isfinite(fe.w) ||
+            !std::isfinite(response.normalLossPower)) { fail(r, 512); return; }
+        if (response.sliding) r.control[6] += 1;
+        uint32_t oldHead = r.heads[i];
+        r.heads[i] = 2 * id;
+        r.next[2 * id] = oldHead;
+        r.next[2 * id + 1] = EMPTY;
+        return;
+    }
     float3 branch = centerBranch(r, i, j);
     float3 translation = xyz(r.velocity[i]) - xyz(r.velocity[j]);
     if (viscosity > 0) translation = xyz(r.forceVelocity[i]) - xyz(r.forceVelocity[j]);
@@ -813,7 +1017,7 @@ void buildHistoryHash(const R& r, uint32_t id) {  // :751-759
     }
     fail(r, 64);
 }
-void gather(const R& r, uint32_t id) {  // :761-811 (grain-grain contacts)
+void gather(const R& r, uint32_t id) {  // :761-811
     if (id >= r.c.N || r.control[0] != 0) return;
     float3 f = {0, 0, 0}, t = {0, 0, 0}, fLow = {0, 0, 0}, tLow = {0, 0, 0};
     float3 stress0 = {0, 0, 0}, stress1 = {0, 0, 0}, stress2 = {0, 0, 0};
@@ -831,13 +1035,30 @@ void gather(const R& r, uint32_t id) {  // :761-811 (grain-grain contacts)
         uint4 key = r.meta[contact];
         float3 force = xyz(r.contactForce[contact]);
         float3 arm = xyz(r.pointDepth[contact]);
+        bool wall = key.y >= r.c.N;
         if (r.c.activeTimestep != 0) {
             float3 localArm = source ? arm : arm - centerBranch(r, key.x, key.y);
             float mobility = 1.0f / r.body[id].y + dot(localArm, localArm) * inverseInertia;
-            float area = r.normalArea[contact].w * 0.5f;
-            float coupling = 2.0f;
-            stiffnessRate += coupling * area * (r.c.kn * (1.0f + r.c.mu) + r.c.kt) * mobility;
-            viscousRate += coupling * area * r.c.normalViscosity * (1.0f + r.c.mu) * mobility;
+            float4 material = {r.c.kn, r.c.kt, r.c.mu, r.c.normalViscosity};
+            if (wall) material = r.wallMaterial[key.y - r.c.N];
+            float area = r.normalArea[contact].w * (wall ? 1.0f : 0.5f);
+            float coupling = wall ? 1.0f : 2.0f;
+            stiffnessRate += coupling * area * (material.x * (1.0f + material.z) + material.y) * mobility;
+            viscousRate += coupling * area * material.w * (1.0f + material.z) * mobility;
+        }
+        if (wall) {
+            compensated3(f, fLow, force);
+            compensated3(t, tLow, cross(arm, force));
+            stress0 = stress0 - force.x * arm;
+            stress1 = stress1 - force.y * arm;
+            stress2 = stress2 - force.z * arm;
+            energy += r.contactForce[contact].w;
+            denom += length(force);
+            maxDepth = std::fmax(maxDepth, r.pointDepth[contact].w / r.body[key.x].w);
+            dissipation += r.slip[contact].w;
+            normalPower += r.normalLossPower[contact];
+            entry = r.next[entry];
+            continue;
         }
         float3 branch = centerBranch(r, key.x, key.y);
         compensated3(f, fLow, source ? force : -force);
@@ -867,7 +1088,7 @@ void gather(const R& r, uint32_t id) {  // :761-811 (grain-grain contacts)
     r.particleDiagnostic[id] = {kinetic, energy, length(f), denom};
     accumulate(r, id, {dissipation, 0, 0, 0});
 }
-void checkContactTimestep(const R& r, uint32_t id) {  // :816-832 (no walls)
+void checkContactTimestep(const R& r, uint32_t id) {  // :816-832
     if (id >= r.c.N || r.control[0] != 0 || r.c.dt == 0) return;
     float dt = r.c.dt;
     float rx = r.timestepRows[id].x, ry = r.timestepRows[id].y;
@@ -878,6 +1099,7 @@ void checkContactTimestep(const R& r, uint32_t id) {  // :816-832 (no walls)
                                      dot(fabs3(xyz(r.invI[3 * id + 2])), one));
     float spin = inverseInertia * (length(xyz(r.momentum[id]) + xyz(r.momentumRemainder[id])) + dt * length(xyz(r.torque[id])));
     float wallSpeed = 0;
+    for (uint32_t w = 0; w < r.c.wallCount; w++) wallSpeed = std::fmax(wallSpeed, length(xyz(r.wallVelocity[w])));
     float motion = dt * (speed + r.body[id].z * spin + wallSpeed);
     bool finite = std::isfinite(rx) && std::isfinite(ry) && std::isfinite(motion) && std::isfinite(spin);
     if (!finite) { fail(r, 512); return; }
@@ -954,8 +1176,9 @@ extern "C" int oracle_dispatch(const char* kernel, uint32_t count, void** buffer
                                const double* values) {
     try {
         R r = bind(buffers, bytes, values);
-        if (r.c.meshGeometry == 0 || r.c.wallCount != 0 || r.c.sourceSurfaceNormal != 0) return 3;
+        if (r.c.meshGeometry == 0 || r.c.sourceSurfaceNormal != 0) return 3;
         std::string name(kernel);
+        const uint32_t wallTasks = r.c.N * r.c.wallCount;
         auto each = [&](auto fn) { for (uint32_t id = 0; id < count; id++) fn(r, id); };
         if (name == "beginStep") each(beginStep);
         else if (name == "advanceCell") advanceCell(r);
@@ -979,6 +1202,10 @@ extern "C" int oracle_dispatch(const char* kernel, uint32_t count, void** buffer
         else if (name == "finishStep") each(finishStep);
         else if (name == "captureContactKinematics") each(captureContactKinematics);
         else if (name == "checkContactTimestep") each(checkContactTimestep);
+        else if (name == "advanceWalls") advanceWalls(r);
+        else if (name == "collectWallContacts") { for (uint32_t id = 0; id < wallTasks; id++) collectWallContacts(r, id); }
+        else if (name == "gatherWallRows") { for (uint32_t id = 0; id < wallTasks; id++) gatherWallRows(r, id); }
+        else if (name == "refreshWallPower") { for (uint32_t id = 0; id < wallTasks; id++) refreshWallPower(r, id); }
         else return 2;
         return 0;
     } catch (...) {

@@ -18,7 +18,7 @@ from dataclasses import replace
 import numpy as np
 
 from .geometry import pack_meshes, sample_arrays
-from .spec import finite_array
+from .spec import box_extents, finite_array
 
 ERRORS = {  # gpu.py:11-26
     2: "degenerate contact distance gradient",
@@ -48,6 +48,7 @@ RESIDUAL_STATE = (
     "cellLog",
 )
 STATE = ("position", "velocity", "rotation", "momentum", "cell", "force", "torque", "accumulated", *RESIDUAL_STATE)
+WALL_STATE = ("wallPlane", "wallRemainder", "wallVelocity", "wallAccumulated", "wallRows")  # gpu.py:32-38
 
 # Field order of the kernels' Resources block (solver.slang:20-100) = C ABI slots.
 BUFFER_ORDER = (
@@ -112,6 +113,7 @@ class Engine:
         self.spec = spec
         self.shapes = list(shapes)
         self.n = len(positions)
+        self.wall_count = len(spec.planes)
         self.viscous = spec.viscous
         self.step = 0
         self.history = False
@@ -168,6 +170,7 @@ class Engine:
             + 1184 * self.n
         )
         estimated += sum(224 * len(s.faces) + 736 + 8 * len(s.samples) for s in self.shapes)
+        estimated += 160 * self.wall_count * self.n + 256 * self.wall_count
         if estimated > spec.working_memory_gib * 1024**3:
             raise MemoryError(f"Estimated working set {estimated / 1024**3:.2f} GiB exceeds budget")
         self.estimated_bytes = estimated
@@ -197,9 +200,9 @@ class Engine:
         self.allocate("fabricRows", 2 * self.pair_capacity, 16)
         self.add("cell", np.array([list(box) + [0], [0, 0, 0, 0], list(box) + [0]], np.float32))
         self.initial_cell = np.asarray(cell, float)
-        for name in ("wallPlane", "wallRemainder", "wallVelocity", "wallMaterial", "wallAccumulated", "wallRows"):
-            self.add(name, np.zeros((1, 4), np.float32))
-        self.add("wallBoxIds", np.zeros(6, np.uint32))
+        if spec.boundary_box is not None:
+            self.initial_cell = box_extents(np.array([p["offset"] for p in spec.planes]), spec.boundary_box, cell)
+        self._wall_buffers()
         self.add("referenceCell", np.array([list(box) + [0], [0, 0, 0, self.cutoff]], np.float32))
         ctrl = np.zeros(16, np.uint32)
         ctrl[4] = 1
@@ -234,6 +237,7 @@ class Engine:
             "referencePosition", "referenceCell", "control", "pairs", "pairCounts", "pairOffsets",
             "particleStress", "particleDiagnostic", "timestepRows", "accumulated",
             "referencePositionRemainder", *RESIDUAL_STATE, *DAMPING_STATE,
+            *(WALL_STATE if self.wall_count else ()),
         )
         self.backups = {name: self.m.buffer(count=self.buffers[name].size // 4) for name in names}
         self.history_backup = {
@@ -269,6 +273,41 @@ class Engine:
 
     def allocate(self, name, count, stride=4):
         self.buffers[name] = self.m.buffer(count=count, stride=stride)
+
+    def _wall_buffers(self):  # gpu.py:232-285
+        spec, l = self.spec, self.length_unit
+        wall_plane = np.zeros((max(1, self.wall_count), 4), np.float64)
+        wall_velocity = np.zeros_like(wall_plane)
+        wall_material = np.zeros_like(wall_plane)
+        for index, plane in enumerate(spec.planes):
+            wall_plane[index] = [*plane["normal"], plane["offset"] / l]
+            wall_velocity[index, :3] = np.array(plane["velocity"]) * self.time_unit / l
+            wall_material[index, :3] = [
+                plane["normal_stiffness"] / spec.normal_stiffness,
+                plane["tangential_stiffness"] / spec.normal_stiffness,
+                plane["friction"],
+            ]
+            wall_material[index, 3] = plane.get("normal_viscosity", 0) / (spec.normal_stiffness * self.time_unit)
+        rounded_plane = wall_plane.astype(np.float32)
+        residual_plane = (wall_plane - rounded_plane).astype(np.float32)
+        if (
+            not np.isfinite(rounded_plane).all()
+            or not np.isfinite(residual_plane).all()
+            or not np.isfinite(wall_velocity.astype(np.float32)).all()
+            or not np.isfinite(wall_material.astype(np.float32)).all()
+            or (self.wall_count and np.any(wall_material[: self.wall_count, :2].astype(np.float32) <= 0))
+        ):
+            raise ValueError("Plane data cannot be represented in normalized FP32 units")
+        for index, plane in enumerate(spec.planes):
+            if plane.get("normal_viscosity", 0) > 0 and np.float32(wall_material[index, 3]) <= 0:
+                raise ValueError("Plane viscosity cannot be represented in normalized FP32 units")
+        self.add("wallPlane", rounded_plane)
+        self.add("wallRemainder", residual_plane)
+        self.add("wallVelocity", wall_velocity.astype(np.float32))
+        self.add("wallMaterial", wall_material.astype(np.float32))
+        self.add("wallBoxIds", np.array(spec.boundary_box or [0] * 6, np.uint32))
+        self.add("wallAccumulated", np.zeros((max(1, self.wall_count * self.n), 4), np.float32))
+        self.add("wallRows", np.zeros((max(1, 3 * self.wall_count * self.n), 4), np.float32))
 
     def _static_geometry(self):  # gpu.py:421-488, mesh branch
         mesh = pack_meshes(self.shapes, self.length_unit)
@@ -315,8 +354,8 @@ class Engine:
             "meshGeometry": 1,
             "sourceSurfaceNormal": 0,
             "periodic": int(spec.periodic),
-            "wallCount": 0,
-            "wallBoxEnabled": 0,
+            "wallCount": self.wall_count,
+            "wallBoxEnabled": int(self.spec.boundary_box is not None),
         }
 
     def bindings(self, dt=None, drag=0):  # gpu.py:490-527
@@ -346,10 +385,14 @@ class Engine:
         self.dispatch("resetMeshContacts", self.pair_capacity, e, b)
         self.dispatch("prepareMeshPairs", self.pair_capacity, e, b)
         self.dispatch("collectMeshFrontiers", 64 * 4096, e, b)
+        if self.wall_count:
+            self.dispatch("collectWallContacts", 64 * self.n * self.wall_count, e, b)
         self.dispatch("clearContactStorage", max(self.n, self.hash_size), e, b)
         self.dispatch("resolveContacts", self.contact_capacity, e, b)
         self.dispatch("buildHistoryHash", self.contact_capacity, e, b)
         self.dispatch("gather", self.n, e, b)
+        if self.wall_count:
+            self.dispatch("gatherWallRows", self.n * self.wall_count, e, b)
         if self.spec.timestep_policy == "active-contact":
             self.dispatch("checkContactTimestep", self.n, e, b)
 
@@ -370,7 +413,7 @@ class Engine:
             raise RejectedStep(ctrl[0])
         return ctrl
 
-    def advance(self, steps, *, rate=(0, 0, 0), drag=0):  # gpu.py:597-647
+    def advance(self, steps, *, rate=(0, 0, 0), drag=0, wall_velocities=None):  # gpu.py:597-647
         if type(steps) is not int or steps < 0:
             raise ValueError("Nonnegative integer step count required")
         rate = finite_array(rate, (3,))
@@ -378,9 +421,13 @@ class Engine:
             raise ValueError("Affine cell rates require a periodic domain")
         if not np.isfinite(drag) or drag < 0:
             raise ValueError("Finite nonnegative drag required")
+        if wall_velocities is not None:
+            if not self.wall_count:
+                raise ValueError("Wall velocities require planes")
+            wall_velocities = finite_array(wall_velocities, (self.wall_count, 3))
         for attempt in range(5):
             try:
-                return self._advance_once(steps, rate=rate, drag=drag)
+                return self._advance_once(steps, rate=rate, drag=drag, wall_velocities=wall_velocities)
             except RejectedStep as exc:
                 capacity = exc.flags & CAPACITY_FLAGS
                 if not capacity or exc.flags & ~CAPACITY_FLAGS or attempt == 4:
@@ -446,13 +493,15 @@ class Engine:
                 self.dispatch("checkContactTimestep", self.n, e, b)
             self.dispatch("beginStep", self.n, e, b)
             self.dispatch("advanceCell", 1, e, b)
+            if self.wall_count:
+                self.dispatch("advanceWalls", 1, e, b)
             self.dispatch("checkRebuild", self.n, e, b)
             self.forces(e, b)
             self.dispatch("finishStep", self.n, e, b)
             self.active = 1 - self.active
             self.history = True
 
-    def _advance_once(self, steps, *, rate=(0, 0, 0), drag=0):  # gpu.py:672-744
+    def _advance_once(self, steps, *, rate=(0, 0, 0), drag=0, wall_velocities=None):  # gpu.py:672-744
         if getattr(self, "invalid", False):
             raise RuntimeError("Engine is invalid; restore a checkpoint")
         if steps < 1:
@@ -461,13 +510,28 @@ class Engine:
         cell = self.m.read(self.buffers["cell"], shape=(3, 4))
         original_cell = cell.copy()
         cell[1, :3] = np.asarray(rate) * self.time_unit
+        original_wall_velocity = normalized_velocity = None
+        if wall_velocities is not None:
+            original_wall_velocity = self.m.read(self.buffers["wallVelocity"], shape=(self.wall_count, 4))
+            normalized_velocity = f4(wall_velocities * self.time_unit / self.length_unit)
         self.buffers["cell"].copy_from_numpy(cell)
+        if normalized_velocity is not None:
+            self.buffers["wallVelocity"].copy_from_numpy(normalized_velocity)
         e = self.m.device.create_command_encoder()
         for name, backup in self.backups.items():
             e.copy_buffer(backup, 0, self.buffers[name], 0, self.buffers[name].size)
         original_active, original_history = self.active, self.history
         committed = self.history_buffers[1 - self.active]
         self._copy_history_backup(e, committed, history_count)
+        if original_wall_velocity is not None:
+            if any(p.get("normal_viscosity", 0) > 0 for p in self.spec.planes) and not np.array_equal(
+                original_wall_velocity, normalized_velocity
+            ):
+                self.forces(e, self.bindings(dt=0))
+                self.active = 1 - self.active
+                self.history = True
+            else:
+                self.dispatch("refreshWallPower", self.n * self.wall_count, e, self.bindings())
         self._encode_steps(e, steps, drag)
         self.m.submit(e)
         self.m.device.wait()
@@ -481,6 +545,8 @@ class Engine:
             self.m.submit(e)
             self.m.device.wait()
             self.buffers["cell"].copy_from_numpy(original_cell)
+            if original_wall_velocity is not None:
+                self.buffers["wallVelocity"].copy_from_numpy(original_wall_velocity)
             self.active, self.history = original_active, original_history
             raise
         self.step += steps
@@ -493,6 +559,11 @@ class Engine:
         if self.viscous:
             for name in DAMPING_STATE:
                 out[name] = self.m.read(self.buffers[name], shape=(self.n, 4))
+        if self.wall_count:
+            for name in WALL_STATE:
+                size = self.wall_count * self.n * (3 if name == "wallRows" else 1) if name in {
+                    "wallRows", "wallAccumulated"} else self.wall_count
+                out[name] = self.m.read(self.buffers[name], shape=(size, 4))
         ctrl = self.check_errors()
         h = self.history_buffers[1 - self.active]
         count = int(ctrl[2])
@@ -507,6 +578,19 @@ class Engine:
         if self.viscous:
             for name in DAMPING_STATE:
                 finite_array(state[name], (self.n, 4))
+        if self.wall_count:
+            for name in WALL_STATE:
+                value = finite_array(state[name])
+                if value.shape != (self.buffers[name].size // 16, 4):
+                    raise ValueError("Checkpoint wall state shape mismatch")
+            normals = np.array([plane["normal"] for plane in self.spec.planes], np.float32)
+            if not np.array_equal(state["wallPlane"][:, :3], normals):
+                raise ValueError("Checkpoint plane normal identity mismatch")
+            offsets = (np.asarray(state["wallPlane"][:, 3], float) + state["wallRemainder"][:, 3]) * self.length_unit
+            if self.spec.boundary_box is not None:
+                box_extents(offsets, self.spec.boundary_box, np.asarray(state["cell"][0, :3], float) * self.length_unit)
+            for name in WALL_STATE:
+                self.buffers[name].copy_from_numpy(np.asarray(state[name], np.float32))
         for name in DAMPING_STATE:
             self.buffers[name].copy_from_numpy(state[name] if self.viscous else np.zeros((self.n, 4), np.float32))
         for name in STATE:
@@ -553,6 +637,30 @@ class Engine:
             self.buffers["velocity"].copy_from_numpy(f4(velocity))
         if momentum is not None:
             self.buffers["momentum"].copy_from_numpy(f4(momentum))
+
+    def walls(self):
+        """Per-plane totals of the last force evaluation in SI units (host FP64 sums of
+        the per-body wall rows; the reference reduces the same rows on the device,
+        diagnostic.slang / gpu.py:774-835)."""
+        if not self.wall_count:
+            return []
+        rows = self.m.read(self.buffers["wallRows"], shape=(self.wall_count, self.n, 3, 4)).astype(np.float64)
+        counts = self.m.read(self.buffers["wallRows"], np.uint32, shape=(self.wall_count, self.n, 3, 4))[:, :, 0, 3]
+        plane = self.m.read(self.buffers["wallPlane"], shape=(self.wall_count, 4)).astype(np.float64)
+        rem = self.m.read(self.buffers["wallRemainder"], shape=(self.wall_count, 4)).astype(np.float64)
+        l, f = self.length_unit, self.force_unit
+        out = []
+        for w in range(self.wall_count):
+            out.append({
+                "normal": plane[w, :3].tolist(),
+                "offset": float((plane[w, 3] + rem[w, 3]) * l),
+                "reaction": (rows[w, :, 0, :3].sum(0) * f).tolist(),
+                "reaction_moment_origin": (rows[w, :, 1, :3].sum(0) * f * l).tolist(),
+                "power": float(rows[w, :, 1, 3].sum() * f * l / self.time_unit),
+                "work": float(rows[w, :, 2, 0].sum() * self.energy_unit),
+                "active_surface_contacts": int(counts[w].sum()),
+            })
+        return out
 
     def read(self, name, dtype=np.float32, rows=4):
         """Whole buffer as an array of `rows`-wide records."""
