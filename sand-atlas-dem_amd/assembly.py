@@ -68,12 +68,14 @@ def uneven_library(count, *, tri_range=(320, 5120), node_range=(256, 4096), surf
     return out
 
 
-def sphere_packing(radii, box, rng, *, iterations=400, tolerance=1e-3):
-    """Periodic overlap-relaxed sphere packing (growth from half size).
+def sphere_packing(radii, box, rng, *, iterations=400, tolerance=1e-3, walled=False, clearance=None):
+    """Overlap-relaxed sphere packing (growth from half size), periodic or walled.
 
     Radii grow from 0.5x to 1x over the first half of the iterations while pairwise
     overlaps are pushed apart symmetrically; stops once the largest overlap is below
-    `tolerance` times the mean diameter.
+    `tolerance` times the mean diameter.  `walled` keeps every centre inside
+    [c, box - c] on each axis instead of wrapping, with c the sphere radius or the
+    given `clearance` (e.g. the grain's bounding radius), scaled like the radii.
     """
     from scipy.spatial import cKDTree
 
@@ -81,17 +83,26 @@ def sphere_packing(radii, box, rng, *, iterations=400, tolerance=1e-3):
     x = rng.uniform(0, box, (n, 3))
     mean = 2 * radii.mean()
     grow = max(1, iterations // 2)
+
+    reach = radii if clearance is None else np.asarray(clearance, float)
+
+    def confine(x, scale):
+        c = reach[:, None] * scale
+        return np.clip(x, c, box - c) if walled else np.mod(x, box)
+
     for it in range(iterations):
         scale = min(1.0, 0.5 + 0.5 * (it + 1) / grow)
         r = radii * scale
-        tree = cKDTree(x, boxsize=box)
+        x = confine(x, scale)
+        tree = cKDTree(x, boxsize=None if walled else box)
         pairs = tree.query_pairs(2 * r.max(), output_type="ndarray")
         if len(pairs) == 0:
             if scale >= 1:
                 break
             continue
         d = x[pairs[:, 1]] - x[pairs[:, 0]]
-        d -= np.round(d / box) * box
+        if not walled:
+            d -= np.round(d / box) * box
         dist = np.linalg.norm(d, axis=1)
         overlap = r[pairs[:, 0]] + r[pairs[:, 1]] - dist
         hit = overlap > 0
@@ -103,8 +114,8 @@ def sphere_packing(radii, box, rng, *, iterations=400, tolerance=1e-3):
         move = np.zeros_like(x)
         np.add.at(move, pairs[:, 0], -push)
         np.add.at(move, pairs[:, 1], push)
-        x = np.mod(x + move, box)
-    return x
+        x = confine(x + move, scale)
+    return confine(x, 1.0)
 
 
 @dataclass
@@ -159,6 +170,56 @@ def packed_workload(name, shapes, n, phi, *, seed, sphere_fraction=None, size_ra
     return Workload(name, shapes, spec, x, np.full(3, box), ids, scales, q, v, tuple(rate), steps)
 
 
+def box_planes(low, high, *, stiffness, friction, viscosity=0.0, velocities=None):
+    """Six inward axis planes xmin,xmax,ymin,ymax,zmin,zmax bounding [low, high]
+    (the reference `boundary_box` order, `boundaries.py:14-16, 45-55`)."""
+    planes = []
+    for axis in range(3):
+        for side, sign in ((low, 1.0), (high, -1.0)):
+            normal = [0.0, 0.0, 0.0]
+            normal[axis] = sign
+            plane = dict(normal=normal, offset=sign * float(side[axis]), velocity=[0.0, 0.0, 0.0],
+                         normal_stiffness=stiffness, tangential_stiffness=0.5 * stiffness, friction=friction)
+            if viscosity:
+                plane["normal_viscosity"] = viscosity
+            planes.append(plane)
+    for index, v in (velocities or {}).items():
+        planes[index]["velocity"] = [float(c) for c in v]
+    return planes
+
+
+def walled_workload(name, shapes, n, phi, *, seed, sphere_fraction=None, size_range=(0.8, 1.2), axial_rate=0.5,
+                    margin=0.05, steps=1000, spec_overrides=None, rng=None, ids=None, scales=None, viscosity=0.0):
+    """Grains packed inside a box of six planes in a nonperiodic cell.
+
+    The top plane (zmax) translates down at `axial_rate` times the box height per
+    second; the other five are fixed.  The cell is the box plus `margin` of its
+    side on every face, so grains stay inside the computational domain.
+    """
+    rng = rng or streams(seed)
+    if ids is None:
+        ids = rng["shapes"].integers(0, len(shapes), n).astype(np.uint32)
+    if scales is None:
+        scales = rng["sizes"].uniform(*size_range, n)
+    d = np.array([shapes[i].equivalent_diameter for i in ids]) * scales
+    side = (float(np.sum(np.pi / 6 * d**3)) / phi) ** (1 / 3)
+    alpha = ((sphere_fraction or phi) / phi) ** (1 / 3)
+    # Centres sit between the contact-sphere and bounding radii from the walls: some
+    # grains start touching a wall, with overlaps well below the pair overlaps.
+    bounding = 0.5 * (0.5 * alpha * d + np.array([shapes[i].radius for i in ids]) * scales)
+    x = sphere_packing(0.5 * alpha * d, side, rng["positions"], walled=True, clearance=bounding)
+    pad = margin * side
+    low, high = np.full(3, pad), np.full(3, pad + side)
+    overrides = dict(spec_overrides or {})
+    planes = box_planes(low, high, stiffness=overrides["normal_stiffness"], friction=overrides.get("friction", 0.5),
+                        viscosity=viscosity, velocities={5: (0.0, 0.0, -axial_rate * side)})
+    spec = Experiment(contact_geometry="mesh", particles=n, periodic=False, planes=planes,
+                      boundary_box=[0, 1, 2, 3, 4, 5], **overrides)
+    q = random_rotations(n, rng["orientations"])
+    return Workload(name, shapes, spec, x + pad, np.full(3, side + 2 * pad), ids, scales, q, np.zeros((n, 3)),
+                    (0, 0, 0), steps)
+
+
 def conservative_timestep(shapes, ids, scales, spec):
     """Largest dt the conservative preflight accepts (`gpu.py:370-413`), rounded down."""
     limits = []
@@ -184,6 +245,8 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
              triangles log-uniform, 256-4,096 nodes by area), phi 0.60
     concave: 1,000 grains from 8 strongly non-convex low-degree harmonic templates
              (1,280 triangles, 1,024 nodes), phi 0.55 -- loose hull bounds
+    walls  : 10,000 grains from the dense library at phi 0.58 in a nonperiodic box
+             of six planes, the top plane moving down at 0.5 box heights per second
     """
     rng = streams(seed)
     d0 = 3e-4
@@ -223,6 +286,13 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
         phi, sphere_fraction, vsig, rate = 0.55, 0.57, 0.0, (0, 0, 0)
         steps = steps or 1000
         cpp = 512
+    elif config == "walls":
+        n = n or 10_000
+        shapes = icosphere_library(templates or 64, subdivisions=(2, 3, 4), sigma=0.15, diameter=d0,
+                                   nodes=nodes or 2048, rng=rng["templates"], seed=seed)
+        phi, sphere_fraction = 0.58, 0.60
+        steps = steps or 5000
+        cpp = 512
     else:
         raise ValueError(f"unknown workload {config!r}")
     ids = rng["shapes"].integers(0, len(shapes), n).astype(np.uint32)
@@ -230,6 +300,9 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
     scales = rng["sizes"].uniform(0.8, 1.2, n)
     dt = conservative_timestep(shapes, ids, scales, probe)
     overrides = dict(base, timestep=dt, max_overlap_fraction=0.5, contacts_per_particle=cpp)
+    if config == "walls":
+        return walled_workload(config, shapes, n, phi, seed=seed, sphere_fraction=sphere_fraction, steps=steps,
+                               spec_overrides=overrides, rng=rng, ids=ids, scales=scales)
     return packed_workload(config, shapes, n, phi, seed=seed, sphere_fraction=sphere_fraction,
                            velocity_sigma=vsig, rate=rate, steps=steps, spec_overrides=overrides, rng=rng,
                            ids=ids, scales=scales)

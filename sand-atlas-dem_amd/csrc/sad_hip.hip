@@ -72,6 +72,10 @@ struct P {
     const uint32_t* oldHash;
     uint32_t* hash;
     float4 *particleStress, *timestepRows, *particleDiagnostic, *accumulated;
+    // Translating planes (c.wallCount rows; per-(wall, grain) rows at wall * N + grain).
+    float4 *wallPlane, *wallRemainder, *wallRows, *wallAccumulated;
+    const float4 *wallVelocity, *wallMaterial;
+    const uint32_t* wallBoxIds;
 };
 
 // Library workspace (device arrays), grown on demand.
@@ -95,12 +99,14 @@ struct W {
     uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue, ascending target template within a queue
     uint32_t* keyCount;   // tasks per (target rank, source) key
     uint32_t* keyStart;   // exclusive scan of keyCount; consumed as fill cursors
+    uint32_t* wallCount;  // plane contacts per (wall, grain) task, task = wall * N + grain
+    uint32_t* wallOffset; // exclusive scan of wallCount (ids follow the mesh contacts)
     // Learned cull (nn.h): per-template BF16 weights and FP32 blocks; stats on separate lines.
     uint32_t nnEnabled, nnAudit, nnTemplates;
     const uint16_t* nnW;
     const float* nnF;
     unsigned long long* nnStats;  // [0] culled, [LINE/2] audited culled nodes, [LINE] audit misses
-    uint32_t stageCap, taskCap, nCap, sumsCap;
+    uint32_t stageCap, taskCap, nCap, sumsCap, wallCap;
     uint32_t pop;         // tasks taken per queue atomic (tunable)
 };
 
@@ -176,6 +182,58 @@ __device__ float cellPower(const P& p, uint32_t i) {
     v3 diagonal = {p.particleStress[3 * i].x, p.particleStress[3 * i + 1].y, p.particleStress[3 * i + 2].z};
     v3 v = xyz(p.velocity[i]);
     return -dot(diagonal + p.body[i].y * v * v, xyz(p.cell[1]));
+}
+
+// ---------------------------------------------------------------- translating planes (solver.slang:135-145, 578-687)
+// Work done on a grain by the walls over dt (solver.slang:135-145): each wall's
+// compensated running total advances by dt times the last gathered wall power.
+__device__ float wallStepWork(const P& p, uint32_t id, float dt) {
+    float total = 0;
+    for (uint32_t wall = 0; wall < p.c.wallCount; wall++) {
+        uint32_t index = wall * p.c.N + id;
+        float increment = dt * p.wallRows[3 * index + 1].w;
+        float4 state = p.wallAccumulated[index];
+        v3 high = {state.x, 0, 0}, low = {state.y, 0, 0};
+        compensated3(high, low, {increment, 0, 0});
+        p.wallAccumulated[index] = make_float4(high.x, low.x, 0, 0);
+        p.wallRows[3 * index + 2] = make_float4(high.x + low.x, 0, 0, 0);
+        total += increment;
+    }
+    return total;
+}
+// Double-float plane distance (solver.slang:578-610): Knuth two-sums with exact
+// FMA product errors, so a far-off plane offset does not swamp the node arm.
+struct f2 { float hi, lo; };
+__device__ __forceinline__ f2 wallTwoSum(float a, float b) {
+    float high = a + b, v = high - a;
+    return {high, (a - (high - v)) + (b - v)};
+}
+__device__ __forceinline__ f2 wallAdd(f2 a, float b) {
+    f2 sum = wallTwoSum(a.hi, b);
+    return wallTwoSum(sum.hi, sum.lo + a.lo);
+}
+__device__ __forceinline__ f2 wallAddDot(f2 sum, v3 a, v3 b) {
+    const float pr[3] = {a.x * b.x, a.y * b.y, a.z * b.z};
+    const float er[3] = {__builtin_fmaf(a.x, b.x, -pr[0]), __builtin_fmaf(a.y, b.y, -pr[1]), __builtin_fmaf(a.z, b.z, -pr[2])};
+    for (int axis = 0; axis < 3; axis++) {
+        sum = wallAdd(sum, pr[axis]);
+        sum = wallAdd(sum, er[axis]);
+    }
+    return sum;
+}
+__device__ f2 wallCentreDistance(const P& p, uint32_t body, uint32_t wall) {
+    float4 plane = p.wallPlane[wall], low = p.wallRemainder[wall];
+    v3 x = xyz(p.position[body]), xl = xyz(p.positionRemainder[body]);
+    f2 sum = {-plane.w, -low.w};
+    sum = wallAddDot(sum, xyz(plane), x);
+    sum = wallAddDot(sum, xyz(plane), xl);
+    sum = wallAddDot(sum, xyz(low), x);
+    return wallAddDot(sum, xyz(low), xl);
+}
+__device__ float localWallDistance(f2 centre, float4 plane, float4 low, v3 arm) {
+    f2 sum = wallAddDot(centre, xyz(plane), arm);
+    sum = wallAddDot(sum, xyz(low), arm);
+    return sum.hi + sum.lo;
 }
 __device__ void binIndex(const P& p, uint32_t i, uint32_t b[3]) {
     v3 g = floor3(xyz(p.position[i]) / xyz(p.cell[0]) * mk3(float(p.c.bins[0]), float(p.c.bins[1]), float(p.c.bins[2])));
@@ -414,7 +472,7 @@ __global__ void k_begin_step(P p) {
     compensated3(v, vl, 0.5f * dt * xyz(p.force[id]) / body.y);
     compensated3(l, ll, 0.5f * dt * xyz(p.torque[id]));
     float loss = -expm1Stable(-p.c.drag * dt) * (0.5f * body.y * dot(v + vl, v + vl) + rotationalEnergy(p, id, l + ll));
-    accumulateRow(p, id, {0, loss, 0.5f * dt * cellPower(p, id), 0.0f});
+    accumulateRow(p, id, {0, loss, 0.5f * dt * cellPower(p, id), wallStepWork(p, id, 0.5f * dt)});
     float damping = expm1Stable(-0.5f * p.c.drag * dt);
     compensated3(v, vl, damping * (v + vl));
     compensated3(l, ll, damping * (l + ll));
@@ -456,6 +514,33 @@ __global__ void k_advance_cell(P p) {
     p.control[4] = 0;
     if (p.c.periodic != 0 && p.referenceCell[1].w >= 0.5f * fminf(box.x, fminf(box.y, box.z))) fail(p, 256);
 }
+// advanceWalls (solver.slang:669-687): one thread; each plane offset moves by
+// dt (n . v) in compensated form, then the optional box of six walls must still
+// enclose a positive volume inside the cell.
+__global__ void k_advance_walls(P p) {
+    if (ld_volatile(p.control) != 0) return;
+    for (uint32_t wall = 0; wall < p.c.wallCount; wall++) {
+        float4 plane = p.wallPlane[wall], residual = p.wallRemainder[wall];
+        v3 high = {plane.w, 0, 0}, low = {residual.w, 0, 0};
+        compensated3(high, low, {dot(xyz(plane), xyz(p.wallVelocity[wall])) * p.c.dt, 0, 0});
+        plane.w = high.x;
+        residual.w = low.x;
+        if (!isfinite(high.x) || !isfinite(low.x)) { fail(p, 512); return; }
+        p.wallPlane[wall] = plane;
+        p.wallRemainder[wall] = residual;
+    }
+    if (p.c.wallBoxEnabled != 0) {
+        const float4 box = p.cell[0];
+        const float extent[3] = {box.x, box.y, box.z};
+        for (uint32_t axis = 0; axis < 3; axis++) {
+            uint32_t lo = p.wallBoxIds[2 * axis], hi = p.wallBoxIds[2 * axis + 1];
+            if (lo >= p.c.wallCount || hi >= p.c.wallCount) { fail(p, 4096); return; }  // ids validated by the host
+            float low = p.wallPlane[lo].w + p.wallRemainder[lo].w;
+            float high = -(p.wallPlane[hi].w + p.wallRemainder[hi].w);
+            if (low < 0 || high > extent[axis] || high <= low) fail(p, 4096);
+        }
+    }
+}
 __global__ void k_check_rebuild(P p) {
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= p.c.N) return;
@@ -476,7 +561,7 @@ __global__ void k_finish_step(P p) {
     compensated3(v, vl, 0.5f * p.c.dt * xyz(p.force[id]) / body.y);
     compensated3(l, ll, 0.5f * p.c.dt * xyz(p.torque[id]));
     float loss = -expm1Stable(-p.c.drag * p.c.dt) * (0.5f * body.y * dot(v + vl, v + vl) + rotationalEnergy(p, id, l + ll));
-    accumulateRow(p, id, {0, loss, 0.5f * p.c.dt * cellPower(p, id), 0.0f});
+    accumulateRow(p, id, {0, loss, 0.5f * p.c.dt * cellPower(p, id), wallStepWork(p, id, 0.5f * p.c.dt)});
     float damping = expm1Stable(-0.5f * p.c.drag * p.c.dt);
     compensated3(v, vl, damping * (v + vl));
     compensated3(l, ll, damping * (l + ll));
@@ -486,9 +571,17 @@ __global__ void k_finish_step(P p) {
     p.momentum[id] = f4(l, 0);
     p.momentumRemainder[id] = f4(ll, 0);
 }
-__global__ void k_capture_kinematics(P p) {
+// gated: the whole-step path launches this whenever planes exist and skips the
+// work unless some contact is viscous (spec `viscous`: the grain-grain or any
+// plane's normal viscosity is positive), as the Engine does on the host.
+__global__ void k_capture_kinematics(P p, int gated) {
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= p.c.N || ld_volatile(p.control) != 0) return;
+    if (gated && !(p.c.normalViscosity > 0)) {
+        bool viscous = false;
+        for (uint32_t wall = 0; wall < p.c.wallCount; wall++) viscous |= p.wallMaterial[wall].w > 0;
+        if (!viscous) return;
+    }
     p.forceVelocity[id] = f4(xyz(p.velocity[id]) + xyz(p.velocityRemainder[id]), 0);
     p.forceOmega[id] = f4(omega(p, id), 0);
 }
@@ -497,7 +590,7 @@ __device__ float inverseInertiaBound(const P& p, uint32_t id) {
     return fmaxf(fmaxf(dot(fabs3(xyz(p.invI[3 * id])), one), dot(fabs3(xyz(p.invI[3 * id + 1])), one)),
                  dot(fabs3(xyz(p.invI[3 * id + 2])), one));
 }
-__global__ void k_check_timestep(P p) {  // solver.slang:816-832 without planes
+__global__ void k_check_timestep(P p) {  // solver.slang:816-832
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= p.c.N || ld_volatile(p.control) != 0 || p.c.dt == 0) return;
     float dt = p.c.dt;
@@ -505,7 +598,9 @@ __global__ void k_check_timestep(P p) {  // solver.slang:816-832 without planes
     v3 v = xyz(p.velocity[id]) + xyz(p.velocityRemainder[id]);
     float speed = length(v) + dt * length(xyz(p.force[id])) / p.body[id].y;
     float spin = inverseInertiaBound(p, id) * (length(xyz(p.momentum[id]) + xyz(p.momentumRemainder[id])) + dt * length(xyz(p.torque[id])));
-    float motion = dt * (speed + p.body[id].z * spin + 0.0f);
+    float wallSpeed = 0;
+    for (uint32_t wall = 0; wall < p.c.wallCount; wall++) wallSpeed = fmaxf(wallSpeed, length(xyz(p.wallVelocity[wall])));
+    float motion = dt * (speed + p.body[id].z * spin + wallSpeed);
     if (!(isfinite(rx) && isfinite(ry) && isfinite(motion) && isfinite(spin))) { fail(p, 512); return; }
     v3 rate = fabs3(xyz(p.cell[1])) * dt;
     if (dt * sqrtf(rx) > 0.1f || dt * ry > 0.1f || dt * spin > 0.01f || motion > 0.25f * p.c.skin ||
@@ -1156,6 +1251,91 @@ __global__ void k_scatter(P p, W w) {
     }
 }
 
+// ---------------------------------------------------------------- plane contacts (solver.slang:596-667)
+// collectWallContacts in the CPU's serial order (cpu_scheduling.slang:133-154):
+// task t = wall * N + grain, the grain's sample clusters in preorder, a leaf's
+// nodes in leaf order, ids after every mesh contact.  One wave per task walks the
+// cluster tree under wave-uniform control; a leaf's nodes (<= 64) go one per lane
+// and hits are ranked by ballot.  The count pass sizes each task, a scan gives the
+// offsets, and the emit pass re-walks and writes the same hits.
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_wall_collect(P p, W w) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t task = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (task >= p.c.wallCount * p.c.N) return;
+    if (ld_volatile(p.control) != 0) {
+        if (!EMIT && lane == 0) w.wallCount[task] = 0;
+        return;
+    }
+    const uint32_t body = task % p.c.N, wall = task / p.c.N;
+    const float4 plane = p.wallPlane[wall], planeLow = p.wallRemainder[wall];
+    const f2 centre = wallCentreDistance(p, body, wall);
+    const float phi = centre.hi + centre.lo;
+    const float4 bd = p.body[body];
+    const float lengthX = length(xyz(p.position[body]));
+    const float eps64 = 64.0f * 1.1920928955078125e-7f;
+    uint32_t count = 0;
+    if (!(phi > bd.z + eps64 * (1.0f + lengthX + fabsf(plane.w)))) {
+        const v4 q = v4of(p.rotation[body]);
+        const v3 normal = qinv(q, xyz(plane)) * bd.x;  // wallClusterMayContact (solver.slang:613-623)
+        const float normalLength = length(normal);
+        const uint32_t shape = p.shapeId[body];
+        const uint4 info = p.meshClusterInfo[shape];
+        const uint32_t samples = p.sampleInfo[shape].x;
+        const uint32_t out = EMIT ? p.control[2] + w.wallOffset[task] : 0;
+        for (uint32_t cluster = info.x; cluster < info.x + info.y;) {
+            const v3 low = xyz(p.meshClusterMin[cluster]), high = xyz(p.meshClusterMax[cluster]);
+            const v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
+            const float support = dot(normal, center) - dot(fabs3(normal), half);
+            const float pad = eps64 * (1.0f + fabsf(plane.w) + lengthX + normalLength * (length(center) + length(half)));
+            const bool candidate = phi + support <= pad;
+            const uint4 range = p.meshClusterRanges[cluster];
+            if (candidate && range.y != 0) {  // appendWallContact (solver.slang:625-638) per node
+                bool hit = false;
+                uint32_t node = 0;
+                v3 arm = {0, 0, 0};
+                float depth = 0, area = 0;
+                if (uint32_t(lane) < range.y) {
+                    node = p.meshSampleIndices[range.x + lane];
+                    const float4 point = p.sample[samples + node];
+                    arm = qrot(q, xyz(point) * bd.x);
+                    const float d = localWallDistance(centre, plane, planeLow, arm);
+                    if (!isfinite(d)) fail(p, 512);
+                    else if (d < 0) {
+                        hit = true;
+                        depth = -d;
+                        area = point.w * bd.x * bd.x;
+                        if (depth > p.c.maxOverlap * bd.w) fail(p, 4);
+                    }
+                }
+                const uint64_t mask = __ballot(hit);
+                if (EMIT && hit) {
+                    const uint32_t id = out + count + __popcll(mask & lanes_below(lane));
+                    if (id < p.c.contactCapacity) {
+                        p.meta[id] = make_uint4(body, p.c.N + wall, node, 1);
+                        p.pointDepth[id] = f4(arm, depth);
+                        p.normalArea[id] = make_float4(plane.x, plane.y, plane.z, area);
+                    }
+                }
+                count += __popcll(mask);
+            }
+            cluster = candidate ? cluster + 1 : range.z;
+        }
+    }
+    if (!EMIT && lane == 0) w.wallCount[task] = count;
+}
+// Plane contacts follow the mesh contacts: control[2] grows by their total (flag 32
+// past capacity, as the serial appends); the mesh total is kept for the gather.
+__global__ void k_wall_total(P p, W w) {
+    if (ld_volatile(p.control) != 0) return;
+    const uint32_t base = p.control[2], added = w.counters[7 * LINE];
+    w.counters[8 * LINE] = base;
+    uint64_t total = uint64_t(base) + added;
+    if (total > p.c.contactCapacity) { fail(p, 32); total = p.c.contactCapacity; }
+    p.control[2] = uint32_t(total);
+    atomicAdd(w.totals + 3, (unsigned long long)added);
+}
+
 // ---------------------------------------------------------------- friction history (solver.slang:689-759)
 __device__ __forceinline__ uint32_t hashKey(uint32_t size, uint4 key) {
     uint32_t h = key.x * 73856093u ^ key.y * 19349663u ^ key.z * 83492791u;
@@ -1184,9 +1364,11 @@ __global__ void k_resolve(P p) {
         uint32_t i = key.x, j = key.y;
         float4 na = p.normalArea[id];
         v3 n = xyz(na);
-        float viscosity = p.c.normalViscosity;
+        const bool wall = j >= p.c.N;
+        float viscosity = wall ? p.wallMaterial[j - p.c.N].w : p.c.normalViscosity;
         v3 wi = viscosity > 0 ? xyz(p.forceOmega[i]) : omega(p, i);
-        v3 wj = viscosity > 0 ? xyz(p.forceOmega[j]) : omega(p, j);
+        v3 wj = {0, 0, 0};
+        if (!wall) wj = viscosity > 0 ? xyz(p.forceOmega[j]) : omega(p, j);
         v3 elastic = {0, 0, 0};
         if (p.c.historyStep != 0) {
             uint32_t slot = hashKey(p.c.hashSize, key);
@@ -1203,6 +1385,43 @@ __global__ void k_resolve(P p) {
         }
         float4 pdv = p.pointDepth[id];
         v3 arm = xyz(pdv);
+        if (wall) {  // planeViscousResponse (plane.slang:13-44), normalResponse weight 1
+            const uint32_t k = j - p.c.N;
+            const float4 material = p.wallMaterial[k];
+            v3 v = viscosity > 0 ? xyz(p.forceVelocity[i]) : xyz(p.velocity[i]) + xyz(p.velocityRemainder[i]);
+            v3 rv = v + cross(wi, arm) - xyz(p.wallVelocity[k]);
+            float fn = 0, lossPower = 0;
+            float4 forceEnergy = {0, 0, 0, 0}, slipLoss = {0, 0, 0, 0};
+            bool sliding = false;
+            if (!(-pdv.w >= 0)) {
+                const float depth = -(-pdv.w), area = na.w, kn = material.x, kt = material.y, mu = material.z;
+                v3 tangent = rv - n * dot(rv, n);
+                v3 trial = elastic - n * dot(elastic, n) + p.c.dt * tangent;
+                v3 slip = trial;
+                float normalVelocity = dot(rv, n);
+                float el = 1.0f * kn * area * depth;
+                fn = viscosity == 0 ? el : fmaxf(0.0f, el - 1.0f * viscosity * area * normalVelocity);
+                lossPower = fmaxf(0.0f, (el - fn) * normalVelocity);
+                v3 ft = -kt * area * slip;
+                float magnitude = length(ft);
+                if (magnitude > mu * fn && magnitude > 0) {
+                    sliding = true;
+                    ft = ft * (mu * fn / magnitude);
+                    slip = -ft / (kt * area);
+                }
+                forceEnergy = f4(fn * n + ft, 0.5f * area * (kn * depth * depth + kt * dot(slip, slip)));
+                slipLoss = f4(slip, fmaxf(0.0f, -dot(ft, trial - slip)));
+            }
+            p.slip[id] = slipLoss;
+            p.contactForce[id] = forceEnergy;
+            p.normalLossPower[id] = lossPower;
+            if (!isfinite(forceEnergy.x) || !isfinite(forceEnergy.y) || !isfinite(forceEnergy.z) ||
+                !isfinite(forceEnergy.w) || !isfinite(lossPower))
+                fail(p, 512);
+            else if (sliding)
+                atomicAdd(p.control + 6, 1u);
+            continue;
+        }
         v3 branch = centerBranch(p, i, j);
         v3 translation = xyz(p.velocity[i]) - xyz(p.velocity[j]);
         if (viscosity > 0) translation = xyz(p.forceVelocity[i]) - xyz(p.forceVelocity[j]);
@@ -1282,7 +1501,8 @@ __global__ __launch_bounds__(256) void k_gather(P p, W w) {
         wave_sync();
         if (lane < count) {
             const int e = lane;
-            const bool source = gget(st, G_SRC, e) != 0.0f;
+            const float kind = gget(st, G_SRC, e);  // 0 target, 1 source, 2 + k plane k
+            const bool source = kind != 0.0f, wall = kind >= 2.0f;
             const v3 force = {gget(st, G_FX, e), gget(st, G_FY, e), gget(st, G_FZ, e)};
             const v3 arm = {gget(st, G_AX, e), gget(st, G_AY, e), gget(st, G_AZ, e)};
             const v3 branch = {gget(st, G_BX, e), gget(st, G_BY, e), gget(st, G_BZ, e)};
@@ -1293,21 +1513,27 @@ __global__ __launch_bounds__(256) void k_gather(P p, W w) {
             ct[3 * GSTRIDE + e] = c.x;
             ct[4 * GSTRIDE + e] = c.y;
             ct[5 * GSTRIDE + e] = c.z;
+            // A plane contact carries its whole force and energy (no half share)
+            // and its stress moment arm is the contact arm itself.
             const float fr[3] = {force.x, force.y, force.z}, bc[3] = {branch.x, branch.y, branch.z};
+            const float ac[3] = {arm.x, arm.y, arm.z};
             for (int r = 0; r < 3; r++)
-                for (int k = 0; k < 3; k++) ct[(6 + 3 * r + k) * GSTRIDE + e] = 0.5f * fr[r] * bc[k];
-            ct[15 * GSTRIDE + e] = 0.5f * gget(st, G_E, e);
+                for (int k = 0; k < 3; k++)
+                    ct[(6 + 3 * r + k) * GSTRIDE + e] = wall ? fr[r] * ac[k] : 0.5f * fr[r] * bc[k];
+            ct[15 * GSTRIDE + e] = wall ? gget(st, G_E, e) : 0.5f * gget(st, G_E, e);
             ct[16 * GSTRIDE + e] = length(force);
             ct[17 * GSTRIDE + e] = gget(st, G_DEPTH, e) / gget(st, G_MINW, e);
-            ct[18 * GSTRIDE + e] = 0.5f * gget(st, G_SLIP, e);
-            ct[19 * GSTRIDE + e] = 0.5f * gget(st, G_NLP, e);
+            ct[18 * GSTRIDE + e] = wall ? gget(st, G_SLIP, e) : 0.5f * gget(st, G_SLIP, e);
+            ct[19 * GSTRIDE + e] = wall ? gget(st, G_NLP, e) : 0.5f * gget(st, G_NLP, e);
             if (p.c.activeTimestep != 0) {
                 const v3 localArm = source ? arm : arm - branch;
                 const float mobility = 1.0f / mass + dot(localArm, localArm) * inverseInertia;
-                const float area = gget(st, G_AREA, e) * 0.5f;
-                const float coupling = 2.0f;
-                ct[20 * GSTRIDE + e] = coupling * area * (p.c.kn * (1.0f + p.c.mu) + p.c.kt) * mobility;
-                ct[21 * GSTRIDE + e] = coupling * area * p.c.normalViscosity * (1.0f + p.c.mu) * mobility;
+                float4 material = make_float4(p.c.kn, p.c.kt, p.c.mu, p.c.normalViscosity);
+                if (wall) material = p.wallMaterial[uint32_t(kind) - 2];
+                const float area = gget(st, G_AREA, e) * (wall ? 1.0f : 0.5f);
+                const float coupling = wall ? 1.0f : 2.0f;
+                ct[20 * GSTRIDE + e] = coupling * area * (material.x * (1.0f + material.z) + material.y) * mobility;
+                ct[21 * GSTRIDE + e] = coupling * area * material.w * (1.0f + material.z) * mobility;
             }
         }
         wave_sync();
@@ -1327,6 +1553,37 @@ __global__ __launch_bounds__(256) void k_gather(P p, W w) {
         }
         wave_sync();
     };
+    // Plane contacts carry the highest ids (after every mesh contact): the list
+    // starts with them, planes descending, each plane's contacts descending.
+    const uint32_t wallBase = p.c.wallCount ? w.counters[8 * LINE] : 0;
+    for (uint32_t k = p.c.wallCount; k-- > 0;) {
+        const uint32_t task = k * p.c.N + id;
+        const uint32_t cnt = w.wallCount[task];
+        const uint32_t off = wallBase + w.wallOffset[task];
+        const float kind = 2.0f + float(k), minW = p.body[id].w;
+        for (uint32_t done = 0; done < cnt;) {
+            uint32_t m = min(cnt - done, uint32_t(GCHUNK - fill));
+            if (uint32_t(lane) < m) {
+                uint32_t contact = off + cnt - 1 - (done + lane);
+                float4 cf = p.contactForce[contact];
+                float4 pdv = p.pointDepth[contact];
+                int e = fill + lane;
+                st[G_FX * GSTRIDE + e] = cf.x; st[G_FY * GSTRIDE + e] = cf.y; st[G_FZ * GSTRIDE + e] = cf.z;
+                st[G_E * GSTRIDE + e] = cf.w;
+                st[G_AX * GSTRIDE + e] = pdv.x; st[G_AY * GSTRIDE + e] = pdv.y; st[G_AZ * GSTRIDE + e] = pdv.z;
+                st[G_DEPTH * GSTRIDE + e] = pdv.w;
+                st[G_SLIP * GSTRIDE + e] = p.slip[contact].w;
+                st[G_NLP * GSTRIDE + e] = p.normalLossPower[contact];
+                st[G_AREA * GSTRIDE + e] = p.c.activeTimestep ? p.normalArea[contact].w : 0.0f;
+                st[G_SRC * GSTRIDE + e] = kind;
+                st[G_BX * GSTRIDE + e] = 0.0f; st[G_BY * GSTRIDE + e] = 0.0f; st[G_BZ * GSTRIDE + e] = 0.0f;
+                st[G_MINW * GSTRIDE + e] = minW;
+            }
+            fill += int(m);
+            done += m;
+            if (fill == GCHUNK) { process(fill); fill = 0; }
+        }
+    }
     const uint32_t first = w.adjOffset[id];
     for (uint32_t k = w.adjCount[id]; k-- > 0;) {
         uint32_t pair = w.adjList[first + k];
@@ -1389,6 +1646,42 @@ __global__ __launch_bounds__(256) void k_gather(P p, W w) {
     float kinetic = 0.5f * (mass * dot(vel, vel) + dot(xyz(p.momentum[id]), omega(p, id)));
     p.particleDiagnostic[id] = make_float4(kinetic, energy, length(f), denom);
     accumulateRow(p, id, {dissipation, 0, 0, 0});
+}
+
+// gatherWallRows (solver.slang:837-856): one thread per (wall, grain) task sums
+// that plane's contacts on the grain in list order (descending id): reaction
+// -F, moment -(x + arm) x F about the origin, power F . v_wall, the contact count
+// (stored as the bits of row 0's w) and the accumulated work.
+__global__ void k_gather_wall_rows(P p, W w) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= p.c.N * p.c.wallCount || ld_volatile(p.control) != 0) return;
+    const uint32_t body = index % p.c.N, wall = index / p.c.N;
+    const uint32_t cnt = w.wallCount[index], off = w.counters[8 * LINE] + w.wallOffset[index];
+    const v3 x = xyz(p.position[body]) + xyz(p.positionRemainder[body]);
+    const v3 velocity = xyz(p.wallVelocity[wall]);
+    v3 reaction = {0, 0, 0}, moment = {0, 0, 0};
+    float power = 0;
+    for (uint32_t k = cnt; k-- > 0;) {
+        const uint32_t contact = off + k;
+        const v3 force = xyz(p.contactForce[contact]);
+        const v3 point = x + xyz(p.pointDepth[contact]);
+        reaction = reaction - force;
+        moment = moment - cross(point, force);
+        power += dot(force, velocity);
+    }
+    const float4 work = p.wallAccumulated[index];
+    p.wallRows[3 * index] = f4(reaction, __uint_as_float(cnt));
+    p.wallRows[3 * index + 1] = f4(moment, power);
+    p.wallRows[3 * index + 2] = make_float4(work.x + work.y, 0, 0, 0);
+}
+// refreshWallPower (solver.slang:950-955): wall power after a velocity change,
+// from the stored reaction.
+__global__ void k_refresh_wall_power(P p) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= p.c.N * p.c.wallCount) return;
+    float4 value = p.wallRows[3 * index + 1];
+    value.w = -dot(xyz(p.wallRows[3 * index]), xyz(p.wallVelocity[index / p.c.N]));
+    p.wallRows[3 * index + 1] = value;
 }
 
 // ---------------------------------------------------------------- scan.slang (the reference Engine's Scan)
@@ -1539,7 +1832,8 @@ thread_local char g_error[256] = "no error";
     X(count_pairs) X(scan_reduce) X(scan_sums) X(scan_apply) X(emit_pairs) X(adj_clear) X(adj_count) X(adj_fill) \
     X(adj_sort) X(reset_mesh) X(prepare_pairs) X(collect) X(contact_total) X(scatter) X(clear_storage) X(resolve) \
     X(build_hash) X(gather) X(finish_step) X(capture_kinematics) X(check_timestep) \
-    X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_offsets) X(bucket_fill)
+    X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_offsets) X(bucket_fill) \
+    X(advance_walls) X(wall_count) X(wall_emit) X(wall_total) X(gather_wall_rows) X(refresh_wall_power)
 enum KernelId {
 #define SAD_ID(name) KID_##name,
     SAD_KERNELS(SAD_ID)
@@ -1592,7 +1886,7 @@ int grow(T*& ptr, uint32_t& cap, size_t need, size_t& bytes) {
 struct Caps {
     uint32_t counters = 0, taskCount = 0, taskOffset = 0, adjCount = 0, adjOffset = 0, adjCursor = 0, adjList = 0;
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
-    uint32_t stageCount = 0, keyCount = 0, keyStart = 0;
+    uint32_t stageCount = 0, keyCount = 0, keyStart = 0, wallCount = 0, wallOffset = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -1622,7 +1916,10 @@ struct ScratchImpl : Scratch {
     int ensure(const Cfg& c) {
         size_t tasks = size_t(2) * c.pairCapacity;
         int rc;
-        if ((rc = grow(w.counters, caps.counters, 16 * LINE, bytes))) return rc;
+        if (!w.counters) {
+            if ((rc = grow(w.counters, caps.counters, 16 * LINE, bytes))) return rc;
+            HIP_OK(hipMemset(w.counters, 0, 16 * LINE * sizeof(uint32_t)));
+        }
         if ((rc = grow(w.stageCount, caps.stageCount, 8 * LINE, bytes))) return rc;
         if (!w.totals) {
             HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.totals), 4 * sizeof(unsigned long long)));
@@ -1635,7 +1932,18 @@ struct ScratchImpl : Scratch {
         if ((rc = grow(w.adjOffset, caps.adjOffset, c.N, bytes))) return rc;
         if ((rc = grow(w.adjCursor, caps.adjCursor, c.N, bytes))) return rc;
         if ((rc = grow(w.adjList, caps.adjList, tasks, bytes))) return rc;
-        size_t sums = (std::max<size_t>(tasks + 1, c.N) + SCAN_TILE - 1) / SCAN_TILE + 1;
+        const size_t wallTasks = size_t(c.wallCount) * c.N;
+        if (wallTasks > 0xffffffffull) {
+            std::snprintf(g_error, sizeof(g_error), "too many (wall, grain) pairs");
+            return SAD_E_ARGUMENT;
+        }
+        if (caps.wallCount < std::max<size_t>(wallTasks, 1)) {  // zeroed: no plane contacts before a collection
+            if ((rc = grow(w.wallCount, caps.wallCount, wallTasks, bytes))) return rc;
+            HIP_OK(hipMemset(w.wallCount, 0, size_t(caps.wallCount) * sizeof(uint32_t)));
+        }
+        if ((rc = grow(w.wallOffset, caps.wallOffset, wallTasks, bytes))) return rc;
+        w.wallCap = uint32_t(std::max<size_t>(wallTasks, 1));
+        size_t sums = (std::max<size_t>(std::max<size_t>(tasks + 1, c.N), wallTasks) + SCAN_TILE - 1) / SCAN_TILE + 1;
         if ((rc = grow(w.sums, caps.sums, sums, bytes))) return rc;
         if ((rc = grow(w.stageKey, caps.stageKey, size_t(8) * c.contactCapacity, bytes))) return rc;
         if ((rc = grow(w.stageMeta, caps.stageMeta, size_t(8) * c.contactCapacity, bytes))) return rc;
@@ -1663,7 +1971,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.counters); (void)hipFree(w.taskCount); (void)hipFree(w.taskOffset); (void)hipFree(w.adjCount);
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
-        (void)hipFree(w.keyCount); (void)hipFree(w.keyStart);
+        (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
 };
@@ -1699,8 +2007,8 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
     c.wallCount = uint32_t(v[SAD_V_WALL_COUNT]);
     c.wallBoxEnabled = uint32_t(v[SAD_V_WALL_BOX_ENABLED]);
     c.sourceSurfaceNormal = uint32_t(v[SAD_V_SOURCE_SURFACE_NORMAL]);
-    if (c.meshGeometry == 0 || c.wallCount != 0 || c.sourceSurfaceNormal != 0) {
-        std::snprintf(g_error, sizeof(g_error), "only the surface-mesh step without planes is implemented");
+    if (c.meshGeometry == 0 || c.sourceSurfaceNormal != 0) {
+        std::snprintf(g_error, sizeof(g_error), "only the surface-mesh step is implemented");
         return SAD_E_UNSUPPORTED;
     }
     if (c.N == 0 || c.hashSize == 0 || (c.hashSize & (c.hashSize - 1)) != 0) {
@@ -1795,6 +2103,30 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
     p.timestepRows = slot<float4>(b, SAD_TIMESTEP_ROWS);
     p.particleDiagnostic = slot<float4>(b, SAD_PARTICLE_DIAGNOSTIC);
     p.accumulated = slot<float4>(b, SAD_ACCUMULATED);
+    if (c.wallCount != 0) {
+        const uint64_t planes = 16ull * c.wallCount, tasks = uint64_t(c.wallCount) * c.N;
+        const Need walls[] = {
+            {SAD_WALL_PLANE, planes, "wallPlane"}, {SAD_WALL_REMAINDER, planes, "wallRemainder"},
+            {SAD_WALL_VELOCITY, planes, "wallVelocity"}, {SAD_WALL_MATERIAL, planes, "wallMaterial"},
+            {SAD_WALL_BOX_IDS, c.wallBoxEnabled ? 24ull : 0ull, "wallBoxIds"},
+            {SAD_WALL_ROWS, 48ull * tasks, "wallRows"}, {SAD_WALL_ACCUMULATED, 16ull * tasks, "wallAccumulated"},
+            {SAD_NORMAL_DAMPING, 16ull * c.N, "normalDamping"},
+        };
+        for (const Need& n : walls) {
+            if (!b[n.slot] || bytes[n.slot] < n.bytes) {
+                std::snprintf(g_error, sizeof(g_error), "buffer %s too small (%llu < %llu bytes)", n.name,
+                              (unsigned long long)bytes[n.slot], (unsigned long long)n.bytes);
+                return SAD_E_ARGUMENT;
+            }
+        }
+        p.wallPlane = slot<float4>(b, SAD_WALL_PLANE);
+        p.wallRemainder = slot<float4>(b, SAD_WALL_REMAINDER);
+        p.wallVelocity = slot<const float4>(b, SAD_WALL_VELOCITY);
+        p.wallMaterial = slot<const float4>(b, SAD_WALL_MATERIAL);
+        p.wallBoxIds = slot<const uint32_t>(b, SAD_WALL_BOX_IDS);
+        p.wallRows = slot<float4>(b, SAD_WALL_ROWS);
+        p.wallAccumulated = slot<float4>(b, SAD_WALL_ACCUMULATED);
+    }
     return SAD_OK;
 }
 
@@ -1891,14 +2223,28 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
     else if (name == "buildHistoryHash") LAUNCH(sc, KID_build_hash, k_build_hash, gridStride, T, s, p);
     else if (name == "gather") LAUNCH(sc, KID_gather, k_gather, dim3(blocks(c.N, 4)), T, s, p, w);
     else if (name == "finishStep") LAUNCH(sc, KID_finish_step, k_finish_step, perGrain, T, s, p);
-    else if (name == "captureContactKinematics") LAUNCH(sc, KID_capture_kinematics, k_capture_kinematics, perGrain, T, s, p);
+    else if (name == "captureContactKinematics") LAUNCH(sc, KID_capture_kinematics, k_capture_kinematics, perGrain, T, s, p, 0);
     else if (name == "checkContactTimestep") LAUNCH(sc, KID_check_timestep, k_check_timestep, perGrain, T, s, p);
-    else {
-        // Reference entry points outside the mesh path (walls, SDF grid, diagnostics,
+    else if (name == "advanceWalls") {
+        if (c.wallCount) LAUNCH(sc, KID_advance_walls, k_advance_walls, dim3(1), dim3(1), s, p);
+    } else if (name == "collectWallContacts") {
+        if (c.wallCount) {
+            const uint32_t tasks = c.wallCount * c.N;
+            LAUNCH(sc, KID_wall_count, k_wall_collect<false>, dim3(blocks(tasks, 4)), T, s, p, w);
+            ScanArgs a{w.wallCount, w.wallOffset, w.sums, tasks, nullptr, 1, nullptr, w.counters + 7 * LINE};
+            launch_scan(sc, a, tasks, s);
+            LAUNCH(sc, KID_wall_emit, k_wall_collect<true>, dim3(blocks(tasks, 4)), T, s, p, w);
+            LAUNCH(sc, KID_wall_total, k_wall_total, dim3(1), dim3(1), s, p, w);
+        }
+    } else if (name == "gatherWallRows") {
+        if (c.wallCount) LAUNCH(sc, KID_gather_wall_rows, k_gather_wall_rows, dim3(blocks(c.wallCount * c.N)), T, s, p, w);
+    } else if (name == "refreshWallPower") {
+        if (c.wallCount) LAUNCH(sc, KID_refresh_wall_power, k_refresh_wall_power, dim3(blocks(c.wallCount * c.N)), T, s, p);
+    } else {
+        // Reference entry points outside the mesh path (SDF grid, diagnostics,
         // the serial verification collector) are known but not provided here.
-        static const char* unsupported[] = {"advanceWalls", "collectWallContacts", "gatherWallRows",
-                                            "refreshWallPower", "countContacts", "emitContacts", "contactTotal",
-                                            "collectMeshContacts", "diagnosticRows", "fabricRows", "smoke"};
+        static const char* unsupported[] = {"countContacts", "emitContacts", "contactTotal", "collectMeshContacts",
+                                            "diagnosticRows", "fabricRows", "smoke"};
         for (const char* u : unsupported)
             if (name == u) {
                 std::snprintf(g_error, sizeof(g_error), "kernel %s is not part of the hip mesh path", u);
@@ -2002,16 +2348,22 @@ int sad_hip_dispatch(const char* kernel, uint32_t count, void* const* buffers, c
 
 // One timestep's launches (per-step order of gpu.py:714-726).
 int launch_step(P& p, ScratchImpl& sc, hipStream_t s) {
-    static const char* sequence[] = {"beginStep", "advanceCell", "checkRebuild", "clearBins", "fillBins",
-                                     "countPairs", "pairScan", "emitPairs", "resetMeshContacts",
-                                     "collectMeshFrontiers", "clearContactStorage", "resolveContacts",
-                                     "buildHistoryHash", "gather"};
+    static const char* sequence[] = {"beginStep", "advanceCell", "advanceWalls", "checkRebuild", "clearBins",
+                                     "fillBins", "countPairs", "pairScan", "emitPairs", "resetMeshContacts",
+                                     "collectMeshFrontiers", "collectWallContacts", "clearContactStorage",
+                                     "resolveContacts", "buildHistoryHash", "gather", "gatherWallRows"};
     int rc;
     if (p.c.activeTimestep && (rc = run_kernel("checkContactTimestep", p, sc, s))) return rc;
     for (const char* name : sequence) {
-        if (std::strcmp(name, "clearBins") == 0 && p.c.normalViscosity > 0 &&
-            (rc = run_kernel("captureContactKinematics", p, sc, s)))
-            return rc;
+        if (std::strcmp(name, "clearBins") == 0) {
+            // The Engine captures contact kinematics when any contact is viscous;
+            // with planes that depends on device-resident wall materials.
+            if (p.c.normalViscosity > 0) {
+                if ((rc = run_kernel("captureContactKinematics", p, sc, s))) return rc;
+            } else if (p.c.wallCount) {
+                LAUNCH(sc, KID_capture_kinematics, k_capture_kinematics, dim3(blocks(p.c.N)), dim3(256), s, p, 1);
+            }
+        }
         if ((rc = run_kernel(name, p, sc, s))) return rc;
     }
     if (p.c.activeTimestep && (rc = run_kernel("checkContactTimestep", p, sc, s))) return rc;
