@@ -11,7 +11,9 @@
 // dot() accumulates from 0.0f left to right, length = sqrtf(dot), normalize =
 // x * (1/sqrtf(dot)), min/max = fminf/fmaxf, sinf/cosf/expf as the correctly rounded
 // FP32 value (below; evidence/.../instrumented-collector.cpp:869-1027).  Build with
-// -O3 -fno-fast-math -ffp-contract=off (cpu.py:323-333).
+// -O3 -fno-fast-math -ffp-contract=off (cpu.py:323-333).  Built with
+// -DSAD_ORACLE_FP64 (liboracle64.so) every FP32 quantity is FP64 instead: the same
+// algorithm at double precision, used only as the yardstick for FP32 drift.
 //
 // Exported C ABI (binding order of include/sand_dem_hip.h, same as the
 // reference's generated cpu_dispatch):
@@ -33,38 +35,48 @@
 
 namespace {
 
-struct float3 { float x, y, z; };
-struct float4 { float x, y, z, w; };
+// Arithmetic type of the restatement: FP32 (the reference's, and the HIP path's)
+// unless built with -DSAD_ORACLE_FP64, which runs the same algorithm in FP64 on
+// the same FP32 inputs (liboracle64.so: the drift yardstick).  Buffers the FP64
+// build reads hold doubles in place of floats, element for element.
+#ifdef SAD_ORACLE_FP64
+using real = double;
+#else
+using real = float;
+#endif
+
+struct float3 { real x, y, z; };
+struct float4 { real x, y, z, w; };
 struct uint4 { uint32_t x, y, z, w; };
 struct uint2 { uint32_t x, y; };
 struct double3 { double x, y, z; };
 
-inline float3 f3(float x, float y, float z) { return {x, y, z}; }
+inline float3 f3(real x, real y, real z) { return {x, y, z}; }
 inline float3 xyz(const float4& a) { return {a.x, a.y, a.z}; }
-inline float4 f4(const float3& a, float w) { return {a.x, a.y, a.z, w}; }
+inline float4 f4(const float3& a, real w) { return {a.x, a.y, a.z, w}; }
 inline float3 operator+(float3 a, float3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 inline float3 operator-(float3 a, float3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline float3 operator*(float3 a, float3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
 inline float3 operator/(float3 a, float3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
-inline float3 operator*(float s, float3 a) { return {s * a.x, s * a.y, s * a.z}; }
-inline float3 operator*(float3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-inline float3 operator/(float3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline float3 operator*(real s, float3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline float3 operator*(float3 a, real s) { return {a.x * s, a.y * s, a.z * s}; }
+inline float3 operator/(float3 a, real s) { return {a.x / s, a.y / s, a.z / s}; }
 inline float3 operator-(float3 a) { return {-a.x, -a.y, -a.z}; }
 inline float4 operator+(float4 a, float4 b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
 inline float4 operator-(float4 a, float4 b) { return {a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
-inline float4 operator*(float4 a, float s) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
-inline float get(const float3& a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
-inline void set(float3& a, int i, float v) { (i == 0 ? a.x : (i == 1 ? a.y : a.z)) = v; }
+inline float4 operator*(float4 a, real s) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
+inline real get(const float3& a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+inline void set(float3& a, int i, real v) { (i == 0 ? a.x : (i == 1 ? a.y : a.z)) = v; }
 
-inline float dot(float3 a, float3 b) {
-    float r = 0.0f;
+inline real dot(float3 a, float3 b) {
+    real r = real(0.0);
     r = r + a.x * b.x;
     r = r + a.y * b.y;
     r = r + a.z * b.z;
     return r;
 }
-inline float dot4(float4 a, float4 b) {
-    float r = 0.0f;
+inline real dot4(float4 a, float4 b) {
+    real r = real(0.0);
     r = r + a.x * b.x;
     r = r + a.y * b.y;
     r = r + a.z * b.z;
@@ -74,7 +86,7 @@ inline float dot4(float4 a, float4 b) {
 inline float3 cross(float3 l, float3 r) {
     return {l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x};
 }
-inline float length(float3 a) { return std::sqrt(dot(a, a)); }
+inline real length(float3 a) { return std::sqrt(dot(a, a)); }
 inline float3 fabs3(float3 a) { return {std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)}; }
 inline float3 fmax3(float3 a, float3 b) { return {std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)}; }
 inline float3 fmin3(float3 a, float3 b) { return {std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)}; }
@@ -84,23 +96,23 @@ inline bool finite3(float3 a) { return std::isfinite(a.x) && std::isfinite(a.y) 
 // (instrumented-collector.cpp:877-943), whose last bit differs between libms
 // (glibc is off by one ulp on ~1% of arguments).  The oracle fixes them to the
 // correctly rounded value, obtained from the FP64 libm functions.
-inline float sin32(float x) { return float(std::sin(double(x))); }
-inline float cos32(float x) { return float(std::cos(double(x))); }
-inline float exp32(float x) { return float(std::exp(double(x))); }
+inline real sin32(real x) { return real(std::sin(double(x))); }
+inline real cos32(real x) { return real(std::cos(double(x))); }
+inline real exp32(real x) { return real(std::exp(double(x))); }
 
 inline float4 normalize4(float4 a) {
-    float s = 1.0f / std::sqrt(dot4(a, a));
+    real s = real(1.0) / std::sqrt(dot4(a, a));
     return a * s;
 }
 
 constexpr uint32_t EMPTY = 0xffffffffu;
-constexpr float EPS = 1.1920928955078125e-7f;
+constexpr real EPS = real(1.1920928955078125e-7);
 
 struct Config {
     uint32_t activeTimestep, N, shapes, pairCapacity, contactCapacity, binCapacity, binCount;
     uint32_t bins[3];
     uint32_t hashSize;
-    float dt, kn, kt, mu, normalViscosity, skin, drag, maxOverlap;
+    real dt, kn, kt, mu, normalViscosity, skin, drag, maxOverlap;
     uint32_t historyStep, meshGeometry, periodic, wallCount, wallBoxEnabled, sourceSurfaceNormal;
     uint32_t threads;
 };
@@ -130,7 +142,7 @@ struct R {
     uint32_t *contactCounts, *contactOffsets;
     uint4* meta;
     float4 *pointDepth, *normalArea, *slip, *contactForce;
-    float* normalLossPower;
+    real* normalLossPower;
     float4 *normalDamping, *forceVelocity, *forceOmega;
     uint4* oldMeta;
     float4 *oldSlip, *oldNormal;
@@ -158,14 +170,14 @@ R bind(void** b, const uint64_t* bytes, const double* v) {
     c.bins[1] = uint32_t(v[SAD_V_BINS_Y]);
     c.bins[2] = uint32_t(v[SAD_V_BINS_Z]);
     c.hashSize = uint32_t(v[SAD_V_HASH_SIZE]);
-    c.dt = float(v[SAD_V_DT]);
-    c.kn = float(v[SAD_V_KN]);
-    c.kt = float(v[SAD_V_KT]);
-    c.mu = float(v[SAD_V_MU]);
-    c.normalViscosity = float(v[SAD_V_NORMAL_VISCOSITY]);
-    c.skin = float(v[SAD_V_SKIN]);
-    c.drag = float(v[SAD_V_DRAG]);
-    c.maxOverlap = float(v[SAD_V_MAX_OVERLAP]);
+    c.dt = real(v[SAD_V_DT]);
+    c.kn = real(v[SAD_V_KN]);
+    c.kt = real(v[SAD_V_KT]);
+    c.mu = real(v[SAD_V_MU]);
+    c.normalViscosity = real(v[SAD_V_NORMAL_VISCOSITY]);
+    c.skin = real(v[SAD_V_SKIN]);
+    c.drag = real(v[SAD_V_DRAG]);
+    c.maxOverlap = real(v[SAD_V_MAX_OVERLAP]);
     c.historyStep = uint32_t(v[SAD_V_HISTORY_STEP]);
     c.meshGeometry = uint32_t(v[SAD_V_MESH_GEOMETRY]);
     c.periodic = uint32_t(v[SAD_V_PERIODIC]);
@@ -223,7 +235,7 @@ R bind(void** b, const uint64_t* bytes, const double* v) {
     r.normalArea = at<float4>(b, SAD_NORMAL_AREA);
     r.slip = at<float4>(b, SAD_SLIP);
     r.contactForce = at<float4>(b, SAD_CONTACT_FORCE);
-    r.normalLossPower = at<float>(b, SAD_NORMAL_LOSS_POWER);
+    r.normalLossPower = at<real>(b, SAD_NORMAL_LOSS_POWER);
     r.normalDamping = at<float4>(b, SAD_NORMAL_DAMPING);
     r.forceVelocity = at<float4>(b, SAD_FORCE_VELOCITY);
     r.forceOmega = at<float4>(b, SAD_FORCE_OMEGA);
@@ -252,7 +264,7 @@ R bind(void** b, const uint64_t* bytes, const double* v) {
 // ---------------------------------------------------------------- helpers (solver.slang:104-186)
 inline float3 qrot(float4 q, float3 v) {  // solver.slang:104
     float3 u = xyz(q);
-    return v + 2.0f * cross(u, cross(u, v) + q.w * v);
+    return v + real(2.0) * cross(u, cross(u, v) + q.w * v);
 }
 inline float3 qinv(float4 q, float3 v) { return qrot({-q.x, -q.y, -q.z, q.w}, v); }  // :105
 inline float4 qmul(float4 a, float4 b) {  // :106
@@ -274,17 +286,17 @@ inline void compensated4(float4& high, float4& low, float4 inc) {  // :115-118
     low = y - moved;
     high = next;
 }
-inline float expm1Stable(float x) {  // :119-122
-    if (std::fabs(x) < 0.01f)
-        return x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f + x / 120.0f))));
-    return exp32(x) - 1.0f;
+inline real expm1Stable(real x) {  // :119-122
+    if (std::fabs(x) < real(0.01))
+        return x * (real(1.0) + x * (real(0.5) + x * (real(1.0) / real(6.0) + x * (real(1.0) / real(24.0) + x / real(120.0)))));
+    return exp32(x) - real(1.0);
 }
 inline float3 expm1Stable3(float3 x) { return {expm1Stable(x.x), expm1Stable(x.y), expm1Stable(x.z)}; }
-inline float4 rotationIncrement(float3 w, float dt, float4 q) {  // :124-129
-    float half = 0.5f * length(w) * dt;
-    float coefficient = std::fabs(half) < 0.001f ? 0.5f * dt * (1.0f - half * half / 6.0f)
+inline float4 rotationIncrement(float3 w, real dt, float4 q) {  // :124-129
+    real half = real(0.5) * length(w) * dt;
+    real coefficient = std::fabs(half) < real(0.001) ? real(0.5) * dt * (real(1.0) - half * half / real(6.0))
                                                  : sin32(half) / length(w);
-    float cm1 = -2.0f * sin32(0.5f * half) * sin32(0.5f * half);
+    real cm1 = -real(2.0) * sin32(real(0.5) * half) * sin32(real(0.5) * half);
     return qmul(f4(coefficient * w, cm1), q);
 }
 inline void accumulate(const R& r, uint32_t id, float4 inc) {  // :130-134
@@ -306,19 +318,19 @@ inline float3 centerBranch(const R& r, uint32_t i, uint32_t j) {
         dy = dy - std::floor(dy / by + 0.5) * by;
         dz = dz - std::floor(dz / bz + 0.5) * bz;
     }
-    return {float(dx), float(dy), float(dz)};
+    return {real(dx), real(dy), real(dz)};
 }
 inline float3 omega(const R& r, uint32_t i) {  // :154-158
     float3 b = qinv(r.rotation[i], xyz(r.momentum[i]) + xyz(r.momentumRemainder[i]));
     float3 w = {dot(xyz(r.invI[3 * i]), b), dot(xyz(r.invI[3 * i + 1]), b), dot(xyz(r.invI[3 * i + 2]), b)};
     return qrot(r.rotation[i], w);
 }
-inline float rotationalEnergy(const R& r, uint32_t i, float3 l) {  // :159-163
+inline real rotationalEnergy(const R& r, uint32_t i, float3 l) {  // :159-163
     float3 b = qinv(r.rotation[i], l);
     float3 w = {dot(xyz(r.invI[3 * i]), b), dot(xyz(r.invI[3 * i + 1]), b), dot(xyz(r.invI[3 * i + 2]), b)};
-    return 0.5f * dot(b, w);
+    return real(0.5) * dot(b, w);
 }
-inline void accumulateNormalLoss(const R& r, uint32_t id, float dt) {  // :164-169
+inline void accumulateNormalLoss(const R& r, uint32_t id, real dt) {  // :164-169
     float4 value = r.normalDamping[id];
     float3 high = {value.x, 0, 0}, low = {value.y, 0, 0};
     compensated3(high, low, {value.z * dt, 0, 0});
@@ -326,18 +338,18 @@ inline void accumulateNormalLoss(const R& r, uint32_t id, float dt) {  // :164-1
     value.y = low.x;
     r.normalDamping[id] = value;
 }
-inline float cellPower(const R& r, uint32_t i) {  // :176-180
+inline real cellPower(const R& r, uint32_t i) {  // :176-180
     float3 diagonal = {r.particleStress[3 * i].x, r.particleStress[3 * i + 1].y, r.particleStress[3 * i + 2].z};
     float3 v = xyz(r.velocity[i]);
     return -dot(diagonal + r.body[i].y * v * v, xyz(r.cell[1]));
 }
 inline float3 image(const R& r, float3 d) {  // :181
     float3 box = xyz(r.cell[0]);
-    return r.c.periodic != 0 ? d - floor3(d / box + f3(0.5f, 0.5f, 0.5f)) * box : d;
+    return r.c.periodic != 0 ? d - floor3(d / box + f3(real(0.5), real(0.5), real(0.5))) * box : d;
 }
 inline void fail(const R& r, uint32_t bit) { r.control[0] |= bit; }  // :183 (serial)
 inline void binIndex(const R& r, uint32_t i, uint32_t b[3]) {  // :184
-    float3 g = floor3(xyz(r.position[i]) / xyz(r.cell[0]) * f3(float(r.c.bins[0]), float(r.c.bins[1]), float(r.c.bins[2])));
+    float3 g = floor3(xyz(r.position[i]) / xyz(r.cell[0]) * f3(real(r.c.bins[0]), real(r.c.bins[1]), real(r.c.bins[2])));
     b[0] = std::min(uint32_t(g.x), r.c.bins[0] - 1);
     b[1] = std::min(uint32_t(g.y), r.c.bins[1] - 1);
     b[2] = std::min(uint32_t(g.z), r.c.bins[2] - 1);
@@ -348,29 +360,29 @@ inline uint32_t modIndex(int a, uint32_t n) { return uint32_t((a % int(n) + int(
 // ---------------------------------------------------------------- mesh queries (solver.slang:203-343)
 inline float3 closestTriangle(float3 p, float3 a, float3 b, float3 c, uint32_t& feature) {  // :203-219
     float3 ab = b - a, ac = c - a, ap = p - a;
-    float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    real d1 = dot(ab, ap), d2 = dot(ac, ap);
     feature = 4;
     if (d1 <= 0 && d2 <= 0) return a;
     float3 bp = p - b;
-    float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    real d3 = dot(ab, bp), d4 = dot(ac, bp);
     if (d3 >= 0 && d4 <= d3) { feature = 5; return b; }
-    float vc = d1 * d4 - d3 * d2;
+    real vc = d1 * d4 - d3 * d2;
     if (vc <= 0 && d1 >= 0 && d3 <= 0) { feature = 1; return a + (d1 / (d1 - d3)) * ab; }
     float3 cp = p - c;
-    float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    real d5 = dot(ab, cp), d6 = dot(ac, cp);
     if (d6 >= 0 && d5 <= d6) { feature = 6; return c; }
-    float vb = d5 * d2 - d1 * d6;
+    real vb = d5 * d2 - d1 * d6;
     if (vb <= 0 && d2 >= 0 && d6 <= 0) { feature = 3; return a + (d2 / (d2 - d6)) * ac; }
-    float va = d3 * d6 - d5 * d4;
+    real va = d3 * d6 - d5 * d4;
     if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
         feature = 2;
         return b + ((d4 - d3) / ((d4 - d3) + (d5 - d6))) * (c - b);
     }
-    float inv = 1.0f / (va + vb + vc);
+    real inv = real(1.0) / (va + vb + vc);
     feature = 0;
     return a + ab * (vb * inv) + ac * (vc * inv);
 }
-inline float meshBoxDistance(const R& r, uint32_t node, float3 p) {  // :220-223
+inline real meshBoxDistance(const R& r, uint32_t node, float3 p) {  // :220-223
     float3 d = fmax3(fmax3(xyz(r.meshMin[node]) - p, p - xyz(r.meshMax[node])), f3(0, 0, 0));
     return dot(d, d);
 }
@@ -378,11 +390,11 @@ inline float meshBoxDistance(const R& r, uint32_t node, float3 p) {  // :220-223
 float4 meshDistance(const R& r, uint32_t shape, float3 p, bool surfaceNormal = false) {
     uint32_t root = r.meshRoots[shape].x;
     uint32_t pending[32];
-    float pendingDistance[32];
+    real pendingDistance[32];
     uint32_t top = 1;
     pending[0] = root;
     pendingDistance[0] = meshBoxDistance(r, root, p);
-    float best = 1e30f;
+    real best = real(1e30);
     float3 nearest = {0, 0, 0};
     uint32_t nearestNormal = 0;
     bool found = false;
@@ -392,9 +404,9 @@ float4 meshDistance(const R& r, uint32_t shape, float3 p, bool surfaceNormal = f
         uint4 link = r.meshLinks[node];
         if (link.z == 0) {
             uint32_t a = node + 1, b = r.meshLinks[a].x;
-            float da = meshBoxDistance(r, a, p), db = meshBoxDistance(r, b, p);
+            real da = meshBoxDistance(r, a, p), db = meshBoxDistance(r, b, p);
             if (da < db) { std::swap(a, b); std::swap(da, db); }
-            if (top + 2 > 32) { fail(r, 1024); return {0, 0, 0, 1e20f}; }
+            if (top + 2 > 32) { fail(r, 1024); return {0, 0, 0, real(1e20)}; }
             if (da <= best) { pending[top] = a; pendingDistance[top++] = da; }
             if (db <= best) { pending[top] = b; pendingDistance[top++] = db; }
             continue;
@@ -403,14 +415,14 @@ float4 meshDistance(const R& r, uint32_t shape, float3 p, bool surfaceNormal = f
             uint32_t triangle = link.y + j, feature;
             float3 q = closestTriangle(p, xyz(r.meshTriangles[3 * triangle]), xyz(r.meshTriangles[3 * triangle + 1]),
                                        xyz(r.meshTriangles[3 * triangle + 2]), feature);
-            float distance = dot(p - q, p - q);
+            real distance = dot(p - q, p - q);
             if (distance < best) { best = distance; nearest = q; nearestNormal = 7 * triangle + feature; found = true; }
         }
     }
     float3 pseudo = found ? xyz(r.meshNormals[nearestNormal]) : f3(0, 0, 0);
-    float distance = std::sqrt(best);
-    float side = dot(p - nearest, pseudo) < 0 ? -1.0f : 1.0f;
-    float3 normal = surfaceNormal ? pseudo : (distance > 1e-12f ? side * (p - nearest) / distance : pseudo);
+    real distance = std::sqrt(best);
+    real side = dot(p - nearest, pseudo) < 0 ? -real(1.0) : real(1.0);
+    float3 normal = surfaceNormal ? pseudo : (distance > real(1e-12) ? side * (p - nearest) / distance : pseudo);
     return f4(normal, side * distance);
 }
 inline uint32_t dominantGroup(float3 direction) {
@@ -422,7 +434,7 @@ bool meshMayContain(const R& r, uint32_t shape, float3 p) {  // :259-272
     uint32_t root = r.meshRoots[shape].x;
     float3 low = xyz(r.meshMin[root]), high = xyz(r.meshMax[root]);
     if (p.x < low.x || p.y < low.y || p.z < low.z || p.x > high.x || p.y > high.y || p.z > high.z) return false;
-    float3 direction = (p - 0.5f * (low + high)) / (high - low);
+    float3 direction = (p - real(0.5) * (low + high)) / (high - low);
     uint4 info = r.meshHullInfo[6 * shape + dominantGroup(direction)];
     for (uint32_t i = 0; i < info.y; i++) {
         float4 plane = r.meshPlanes[info.x + i];
@@ -434,8 +446,8 @@ bool meshBoxMayContact(const R& r, uint32_t shape, float3 low, float3 high) {  /
     uint32_t root = r.meshRoots[shape].x;
     float3 tl = xyz(r.meshMin[root]), th = xyz(r.meshMax[root]);
     if (high.x < tl.x || high.y < tl.y || high.z < tl.z || low.x > th.x || low.y > th.y || low.z > th.z) return false;
-    float3 center = 0.5f * (low + high), half = 0.5f * (high - low);
-    float3 direction = (center - 0.5f * (tl + th)) / (th - tl);
+    float3 center = real(0.5) * (low + high), half = real(0.5) * (high - low);
+    float3 direction = (center - real(0.5) * (tl + th)) / (th - tl);
     uint4 info = r.meshHullInfo[6 * shape + dominantGroup(direction)];
     for (uint32_t i = 0; i < info.y; i++) {
         float4 plane = r.meshPlanes[info.x + i];
@@ -443,18 +455,18 @@ bool meshBoxMayContact(const R& r, uint32_t shape, float3 low, float3 high) {  /
     }
     return true;
 }
-inline float maxabs(float3 a) { return std::fmax(std::fmax(std::fabs(a.x), std::fabs(a.y)), std::fabs(a.z)); }
+inline real maxabs(float3 a) { return std::fmax(std::fmax(std::fabs(a.x), std::fabs(a.y)), std::fabs(a.z)); }
 bool meshClusterMayContact(const R& r, uint32_t src, uint32_t dst, uint32_t cluster) {  // :290-305
     float3 sl = xyz(r.meshClusterMin[cluster]), sh = xyz(r.meshClusterMax[cluster]);
     float3 branch = centerBranch(r, src, dst);
-    float3 low = {1e30f, 1e30f, 1e30f}, high = {-1e30f, -1e30f, -1e30f};
+    float3 low = {real(1e30), real(1e30), real(1e30)}, high = {-real(1e30), -real(1e30), -real(1e30)};
     for (uint32_t corner = 0; corner < 8; corner++) {
         float3 p = {(corner & 1) ? sh.x : sl.x, (corner & 2) ? sh.y : sl.y, (corner & 4) ? sh.z : sl.z};
         p = qinv(r.rotation[dst], qrot(r.rotation[src], p * r.body[src].x) - branch) / r.body[dst].x;
         low = fmin3(low, p);
         high = fmax3(high, p);
     }
-    float pad = 64.0f * EPS * (1.0f + maxabs(low) + maxabs(high) + length(branch) / r.body[dst].x);
+    real pad = real(64.0) * EPS * (real(1.0) + maxabs(low) + maxabs(high) + length(branch) / r.body[dst].x);
     low = low - f3(pad, pad, pad);
     high = high + f3(pad, pad, pad);
     return meshBoxMayContact(r, r.shapeId[dst], low, high);
@@ -468,7 +480,7 @@ bool nodeContact(const R& r, uint32_t src, uint32_t dst, uint32_t node, float4& 
     float3 branch = centerBranch(r, src, dst);
     float3 local = qinv(r.rotation[dst], arm - branch) / r.body[dst].x;
     uint32_t targetShape = r.shapeId[dst];
-    float planePad = 64.0f * EPS * (1.0f + maxabs(local));
+    real planePad = real(64.0) * EPS * (real(1.0) + maxabs(local));
     if (dot(xyz(rejectionPlane), local) + rejectionPlane.w > planePad) { pd = {0, 0, 0, 0}; na = {0, 0, 0, 0}; return false; }
     if (!meshMayContain(r, targetShape, local)) { pd = {0, 0, 0, 0}; na = {0, 0, 0, 0}; return false; }
     float4 f = meshDistance(r, targetShape, local);
@@ -476,8 +488,8 @@ bool nodeContact(const R& r, uint32_t src, uint32_t dst, uint32_t node, float4& 
     na = {0, 0, 0, 0};
     if (pd.w <= 0) return false;
     float3 grad = qrot(r.rotation[dst], xyz(f));
-    float len = length(grad);
-    if (len < 1e-7f || !std::isfinite(pd.w)) { fail(r, 2); return false; }
+    real len = length(grad);
+    if (len < real(1e-7) || !std::isfinite(pd.w)) { fail(r, 2); return false; }
     na = f4(grad / len, sp.w * r.body[src].x * r.body[src].x);
     if (pd.w > r.c.maxOverlap * std::fmin(r.body[src].w, r.body[dst].w)) fail(r, 4);
     return true;
@@ -492,12 +504,12 @@ void cpuCollectMesh(const R& r, uint32_t pair) {
         uint32_t i = pass == 0 ? ij.x : ij.y, j = pass == 0 ? ij.y : ij.x;
         float3 sourceCentre = -qinv(r.rotation[j], centerBranch(r, i, j)) / r.body[j].x;
         float4 rejectionPlane = {0, 0, 0, 0};
-        float largest = -1e30f;
+        real largest = -real(1e30);
         for (uint32_t group = 0; group < 6; group++) {
             uint4 hull = r.meshHullInfo[6 * r.shapeId[j] + group];
             for (uint32_t plane = 0; plane < hull.y; plane++) {
                 float4 enclosure = r.meshPlanes[hull.x + plane];
-                float violation = dot(xyz(enclosure), sourceCentre) + enclosure.w;
+                real violation = dot(xyz(enclosure), sourceCentre) + enclosure.w;
                 if (violation > largest) { largest = violation; rejectionPlane = enclosure; }
             }
         }
@@ -526,11 +538,11 @@ void cpuCollectMesh(const R& r, uint32_t pair) {
 }
 
 // ---------------------------------------------------------------- translating planes (solver.slang:135-145, 578-667)
-float wallStepWork(const R& r, uint32_t id, float dt) {  // :135-145
-    float total = 0;
+real wallStepWork(const R& r, uint32_t id, real dt) {  // :135-145
+    real total = 0;
     for (uint32_t w = 0; w < r.c.wallCount; w++) {
         uint32_t index = w * r.c.N + id;
-        float increment = dt * r.wallRows[3 * index + 1].w;
+        real increment = dt * r.wallRows[3 * index + 1].w;
         float4 state = r.wallAccumulated[index];
         float3 high = {state.x, 0, 0}, low = {state.y, 0, 0};
         compensated3(high, low, {increment, 0, 0});
@@ -540,19 +552,19 @@ float wallStepWork(const R& r, uint32_t id, float dt) {  // :135-145
     }
     return total;
 }
-struct float2p { float x, y; };
-inline float2p wallTwoSum(float a, float b) {  // :578-582 (materialization mask 0)
-    float high = a + b, v = high - a;
-    float low = (a - (high - v)) + (b - v);
+struct float2p { real x, y; };
+inline float2p wallTwoSum(real a, real b) {  // :578-582 (materialization mask 0)
+    real high = a + b, v = high - a;
+    real low = (a - (high - v)) + (b - v);
     return {high, low};
 }
-inline float2p wallAdd(float2p a, float b) {  // :583-586
+inline float2p wallAdd(float2p a, real b) {  // :583-586
     float2p sum = wallTwoSum(a.x, b);
     return wallTwoSum(sum.x, sum.y + a.y);
 }
 inline float2p wallAddDot(float2p sum, float3 a, float3 b) {  // :587-594
-    float product[3] = {a.x * b.x, a.y * b.y, a.z * b.z};
-    float error[3] = {std::fmaf(a.x, b.x, -product[0]), std::fmaf(a.y, b.y, -product[1]), std::fmaf(a.z, b.z, -product[2])};
+    real product[3] = {a.x * b.x, a.y * b.y, a.z * b.z};
+    real error[3] = {std::fma(a.x, b.x, -product[0]), std::fma(a.y, b.y, -product[1]), std::fma(a.z, b.z, -product[2])};
     for (int axis = 0; axis < 3; axis++) {
         sum = wallAdd(sum, product[axis]);
         sum = wallAdd(sum, error[axis]);
@@ -568,7 +580,7 @@ float2p wallCentreDistance(const R& r, uint32_t body, uint32_t wall) {  // :596-
     sum = wallAddDot(sum, xyz(low), p);
     return wallAddDot(sum, xyz(low), pl);
 }
-float localWallDistance(const R& r, uint32_t body, uint32_t wall, float3 arm) {  // :605-610
+real localWallDistance(const R& r, uint32_t body, uint32_t wall, float3 arm) {  // :605-610
     float2p sum = wallCentreDistance(r, body, wall);
     sum = wallAddDot(sum, xyz(r.wallPlane[wall]), arm);
     sum = wallAddDot(sum, xyz(r.wallRemainder[wall]), arm);
@@ -578,19 +590,19 @@ bool wallClusterMayContact(const R& r, uint32_t body, uint32_t wall, uint32_t cl
     float3 low = xyz(r.meshClusterMin[cluster]), high = xyz(r.meshClusterMax[cluster]);
     float4 plane = r.wallPlane[wall];
     float3 normal = qinv(r.rotation[body], xyz(plane)) * r.body[body].x;
-    float3 center = 0.5f * (low + high), half = 0.5f * (high - low);
+    float3 center = real(0.5) * (low + high), half = real(0.5) * (high - low);
     float2p centreDistance = wallCentreDistance(r, body, wall);
-    float base = centreDistance.x + centreDistance.y;
-    float support = dot(normal, center) - dot(fabs3(normal), half);
-    float pad = 64.0f * 1.1920928955078125e-7f *
-                (1.0f + std::fabs(plane.w) + length(xyz(r.position[body])) + length(normal) * (length(center) + length(half)));
+    real base = centreDistance.x + centreDistance.y;
+    real support = dot(normal, center) - dot(fabs3(normal), half);
+    real pad = real(64.0) * real(1.1920928955078125e-7) *
+                (real(1.0) + std::fabs(plane.w) + length(xyz(r.position[body])) + length(normal) * (length(center) + length(half)));
     return base + support <= pad;
 }
 void appendWallContact(const R& r, uint32_t body, uint32_t wall, uint32_t node) {  // :625-638
     float4 point = r.sample[r.sampleInfo[r.shapeId[body]].x + node];
     float3 arm = qrot(r.rotation[body], xyz(point) * r.body[body].x);
     float4 plane = r.wallPlane[wall];
-    float phi = localWallDistance(r, body, wall, arm);
+    real phi = localWallDistance(r, body, wall, arm);
     if (!std::isfinite(phi)) { fail(r, 512); return; }
     if (phi >= 0) return;
     if (-phi > r.c.maxOverlap * r.body[body].w) fail(r, 4);
@@ -605,8 +617,8 @@ void collectWallContacts(const R& r, uint32_t id) {  // CPU serial order, cpu_sc
     if (wall >= r.c.wallCount || r.control[0] != 0) return;
     float4 plane = r.wallPlane[wall];
     float2p centreDistance = wallCentreDistance(r, body, wall);
-    float phi = centreDistance.x + centreDistance.y;
-    float pad = 64.0f * 1.1920928955078125e-7f * (1.0f + length(xyz(r.position[body])) + std::fabs(plane.w));
+    real phi = centreDistance.x + centreDistance.y;
+    real pad = real(64.0) * real(1.1920928955078125e-7) * (real(1.0) + length(xyz(r.position[body])) + std::fabs(plane.w));
     if (phi > r.body[body].z + pad) return;
     uint4 info = r.meshClusterInfo[r.shapeId[body]];
     uint32_t cluster = info.x;
@@ -633,45 +645,45 @@ void advanceWalls(const R& r) {  // :669-687
     if (r.c.wallBoxEnabled != 0) {
         for (uint32_t axis = 0; axis < 3; axis++) {
             uint32_t lo = r.wallBoxIds[2 * axis], hi = r.wallBoxIds[2 * axis + 1];
-            float low = r.wallPlane[lo].w + r.wallRemainder[lo].w;
-            float high = -(r.wallPlane[hi].w + r.wallRemainder[hi].w);
+            real low = r.wallPlane[lo].w + r.wallRemainder[lo].w;
+            real high = -(r.wallPlane[hi].w + r.wallRemainder[hi].w);
             if (low < 0 || high > get(xyz(r.cell[0]), int(axis)) || high <= low) fail(r, 4096);
         }
     }
 }
 // planeViscousResponse (plane.slang:13-44) with normalResponse weight 1 (normal.slang:3-11)
-struct PlaneResponse { float4 forceEnergy, slipLoss; bool sliding; float normalLossPower; };
-PlaneResponse planeViscousResponse(float3 normal, float signedDistance, float area, float3 rv, float3 oldSlip, float kn,
-                                   float kt, float mu, float viscosity, float dt) {
+struct PlaneResponse { float4 forceEnergy, slipLoss; bool sliding; real normalLossPower; };
+PlaneResponse planeViscousResponse(float3 normal, real signedDistance, real area, float3 rv, float3 oldSlip, real kn,
+                                   real kt, real mu, real viscosity, real dt) {
     PlaneResponse value{{0, 0, 0, 0}, {0, 0, 0, 0}, false, 0};
     if (signedDistance >= 0) return value;
-    float depth = -signedDistance;
+    real depth = -signedDistance;
     float3 tangent = rv - normal * dot(rv, normal);
     float3 trial = oldSlip - normal * dot(oldSlip, normal) + dt * tangent;
     float3 slip = trial;
-    float normalVelocity = dot(rv, normal);
-    float fn = 0;
+    real normalVelocity = dot(rv, normal);
+    real fn = 0;
     {
-        float elastic = 1.0f * kn * area * depth;
-        fn = viscosity == 0 ? elastic : std::fmax(0.0f, elastic - 1.0f * viscosity * area * normalVelocity);
-        value.normalLossPower = std::fmax(0.0f, (elastic - fn) * normalVelocity);
+        real elastic = real(1.0) * kn * area * depth;
+        fn = viscosity == 0 ? elastic : std::fmax(real(0.0), elastic - real(1.0) * viscosity * area * normalVelocity);
+        value.normalLossPower = std::fmax(real(0.0), (elastic - fn) * normalVelocity);
     }
     float3 ft = -kt * area * slip;
-    float magnitude = length(ft);
+    real magnitude = length(ft);
     if (magnitude > mu * fn && magnitude > 0) {
         value.sliding = true;
         ft = ft * (mu * fn / magnitude);
         slip = -ft / (kt * area);
     }
-    value.forceEnergy = f4(fn * normal + ft, 0.5f * area * (kn * depth * depth + kt * dot(slip, slip)));
-    value.slipLoss = f4(slip, std::fmax(0.0f, -dot(ft, trial - slip)));
+    value.forceEnergy = f4(fn * normal + ft, real(0.5) * area * (kn * depth * depth + kt * dot(slip, slip)));
+    value.slipLoss = f4(slip, std::fmax(real(0.0), -dot(ft, trial - slip)));
     return value;
 }
 void gatherWallRows(const R& r, uint32_t index) {  // :837-856
     if (index >= r.c.N * r.c.wallCount || r.control[0] != 0) return;
     uint32_t body = index % r.c.N, wall = index / r.c.N, count = 0, visited = 0;
     float3 reaction = {0, 0, 0}, moment = {0, 0, 0};
-    float power = 0;
+    real power = 0;
     uint32_t entry = r.heads[body];
     while (entry != EMPTY) {
         if (visited++ > 2 * r.c.contactCapacity) { fail(r, 128); return; }
@@ -687,9 +699,9 @@ void gatherWallRows(const R& r, uint32_t index) {  // :837-856
         entry = r.next[entry];
     }
     float4 work = r.wallAccumulated[index];
-    float countBits;
+    float countBits;  // the count's 32-bit pattern, stored as an FP32 value
     std::memcpy(&countBits, &count, 4);
-    r.wallRows[3 * index] = f4(reaction, countBits);
+    r.wallRows[3 * index] = f4(reaction, real(countBits));
     r.wallRows[3 * index + 1] = f4(moment, power);
     r.wallRows[3 * index + 2] = {work.x + work.y, 0, 0, 0};
 }
@@ -703,26 +715,26 @@ void refreshWallPower(const R& r, uint32_t index) {  // :950-955
 // ---------------------------------------------------------------- kernels
 void beginStep(const R& r, uint32_t id) {  // solver.slang:859-888
     if (id >= r.c.N || r.control[0] != 0) return;
-    float dt = r.c.dt;
+    real dt = r.c.dt;
     float3 rate = xyz(r.cell[1]);
-    accumulateNormalLoss(r, id, 0.5f * dt);
+    accumulateNormalLoss(r, id, real(0.5) * dt);
     float3 v = xyz(r.velocity[id]), vl = xyz(r.velocityRemainder[id]);
     float3 l = xyz(r.momentum[id]), ll = xyz(r.momentumRemainder[id]);
-    compensated3(v, vl, 0.5f * dt * xyz(r.force[id]) / r.body[id].y);
-    compensated3(l, ll, 0.5f * dt * xyz(r.torque[id]));
-    float loss = -expm1Stable(-r.c.drag * dt) * (0.5f * r.body[id].y * dot(v + vl, v + vl) + rotationalEnergy(r, id, l + ll));
-    accumulate(r, id, {0, loss, 0.5f * dt * cellPower(r, id), wallStepWork(r, id, 0.5f * dt)});
-    float damping = expm1Stable(-0.5f * r.c.drag * dt);
+    compensated3(v, vl, real(0.5) * dt * xyz(r.force[id]) / r.body[id].y);
+    compensated3(l, ll, real(0.5) * dt * xyz(r.torque[id]));
+    real loss = -expm1Stable(-r.c.drag * dt) * (real(0.5) * r.body[id].y * dot(v + vl, v + vl) + rotationalEnergy(r, id, l + ll));
+    accumulate(r, id, {0, loss, real(0.5) * dt * cellPower(r, id), wallStepWork(r, id, real(0.5) * dt)});
+    real damping = expm1Stable(-real(0.5) * r.c.drag * dt);
     compensated3(v, vl, damping * (v + vl));
     compensated3(l, ll, damping * (l + ll));
     r.momentum[id] = f4(l, 0);
     r.momentumRemainder[id] = f4(ll, 0);
     float4 q = r.rotation[id], ql = r.rotationRemainder[id];
     float3 w = omega(r, id);
-    r.rotation[id] = normalize4(q + rotationIncrement(w, 0.5f * dt, q));
+    r.rotation[id] = normalize4(q + rotationIncrement(w, real(0.5) * dt, q));
     w = omega(r, id);
     compensated4(q, ql, rotationIncrement(w, dt, q));
-    if (std::fabs(dot4(q, q) - 1.0f) > 2e-6f) { q = normalize4(q); ql = {0, 0, 0, 0}; }
+    if (std::fabs(dot4(q, q) - real(1.0)) > real(2e-6)) { q = normalize4(q); ql = {0, 0, 0, 0}; }
     r.rotation[id] = q;
     r.rotationRemainder[id] = ql;
     float3 logarithm = xyz(r.cellLog[0]), residual = xyz(r.cellLog[1]);
@@ -750,26 +762,26 @@ void advanceCell(const R& r) {  // :890-897
     r.cellLog[1] = f4(residual, 0);
     r.cell[0] = f4(xyz(r.cell[2]) * f3(exp32(logarithm.x), exp32(logarithm.y), exp32(logarithm.z)), 0);
     r.control[4] = 0;
-    if (r.c.periodic != 0 && r.referenceCell[1].w >= 0.5f * std::fmin(r.cell[0].x, std::fmin(r.cell[0].y, r.cell[0].z)))
+    if (r.c.periodic != 0 && r.referenceCell[1].w >= real(0.5) * std::fmin(r.cell[0].x, std::fmin(r.cell[0].y, r.cell[0].z)))
         fail(r, 256);
 }
 void checkRebuild(const R& r, uint32_t id) {  // :899-904
     if (id >= r.c.N) return;
     float3 d = image(r, (xyz(r.position[id]) - xyz(r.referencePosition[id])) +
                             (xyz(r.positionRemainder[id]) - xyz(r.referencePositionRemainder[id])));
-    float deformation = length(xyz(r.cell[0]) - xyz(r.referenceCell[0]));
-    if (length(d) + deformation > 0.5f * r.c.skin) r.control[4] |= 1;
+    real deformation = length(xyz(r.cell[0]) - xyz(r.referenceCell[0]));
+    if (length(d) + deformation > real(0.5) * r.c.skin) r.control[4] |= 1;
 }
 void finishStep(const R& r, uint32_t id) {  // :906-920
     if (id >= r.c.N || r.control[0] != 0) return;
-    accumulateNormalLoss(r, id, 0.5f * r.c.dt);
+    accumulateNormalLoss(r, id, real(0.5) * r.c.dt);
     float3 v = xyz(r.velocity[id]), vl = xyz(r.velocityRemainder[id]);
     float3 l = xyz(r.momentum[id]), ll = xyz(r.momentumRemainder[id]);
-    compensated3(v, vl, 0.5f * r.c.dt * xyz(r.force[id]) / r.body[id].y);
-    compensated3(l, ll, 0.5f * r.c.dt * xyz(r.torque[id]));
-    float loss = -expm1Stable(-r.c.drag * r.c.dt) * (0.5f * r.body[id].y * dot(v + vl, v + vl) + rotationalEnergy(r, id, l + ll));
-    accumulate(r, id, {0, loss, 0.5f * r.c.dt * cellPower(r, id), wallStepWork(r, id, 0.5f * r.c.dt)});
-    float damping = expm1Stable(-0.5f * r.c.drag * r.c.dt);
+    compensated3(v, vl, real(0.5) * r.c.dt * xyz(r.force[id]) / r.body[id].y);
+    compensated3(l, ll, real(0.5) * r.c.dt * xyz(r.torque[id]));
+    real loss = -expm1Stable(-r.c.drag * r.c.dt) * (real(0.5) * r.body[id].y * dot(v + vl, v + vl) + rotationalEnergy(r, id, l + ll));
+    accumulate(r, id, {0, loss, real(0.5) * r.c.dt * cellPower(r, id), wallStepWork(r, id, real(0.5) * r.c.dt)});
+    real damping = expm1Stable(-real(0.5) * r.c.drag * r.c.dt);
     compensated3(v, vl, damping * (v + vl));
     compensated3(l, ll, damping * (l + ll));
     if (!finite3(v) || !finite3(l)) { fail(r, 512); return; }
@@ -805,9 +817,9 @@ void fillBins(const R& r, uint32_t id) {  // :351-359
 uint32_t findPairs(const R& r, uint32_t id, bool emit) {  // :360-384
     uint32_t b[3];
     binIndex(r, id, b);
-    float cutoff = r.referenceCell[1].w;
+    real cutoff = r.referenceCell[1].w;
     float3 cellBox = xyz(r.cell[0]);
-    float3 g = f3(cutoff, cutoff, cutoff) / cellBox * f3(float(r.c.bins[0]), float(r.c.bins[1]), float(r.c.bins[2]));
+    float3 g = f3(cutoff, cutoff, cutoff) / cellBox * f3(real(r.c.bins[0]), real(r.c.bins[1]), real(r.c.bins[2]));
     int reach[3] = {int(std::ceil(g.x)), int(std::ceil(g.y)), int(std::ceil(g.z))};
     int low[3], width[3];
     for (int a = 0; a < 3; a++) {
@@ -833,7 +845,7 @@ uint32_t findPairs(const R& r, uint32_t id, bool emit) {  // :360-384
                     uint32_t j = r.binParticles[bi * r.c.binCapacity + slot];
                     if (j <= id) continue;
                     float3 d = centerBranch(r, id, j);
-                    float range = r.body[id].z + r.body[j].z + r.c.skin;
+                    real range = r.body[id].z + r.body[j].z + r.c.skin;
                     if (dot(d, d) > range * range) continue;
                     if (emit) {
                         uint32_t index = r.pairOffsets[id] + count;
@@ -875,8 +887,8 @@ void resetMeshContacts(const R& r, uint32_t id) {  // :397-403
     if (id == 0) { r.control[2] = 0; r.control[6] = 0; }
 }
 void meshPairBounds(const R& r, uint32_t direction, uint32_t cluster, float3& low, float3& high) {  // :404-417
-    float3 center = 0.5f * (xyz(r.meshClusterMin[cluster]) + xyz(r.meshClusterMax[cluster]));
-    float3 half = 0.5f * (xyz(r.meshClusterMax[cluster]) - xyz(r.meshClusterMin[cluster]));
+    float3 center = real(0.5) * (xyz(r.meshClusterMin[cluster]) + xyz(r.meshClusterMax[cluster]));
+    float3 half = real(0.5) * (xyz(r.meshClusterMax[cluster]) - xyz(r.meshClusterMin[cluster]));
     float3 c, e;
     for (int axis = 0; axis < 3; axis++) {
         float4 row = r.meshPairTransform[4 * direction + axis];
@@ -885,7 +897,7 @@ void meshPairBounds(const R& r, uint32_t direction, uint32_t cluster, float3& lo
     }
     low = c - e;
     high = c + e;
-    float pad = 256.0f * EPS * (1.0f + maxabs(low) + maxabs(high) + r.meshPairTransform[4 * direction + 3].x);
+    real pad = real(256.0) * EPS * (real(1.0) + maxabs(low) + maxabs(high) + r.meshPairTransform[4 * direction + 3].x);
     low = low - f3(pad, pad, pad);
     high = high + f3(pad, pad, pad);
 }
@@ -898,7 +910,7 @@ void prepareMeshPairs(const R& r, uint32_t id) {  // :469-489
     for (uint32_t pass = 0; pass < 2; pass++) {
         if ((mask & (1u << pass)) == 0) continue;
         uint32_t i = pass == 0 ? ij.x : ij.y, j = pass == 0 ? ij.y : ij.x;
-        float ratio = r.body[i].x / r.body[j].x;
+        real ratio = r.body[i].x / r.body[j].x;
         float3 a = qinv(r.rotation[j], qrot(r.rotation[i], f3(ratio, 0, 0)));
         float3 b = qinv(r.rotation[j], qrot(r.rotation[i], f3(0, ratio, 0)));
         float3 c = qinv(r.rotation[j], qrot(r.rotation[i], f3(0, 0, ratio)));
@@ -920,12 +932,12 @@ inline uint32_t hashKey(const R& r, uint4 key) {  // :689
     h ^= h >> 16;
     return h & (r.c.hashSize - 1);
 }
-inline float3 transported(float3 x, float3 a, float3 b, float spin) {  // :691-696
-    float c = std::fmin(std::fmax(dot(a, b), -1.0f), 1.0f);
-    if (c < -0.999999f) return {0, 0, 0};
+inline float3 transported(float3 x, float3 a, float3 b, real spin) {  // :691-696
+    real c = std::fmin(std::fmax(dot(a, b), -real(1.0)), real(1.0));
+    if (c < -real(0.999999)) return {0, 0, 0};
     float3 v = cross(a, b);
-    float3 out = x + cross(v, x) + cross(v, cross(v, x)) / (1.0f + c);
-    out = out * cos32(spin) + cross(b, out) * sin32(spin) + b * dot(b, out) * (1.0f - cos32(spin));
+    float3 out = x + cross(v, x) + cross(v, cross(v, x)) / (real(1.0) + c);
+    out = out * cos32(spin) + cross(b, out) * sin32(spin) + b * dot(b, out) * (real(1.0) - cos32(spin));
     return out - b * dot(out, b);
 }
 void resolveContacts(const R& r, uint32_t id) {  // :703-749
@@ -934,7 +946,7 @@ void resolveContacts(const R& r, uint32_t id) {  // :703-749
     uint32_t i = key.x, j = key.y;
     float3 n = xyz(r.normalArea[id]);
     bool wall = j >= r.c.N;
-    float viscosity = r.c.normalViscosity;
+    real viscosity = r.c.normalViscosity;
     if (wall) viscosity = r.wallMaterial[j - r.c.N].w;
     float3 wi = viscosity > 0 ? xyz(r.forceOmega[i]) : omega(r, i), wj = {0, 0, 0};
     if (!wall) wj = viscosity > 0 ? xyz(r.forceOmega[j]) : omega(r, j);
@@ -946,7 +958,7 @@ void resolveContacts(const R& r, uint32_t id) {  // :703-749
             if (old == EMPTY) break;
             uint4 om = r.oldMeta[old];
             if (om.x == key.x && om.y == key.y && om.z == key.z) {
-                elastic = transported(xyz(r.oldSlip[old]), xyz(r.oldNormal[old]), n, 0.5f * dot(wi + wj, n) * r.c.dt);
+                elastic = transported(xyz(r.oldSlip[old]), xyz(r.oldNormal[old]), n, real(0.5) * dot(wi + wj, n) * r.c.dt);
                 break;
             }
             slot = (slot + 1) & (r.c.hashSize - 1);
@@ -979,25 +991,25 @@ void resolveContacts(const R& r, uint32_t id) {  // :703-749
     float3 rv = translation - xyz(r.cell[1]) * branch + cross(wi, arm) - cross(wj, arm - branch);
     elastic = elastic + (rv - n * dot(rv, n)) * r.c.dt;
     float3 trial = elastic;
-    float area = r.normalArea[id].w, depth = r.pointDepth[id].w;
+    real area = r.normalArea[id].w, depth = r.pointDepth[id].w;
     // normalResponse (normal.slang:3-11) with weight 0.5
-    float fn = 0, lossPower = 0;
-    float normalVelocity = dot(rv, n);
+    real fn = 0, lossPower = 0;
+    real normalVelocity = dot(rv, n);
     if (depth > 0) {
-        float el = 0.5f * r.c.kn * area * depth;
-        fn = viscosity == 0 ? el : std::fmax(0.0f, el - 0.5f * viscosity * area * normalVelocity);
-        lossPower = std::fmax(0.0f, (el - fn) * normalVelocity);
+        real el = real(0.5) * r.c.kn * area * depth;
+        fn = viscosity == 0 ? el : std::fmax(real(0.0), el - real(0.5) * viscosity * area * normalVelocity);
+        lossPower = std::fmax(real(0.0), (el - fn) * normalVelocity);
     }
     r.normalLossPower[id] = lossPower;
-    float3 ft = -0.5f * r.c.kt * area * elastic;
-    float magnitude = length(ft);
+    float3 ft = -real(0.5) * r.c.kt * area * elastic;
+    real magnitude = length(ft);
     if (magnitude > r.c.mu * fn && magnitude > 0) {
         ft = ft * (r.c.mu * fn / magnitude);
-        elastic = -ft / (0.5f * r.c.kt * area);
+        elastic = -ft / (real(0.5) * r.c.kt * area);
         r.control[6] += 1;
     }
-    r.slip[id] = f4(elastic, std::fmax(0.0f, -dot(ft, trial - elastic)));
-    r.contactForce[id] = f4(fn * n + ft, 0.25f * area * (r.c.kn * depth * depth + r.c.kt * dot(elastic, elastic)));
+    r.slip[id] = f4(elastic, std::fmax(real(0.0), -dot(ft, trial - elastic)));
+    r.contactForce[id] = f4(fn * n + ft, real(0.25) * area * (r.c.kn * depth * depth + r.c.kt * dot(elastic, elastic)));
     float4 cf = r.contactForce[id];
     if (!std::isfinite(cf.x) || !std::isfinite(cf.y) || !std::isfinite(cf.z) || !std::isfinite(cf.w) ||
         !std::isfinite(lossPower)) { fail(r, 512); return; }
@@ -1021,10 +1033,10 @@ void gather(const R& r, uint32_t id) {  // :761-811
     if (id >= r.c.N || r.control[0] != 0) return;
     float3 f = {0, 0, 0}, t = {0, 0, 0}, fLow = {0, 0, 0}, tLow = {0, 0, 0};
     float3 stress0 = {0, 0, 0}, stress1 = {0, 0, 0}, stress2 = {0, 0, 0};
-    float energy = 0, denom = 0, maxDepth = 0, dissipation = 0, normalPower = 0;
-    float stiffnessRate = 0, viscousRate = 0;
+    real energy = 0, denom = 0, maxDepth = 0, dissipation = 0, normalPower = 0;
+    real stiffnessRate = 0, viscousRate = 0;
     float3 one = {1, 1, 1};
-    float inverseInertia = std::fmax(std::fmax(dot(fabs3(xyz(r.invI[3 * id])), one), dot(fabs3(xyz(r.invI[3 * id + 1])), one)),
+    real inverseInertia = std::fmax(std::fmax(dot(fabs3(xyz(r.invI[3 * id])), one), dot(fabs3(xyz(r.invI[3 * id + 1])), one)),
                                      dot(fabs3(xyz(r.invI[3 * id + 2])), one));
     uint32_t entry = r.heads[id];
     uint32_t visited = 0;
@@ -1038,13 +1050,13 @@ void gather(const R& r, uint32_t id) {  // :761-811
         bool wall = key.y >= r.c.N;
         if (r.c.activeTimestep != 0) {
             float3 localArm = source ? arm : arm - centerBranch(r, key.x, key.y);
-            float mobility = 1.0f / r.body[id].y + dot(localArm, localArm) * inverseInertia;
+            real mobility = real(1.0) / r.body[id].y + dot(localArm, localArm) * inverseInertia;
             float4 material = {r.c.kn, r.c.kt, r.c.mu, r.c.normalViscosity};
             if (wall) material = r.wallMaterial[key.y - r.c.N];
-            float area = r.normalArea[contact].w * (wall ? 1.0f : 0.5f);
-            float coupling = wall ? 1.0f : 2.0f;
-            stiffnessRate += coupling * area * (material.x * (1.0f + material.z) + material.y) * mobility;
-            viscousRate += coupling * area * material.w * (1.0f + material.z) * mobility;
+            real area = r.normalArea[contact].w * (wall ? real(1.0) : real(0.5));
+            real coupling = wall ? real(1.0) : real(2.0);
+            stiffnessRate += coupling * area * (material.x * (real(1.0) + material.z) + material.y) * mobility;
+            viscousRate += coupling * area * material.w * (real(1.0) + material.z) * mobility;
         }
         if (wall) {
             compensated3(f, fLow, force);
@@ -1063,14 +1075,14 @@ void gather(const R& r, uint32_t id) {  // :761-811
         float3 branch = centerBranch(r, key.x, key.y);
         compensated3(f, fLow, source ? force : -force);
         compensated3(t, tLow, source ? cross(arm, force) : cross(arm - branch, -force));
-        stress0 = stress0 - 0.5f * force.x * branch;
-        stress1 = stress1 - 0.5f * force.y * branch;
-        stress2 = stress2 - 0.5f * force.z * branch;
-        energy += 0.5f * r.contactForce[contact].w;
+        stress0 = stress0 - real(0.5) * force.x * branch;
+        stress1 = stress1 - real(0.5) * force.y * branch;
+        stress2 = stress2 - real(0.5) * force.z * branch;
+        energy += real(0.5) * r.contactForce[contact].w;
         denom += length(force);
         maxDepth = std::fmax(maxDepth, r.pointDepth[contact].w / std::fmin(r.body[key.x].w, r.body[key.y].w));
-        dissipation += 0.5f * r.slip[contact].w;
-        normalPower += 0.5f * r.normalLossPower[contact];
+        dissipation += real(0.5) * r.slip[contact].w;
+        normalPower += real(0.5) * r.normalLossPower[contact];
         entry = r.next[entry];
     }
     f = f + fLow;
@@ -1084,28 +1096,28 @@ void gather(const R& r, uint32_t id) {  // :761-811
     r.particleStress[3 * id] = f4(stress0, 0);
     r.particleStress[3 * id + 1] = f4(stress1, 0);
     r.particleStress[3 * id + 2] = f4(stress2, 0);
-    float kinetic = 0.5f * (r.body[id].y * dot(xyz(r.velocity[id]), xyz(r.velocity[id])) + dot(xyz(r.momentum[id]), omega(r, id)));
+    real kinetic = real(0.5) * (r.body[id].y * dot(xyz(r.velocity[id]), xyz(r.velocity[id])) + dot(xyz(r.momentum[id]), omega(r, id)));
     r.particleDiagnostic[id] = {kinetic, energy, length(f), denom};
     accumulate(r, id, {dissipation, 0, 0, 0});
 }
 void checkContactTimestep(const R& r, uint32_t id) {  // :816-832
     if (id >= r.c.N || r.control[0] != 0 || r.c.dt == 0) return;
-    float dt = r.c.dt;
-    float rx = r.timestepRows[id].x, ry = r.timestepRows[id].y;
+    real dt = r.c.dt;
+    real rx = r.timestepRows[id].x, ry = r.timestepRows[id].y;
     float3 v = xyz(r.velocity[id]) + xyz(r.velocityRemainder[id]);
-    float speed = length(v) + dt * length(xyz(r.force[id])) / r.body[id].y;
+    real speed = length(v) + dt * length(xyz(r.force[id])) / r.body[id].y;
     float3 one = {1, 1, 1};
-    float inverseInertia = std::fmax(std::fmax(dot(fabs3(xyz(r.invI[3 * id])), one), dot(fabs3(xyz(r.invI[3 * id + 1])), one)),
+    real inverseInertia = std::fmax(std::fmax(dot(fabs3(xyz(r.invI[3 * id])), one), dot(fabs3(xyz(r.invI[3 * id + 1])), one)),
                                      dot(fabs3(xyz(r.invI[3 * id + 2])), one));
-    float spin = inverseInertia * (length(xyz(r.momentum[id]) + xyz(r.momentumRemainder[id])) + dt * length(xyz(r.torque[id])));
-    float wallSpeed = 0;
+    real spin = inverseInertia * (length(xyz(r.momentum[id]) + xyz(r.momentumRemainder[id])) + dt * length(xyz(r.torque[id])));
+    real wallSpeed = 0;
     for (uint32_t w = 0; w < r.c.wallCount; w++) wallSpeed = std::fmax(wallSpeed, length(xyz(r.wallVelocity[w])));
-    float motion = dt * (speed + r.body[id].z * spin + wallSpeed);
+    real motion = dt * (speed + r.body[id].z * spin + wallSpeed);
     bool finite = std::isfinite(rx) && std::isfinite(ry) && std::isfinite(motion) && std::isfinite(spin);
     if (!finite) { fail(r, 512); return; }
     float3 rate = fabs3(xyz(r.cell[1])) * dt;
-    if (dt * std::sqrt(rx) > 0.1f || dt * ry > 0.1f || dt * spin > 0.01f || motion > 0.25f * r.c.skin ||
-        rate.x > 0.01f || rate.y > 0.01f || rate.z > 0.01f)
+    if (dt * std::sqrt(rx) > real(0.1) || dt * ry > real(0.1) || dt * spin > real(0.01) || motion > real(0.25) * r.c.skin ||
+        rate.x > real(0.01) || rate.y > real(0.01) || rate.z > real(0.01))
         fail(r, 8192);
 }
 
@@ -1215,6 +1227,10 @@ extern "C" int oracle_dispatch(const char* kernel, uint32_t count, void** buffer
 
 extern "C" int oracle_probe(const char* kernel, const uint32_t* uniforms, void** buffers, const uint64_t* bytes,
                             const double* values, const void* input, void* output) {
+#ifdef SAD_ORACLE_FP64
+    (void)kernel, (void)uniforms, (void)buffers, (void)bytes, (void)values, (void)input, (void)output;
+    return 3;  // probe inputs and outputs are FP32 rows; probes are for the FP32 build
+#endif
     try {
         R r = bind(buffers, bytes, values);
         std::string name(kernel);
@@ -1249,7 +1265,7 @@ extern "C" int oracle_probe(const char* kernel, const uint32_t* uniforms, void**
                 bool hit = nodeContact(r, rows[id].x, rows[id].y, rows[id].z, pd, na);
                 out[3 * id] = pd;
                 out[3 * id + 1] = na;
-                out[3 * id + 2] = {hit ? 1.0f : 0.0f, 0, 0, 0};
+                out[3 * id + 2] = {hit ? real(1.0) : real(0.0), 0, 0, 0};
             }
         } else if (name == "transportProbe") {
             // input float4 rows (x, a, b, spin in b.w) -> transported vector.

@@ -16,12 +16,22 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIBRARY = HERE / "liboracle.so"
+LIBRARY64 = HERE / "liboracle64.so"
 REF_COLLECTOR = HERE / "_ref" / "ref_collector.so"
+# Slots the restatement reads as integers (cpu_backend.cpp `bind`); every other
+# slot holds FP32 values, which the FP64 build reads as doubles.
+INTEGER_SLOTS = frozenset((
+    "control", "shapeId", "sampleInfo", "meshRoots", "meshHullInfo", "meshLinks", "meshFrontier", "meshPairMask",
+    "meshClusterInfo", "meshClusterRanges", "meshSampleIndices", "binCounts", "binParticles", "pairCounts",
+    "pairOffsets", "pairs", "contactCounts", "contactOffsets", "meta", "oldMeta", "oldHash", "hash", "heads", "next",
+    "wallBoxIds",
+))
 
 
 def build(reference=True):
-    """Compile the oracle (and the reference collector when the reference exists)."""
-    targets = ["liboracle.so"] + (["ref"] if reference else [])
+    """Compile the oracle, its FP64 twin, and the reference collector when the
+    reference exists."""
+    targets = ["liboracle.so", "liboracle64.so"] + (["ref"] if reference else [])
     subprocess.run(["make", "-s", "-C", str(HERE), *targets], check=True)
 
 
@@ -51,6 +61,50 @@ class Buffer:
         return self.array.copy()
 
 
+class WideBuffer(Buffer):
+    """FP32-layout buffer with an FP64 shadow for the FP64 build.
+
+    The FP32 bytes stay the buffer's face (reads, uploads, integer use); once the
+    buffer is bound to an FP32 slot its values live in `wide` (float64, one double
+    per float) and the FP32 face is refreshed from it, rounded, when read.
+    """
+
+    def __init__(self, array=None, *, count=None, stride=4):
+        super().__init__(array, count=count, stride=stride)
+        self.wide = None
+        self.stale = False
+
+    def widen(self):
+        if self.wide is None:
+            self.wide = self.array[: self.size // 4 * 4].view(np.float32).astype(np.float64)
+        self.stale = True  # a kernel may write it
+        return self.wide
+
+    def sync(self):
+        if self.wide is not None and self.stale:
+            self.array[: self.wide.size * 4] = self.wide.astype(np.float32).view(np.uint8)
+            self.stale = False
+
+    def copy_from_numpy(self, array):
+        super().copy_from_numpy(array)
+        self.wide, self.stale = None, False
+
+    def to_numpy(self):
+        self.sync()
+        return super().to_numpy()
+
+    def write_prefix(self, data):
+        self.sync()
+        self.array[: data.size] = data
+        if self.wide is not None:
+            n = data.size // 4
+            self.wide[:n] = self.array[: 4 * n].view(np.float32)
+
+    def read64(self):
+        """The values at full precision (FP32 face if never bound as floats)."""
+        return self.wide.copy() if self.wide is not None else self.array.view(np.float32).astype(np.float64)
+
+
 class Encoder:
     def __init__(self):
         self.operations = []
@@ -58,31 +112,52 @@ class Encoder:
     def copy_buffer(self, dest, dest_offset, source, source_offset, size):
         if min(dest_offset, source_offset, size) < 0 or dest_offset + size > dest.size or source_offset + size > source.size:
             raise ValueError("CPU buffer copy out of bounds")
-        self.operations.append(
-            lambda: np.copyto(dest.array[dest_offset : dest_offset + size], source.array[source_offset : source_offset + size])
-        )
+
+        def copy():
+            wide = [b for b in (dest, source) if getattr(b, "wide", None) is not None]
+            if not wide:
+                np.copyto(dest.array[dest_offset : dest_offset + size], source.array[source_offset : source_offset + size])
+                return
+            if (dest_offset | source_offset | size) % 4:
+                raise ValueError("FP64 oracle copies whole elements")
+            src = source.widen()[source_offset // 4 : (source_offset + size) // 4]
+            dest.widen()[dest_offset // 4 : (dest_offset + size) // 4] = src
+
+        self.operations.append(copy)
 
 
-def pointer_table(bindings, order):
-    arrays = [bindings[name].array for name in order]
+def pointer_table(bindings, order, wide=False):
+    if wide:
+        arrays = []
+        for name in order:
+            b = bindings[name]
+            arrays.append(b.array if name in INTEGER_SLOTS else b.widen())
+        sizes = [bindings[name].size for name in order]  # FP32 byte sizes: element counts are unchanged
+    else:
+        arrays = [bindings[name].array for name in order]
+        sizes = [a.nbytes for a in arrays]
     pointers = (ct.c_void_p * len(arrays))(*(a.ctypes.data for a in arrays))
-    sizes = (ct.c_uint64 * len(arrays))(*(a.nbytes for a in arrays))
-    return pointers, sizes
+    return pointers, (ct.c_uint64 * len(arrays))(*sizes)
 
 
-def make_oracle(package, threads=1, collector="oracle"):
+def make_oracle(package, threads=1, collector="oracle", precision="fp32"):
     """Create the oracle adapter bound to the package's ABI tables.
 
     collector="reference" routes collectMeshFrontiers through the reference's own
     generated collector (`_ref/ref_collector.so`, cpu_dispatch ABI), everything else
-    through the restatement.
+    through the restatement.  precision="fp64" runs the FP64 build on FP64 shadows
+    of the FP32 buffers (`read64` gives the full-precision values).
     """
     order = package.engine.BUFFER_ORDER
     config_values = package.engine.config_values
-    if not LIBRARY.exists():
+    wide = precision == "fp64"
+    if precision not in ("fp32", "fp64") or (wide and collector != "oracle"):
+        raise ValueError("precision is fp32 or fp64 (fp64 with the restatement's collector)")
+    path = LIBRARY64 if wide else LIBRARY
+    if not path.exists():
         build(reference=False)
     lib, dispatch_fn = _load(
-        LIBRARY, "oracle_dispatch",
+        path, "oracle_dispatch",
         [ct.c_char_p, ct.c_uint32, ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_double)],
     )
     probe_fn = lib.oracle_probe
@@ -103,9 +178,12 @@ def make_oracle(package, threads=1, collector="oracle"):
         create.restype = ct.c_void_p
         ref_scratch = create()
 
+    buffer_class = WideBuffer if wide else Buffer
+
     class Oracle:
         name = "cpu"
-        buffer_type = Buffer
+        buffer_type = buffer_class
+        precision = "fp64" if wide else "fp32"
 
         def __init__(self):
             self.threads = threads
@@ -113,14 +191,14 @@ def make_oracle(package, threads=1, collector="oracle"):
             self.calls = 0
 
         def buffer(self, array=None, *, count=None, stride=4):
-            return Buffer(array, count=count, stride=stride)
+            return buffer_class(array, count=count, stride=stride)
 
         def create_command_encoder(self):
             return Encoder()
 
         def dispatch(self, name, n, encoder=None, *, r):
             def execute():
-                pointers, sizes = pointer_table(r, order)
+                pointers, sizes = pointer_table(r, order, wide)
                 values = config_values(r["c"], self.threads)
                 vals = values.ctypes.data_as(ct.POINTER(ct.c_double))
                 if name == "collectMeshFrontiers" and ref_fn is not None:
@@ -159,12 +237,16 @@ def make_oracle(package, threads=1, collector="oracle"):
             pass
 
         def read(self, buffer, dtype=np.float32, shape=None):
+            if wide:
+                buffer.sync()
             array = buffer.array.view(dtype).copy()
             return array if shape is None else array.reshape(shape)
 
         def read_prefix(self, buffer, count, dtype=np.float32, shape=None):
             if type(count) is not int or count < 0 or count * np.dtype(dtype).itemsize > buffer.size:
                 raise ValueError("CPU buffer prefix out of bounds")
+            if wide:
+                buffer.sync()
             array = buffer.array[: count * np.dtype(dtype).itemsize].view(dtype).copy()
             return array if shape is None else array.reshape(shape)
 
@@ -172,6 +254,9 @@ def make_oracle(package, threads=1, collector="oracle"):
             data = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
             if data.size > buffer.size:
                 raise ValueError("CPU buffer prefix out of bounds")
-            buffer.array[: data.size] = data
+            if wide:
+                buffer.write_prefix(data)
+            else:
+                buffer.array[: data.size] = data
 
     return Oracle()
