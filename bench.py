@@ -32,6 +32,8 @@ WORKLOADS = {
               "256-4,096 nodes by area), phi 0.60, mu 0.5",
     "concave": "1,000 grains, 8 strongly non-convex harmonic templates (1,280 triangles, 1,024 nodes), phi 0.55, mu 0.5",
     "small": "1,000 icospheres (subdiv 2, sigma 0.1), 512 nodes/grain, phi 0.30, v ~ N(0, 0.01 m/s), mu 0.5",
+    "walls": "10,000 grains from the dense library at phi 0.58 in a nonperiodic box of six planes, top plane moving "
+             "down at 0.5 box heights/s, mu 0.5",
 }
 
 
@@ -309,6 +311,10 @@ def main():
         "parity": parity,
         "nn_culled": nn,
     }
+    if e.wall_count:  # wall reactions and work after the timed run (SI units)
+        e.restore(plain_state, step0 + args.steps)
+        out["walls"] = [{"normal": x["normal"], "offset_m": x["offset"], "reaction_N": list(x["reaction"]),
+                         "work_J": x["work"], "contacts": x["active_surface_contacts"]} for x in e.walls()]
     print(json.dumps(out), flush=True)
 
 
