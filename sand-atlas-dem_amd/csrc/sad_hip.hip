@@ -1006,19 +1006,132 @@ __device__ bool cullBatch(const P& p, const W& w, const TaskFrame& f, const uint
     return lane < 32 ? verdict[0] : second;
 }
 
-// Exact queries of queue[0..qn) in 64-node batches, ranked in queue order.
+// Exact queries of queue[0..qn), ranked in queue order, in two phases.
+//
+// Phase 1 finds every queued node's nearest triangle with lane refill: each lane
+// runs one BVH traversal (meshDistanceWide's exact visit order, one stack entry per
+// wave iteration) and, when it finishes, takes the next unqueried node, so lanes do
+// not idle while the wave's slowest traversal runs (per-query visit counts differ
+// by ~3x).  The winner's triangle index goes to nearest[k].
+//
+// Phase 2 takes the queue 64 nodes at a time in order, rebuilds each query's result
+// from its nearest triangle (closestTriangle is deterministic, so the point, squared
+// distance and feature equal the traversal's) and stages hits by wave rank: the same
+// contacts, values and ranks as querying each batch to completion.
+constexpr uint32_t NO_TRIANGLE = 0xffffffffu;
+__device__ __forceinline__ v3 queryPoint(const TaskFrame& f, float4 sp, v3& arm) {
+    arm = qrot(f.qi, xyz(sp) * f.s_i);
+    return qinv(f.qj, arm - f.branch) / f.s_j;
+}
 template <typename D>
 __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
-                           const uint32_t* queue, uint32_t qn, uint32_t* sref, D* sdist, uint32_t& running,
-                           int lane) {
-    for (uint32_t q = 0; q < qn; q += WAVE) {
-        uint32_t k = q + lane;
+                           const uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
+                           uint32_t& running, int lane) {
+    if (qn == 0) return;
+    const uint32_t root = p.meshRoots[f.sj].x;
+    const uint32_t rootRef = nodeRef(p, root);
+    const v3 rootLo = xyz(p.meshMin[root]), rootHi = xyz(p.meshMax[root]);
+    const float4* __restrict__ wide = w.wide;
+    uint32_t k = lane, top = 0, bestTri = NO_TRIANGLE;
+    float best = 1e30f;
+    v3 q = {0, 0, 0};
+    bool active = k < qn;
+    auto start = [&]() {
+        v3 arm;
+        q = queryPoint(f, p.sample[f.sampleBase + queue[k]], arm);
+        sref[0] = rootRef;
+        dstore(sdist, boxDistance(rootLo, rootHi, q));
+        top = 1;
+        best = 1e30f;
+        bestTri = NO_TRIANGLE;
+    };
+    if (active) start();
+    uint32_t next = WAVE;  // first unassigned queue entry (wave-uniform)
+    while (__ballot(active) != 0) {
+        if (active) {
+            --top;
+            const uint32_t ref = sref[top * WAVE];
+            if (!(dload(sdist + top * WAVE) > best)) {
+                if ((ref & LEAF) == 0) {
+                    const float4* node = wide + 4 * size_t(ref);
+                    float4 w0 = node[0], w1 = node[1], w2 = node[2], w3 = node[3];
+                    float da = boxDistance(xyz(w0), xyz(w1), q), db = boxDistance(xyz(w2), xyz(w3), q);
+                    uint32_t a = __float_as_uint(w0.w), b = __float_as_uint(w1.w);
+                    if (da < db) { uint32_t t = a; a = b; b = t; float u = da; da = db; db = u; }
+                    if (top + 2 > WSTACK) {
+                        fail(p, 1024);
+                        top = 0;
+                        bestTri = NO_TRIANGLE;
+                    } else {
+                        if (da <= best) { sref[top * WAVE] = a; dstore(sdist + top * WAVE, da); top++; }
+                        if (db <= best) { sref[top * WAVE] = b; dstore(sdist + top * WAVE, db); top++; }
+                    }
+                } else {
+                    uint32_t first = ref & 0x0fffffffu, count = ((ref >> 28) & 3u) + 1;
+                    for (uint32_t c = 0; c < count; c++) {
+                        uint32_t triangle = first + c, feature;
+                        const float4* t = p.meshTriangles + 3 * size_t(triangle);
+                        v3 x = closestTriangle(q, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
+                        float distance = dot(q - x, q - x);
+                        if (distance < best) { best = distance; bestTri = triangle; }
+                    }
+                }
+            }
+            if (top == 0) {
+                nearestTri[k] = bestTri;
+                active = false;
+            }
+        }
+        const uint64_t idle = __ballot(!active);
+        if (next < qn && idle != 0) {
+            const uint32_t slot = next + __popcll(idle & lanes_below(lane));
+            if (!active && slot < qn) {
+                k = slot;
+                active = true;
+                start();
+            }
+            next += __popcll(idle);
+        }
+    }
+    wave_sync();
+    for (uint32_t b = 0; b < qn; b += WAVE) {
+        const uint32_t kk = b + lane;
         bool hit = false;
         uint32_t node = 0;
         float4 pd = make_float4(0, 0, 0, 0), na = make_float4(0, 0, 0, 0);
-        if (k < qn) {
-            node = queue[k];
-            hit = queryNode(p, w, f, node, sref, sdist, pd, na);
+        if (kk < qn) {
+            node = queue[kk];
+            const float4 sp = p.sample[f.sampleBase + node];
+            v3 arm;
+            const v3 x = queryPoint(f, sp, arm);
+            // meshDistanceWide's result from the nearest triangle.
+            const uint32_t triangle = nearestTri[kk];
+            float bestD = 1e30f;
+            v3 nearest = {0, 0, 0}, pseudo = {0, 0, 0};
+            if (triangle != NO_TRIANGLE) {
+                uint32_t feature;
+                const float4* t = p.meshTriangles + 3 * size_t(triangle);
+                nearest = closestTriangle(x, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
+                bestD = dot(x - nearest, x - nearest);
+                pseudo = xyz(p.meshNormals[7 * triangle + feature]);
+            }
+            const float distance = sqrtf(bestD);
+            const float side = dot(x - nearest, pseudo) < 0 ? -1.0f : 1.0f;
+            const v3 normal = distance > 1e-12f ? side * (x - nearest) / distance : pseudo;
+            // The rest of queryNode.
+            const float depth = -(side * distance) * f.s_j;
+            pd = f4(arm, depth);
+            if (depth > 0) {
+                const v3 grad = qrot(f.qj, normal);
+                const float len = length(grad);
+                if (len < 1e-7f || !isfinite(depth)) {
+                    fail(p, 2);
+                } else {
+                    na = f4(grad / len, sp.w * f.s_i * f.s_i);
+                    if (depth > p.c.maxOverlap * fminf(f.wi, f.wj)) fail(p, 4);
+                    hit = true;
+                }
+            }
         }
         stageHits(p, w, f, task, stageQueue, hit, node, pd, na, running, lane);
     }
@@ -1062,8 +1175,8 @@ __device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32
 
 template <bool NN, typename D>
 __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
-                           uint32_t* queue, uint32_t qn, uint32_t* sref, D* sdist, uint32_t& running, int lane,
-                           uint16_t* nnA, uint16_t* nnB) {
+                           uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
+                           uint32_t& running, int lane, uint16_t* nnA, uint16_t* nnB) {
     // Templates whose margin is infinite (calibration found no safe cull) skip the network.
     if constexpr (NN) {
         if (f.sj < w.nnTemplates && w.nnF[size_t(f.sj) * nn::FSTRIDE + nn::FMARGIN] < 1e30f) {
@@ -1071,7 +1184,7 @@ __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
             wave_sync();
         }
     }
-    queryQueue(p, w, f, task, stageQueue, queue, qn, sref, sdist, running, lane);
+    queryQueue(p, w, f, task, stageQueue, queue, nearestTri, qn, sref, sdist, running, lane);
 }
 
 // One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
@@ -1086,6 +1199,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
     __shared__ uint32_t stackRef[COLLECT_WAVES][WSTACK * WAVE];
     __shared__ D stackDist[COLLECT_WAVES][WSTACK * WAVE];
     __shared__ uint32_t queues[COLLECT_WAVES][QN];
+    __shared__ uint32_t nearestTris[COLLECT_WAVES][QN];
     __shared__ uint64_t candidates[COLLECT_WAVES][MAX_CLUSTERS / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t* sref = &stackRef[wv][lane];
@@ -1093,6 +1207,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
     uint16_t* nnA = reinterpret_cast<uint16_t*>(stackRef[wv]);   // 32 x 64 BF16 = the stack's 4 KB
     uint16_t* nnB = reinterpret_cast<uint16_t*>(stackDist[wv]);
     uint32_t* queue = queues[wv];
+    uint32_t* nearestTri = nearestTris[wv];
     uint64_t* cand = candidates[wv];
     uint32_t queried = 0;
     // XCD-affine queues: tasks are grouped by target template; a wave drains the
@@ -1191,7 +1306,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     uint32_t n = __popcll(m);
                     if (qn + n > QN) {
                         wave_sync();
-                        flushQueue<NN, D>(p, w, f, task, home, queue, qn, sref, sdist, running, lane, nnA, nnB);
+                        flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB);
                         queried += qn;
                         qn = 0;
                         wave_sync();
@@ -1202,7 +1317,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 cluster = candidate ? cluster + 1 : range.z;
             }
             wave_sync();
-            flushQueue<NN, D>(p, w, f, task, home, queue, qn, sref, sdist, running, lane, nnA, nnB);
+            flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB);
             queried += qn;
             wave_sync();
         }
