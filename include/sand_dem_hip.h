@@ -78,8 +78,9 @@ enum sad_side { SAD_SIDE_META = 0, SAD_SIDE_SLIP, SAD_SIDE_NORMAL_AREA, SAD_SIDE
  * demand, never shrinks.  Returns NULL on failure. */
 void* sad_hip_scratch_create(void);
 void sad_hip_scratch_destroy(void* scratch);
-/* stats[0..7]: cumulative steps (collections), directed candidate pairs, exactly
- * queried nodes and contacts since creation; workspace bytes; three reserved. */
+/* stats[0..8]: cumulative steps (collections), directed candidate pairs, exactly
+ * queried nodes and contacts since creation; workspace bytes; learned-cull culled,
+ * audited and missed nodes; nodes passed over by the temporal skip. */
 int sad_hip_scratch_stats(void* scratch, uint64_t* stats);
 
 /* Per-kernel HIP-event timing on the launch stream (no reference counterpart;
@@ -130,12 +131,21 @@ int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buf
 int sad_hip_scan_kernel(const char* kernel, uint32_t n, const uint32_t* input,
                         uint32_t* output, uint32_t* aux, void* stream);
 
-/* Launch-shape tunables of the collection stage (no reference counterpart; none
- * changes any result): "collect_blocks_per_cu" (1..32 workgroups per CU in the
- * persistent launch), "collect_pop" (1..16 tasks taken per queue atomic),
- * "collect_queue" (128 or 256 queued nodes per wavefront).  value < 0 only reads;
- * *previous (if non-null) receives the value before the call. */
+/* Tunables of the collection stage (no reference counterpart; none changes any
+ * result): "collect_blocks_per_cu" (1..32 workgroups per CU in the persistent
+ * launch), "collect_pop" (1..16 tasks taken per queue atomic), "collect_queue"
+ * (128 or 256 queued nodes per wavefront), "temporal_skip" (0/1, default 0: pass
+ * over nodes whose signed distance at an anchor pose exceeds a rigorous bound on
+ * their motion since), "skip_limit_micro" (2000..250000: re-anchor a directed pair
+ * once its motion bound reaches this many 1e-6 template units).  value < 0 only
+ * reads; *previous (if non-null) receives the value before the call. */
 int sad_hip_tune(void* scratch, const char* name, int64_t value, int64_t* previous);
+
+/* Forget every temporal-skip anchor of this workspace (no reference counterpart).
+ * A caller that enables "temporal_skip" calls this whenever it rewrites template
+ * geometry, body scales or shape ids in buffers it keeps passing (a new binding of
+ * those buffers is detected by address). */
+int sad_hip_invalidate(void* scratch);
 
 /* ---- Learned signed-distance cull (no reference counterpart).  One MLP per
  * template: positional encoding (xyz + sin/cos of 2^k pi x, k < 6) -> 64 -> 64 ->

@@ -99,6 +99,14 @@ struct W {
     uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue, ascending target template within a queue
     uint32_t* keyCount;   // tasks per (target rank, source) key
     uint32_t* keyStart;   // exclusive scan of keyCount; consumed as fill cursors
+    // Temporal node skip (opt-in): per directed-pair task an anchor (both
+    // rotations, the branch, and the pair/epoch key) and one byte per source node:
+    // its signed distance to the target at the anchor poses, quantized down.
+    float4* anchor;       // 4 rows per task
+    uint8_t* nodeDist;    // nodeStride bytes per task, indexed by node id
+    uint32_t nodeStride;  // bytes per task (>= nodes of any template, multiple of 64)
+    uint32_t skip;        // 1 = on
+    float skipLimit;      // re-anchor once the motion bound reaches this (template units)
     uint32_t* wallCount;  // plane contacts per (wall, grain) task, task = wall * N + grain
     uint32_t* wallOffset; // exclusive scan of wallCount (ids follow the mesh contacts)
     // Learned cull (nn.h): per-template BF16 weights and FP32 blocks; stats on separate lines.
@@ -833,6 +841,7 @@ __global__ void k_reset_mesh(P p, W w) {  // resetMeshContacts (solver.slang:397
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id == 0) {
         w.counters[2 * LINE] = 0;
+        w.counters[11 * LINE] = 0;
         for (int q = 0; q < 8; q++) w.stageCount[q * LINE] = 0;
     }
     if (ld_volatile(p.control) != 0) return;
@@ -928,6 +937,24 @@ struct TaskFrame {
     v3 branch;
     float4 plane;
 };
+// Every lane computes the same task frame; reading it back from the first lane
+// tells the compiler so, which moves it to scalar registers.
+__device__ __forceinline__ uint32_t uniformU(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uniformF(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+__device__ __forceinline__ void uniformFrame(TaskFrame& f) {
+    f.i = uniformU(f.i); f.j = uniformU(f.j); f.si = uniformU(f.si); f.sj = uniformU(f.sj);
+    f.sampleBase = uniformU(f.sampleBase);
+    f.qi = {uniformF(f.qi.x), uniformF(f.qi.y), uniformF(f.qi.z), uniformF(f.qi.w)};
+    f.qj = {uniformF(f.qj.x), uniformF(f.qj.y), uniformF(f.qj.z), uniformF(f.qj.w)};
+    f.s_i = uniformF(f.s_i); f.s_j = uniformF(f.s_j); f.wi = uniformF(f.wi); f.wj = uniformF(f.wj);
+    f.branch = {uniformF(f.branch.x), uniformF(f.branch.y), uniformF(f.branch.z)};
+    f.plane = make_float4(uniformF(f.plane.x), uniformF(f.plane.y), uniformF(f.plane.z), uniformF(f.plane.w));
+}
+template <typename T>
+__device__ __forceinline__ T* uniformPtr(T* ptr) {
+    const uint64_t v = reinterpret_cast<uint64_t>(ptr);
+    return reinterpret_cast<T*>((uint64_t(uniformU(uint32_t(v >> 32))) << 32) | uniformU(uint32_t(v)));
+}
 
 // Exact part of nodeContact (solver.slang:319-343) for one queued node.
 template <typename D>
@@ -1023,10 +1050,19 @@ __device__ __forceinline__ v3 queryPoint(const TaskFrame& f, float4 sp, v3& arm)
     arm = qrot(f.qi, xyz(sp) * f.s_i);
     return qinv(f.qj, arm - f.branch) / f.s_j;
 }
+// Temporal skip constants: distances are stored in units of 2^-10 template units
+// (rounded down, clamped to 255); a node is skipped only when its stored distance
+// exceeds the motion bound plus SKIP_MARGIN, which covers the FP32 error of both
+// distance evaluations and of the node transforms (~1e-5 at these magnitudes).
+constexpr float SKIP_QUANTUM = 1.0f / 1024.0f;
+constexpr float SKIP_MARGIN = 1e-3f;
+__device__ __forceinline__ uint8_t quantizeDistance(float d) {
+    return d > 0.0f ? uint8_t(fminf(d * 1024.0f, 255.0f)) : uint8_t(0);
+}
 template <typename D>
 __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
                            const uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
-                           uint32_t& running, int lane) {
+                           uint32_t& running, int lane, uint8_t* refreshDist) {
     if (qn == 0) return;
     const uint32_t root = p.meshRoots[f.sj].x;
     const uint32_t rootRef = nodeRef(p, root);
@@ -1118,6 +1154,7 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
             const float distance = sqrtf(bestD);
             const float side = dot(x - nearest, pseudo) < 0 ? -1.0f : 1.0f;
             const v3 normal = distance > 1e-12f ? side * (x - nearest) / distance : pseudo;
+            if (refreshDist) refreshDist[node] = quantizeDistance(side * distance);
             // The rest of queryNode.
             const float depth = -(side * distance) * f.s_j;
             pd = f4(arm, depth);
@@ -1176,7 +1213,7 @@ __device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32
 template <bool NN, typename D>
 __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
                            uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
-                           uint32_t& running, int lane, uint16_t* nnA, uint16_t* nnB) {
+                           uint32_t& running, int lane, uint16_t* nnA, uint16_t* nnB, uint8_t* refreshDist) {
     // Templates whose margin is infinite (calibration found no safe cull) skip the network.
     if constexpr (NN) {
         if (f.sj < w.nnTemplates && w.nnF[size_t(f.sj) * nn::FSTRIDE + nn::FMARGIN] < 1e30f) {
@@ -1184,7 +1221,7 @@ __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
             wave_sync();
         }
     }
-    queryQueue(p, w, f, task, stageQueue, queue, nearestTri, qn, sref, sdist, running, lane);
+    queryQueue(p, w, f, task, stageQueue, queue, nearestTri, qn, sref, sdist, running, lane, refreshDist);
 }
 
 // One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
@@ -1275,6 +1312,46 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 }
                 f.plane = i0 == 0xffffffffu ? make_float4(0, 0, 0, 0) : p.meshPlanes[i0];
             }
+            uniformFrame(f);
+            // Temporal skip.  The node's target-frame point is
+            //   x = R_j^T (R_i a - b) / s_j,   a = s_i p,
+            // and for unit quaternions |R(q) v - R(q') v| <= 2 |q - q'| |v|, so since
+            // the anchor it has moved at most
+            //   (2 |dq_j| (|a| + |b|) + 2 |dq_i| |a| + |db|) / s_j = c0 + c1 |p|.
+            // The exact signed distance is 1-Lipschitz in x, so a node whose anchor
+            // distance exceeds that bound (plus SKIP_MARGIN) is still outside the
+            // target: not a contact, and skipping it leaves every hit and its rank
+            // unchanged.  A task without a valid anchor, or whose bound has grown
+            // past skipLimit, queries every node and re-anchors.
+            bool skipMode = false;
+            float c0 = 0.0f, c1 = 0.0f;
+            uint8_t* dist = nullptr;
+            uint8_t* refreshDist = nullptr;
+            uint32_t skipped = 0;
+            if (w.skip) {
+                dist = w.nodeDist + size_t(task) * w.nodeStride;
+                const float4* an = w.anchor + 4 * size_t(task);
+                const float4 a0 = an[0], a1 = an[1], a2 = an[2], a3 = an[3];
+                const bool valid = __float_as_uint(a3.w) == 1u && __float_as_uint(a3.x) == f.i &&
+                                   __float_as_uint(a3.y) == f.j && __float_as_uint(a3.z) == ld_volatile(w.counters + 10 * LINE);
+                const float dqi = sqrtf(dot4(f.qi - v4of(a0), f.qi - v4of(a0)));
+                const float dqj = sqrtf(dot4(f.qj - v4of(a1), f.qj - v4of(a1)));
+                const float db = length(f.branch - xyz(a2));
+                c0 = 1.001f * (2.0f * dqj * length(f.branch) + db) / f.s_j;
+                c1 = 1.001f * (2.0f * dqj + 2.0f * dqi) * f.s_i / f.s_j;
+                skipMode = valid && c0 + c1 * (p.body[f.i].z / f.s_i) + SKIP_MARGIN < w.skipLimit;
+                c0 = uniformF(c0);
+                c1 = uniformF(c1);
+                skipMode = uniformU(skipMode ? 1u : 0u) != 0u;
+                dist = uniformPtr(dist);
+                if (!skipMode) {
+                    refreshDist = dist;
+                    const uint32_t words = (p.sampleInfo[f.si].y + 3) / 4;
+                    uint32_t* d32 = reinterpret_cast<uint32_t*>(dist);
+                    for (uint32_t k = lane; k < words; k += WAVE) d32[k] = 0u;
+                    __threadfence_block();
+                }
+            }
             // Cluster culls for every cluster of the source template, lane-parallel.
             uint4 info = p.meshClusterInfo[f.si];
             for (uint32_t base = 0; base < info.y; base += WAVE) {
@@ -1294,19 +1371,26 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 if (candidate && range.y > 0) {
                     bool pass_ = false;
                     uint32_t node = 0;
+                    bool far = false;
                     if (uint32_t(lane) < range.y) {
                         node = p.meshSampleIndices[range.x + lane];
                         float4 sp = p.sample[f.sampleBase + node];
-                        v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
-                        v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
-                        float planePad = 64.0f * DEM_EPS * (1.0f + maxabs(local));
-                        pass_ = !(dot(xyz(f.plane), local) + f.plane.w > planePad) && meshMayContainFast(p, w, f.sj, local);
+                        if (skipMode)
+                            far = float(dist[node]) * SKIP_QUANTUM > c0 + c1 * length(xyz(sp)) + SKIP_MARGIN;
+                        if (!far) {
+                            v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
+                            v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
+                            float planePad = 64.0f * DEM_EPS * (1.0f + maxabs(local));
+                            pass_ = !(dot(xyz(f.plane), local) + f.plane.w > planePad) &&
+                                    meshMayContainFast(p, w, f.sj, local);
+                        }
                     }
+                    if (skipMode) skipped += __popcll(__ballot(far));
                     uint64_t m = __ballot(pass_);
                     uint32_t n = __popcll(m);
                     if (qn + n > QN) {
                         wave_sync();
-                        flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB);
+                        flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist);
                         queried += qn;
                         qn = 0;
                         wave_sync();
@@ -1317,9 +1401,18 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 cluster = candidate ? cluster + 1 : range.z;
             }
             wave_sync();
-            flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB);
+            flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist);
             queried += qn;
             wave_sync();
+            if (refreshDist && lane == 0) {
+                float4* an = w.anchor + 4 * size_t(task);
+                an[0] = f4(f.qi);
+                an[1] = f4(f.qj);
+                an[2] = f4(f.branch, 0.0f);
+                an[3] = make_float4(__uint_as_float(f.i), __uint_as_float(f.j),
+                                    __uint_as_float(ld_volatile(w.counters + 10 * LINE)), __uint_as_float(1u));
+            }
+            if (skipped && lane == 0) atomicAdd(w.counters + 11 * LINE, skipped);
         }
         if (lane == 0) w.taskCount[task] = running;
     }
@@ -1348,6 +1441,7 @@ __global__ void k_contact_total(P p, W w) {
         atomicAdd(w.totals + 1, 2ull * pairs);
         atomicAdd(w.totals + 2, (unsigned long long)w.counters[2 * LINE]);
         atomicAdd(w.totals + 3, (unsigned long long)total);
+        atomicAdd(w.totals + 4, (unsigned long long)w.counters[11 * LINE]);
     }
 }
 __global__ void k_scatter(P p, W w) {
@@ -1799,6 +1893,13 @@ __global__ void k_refresh_wall_power(P p) {
     p.wallRows[3 * index + 1] = value;
 }
 
+// Largest sample-node count of any template (sizes the temporal-skip rows).
+__global__ void k_max_nodes(P p, uint32_t* out) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < p.c.shapes) atomicMax(out, p.sampleInfo[t].y);
+}
+__global__ void k_set_word(uint32_t* word, uint32_t value) { *word = value; }
+
 // ---------------------------------------------------------------- scan.slang (the reference Engine's Scan)
 // blockScan: exclusive uint scan of each 256-element block, block totals out
 // (scan.slang:1-21, unsigned wrap-around as on the GPU); addBlocks: add the scanned
@@ -2010,6 +2111,18 @@ struct ScratchImpl : Scratch {
     int blocksPerCU = SAD_COLLECT_BLOCKS_PER_CU;
     uint32_t pop = POP;
     int queue = QCAP;
+    // Temporal node skip (sad_hip_tune "temporal_skip", "skip_limit_micro"): off by
+    // default; the caller that enables it calls sad_hip_invalidate whenever it
+    // rewrites template geometry, body scales or shape ids in place.
+    int skip = 0;
+    int skipLimitMicro = 6000;
+    uint32_t epoch = 1, epochWritten = 0;  // anchors carry the epoch they were taken in
+    std::vector<uint64_t> lastTable;        // binding table of the previous call
+    const void* maxNodesFor = nullptr;
+    uint32_t maxNodesShapes = 0, maxNodes = 0;
+    size_t nodeDistCap = 0;
+    uint32_t anchorCap = 0;
+    uint32_t* probeWord = nullptr;
     // Captured timesteps keyed by everything a step's launches bake in.
     struct Graph { std::vector<uint64_t> key; hipGraphExec_t exec; };
     std::vector<Graph> graphs;
@@ -2037,8 +2150,8 @@ struct ScratchImpl : Scratch {
         }
         if ((rc = grow(w.stageCount, caps.stageCount, 8 * LINE, bytes))) return rc;
         if (!w.totals) {
-            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.totals), 4 * sizeof(unsigned long long)));
-            HIP_OK(hipMemset(w.totals, 0, 4 * sizeof(unsigned long long)));
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.totals), 8 * sizeof(unsigned long long)));
+            HIP_OK(hipMemset(w.totals, 0, 8 * sizeof(unsigned long long)));
             totals = w.totals;
         }
         if ((rc = grow(w.taskCount, caps.taskCount, tasks, bytes))) return rc;
@@ -2077,6 +2190,72 @@ struct ScratchImpl : Scratch {
         }
         return SAD_OK;
     }
+    // Binding-table identity and the temporal-skip workspace.  Called outside any
+    // graph capture (it may synchronize once per new template set).
+    int ensure_skip(const P& p, void* const* table, const uint64_t* sizes) {
+        // Identity of the geometry the stored distances refer to: the template,
+        // body and shape buffers (the history sides alternate every step and are
+        // not part of it) and the grain state's address.
+        static const int slots[] = {SAD_SHAPE_ID, SAD_BODY, SAD_SAMPLE_INFO, SAD_SAMPLE, SAD_MESH_ROOTS,
+                                    SAD_MESH_HULL_INFO, SAD_MESH_PLANES, SAD_MESH_MIN, SAD_MESH_MAX, SAD_MESH_LINKS,
+                                    SAD_MESH_TRIANGLES, SAD_MESH_NORMALS, SAD_MESH_CLUSTER_INFO,
+                                    SAD_MESH_CLUSTER_RANGES, SAD_MESH_CLUSTER_MIN, SAD_MESH_CLUSTER_MAX,
+                                    SAD_MESH_SAMPLE_INDICES, SAD_POSITION};
+        std::vector<uint64_t> key;
+        key.reserve(2 * (sizeof(slots) / sizeof(slots[0])) + 1);
+        for (int k : slots) {
+            key.push_back(reinterpret_cast<uint64_t>(table[k]));
+            key.push_back(sizes[k]);
+        }
+        key.push_back(p.c.N);
+        if (key != lastTable) { lastTable.swap(key); epoch++; }
+        w.skip = 0;
+        if (!skip) return SAD_OK;
+        if (maxNodesFor != p.sampleInfo || maxNodesShapes != p.c.shapes) {
+            if (!probeWord) HIP_OK(hipMalloc(reinterpret_cast<void**>(&probeWord), sizeof(uint32_t)));
+            HIP_OK(hipMemset(probeWord, 0, sizeof(uint32_t)));
+            hipLaunchKernelGGL(k_max_nodes, dim3((std::max(p.c.shapes, 1u) + 255) / 256), dim3(256), 0, nullptr, p, probeWord);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpy(&maxNodes, probeWord, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            maxNodesFor = p.sampleInfo;
+            maxNodesShapes = p.c.shapes;
+        }
+        const uint32_t stride = (std::max(maxNodes, 1u) + 63) / 64 * 64;
+        const size_t tasks = std::max<size_t>(w.taskCap, 1);
+        const size_t bytesNeeded = tasks * stride;
+        if (bytesNeeded > (size_t(48) << 30)) return SAD_OK;  // more than 48 GiB: leave the skip off
+        if (anchorCap < 4 * tasks) {
+            if (w.anchor) { (void)hipFree(w.anchor); bytes -= size_t(anchorCap) * sizeof(float4); }
+            w.anchor = nullptr;
+            anchorCap = 0;
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.anchor), 4 * tasks * sizeof(float4)));
+            HIP_OK(hipMemset(w.anchor, 0, 4 * tasks * sizeof(float4)));
+            anchorCap = uint32_t(4 * tasks);
+            bytes += size_t(anchorCap) * sizeof(float4);
+        }
+        if (nodeDistCap < bytesNeeded || w.nodeStride != stride) {
+            if (w.nodeDist) { (void)hipFree(w.nodeDist); bytes -= nodeDistCap; }
+            w.nodeDist = nullptr;
+            nodeDistCap = 0;
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.nodeDist), bytesNeeded));
+            nodeDistCap = bytesNeeded;
+            bytes += bytesNeeded;
+            epoch++;  // stored distances are gone: no anchor may be trusted
+        }
+        w.nodeStride = stride;
+        w.skipLimit = float(skipLimitMicro) * 1e-6f;
+        w.skip = 1;
+        return SAD_OK;
+    }
+    // The device copy of the epoch, written on the launch stream before any
+    // collection that follows a change (never inside a capture).
+    int sync_epoch(hipStream_t s) {
+        if (epochWritten == epoch || !w.counters) return SAD_OK;
+        hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, s, w.counters + 10 * LINE, epoch);
+        HIP_OK(hipGetLastError());
+        epochWritten = epoch;
+        return SAD_OK;
+    }
     ~ScratchImpl() {
         drop_graphs();
         if (captureStream) (void)hipStreamDestroy(captureStream);
@@ -2087,6 +2266,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
+        (void)hipFree(w.anchor); (void)hipFree(w.nodeDist); (void)hipFree(probeWord);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
 };
@@ -2399,12 +2579,13 @@ void sad_hip_scratch_destroy(void* scratch) { delete static_cast<ScratchImpl*>(s
 int sad_hip_scratch_stats(void* scratch, uint64_t* stats) {
     if (!scratch || !stats) return SAD_E_ARGUMENT;
     auto* sc = static_cast<ScratchImpl*>(scratch);
-    unsigned long long totals[4] = {0, 0, 0, 0};
+    unsigned long long totals[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (sc->w.totals) {
         hipError_t e = hipMemcpy(totals, sc->w.totals, sizeof(totals), hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_fail(e, "stats");
     }
     for (int k = 0; k < 4; k++) stats[k] = totals[k];
+    stats[8] = totals[4];
     stats[4] = sc->bytes;
     unsigned long long nnv[3] = {0, 0, 0};
     if (sc->w.nnStats) {
@@ -2454,6 +2635,8 @@ int sad_hip_dispatch(const char* kernel, uint32_t count, void* const* buffers, c
         if (rc) return rc;
         auto* sc = static_cast<ScratchImpl*>(scratch);
         if ((rc = sc->ensure(p.c))) return rc;
+        if ((rc = sc->ensure_skip(p, buffers, bytes))) return rc;
+        if ((rc = sc->sync_epoch(static_cast<hipStream_t>(stream)))) return rc;
         return run_kernel(kernel, p, *sc, static_cast<hipStream_t>(stream));
     } catch (...) {
         std::snprintf(g_error, sizeof(g_error), "exception in dispatch");
@@ -2569,6 +2752,8 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const
             if ((rc = sc->ensure(p.c))) return rc;
             if ((rc = sc->ensure_nodes(p.meshNodes))) return rc;
             if (p.templates <= MAX_TEMPLATES && (rc = sc->ensure_keys(p.templates))) return rc;
+            if ((rc = sc->ensure_skip(p, table, sizes))) return rc;
+            if ((rc = sc->sync_epoch(s))) return rc;
             if (graph && !sc->profiling) {
                 sc->w.pop = sc->pop;
                 rc = graph_step(p, *sc, graph_key(table, sizes, vals, *sc), s);
@@ -2607,11 +2792,29 @@ int sad_hip_tune(void* scratch, const char* name, int64_t value, int64_t* previo
             if (value != 128 && value != 256) return SAD_E_ARGUMENT;
             sc->queue = int(value);
         }
+    } else if (key == "temporal_skip") {
+        old = sc->skip;
+        if (value >= 0) {
+            if (value > 1) return SAD_E_ARGUMENT;
+            sc->skip = int(value);
+        }
+    } else if (key == "skip_limit_micro") {
+        old = sc->skipLimitMicro;
+        if (value >= 0) {
+            if (value < 2000 || value > 250000) return SAD_E_ARGUMENT;  // above the margin, inside the byte range
+            sc->skipLimitMicro = int(value);
+        }
     } else {
         std::snprintf(g_error, sizeof(g_error), "unknown tunable %s", name);
         return SAD_E_ARGUMENT;
     }
     if (previous) *previous = old;
+    return SAD_OK;
+}
+
+int sad_hip_invalidate(void* scratch) {
+    if (!scratch) return SAD_E_ARGUMENT;
+    static_cast<ScratchImpl*>(scratch)->epoch++;
     return SAD_OK;
 }
 

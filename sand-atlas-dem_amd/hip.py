@@ -32,6 +32,8 @@ class Buffer:
 
     def __init__(self, backend, array=None, *, count=None, stride=4):
         torch = backend.torch
+        self.backend = backend
+        backend.invalidate()
         if array is not None:
             host = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
             self.tensor = torch.from_numpy(host.copy()).to(backend.torch_device)
@@ -46,6 +48,7 @@ class Buffer:
             raise ValueError("HIP buffer upload size mismatch")
         import torch
 
+        self.backend.invalidate()
         self.tensor.copy_(torch.from_numpy(data.copy()))
 
     def to_numpy(self):
@@ -112,7 +115,7 @@ class Kernel:
 class HipBackend:
     name = "hip"
 
-    def __init__(self, device=0, graph=False):
+    def __init__(self, device=0, graph=False, temporal_skip=True):
         self.torch = _torch()
         self.torch_device = self.torch.device("cuda", device)
         self.torch.cuda.set_device(self.torch_device)
@@ -124,6 +127,14 @@ class HipBackend:
         self.graph = graph
         self.buffer_type = Buffer
         self.kernels = {}
+        # Temporal node skip: identical results, fewer exact queries between pose
+        # changes.  Every upload through this adapter drops the anchors, so stored
+        # distances never outlive the geometry they were measured on.
+        self.tune("temporal_skip", int(bool(temporal_skip)))
+
+    def invalidate(self):
+        """Forget the collection stage's temporal-skip anchors (after any upload)."""
+        self._check(self.lib.sad_hip_invalidate(self.scratch), "invalidate")
 
     def __del__(self):
         lib, scratch = getattr(self, "lib", None), getattr(self, "scratch", None)
@@ -265,14 +276,15 @@ class HipBackend:
         if data.size > buffer.size:
             raise ValueError("HIP buffer prefix out of bounds")
         if data.size:
+            self.invalidate()
             buffer.tensor[: data.size].copy_(self.torch.from_numpy(data.copy()))
 
     def stats(self):
         """Cumulative device work counters of this backend's workspace."""
-        out = (ct.c_uint64 * 8)()
+        out = (ct.c_uint64 * 9)()
         self._check(self.lib.sad_hip_scratch_stats(self.scratch, out), "stats")
         keys = ("collections", "directed_pairs", "queried_nodes", "contacts", "scratch_bytes", "nn_culled",
-                "nn_audited", "nn_misses")
+                "nn_audited", "nn_misses", "skipped_nodes")
         return dict(zip(keys, (int(v) for v in out)))
 
     def profile(self, enable=True):
