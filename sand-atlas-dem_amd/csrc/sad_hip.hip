@@ -103,7 +103,14 @@ struct W {
     // rotations, the branch, and the pair/epoch key) and one byte per source node:
     // its signed distance to the target at the anchor poses, quantized down.
     float4* anchor;       // 4 rows per task
-    uint8_t* nodeDist;    // nodeStride bytes per task, indexed by node id
+    int8_t* nodeDist;     // nodeStride bytes per task, indexed by node id
+    uint16_t* nodeTri;    // nodeStride entries per task: 1 + the node's last nearest triangle
+                          // (relative to its template's first), 0 = none
+    int8_t* leafDist;     // clusterStride bytes per task: per sample-tree leaf, a lower bound of
+                          // its nodes' signed distances at the anchor poses
+    uint32_t clusterStride;
+    const uint32_t* triBase;  // per template: first triangle, triangle count (2 words)
+    uint32_t warm;        // 1 = warm-start the exact queries from nodeTri
     uint32_t nodeStride;  // bytes per task (>= nodes of any template, multiple of 64)
     uint32_t skip;        // 1 = on
     float skipLimit;      // re-anchor once the motion bound reaches this (template units)
@@ -317,6 +324,32 @@ __device__ __forceinline__ bool planesPass(const float4* __restrict__ planes, ui
     }
     return true;
 }
+// meshMayContainFast's test with its evidence: 0 when q passes, otherwise a
+// positive lower bound on q's distance to the mesh (how far q lies outside the
+// root box, or outside the first violated hull plane of its group; the planes
+// have unit normals and contain the mesh).
+__device__ __forceinline__ float planesExcess(const float4* __restrict__ planes, uint32_t first, uint32_t count, v3 x) {
+    for (uint32_t k = 0; k < count; k += PLANE_BATCH) {
+        float4 pl[PLANE_BATCH];
+#pragma unroll
+        for (uint32_t j = 0; j < PLANE_BATCH; j++) pl[j] = planes[first + min(k + j, count - 1)];
+        float worst = 0.0f;
+#pragma unroll
+        for (uint32_t j = 0; j < PLANE_BATCH; j++) worst = fmaxf(worst, dot(xyz(pl[j]), x) + pl[j].w);
+        if (worst > 0) return worst;
+    }
+    return 0.0f;
+}
+__device__ float meshOutsideFast(const P& p, uint32_t shape, v3 q) {
+    uint32_t root = p.meshRoots[shape].x;
+    v3 low = xyz(p.meshMin[root]), high = xyz(p.meshMax[root]);
+    if (q.x < low.x || q.y < low.y || q.z < low.z || q.x > high.x || q.y > high.y || q.z > high.z)
+        return fmaxf(fmaxf(fmaxf(low.x - q.x, q.x - high.x), fmaxf(low.y - q.y, q.y - high.y)),
+                     fmaxf(low.z - q.z, q.z - high.z));
+    v3 direction = (q - 0.5f * (low + high)) / (high - low);
+    uint4 info = p.meshHullInfo[6 * shape + dominantGroup(direction)];
+    return info.y == 0 ? 0.0f : planesExcess(p.meshPlanes, info.x, info.y, q);
+}
 __device__ bool meshMayContainFast(const P& p, const W& w, uint32_t shape, v3 q) {
     uint32_t root = p.meshRoots[shape].x;
     v3 low = xyz(p.meshMin[root]), high = xyz(p.meshMax[root]);
@@ -324,6 +357,44 @@ __device__ bool meshMayContainFast(const P& p, const W& w, uint32_t shape, v3 q)
     v3 direction = (q - 0.5f * (low + high)) / (high - low);
     uint4 info = p.meshHullInfo[6 * shape + dominantGroup(direction)];
     return info.y == 0 || planesPass(p.meshPlanes, info.x, info.y, q, mk3(0, 0, 0), false);
+}
+// meshClusterMayContactFast's test with its evidence: 0 when the cluster passes,
+// otherwise a positive lower bound on the distance from any point of the cluster
+// box to the mesh (the gap between the padded transformed box and the root box,
+// or how far the box lies outside the first violated hull plane of its group).
+__device__ float meshClusterOutsideFast(const P& p, v4 qs, v4 qd, float ss, float sd, v3 branch, uint32_t dstShape,
+                                        uint32_t cluster) {
+    v3 sl = xyz(p.meshClusterMin[cluster]), sh = xyz(p.meshClusterMax[cluster]);
+    v3 low = {1e30f, 1e30f, 1e30f}, high = {-1e30f, -1e30f, -1e30f};
+    for (uint32_t corner = 0; corner < 8; corner++) {
+        v3 q = {(corner & 1) ? sh.x : sl.x, (corner & 2) ? sh.y : sl.y, (corner & 4) ? sh.z : sl.z};
+        q = qinv(qd, qrot(qs, q * ss) - branch) / sd;
+        low = fmin3(low, q);
+        high = fmax3(high, q);
+    }
+    float pad = 64.0f * DEM_EPS * (1.0f + maxabs(low) + maxabs(high) + length(branch) / sd);
+    low = low - mk3(pad, pad, pad);
+    high = high + mk3(pad, pad, pad);
+    uint32_t root = p.meshRoots[dstShape].x;
+    v3 tl = xyz(p.meshMin[root]), th = xyz(p.meshMax[root]);
+    if (high.x < tl.x || high.y < tl.y || high.z < tl.z || low.x > th.x || low.y > th.y || low.z > th.z)
+        return fmaxf(fmaxf(fmaxf(tl.x - high.x, low.x - th.x), fmaxf(tl.y - high.y, low.y - th.y)),
+                     fmaxf(tl.z - high.z, low.z - th.z));
+    v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
+    v3 direction = (center - 0.5f * (tl + th)) / (th - tl);
+    uint4 info = p.meshHullInfo[6 * dstShape + dominantGroup(direction)];
+    if (info.y == 0) return 0.0f;
+    for (uint32_t k = 0; k < info.y; k += PLANE_BATCH) {
+        float4 pl[PLANE_BATCH];
+#pragma unroll
+        for (uint32_t j = 0; j < PLANE_BATCH; j++) pl[j] = p.meshPlanes[info.x + min(k + j, info.y - 1)];
+        float worst = 0.0f;
+#pragma unroll
+        for (uint32_t j = 0; j < PLANE_BATCH; j++)
+            worst = fmaxf(worst, dot(xyz(pl[j]), center) + pl[j].w - dot(fabs3(xyz(pl[j])), half));
+        if (worst > 0) return worst;
+    }
+    return 0.0f;
 }
 __device__ bool meshClusterMayContactFast(const P& p, const W& w, v4 qs, v4 qd, float ss, float sd, v3 branch,
                                           uint32_t dstShape, uint32_t cluster) {
@@ -1050,36 +1121,58 @@ __device__ __forceinline__ v3 queryPoint(const TaskFrame& f, float4 sp, v3& arm)
     arm = qrot(f.qi, xyz(sp) * f.s_i);
     return qinv(f.qj, arm - f.branch) / f.s_j;
 }
-// Temporal skip constants: distances are stored in units of 2^-10 template units
-// (rounded down, clamped to 255); a node is skipped only when its stored distance
-// exceeds the motion bound plus SKIP_MARGIN, which covers the FP32 error of both
-// distance evaluations and of the node transforms (~1e-5 at these magnitudes).
-constexpr float SKIP_QUANTUM = 1.0f / 1024.0f;
+// Temporal skip constants: signed distances are stored in units of 2^-9 template
+// units (clamped to +-127); a node is passed over (or queued untested) only when
+// its stored distance clears the motion bound by SKIP_MARGIN, which covers the
+// FP32 error of both distance evaluations and of the node transforms (~1e-5 at
+// these magnitudes).
+constexpr float SKIP_QUANTUM = 1.0f / 512.0f;
 constexpr float SKIP_MARGIN = 1e-3f;
-__device__ __forceinline__ uint8_t quantizeDistance(float d) {
-    return d > 0.0f ? uint8_t(fminf(d * 1024.0f, 255.0f)) : uint8_t(0);
+// Rounded toward zero: a lower bound of a positive distance, an upper bound of a
+// negative one; NaN stores 0 (neither far nor inside).
+__device__ __forceinline__ int8_t quantizeDistance(float d) {
+    if (!(d == d)) return 0;
+    return int8_t(fminf(fmaxf(d * 512.0f, -127.0f), 127.0f));
 }
 template <typename D>
 __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
                            const uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
-                           uint32_t& running, int lane, uint8_t* refreshDist) {
+                           uint32_t& running, int lane, int8_t* refreshDist, uint16_t* warmTri) {
     if (qn == 0) return;
     const uint32_t root = p.meshRoots[f.sj].x;
     const uint32_t rootRef = nodeRef(p, root);
     const v3 rootLo = xyz(p.meshMin[root]), rootHi = xyz(p.meshMax[root]);
     const float4* __restrict__ wide = w.wide;
     uint32_t k = lane, top = 0, bestTri = NO_TRIANGLE;
-    float best = 1e30f;
+    float best = 1e30f, prune = 1e30f;
     v3 q = {0, 0, 0};
     bool active = k < qn;
+    // Warm start: the squared distance to the node's nearest triangle of an earlier
+    // step is an upper bound on this query's minimum, so subtrees whose box lies
+    // strictly farther can be pruned from the start.  Triangles at the minimum are
+    // never pruned (their boxes are no farther than the minimum), and the record
+    // test still starts from 1e30, so the winner -- the first triangle in visit
+    // order at the minimum distance -- is the one the plain traversal finds.
+    const uint32_t triFirst = warmTri ? w.triBase[2 * f.sj] : 0u, triCount = warmTri ? w.triBase[2 * f.sj + 1] : 0u;
     auto start = [&]() {
         v3 arm;
-        q = queryPoint(f, p.sample[f.sampleBase + queue[k]], arm);
+        const uint32_t node = queue[k];
+        q = queryPoint(f, p.sample[f.sampleBase + node], arm);
         sref[0] = rootRef;
         dstore(sdist, boxDistance(rootLo, rootHi, q));
         top = 1;
         best = 1e30f;
+        prune = 1e30f;
         bestTri = NO_TRIANGLE;
+        if (warmTri) {
+            const uint32_t last = warmTri[node];
+            if (last != 0u && last <= triCount) {
+                uint32_t feature;
+                const float4* t = p.meshTriangles + 3 * size_t(triFirst + last - 1);
+                const v3 x = closestTriangle(q, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
+                prune = dot(q - x, q - x);
+            }
+        }
     };
     if (active) start();
     uint32_t next = WAVE;  // first unassigned queue entry (wave-uniform)
@@ -1087,7 +1180,8 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
         if (active) {
             --top;
             const uint32_t ref = sref[top * WAVE];
-            if (!(dload(sdist + top * WAVE) > best)) {
+            const float bound = fminf(best, prune);
+            if (!(dload(sdist + top * WAVE) > bound)) {
                 if ((ref & LEAF) == 0) {
                     const float4* node = wide + 4 * size_t(ref);
                     float4 w0 = node[0], w1 = node[1], w2 = node[2], w3 = node[3];
@@ -1099,8 +1193,8 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
                         top = 0;
                         bestTri = NO_TRIANGLE;
                     } else {
-                        if (da <= best) { sref[top * WAVE] = a; dstore(sdist + top * WAVE, da); top++; }
-                        if (db <= best) { sref[top * WAVE] = b; dstore(sdist + top * WAVE, db); top++; }
+                        if (da <= bound) { sref[top * WAVE] = a; dstore(sdist + top * WAVE, da); top++; }
+                        if (db <= bound) { sref[top * WAVE] = b; dstore(sdist + top * WAVE, db); top++; }
                     }
                 } else {
                     uint32_t first = ref & 0x0fffffffu, count = ((ref >> 28) & 3u) + 1;
@@ -1155,6 +1249,7 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
             const float side = dot(x - nearest, pseudo) < 0 ? -1.0f : 1.0f;
             const v3 normal = distance > 1e-12f ? side * (x - nearest) / distance : pseudo;
             if (refreshDist) refreshDist[node] = quantizeDistance(side * distance);
+            if (warmTri && triangle != NO_TRIANGLE) warmTri[node] = uint16_t(triangle - triFirst + 1);
             // The rest of queryNode.
             const float depth = -(side * distance) * f.s_j;
             pd = f4(arm, depth);
@@ -1213,7 +1308,8 @@ __device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32
 template <bool NN, typename D>
 __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
                            uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
-                           uint32_t& running, int lane, uint16_t* nnA, uint16_t* nnB, uint8_t* refreshDist) {
+                           uint32_t& running, int lane, uint16_t* nnA, uint16_t* nnB, int8_t* refreshDist,
+                           uint16_t* warmTri) {
     // Templates whose margin is infinite (calibration found no safe cull) skip the network.
     if constexpr (NN) {
         if (f.sj < w.nnTemplates && w.nnF[size_t(f.sj) * nn::FSTRIDE + nn::FMARGIN] < 1e30f) {
@@ -1221,7 +1317,7 @@ __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
             wave_sync();
         }
     }
-    queryQueue(p, w, f, task, stageQueue, queue, nearestTri, qn, sref, sdist, running, lane, refreshDist);
+    queryQueue(p, w, f, task, stageQueue, queue, nearestTri, qn, sref, sdist, running, lane, refreshDist, warmTri);
 }
 
 // One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
@@ -1288,11 +1384,60 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             float4 bi = p.body[f.i], bj = p.body[f.j];
             f.s_i = bi.x; f.s_j = bj.x; f.wi = bi.w; f.wj = bj.w;
             f.branch = centerBranch(p, f.i, f.j);
+            // Temporal skip.  The node's target-frame point is
+            //   x = R_j^T (R_i a - b) / s_j,   a = s_i p,
+            // and for unit quaternions |R(q) v - R(q') v| <= 2 |q - q'| |v|, so since
+            // the anchor it has moved at most
+            //   (2 |dq_j| (|a| + |b|) + 2 |dq_i| |a| + |db|) / s_j = c0 + c1 |p|.
+            // The exact signed distance is 1-Lipschitz in x.  A node whose anchor
+            // distance exceeds that bound (plus SKIP_MARGIN) is still outside the
+            // target: not a contact, passed over.  One whose anchor distance is below
+            // minus the bound is still inside the target, at least that far from the
+            // hull (which contains the mesh), so the rejection plane and hull tests
+            // would pass it: it is queued without them.  Either way the queue holds
+            // every contact in walk order and the hits and their ranks are unchanged.
+            // A task without a valid anchor, or whose bound has grown past
+            // skipLimit, tests and queries every node and re-anchors; the rejection
+            // plane of the anchor (any hull plane is a valid filter) is reused.
+            bool skipMode = false;
+            float c0 = 0.0f, c1 = 0.0f;
+            int8_t* dist = nullptr;
+            int8_t* refreshDist = nullptr;
+            uint32_t skipped = 0, anchoredPlane = 0xffffffffu;
+            uint16_t* warmTri = w.warm ? uniformPtr(w.nodeTri + size_t(task) * w.nodeStride) : nullptr;
+            if (w.skip) {
+                dist = w.nodeDist + size_t(task) * w.nodeStride;
+                const float4* an = w.anchor + 4 * size_t(task);
+                const float4 a0 = an[0], a1 = an[1], a2 = an[2], a3 = an[3];
+                const bool valid = __float_as_uint(a3.w) == 1u && __float_as_uint(a3.x) == f.i &&
+                                   __float_as_uint(a3.y) == f.j && __float_as_uint(a3.z) == ld_volatile(w.counters + 10 * LINE);
+                const float dqi = sqrtf(dot4(f.qi - v4of(a0), f.qi - v4of(a0)));
+                const float dqj = sqrtf(dot4(f.qj - v4of(a1), f.qj - v4of(a1)));
+                const float db = length(f.branch - xyz(a2));
+                c0 = 1.001f * (2.0f * dqj * length(f.branch) + db) / f.s_j;
+                c1 = 1.001f * (2.0f * dqj + 2.0f * dqi) * f.s_i / f.s_j;
+                skipMode = valid && c0 + c1 * (p.body[f.i].z / f.s_i) + SKIP_MARGIN < w.skipLimit;
+                c0 = uniformF(c0);
+                c1 = uniformF(c1);
+                skipMode = uniformU(skipMode ? 1u : 0u) != 0u;
+                dist = uniformPtr(dist);
+                if (skipMode) {
+                    anchoredPlane = uniformU(__float_as_uint(a2.w));
+                } else {
+                    refreshDist = dist;
+                    const uint32_t words = (p.sampleInfo[f.si].y + 3) / 4;
+                    uint32_t* d32 = reinterpret_cast<uint32_t*>(dist);
+                    for (uint32_t k = lane; k < words; k += WAVE) d32[k] = 0u;
+                    __threadfence_block();
+                }
+            }
             // Rejection plane: largest violation at the source centre, first in
             // plane order on ties (strict >, initial -1e30), as a wave reduction
             // over (value, sequence) pairs.
-            v3 centre = -qinv(f.qj, f.branch) / f.s_j;
-            {
+            if (skipMode) {
+                f.plane = anchoredPlane == 0xffffffffu ? make_float4(0, 0, 0, 0) : p.meshPlanes[anchoredPlane];
+            } else {
+                v3 centre = -qinv(f.qj, f.branch) / f.s_j;
                 float v0 = -1e30f;
                 uint32_t s0 = 0xffffffffu, i0 = 0xffffffffu;
                 uint32_t sq = 0;
@@ -1310,79 +1455,83 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     uint32_t os = __shfl_xor(s0, d), oi = __shfl_xor(i0, d);
                     if (ov > v0 || (ov == v0 && os < s0)) { v0 = ov; s0 = os; i0 = oi; }
                 }
+                anchoredPlane = i0;
                 f.plane = i0 == 0xffffffffu ? make_float4(0, 0, 0, 0) : p.meshPlanes[i0];
             }
             uniformFrame(f);
-            // Temporal skip.  The node's target-frame point is
-            //   x = R_j^T (R_i a - b) / s_j,   a = s_i p,
-            // and for unit quaternions |R(q) v - R(q') v| <= 2 |q - q'| |v|, so since
-            // the anchor it has moved at most
-            //   (2 |dq_j| (|a| + |b|) + 2 |dq_i| |a| + |db|) / s_j = c0 + c1 |p|.
-            // The exact signed distance is 1-Lipschitz in x, so a node whose anchor
-            // distance exceeds that bound (plus SKIP_MARGIN) is still outside the
-            // target: not a contact, and skipping it leaves every hit and its rank
-            // unchanged.  A task without a valid anchor, or whose bound has grown
-            // past skipLimit, queries every node and re-anchors.
-            bool skipMode = false;
-            float c0 = 0.0f, c1 = 0.0f;
-            uint8_t* dist = nullptr;
-            uint8_t* refreshDist = nullptr;
-            uint32_t skipped = 0;
-            if (w.skip) {
-                dist = w.nodeDist + size_t(task) * w.nodeStride;
-                const float4* an = w.anchor + 4 * size_t(task);
-                const float4 a0 = an[0], a1 = an[1], a2 = an[2], a3 = an[3];
-                const bool valid = __float_as_uint(a3.w) == 1u && __float_as_uint(a3.x) == f.i &&
-                                   __float_as_uint(a3.y) == f.j && __float_as_uint(a3.z) == ld_volatile(w.counters + 10 * LINE);
-                const float dqi = sqrtf(dot4(f.qi - v4of(a0), f.qi - v4of(a0)));
-                const float dqj = sqrtf(dot4(f.qj - v4of(a1), f.qj - v4of(a1)));
-                const float db = length(f.branch - xyz(a2));
-                c0 = 1.001f * (2.0f * dqj * length(f.branch) + db) / f.s_j;
-                c1 = 1.001f * (2.0f * dqj + 2.0f * dqi) * f.s_i / f.s_j;
-                skipMode = valid && c0 + c1 * (p.body[f.i].z / f.s_i) + SKIP_MARGIN < w.skipLimit;
-                c0 = uniformF(c0);
-                c1 = uniformF(c1);
-                skipMode = uniformU(skipMode ? 1u : 0u) != 0u;
-                dist = uniformPtr(dist);
-                if (!skipMode) {
-                    refreshDist = dist;
-                    const uint32_t words = (p.sampleInfo[f.si].y + 3) / 4;
-                    uint32_t* d32 = reinterpret_cast<uint32_t*>(dist);
-                    for (uint32_t k = lane; k < words; k += WAVE) d32[k] = 0u;
-                    __threadfence_block();
-                }
-            }
-            // Cluster culls for every cluster of the source template, lane-parallel.
-            uint4 info = p.meshClusterInfo[f.si];
+            // Leaf culls, lane-parallel.  Internal clusters are not tested: a leaf's
+            // own test is a valid filter, and a leaf the preorder walk would prune
+            // with an ancestor holds only nodes outside the target, so the hits and
+            // their walk order (leaves in preorder, nodes in leaf order) are the
+            // same.  Under the temporal skip a leaf whose stored bound clears the
+            // motion bound of its box is passed over untested; a re-anchoring task
+            // records the cull's evidence for each rejected leaf.
+            const uint4 info = p.meshClusterInfo[f.si];
+            int8_t* leafDist = w.skip ? uniformPtr(w.leafDist + size_t(task) * w.clusterStride) : nullptr;
+            uint32_t leafSkipped = 0;  // per lane, summed below
             for (uint32_t base = 0; base < info.y; base += WAVE) {
-                uint32_t c = base + lane;
-                bool ok = c < info.y && meshClusterMayContactFast(p, w, f.qi, f.qj, f.s_i, f.s_j, f.branch, f.sj, info.x + c);
-                uint64_t m = __ballot(ok);
+                const uint32_t c = base + lane;
+                bool ok = false;
+                if (c < info.y) {
+                    const uint4 range = p.meshClusterRanges[info.x + c];
+                    if (range.y > 0) {
+                        bool farLeaf = false;
+                        if (skipMode) {
+                            const v3 lo = xyz(p.meshClusterMin[info.x + c]), hi = xyz(p.meshClusterMax[info.x + c]);
+                            const float radius = length(fmax3(fabs3(lo), fabs3(hi)));
+                            farLeaf = float(leafDist[c]) * SKIP_QUANTUM > c0 + c1 * radius + SKIP_MARGIN;
+                            if (farLeaf) leafSkipped += range.y;
+                        }
+                        if (!farLeaf) {
+                            const float out =
+                                meshClusterOutsideFast(p, f.qi, f.qj, f.s_i, f.s_j, f.branch, f.sj, info.x + c);
+                            ok = !(out > 0.0f);
+                            if (refreshDist && !ok) leafDist[c] = quantizeDistance(out);
+                        }
+                    }
+                }
+                const uint64_t m = __ballot(ok);
                 if (lane == 0) cand[base / WAVE] = m;
             }
+            if (skipMode) {
+                for (int d = 1; d < WAVE; d <<= 1) leafSkipped += __shfl_xor(leafSkipped, d);
+                skipped += leafSkipped;
+            }
             wave_sync();
-            // Stackless preorder walk; candidate leaves feed the query queue.
+            // Candidate leaves in preorder feed the query queue.
             uint32_t qn = 0;
-            uint32_t cluster = info.x, end = info.x + info.y;
-            while (cluster < end) {
-                uint32_t rel = cluster - info.x;
-                bool candidate = (cand[rel / WAVE] >> (rel % WAVE)) & 1ull;
-                uint4 range = p.meshClusterRanges[cluster];
-                if (candidate && range.y > 0) {
+            const uint32_t words = (info.y + WAVE - 1) / WAVE;
+            for (uint32_t word = 0; word < words; word++) {
+                uint64_t bits = cand[word];
+                while (bits != 0) {
+                    const uint32_t rel = word * WAVE + uint32_t(__builtin_ctzll(bits));
+                    bits &= bits - 1;
+                    const uint4 range = p.meshClusterRanges[info.x + rel];
                     bool pass_ = false;
                     uint32_t node = 0;
                     bool far = false;
                     if (uint32_t(lane) < range.y) {
                         node = p.meshSampleIndices[range.x + lane];
                         float4 sp = p.sample[f.sampleBase + node];
-                        if (skipMode)
-                            far = float(dist[node]) * SKIP_QUANTUM > c0 + c1 * length(xyz(sp)) + SKIP_MARGIN;
-                        if (!far) {
+                        bool inside = false;
+                        if (skipMode) {
+                            const float stored = float(dist[node]) * SKIP_QUANTUM;
+                            const float reach = c0 + c1 * length(xyz(sp)) + SKIP_MARGIN;
+                            far = stored > reach;
+                            inside = stored < -reach;
+                        }
+                        if (inside) {
+                            pass_ = true;
+                        } else if (!far) {
                             v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
                             v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
                             float planePad = 64.0f * DEM_EPS * (1.0f + maxabs(local));
-                            pass_ = !(dot(xyz(f.plane), local) + f.plane.w > planePad) &&
-                                    meshMayContainFast(p, w, f.sj, local);
+                            // The rejection plane and hull tests; a node they reject
+                            // keeps their evidence as its distance lower bound.
+                            const float rejection = dot(xyz(f.plane), local) + f.plane.w;
+                            const float outside = rejection > planePad ? rejection : meshOutsideFast(p, f.sj, local);
+                            pass_ = !(outside > 0.0f);
+                            if (refreshDist && !pass_) refreshDist[node] = quantizeDistance(outside);
                         }
                     }
                     if (skipMode) skipped += __popcll(__ballot(far));
@@ -1390,7 +1539,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     uint32_t n = __popcll(m);
                     if (qn + n > QN) {
                         wave_sync();
-                        flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist);
+                        flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist, warmTri);
                         queried += qn;
                         qn = 0;
                         wave_sync();
@@ -1398,17 +1547,32 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     if (pass_) queue[qn + __popcll(m & lanes_below(lane))] = node;
                     qn += n;
                 }
-                cluster = candidate ? cluster + 1 : range.z;
             }
             wave_sync();
-            flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist);
+            flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist, warmTri);
             queried += qn;
             wave_sync();
+            if (refreshDist) {
+                // Each candidate leaf's bound: the least of its nodes' stored values.
+                __threadfence_block();
+                for (uint32_t word = 0; word < words; word++) {
+                    uint64_t bits = cand[word];
+                    while (bits != 0) {
+                        const uint32_t rel = word * WAVE + uint32_t(__builtin_ctzll(bits));
+                        bits &= bits - 1;
+                        const uint4 range = p.meshClusterRanges[info.x + rel];
+                        int v = 127;
+                        if (uint32_t(lane) < range.y) v = dist[p.meshSampleIndices[range.x + lane]];
+                        for (int d = 1; d < WAVE; d <<= 1) v = min(v, __shfl_xor(v, d));
+                        if (lane == 0) leafDist[rel] = int8_t(v);
+                    }
+                }
+            }
             if (refreshDist && lane == 0) {
                 float4* an = w.anchor + 4 * size_t(task);
                 an[0] = f4(f.qi);
                 an[1] = f4(f.qj);
-                an[2] = f4(f.branch, 0.0f);
+                an[2] = f4(f.branch, __uint_as_float(anchoredPlane));
                 an[3] = make_float4(__uint_as_float(f.i), __uint_as_float(f.j),
                                     __uint_as_float(ld_volatile(w.counters + 10 * LINE)), __uint_as_float(1u));
             }
@@ -1893,10 +2057,25 @@ __global__ void k_refresh_wall_power(P p) {
     p.wallRows[3 * index + 1] = value;
 }
 
-// Largest sample-node count of any template (sizes the temporal-skip rows).
-__global__ void k_max_nodes(P p, uint32_t* out) {
+// Per template: the largest sample-node count (sizes the temporal rows), and the
+// first triangle and triangle count of its BVH (leftmost and rightmost leaves of
+// the preorder tree; an internal node's children are n + 1 and that child's
+// escape link), with the largest count.
+__global__ void k_template_extent(P p, uint32_t* out, uint32_t* triBase) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < p.c.shapes) atomicMax(out, p.sampleInfo[t].y);
+    if (t >= p.c.shapes || t >= p.templates) return;
+    atomicMax(out, p.sampleInfo[t].y);
+    const uint32_t root = p.meshRoots[t].x;
+    uint32_t n = root;
+    for (int d = 0; d < 64 && p.meshLinks[n].z == 0; d++) n = n + 1;
+    const uint32_t first = p.meshLinks[n].y;
+    n = root;
+    for (int d = 0; d < 64 && p.meshLinks[n].z == 0; d++) n = p.meshLinks[n + 1].x;
+    const uint32_t count = p.meshLinks[n].y + p.meshLinks[n].z - first;
+    triBase[2 * t] = first;
+    triBase[2 * t + 1] = count;
+    atomicMax(out + 1, count);
+    atomicMax(out + 2, p.meshClusterInfo[t].y);
 }
 __global__ void k_set_word(uint32_t* word, uint32_t value) { *word = value; }
 
@@ -2123,6 +2302,10 @@ struct ScratchImpl : Scratch {
     size_t nodeDistCap = 0;
     uint32_t anchorCap = 0;
     uint32_t* probeWord = nullptr;
+    uint32_t* triBase = nullptr;
+    uint32_t triBaseCap = 0, maxTriangles = 0, maxClusters = 0;
+    size_t leafDistCap = 0;
+    const void* triBaseFor = nullptr;
     // Captured timesteps keyed by everything a step's launches bake in.
     struct Graph { std::vector<uint64_t> key; hipGraphExec_t exec; };
     std::vector<Graph> graphs;
@@ -2210,20 +2393,34 @@ struct ScratchImpl : Scratch {
         key.push_back(p.c.N);
         if (key != lastTable) { lastTable.swap(key); epoch++; }
         w.skip = 0;
+        w.warm = 0;
         if (!skip) return SAD_OK;
-        if (maxNodesFor != p.sampleInfo || maxNodesShapes != p.c.shapes) {
-            if (!probeWord) HIP_OK(hipMalloc(reinterpret_cast<void**>(&probeWord), sizeof(uint32_t)));
-            HIP_OK(hipMemset(probeWord, 0, sizeof(uint32_t)));
-            hipLaunchKernelGGL(k_max_nodes, dim3((std::max(p.c.shapes, 1u) + 255) / 256), dim3(256), 0, nullptr, p, probeWord);
+        if (maxNodesFor != p.sampleInfo || maxNodesShapes != p.c.shapes || triBaseFor != p.meshLinks) {
+            if (!probeWord) HIP_OK(hipMalloc(reinterpret_cast<void**>(&probeWord), 3 * sizeof(uint32_t)));
+            if (triBaseCap < 2 * std::max(p.c.shapes, 1u)) {
+                if (triBase) (void)hipFree(triBase);
+                triBase = nullptr;
+                triBaseCap = 2 * std::max(p.c.shapes, 1u);
+                HIP_OK(hipMalloc(reinterpret_cast<void**>(&triBase), triBaseCap * sizeof(uint32_t)));
+            }
+            HIP_OK(hipMemset(probeWord, 0, 3 * sizeof(uint32_t)));
+            HIP_OK(hipMemset(triBase, 0, triBaseCap * sizeof(uint32_t)));
+            hipLaunchKernelGGL(k_template_extent, dim3((std::max(p.c.shapes, 1u) + 255) / 256), dim3(256), 0, nullptr,
+                               p, probeWord, triBase);
             HIP_OK(hipGetLastError());
-            HIP_OK(hipMemcpy(&maxNodes, probeWord, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            uint32_t extent[3] = {0, 0, 0};
+            HIP_OK(hipMemcpy(extent, probeWord, sizeof(extent), hipMemcpyDeviceToHost));
+            maxNodes = extent[0];
+            maxTriangles = extent[1];
+            maxClusters = extent[2];
             maxNodesFor = p.sampleInfo;
             maxNodesShapes = p.c.shapes;
+            triBaseFor = p.meshLinks;
         }
         const uint32_t stride = (std::max(maxNodes, 1u) + 63) / 64 * 64;
         const size_t tasks = std::max<size_t>(w.taskCap, 1);
         const size_t bytesNeeded = tasks * stride;
-        if (bytesNeeded > (size_t(48) << 30)) return SAD_OK;  // more than 48 GiB: leave the skip off
+        if (3 * bytesNeeded > (size_t(64) << 30)) return SAD_OK;  // more than 64 GiB of rows: leave it off
         if (anchorCap < 4 * tasks) {
             if (w.anchor) { (void)hipFree(w.anchor); bytes -= size_t(anchorCap) * sizeof(float4); }
             w.anchor = nullptr;
@@ -2235,14 +2432,32 @@ struct ScratchImpl : Scratch {
         }
         if (nodeDistCap < bytesNeeded || w.nodeStride != stride) {
             if (w.nodeDist) { (void)hipFree(w.nodeDist); bytes -= nodeDistCap; }
+            if (w.nodeTri) { (void)hipFree(w.nodeTri); bytes -= 2 * nodeDistCap; }
             w.nodeDist = nullptr;
+            w.nodeTri = nullptr;
             nodeDistCap = 0;
             HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.nodeDist), bytesNeeded));
+            // Any content is safe for the warm start (a triangle index in range is an
+            // upper bound of the minimum; out of range is ignored): no clearing.
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.nodeTri), 2 * bytesNeeded));
             nodeDistCap = bytesNeeded;
-            bytes += bytesNeeded;
+            bytes += 3 * bytesNeeded;
             epoch++;  // stored distances are gone: no anchor may be trusted
         }
         w.nodeStride = stride;
+        const uint32_t clusterStride = (std::max(maxClusters, 1u) + 63) / 64 * 64;
+        if (leafDistCap < tasks * clusterStride || w.clusterStride != clusterStride) {
+            if (w.leafDist) { (void)hipFree(w.leafDist); bytes -= leafDistCap; }
+            w.leafDist = nullptr;
+            leafDistCap = 0;
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.leafDist), tasks * clusterStride));
+            leafDistCap = tasks * clusterStride;
+            bytes += leafDistCap;
+            epoch++;
+        }
+        w.clusterStride = clusterStride;
+        w.triBase = triBase;
+        w.warm = maxTriangles <= 65534u ? 1u : 0u;
         w.skipLimit = float(skipLimitMicro) * 1e-6f;
         w.skip = 1;
         return SAD_OK;
@@ -2266,7 +2481,8 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
-        (void)hipFree(w.anchor); (void)hipFree(w.nodeDist); (void)hipFree(probeWord);
+        (void)hipFree(w.anchor); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
+        (void)hipFree(triBase); (void)hipFree(w.leafDist);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
 };
