@@ -103,13 +103,14 @@ struct W {
     // rotations, the branch, and the pair/epoch key) and one byte per source node:
     // its signed distance to the target at the anchor poses, quantized down.
     float4* anchor;       // 4 rows per task
-    int8_t* nodeDist;     // nodeStride bytes per task, indexed by node id
-    uint16_t* nodeTri;    // nodeStride entries per task: 1 + the node's last nearest triangle
+    int8_t* nodeDist;     // nodeStride bytes per task, indexed by the node's slot (key position in
+                          // its template, so a leaf's nodes are contiguous)
+    uint16_t* nodeTri;    // nodeStride entries per task (by slot): 1 + the node's last nearest triangle
                           // (relative to its template's first), 0 = none
     int8_t* leafDist;     // clusterStride bytes per task: per sample-tree leaf, a lower bound of
                           // its nodes' signed distances at the anchor poses
     uint32_t clusterStride;
-    const uint32_t* triBase;  // per template: first triangle, triangle count (2 words)
+    const uint32_t* triBase;  // per template: first triangle, triangle count, first sample key
     uint32_t warm;        // 1 = warm-start the exact queries from nodeTri
     uint32_t nodeStride;  // bytes per task (>= nodes of any template, multiple of 64)
     uint32_t skip;        // 1 = on
@@ -1087,7 +1088,7 @@ __device__ bool cullBatch(const P& p, const W& w, const TaskFrame& f, const uint
             uint32_t k = q + half * 32 + lane;
             uint16_t* row = bufA + lane * nn::K1;
             if (k < qn) {
-                float4 sp = p.sample[f.sampleBase + queue[k]];
+                float4 sp = p.sample[f.sampleBase + (queue[k] & 0xffffu)];
                 v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
                 v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
                 nn_features(local, fb, row);
@@ -1153,10 +1154,10 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
     // never pruned (their boxes are no farther than the minimum), and the record
     // test still starts from 1e30, so the winner -- the first triangle in visit
     // order at the minimum distance -- is the one the plain traversal finds.
-    const uint32_t triFirst = warmTri ? w.triBase[2 * f.sj] : 0u, triCount = warmTri ? w.triBase[2 * f.sj + 1] : 0u;
+    const uint32_t triFirst = warmTri ? w.triBase[3 * f.sj] : 0u, triCount = warmTri ? w.triBase[3 * f.sj + 1] : 0u;
     auto start = [&]() {
         v3 arm;
-        const uint32_t node = queue[k];
+        const uint32_t node = queue[k] & 0xffffu, slot = queue[k] >> 16;
         q = queryPoint(f, p.sample[f.sampleBase + node], arm);
         sref[0] = rootRef;
         dstore(sdist, boxDistance(rootLo, rootHi, q));
@@ -1165,7 +1166,7 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
         prune = 1e30f;
         bestTri = NO_TRIANGLE;
         if (warmTri) {
-            const uint32_t last = warmTri[node];
+            const uint32_t last = warmTri[slot];
             if (last != 0u && last <= triCount) {
                 uint32_t feature;
                 const float4* t = p.meshTriangles + 3 * size_t(triFirst + last - 1);
@@ -1230,7 +1231,8 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
         uint32_t node = 0;
         float4 pd = make_float4(0, 0, 0, 0), na = make_float4(0, 0, 0, 0);
         if (kk < qn) {
-            node = queue[kk];
+            node = queue[kk] & 0xffffu;
+            const uint32_t slot = queue[kk] >> 16;
             const float4 sp = p.sample[f.sampleBase + node];
             v3 arm;
             const v3 x = queryPoint(f, sp, arm);
@@ -1248,8 +1250,8 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
             const float distance = sqrtf(bestD);
             const float side = dot(x - nearest, pseudo) < 0 ? -1.0f : 1.0f;
             const v3 normal = distance > 1e-12f ? side * (x - nearest) / distance : pseudo;
-            if (refreshDist) refreshDist[node] = quantizeDistance(side * distance);
-            if (warmTri && triangle != NO_TRIANGLE) warmTri[node] = uint16_t(triangle - triFirst + 1);
+            if (refreshDist) refreshDist[slot] = quantizeDistance(side * distance);
+            if (warmTri && triangle != NO_TRIANGLE) warmTri[slot] = uint16_t(triangle - triFirst + 1);
             // The rest of queryNode.
             const float depth = -(side * distance) * f.s_j;
             pd = f4(arm, depth);
@@ -1280,7 +1282,7 @@ __device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32
     for (uint32_t q = 0; q < qn; q += WAVE) {
         uint32_t k = q + lane;
         bool culled = cullBatch(p, w, f, queue, q, qn, nnA, nnB, lane);
-        uint32_t node = k < qn ? queue[k] : 0u;
+        const uint32_t word = k < qn ? queue[k] : 0u, node = word & 0xffffu;
         culledTotal += __popcll(__ballot(culled));
         if (w.nnAudit) {
             bool hit = false;
@@ -1292,7 +1294,7 @@ __device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32
             bool keep = k < qn && !culled;
             uint64_t m = __ballot(keep);
             wave_sync();
-            if (keep) queue[kept + __popcll(m & lanes_below(lane))] = node;
+            if (keep) queue[kept + __popcll(m & lanes_below(lane))] = word;
             kept += __popcll(m);
             wave_sync();
         }
@@ -1468,6 +1470,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             // records the cull's evidence for each rejected leaf.
             const uint4 info = p.meshClusterInfo[f.si];
             int8_t* leafDist = w.skip ? uniformPtr(w.leafDist + size_t(task) * w.clusterStride) : nullptr;
+            const uint32_t keysBase = w.skip ? uniformU(w.triBase[3 * f.si + 2]) : 0u;
             uint32_t leafSkipped = 0;  // per lane, summed below
             for (uint32_t base = 0; base < info.y; base += WAVE) {
                 const uint32_t c = base + lane;
@@ -1510,12 +1513,14 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     bool pass_ = false;
                     uint32_t node = 0;
                     bool far = false;
+                    uint32_t slot = 0;
                     if (uint32_t(lane) < range.y) {
                         node = p.meshSampleIndices[range.x + lane];
+                        if (w.skip) slot = range.x + lane - keysBase;
                         float4 sp = p.sample[f.sampleBase + node];
                         bool inside = false;
                         if (skipMode) {
-                            const float stored = float(dist[node]) * SKIP_QUANTUM;
+                            const float stored = float(dist[slot]) * SKIP_QUANTUM;
                             const float reach = c0 + c1 * length(xyz(sp)) + SKIP_MARGIN;
                             far = stored > reach;
                             inside = stored < -reach;
@@ -1531,7 +1536,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                             const float rejection = dot(xyz(f.plane), local) + f.plane.w;
                             const float outside = rejection > planePad ? rejection : meshOutsideFast(p, f.sj, local);
                             pass_ = !(outside > 0.0f);
-                            if (refreshDist && !pass_) refreshDist[node] = quantizeDistance(outside);
+                            if (refreshDist && !pass_) refreshDist[slot] = quantizeDistance(outside);
                         }
                     }
                     if (skipMode) skipped += __popcll(__ballot(far));
@@ -1544,7 +1549,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                         qn = 0;
                         wave_sync();
                     }
-                    if (pass_) queue[qn + __popcll(m & lanes_below(lane))] = node;
+                    if (pass_) queue[qn + __popcll(m & lanes_below(lane))] = node | (slot << 16);
                     qn += n;
                 }
             }
@@ -1562,7 +1567,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                         bits &= bits - 1;
                         const uint4 range = p.meshClusterRanges[info.x + rel];
                         int v = 127;
-                        if (uint32_t(lane) < range.y) v = dist[p.meshSampleIndices[range.x + lane]];
+                        if (uint32_t(lane) < range.y) v = dist[range.x + lane - keysBase];
                         for (int d = 1; d < WAVE; d <<= 1) v = min(v, __shfl_xor(v, d));
                         if (lane == 0) leafDist[rel] = int8_t(v);
                     }
@@ -2057,10 +2062,10 @@ __global__ void k_refresh_wall_power(P p) {
     p.wallRows[3 * index + 1] = value;
 }
 
-// Per template: the largest sample-node count (sizes the temporal rows), and the
-// first triangle and triangle count of its BVH (leftmost and rightmost leaves of
-// the preorder tree; an internal node's children are n + 1 and that child's
-// escape link), with the largest count.
+// Per template: the largest sample-node count (sizes the temporal rows), the first
+// triangle and triangle count of its BVH (leftmost and rightmost leaves of the
+// preorder tree; an internal node's children are n + 1 and that child's escape
+// link), its first sample key, and the largest triangle and cluster counts.
 __global__ void k_template_extent(P p, uint32_t* out, uint32_t* triBase) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= p.c.shapes || t >= p.templates) return;
@@ -2072,10 +2077,16 @@ __global__ void k_template_extent(P p, uint32_t* out, uint32_t* triBase) {
     n = root;
     for (int d = 0; d < 64 && p.meshLinks[n].z == 0; d++) n = p.meshLinks[n + 1].x;
     const uint32_t count = p.meshLinks[n].y + p.meshLinks[n].z - first;
-    triBase[2 * t] = first;
-    triBase[2 * t + 1] = count;
+    // The template's sample keys start at its first leaf (leaves are laid out in
+    // preorder): a node's slot is its key position minus this base.
+    const uint4 info = p.meshClusterInfo[t];
+    uint32_t c = info.x;
+    while (c + 1 < info.x + info.y && p.meshClusterRanges[c].y == 0) c++;
+    triBase[3 * t] = first;
+    triBase[3 * t + 1] = count;
+    triBase[3 * t + 2] = p.meshClusterRanges[c].x;
     atomicMax(out + 1, count);
-    atomicMax(out + 2, p.meshClusterInfo[t].y);
+    atomicMax(out + 2, info.y);
 }
 __global__ void k_set_word(uint32_t* word, uint32_t value) { *word = value; }
 
@@ -2397,10 +2408,10 @@ struct ScratchImpl : Scratch {
         if (!skip) return SAD_OK;
         if (maxNodesFor != p.sampleInfo || maxNodesShapes != p.c.shapes || triBaseFor != p.meshLinks) {
             if (!probeWord) HIP_OK(hipMalloc(reinterpret_cast<void**>(&probeWord), 3 * sizeof(uint32_t)));
-            if (triBaseCap < 2 * std::max(p.c.shapes, 1u)) {
+            if (triBaseCap < 3 * std::max(p.c.shapes, 1u)) {
                 if (triBase) (void)hipFree(triBase);
                 triBase = nullptr;
-                triBaseCap = 2 * std::max(p.c.shapes, 1u);
+                triBaseCap = 3 * std::max(p.c.shapes, 1u);
                 HIP_OK(hipMalloc(reinterpret_cast<void**>(&triBase), triBaseCap * sizeof(uint32_t)));
             }
             HIP_OK(hipMemset(probeWord, 0, 3 * sizeof(uint32_t)));
