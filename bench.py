@@ -190,7 +190,7 @@ def main():
     log(f"[bench] warmup {args.warmup} steps done")
     tuned = None
     if not args.no_autotune:
-        tuned = autotune.autotune(e, rate=w.rate, steps=3, log=log)
+        tuned = autotune.autotune(e, rate=w.rate, steps=6, log=log)
         log(f"[bench] autotuned launch shape {tuned['best']} ({tuned['ms_per_step']:.3f} ms/step)")
     start, step0 = e.state(), e.step
     e.restore(start, step0)

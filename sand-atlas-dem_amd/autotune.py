@@ -2,11 +2,11 @@
 
 The collection kernel is a persistent work-queue launch whose shape has three
 knobs (`sad_hip_tune`): workgroups per CU, tasks taken per queue atomic, and the
-per-wavefront query-queue capacity; a fourth sets how far a directed pair may move
-before its temporal-skip anchor is renewed.  None of them changes any result (tasks
-are ranked by id, hits by their running count within a task, and the skip only
-passes over nodes proven outside), so the tuner is free to pick whichever is
-fastest on the workload at hand.  It times a few real steps per
+per-wavefront query-queue capacity.  None of them changes any result (tasks are
+ranked by id, hits by their running count within a task), so the tuner is free to
+pick whichever is fastest on the workload at hand.  The temporal-skip re-anchor
+distance is left at its default: its cost shows over hundreds of steps, not in a
+few timed ones.  It times a few real steps per
 candidate from one saved state (coordinate descent over the knobs, defaults
 first), restores the state afterwards and optionally records the winner in a JSON
 cache keyed by device and workload signature.
@@ -22,7 +22,6 @@ SPACE = {
     "collect_blocks_per_cu": (2, 3, 4, 6, 8),
     "collect_pop": (1, 2, 4),
     "collect_queue": (128, 256),
-    "skip_limit_micro": (3000, 6000, 12000, 24000),
 }
 
 

@@ -47,8 +47,8 @@ def test_graph_captured_steps_match_eager(dense):
 
 
 @pytest.mark.parametrize("setting", [
-    {"collect_blocks_per_cu": 2, "collect_pop": 4, "collect_queue": 128},
-    {"collect_blocks_per_cu": 8, "collect_pop": 2, "collect_queue": 256},
+    {"collect_blocks_per_cu": 2, "collect_pop": 4, "collect_queue": 128, "skip_limit_micro": 2000},
+    {"collect_blocks_per_cu": 8, "collect_pop": 2, "collect_queue": 256, "skip_limit_micro": 250000},
 ])
 def test_launch_shapes_do_not_change_results(dense, setting):
     from sand_atlas_dem_amd.hip import HipBackend
