@@ -34,6 +34,8 @@ WORKLOADS = {
     "small": "1,000 icospheres (subdiv 2, sigma 0.1), 512 nodes/grain, phi 0.30, v ~ N(0, 0.01 m/s), mu 0.5",
     "walls": "10,000 grains from the dense library at phi 0.58 in a nonperiodic box of six planes, top plane moving "
              "down at 0.5 box heights/s, mu 0.5",
+    "layer": "10,000 grains from the dense library at phi 0.58 between two z planes, x and y periodic, top plane "
+             "moving down at 0.5 layer heights/s, mu 0.5",
 }
 
 

@@ -305,6 +305,12 @@ inline void accumulate(const R& r, uint32_t id, float4 inc) {  // :130-134
     r.accumulated[id] = high;
     r.accumulatedRemainder[id] = low;
 }
+// Periodicity: Config `periodic` is 0 (none) or 1 (every axis) as the reference
+// sends it; 8 | mask (x = 1, y = 2, z = 4) selects axes (no reference counterpart:
+// walls across one axis of an otherwise periodic cell).  0 and 1 take exactly the
+// reference's paths.
+inline bool periodicAxis(const Config& c, int a) { return c.periodic == 1u || (c.periodic >= 8u && ((c.periodic >> a) & 1u)); }
+inline bool anyPeriodic(const Config& c) { return c.periodic == 1u || (c.periodic >= 8u && (c.periodic & 7u) != 0u); }
 // CPU override (cpu_scheduling.slang:10-15): the branch subtraction in FP64.
 inline float3 centerBranch(const R& r, uint32_t i, uint32_t j) {
     float4 c = r.cell[0], pj = r.position[j], pi = r.position[i];
@@ -313,11 +319,9 @@ inline float3 centerBranch(const R& r, uint32_t i, uint32_t j) {
     double dx = (double(pj.x) - double(pi.x)) + (double(rj.x) - double(ri.x));
     double dy = (double(pj.y) - double(pi.y)) + (double(rj.y) - double(ri.y));
     double dz = (double(pj.z) - double(pi.z)) + (double(rj.z) - double(ri.z));
-    if (r.c.periodic != 0) {
-        dx = dx - std::floor(dx / bx + 0.5) * bx;
-        dy = dy - std::floor(dy / by + 0.5) * by;
-        dz = dz - std::floor(dz / bz + 0.5) * bz;
-    }
+    if (periodicAxis(r.c, 0)) dx = dx - std::floor(dx / bx + 0.5) * bx;
+    if (periodicAxis(r.c, 1)) dy = dy - std::floor(dy / by + 0.5) * by;
+    if (periodicAxis(r.c, 2)) dz = dz - std::floor(dz / bz + 0.5) * bz;
     return {real(dx), real(dy), real(dz)};
 }
 inline float3 omega(const R& r, uint32_t i) {  // :154-158
@@ -345,7 +349,9 @@ inline real cellPower(const R& r, uint32_t i) {  // :176-180
 }
 inline float3 image(const R& r, float3 d) {  // :181
     float3 box = xyz(r.cell[0]);
-    return r.c.periodic != 0 ? d - floor3(d / box + f3(real(0.5), real(0.5), real(0.5))) * box : d;
+    if (!anyPeriodic(r.c)) return d;
+    float3 w = d - floor3(d / box + f3(real(0.5), real(0.5), real(0.5))) * box;
+    return {periodicAxis(r.c, 0) ? w.x : d.x, periodicAxis(r.c, 1) ? w.y : d.y, periodicAxis(r.c, 2) ? w.z : d.z};
 }
 inline void fail(const R& r, uint32_t bit) { r.control[0] |= bit; }  // :183 (serial)
 inline void binIndex(const R& r, uint32_t i, uint32_t b[3]) {  // :184
@@ -743,10 +749,15 @@ void beginStep(const R& r, uint32_t id) {  // solver.slang:859-888
     float3 p = xyz(r.position[id]), pl = xyz(r.positionRemainder[id]);
     compensated3(p, pl, expm1Stable3(rate * dt) * (p + pl) + dt * (v + vl));
     if (!finite3(p) || !finite3(pl)) { fail(r, 512); return; }
-    if (r.c.periodic != 0) p = p - floor3(p / box) * box;
+    if (r.c.periodic == 1) p = p - floor3(p / box) * box;
     else {
+        if (anyPeriodic(r.c)) {
+            float3 w = p - floor3(p / box) * box;
+            p = {periodicAxis(r.c, 0) ? w.x : p.x, periodicAxis(r.c, 1) ? w.y : p.y, periodicAxis(r.c, 2) ? w.z : p.z};
+        }
         float3 s = p + pl, t = (p - box) + pl;
-        if (s.x < 0 || s.y < 0 || s.z < 0 || t.x >= 0 || t.y >= 0 || t.z >= 0) { fail(r, 2048); return; }
+        for (int a = 0; a < 3; a++)
+            if (!periodicAxis(r.c, a) && (get(s, a) < 0 || get(t, a) >= 0)) { fail(r, 2048); return; }
     }
     r.position[id] = f4(p, 0);
     r.positionRemainder[id] = f4(pl, 0);
@@ -762,8 +773,11 @@ void advanceCell(const R& r) {  // :890-897
     r.cellLog[1] = f4(residual, 0);
     r.cell[0] = f4(xyz(r.cell[2]) * f3(exp32(logarithm.x), exp32(logarithm.y), exp32(logarithm.z)), 0);
     r.control[4] = 0;
-    if (r.c.periodic != 0 && r.referenceCell[1].w >= real(0.5) * std::fmin(r.cell[0].x, std::fmin(r.cell[0].y, r.cell[0].z)))
+    if (r.c.periodic == 1 && r.referenceCell[1].w >= real(0.5) * std::fmin(r.cell[0].x, std::fmin(r.cell[0].y, r.cell[0].z)))
         fail(r, 256);
+    if (r.c.periodic >= 8u)
+        for (int a = 0; a < 3; a++)
+            if (periodicAxis(r.c, a) && r.referenceCell[1].w >= real(0.5) * get(xyz(r.cell[0]), a)) fail(r, 256);
 }
 void checkRebuild(const R& r, uint32_t id) {  // :899-904
     if (id >= r.c.N) return;
@@ -803,9 +817,10 @@ void fillBins(const R& r, uint32_t id) {  // :351-359
     if (id >= r.c.N || r.control[4] == 0) return;
     float3 p = xyz(r.position[id]), pl = xyz(r.positionRemainder[id]);
     if (!finite3(p) || !finite3(pl)) { fail(r, 512); return; }
-    if (r.c.periodic == 0) {
+    if (r.c.periodic != 1) {
         float3 s = p + pl, t = (p - xyz(r.cell[0])) + pl;
-        if (s.x < 0 || s.y < 0 || s.z < 0 || t.x >= 0 || t.y >= 0 || t.z >= 0) { fail(r, 2048); return; }
+        for (int a = 0; a < 3; a++)
+            if (!periodicAxis(r.c, a) && (get(s, a) < 0 || get(t, a) >= 0)) { fail(r, 2048); return; }
     }
     uint32_t b[3];
     binIndex(r, id, b);
@@ -825,20 +840,16 @@ uint32_t findPairs(const R& r, uint32_t id, bool emit) {  // :360-384
     for (int a = 0; a < 3; a++) {
         low[a] = std::max(int(b[a]) - reach[a], 0);
         int high = std::min(int(b[a]) + reach[a], int(r.c.bins[a]) - 1);
-        width[a] = r.c.periodic != 0 ? int(std::min(uint32_t(2 * reach[a] + 1), r.c.bins[a])) : high - low[a] + 1;
+        width[a] = periodicAxis(r.c, a) ? int(std::min(uint32_t(2 * reach[a] + 1), r.c.bins[a])) : high - low[a] + 1;
     }
     uint32_t count = 0;
     for (int x = 0; x < width[0]; x++)
         for (int y = 0; y < width[1]; y++)
             for (int z = 0; z < width[2]; z++) {
                 uint32_t c[3];
-                if (r.c.periodic != 0) {
-                    c[0] = modIndex(int(b[0]) - reach[0] + x, r.c.bins[0]);
-                    c[1] = modIndex(int(b[1]) - reach[1] + y, r.c.bins[1]);
-                    c[2] = modIndex(int(b[2]) - reach[2] + z, r.c.bins[2]);
-                } else {
-                    c[0] = uint32_t(low[0] + x); c[1] = uint32_t(low[1] + y); c[2] = uint32_t(low[2] + z);
-                }
+                const int at[3] = {x, y, z};
+                for (int a = 0; a < 3; a++)
+                    c[a] = periodicAxis(r.c, a) ? modIndex(int(b[a]) - reach[a] + at[a], r.c.bins[a]) : uint32_t(low[a] + at[a]);
                 uint32_t bi = binFlat(r, c);
                 uint32_t occupancy = std::min(r.binCounts[bi], r.c.binCapacity);
                 for (uint32_t slot = 0; slot < occupancy; slot++) {

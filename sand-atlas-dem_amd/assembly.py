@@ -73,9 +73,10 @@ def sphere_packing(radii, box, rng, *, iterations=400, tolerance=1e-3, walled=Fa
 
     Radii grow from 0.5x to 1x over the first half of the iterations while pairwise
     overlaps are pushed apart symmetrically; stops once the largest overlap is below
-    `tolerance` times the mean diameter.  `walled` keeps every centre inside
-    [c, box - c] on each axis instead of wrapping, with c the sphere radius or the
-    given `clearance` (e.g. the grain's bounding radius), scaled like the radii.
+    `tolerance` times the mean diameter.  `walled` (a bool, or one per axis) keeps
+    every centre inside [c, box - c] on the walled axes instead of wrapping, with c
+    the sphere radius or the given `clearance` (e.g. the grain's bounding radius),
+    scaled like the radii.
     """
     from scipy.spatial import cKDTree
 
@@ -85,24 +86,25 @@ def sphere_packing(radii, box, rng, *, iterations=400, tolerance=1e-3, walled=Fa
     grow = max(1, iterations // 2)
 
     reach = radii if clearance is None else np.asarray(clearance, float)
+    walls = np.broadcast_to(np.asarray(walled, bool), (3,))
+    boxsize = None if walls.all() else (box if not walls.any() else np.where(walls, 1e9, box))
 
     def confine(x, scale):
         c = reach[:, None] * scale
-        return np.clip(x, c, box - c) if walled else np.mod(x, box)
+        return np.where(walls, np.clip(x, c, box - c), np.mod(x, box))
 
     for it in range(iterations):
         scale = min(1.0, 0.5 + 0.5 * (it + 1) / grow)
         r = radii * scale
         x = confine(x, scale)
-        tree = cKDTree(x, boxsize=None if walled else box)
+        tree = cKDTree(x, boxsize=boxsize)
         pairs = tree.query_pairs(2 * r.max(), output_type="ndarray")
         if len(pairs) == 0:
             if scale >= 1:
                 break
             continue
         d = x[pairs[:, 1]] - x[pairs[:, 0]]
-        if not walled:
-            d -= np.round(d / box) * box
+        d -= np.where(walls, 0.0, np.round(d / box) * box)
         dist = np.linalg.norm(d, axis=1)
         overlap = r[pairs[:, 0]] + r[pairs[:, 1]] - dist
         hit = overlap > 0
@@ -220,6 +222,41 @@ def walled_workload(name, shapes, n, phi, *, seed, sphere_fraction=None, size_ra
                     (0, 0, 0), steps)
 
 
+def layer_workload(name, shapes, n, phi, *, seed, sphere_fraction=None, size_range=(0.8, 1.2), axial_rate=0.5,
+                   margin=0.05, steps=1000, spec_overrides=None, rng=None, ids=None, scales=None, viscosity=0.0):
+    """Grains between two z planes in a cell periodic in x and y (per-axis periodicity).
+
+    The top plane translates down at `axial_rate` times the layer height per second;
+    the bottom plane is fixed.  Along z the cell is the layer plus `margin` of its
+    height on each side.
+    """
+    rng = rng or streams(seed)
+    if ids is None:
+        ids = rng["shapes"].integers(0, len(shapes), n).astype(np.uint32)
+    if scales is None:
+        scales = rng["sizes"].uniform(*size_range, n)
+    d = np.array([shapes[i].equivalent_diameter for i in ids]) * scales
+    side = (float(np.sum(np.pi / 6 * d**3)) / phi) ** (1 / 3)
+    alpha = ((sphere_fraction or phi) / phi) ** (1 / 3)
+    bounding = 0.5 * (0.5 * alpha * d + np.array([shapes[i].radius for i in ids]) * scales)
+    x = sphere_packing(0.5 * alpha * d, side, rng["positions"], walled=(False, False, True), clearance=bounding)
+    pad = margin * side
+    overrides = dict(spec_overrides or {})
+    kn, mu = overrides["normal_stiffness"], overrides.get("friction", 0.5)
+    planes = []
+    for normal, offset, velocity in (((0.0, 0.0, 1.0), pad, (0.0, 0.0, 0.0)),
+                                     ((0.0, 0.0, -1.0), -(pad + side), (0.0, 0.0, -axial_rate * side))):
+        plane = dict(normal=list(normal), offset=offset, velocity=list(velocity), normal_stiffness=kn,
+                     tangential_stiffness=0.5 * kn, friction=mu)
+        if viscosity:
+            plane["normal_viscosity"] = viscosity
+        planes.append(plane)
+    spec = Experiment(contact_geometry="mesh", particles=n, periodic=(True, True, False), planes=planes, **overrides)
+    q = random_rotations(n, rng["orientations"])
+    return Workload(name, shapes, spec, x + np.array([0.0, 0.0, pad]), np.array([side, side, side + 2 * pad]), ids,
+                    scales, q, np.zeros((n, 3)), (0, 0, 0), steps)
+
+
 def conservative_timestep(shapes, ids, scales, spec):
     """Largest dt the conservative preflight accepts (`gpu.py:370-413`), rounded down."""
     limits = []
@@ -247,6 +284,8 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
              (1,280 triangles, 1,024 nodes), phi 0.55 -- loose hull bounds
     walls  : 10,000 grains from the dense library at phi 0.58 in a nonperiodic box
              of six planes, the top plane moving down at 0.5 box heights per second
+    layer  : the same grains between two z planes in a cell periodic in x and y, the
+             top plane moving down at 0.5 layer heights per second
     """
     rng = streams(seed)
     d0 = 3e-4
@@ -286,7 +325,7 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
         phi, sphere_fraction, vsig, rate = 0.55, 0.57, 0.0, (0, 0, 0)
         steps = steps or 1000
         cpp = 512
-    elif config == "walls":
+    elif config in ("walls", "layer"):
         n = n or 10_000
         shapes = icosphere_library(templates or 64, subdivisions=(2, 3, 4), sigma=0.15, diameter=d0,
                                    nodes=nodes or 2048, rng=rng["templates"], seed=seed)
@@ -300,6 +339,9 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
     scales = rng["sizes"].uniform(0.8, 1.2, n)
     dt = conservative_timestep(shapes, ids, scales, probe)
     overrides = dict(base, timestep=dt, max_overlap_fraction=0.5, contacts_per_particle=cpp)
+    if config == "layer":
+        return layer_workload(config, shapes, n, phi, seed=seed, sphere_fraction=sphere_fraction, steps=steps,
+                              spec_overrides=overrides, rng=rng, ids=ids, scales=scales)
     if config == "walls":
         return walled_workload(config, shapes, n, phi, seed=seed, sphere_fraction=sphere_fraction, steps=steps,
                                spec_overrides=overrides, rng=rng, ids=ids, scales=scales)

@@ -155,6 +155,17 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // ---------------------------------------------------------------- shared helpers (solver.slang:130-186)
+// Periodicity: Config `periodic` is 0 (none) or 1 (every axis) as the reference
+// sends it; 8 | mask (x = 1, y = 2, z = 4) selects axes (no reference counterpart:
+// walls across one axis of an otherwise periodic cell).  0 and 1 take exactly the
+// reference's paths.
+__device__ __forceinline__ bool periodicAxis(const Cfg& c, int a) {
+    return c.periodic == 1u || (c.periodic >= 8u && ((c.periodic >> a) & 1u));
+}
+__device__ __forceinline__ bool anyPeriodic(const Cfg& c) {
+    return c.periodic == 1u || (c.periodic >= 8u && (c.periodic & 7u) != 0u);
+}
+__device__ __forceinline__ float axisOf(v3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
 __device__ v3 centerBranch(const P& p, uint32_t i, uint32_t j) {  // FP64 (cpu_scheduling.slang:10-15)
     float4 c = p.cell[0], pj = p.position[j], pi = p.position[i];
     float4 rj = p.positionRemainder[j], ri = p.positionRemainder[i];
@@ -162,11 +173,9 @@ __device__ v3 centerBranch(const P& p, uint32_t i, uint32_t j) {  // FP64 (cpu_s
     double dx = (double(pj.x) - double(pi.x)) + (double(rj.x) - double(ri.x));
     double dy = (double(pj.y) - double(pi.y)) + (double(rj.y) - double(ri.y));
     double dz = (double(pj.z) - double(pi.z)) + (double(rj.z) - double(ri.z));
-    if (p.c.periodic != 0) {
-        dx = dx - floor(dx / bx + 0.5) * bx;
-        dy = dy - floor(dy / by + 0.5) * by;
-        dz = dz - floor(dz / bz + 0.5) * bz;
-    }
+    if (periodicAxis(p.c, 0)) dx = dx - floor(dx / bx + 0.5) * bx;
+    if (periodicAxis(p.c, 1)) dy = dy - floor(dy / by + 0.5) * by;
+    if (periodicAxis(p.c, 2)) dz = dz - floor(dz / bz + 0.5) * bz;
     return {float(dx), float(dy), float(dz)};
 }
 __device__ v3 omega(const P& p, uint32_t i) {
@@ -572,10 +581,16 @@ __global__ void k_begin_step(P p) {
     v3 x = xyz(p.position[id]), xl = xyz(p.positionRemainder[id]);
     compensated3(x, xl, expm1Stable3(rate * dt) * (x + xl) + dt * (v + vl));
     if (!finite3(x) || !finite3(xl)) { fail(p, 512); return; }
-    if (p.c.periodic != 0) x = x - floor3(x / box) * box;
+    if (p.c.periodic == 1) x = x - floor3(x / box) * box;
     else {
+        if (anyPeriodic(p.c)) {
+            v3 wrapped = x - floor3(x / box) * box;
+            x = {periodicAxis(p.c, 0) ? wrapped.x : x.x, periodicAxis(p.c, 1) ? wrapped.y : x.y,
+                 periodicAxis(p.c, 2) ? wrapped.z : x.z};
+        }
         v3 s = x + xl, t = (x - box) + xl;
-        if (s.x < 0 || s.y < 0 || s.z < 0 || t.x >= 0 || t.y >= 0 || t.z >= 0) { fail(p, 2048); return; }
+        for (int a = 0; a < 3; a++)
+            if (!periodicAxis(p.c, a) && (axisOf(s, a) < 0 || axisOf(t, a) >= 0)) { fail(p, 2048); return; }
     }
     p.position[id] = f4(x, 0);
     p.positionRemainder[id] = f4(xl, 0);
@@ -592,7 +607,10 @@ __global__ void k_advance_cell(P p) {
     v3 box = xyz(p.cell[2]) * mk3(dem_expf(logarithm.x), dem_expf(logarithm.y), dem_expf(logarithm.z));
     p.cell[0] = f4(box, 0);
     p.control[4] = 0;
-    if (p.c.periodic != 0 && p.referenceCell[1].w >= 0.5f * fminf(box.x, fminf(box.y, box.z))) fail(p, 256);
+    if (p.c.periodic == 1 && p.referenceCell[1].w >= 0.5f * fminf(box.x, fminf(box.y, box.z))) fail(p, 256);
+    if (p.c.periodic >= 8u)
+        for (int a = 0; a < 3; a++)
+            if (periodicAxis(p.c, a) && p.referenceCell[1].w >= 0.5f * axisOf(box, a)) fail(p, 256);
 }
 // advanceWalls (solver.slang:669-687): one thread; each plane offset moves by
 // dt (n . v) in compensated form, then the optional box of six walls must still
@@ -627,7 +645,10 @@ __global__ void k_check_rebuild(P p) {
     v3 d = (xyz(p.position[id]) - xyz(p.referencePosition[id])) +
            (xyz(p.positionRemainder[id]) - xyz(p.referencePositionRemainder[id]));
     v3 box = xyz(p.cell[0]);
-    if (p.c.periodic != 0) d = d - floor3(d / box + mk3(0.5f, 0.5f, 0.5f)) * box;
+    if (anyPeriodic(p.c)) {
+        v3 w = d - floor3(d / box + mk3(0.5f, 0.5f, 0.5f)) * box;
+        d = {periodicAxis(p.c, 0) ? w.x : d.x, periodicAxis(p.c, 1) ? w.y : d.y, periodicAxis(p.c, 2) ? w.z : d.z};
+    }
     float deformation = length(box - xyz(p.referenceCell[0]));
     if (length(d) + deformation > 0.5f * p.c.skin) atomicOr(p.control + 4, 1u);
 }
@@ -699,9 +720,10 @@ __global__ void k_fill_bins(P p) {
     if (id >= p.c.N || p.control[4] == 0) return;
     v3 x = xyz(p.position[id]), xl = xyz(p.positionRemainder[id]);
     if (!finite3(x) || !finite3(xl)) { fail(p, 512); return; }
-    if (p.c.periodic == 0) {
+    if (p.c.periodic != 1) {
         v3 s = x + xl, t = (x - xyz(p.cell[0])) + xl;
-        if (s.x < 0 || s.y < 0 || s.z < 0 || t.x >= 0 || t.y >= 0 || t.z >= 0) { fail(p, 2048); return; }
+        for (int a = 0; a < 3; a++)
+            if (!periodicAxis(p.c, a) && (axisOf(s, a) < 0 || axisOf(t, a) >= 0)) { fail(p, 2048); return; }
     }
     uint32_t b[3];
     binIndex(p, id, b);
@@ -733,7 +755,7 @@ __device__ uint32_t findPairs(const P& p, uint32_t id, bool emit) {
     for (int a = 0; a < 3; a++) {
         low[a] = max(int(b[a]) - reach[a], 0);
         int high = min(int(b[a]) + reach[a], int(p.c.bins[a]) - 1);
-        width[a] = p.c.periodic != 0 ? int(min(uint32_t(2 * reach[a] + 1), p.c.bins[a])) : high - low[a] + 1;
+        width[a] = periodicAxis(p.c, a) ? int(min(uint32_t(2 * reach[a] + 1), p.c.bins[a])) : high - low[a] + 1;
     }
     float4 bi = p.body[id];
     uint32_t count = 0;
@@ -742,13 +764,10 @@ __device__ uint32_t findPairs(const P& p, uint32_t id, bool emit) {
         for (int y = 0; y < width[1]; y++)
             for (int z = 0; z < width[2]; z++) {
                 uint32_t c[3];
-                if (p.c.periodic != 0) {
-                    c[0] = modIndex(int(b[0]) - reach[0] + x, p.c.bins[0]);
-                    c[1] = modIndex(int(b[1]) - reach[1] + y, p.c.bins[1]);
-                    c[2] = modIndex(int(b[2]) - reach[2] + z, p.c.bins[2]);
-                } else {
-                    c[0] = uint32_t(low[0] + x); c[1] = uint32_t(low[1] + y); c[2] = uint32_t(low[2] + z);
-                }
+                const int at[3] = {x, y, z};
+                for (int a = 0; a < 3; a++)
+                    c[a] = periodicAxis(p.c, a) ? modIndex(int(b[a]) - reach[a] + at[a], p.c.bins[a])
+                                                : uint32_t(low[a] + at[a]);
                 uint32_t flat = binFlat(p, c);
                 uint32_t occupancy = min(p.binCounts[flat], p.c.binCapacity);
                 const uint32_t* slots = p.binParticles + size_t(flat) * p.c.binCapacity;

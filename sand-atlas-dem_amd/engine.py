@@ -18,7 +18,7 @@ from dataclasses import replace
 import numpy as np
 
 from .geometry import pack_meshes, sample_arrays
-from .spec import box_extents, finite_array
+from .spec import box_extents, periodic_axes, periodic_code, finite_array
 
 ERRORS = {  # gpu.py:11-26
     2: "degenerate contact distance gradient",
@@ -107,7 +107,8 @@ class Engine:
         cell = finite_array(cell, (3,), positive=True)
         if positions.ndim != 2 or positions.shape[1] != 3 or len(positions) < 1:
             raise ValueError("At least one three-dimensional position required")
-        if not spec.periodic and (np.any(positions < 0) or np.any(positions >= cell)):
+        open_axes = ~periodic_axes(spec.periodic)
+        if np.any((positions < 0)[:, open_axes]) or np.any((positions >= cell)[:, open_axes]):
             raise ValueError("Nonperiodic particle centres must lie in [0, cell)")
         self.m = backend
         self.spec = spec
@@ -153,7 +154,8 @@ class Engine:
         self.skin = float(spec.skin * 2 * self.body[:, 2].max())
         self.cutoff = float(2 * self.body[:, 2].max() + self.skin)
         box = np.asarray(cell, float) / l
-        if spec.periodic and self.cutoff >= 0.5 * box.min():
+        axes = periodic_axes(spec.periodic)
+        if axes.any() and self.cutoff >= 0.5 * box[axes].min():
             raise ValueError("Cell too small for unique periodic images")
         self.bins = np.maximum(1, np.minimum(64, np.floor(box / self.cutoff))).astype(np.uint32)
         self.bin_count = int(self.bins.prod())
@@ -353,7 +355,7 @@ class Engine:
             "historyStep": int(self.history),
             "meshGeometry": 1,
             "sourceSurfaceNormal": 0,
-            "periodic": int(spec.periodic),
+            "periodic": periodic_code(spec.periodic),
             "wallCount": self.wall_count,
             "wallBoxEnabled": int(self.spec.boundary_box is not None),
         }
@@ -417,7 +419,7 @@ class Engine:
         if type(steps) is not int or steps < 0:
             raise ValueError("Nonnegative integer step count required")
         rate = finite_array(rate, (3,))
-        if not self.spec.periodic and np.any(rate != 0):
+        if np.any(rate[~periodic_axes(self.spec.periodic)] != 0):
             raise ValueError("Affine cell rates require a periodic domain")
         if not np.isfinite(drag) or drag < 0:
             raise ValueError("Finite nonnegative drag required")

@@ -39,9 +39,35 @@ def box_extents(offsets, indices, domain=None):
     return high - low
 
 
+def periodic_axes(periodic):
+    """Per-axis periodicity: a bool (the reference's, every axis) or three bools."""
+    if type(periodic) is bool:
+        return np.array([periodic] * 3)
+    if isinstance(periodic, (list, tuple)) and len(periodic) == 3 and all(type(v) is bool for v in periodic):
+        return np.array(periodic)
+    raise ValueError("periodic must be an explicit boolean or three booleans (x, y, z)")
+
+
+def periodic_code(periodic):
+    """Config `periodic`: 0 none, 1 every axis (the reference's values), 8 | mask for
+    a subset (x = 1, y = 2, z = 4)."""
+    axes = periodic_axes(periodic)
+    if axes.all():
+        return 1
+    if not axes.any():
+        return 0
+    return 8 | int(axes[0]) | int(axes[1]) << 1 | int(axes[2]) << 2
+
+
 def validate_planes(planes, periodic, boundary_box):
-    """Fixed-normal translating penalty planes (reference `boundaries.py:20-59`)."""
-    if not isinstance(planes, list) or (planes and periodic):
+    """Fixed-normal translating penalty planes (reference `boundaries.py:20-59`).
+
+    The reference forbids planes in a periodic domain.  With per-axis periodicity
+    (no reference counterpart) a plane may bound the nonperiodic axes: its normal
+    must have no component along a periodic axis.
+    """
+    axes = periodic_axes(periodic)
+    if not isinstance(planes, list) or (planes and axes.all()):
         raise ValueError("Planes require a nonperiodic domain and a list")
     for plane in planes:
         if not isinstance(plane, dict) or not PLANE_FIELDS <= set(plane) or set(plane) - PLANE_FIELDS - {"normal_viscosity"}:
@@ -49,6 +75,8 @@ def validate_planes(planes, periodic, boundary_box):
         normal = finite_array(plane["normal"], (3,))
         if abs(np.linalg.norm(normal) - 1) > 1e-10:
             raise ValueError("Plane normal must have unit length; automatic normalization is forbidden")
+        if np.any(np.asarray(normal)[axes] != 0):
+            raise ValueError("A plane normal must not point along a periodic axis")
         finite_array(plane["velocity"], (3,))
         finite_array(plane["offset"], ())
         if finite_array(plane.get("normal_viscosity", 0), ()) < 0:
@@ -58,6 +86,8 @@ def validate_planes(planes, periodic, boundary_box):
             if value < 0 or (name != "friction" and value == 0):
                 raise ValueError("Plane stiffness must be positive and friction nonnegative")
     if boundary_box is not None:
+        if axes.any():
+            raise ValueError("boundary_box requires a nonperiodic domain")
         if (
             not isinstance(boundary_box, list)
             or len(boundary_box) != 6
@@ -114,8 +144,9 @@ class Experiment:
             raise ValueError("Unsupported timestep policy")
         if self.shape_sampling not in {"random", "balanced"}:
             raise ValueError("shape_sampling must be random or balanced")
-        if type(self.periodic) is not bool:
-            raise ValueError("periodic must be an explicit boolean")
+        periodic_axes(self.periodic)
+        if isinstance(self.periodic, list):
+            self.periodic = tuple(self.periodic)
         validate_planes(self.planes, self.periodic, self.boundary_box)
         for name in (
             "particles", "seed", "contacts_per_particle", "pairs_per_particle",

@@ -32,7 +32,7 @@ def _same(a, b):
         assert np.array_equal(a[name], b[name], equal_nan=True), name
 
 
-@pytest.mark.parametrize("config", ["dense", "uneven", "walls"])
+@pytest.mark.parametrize("config", ["dense", "uneven", "walls", "layer"])
 def test_skip_leaves_results_unchanged(config):
     from sand_atlas_dem_amd import assembly
     from sand_atlas_dem_amd.hip import HipBackend
