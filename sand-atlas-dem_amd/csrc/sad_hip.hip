@@ -1221,7 +1221,15 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
                     for (uint32_t c = 0; c < count; c++) {
                         uint32_t triangle = first + c, feature;
                         const float4* t = p.meshTriangles + 3 * size_t(triangle);
-                        v3 x = closestTriangle(q, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
+                        const v3 a = xyz(t[0]);
+                        // A triangle whose plane is farther than the current bound
+                        // cannot reach the minimum (the distance to a triangle is at
+                        // least the distance to its plane; 1e-6 absolute and 1e-4
+                        // relative cover the FP32 normal and both evaluations), so
+                        // skipping it cannot change which triangle wins.
+                        const float h = fabsf(dot(q - a, xyz(p.meshNormals[7 * size_t(triangle)]))) - 1e-6f;
+                        if (h > 0.0f && h * h > 1.0001f * fminf(best, prune)) continue;
+                        v3 x = closestTriangle(q, a, xyz(t[1]), xyz(t[2]), feature);
                         float distance = dot(q - x, q - x);
                         if (distance < best) { best = distance; bestTri = triangle; }
                     }
