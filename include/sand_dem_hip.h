@@ -131,6 +131,18 @@ int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buf
 int sad_hip_scan_kernel(const char* kernel, uint32_t n, const uint32_t* input,
                         uint32_t* output, uint32_t* aux, void* stream);
 
+/* The diagnostics reductions (diagnostic.slang, driven by Engine.diagnostics at
+ * gpu.py:746-792): one total per group of 256 records, summed by the pairwise
+ * halving tree 128 -> 1 of that shader.  "reduceRows": 8 float4 rows per record,
+ * row 4 .w takes the maximum; "reduceFabric": 2 rows; "reduceWallRows": 3 rows
+ * per (wall, record) at input[3 * (wall * n + id) + row], output[3 * (wall *
+ * output_count + group) + row], row 0 .w an integer count carried as FP32 bits
+ * (walls and output_count are ignored by the other two).  Byte sizes are checked
+ * against what the launch touches. */
+int sad_hip_reduce_kernel(const char* kernel, uint32_t n, uint32_t walls, uint32_t output_count,
+                          const void* input, uint64_t input_bytes, void* output,
+                          uint64_t output_bytes, void* stream);
+
 /* Tunables of the collection stage (no reference counterpart; none changes any
  * result): "collect_blocks_per_cu" (1..32 workgroups per CU in the persistent
  * launch), "collect_pop" (1..16 tasks taken per queue atomic), "collect_queue"

@@ -100,6 +100,10 @@ inline real sin32(real x) { return real(std::sin(double(x))); }
 inline real cos32(real x) { return real(std::cos(double(x))); }
 inline real exp32(real x) { return real(std::exp(double(x))); }
 
+inline float3 normalize3(float3 a) {
+    real s = real(1.0) / std::sqrt(dot(a, a));
+    return a * s;
+}
 inline float4 normalize4(float4 a) {
     real s = real(1.0) / std::sqrt(dot4(a, a));
     return a * s;
@@ -147,7 +151,7 @@ struct R {
     uint4* oldMeta;
     float4 *oldSlip, *oldNormal;
     uint32_t *oldHash, *hash, *heads, *next;
-    float4 *particleStress, *timestepRows, *particleDiagnostic, *accumulated;
+    float4 *particleStress, *timestepRows, *particleDiagnostic, *accumulated, *diagnosticRows, *fabricRows;
     float4 *wallPlane, *wallRemainder, *wallVelocity, *wallMaterial, *wallRows, *wallAccumulated;
     uint32_t* wallBoxIds;
     uint64_t metaCount;  // capacity of the meta/pointDepth/normalArea views
@@ -249,6 +253,8 @@ R bind(void** b, const uint64_t* bytes, const double* v) {
     r.particleStress = at<float4>(b, SAD_PARTICLE_STRESS);
     r.timestepRows = at<float4>(b, SAD_TIMESTEP_ROWS);
     r.particleDiagnostic = at<float4>(b, SAD_PARTICLE_DIAGNOSTIC);
+    r.diagnosticRows = at<float4>(b, SAD_DIAGNOSTIC_ROWS);
+    r.fabricRows = at<float4>(b, SAD_FABRIC_ROWS);
     r.accumulated = at<float4>(b, SAD_ACCUMULATED);
     r.wallPlane = at<float4>(b, SAD_WALL_PLANE);
     r.wallRemainder = at<float4>(b, SAD_WALL_REMAINDER);
@@ -1193,6 +1199,35 @@ int collectParallel(R& r, uint32_t count, uint32_t threads) {
     return 0;
 }
 
+// ---------------------------------------------------------------- evidence rows (solver.slang:922-947)
+void diagnosticRows(const R& r, uint32_t id) {  // :922-936
+    if (id >= r.c.N) return;
+    float3 v = xyz(r.velocity[id]);
+    real m = r.body[id].y;
+    real kinetic = real(0.5) * (m * dot(v, v) + dot(xyz(r.momentum[id]), omega(r, id)));
+    r.diagnosticRows[id * 8] = f4(xyz(r.particleStress[id * 3]), kinetic);
+    r.diagnosticRows[id * 8 + 1] = f4(xyz(r.particleStress[id * 3 + 1]), r.particleDiagnostic[id].y);
+    r.diagnosticRows[id * 8 + 2] = f4(xyz(r.particleStress[id * 3 + 2]), length(xyz(r.force[id])));
+    r.diagnosticRows[id * 8 + 3] = f4(m * v.x * v, r.particleDiagnostic[id].w);
+    r.diagnosticRows[id * 8 + 4] = f4(m * v.y * v, r.force[id].w);
+    r.diagnosticRows[id * 8 + 5] = f4(m * v.z * v, r.torque[id].w);
+    r.diagnosticRows[id * 8 + 6] = r.accumulated[id];
+    float4 normalState = r.normalDamping[id];
+    r.diagnosticRows[id * 8 + 7] = {normalState.x + normalState.y, normalState.z, 0, 0};
+}
+void fabricRows(const R& r, uint32_t id) {  // :938-947
+    if (id >= r.c.pairCapacity) return;
+    float4 a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
+    if (id < r.control[1] && r.contactCounts[id] > 0) {
+        uint2 ij = r.pairs[id];
+        float3 n = normalize3(centerBranch(r, ij.x, ij.y));
+        a = f4(n * n, n.x * n.y);
+        b = {n.x * n.z, n.y * n.z, 1, 0};
+    }
+    r.fabricRows[2 * id] = a;
+    r.fabricRows[2 * id + 1] = b;
+}
+
 }  // namespace
 
 extern "C" int oracle_dispatch(const char* kernel, uint32_t count, void** buffers, const uint64_t* bytes,
@@ -1225,6 +1260,8 @@ extern "C" int oracle_dispatch(const char* kernel, uint32_t count, void** buffer
         else if (name == "finishStep") each(finishStep);
         else if (name == "captureContactKinematics") each(captureContactKinematics);
         else if (name == "checkContactTimestep") each(checkContactTimestep);
+        else if (name == "diagnosticRows") each(diagnosticRows);
+        else if (name == "fabricRows") each(fabricRows);
         else if (name == "advanceWalls") advanceWalls(r);
         else if (name == "collectWallContacts") { for (uint32_t id = 0; id < wallTasks; id++) collectWallContacts(r, id); }
         else if (name == "gatherWallRows") { for (uint32_t id = 0; id < wallTasks; id++) gatherWallRows(r, id); }

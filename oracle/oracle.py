@@ -126,6 +126,43 @@ class Encoder:
         self.operations.append(copy)
 
 
+REDUCTION_ROWS = {"reduceRows": 8, "reduceFabric": 2, "reduceWallRows": 3}
+
+
+def reduce_groups(name, records, walls, n, dtype):
+    """Group totals of evidence rows in the shader's tree order (diagnostic.slang;
+    the reference CPU adapter restates it at cpu.py:491-532).
+
+    `records` holds walls x n records of `rows` float4 rows.  Each group of 256
+    records (zero padded) is halved pairwise, 128 -> 1: lane i adds lane i + step.
+    reduceRows keeps the maximum in row 4 .w; reduceWallRows carries row 0 .w as an
+    unsigned count (FP32 bits) summed with wrap-around.
+    """
+    rows = REDUCTION_ROWS[name]
+    groups = (n + 255) // 256
+    values = np.zeros((walls, groups * 256, rows, 4), dtype)
+    values[:, :n] = records[: walls * n * rows * 4].reshape(walls, n, rows, 4)
+    values = values.reshape(walls, groups, 256, rows, 4)
+    counts = None
+    if name == "reduceWallRows":
+        counts = values[..., 0, 3].astype(np.float32).view(np.uint32).copy()
+        values[..., 0, 3] = 0
+    step = 128
+    while step:
+        low, high = values[:, :, :step], values[:, :, step : 2 * step]
+        maximum = np.maximum(low[..., 4, 3], high[..., 4, 3]) if name == "reduceRows" else None
+        low += high
+        if maximum is not None:
+            low[..., 4, 3] = maximum
+        if counts is not None:
+            counts[:, :, :step] += counts[:, :, step : 2 * step]
+        step //= 2
+    totals = values[:, :, 0].copy()
+    if counts is not None:
+        totals[..., 0, 3] = counts[:, :, 0].view(np.float32)
+    return totals
+
+
 def pointer_table(bindings, order, wide=False):
     if wide:
         arrays = []
@@ -215,6 +252,41 @@ def make_oracle(package, threads=1, collector="oracle", precision="fp32"):
                 execute()
             else:
                 encoder.operations.append(execute)
+
+        def kernel(self, name, module="solver.slang"):
+            if module != "diagnostic.slang" or name not in REDUCTION_ROWS:
+                raise NotImplementedError(f"oracle kernel {module}:{name}")
+            oracle = self
+
+            class Reduction:
+                def dispatch(self, *, thread_count, command_encoder=None, **kwargs):
+                    n = int(kwargs["N"])
+                    walls = int(thread_count[1]) if name == "reduceWallRows" else 1
+                    groups = (n + 255) // 256
+                    output_count = int(kwargs.get("outputCount", groups))
+                    src, dst = kwargs["input"], kwargs["output"]
+
+                    def execute():
+                        rows = REDUCTION_ROWS[name]
+                        if wide:
+                            records, out, dtype = src.widen(), dst.widen(), np.float64
+                        else:
+                            records, out, dtype = src.array.view(np.float32), dst.array.view(np.float32), np.float32
+                        if n == 0:
+                            return
+                        if records.size < walls * n * rows * 4 or out.size < walls * output_count * rows * 4:
+                            raise ValueError(f"{name}: buffers too small")
+                        totals = reduce_groups(name, records, walls, n, dtype)
+                        target = out[: walls * output_count * rows * 4].reshape(walls, output_count, rows, 4)
+                        target[:, :groups] = totals
+                        oracle.calls += 1
+
+                    if command_encoder is None:
+                        execute()
+                    else:
+                        command_encoder.operations.append(execute)
+
+            return Reduction()
 
         def probe(self, name, uniforms, bindings, data, out):
             pointers, sizes = pointer_table(bindings, order)

@@ -58,6 +58,7 @@ DEM_D v3 floor3(v3 a) { return {floorf(a.x), floorf(a.y), floorf(a.z)}; }
 DEM_D bool finite3(v3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
 DEM_D float maxabs(v3 a) { return fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(a.z)); }
 DEM_D v4 normalize4(v4 a) { float s = 1.0f / sqrtf(dot4(a, a)); return a * s; }
+DEM_D v3 normalize3(v3 a) { float s = 1.0f / sqrtf(dot(a, a)); return a * s; }
 
 constexpr float DEM_EPS = 1.1920928955078125e-7f;
 constexpr uint32_t DEM_EMPTY = 0xffffffffu;

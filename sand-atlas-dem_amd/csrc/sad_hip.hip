@@ -72,6 +72,10 @@ struct P {
     const uint32_t* oldHash;
     uint32_t* hash;
     float4 *particleStress, *timestepRows, *particleDiagnostic, *accumulated;
+    // Evidence rows for the diagnostics reductions (8 per grain, 2 per pair slot);
+    // optional bindings, checked when their kernels run.
+    float4 *diagnosticRows, *fabricRows;
+    uint64_t diagnosticBytes, fabricBytes;
     // Translating planes (c.wallCount rows; per-(wall, grain) rows at wall * N + grain).
     float4 *wallPlane, *wallRemainder, *wallRows, *wallAccumulated;
     const float4 *wallVelocity, *wallMaterial;
@@ -2089,6 +2093,78 @@ __global__ void k_refresh_wall_power(P p) {
     p.wallRows[3 * index + 1] = value;
 }
 
+// ---------------------------------------------------------------- evidence rows (solver.slang:922-947)
+__global__ void k_diagnostic_rows(P p) {  // :922-936
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.N) return;
+    v3 v = xyz(p.velocity[id]);
+    float m = p.body[id].y;
+    float kinetic = 0.5f * (m * dot(v, v) + dot(xyz(p.momentum[id]), omega(p, id)));
+    float4* row = p.diagnosticRows + 8 * size_t(id);
+    row[0] = f4(xyz(p.particleStress[3 * id]), kinetic);
+    row[1] = f4(xyz(p.particleStress[3 * id + 1]), p.particleDiagnostic[id].y);
+    row[2] = f4(xyz(p.particleStress[3 * id + 2]), length(xyz(p.force[id])));
+    row[3] = f4(m * v.x * v, p.particleDiagnostic[id].w);
+    row[4] = f4(m * v.y * v, p.force[id].w);
+    row[5] = f4(m * v.z * v, p.torque[id].w);
+    row[6] = p.accumulated[id];
+    float4 normalState = p.normalDamping[id];
+    row[7] = make_float4(normalState.x + normalState.y, normalState.z, 0, 0);
+}
+__global__ void k_fabric_rows(P p) {  // :938-947
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= p.c.pairCapacity) return;
+    float4 a = make_float4(0, 0, 0, 0), b = a;
+    if (id < p.control[1] && p.contactCounts[id] > 0) {
+        uint2 ij = p.pairs[id];
+        v3 n = normalize3(centerBranch(p, ij.x, ij.y));
+        a = f4(n * n, n.x * n.y);
+        b = make_float4(n.x * n.z, n.y * n.z, 1, 0);
+    }
+    p.fabricRows[2 * size_t(id)] = a;
+    p.fabricRows[2 * size_t(id) + 1] = b;
+}
+
+// Group totals of evidence rows (diagnostic.slang).  One 256-lane group per 256
+// records; the LDS tree halves 128 -> 1 pairwise in exactly that order, so the FP32
+// totals are reproducible.  MODE 0: 8 rows per grain, row 4 .w keeps the maximum;
+// MODE 1: 2 rows per pair slot; MODE 2: 3 rows per (wall, grain) with blockIdx.y the
+// wall, row 0 .w an integer count carried as FP32 bits.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_reduce_rows(uint32_t n, uint32_t outputCount, const float4* in, float4* out) {
+    constexpr int ROWS = MODE == 0 ? 8 : (MODE == 1 ? 2 : 3);
+    __shared__ float4 values[256];
+    __shared__ uint32_t counts[256];
+    const uint32_t lane = threadIdx.x, id = blockIdx.x * 256 + lane;
+    const size_t base = size_t(blockIdx.y) * n;
+    for (int row = 0; row < ROWS; row++) {
+        float4 value = make_float4(0, 0, 0, 0);
+        if (id < n) value = in[ROWS * (base + id) + row];
+        if (MODE == 2 && row == 0) {
+            counts[lane] = __float_as_uint(value.w);
+            value.w = 0;
+        }
+        values[lane] = value;
+        __syncthreads();
+        for (uint32_t step = 128; step > 0; step >>= 1) {
+            if (lane < step) {
+                float4 a = values[lane], b = values[lane + step];
+                float4 sum = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+                if (MODE == 0 && row == 4) sum.w = (a.w > b.w || a.w != a.w) ? a.w : b.w;
+                if (MODE == 2 && row == 0) counts[lane] += counts[lane + step];
+                values[lane] = sum;
+            }
+            __syncthreads();
+        }
+        if (lane == 0) {
+            float4 total = values[0];
+            if (MODE == 2 && row == 0) total.w = __uint_as_float(counts[0]);
+            out[ROWS * (size_t(blockIdx.y) * outputCount + blockIdx.x) + row] = total;
+        }
+        __syncthreads();
+    }
+}
+
 // Per template: the largest sample-node count (sizes the temporal rows), the first
 // triangle and triangle count of its BVH (leftmost and rightmost leaves of the
 // preorder tree; an internal node's children are n + 1 and that child's escape
@@ -2266,7 +2342,8 @@ thread_local char g_error[256] = "no error";
     X(adj_sort) X(reset_mesh) X(prepare_pairs) X(collect) X(contact_total) X(scatter) X(clear_storage) X(resolve) \
     X(build_hash) X(gather) X(finish_step) X(capture_kinematics) X(check_timestep) \
     X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_offsets) X(bucket_fill) \
-    X(advance_walls) X(wall_count) X(wall_emit) X(wall_total) X(gather_wall_rows) X(refresh_wall_power)
+    X(advance_walls) X(wall_count) X(wall_emit) X(wall_total) X(gather_wall_rows) X(refresh_wall_power) \
+    X(diagnostic_rows) X(fabric_rows)
 enum KernelId {
 #define SAD_ID(name) KID_##name,
     SAD_KERNELS(SAD_ID)
@@ -2580,6 +2657,17 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
         {SAD_BIN_PARTICLES, 4ull * c.binCount * c.binCapacity, "binParticles"},
         {SAD_PARTICLE_STRESS, 48ull * c.N, "particleStress"}, {SAD_MESH_PAIR_MASK, 4ull * c.pairCapacity, "meshPairMask"},
         {SAD_MESH_PAIR_TRANSFORM, 128ull * c.pairCapacity, "meshPairTransform"},
+        {SAD_FORCE, 16ull * c.N, "force"}, {SAD_TORQUE, 16ull * c.N, "torque"},
+        {SAD_ACCUMULATED, 16ull * c.N, "accumulated"}, {SAD_ACCUMULATED_REMAINDER, 16ull * c.N, "accumulatedRemainder"},
+        {SAD_PARTICLE_DIAGNOSTIC, 16ull * c.N, "particleDiagnostic"}, {SAD_TIMESTEP_ROWS, 16ull * c.N, "timestepRows"},
+        {SAD_VELOCITY_REMAINDER, 16ull * c.N, "velocityRemainder"},
+        {SAD_MOMENTUM_REMAINDER, 16ull * c.N, "momentumRemainder"},
+        {SAD_ROTATION_REMAINDER, 16ull * c.N, "rotationRemainder"},
+        {SAD_REFERENCE_POSITION, 16ull * c.N, "referencePosition"},
+        {SAD_REFERENCE_POSITION_REMAINDER, 16ull * c.N, "referencePositionRemainder"},
+        {SAD_NORMAL_DAMPING, 16ull * c.N, "normalDamping"}, {SAD_FORCE_VELOCITY, 16ull * c.N, "forceVelocity"},
+        {SAD_FORCE_OMEGA, 16ull * c.N, "forceOmega"}, {SAD_CELL_LOG, 32, "cellLog"},
+        {SAD_REFERENCE_CELL, 32, "referenceCell"},
     };
     for (const Need& n : needs) {
         if (!b[n.slot] || bytes[n.slot] < n.bytes) {
@@ -2652,6 +2740,10 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
     p.timestepRows = slot<float4>(b, SAD_TIMESTEP_ROWS);
     p.particleDiagnostic = slot<float4>(b, SAD_PARTICLE_DIAGNOSTIC);
     p.accumulated = slot<float4>(b, SAD_ACCUMULATED);
+    p.diagnosticRows = slot<float4>(b, SAD_DIAGNOSTIC_ROWS);
+    p.diagnosticBytes = b[SAD_DIAGNOSTIC_ROWS] ? bytes[SAD_DIAGNOSTIC_ROWS] : 0;
+    p.fabricRows = slot<float4>(b, SAD_FABRIC_ROWS);
+    p.fabricBytes = b[SAD_FABRIC_ROWS] ? bytes[SAD_FABRIC_ROWS] : 0;
     if (c.wallCount != 0) {
         const uint64_t planes = 16ull * c.wallCount, tasks = uint64_t(c.wallCount) * c.N;
         const Need walls[] = {
@@ -2789,11 +2881,23 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         if (c.wallCount) LAUNCH(sc, KID_gather_wall_rows, k_gather_wall_rows, dim3(blocks(c.wallCount * c.N)), T, s, p, w);
     } else if (name == "refreshWallPower") {
         if (c.wallCount) LAUNCH(sc, KID_refresh_wall_power, k_refresh_wall_power, dim3(blocks(c.wallCount * c.N)), T, s, p);
+    } else if (name == "diagnosticRows") {
+        if (!p.diagnosticRows || p.diagnosticBytes < 128ull * c.N) {
+            std::snprintf(g_error, sizeof(g_error), "buffer diagnosticRows too small for %u grains", c.N);
+            return SAD_E_ARGUMENT;
+        }
+        LAUNCH(sc, KID_diagnostic_rows, k_diagnostic_rows, perGrain, T, s, p);
+    } else if (name == "fabricRows") {
+        if (!p.fabricRows || p.fabricBytes < 32ull * c.pairCapacity) {
+            std::snprintf(g_error, sizeof(g_error), "buffer fabricRows too small for %u pair slots", c.pairCapacity);
+            return SAD_E_ARGUMENT;
+        }
+        LAUNCH(sc, KID_fabric_rows, k_fabric_rows, dim3(blocks(c.pairCapacity)), T, s, p);
     } else {
-        // Reference entry points outside the mesh path (SDF grid, diagnostics,
-        // the serial verification collector) are known but not provided here.
+        // Reference entry points outside the mesh path (SDF grid, the serial
+        // verification collector) are known but not provided here.
         static const char* unsupported[] = {"countContacts", "emitContacts", "contactTotal", "collectMeshContacts",
-                                            "diagnosticRows", "fabricRows", "smoke"};
+                                            "smoke"};
         for (const char* u : unsupported)
             if (name == u) {
                 std::snprintf(g_error, sizeof(g_error), "kernel %s is not part of the hip mesh path", u);
@@ -3186,6 +3290,37 @@ int sad_hip_scan_kernel(const char* kernel, uint32_t n, const uint32_t* input, u
         std::snprintf(g_error, sizeof(g_error), "unknown scan kernel %s", kernel);
         return SAD_E_ARGUMENT;
     }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAD_OK : hip_fail(e, kernel);
+}
+
+int sad_hip_reduce_kernel(const char* kernel, uint32_t n, uint32_t walls, uint32_t output_count, const void* input,
+                          uint64_t input_bytes, void* output, uint64_t output_bytes, void* stream) {
+    if (!kernel || !input || !output) return SAD_E_ARGUMENT;
+    int mode = -1;
+    uint64_t rows = 0;
+    if (std::strcmp(kernel, "reduceRows") == 0) mode = 0, rows = 8;
+    else if (std::strcmp(kernel, "reduceFabric") == 0) mode = 1, rows = 2;
+    else if (std::strcmp(kernel, "reduceWallRows") == 0) mode = 2, rows = 3;
+    if (mode < 0) {
+        std::snprintf(g_error, sizeof(g_error), "unknown reduction kernel %s", kernel);
+        return SAD_E_ARGUMENT;
+    }
+    if (n == 0) return SAD_OK;
+    const uint32_t groups = blocks(n);
+    if (mode != 2) walls = 1, output_count = groups;
+    if (walls == 0 || walls > 65535 || output_count < groups ||
+        input_bytes < 16 * rows * uint64_t(walls) * n || output_bytes < 16 * rows * uint64_t(walls) * output_count) {
+        std::snprintf(g_error, sizeof(g_error), "%s: %u walls, %u records, %u outputs do not fit the buffers", kernel,
+                      walls, n, output_count);
+        return SAD_E_ARGUMENT;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const float4* in = static_cast<const float4*>(input);
+    float4* out = static_cast<float4*>(output);
+    if (mode == 0) hipLaunchKernelGGL(k_reduce_rows<0>, dim3(groups), dim3(256), 0, s, n, output_count, in, out);
+    else if (mode == 1) hipLaunchKernelGGL(k_reduce_rows<1>, dim3(groups), dim3(256), 0, s, n, output_count, in, out);
+    else hipLaunchKernelGGL(k_reduce_rows<2>, dim3(groups, walls), dim3(256), 0, s, n, output_count, in, out);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SAD_OK : hip_fail(e, kernel);
 }

@@ -234,6 +234,11 @@ class Engine:
         self.allocate("pointDepth", self.contact_capacity, 16)
         self.allocate("contactForce", self.contact_capacity, 16)
         self.allocate("normalLossPower", self.contact_capacity)
+        # Group-total levels of the diagnostics reductions (gpu.py:322-340): each
+        # level holds one total per 256 records of the level below.
+        self.diag_buffers = self._reduction_levels(self.n, 8)
+        self.fabric_buffers = self._reduction_levels(self.pair_capacity, 2)
+        self.wall_diag_buffers = self._reduction_levels(self.n, 3 * self.wall_count) if self.wall_count else []
         names = (
             "position", "velocity", "rotation", "momentum", "cell", "force", "torque",
             "referencePosition", "referenceCell", "control", "pairs", "pairCounts", "pairOffsets",
@@ -246,6 +251,13 @@ class Engine:
             name: self.m.buffer(count=self.hash_size if name == "hash" else 4)
             for name in ("meta", "slip", "normalArea", "hash")
         }
+
+    def _reduction_levels(self, size, rows):
+        levels = []
+        while size > 1:
+            size = (size + 255) // 256
+            levels.append((size, self.m.buffer(count=size * rows, stride=16)))
+        return levels
 
     def _preflight_dt(self):  # gpu.py:370-413
         if self.spec.timestep_policy == "active-contact":
@@ -639,6 +651,113 @@ class Engine:
             self.buffers["velocity"].copy_from_numpy(f4(velocity))
         if momentum is not None:
             self.buffers["momentum"].copy_from_numpy(f4(momentum))
+
+    def diagnostics(self):  # gpu.py:746-891
+        """Whole-assembly evidence of the last completed step in SI units: stress,
+        energies, fabric, coordination and per-plane reactions.  Per-grain and
+        per-pair rows are totalled on the device by the diagnostic.slang tree."""
+        e = self.m.device.create_command_encoder()
+        self.dispatch("diagnosticRows", self.n, e, self.bindings())
+        source = self._reduce(e, "reduceRows", self.buffers["diagnosticRows"], self.n, self.diag_buffers)
+        self.dispatch("fabricRows", self.pair_capacity, e, self.bindings())
+        fabric_source = self._reduce(e, "reduceFabric", self.buffers["fabricRows"], self.pair_capacity, self.fabric_buffers)
+        wall_source = self._reduce(e, "reduceWallRows", self.buffers["wallRows"], self.n, self.wall_diag_buffers)
+        self.m.submit(e)
+        self.m.device.wait()
+        data = self.m.read(source, shape=(-1, 4))[:8]
+        fabric_data = self.m.read(fabric_source, shape=(-1, 4))[:2]
+        active_pairs = round(float(fabric_data[1, 2]))
+        fabric = np.array(
+            [
+                [fabric_data[0, 0], fabric_data[0, 3], fabric_data[1, 0]],
+                [fabric_data[0, 3], fabric_data[0, 1], fabric_data[1, 1]],
+                [fabric_data[1, 0], fabric_data[1, 1], fabric_data[0, 2]],
+            ],
+            float,
+        ) / max(active_pairs, 1)
+        cell = self.m.read(self.buffers["cell"], shape=(-1, 4))[:3]
+        ctrl = self.check_errors()
+        box = cell[0, :3].astype(float) * self.length_unit
+        domain = box.copy()
+        wall_results = []
+        if self.wall_count:
+            wall_data = self.m.read(wall_source, shape=(-1, 4))[: 3 * self.wall_count].reshape(self.wall_count, 3, 4)
+            counts = wall_data[:, 0, 3].copy().view(np.uint32)
+            plane = self.m.read(self.buffers["wallPlane"], shape=(-1, 4))[: self.wall_count].astype(float)
+            remainder = self.m.read(self.buffers["wallRemainder"], shape=(-1, 4))[: self.wall_count].astype(float)
+            wall_velocity = self.m.read(self.buffers["wallVelocity"], shape=(-1, 4))[: self.wall_count].astype(float)
+            offsets = (plane[:, 3] + remainder[:, 3]) * self.length_unit
+            for w in range(self.wall_count):
+                wall_results.append(
+                    dict(
+                        index=w,
+                        normal=plane[w, :3].tolist(),
+                        offset=float(offsets[w]),
+                        velocity=(wall_velocity[w, :3] * self.length_unit / self.time_unit).tolist(),
+                        reaction=(wall_data[w, 0, :3].astype(float) * self.force_unit).tolist(),
+                        reaction_moment_origin=(wall_data[w, 1, :3].astype(float) * self.energy_unit).tolist(),
+                        active_surface_contacts=int(counts[w]),
+                        power=float(wall_data[w, 1, 3] * self.energy_unit / self.time_unit),
+                        work=float(wall_data[w, 2, 0] * self.energy_unit),
+                    )
+                )
+            if self.spec.boundary_box is not None:
+                box = box_extents(offsets, self.spec.boundary_box, domain)
+        volume = float(box.prod())
+        contact = data[:3, :3].astype(float) * self.energy_unit / volume
+        kinetic = data[3:6, :3].astype(float) * self.energy_unit / volume
+        total = contact + kinetic
+        symmetric = 0.5 * (total + total.T)
+        radial = 0.5 * (symmetric[0, 0] + symmetric[1, 1])
+        solid = sum(self.shapes[int(i)].volume * s**3 for i, s in zip(self.ids, self.scales))
+        return {
+            "step": self.step,
+            "elapsed_time": self.step * self.spec.timestep,
+            "cell": box.tolist(),
+            "domain_cell": domain.tolist(),
+            "walls": wall_results,
+            "stress": total.tolist(),
+            "contact_stress": contact.tolist(),
+            "kinetic_stress": kinetic.tolist(),
+            "stress_antisymmetry": float(np.linalg.norm(total - total.T) / max(np.linalg.norm(total), 1e-30)),
+            "p": float(np.trace(symmetric) / 3),
+            "q": float(symmetric[2, 2] - radial),
+            "axial_strain": float(1 - box[2] / self.initial_cell[2]),
+            "volumetric_strain": float(np.sum(1 - box / self.initial_cell)),
+            "volume_ratio": float(volume / np.prod(self.initial_cell)),
+            "log_strain": (-np.log(box / self.initial_cell)).tolist(),
+            "void_ratio": float(volume / solid - 1),
+            "kinetic_energy": float(data[0, 3] * self.energy_unit),
+            "elastic_energy_proxy": float(data[1, 3] * self.energy_unit),
+            "contact_normal": self.spec.contact_normal,
+            "elastic_energy_interpretation": "existing nodal spring energy proxy",
+            "unbalanced_force": float(data[2, 3] / max(data[3, 3], 1e-30)),
+            "max_overlap_fraction": float(data[4, 3]),
+            "friction_dissipation_last_step": float(data[5, 3] * self.energy_unit),
+            "candidate_pairs": int(ctrl[1]),
+            "active_surface_contacts": int(ctrl[2]),
+            "active_grain_pairs": active_pairs,
+            "coordination": 2 * active_pairs / self.n,
+            "fabric": fabric.tolist(),
+            "sliding_fraction": float(ctrl[6] / max(ctrl[2], 1)),
+            "cell_log_rate": (cell[1, :3].astype(float) / self.time_unit).tolist(),
+            "friction_dissipation": float(data[6, 0] * self.energy_unit),
+            "drag_dissipation": float(data[6, 1] * self.energy_unit),
+            "cell_work": float(data[6, 2] * self.energy_unit),
+            "wall_work": float(data[6, 3] * self.energy_unit),
+            "normal_contact_dissipation": float(data[7, 0] * self.energy_unit),
+            "normal_contact_power": float(data[7, 1] * self.energy_unit / self.time_unit),
+        }
+
+    def _reduce(self, e, name, source, n, levels):
+        """Encode the reduction levels; returns the buffer holding the final totals."""
+        kernel = self.m.kernel(name, "diagnostic.slang")
+        for size, dest in levels:
+            extra = {"outputCount": size} if name == "reduceWallRows" else {}
+            walls = self.wall_count if name == "reduceWallRows" else 1
+            kernel.dispatch(thread_count=[size * 256, walls, 1], command_encoder=e, N=n, input=source, output=dest, **extra)
+            source, n = dest, size
+        return source
 
     def walls(self):
         """Per-plane totals of the last force evaluation in SI units (host FP64 sums of

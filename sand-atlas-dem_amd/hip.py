@@ -79,13 +79,21 @@ class Kernel:
     """`m.kernel(name, module)` of the adapter protocol (gpu.py:75-90 Scan, 529-530).
 
     solver.slang kernels go through the library's dispatch table; scan.slang's
-    blockScan/addBlocks through `sad_hip_scan_kernel`.  diagnostic.slang reductions
-    are not part of this backend.
+    blockScan/addBlocks through `sad_hip_scan_kernel`; diagnostic.slang's
+    reduceRows/reduceFabric/reduceWallRows through `sad_hip_reduce_kernel`.
     """
 
+    MODULES = {
+        "solver.slang": None,
+        "scan.slang": {"blockScan", "addBlocks"},
+        "diagnostic.slang": {"reduceRows", "reduceFabric", "reduceWallRows"},
+    }
+
     def __init__(self, backend, name, module):
-        if module not in {"solver.slang", "scan.slang"}:
+        if module not in self.MODULES:
             raise NotImplementedError(f"{module} kernels are not provided by the hip backend")
+        if self.MODULES[module] is not None and name not in self.MODULES[module]:
+            raise NotImplementedError(f"{module} has no kernel {name}")
         self.m, self.name, self.module = backend, name, module
 
     def dispatch(self, *, thread_count, command_encoder=None, vars=None, **kwargs):
@@ -93,6 +101,8 @@ class Kernel:
             r = (vars or {}).get("r", kwargs.get("r"))
             return self.m.dispatch(self.name, int(thread_count[0]), command_encoder, r=r)
         n = int(kwargs["N"])
+        if self.module == "diagnostic.slang":
+            return self._reduce(n, thread_count, command_encoder, kwargs)
         if self.name == "blockScan":
             args = (kwargs["input"], kwargs["output"], kwargs["totals"])
         elif self.name == "addBlocks":
@@ -105,6 +115,24 @@ class Kernel:
             self.m._check(self.m.lib.sad_hip_scan_kernel(
                 self.name.encode(), n, None if src is None else ct.c_void_p(src.ptr), ct.c_void_p(dst.ptr),
                 ct.c_void_p(aux.ptr), self.m.stream), self.name)
+
+        if command_encoder is None:
+            execute()
+        else:
+            command_encoder.operations.append(execute)
+
+    def _reduce(self, n, thread_count, command_encoder, kwargs):
+        walls = int(thread_count[1]) if self.name == "reduceWallRows" else 1
+        groups = (n + 255) // 256
+        output_count = int(kwargs.get("outputCount", groups))
+        if int(thread_count[0]) != 256 * groups:
+            raise ValueError(f"{self.name}: {thread_count[0]} threads for {n} records")
+        src, dst = kwargs["input"], kwargs["output"]
+
+        def execute():
+            self.m._check(self.m.lib.sad_hip_reduce_kernel(
+                self.name.encode(), n, walls, output_count, ct.c_void_p(src.ptr), src.size, ct.c_void_p(dst.ptr),
+                dst.size, self.m.stream), self.name)
 
         if command_encoder is None:
             execute()
