@@ -17,7 +17,8 @@ from dataclasses import replace
 
 import numpy as np
 
-from .geometry import pack_meshes, sample_arrays
+from .geometry import sample_arrays
+from .mesh_cache import packed_meshes
 from .spec import box_extents, periodic_axes, periodic_code, finite_array
 
 ERRORS = {  # gpu.py:11-26
@@ -324,7 +325,7 @@ class Engine:
         self.add("wallRows", np.zeros((max(1, 3 * self.wall_count * self.n), 4), np.float32))
 
     def _static_geometry(self):  # gpu.py:421-488, mesh branch
-        mesh = pack_meshes(self.shapes, self.length_unit)
+        mesh, self.mesh_cache_receipt = packed_meshes(self.shapes, self.length_unit)
         roots = mesh["meshRoots"]
         if np.any(roots[:, 1] - roots[:, 0] > 65535):
             raise ValueError("A template BVH exceeds 65,535 nodes (device traversal stack holds 16-bit offsets)")

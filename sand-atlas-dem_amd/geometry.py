@@ -13,6 +13,8 @@ mass properties are this package's own: the reference obtains them from trimesh
 (`geometry.py:166-259`), which is not a dependency here.
 """
 
+import ctypes as ct
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -376,116 +378,105 @@ def _hull_groups(vertices):
     return [planes[group == g] for g in range(6)]
 
 
-class _Packer:
-    def __init__(self):
-        self.a = {name: [] for name in (
-            "meshFrontier", "meshClusterInfo", "meshClusterRanges", "meshClusterMin",
-            "meshClusterMax", "meshSampleIndices", "meshHullInfo", "meshPlanes",
-            "meshRoots", "meshMin", "meshMax", "meshLinks", "meshTriangles", "meshNormals",
-        )}
-
-    def clusters(self, points):
-        """Preorder median-split sample tree; leaves hold <=64 node keys."""
-        ranges, lo, hi, keys = (self.a[k] for k in (
-            "meshClusterRanges", "meshClusterMin", "meshClusterMax", "meshSampleIndices"))
-        first = len(ranges)
-
-        def split(ids):
-            node = len(ranges)
-            ranges.append([0, 0, 0, 0])
-            lo.append([*_round_out(points[ids].min(0), -np.inf), 0])
-            hi.append([*_round_out(points[ids].max(0), np.inf), 0])
-            if len(ids) > 64:
-                axis = np.ptp(points[ids], axis=0).argmax()
-                order = ids[np.argsort(points[ids, axis], kind="stable")]
-                half = len(order) // 2
-                split(order[:half])
-                split(order[half:])
-            else:
-                ranges[node][0:2] = [len(keys), len(ids)]
-                keys.extend(ids.tolist())
-            ranges[node][2] = len(ranges)
-
-        split(np.arange(len(points)))
-        self.a["meshClusterInfo"].append([first, len(ranges) - first, 0, 0])
-        # Up to 32 disjoint complete subtrees; split the largest internal one first.
-        pending = [first]
-        while len(pending) < 32:
-            internal = [n for n in pending if ranges[n][1] == 0]
-            if not internal:
-                break
-            node = max(internal, key=lambda n: ranges[n][2] - n)
-            at = pending.index(node)
-            pending[at : at + 1] = [node + 1, ranges[node + 1][2]]
-        self.a["meshFrontier"].extend([[n, ranges[n][2], 0, 0] for n in pending])
-        self.a["meshFrontier"].extend([[0, 0, 0, 0]] * (32 - len(pending)))
-
-    def bvh(self, tri, feature):
-        """Median-split triangle BVH, <=4 triangles per leaf, escape links."""
-        links, lo, hi = self.a["meshLinks"], self.a["meshMin"], self.a["meshMax"]
-        root = len(links)
-        first_triangle = len(self.a["meshTriangles"]) // 3
-        order, depths = [], []
-
-        def build(ids, depth):
-            node = len(links)
-            depths.append(depth)
-            lo.append([*_round_out(tri[ids].min(axis=(0, 1)), -np.inf), 0])
-            hi.append([*_round_out(tri[ids].max(axis=(0, 1)), np.inf), 0])
-            links.append([0, 0, 0, 0])
-            if len(ids) <= 4:
-                links[node][1:3] = [first_triangle + len(order), len(ids)]
-                order.extend(ids.tolist())
-            else:
-                centres = tri[ids].mean(axis=1)
-                axis = np.ptp(centres, axis=0).argmax()
-                ids = ids[np.argsort(centres[:, axis], kind="stable")]
-                half = len(ids) // 2
-                build(ids[:half], depth + 1)
-                build(ids[half:], depth + 1)
-            links[node][0] = len(links)
-
-        build(np.arange(len(tri)), 0)
-        if max(depths) > 30:
-            raise ValueError("Mesh hierarchy exceeds GPU traversal stack depth")
-        self.a["meshRoots"].append([root, len(links), max(depths), 0])
-        self.a["meshTriangles"].extend(tri[order].reshape(-1, 3))
-        self.a["meshNormals"].extend(feature[order].reshape(-1, 3))
-
-    def add(self, shape, length_unit):
-        vertices = np.asarray(shape.vertices, float) / length_unit
-        faces = np.asarray(shape.faces, np.int64)
-        self.clusters((np.asarray(shape.samples) / length_unit).astype(F32))
-        for group in _hull_groups(vertices):
-            self.a["meshHullInfo"].append([len(self.a["meshPlanes"]), len(group), 0, 0])
-            self.a["meshPlanes"].extend(group)
-        _, inverse = _check_manifold(faces)
-        feature = _pseudonormals(vertices, faces, inverse)
-        tri = vertices[faces].astype(F32)
-        rounded = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
-        if np.any(np.linalg.norm(rounded, axis=1) == 0):
-            raise ValueError("Triangle degenerates in normalized FP32")
-        self.bvh(tri, feature)
-
-    def arrays(self):
-        out = {}
-        for name, rows in self.a.items():
-            if name in _UINT_ARRAYS:
-                out[name] = np.asarray(rows, np.uint32)
-            elif name in ("meshTriangles", "meshNormals"):
-                a = np.asarray(rows, F32).reshape(-1, 3)
-                out[name] = np.pad(a, ((0, 0), (0, 1)))
-            else:
-                out[name] = np.asarray(rows, F32).reshape(-1, 4) if rows else np.zeros((0, 4), F32)
-        return out
+def _prepare(shape, length_unit):
+    """Per-template inputs of the packer, in the reference's check order
+    (`mesh.py:22-30, 74-160`): FP32 nodes, hull plane groups, manifold check,
+    pseudonormals, FP32 triangles and their vertex means."""
+    if shape.provenance.get("display_mesh_only"):
+        raise ValueError("Mesh query requires physical source triangles, not a display-only mesh")
+    vertices = np.asarray(shape.vertices, float) / length_unit
+    faces = np.asarray(shape.faces, np.int64)
+    points = np.ascontiguousarray((np.asarray(shape.samples) / length_unit).astype(F32))
+    if len(points) == 0:
+        raise ValueError("A template needs at least one surface node")
+    groups = _hull_groups(vertices)
+    _, inverse = _check_manifold(faces)
+    feature = _pseudonormals(vertices, faces, inverse)
+    tri = np.ascontiguousarray(vertices[faces].astype(F32))
+    rounded = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    if np.any(np.linalg.norm(rounded, axis=1) == 0):
+        raise ValueError("Triangle degenerates in normalized FP32")
+    centres = np.ascontiguousarray(tri.mean(axis=1))
+    return points, groups, tri, feature, centres
 
 
-def pack_meshes(shapes, length_unit):
-    """Pack a template library into the reference's 14 mesh arrays (normalized units)."""
-    packer = _Packer()
-    for shape in shapes:
-        packer.add(shape, length_unit)
-    return packer.arrays()
+def _pointer(array, row=0):
+    return ct.c_void_p(int(array.ctypes.data) + int(row) * int(array.strides[0]))
+
+
+def pack_meshes(shapes, length_unit, *, threads=None):
+    """Pack a template library into the reference's 14 mesh arrays (normalized units;
+    `mesh.py:10-222`).
+
+    Templates are independent: their inputs are prepared and their two hierarchies
+    built (native, `libsad_pack.so`) on a thread pool, each writing its own rows of
+    the preallocated arrays at offsets fixed up front, so the result is the serial
+    packer's, row for row, for any thread count.
+    """
+    from concurrent.futures import ThreadPoolExecutor
+
+    from . import native
+
+    lib = native.packer()
+    shapes = list(shapes)
+    threads = max(1, min(16, os.cpu_count() or 1) if threads is None else int(threads))
+    with ThreadPoolExecutor(threads) as pool:
+        prepared = list(pool.map(lambda shape: _prepare(shape, length_unit), shapes))
+        t = len(shapes)
+        tris = np.array([len(p[2]) for p in prepared], np.int64)
+        points = np.array([len(p[0]) for p in prepared], np.int64)
+        nodes = np.array([lib.sad_pack_node_count(int(n), 4) for n in tris], np.int64)
+        clusters = np.array([lib.sad_pack_node_count(int(n), 64) for n in points], np.int64)
+        planes = np.array([sum(len(g) for g in p[1]) for p in prepared], np.int64)
+        if max(nodes.sum(), clusters.sum(), 7 * tris.sum(), points.sum()) >= 1 << 32:
+            raise ValueError("Packed template library exceeds 32-bit indices")
+
+        def starts(counts):
+            return np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+
+        node0, tri0, cluster0, key0, plane0 = (starts(c) for c in (nodes, tris, clusters, points, planes))
+        u4 = lambda rows: np.zeros((int(rows), 4), np.uint32)  # noqa: E731
+        f4 = lambda rows: np.zeros((int(rows), 4), F32)  # noqa: E731
+        out = {
+            "meshFrontier": u4(32 * t), "meshClusterInfo": u4(t), "meshClusterRanges": u4(clusters.sum()),
+            "meshClusterMin": f4(clusters.sum()), "meshClusterMax": f4(clusters.sum()),
+            "meshSampleIndices": np.zeros(int(points.sum()), np.uint32), "meshHullInfo": u4(6 * t),
+            "meshPlanes": f4(planes.sum()), "meshRoots": u4(t), "meshMin": f4(nodes.sum()),
+            "meshMax": f4(nodes.sum()), "meshLinks": u4(nodes.sum()), "meshTriangles": f4(3 * tris.sum()),
+            "meshNormals": f4(7 * tris.sum()),
+        }
+
+        def pack(k):
+            p_points, groups, tri, feature, centres = prepared[k]
+            depth = ct.c_uint32(0)
+            rc = lib.sad_pack_clusters(
+                _pointer(p_points), len(p_points), int(cluster0[k]), int(key0[k]),
+                _pointer(out["meshClusterRanges"], cluster0[k]), _pointer(out["meshClusterMin"], cluster0[k]),
+                _pointer(out["meshClusterMax"], cluster0[k]), _pointer(out["meshSampleIndices"], key0[k]),
+                _pointer(out["meshFrontier"], 32 * k))
+            if rc:
+                raise ValueError(f"cluster tree packing failed ({rc})")
+            order = np.zeros(len(tri), np.uint32)
+            rc = lib.sad_pack_bvh(
+                _pointer(tri), _pointer(centres), len(tri), int(node0[k]), int(tri0[k]),
+                _pointer(out["meshLinks"], node0[k]), _pointer(out["meshMin"], node0[k]),
+                _pointer(out["meshMax"], node0[k]), _pointer(order), ct.byref(depth))
+            if rc == 5:
+                raise ValueError("Mesh hierarchy exceeds GPU traversal stack depth")
+            if rc:
+                raise ValueError(f"triangle hierarchy packing failed ({rc})")
+            out["meshClusterInfo"][k] = [cluster0[k], clusters[k], 0, 0]
+            out["meshRoots"][k] = [node0[k], node0[k] + nodes[k], depth.value, 0]
+            out["meshTriangles"][3 * tri0[k] : 3 * (tri0[k] + tris[k]), :3] = tri[order].reshape(-1, 3)
+            out["meshNormals"][7 * tri0[k] : 7 * (tri0[k] + tris[k]), :3] = feature[order].reshape(-1, 3)
+            at = int(plane0[k])
+            for g, group in enumerate(groups):
+                out["meshHullInfo"][6 * k + g] = [at, len(group), 0, 0]
+                out["meshPlanes"][at : at + len(group)] = group
+                at += len(group)
+
+        list(pool.map(pack, range(t)))
+    return out
 
 
 def sample_arrays(shapes, length_unit):

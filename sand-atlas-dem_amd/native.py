@@ -1,4 +1,5 @@
-"""Build and load the in-tree HIP library (`libsad_hip.so`, gfx950).
+"""Build and load the in-tree libraries: the HIP library (`libsad_hip.so`, gfx950)
+and the host-side mesh packer (`libsad_pack.so`).
 
 The library is compiled with hipcc directly (no JIT cache), in-tree, so the same
 file is what tests, the smoke check and the benchmark load.  FP32 semantics are
@@ -38,6 +39,46 @@ def compile_library(output=HERE / "libsad_hip.so", defines=()):
         raise RuntimeError(f"hipcc failed ({result.returncode}):\n{result.stderr[-4000:]}")
     tmp.replace(output)
     return output
+
+
+PACK_SOURCE = HERE / "csrc" / "sad_pack.cpp"
+PACK_LIBRARY = HERE / "libsad_pack.so"
+INCLUDE = HERE.parent / "include"
+CXX = os.environ.get("CXX", "g++")
+
+
+def compile_packer(output=PACK_LIBRARY):
+    """Compile the host-side mesh packer (`csrc/sad_pack.cpp`, plain C++; always
+    rebuilds).  No fast-math: bounds are rounded outward with nextafter."""
+    tmp = Path(str(output) + ".tmp")
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+           f"-I{INCLUDE}", "-o", str(tmp), str(PACK_SOURCE)]
+    result = subprocess.run(cmd, capture_output=True, text=True)
+    if result.returncode:
+        raise RuntimeError(f"{CXX} failed ({result.returncode}):\n{result.stderr[-4000:]}")
+    tmp.replace(output)
+    return output
+
+
+_PACK = None
+PACK_EXPORTED = ("sad_pack_node_count", "sad_pack_bvh", "sad_pack_clusters")
+
+
+def packer():
+    """The loaded mesh-packer library; raises if it has not been built."""
+    global _PACK
+    if _PACK is None:
+        if not PACK_LIBRARY.exists():
+            raise FileNotFoundError(f"{PACK_LIBRARY} is missing: build it with __graft_entry__.build() first")
+        lib = ct.CDLL(str(PACK_LIBRARY))
+        u32, vp = ct.c_uint32, ct.c_void_p
+        lib.sad_pack_node_count.restype, lib.sad_pack_node_count.argtypes = u32, [u32, u32]
+        lib.sad_pack_bvh.restype = ct.c_int
+        lib.sad_pack_bvh.argtypes = [vp, vp, u32, u32, u32, vp, vp, vp, vp, ct.POINTER(u32)]
+        lib.sad_pack_clusters.restype = ct.c_int
+        lib.sad_pack_clusters.argtypes = [vp, u32, u32, u32, vp, vp, vp, vp, vp]
+        _PACK = lib
+    return _PACK
 
 
 _LIB = None

@@ -26,6 +26,10 @@ def load_package():
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 load_package()
+# Every engine packs its templates with the native packer; build it if a fresh
+# checkout has not been built yet (plain g++, a few seconds).
+if not sys.modules["sand_atlas_dem_amd"].native.PACK_LIBRARY.exists():
+    sys.modules["sand_atlas_dem_amd"].native.compile_packer()
 
 
 def pytest_configure(config):

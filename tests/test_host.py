@@ -137,6 +137,16 @@ def test_c_abi_library_exports_every_declared_symbol(pkg):
     assert lib.sad_hip_abi_version() == 1
 
 
+def test_packer_library_exports_every_declared_symbol(pkg):
+    """The host mesh packer exports the entry points its header declares."""
+    header = (ROOT / "include" / "sand_dem_pack.h").read_text()
+    declared = set(re.findall(r"\b(sad_pack_\w+)\s*\(", header))
+    assert declared == set(pkg.native.PACK_EXPORTED)
+    lib = ct.CDLL(str(pkg.native.PACK_LIBRARY))
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
 def test_abi_tables_match_header(pkg):
     header = (ROOT / "include" / "sand_dem_hip.h").read_text()
     body = header[header.index("enum sad_buffer"):header.index("SAD_BUFFER_COUNT")]

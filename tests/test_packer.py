@@ -1,0 +1,67 @@
+"""The native threaded mesh packer against the serial restatement of the
+reference's packing loop (oracle/mesh_pack.py; `mesh.py:10-222`) -- CPU only.
+
+Every one of the 14 arrays must be equal row for row, on icosphere, harmonic
+(sand-like, 256-4,096 nodes), box and concave templates, with one thread and
+with eight."""
+
+import time
+
+import numpy as np
+import pytest
+
+
+def library(pkg, kind):
+    from sand_atlas_dem_amd import assembly
+
+    if kind == "box":
+        return [pkg.geometry.box_shape([1e-3, 2e-3, 1.5e-3], nodes=300, seed=3)]
+    if kind == "concave":
+        v, f = pkg.geometry.concave_prism_mesh()
+        return [pkg.geometry.make_shape("prism", v * 1e-3, f, nodes=200, seed=4)]
+    w = assembly.config_workload(kind, n=64, templates=12)
+    return w.shapes
+
+
+@pytest.mark.parametrize("kind", ["dense", "uneven", "box", "concave"])
+def test_native_packer_equals_serial_restatement(pkg, kind):
+    from oracle.mesh_pack import pack_meshes_serial
+
+    shapes = library(pkg, kind)
+    unit = float(np.median([s.equivalent_diameter for s in shapes]))
+    want = pack_meshes_serial(pkg.geometry, shapes, unit)
+    for threads in (1, 8):
+        got = pkg.geometry.pack_meshes(shapes, unit, threads=threads)
+        assert got.keys() == want.keys()
+        for name in want:
+            assert got[name].dtype == want[name].dtype, name
+            assert got[name].shape == want[name].shape, name
+            assert np.array_equal(got[name].view(np.uint8), want[name].view(np.uint8)), name
+
+
+def test_node_count_matches_split_recursion(pkg):
+    lib = pkg.native.packer()
+
+    def nodes(n, leaf):
+        return 1 if n <= leaf else 1 + nodes(n // 2, leaf) + nodes(n - n // 2, leaf)
+
+    for n in (1, 4, 5, 63, 64, 65, 1000, 4097, 20480):
+        assert lib.sad_pack_node_count(n, 4) == nodes(n, 4)
+        assert lib.sad_pack_node_count(n, 64) == nodes(n, 64)
+    assert lib.sad_pack_node_count(0, 4) == 0
+
+
+def test_threaded_packer_is_faster_on_a_large_library(pkg):
+    """424-template-scale work in well under the serial restatement's time (the
+    native tree build and the thread pool; a loose bound, this is not a benchmark)."""
+    from oracle.mesh_pack import pack_meshes_serial
+
+    shapes = library(pkg, "uneven")
+    unit = float(np.median([s.equivalent_diameter for s in shapes]))
+    t0 = time.perf_counter()
+    pack_meshes_serial(pkg.geometry, shapes, unit)
+    serial = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pkg.geometry.pack_meshes(shapes, unit, threads=8)
+    native = time.perf_counter() - t0
+    assert native < serial / 2, (native, serial)
