@@ -42,6 +42,17 @@ int sad_pack_bvh(const float* triangles, const float* centres, uint32_t count, u
 int sad_pack_clusters(const float* points, uint32_t count, uint32_t cluster_base, uint32_t key_base,
                       uint32_t* ranges, float* lower, float* upper, uint32_t* keys, uint32_t* frontier);
 
+/* Signed distance grid of one closed, consistently wound triangle mesh (the SDF
+ * geometry's field; the reference samples trimesh's signed distance on the same
+ * grid, geometry.py:189-215).  Grid point (x, y, z) is origin + spacing * (x, y, z)
+ * and goes to out[(x * ny + y) * nz + z]; its value is the exact distance to the
+ * nearest triangle (FP64, BVH), negative inside by the angle-weighted pseudonormal
+ * of the nearest feature.  `triangles`: count x 3 vertices x xyz; `normals`: count
+ * x 7 x xyz unit pseudonormals (face, edges AB/BC/CA, vertices A/B/C).  `threads`
+ * host threads share the x slabs. */
+int sad_sdf_grid(const double* triangles, const double* normals, uint32_t count, const double* origin,
+                 double spacing, const uint32_t* dims, uint32_t threads, float* out);
+
 #ifdef __cplusplus
 }
 #endif

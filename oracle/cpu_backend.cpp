@@ -155,6 +155,10 @@ struct R {
     float4 *wallPlane, *wallRemainder, *wallVelocity, *wallMaterial, *wallRows, *wallAccumulated;
     uint32_t* wallBoxIds;
     uint64_t metaCount;  // capacity of the meta/pointDepth/normalArea views
+    uint4* grids;        // SDF path: per template (first value, nx, ny, nz)
+    float4* gridOrigin;  // per template (origin xyz, spacing)
+    real* distance;
+    uint64_t distanceCount;
 };
 
 template <class T>
@@ -234,6 +238,10 @@ R bind(void** b, const uint64_t* bytes, const double* v) {
     r.pairs = at<uint2>(b, SAD_PAIRS);
     r.contactCounts = at<uint32_t>(b, SAD_CONTACT_COUNTS);
     r.contactOffsets = at<uint32_t>(b, SAD_CONTACT_OFFSETS);
+    r.grids = at<uint4>(b, SAD_GRIDS);
+    r.gridOrigin = at<float4>(b, SAD_GRID_ORIGIN);
+    r.distance = at<real>(b, SAD_DISTANCE);
+    r.distanceCount = bytes[SAD_DISTANCE] / 4;
     r.meta = at<uint4>(b, SAD_META);
     r.pointDepth = at<float4>(b, SAD_POINT_DEPTH);
     r.normalArea = at<float4>(b, SAD_NORMAL_AREA);
@@ -483,6 +491,38 @@ bool meshClusterMayContact(const R& r, uint32_t src, uint32_t dst, uint32_t clus
     high = high + f3(pad, pad, pad);
     return meshBoxMayContact(r, r.shapeId[dst], low, high);
 }
+// sdf (solver.slang:188-200): trilinear value and its piecewise gradient; 1e20 outside
+// the grid.  lerp(x, y, s) is x + s (y - x).  A corner index beyond the distance
+// buffer (inconsistent grid rows) raises flag 2 and reads as outside.
+float4 sdf(const R& r, uint32_t s, float3 p) {
+    uint4 meta = r.grids[s];
+    float4 o = r.gridOrigin[s];
+    float3 g = (p - xyz(o)) / o.w;
+    const float4 outside = {0, 0, 0, real(1e20)};
+    real dims[3] = {real(int64_t(meta.y) - 1), real(int64_t(meta.z) - 1), real(int64_t(meta.w) - 1)};
+    real gs[3] = {g.x, g.y, g.z};
+    for (int a = 0; a < 3; a++)
+        if (!(gs[a] >= 0) || gs[a] >= dims[a]) return outside;
+    int ax = int(std::floor(g.x)), ay = int(std::floor(g.y)), az = int(std::floor(g.z));
+    float3 t = {g.x - real(ax), g.y - real(ay), g.z - real(az)};
+    float3 x = f3(real(1.0), real(1.0), real(1.0)) - t;
+    real value = 0;
+    float3 grad = {0, 0, 0};
+    for (uint32_t i = 0; i < 2; i++)
+        for (uint32_t j = 0; j < 2; j++)
+            for (uint32_t k = 0; k < 2; k++) {
+                float3 b = {real(i), real(j), real(k)};
+                float3 w = x + b * (t - x);
+                uint32_t index = meta.x + ((uint32_t(ax) + i) * meta.z + (uint32_t(ay) + j)) * meta.w + (uint32_t(az) + k);
+                if (index >= r.distanceCount) { fail(r, 2); return outside; }
+                real f = r.distance[index];
+                value += f * w.x * w.y * w.z;
+                float3 d = {(real(2.0) * real(i) - 1) * w.y * w.z, (real(2.0) * real(j) - 1) * w.x * w.z,
+                            (real(2.0) * real(k) - 1) * w.x * w.y};
+                grad = grad + f * d / o.w;
+            }
+    return f4(grad, value);
+}
 // nodeContact (solver.slang:319-343) with the CPU rejection plane (cpu.py:59-76).
 bool nodeContact(const R& r, uint32_t src, uint32_t dst, uint32_t node, float4& pd, float4& na,
                  float4 rejectionPlane = {0, 0, 0, 0}) {
@@ -492,10 +532,15 @@ bool nodeContact(const R& r, uint32_t src, uint32_t dst, uint32_t node, float4& 
     float3 branch = centerBranch(r, src, dst);
     float3 local = qinv(r.rotation[dst], arm - branch) / r.body[dst].x;
     uint32_t targetShape = r.shapeId[dst];
-    real planePad = real(64.0) * EPS * (real(1.0) + maxabs(local));
-    if (dot(xyz(rejectionPlane), local) + rejectionPlane.w > planePad) { pd = {0, 0, 0, 0}; na = {0, 0, 0, 0}; return false; }
-    if (!meshMayContain(r, targetShape, local)) { pd = {0, 0, 0, 0}; na = {0, 0, 0, 0}; return false; }
-    float4 f = meshDistance(r, targetShape, local);
+    float4 f;
+    if (r.c.meshGeometry != 0) {
+        real planePad = real(64.0) * EPS * (real(1.0) + maxabs(local));
+        if (dot(xyz(rejectionPlane), local) + rejectionPlane.w > planePad) { pd = {0, 0, 0, 0}; na = {0, 0, 0, 0}; return false; }
+        if (!meshMayContain(r, targetShape, local)) { pd = {0, 0, 0, 0}; na = {0, 0, 0, 0}; return false; }
+        f = meshDistance(r, targetShape, local);
+    } else {
+        f = sdf(r, targetShape, local);
+    }
     pd = f4(arm, -f.w * r.body[dst].x);
     na = {0, 0, 0, 0};
     if (pd.w <= 0) return false;
@@ -547,6 +592,51 @@ void cpuCollectMesh(const R& r, uint32_t pair) {
             cluster = candidate ? cluster + 1 : range.z;
         }
     }
+}
+
+// ---------------------------------------------------------------- SDF contacts (cpu_scheduling.slang:102-132)
+// Both grains' nodes in order, s < a the source grain's: pair-major, node order.
+uint32_t pairNodeContacts(const R& r, uint32_t pair, bool emit) {
+    uint2 ij = r.pairs[pair];
+    uint32_t a = r.sampleInfo[r.shapeId[ij.x]].y;
+    uint32_t total = a + r.sampleInfo[r.shapeId[ij.y]].y, count = 0;
+    uint32_t offset = emit ? r.contactOffsets[pair] : 0;
+    for (uint32_t s = 0; s < total; s++) {
+        uint32_t i = s < a ? ij.x : ij.y, j = s < a ? ij.y : ij.x, node = s < a ? s : s - a;
+        float4 pd, na;
+        if (!nodeContact(r, i, j, node, pd, na)) continue;
+        count++;
+        if (!emit) continue;
+        if (offset >= r.c.contactCapacity) { fail(r, 32); return count; }
+        r.meta[offset] = {i, j, node, 0};
+        r.pointDepth[offset] = pd;
+        r.normalArea[offset] = na;
+        offset++;
+    }
+    return count;
+}
+void countContacts(const R& r, uint32_t pair) {  // :102-116
+    if (pair >= r.c.pairCapacity) return;
+    uint32_t count = 0;
+    if (pair < r.control[1] && r.control[0] == 0) count = pairNodeContacts(r, pair, false);
+    r.contactCounts[pair] = count;
+}
+void emitContacts(const R& r, uint32_t pair) {  // :118-131
+    if (pair >= std::min(r.control[1], r.c.pairCapacity) || r.control[0] != 0) return;
+    pairNodeContacts(r, pair, true);
+}
+void contactScan(const R& r) {  // Scan over contactCounts (gpu.py:539-541): exclusive, wrapping
+    uint32_t sum = 0;
+    for (uint32_t pair = 0; pair < r.c.pairCapacity; pair++) {
+        r.contactOffsets[pair] = sum;
+        sum += r.contactCounts[pair];
+    }
+}
+void contactTotal(const R& r) {  // solver.slang:568-574
+    uint32_t n = r.c.pairCapacity;
+    r.control[2] = r.contactOffsets[n - 1] + r.contactCounts[n - 1];
+    r.control[6] = 0;
+    if (r.control[2] > r.c.contactCapacity) fail(r, 32);
 }
 
 // ---------------------------------------------------------------- translating planes (solver.slang:135-145, 578-667)
@@ -632,6 +722,11 @@ void collectWallContacts(const R& r, uint32_t id) {  // CPU serial order, cpu_sc
     real phi = centreDistance.x + centreDistance.y;
     real pad = real(64.0) * real(1.1920928955078125e-7) * (real(1.0) + length(xyz(r.position[body])) + std::fabs(plane.w));
     if (phi > r.body[body].z + pad) return;
+    if (r.c.meshGeometry == 0) {
+        uint32_t count = r.sampleInfo[r.shapeId[body]].y;
+        for (uint32_t node = 0; node < count; node++) appendWallContact(r, body, wall, node);
+        return;
+    }
     uint4 info = r.meshClusterInfo[r.shapeId[body]];
     uint32_t cluster = info.x;
     while (cluster < info.x + info.y) {
@@ -1234,7 +1329,8 @@ extern "C" int oracle_dispatch(const char* kernel, uint32_t count, void** buffer
                                const double* values) {
     try {
         R r = bind(buffers, bytes, values);
-        if (r.c.meshGeometry == 0 || r.c.sourceSurfaceNormal != 0) return 3;
+        if (r.c.sourceSurfaceNormal != 0) return 3;
+        if (r.c.meshGeometry == 0 && (!r.grids || !r.gridOrigin || !r.distance)) return 2;
         std::string name(kernel);
         const uint32_t wallTasks = r.c.N * r.c.wallCount;
         auto each = [&](auto fn) { for (uint32_t id = 0; id < count; id++) fn(r, id); };
@@ -1260,6 +1356,10 @@ extern "C" int oracle_dispatch(const char* kernel, uint32_t count, void** buffer
         else if (name == "finishStep") each(finishStep);
         else if (name == "captureContactKinematics") each(captureContactKinematics);
         else if (name == "checkContactTimestep") each(checkContactTimestep);
+        else if (name == "countContacts") each(countContacts);
+        else if (name == "contactScan") contactScan(r);
+        else if (name == "contactTotal") contactTotal(r);
+        else if (name == "emitContacts") each(emitContacts);
         else if (name == "diagnosticRows") each(diagnosticRows);
         else if (name == "fabricRows") each(fabricRows);
         else if (name == "advanceWalls") advanceWalls(r);

@@ -24,7 +24,7 @@ INTEGER_SLOTS = frozenset((
     "control", "shapeId", "sampleInfo", "meshRoots", "meshHullInfo", "meshLinks", "meshFrontier", "meshPairMask",
     "meshClusterInfo", "meshClusterRanges", "meshSampleIndices", "binCounts", "binParticles", "pairCounts",
     "pairOffsets", "pairs", "contactCounts", "contactOffsets", "meta", "oldMeta", "oldHash", "hash", "heads", "next",
-    "wallBoxIds",
+    "wallBoxIds", "grids",
 ))
 
 

@@ -168,7 +168,7 @@ def packed_workload(name, shapes, n, phi, *, seed, sphere_fraction=None, size_ra
     x = sphere_packing(0.5 * alpha * d, box, rng["positions"])
     q = random_rotations(n, rng["orientations"])
     v = rng["velocities"].normal(0, velocity_sigma, (n, 3)) if velocity_sigma else np.zeros((n, 3))
-    spec = Experiment(contact_geometry="mesh", particles=n, **(spec_overrides or {}))
+    spec = Experiment(**dict(dict(contact_geometry="mesh"), particles=n, **(spec_overrides or {})))
     return Workload(name, shapes, spec, x, np.full(3, box), ids, scales, q, v, tuple(rate), steps)
 
 
@@ -272,7 +272,7 @@ def conservative_timestep(shapes, ids, scales, spec):
     return float(np.floor(dt / 10**exponent * 100) / 100 * 10**exponent)
 
 
-def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, steps=None):
+def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, steps=None, resolution=None):
     """The named benchmark configurations (sizes overridable for tests).
 
     small  : 1,000 icospheres (subdiv 2, sigma 0.1), 512 nodes, phi 0.30, v ~ N(0, 0.01 m/s)
@@ -286,6 +286,8 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
              of six planes, the top plane moving down at 0.5 box heights per second
     layer  : the same grains between two z planes in a cell periodic in x and y, the
              top plane moving down at 0.5 layer heights per second
+    sdf    : the dense assembly with the reference's default contact geometry, SDF
+             grids (resolution 64: 64 points across the largest extent, 3 of padding)
     """
     rng = streams(seed)
     d0 = 3e-4
@@ -307,6 +309,14 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
         phi, sphere_fraction, vsig, rate = 0.60, 0.62, 0.0, (-0.5, -0.5, -0.5)
         steps = steps or 2000
         cpp = 512  # ~190 node contacts per grain at phi 0.60: headroom over the default 256
+    elif config == "sdf":
+        n = n or 10_000
+        shapes = icosphere_library(templates or 64, subdivisions=(2, 3, 4), sigma=0.15, diameter=d0,
+                                   nodes=nodes or 2048, rng=rng["templates"], seed=seed)
+        shapes = [geometry.with_sdf(shape, resolution or 64) for shape in shapes]
+        phi, sphere_fraction, vsig, rate = 0.60, 0.62, 0.0, (-0.5, -0.5, -0.5)
+        steps = steps or 2000
+        cpp = 512
     elif config == "uneven":
         n = n or 50_000
         shapes = uneven_library(templates or 424, diameter=d0, rng=rng["templates"], seed=seed)
@@ -339,6 +349,8 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
     scales = rng["sizes"].uniform(0.8, 1.2, n)
     dt = conservative_timestep(shapes, ids, scales, probe)
     overrides = dict(base, timestep=dt, max_overlap_fraction=0.5, contacts_per_particle=cpp)
+    if config == "sdf":
+        overrides["contact_geometry"] = "sdf"
     if config == "layer":
         return layer_workload(config, shapes, n, phi, seed=seed, sphere_fraction=sphere_fraction, steps=steps,
                               spec_overrides=overrides, rng=rng, ids=ids, scales=scales)

@@ -62,7 +62,13 @@ struct P {
     const uint32_t* meshSampleIndices;
     uint32_t *binCounts, *binParticles, *pairCounts, *pairOffsets;
     uint2* pairs;
-    uint32_t* contactCounts;
+    uint32_t *contactCounts, *contactOffsets;
+    // SDF geometry: per template (first value, nx, ny, nz) and (origin, spacing),
+    // and the grid values.
+    const uint4* grids;
+    const float4* gridOrigin;
+    const float* distance;
+    uint64_t distanceCount;
     uint4* meta;
     float4 *pointDepth, *normalArea, *slip, *contactForce;
     float* normalLossPower;
@@ -1660,6 +1666,118 @@ __global__ void k_scatter(P p, W w) {
     }
 }
 
+// ---------------------------------------------------------------- SDF contacts (solver.slang:188-200, 319-343, 536-574)
+// sdf: the template grid's trilinear value and piecewise gradient, 1e20 outside the
+// grid; lerp(x, y, s) is x + s (y - x).  A corner index beyond the distance buffer
+// (inconsistent grid rows) raises flag 2 and reads as outside.
+__device__ v4 sdfSample(const P& p, uint32_t s, v3 q) {
+    const uint4 meta = p.grids[s];
+    const float4 o = p.gridOrigin[s];
+    const v3 g = (q - xyz(o)) / o.w;
+    const v4 outside = {0, 0, 0, 1e20f};
+    if (!(g.x >= 0.0f) || !(g.y >= 0.0f) || !(g.z >= 0.0f) || g.x >= float(int64_t(meta.y) - 1) ||
+        g.y >= float(int64_t(meta.z) - 1) || g.z >= float(int64_t(meta.w) - 1))
+        return outside;
+    const int ax = int(floorf(g.x)), ay = int(floorf(g.y)), az = int(floorf(g.z));
+    const v3 t = {g.x - float(ax), g.y - float(ay), g.z - float(az)};
+    const v3 x = mk3(1.0f, 1.0f, 1.0f) - t;
+    float value = 0;
+    v3 grad = {0, 0, 0};
+    for (uint32_t i = 0; i < 2; i++)
+        for (uint32_t j = 0; j < 2; j++)
+            for (uint32_t k = 0; k < 2; k++) {
+                const v3 b = mk3(float(i), float(j), float(k));
+                const v3 w = x + b * (t - x);
+                const uint32_t index =
+                    meta.x + ((uint32_t(ax) + i) * meta.z + (uint32_t(ay) + j)) * meta.w + (uint32_t(az) + k);
+                if (index >= p.distanceCount) { fail(p, 2); return outside; }
+                const float f = p.distance[index];
+                value += f * w.x * w.y * w.z;
+                const v3 d = {(2.0f * float(i) - 1.0f) * w.y * w.z, (2.0f * float(j) - 1.0f) * w.x * w.z,
+                              (2.0f * float(k) - 1.0f) * w.x * w.y};
+                grad = grad + f * d / o.w;
+            }
+    return {grad.x, grad.y, grad.z, value};
+}
+// nodeContact (solver.slang:319-343), SDF branch.
+__device__ bool sdfNodeContact(const P& p, uint32_t src, uint32_t dst, uint32_t node, float4& pd, float4& na) {
+    const float4 bs = p.body[src], bd = p.body[dst];
+    const float4 sp = p.sample[p.sampleInfo[p.shapeId[src]].x + node];
+    const v4 qj = v4of(p.rotation[dst]);
+    const v3 arm = qrot(v4of(p.rotation[src]), xyz(sp) * bs.x);
+    const v3 local = qinv(qj, arm - centerBranch(p, src, dst)) / bd.x;
+    const v4 f = sdfSample(p, p.shapeId[dst], local);
+    const float depth = -f.w * bd.x;
+    pd = f4(arm, depth);
+    na = make_float4(0, 0, 0, 0);
+    if (depth <= 0) return false;
+    const v3 grad = qrot(qj, xyz(f));
+    const float len = length(grad);
+    if (len < 1e-7f || !isfinite(depth)) { fail(p, 2); return false; }
+    na = f4(grad / len, sp.w * bs.x * bs.x);
+    if (depth > p.c.maxOverlap * fminf(bs.w, bd.w)) fail(p, 4);
+    return true;
+}
+// countContacts / emitContacts in the CPU's order (cpu_scheduling.slang:102-131):
+// one wave per pair takes both grains' nodes (s < a: the first grain's) 64 at a
+// time; hits are ranked by ballot, so they are written in node order.  The counts
+// and offsets of the two directions (first grain as source, then the second) also
+// go to the per-direction task rows the gather walks, as after a mesh collection.
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pair >= p.c.pairCapacity) return;
+    const bool live = pair < p.control[1] && ld_volatile(p.control) == 0;
+    if (EMIT && !live) return;
+    uint32_t count = 0, forward = 0;  // all hits, hits with the first grain as source
+    if (live) {
+        const uint2 ij = p.pairs[pair];
+        const uint32_t a = p.sampleInfo[p.shapeId[ij.x]].y, total = a + p.sampleInfo[p.shapeId[ij.y]].y;
+        const uint32_t base = EMIT ? p.contactOffsets[pair] : 0;
+        for (uint32_t first = 0; first < total; first += 64) {
+            const uint32_t s = first + lane;
+            const bool mine = s < a;
+            const uint32_t i = mine ? ij.x : ij.y, j = mine ? ij.y : ij.x, node = mine ? s : s - a;
+            float4 pd, na;
+            const bool hit = s < total && sdfNodeContact(p, i, j, node, pd, na);
+            const uint64_t mask = __ballot(hit);
+            if (EMIT && hit) {
+                const uint32_t id = base + count + __popcll(mask & lanes_below(lane));
+                if (id < p.c.contactCapacity) {
+                    p.meta[id] = make_uint4(i, j, node, 0);
+                    p.pointDepth[id] = pd;
+                    p.normalArea[id] = na;
+                } else {
+                    fail(p, 32);
+                }
+            }
+            count += __popcll(mask);
+            forward += __popcll(__ballot(hit && mine));
+        }
+        if (EMIT && lane == 0) {
+            w.taskOffset[2 * pair] = base;
+            w.taskOffset[2 * pair + 1] = base + forward;
+        }
+    }
+    if (!EMIT && lane == 0) {
+        p.contactCounts[pair] = count;
+        w.taskCount[2 * pair] = forward;
+        w.taskCount[2 * pair + 1] = count - forward;
+    }
+}
+// contactTotal (solver.slang:568-574).
+__global__ void k_sdf_total(P p, W w) {
+    const uint32_t n = p.c.pairCapacity;
+    const uint32_t total = p.contactOffsets[n - 1] + p.contactCounts[n - 1];
+    p.control[2] = total;
+    p.control[6] = 0;
+    if (total > p.c.contactCapacity) fail(p, 32);
+    atomicAdd(w.totals + 0, 1ull);
+    atomicAdd(w.totals + 1, 2ull * min(p.control[1], n));
+    atomicAdd(w.totals + 3, (unsigned long long)total);
+}
+
 // ---------------------------------------------------------------- plane contacts (solver.slang:596-667)
 // collectWallContacts in the CPU's serial order (cpu_scheduling.slang:133-154):
 // task t = wall * N + grain, the grain's sample clusters in preorder, a leaf's
@@ -1689,46 +1807,56 @@ __global__ __launch_bounds__(256) void k_wall_collect(P p, W w) {
         const v3 normal = qinv(q, xyz(plane)) * bd.x;  // wallClusterMayContact (solver.slang:613-623)
         const float normalLength = length(normal);
         const uint32_t shape = p.shapeId[body];
-        const uint4 info = p.meshClusterInfo[shape];
         const uint32_t samples = p.sampleInfo[shape].x;
         const uint32_t out = EMIT ? p.control[2] + w.wallOffset[task] : 0;
-        for (uint32_t cluster = info.x; cluster < info.x + info.y;) {
-            const v3 low = xyz(p.meshClusterMin[cluster]), high = xyz(p.meshClusterMax[cluster]);
-            const v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
-            const float support = dot(normal, center) - dot(fabs3(normal), half);
-            const float pad = eps64 * (1.0f + fabsf(plane.w) + lengthX + normalLength * (length(center) + length(half)));
-            const bool candidate = phi + support <= pad;
-            const uint4 range = p.meshClusterRanges[cluster];
-            if (candidate && range.y != 0) {  // appendWallContact (solver.slang:625-638) per node
-                bool hit = false;
-                uint32_t node = 0;
-                v3 arm = {0, 0, 0};
-                float depth = 0, area = 0;
-                if (uint32_t(lane) < range.y) {
-                    node = p.meshSampleIndices[range.x + lane];
-                    const float4 point = p.sample[samples + node];
-                    arm = qrot(q, xyz(point) * bd.x);
-                    const float d = localWallDistance(centre, plane, planeLow, arm);
-                    if (!isfinite(d)) fail(p, 512);
-                    else if (d < 0) {
-                        hit = true;
-                        depth = -d;
-                        area = point.w * bd.x * bd.x;
-                        if (depth > p.c.maxOverlap * bd.w) fail(p, 4);
-                    }
+        // appendWallContact (solver.slang:625-638) for up to 64 nodes, one per lane,
+        // ranked by ballot.
+        auto appendNodes = [&](bool active, uint32_t node) {
+            bool hit = false;
+            v3 arm = {0, 0, 0};
+            float depth = 0, area = 0;
+            if (active) {
+                const float4 point = p.sample[samples + node];
+                arm = qrot(q, xyz(point) * bd.x);
+                const float d = localWallDistance(centre, plane, planeLow, arm);
+                if (!isfinite(d)) fail(p, 512);
+                else if (d < 0) {
+                    hit = true;
+                    depth = -d;
+                    area = point.w * bd.x * bd.x;
+                    if (depth > p.c.maxOverlap * bd.w) fail(p, 4);
                 }
-                const uint64_t mask = __ballot(hit);
-                if (EMIT && hit) {
-                    const uint32_t id = out + count + __popcll(mask & lanes_below(lane));
-                    if (id < p.c.contactCapacity) {
-                        p.meta[id] = make_uint4(body, p.c.N + wall, node, 1);
-                        p.pointDepth[id] = f4(arm, depth);
-                        p.normalArea[id] = make_float4(plane.x, plane.y, plane.z, area);
-                    }
-                }
-                count += __popcll(mask);
             }
-            cluster = candidate ? cluster + 1 : range.z;
+            const uint64_t mask = __ballot(hit);
+            if (EMIT && hit) {
+                const uint32_t id = out + count + __popcll(mask & lanes_below(lane));
+                if (id < p.c.contactCapacity) {
+                    p.meta[id] = make_uint4(body, p.c.N + wall, node, 1);
+                    p.pointDepth[id] = f4(arm, depth);
+                    p.normalArea[id] = make_float4(plane.x, plane.y, plane.z, area);
+                }
+            }
+            count += __popcll(mask);
+        };
+        if (p.c.meshGeometry == 0) {  // every node in order (SDF geometry has no sample tree)
+            const uint32_t nodes = p.sampleInfo[shape].y;
+            for (uint32_t first = 0; first < nodes; first += 64) appendNodes(first + lane < nodes, first + lane);
+        } else {
+            const uint4 info = p.meshClusterInfo[shape];
+            for (uint32_t cluster = info.x; cluster < info.x + info.y;) {
+                const v3 low = xyz(p.meshClusterMin[cluster]), high = xyz(p.meshClusterMax[cluster]);
+                const v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
+                const float support = dot(normal, center) - dot(fabs3(normal), half);
+                const float pad =
+                    eps64 * (1.0f + fabsf(plane.w) + lengthX + normalLength * (length(center) + length(half)));
+                const bool candidate = phi + support <= pad;
+                const uint4 range = p.meshClusterRanges[cluster];
+                if (candidate && range.y != 0) {
+                    const bool active = uint32_t(lane) < range.y;
+                    appendNodes(active, active ? p.meshSampleIndices[range.x + lane] : 0u);
+                }
+                cluster = candidate ? cluster + 1 : range.z;
+            }
         }
     }
     if (!EMIT && lane == 0) w.wallCount[task] = count;
@@ -2343,7 +2471,7 @@ thread_local char g_error[256] = "no error";
     X(build_hash) X(gather) X(finish_step) X(capture_kinematics) X(check_timestep) \
     X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_offsets) X(bucket_fill) \
     X(advance_walls) X(wall_count) X(wall_emit) X(wall_total) X(gather_wall_rows) X(refresh_wall_power) \
-    X(diagnostic_rows) X(fabric_rows)
+    X(diagnostic_rows) X(fabric_rows) X(sdf_count) X(sdf_total) X(sdf_emit)
 enum KernelId {
 #define SAD_ID(name) KID_##name,
     SAD_KERNELS(SAD_ID)
@@ -2509,7 +2637,7 @@ struct ScratchImpl : Scratch {
         if (key != lastTable) { lastTable.swap(key); epoch++; }
         w.skip = 0;
         w.warm = 0;
-        if (!skip) return SAD_OK;
+        if (!skip || p.c.meshGeometry == 0) return SAD_OK;  // the SDF step has no node queries to skip
         if (maxNodesFor != p.sampleInfo || maxNodesShapes != p.c.shapes || triBaseFor != p.meshLinks) {
             if (!probeWord) HIP_OK(hipMalloc(reinterpret_cast<void**>(&probeWord), 3 * sizeof(uint32_t)));
             if (triBaseCap < 3 * std::max(p.c.shapes, 1u)) {
@@ -2633,8 +2761,8 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
     c.wallCount = uint32_t(v[SAD_V_WALL_COUNT]);
     c.wallBoxEnabled = uint32_t(v[SAD_V_WALL_BOX_ENABLED]);
     c.sourceSurfaceNormal = uint32_t(v[SAD_V_SOURCE_SURFACE_NORMAL]);
-    if (c.meshGeometry == 0 || c.sourceSurfaceNormal != 0) {
-        std::snprintf(g_error, sizeof(g_error), "only the surface-mesh step is implemented");
+    if (c.sourceSurfaceNormal != 0) {
+        std::snprintf(g_error, sizeof(g_error), "source-surface contact normals are not implemented");
         return SAD_E_UNSUPPORTED;
     }
     if (c.N == 0 || c.hashSize == 0 || (c.hashSize & (c.hashSize - 1)) != 0) {
@@ -2668,6 +2796,11 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
         {SAD_NORMAL_DAMPING, 16ull * c.N, "normalDamping"}, {SAD_FORCE_VELOCITY, 16ull * c.N, "forceVelocity"},
         {SAD_FORCE_OMEGA, 16ull * c.N, "forceOmega"}, {SAD_CELL_LOG, 32, "cellLog"},
         {SAD_REFERENCE_CELL, 32, "referenceCell"},
+        {SAD_CONTACT_OFFSETS, 4ull * c.pairCapacity, "contactOffsets"},
+        {SAD_SAMPLE_INFO, 16ull * c.shapes, "sampleInfo"},
+        {SAD_GRIDS, c.meshGeometry ? 0ull : 16ull * c.shapes, "grids"},
+        {SAD_GRID_ORIGIN, c.meshGeometry ? 0ull : 16ull * c.shapes, "gridOrigin"},
+        {SAD_DISTANCE, c.meshGeometry ? 0ull : 4ull, "distance"},
     };
     for (const Need& n : needs) {
         if (!b[n.slot] || bytes[n.slot] < n.bytes) {
@@ -2740,6 +2873,11 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
     p.timestepRows = slot<float4>(b, SAD_TIMESTEP_ROWS);
     p.particleDiagnostic = slot<float4>(b, SAD_PARTICLE_DIAGNOSTIC);
     p.accumulated = slot<float4>(b, SAD_ACCUMULATED);
+    p.contactOffsets = slot<uint32_t>(b, SAD_CONTACT_OFFSETS);
+    p.grids = slot<const uint4>(b, SAD_GRIDS);
+    p.gridOrigin = slot<const float4>(b, SAD_GRID_ORIGIN);
+    p.distance = slot<const float>(b, SAD_DISTANCE);
+    p.distanceCount = b[SAD_DISTANCE] ? bytes[SAD_DISTANCE] / 4 : 0;
     p.diagnosticRows = slot<float4>(b, SAD_DIAGNOSTIC_ROWS);
     p.diagnosticBytes = b[SAD_DIAGNOSTIC_ROWS] ? bytes[SAD_DIAGNOSTIC_ROWS] : 0;
     p.fabricRows = slot<float4>(b, SAD_FABRIC_ROWS);
@@ -2881,6 +3019,22 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         if (c.wallCount) LAUNCH(sc, KID_gather_wall_rows, k_gather_wall_rows, dim3(blocks(c.wallCount * c.N)), T, s, p, w);
     } else if (name == "refreshWallPower") {
         if (c.wallCount) LAUNCH(sc, KID_refresh_wall_power, k_refresh_wall_power, dim3(blocks(c.wallCount * c.N)), T, s, p);
+    } else if (name == "countContacts") {
+        // The gather walks each grain's pairs in pair order (as after a mesh collection).
+        LAUNCH(sc, KID_adj_clear, k_adj_clear, perGrain, T, s, p, w);
+        LAUNCH(sc, KID_adj_count, k_adj_count, gridStride, T, s, p, w);
+        ScanArgs adj{w.adjCount, w.adjOffset, w.sums, c.N, nullptr, 1, nullptr, nullptr};
+        launch_scan(sc, adj, c.N, s);
+        LAUNCH(sc, KID_adj_fill, k_adj_fill, gridStride, T, s, p, w);
+        LAUNCH(sc, KID_adj_sort, k_adj_sort, perGrain, T, s, p, w);
+        LAUNCH(sc, KID_sdf_count, k_sdf_contacts<false>, dim3(blocks(c.pairCapacity, 4)), T, s, p, w);
+    } else if (name == "contactScan") {
+        ScanArgs a{p.contactCounts, p.contactOffsets, w.sums, c.pairCapacity, nullptr, 1, nullptr, nullptr};
+        launch_scan(sc, a, c.pairCapacity, s);
+    } else if (name == "contactTotal") {
+        LAUNCH(sc, KID_sdf_total, k_sdf_total, dim3(1), dim3(1), s, p, w);
+    } else if (name == "emitContacts") {
+        LAUNCH(sc, KID_sdf_emit, k_sdf_contacts<true>, dim3(blocks(c.pairCapacity, 4)), T, s, p, w);
     } else if (name == "diagnosticRows") {
         if (!p.diagnosticRows || p.diagnosticBytes < 128ull * c.N) {
             std::snprintf(g_error, sizeof(g_error), "buffer diagnosticRows too small for %u grains", c.N);
@@ -2894,10 +3048,9 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         }
         LAUNCH(sc, KID_fabric_rows, k_fabric_rows, dim3(blocks(c.pairCapacity)), T, s, p);
     } else {
-        // Reference entry points outside the mesh path (SDF grid, the serial
-        // verification collector) are known but not provided here.
-        static const char* unsupported[] = {"countContacts", "emitContacts", "contactTotal", "collectMeshContacts",
-                                            "smoke"};
+        // Reference entry points this backend does not provide: the serial
+        // verification collector and the smoke kernel.
+        static const char* unsupported[] = {"collectMeshContacts", "smoke"};
         for (const char* u : unsupported)
             if (name == u) {
                 std::snprintf(g_error, sizeof(g_error), "kernel %s is not part of the hip mesh path", u);
@@ -3004,13 +3157,20 @@ int sad_hip_dispatch(const char* kernel, uint32_t count, void* const* buffers, c
 
 // One timestep's launches (per-step order of gpu.py:714-726).
 int launch_step(P& p, ScratchImpl& sc, hipStream_t s) {
-    static const char* sequence[] = {"beginStep", "advanceCell", "advanceWalls", "checkRebuild", "clearBins",
-                                     "fillBins", "countPairs", "pairScan", "emitPairs", "resetMeshContacts",
-                                     "collectMeshFrontiers", "collectWallContacts", "clearContactStorage",
-                                     "resolveContacts", "buildHistoryHash", "gather", "gatherWallRows"};
+    static const char* mesh[] = {"beginStep", "advanceCell", "advanceWalls", "checkRebuild", "clearBins",
+                                 "fillBins", "countPairs", "pairScan", "emitPairs", "resetMeshContacts",
+                                 "collectMeshFrontiers", "collectWallContacts", "clearContactStorage",
+                                 "resolveContacts", "buildHistoryHash", "gather", "gatherWallRows"};
+    static const char* sdf[] = {"beginStep", "advanceCell", "advanceWalls", "checkRebuild", "clearBins",
+                                "fillBins", "countPairs", "pairScan", "emitPairs", "countContacts", "contactScan",
+                                "contactTotal", "emitContacts", "collectWallContacts", "clearContactStorage",
+                                "resolveContacts", "buildHistoryHash", "gather", "gatherWallRows"};
     int rc;
     if (p.c.activeTimestep && (rc = run_kernel("checkContactTimestep", p, sc, s))) return rc;
-    for (const char* name : sequence) {
+    const char* const* sequence = p.c.meshGeometry ? mesh : sdf;
+    const size_t steps = p.c.meshGeometry ? sizeof(mesh) / sizeof(mesh[0]) : sizeof(sdf) / sizeof(sdf[0]);
+    for (size_t k = 0; k < steps; k++) {
+        const char* name = sequence[k];
         if (std::strcmp(name, "clearBins") == 0) {
             // The Engine captures contact kinematics when any contact is viscous;
             // with planes that depends on device-resident wall materials.

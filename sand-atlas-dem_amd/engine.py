@@ -17,7 +17,7 @@ from dataclasses import replace
 
 import numpy as np
 
-from .geometry import sample_arrays
+from .geometry import _UINT_ARRAYS, sample_arrays
 from .mesh_cache import packed_meshes
 from .spec import box_extents, periodic_axes, periodic_code, finite_array
 
@@ -103,7 +103,7 @@ class Engine:
     """Mesh-path engine over an explicit backend adapter (gpu.py:94-368)."""
 
     def __init__(self, shapes, spec, positions, cell, shape_ids, scales, *, backend):
-        spec.require_mesh_path()
+        spec.require_supported_path()
         positions = finite_array(positions)
         cell = finite_array(cell, (3,), positive=True)
         if positions.ndim != 2 or positions.shape[1] != 3 or len(positions) < 1:
@@ -114,6 +114,7 @@ class Engine:
         self.m = backend
         self.spec = spec
         self.shapes = list(shapes)
+        self.mesh_geometry = spec.contact_geometry == "mesh"
         self.n = len(positions)
         self.wall_count = len(spec.planes)
         self.viscous = spec.viscous
@@ -172,7 +173,12 @@ class Engine:
             + 8 * self.bin_count * self.bin_capacity
             + 1184 * self.n
         )
-        estimated += sum(224 * len(s.faces) + 736 + 8 * len(s.samples) for s in self.shapes)
+        if self.mesh_geometry:
+            estimated += sum(224 * len(s.faces) + 736 + 8 * len(s.samples) for s in self.shapes)
+        else:
+            for shape in self.shapes:
+                shape.validate_sdf()
+            estimated += sum(np.asarray(s.sdf).nbytes + 32 * len(s.samples) for s in self.shapes)
         estimated += 160 * self.wall_count * self.n + 256 * self.wall_count
         if estimated > spec.working_memory_gib * 1024**3:
             raise MemoryError(f"Estimated working set {estimated / 1024**3:.2f} GiB exceeds budget")
@@ -324,7 +330,10 @@ class Engine:
         self.add("wallAccumulated", np.zeros((max(1, self.wall_count * self.n), 4), np.float32))
         self.add("wallRows", np.zeros((max(1, 3 * self.wall_count * self.n), 4), np.float32))
 
-    def _static_geometry(self):  # gpu.py:421-488, mesh branch
+    def _static_geometry(self):  # gpu.py:421-488
+        if not self.mesh_geometry:
+            self._sdf_geometry()
+            return
         mesh, self.mesh_cache_receipt = packed_meshes(self.shapes, self.length_unit)
         roots = mesh["meshRoots"]
         if np.any(roots[:, 1] - roots[:, 0] > 65535):
@@ -344,6 +353,31 @@ class Engine:
         self.add("grids", np.zeros((1, 4), np.uint32))
         self.add("gridOrigin", np.zeros((1, 4), np.float32))
         self.add("distance", np.zeros(4, np.float32))
+
+    def _sdf_geometry(self):
+        """SDF branch (gpu.py:427-488): one-row mesh placeholders, then per template
+        the grid row (first value, nx, ny, nz), origin and spacing in normalized
+        units, and the grid values (normalized) concatenated in C order."""
+        for name in ("meshHullInfo", "meshPlanes", "meshRoots", "meshLinks", "meshMin", "meshMax", "meshTriangles",
+                     "meshNormals", "meshFrontier", "meshClusterInfo", "meshClusterRanges", "meshClusterMin",
+                     "meshClusterMax", "meshSampleIndices"):
+            self.add(name, np.zeros((1, 4), np.uint32 if name in _UINT_ARRAYS else np.float32))
+        self.mesh_cache_receipt = dict(status="sdf")
+        l = self.length_unit
+        grids, origins, fields, first = [], [], [], 0
+        for shape in self.shapes:
+            grids.append([first, *np.asarray(shape.sdf).shape])
+            origins.append([*(np.asarray(shape.origin) / l), shape.spacing / l])
+            fields.append((np.asarray(shape.sdf) / l).ravel())
+            first += np.asarray(shape.sdf).size
+        if first >= 1 << 32:
+            raise ValueError("SDF grids exceed 32-bit indices")
+        info, samples = sample_arrays(self.shapes, l)
+        self.add("grids", np.asarray(grids, np.uint32))
+        self.add("sampleInfo", info)
+        self.add("gridOrigin", np.asarray(origins, np.float32))
+        self.add("distance", np.concatenate(fields).astype(np.float32))
+        self.add("sample", samples)
 
     def config(self, dt=None, drag=0):  # gpu.py:493-518
         spec = self.spec
@@ -366,7 +400,7 @@ class Engine:
             "drag": drag * self.time_unit,
             "maxOverlap": spec.max_overlap_fraction,
             "historyStep": int(self.history),
-            "meshGeometry": 1,
+            "meshGeometry": int(self.mesh_geometry),
             "sourceSurfaceNormal": 0,
             "periodic": periodic_code(spec.periodic),
             "wallCount": self.wall_count,
@@ -397,9 +431,15 @@ class Engine:
         self.dispatch("countPairs", self.n, e, b)
         self.dispatch("pairScan", self.n, e, b)
         self.dispatch("emitPairs", self.n, e, b)
-        self.dispatch("resetMeshContacts", self.pair_capacity, e, b)
-        self.dispatch("prepareMeshPairs", self.pair_capacity, e, b)
-        self.dispatch("collectMeshFrontiers", 64 * 4096, e, b)
+        if self.mesh_geometry:  # _encode_contacts (gpu.py:532-545)
+            self.dispatch("resetMeshContacts", self.pair_capacity, e, b)
+            self.dispatch("prepareMeshPairs", self.pair_capacity, e, b)
+            self.dispatch("collectMeshFrontiers", 64 * 4096, e, b)
+        else:
+            self.dispatch("countContacts", 64 * self.pair_capacity, e, b)
+            self.dispatch("contactScan", self.pair_capacity, e, b)
+            self.dispatch("contactTotal", 1, e, b)
+            self.dispatch("emitContacts", 64 * self.pair_capacity, e, b)
         if self.wall_count:
             self.dispatch("collectWallContacts", 64 * self.n * self.wall_count, e, b)
         self.dispatch("clearContactStorage", max(self.n, self.hash_size), e, b)

@@ -26,8 +26,10 @@ F32 = np.float32
 class Shape:
     """One grain template in metres, centred at its centre of mass.
 
-    Mirrors the mesh-path fields of the reference `Shape` (`geometry.py:17-31`);
-    the SDF grid of the reference's other geometry path is not carried.
+    Mirrors the reference `Shape` (`geometry.py:17-31`).  The SDF-geometry field
+    (`sdf`, `origin`, `spacing`: values in metres on the grid origin + spacing *
+    index, negative inside) is optional here and made by `with_sdf`; the mesh path
+    does not read it.
     """
 
     id: str
@@ -39,6 +41,9 @@ class Shape:
     inertia: np.ndarray  # unit density; multiply by kg/m^3
     radius: float
     provenance: dict = field(default_factory=dict)
+    sdf: np.ndarray = None
+    origin: np.ndarray = None
+    spacing: float = 0.0
 
     def validate(self):
         for a in (self.vertices, self.samples, self.areas, self.inertia):
@@ -50,6 +55,46 @@ class Shape:
             raise ValueError("Invalid surface samples")
         if np.linalg.eigvalsh(self.inertia).min() <= 0:
             raise ValueError("Inertia is not positive definite")
+
+    def validate_sdf(self):
+        """The reference's field checks (`geometry.py:32-74`): a finite 3-D grid at
+        least 4 wide, negative somewhere and positive somewhere, with an exterior
+        boundary layer."""
+        if self.sdf is None:
+            raise ValueError(f"Shape {self.id} has no SDF grid (make one with geometry.with_sdf)")
+        sdf = np.asarray(self.sdf)
+        if not (np.isfinite(sdf).all() and np.isfinite(self.origin).all()):
+            raise ValueError("Nonfinite shape data")
+        if sdf.ndim != 3 or min(sdf.shape) < 4 or not (sdf.min() < 0 < sdf.max()):
+            raise ValueError("SDF must be a 3D signed field, negative inside")
+        if not self.spacing > 0:
+            raise ValueError("Invalid shape measure")
+        boundary = np.concatenate([sdf[0].ravel(), sdf[-1].ravel(), sdf[:, 0].ravel(), sdf[:, -1].ravel(),
+                                   sdf[:, :, 0].ravel(), sdf[:, :, -1].ravel()])
+        if np.any(boundary <= 0):
+            raise ValueError("SDF exterior padding is not outside the grain")
+
+    def query(self, points):
+        """Trilinear value and exact piecewise gradient of the grid (FP64;
+        `geometry.py:76-103`): +inf outside the grid."""
+        p = np.atleast_2d(np.asarray(points, float))
+        g = (p - self.origin) / self.spacing
+        ijk = np.floor(g).astype(np.int64)
+        valid = np.all((ijk >= 0) & (ijk < np.array(self.sdf.shape) - 1), axis=1)
+        values = np.full(len(p), np.inf)
+        gradients = np.zeros((len(p), 3))
+        if np.any(valid):
+            base, t = ijk[valid], g[valid] - ijk[valid]
+            v, grad = np.zeros(len(base)), np.zeros((len(base), 3))
+            for bits in np.ndindex(2, 2, 2):
+                w = np.where(np.array(bits, bool), t, 1 - t)
+                f = self.sdf[tuple((base + np.array(bits)).T)].astype(float)
+                v += f * w.prod(axis=1)
+                for axis in range(3):
+                    other = [d for d in range(3) if d != axis]
+                    grad[:, axis] += f * (2 * bits[axis] - 1) * w[:, other].prod(axis=1) / self.spacing
+            values[valid], gradients[valid] = v, grad
+        return values, gradients
 
     @property
     def equivalent_diameter(self):
@@ -398,6 +443,47 @@ def _prepare(shape, length_unit):
         raise ValueError("Triangle degenerates in normalized FP32")
     centres = np.ascontiguousarray(tri.mean(axis=1))
     return points, groups, tri, feature, centres
+
+
+def sdf_grid(vertices, faces, resolution=64, *, threads=None):
+    """Signed distance grid of a closed mesh on the reference's grid
+    (`geometry.py:189-215`): spacing = largest extent / (resolution - 1), origin =
+    lower bound - 3 spacings, ceil(extents / spacing) + 7 points per axis.  Values
+    (FP32, metres, negative inside) are exact FP64 distances from the native
+    producer (`sad_sdf_grid`).  Returns (sdf, origin, spacing)."""
+    from . import native
+
+    if resolution < 8:
+        raise ValueError("SDF resolution >= 8 required")
+    vertices = np.asarray(vertices, float)
+    faces = np.asarray(faces, np.int64)
+    _, inverse = _check_manifold(faces)
+    feature = np.ascontiguousarray(_pseudonormals(vertices, faces, inverse))
+    low, high = vertices.min(0), vertices.max(0)
+    extents = high - low
+    spacing = float(extents.max() / (resolution - 1))
+    origin = low - 3 * spacing
+    dims = (np.ceil(extents / spacing).astype(np.int64) + 7).astype(np.uint32)
+    tri = np.ascontiguousarray(vertices[faces])
+    out = np.empty(tuple(int(d) for d in dims), np.float32)
+    threads = max(1, min(16, os.cpu_count() or 1) if threads is None else int(threads))
+    o = np.ascontiguousarray(origin, float)
+    rc = native.packer().sad_sdf_grid(_pointer(tri), _pointer(feature), len(tri), _pointer(o), spacing,
+                                      _pointer(dims), threads, _pointer(out))
+    if rc:
+        raise ValueError(f"SDF grid failed ({rc})")
+    return out, origin, spacing
+
+
+def with_sdf(shape, resolution=64, *, threads=None):
+    """`shape` with its SDF grid (the reference's `from_mesh` field)."""
+    from dataclasses import replace
+
+    sdf, origin, spacing = sdf_grid(shape.vertices, shape.faces, resolution, threads=threads)
+    out = replace(shape, sdf=sdf, origin=origin, spacing=spacing,
+                  provenance=dict(shape.provenance, resolution=resolution, sdf_sign="negative_inside"))
+    out.validate_sdf()
+    return out
 
 
 def _pointer(array, row=0):

@@ -61,7 +61,7 @@ def compile_packer(output=PACK_LIBRARY):
 
 
 _PACK = None
-PACK_EXPORTED = ("sad_pack_node_count", "sad_pack_bvh", "sad_pack_clusters")
+PACK_EXPORTED = ("sad_pack_node_count", "sad_pack_bvh", "sad_pack_clusters", "sad_sdf_grid")
 
 
 def packer():
@@ -77,6 +77,8 @@ def packer():
         lib.sad_pack_bvh.argtypes = [vp, vp, u32, u32, u32, vp, vp, vp, vp, ct.POINTER(u32)]
         lib.sad_pack_clusters.restype = ct.c_int
         lib.sad_pack_clusters.argtypes = [vp, u32, u32, u32, vp, vp, vp, vp, vp]
+        lib.sad_sdf_grid.restype = ct.c_int
+        lib.sad_sdf_grid.argtypes = [vp, vp, u32, vp, ct.c_double, vp, u32, vp]
         _PACK = lib
     return _PACK
 

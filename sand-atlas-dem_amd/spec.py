@@ -180,9 +180,8 @@ class Experiment:
     def viscous(self):
         return self.normal_viscosity > 0 or any(p.get("normal_viscosity", 0) > 0 for p in self.planes)
 
-    def require_mesh_path(self):
-        """The `hip` backend implements the surface-mesh step (with or without planes)."""
-        if self.contact_geometry != "mesh":
-            raise NotImplementedError("The hip backend implements contact_geometry='mesh' only")
+    def require_supported_path(self):
+        """The `hip` backend implements the surface-mesh and SDF-grid steps (with or
+        without planes) with target-gradient contact normals."""
         if self.contact_normal != "target-gradient":
             raise NotImplementedError("The hip backend implements target-gradient contact normals only")

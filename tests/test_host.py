@@ -140,7 +140,7 @@ def test_c_abi_library_exports_every_declared_symbol(pkg):
 def test_packer_library_exports_every_declared_symbol(pkg):
     """The host mesh packer exports the entry points its header declares."""
     header = (ROOT / "include" / "sand_dem_pack.h").read_text()
-    declared = set(re.findall(r"\b(sad_pack_\w+)\s*\(", header))
+    declared = set(re.findall(r"\b(sad_(?:pack|sdf)_\w+)\s*\(", header))
     assert declared == set(pkg.native.PACK_EXPORTED)
     lib = ct.CDLL(str(pkg.native.PACK_LIBRARY))
     for name in declared:
@@ -169,6 +169,9 @@ def test_backend_selection_is_explicit(pkg):
 def test_engine_rejects_unsupported_paths(pkg, oracle_module, box):
     cpu = oracle_module.make_oracle(pkg)
     with pytest.raises(NotImplementedError):
+        pkg.Engine([box], pkg.Experiment(contact_normal="source-surface-experimental", friction=0, timestep=1e-9),
+                   [[0.005] * 3], [0.01] * 3, [0], [1], backend=cpu)
+    with pytest.raises(ValueError, match="no SDF grid"):  # the SDF geometry needs each template's grid
         pkg.Engine([box], pkg.Experiment(contact_geometry="sdf", timestep=1e-9), [[0.005] * 3], [0.01] * 3, [0], [1],
                    backend=cpu)
     with pytest.raises(ValueError, match="Timestep"):
