@@ -36,6 +36,8 @@ WORKLOADS = {
              "down at 0.5 box heights/s, mu 0.5",
     "layer": "10,000 grains from the dense library at phi 0.58 between two z planes, x and y periodic, top plane "
              "moving down at 0.5 layer heights/s, mu 0.5",
+    "sdf": "the dense assembly with SDF-grid contact geometry (grids at resolution 64, trilinear field): 10,000 "
+           "grains, 64 perturbed icospheres, 2,048 nodes/grain, phi 0.60, isotropic compression 0.5/s, mu 0.5",
 }
 
 
@@ -50,7 +52,7 @@ def load_package():
     return load()
 
 
-def kernel_bytes(name, n, pd, q, c, hash_size):
+def kernel_bytes(name, n, pd, q, c, hash_size, sdf_static=0):
     """Algorithmic HBM bytes per launch (per-unit figures in DESIGN.md)."""
     table = {
         "k_collect": 92 * pd + 20 * q + 48 * c,
@@ -61,6 +63,11 @@ def kernel_bytes(name, n, pd, q, c, hash_size):
         "k_clear_storage": 4 * hash_size,
         "k_begin_step": 208 * n,
         "k_finish_step": 160 * n,
+        # SDF geometry: per directed pair its rows (pair, ids, bodies, poses, grid rows)
+        # and its 256-B hit bitmask (2,048 nodes), the templates' nodes and grids once
+        # (they stay in the caches), per contact its rows.
+        "k_sdf_count": 376 * pd + sdf_static,
+        "k_sdf_emit": 376 * pd + 48 * c,
     }
     return table.get(name)
 
@@ -191,7 +198,8 @@ def main():
     hip.wait()
     log(f"[bench] warmup {args.warmup} steps done")
     tuned = None
-    if not args.no_autotune:
+    sdf = not e.mesh_geometry  # no collection launch shape to tune, no learned cull
+    if not args.no_autotune and not sdf:
         tuned = autotune.autotune(e, rate=w.rate, steps=6, log=log)
         log(f"[bench] autotuned launch shape {tuned['best']} ({tuned['ms_per_step']:.3f} ms/step)")
     start, step0 = e.state(), e.step
@@ -222,10 +230,11 @@ def main():
     pd = (after["directed_pairs"] - before["directed_pairs"]) / steps_p
     q = (after["queried_nodes"] - before["queried_nodes"]) / steps_p
     c = (after["contacts"] - before["contacts"]) / steps_p
+    sdf_static = sum(16 * len(s.samples) + 4 * s.sdf.size for s in w.shapes) if sdf else 0
     kernels = {}
     for name, (ms, launches) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
         avg = ms / launches
-        b = kernel_bytes(name, n, pd, q, c, e.hash_size)
+        b = kernel_bytes(name, n, pd, q, c, e.hash_size, sdf_static)
         kernels[name] = {
             "avg_ms": round(avg, 5),
             "share": round(ms / sum(v[0] for v in prof.values()), 4),
@@ -233,9 +242,9 @@ def main():
         }
     dominant = max(prof, key=lambda k: prof[k][0])
     dom_ms = prof[dominant][0] / prof[dominant][1]
-    dom_bytes = kernel_bytes(dominant, n, pd, q, c, e.hash_size) or 0
+    dom_bytes = kernel_bytes(dominant, n, pd, q, c, e.hash_size, sdf_static) or 0
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    step_bytes = 296 * n + 92 * pd + 20 * q + 240 * c
+    step_bytes = 296 * n + (752 * pd + sdf_static if sdf else 92 * pd + 20 * q) + 240 * c
     log(f"[bench] dominant kernel {dominant}: {dom_ms:.3f} ms/launch; P_d {pd:.0f} Q {q:.0f} C {c:.0f} per step")
 
     cpu_baseline = None
@@ -273,7 +282,7 @@ def main():
         }
         log(f"[bench] cpu baseline {cpu_baseline['value']:.4g} grain-steps/s on {threads} threads; parity {parity}")
 
-    nn = None if args.no_nn else nn_leg(args, w, e, hip, start, step0, plain_state, log)
+    nn = None if args.no_nn or sdf else nn_leg(args, w, e, hip, start, step0, plain_state, log)
     if nn and nn.get("value"):
         nn["speedup_vs_plain"] = nn["value"] / value
 

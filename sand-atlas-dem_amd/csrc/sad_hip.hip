@@ -125,6 +125,9 @@ struct W {
     uint32_t nodeStride;  // bytes per task (>= nodes of any template, multiple of 64)
     uint32_t skip;        // 1 = on
     float skipLimit;      // re-anchor once the motion bound reaches this (template units)
+    float* sdfReach;      // SDF geometry, per template: squared radius beyond which the field is >= 0
+    uint32_t* sdfHits;    // SDF geometry, per directed task: bitmask of its contact nodes (count pass)
+    uint32_t sdfWords;    // 32-bit words per task in sdfHits (>= largest node count / 32, even)
     uint32_t* wallCount;  // plane contacts per (wall, grain) task, task = wall * N + grain
     uint32_t* wallOffset; // exclusive scan of wallCount (ids follow the mesh contacts)
     // Learned cull (nn.h): per-template BF16 weights and FP32 blocks; stats on separate lines.
@@ -1699,61 +1702,163 @@ __device__ v4 sdfSample(const P& p, uint32_t s, v3 q) {
             }
     return {grad.x, grad.y, grad.z, value};
 }
-// nodeContact (solver.slang:319-343), SDF branch.
-__device__ bool sdfNodeContact(const P& p, uint32_t src, uint32_t dst, uint32_t node, float4& pd, float4& na) {
-    const float4 bs = p.body[src], bd = p.body[dst];
-    const float4 sp = p.sample[p.sampleInfo[p.shapeId[src]].x + node];
-    const v4 qj = v4of(p.rotation[dst]);
-    const v3 arm = qrot(v4of(p.rotation[src]), xyz(sp) * bs.x);
-    const v3 local = qinv(qj, arm - centerBranch(p, src, dst)) / bd.x;
-    const v4 f = sdfSample(p, p.shapeId[dst], local);
-    const float depth = -f.w * bd.x;
+// Per template, the squared radius (template units) outside which every grid cell
+// has only nonnegative corners: the largest |corner| over negative grid values, plus
+// two spacings (more than a cell diagonal, covering the rounding of the cell index).
+// Trilinear weights are nonnegative, so beyond it the field is >= 0 and no node can
+// be in contact; no value below zero gives -1 (nothing reaches).
+__global__ __launch_bounds__(256) void k_sdf_reach(P p, W w, uint32_t* largest) {
+    __shared__ float best[256];
+    const uint32_t shape = blockIdx.x;
+    if (threadIdx.x == 0) atomicMax(largest, p.sampleInfo[shape].y);
+    const uint4 meta = p.grids[shape];
+    const float4 o = p.gridOrigin[shape];
+    const uint64_t count = uint64_t(meta.y) * meta.z * meta.w;
+    float r = -1.0f;
+    for (uint64_t k = threadIdx.x; k < count; k += 256) {
+        const uint64_t index = meta.x + k;
+        if (index >= p.distanceCount || !(p.distance[index] < 0.0f)) continue;
+        const uint32_t ix = uint32_t(k / (uint64_t(meta.z) * meta.w)), rest = uint32_t(k % (uint64_t(meta.z) * meta.w));
+        const v3 g = xyz(o) + o.w * mk3(float(ix), float(rest / meta.w), float(rest % meta.w));
+        r = fmaxf(r, length(g));
+    }
+    best[threadIdx.x] = r;
+    __syncthreads();
+    for (uint32_t step = 128; step > 0; step >>= 1) {
+        if (threadIdx.x < step) best[threadIdx.x] = fmaxf(best[threadIdx.x], best[threadIdx.x + step]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float reach = best[0] < 0.0f ? -1.0f : (best[0] + 2.0f * o.w) * 1.0001f;
+        w.sdfReach[shape] = reach < 0.0f ? -1.0f : reach * reach;
+    }
+}
+// nodeContact (solver.slang:319-343), SDF branch, after the node's local point.
+__device__ bool sdfNodeContact(const P& p, uint32_t targetShape, v3 arm, v3 local, v4 qj, float area, float si,
+                               float wi, float sj, float wj, float4& pd, float4& na) {
+    const v4 f = sdfSample(p, targetShape, local);
+    const float depth = -f.w * sj;
     pd = f4(arm, depth);
     na = make_float4(0, 0, 0, 0);
     if (depth <= 0) return false;
     const v3 grad = qrot(qj, xyz(f));
     const float len = length(grad);
     if (len < 1e-7f || !isfinite(depth)) { fail(p, 2); return false; }
-    na = f4(grad / len, sp.w * bs.x * bs.x);
-    if (depth > p.c.maxOverlap * fminf(bs.w, bd.w)) fail(p, 4);
+    na = f4(grad / len, area * si * si);
+    if (depth > p.c.maxOverlap * fminf(wi, wj)) fail(p, 4);
     return true;
 }
 // countContacts / emitContacts in the CPU's order (cpu_scheduling.slang:102-131):
-// one wave per pair takes both grains' nodes (s < a: the first grain's) 64 at a
-// time; hits are ranked by ballot, so they are written in node order.  The counts
-// and offsets of the two directions (first grain as source, then the second) also
-// go to the per-direction task rows the gather walks, as after a mesh collection.
+// one wave per pair takes the first grain's nodes (source, the partner the target),
+// then the second's; contacts are written in node order.
+//
+// Count pass: a node whose local point lies beyond the target's reach is not in
+// contact: |local| = s_i |p - R_i^T b / s_i| / s_j (rotations keep lengths), so the
+// screen costs one subtraction and a dot product per node, with a 1e-4 relative
+// margin over the FP32 error of both evaluations.  Survivors are queued in node order
+// in LDS and evaluated exactly 64 at a time (the node's local point as the reference
+// computes it, then the field); each direction's contact nodes are recorded as a
+// bitmask.  Emit pass: only the recorded nodes are evaluated again (the evaluation is
+// deterministic, so the same nodes hit with the same values), 64 node ids per step,
+// ranked by ballot.  The counts and offsets of the two directions also go to the
+// per-direction task rows the gather walks, as after a mesh collection.
+constexpr uint32_t SDF_QUEUE = 128, SDF_MASK_WORDS = 128;  // LDS per wave: queue, bitmask (4,096 nodes)
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
+    __shared__ uint32_t queueMem[4][SDF_QUEUE];
+    __shared__ uint32_t maskMem[4][SDF_MASK_WORDS];
     const int lane = threadIdx.x & 63;
+    uint32_t* queue = queueMem[threadIdx.x >> 6];
+    uint32_t* bits = maskMem[threadIdx.x >> 6];
     const uint32_t pair = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (pair >= p.c.pairCapacity) return;
     const bool live = pair < p.control[1] && ld_volatile(p.control) == 0;
     if (EMIT && !live) return;
-    uint32_t count = 0, forward = 0;  // all hits, hits with the first grain as source
+    uint32_t count = 0, forward = 0, evaluated = 0;  // hits, hits of the first direction, nodes
     if (live) {
         const uint2 ij = p.pairs[pair];
-        const uint32_t a = p.sampleInfo[p.shapeId[ij.x]].y, total = a + p.sampleInfo[p.shapeId[ij.y]].y;
         const uint32_t base = EMIT ? p.contactOffsets[pair] : 0;
-        for (uint32_t first = 0; first < total; first += 64) {
-            const uint32_t s = first + lane;
-            const bool mine = s < a;
-            const uint32_t i = mine ? ij.x : ij.y, j = mine ? ij.y : ij.x, node = mine ? s : s - a;
-            float4 pd, na;
-            const bool hit = s < total && sdfNodeContact(p, i, j, node, pd, na);
-            const uint64_t mask = __ballot(hit);
-            if (EMIT && hit) {
-                const uint32_t id = base + count + __popcll(mask & lanes_below(lane));
-                if (id < p.c.contactCapacity) {
-                    p.meta[id] = make_uint4(i, j, node, 0);
-                    p.pointDepth[id] = pd;
-                    p.normalArea[id] = na;
-                } else {
-                    fail(p, 32);
+        const uint32_t words = w.sdfWords;
+        for (uint32_t pass = 0; pass < 2; pass++) {
+            const uint32_t i = pass ? ij.y : ij.x, j = pass ? ij.x : ij.y;
+            const uint32_t targetShape = p.shapeId[j];
+            const uint4 info = p.sampleInfo[p.shapeId[i]];
+            const float4 bi = p.body[i], bj = p.body[j];
+            const v4 qi = v4of(p.rotation[i]), qj = v4of(p.rotation[j]);
+            const v3 branch = centerBranch(p, i, j);
+            uint32_t* record = w.sdfHits + size_t(2 * pair + pass) * words;
+            evaluated += info.y;
+            // Exact evaluation of one node per active lane; hits ranked by ballot.
+            auto evaluate = [&](bool active, uint32_t node) {
+                bool hit = false;
+                float4 pd, na;
+                if (active) {
+                    const float4 sp = p.sample[info.x + node];
+                    const v3 arm = qrot(qi, xyz(sp) * bi.x);
+                    const v3 local = qinv(qj, arm - branch) / bj.x;
+                    hit = sdfNodeContact(p, targetShape, arm, local, qj, sp.w, bi.x, bi.w, bj.x, bj.w, pd, na);
                 }
+                const uint64_t mask = __ballot(hit);
+                if (EMIT && hit) {
+                    const uint32_t id = base + count + __popcll(mask & lanes_below(lane));
+                    if (id < p.c.contactCapacity) {
+                        p.meta[id] = make_uint4(i, j, node, 0);
+                        p.pointDepth[id] = pd;
+                        p.normalArea[id] = na;
+                    } else {
+                        fail(p, 32);
+                    }
+                }
+                if (!EMIT && hit) atomicOr(bits + node / 32, 1u << (node % 32));
+                count += __popcll(mask);
+            };
+            // Node-ordered queue of candidates, evaluated 64 at a time.
+            uint32_t queued = 0;
+            auto push = [&](bool keep, uint32_t node) {
+                const uint64_t mask = __ballot(keep);
+                if (keep) queue[queued + __popcll(mask & lanes_below(lane))] = node;
+                queued += __popcll(mask);
+                wave_sync();
+                if (queued >= 64) {
+                    evaluate(true, queue[lane]);
+                    wave_sync();
+                    const uint32_t rest = queued - 64;
+                    const uint32_t moved = uint32_t(lane) < rest ? queue[64 + lane] : 0;
+                    wave_sync();
+                    if (uint32_t(lane) < rest) queue[lane] = moved;
+                    queued = rest;
+                    wave_sync();
+                }
+            };
+            if (EMIT) {
+                for (uint32_t first = 0; first < info.y && first / 32 < words; first += 64) {
+                    const uint64_t set = uint64_t(record[first / 32]) | (uint64_t(record[first / 32 + 1]) << 32);
+                    if (set) push((set >> lane) & 1u, first + lane);
+                }
+                if (queued) evaluate(uint32_t(lane) < queued, uint32_t(lane) < queued ? queue[lane] : 0);
+                wave_sync();
+            } else {
+                const v3 centre = qinv(qi, branch) / bi.x;  // the target's centre in source template units
+                const float bound = w.sdfReach[targetShape] * (bj.x * bj.x) * 1.0001f, scale = bi.x * bi.x;
+                const bool recorded = info.y <= 32 * min(words, SDF_MASK_WORDS);
+                for (uint32_t k = lane; k < words && k < SDF_MASK_WORDS; k += 64) bits[k] = 0;
+                wave_sync();
+                for (uint32_t first = 0; first < info.y; first += 64) {
+                    const uint32_t node = first + lane;
+                    bool keep = false;
+                    if (node < info.y) {
+                        const v3 d = xyz(p.sample[info.x + node]) - centre;
+                        keep = !(scale * dot(d, d) > bound);
+                    }
+                    push(keep, node);
+                }
+                if (queued) evaluate(uint32_t(lane) < queued, uint32_t(lane) < queued ? queue[lane] : 0);
+                wave_sync();
+                if (!recorded) fail(p, 2);  // more nodes than the bitmask rows hold
+                for (uint32_t k = lane; k < words; k += 64) record[k] = k < SDF_MASK_WORDS ? bits[k] : 0;
+                wave_sync();
             }
-            count += __popcll(mask);
-            forward += __popcll(__ballot(hit && mine));
+            if (pass == 0) forward = count;
         }
         if (EMIT && lane == 0) {
             w.taskOffset[2 * pair] = base;
@@ -1761,6 +1866,7 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
         }
     }
     if (!EMIT && lane == 0) {
+        if (live) atomicAdd(w.totals + 2, (unsigned long long)evaluated);  // the "queried nodes" statistic
         p.contactCounts[pair] = count;
         w.taskCount[2 * pair] = forward;
         w.taskCount[2 * pair + 1] = count - forward;
@@ -2524,7 +2630,7 @@ int grow(T*& ptr, uint32_t& cap, size_t need, size_t& bytes) {
 struct Caps {
     uint32_t counters = 0, taskCount = 0, taskOffset = 0, adjCount = 0, adjOffset = 0, adjCursor = 0, adjList = 0;
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
-    uint32_t stageCount = 0, keyCount = 0, keyStart = 0, wallCount = 0, wallOffset = 0;
+    uint32_t stageCount = 0, keyCount = 0, keyStart = 0, wallCount = 0, wallOffset = 0, sdfReach = 0, sdfHits = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -2558,6 +2664,32 @@ struct ScratchImpl : Scratch {
         graphs.clear();
     }
     int ensure_nodes(uint32_t nodes) { return grow(w.wide, caps.wide, 4 * size_t(std::max(nodes, 1u)), bytes); }
+    // SDF geometry: per-template reach and the hit-bitmask rows, refreshed whenever
+    // the geometry epoch changes (an upload, or new bindings); outside any capture.
+    uint32_t reachEpoch = 0, reachShapes = 0;
+    int ensure_sdf(P& p, hipStream_t s) {
+        if (p.c.meshGeometry) return SAD_OK;
+        int rc = grow(w.sdfReach, caps.sdfReach, std::max(p.c.shapes, 1u), bytes);
+        if (rc) return rc;
+        if (reachEpoch != epoch || reachShapes != p.c.shapes) {
+            if (!probeWord) HIP_OK(hipMalloc(reinterpret_cast<void**>(&probeWord), 3 * sizeof(uint32_t)));
+            HIP_OK(hipMemsetAsync(probeWord, 0, sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_sdf_reach, dim3(std::max(p.c.shapes, 1u)), dim3(256), 0, s, p, w, probeWord);
+            HIP_OK(hipGetLastError());
+            uint32_t largest = 0;
+            HIP_OK(hipMemcpyAsync(&largest, probeWord, sizeof(largest), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            if (largest > 32 * SDF_MASK_WORDS) {
+                std::snprintf(g_error, sizeof(g_error), "SDF geometry: a template has %u nodes (at most %u)", largest,
+                              32 * SDF_MASK_WORDS);
+                return SAD_E_UNSUPPORTED;
+            }
+            w.sdfWords = std::max(2u, (largest + 63) / 64 * 2);
+            reachEpoch = epoch;
+            reachShapes = p.c.shapes;
+        }
+        return grow(w.sdfHits, caps.sdfHits, size_t(2) * std::max(p.c.pairCapacity, 1u) * w.sdfWords, bytes);
+    }
     int ensure_keys(uint32_t templates) {
         const uint32_t keys = bucketShape(templates).keys();
         int rc = grow(w.keyCount, caps.keyCount, keys, bytes);
@@ -2626,7 +2758,8 @@ struct ScratchImpl : Scratch {
                                     SAD_MESH_HULL_INFO, SAD_MESH_PLANES, SAD_MESH_MIN, SAD_MESH_MAX, SAD_MESH_LINKS,
                                     SAD_MESH_TRIANGLES, SAD_MESH_NORMALS, SAD_MESH_CLUSTER_INFO,
                                     SAD_MESH_CLUSTER_RANGES, SAD_MESH_CLUSTER_MIN, SAD_MESH_CLUSTER_MAX,
-                                    SAD_MESH_SAMPLE_INDICES, SAD_POSITION};
+                                    SAD_MESH_SAMPLE_INDICES, SAD_GRIDS, SAD_GRID_ORIGIN, SAD_DISTANCE,
+                                    SAD_POSITION};
         std::vector<uint64_t> key;
         key.reserve(2 * (sizeof(slots) / sizeof(slots[0])) + 1);
         for (int k : slots) {
@@ -2721,7 +2854,7 @@ struct ScratchImpl : Scratch {
         for (hipEvent_t e : pool) (void)hipEventDestroy(e);
         (void)hipFree(w.totals);
         (void)hipFree(w.counters); (void)hipFree(w.taskCount); (void)hipFree(w.taskOffset); (void)hipFree(w.adjCount);
-        (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums);
+        (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums); (void)hipFree(w.sdfReach); (void)hipFree(w.sdfHits);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
         (void)hipFree(w.anchor); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
@@ -3147,6 +3280,7 @@ int sad_hip_dispatch(const char* kernel, uint32_t count, void* const* buffers, c
         auto* sc = static_cast<ScratchImpl*>(scratch);
         if ((rc = sc->ensure(p.c))) return rc;
         if ((rc = sc->ensure_skip(p, buffers, bytes))) return rc;
+        if ((rc = sc->ensure_sdf(p, static_cast<hipStream_t>(stream)))) return rc;
         if ((rc = sc->sync_epoch(static_cast<hipStream_t>(stream)))) return rc;
         return run_kernel(kernel, p, *sc, static_cast<hipStream_t>(stream));
     } catch (...) {
@@ -3271,6 +3405,7 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const
             if ((rc = sc->ensure_nodes(p.meshNodes))) return rc;
             if (p.templates <= MAX_TEMPLATES && (rc = sc->ensure_keys(p.templates))) return rc;
             if ((rc = sc->ensure_skip(p, table, sizes))) return rc;
+            if ((rc = sc->ensure_sdf(p, s))) return rc;  // no allocation inside a captured step
             if ((rc = sc->sync_epoch(s))) return rc;
             if (graph && !sc->profiling) {
                 sc->w.pop = sc->pop;
