@@ -98,7 +98,12 @@ const char* sad_hip_kernel_name(uint32_t kernel);
  * Supported names: beginStep advanceCell checkRebuild clearBins fillBins
  * countPairs pairScan emitPairs resetMeshContacts prepareMeshPairs
  * collectMeshFrontiers clearContactStorage resolveContacts buildHistoryHash
- * gather finishStep captureContactKinematics checkContactTimestep. */
+ * gather finishStep captureContactKinematics checkContactTimestep advanceWalls
+ * collectWallContacts gatherWallRows refreshWallPower diagnosticRows fabricRows,
+ * and for the SDF geometry (values[SAD_V_MESH_GEOMETRY] == 0) countContacts
+ * contactTotal emitContacts.  "pairScan" and "contactScan" are the reference
+ * Scan's exclusive scans (gpu.py:553-555, 539-541) of pairCounts and
+ * contactCounts as single names. */
 int sad_hip_dispatch(const char* kernel, uint32_t count, void* const* buffers,
                      const uint64_t* bytes, const double* values, void* scratch,
                      void* stream);
