@@ -1673,14 +1673,41 @@ __global__ void k_scatter(P p, W w) {
 // sdf: the template grid's trilinear value and piecewise gradient, 1e20 outside the
 // grid; lerp(x, y, s) is x + s (y - x).  A corner index beyond the distance buffer
 // (inconsistent grid rows) raises flag 2 and reads as outside.
-__device__ v4 sdfSample(const P& p, uint32_t s, v3 q) {
+// Split in two so a caller can issue the corner loads of several points before it
+// uses any: sdfLoad fetches the 8 corners (in-bounds reads only; the cell is taken
+// only for points inside the grid), sdfFinish forms the value and gradient.
+struct SdfCorners {
+    float f[8];
+    v3 g;
+    float spacing;
+    bool inside, ok;
+};
+__device__ SdfCorners sdfLoad(const P& p, uint32_t s, v3 q) {
+    SdfCorners c;
     const uint4 meta = p.grids[s];
     const float4 o = p.gridOrigin[s];
-    const v3 g = (q - xyz(o)) / o.w;
+    c.g = (q - xyz(o)) / o.w;
+    c.spacing = o.w;
+    c.inside = c.g.x >= 0.0f && c.g.y >= 0.0f && c.g.z >= 0.0f && !(c.g.x >= float(int64_t(meta.y) - 1)) &&
+               !(c.g.y >= float(int64_t(meta.z) - 1)) && !(c.g.z >= float(int64_t(meta.w) - 1));
+    const uint32_t ax = uint32_t(int(floorf(c.inside ? c.g.x : 0.0f)));
+    const uint32_t ay = uint32_t(int(floorf(c.inside ? c.g.y : 0.0f)));
+    const uint32_t az = uint32_t(int(floorf(c.inside ? c.g.z : 0.0f)));
+    c.ok = true;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) {
+        const uint32_t index = meta.x + ((ax + (k >> 2)) * meta.z + (ay + ((k >> 1) & 1u))) * meta.w + (az + (k & 1u));
+        const bool fits = index < p.distanceCount;
+        c.ok = c.ok && fits;
+        c.f[k] = p.distance[fits ? index : 0u];
+    }
+    return c;
+}
+__device__ v4 sdfFinish(const P& p, const SdfCorners& c) {
     const v4 outside = {0, 0, 0, 1e20f};
-    if (!(g.x >= 0.0f) || !(g.y >= 0.0f) || !(g.z >= 0.0f) || g.x >= float(int64_t(meta.y) - 1) ||
-        g.y >= float(int64_t(meta.z) - 1) || g.z >= float(int64_t(meta.w) - 1))
-        return outside;
+    if (!c.inside) return outside;
+    if (!c.ok) { fail(p, 2); return outside; }
+    const v3 g = c.g;
     const int ax = int(floorf(g.x)), ay = int(floorf(g.y)), az = int(floorf(g.z));
     const v3 t = {g.x - float(ax), g.y - float(ay), g.z - float(az)};
     const v3 x = mk3(1.0f, 1.0f, 1.0f) - t;
@@ -1691,17 +1718,15 @@ __device__ v4 sdfSample(const P& p, uint32_t s, v3 q) {
             for (uint32_t k = 0; k < 2; k++) {
                 const v3 b = mk3(float(i), float(j), float(k));
                 const v3 w = x + b * (t - x);
-                const uint32_t index =
-                    meta.x + ((uint32_t(ax) + i) * meta.z + (uint32_t(ay) + j)) * meta.w + (uint32_t(az) + k);
-                if (index >= p.distanceCount) { fail(p, 2); return outside; }
-                const float f = p.distance[index];
+                const float f = c.f[4 * i + 2 * j + k];
                 value += f * w.x * w.y * w.z;
                 const v3 d = {(2.0f * float(i) - 1.0f) * w.y * w.z, (2.0f * float(j) - 1.0f) * w.x * w.z,
                               (2.0f * float(k) - 1.0f) * w.x * w.y};
-                grad = grad + f * d / o.w;
+                grad = grad + f * d / c.spacing;
             }
     return {grad.x, grad.y, grad.z, value};
 }
+__device__ v4 sdfSample(const P& p, uint32_t s, v3 q) { return sdfFinish(p, sdfLoad(p, s, q)); }
 // Per template, the squared radius (template units) outside which every grid cell
 // has only nonnegative corners: the largest |corner| over negative grid values, plus
 // two spacings (more than a cell diagonal, covering the rounding of the cell index).
@@ -1734,9 +1759,9 @@ __global__ __launch_bounds__(256) void k_sdf_reach(P p, W w, uint32_t* largest) 
     }
 }
 // nodeContact (solver.slang:319-343), SDF branch, after the node's local point.
-__device__ bool sdfNodeContact(const P& p, uint32_t targetShape, v3 arm, v3 local, v4 qj, float area, float si,
+__device__ bool sdfNodeContact(const P& p, const SdfCorners& corners, v3 arm, v4 qj, float area, float si,
                                float wi, float sj, float wj, float4& pd, float4& na) {
-    const v4 f = sdfSample(p, targetShape, local);
+    const v4 f = sdfFinish(p, corners);
     const float depth = -f.w * sj;
     pd = f4(arm, depth);
     na = make_float4(0, 0, 0, 0);
@@ -1762,7 +1787,7 @@ __device__ bool sdfNodeContact(const P& p, uint32_t targetShape, v3 arm, v3 loca
 // deterministic, so the same nodes hit with the same values), 64 node ids per step,
 // ranked by ballot.  The counts and offsets of the two directions also go to the
 // per-direction task rows the gather walks, as after a mesh collection.
-constexpr uint32_t SDF_QUEUE = 128, SDF_MASK_WORDS = 128;  // LDS per wave: queue, bitmask (4,096 nodes)
+constexpr uint32_t SDF_QUEUE = 192, SDF_MASK_WORDS = 128;  // LDS per wave: queue, bitmask (4,096 nodes)
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
     __shared__ uint32_t queueMem[4][SDF_QUEUE];
@@ -1789,41 +1814,55 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
             uint32_t* record = w.sdfHits + size_t(2 * pair + pass) * words;
             evaluated += info.y;
             // Exact evaluation of one node per active lane; hits ranked by ballot.
-            auto evaluate = [&](bool active, uint32_t node) {
-                bool hit = false;
-                float4 pd, na;
-                if (active) {
-                    const float4 sp = p.sample[info.x + node];
-                    const v3 arm = qrot(qi, xyz(sp) * bi.x);
-                    const v3 local = qinv(qj, arm - branch) / bj.x;
-                    hit = sdfNodeContact(p, targetShape, arm, local, qj, sp.w, bi.x, bi.w, bj.x, bj.w, pd, na);
+            // Exact evaluation of queue entries [0, n), n <= 128, two per lane (entries
+            // lane and 64 + lane): both points' corner loads are issued before either
+            // is used.  A lane without a second entry repeats entry 0 (same flags) and
+            // discards it.  Hits are ranked by ballot in entry order.
+            auto evaluate = [&](uint32_t n) {
+                uint32_t node[2];
+                v3 arm[2];
+                float area[2];
+                SdfCorners corners[2];
+#pragma unroll
+                for (uint32_t u = 0; u < 2; u++) {
+                    const uint32_t e = 64 * u + lane;
+                    node[u] = queue[e < n ? e : 0u];
+                    const float4 sp = p.sample[info.x + node[u]];
+                    arm[u] = qrot(qi, xyz(sp) * bi.x);
+                    area[u] = sp.w;
+                    corners[u] = sdfLoad(p, targetShape, qinv(qj, arm[u] - branch) / bj.x);
                 }
-                const uint64_t mask = __ballot(hit);
-                if (EMIT && hit) {
-                    const uint32_t id = base + count + __popcll(mask & lanes_below(lane));
-                    if (id < p.c.contactCapacity) {
-                        p.meta[id] = make_uint4(i, j, node, 0);
-                        p.pointDepth[id] = pd;
-                        p.normalArea[id] = na;
-                    } else {
-                        fail(p, 32);
+                for (uint32_t u = 0; u < 2 && 64 * u < n; u++) {
+                    float4 pd, na;
+                    const bool hit = sdfNodeContact(p, corners[u], arm[u], qj, area[u], bi.x, bi.w, bj.x, bj.w, pd, na) &&
+                                     64 * u + lane < n;
+                    const uint64_t mask = __ballot(hit);
+                    if (EMIT && hit) {
+                        const uint32_t id = base + count + __popcll(mask & lanes_below(lane));
+                        if (id < p.c.contactCapacity) {
+                            p.meta[id] = make_uint4(i, j, node[u], 0);
+                            p.pointDepth[id] = pd;
+                            p.normalArea[id] = na;
+                        } else {
+                            fail(p, 32);
+                        }
                     }
+                    if (!EMIT && hit) atomicOr(bits + node[u] / 32, 1u << (node[u] % 32));
+                    count += __popcll(mask);
                 }
-                if (!EMIT && hit) atomicOr(bits + node / 32, 1u << (node % 32));
-                count += __popcll(mask);
             };
-            // Node-ordered queue of candidates, evaluated 64 at a time.
+            // Node-ordered queue of candidates, evaluated 128 at a time.
             uint32_t queued = 0;
             auto push = [&](bool keep, uint32_t node) {
                 const uint64_t mask = __ballot(keep);
                 if (keep) queue[queued + __popcll(mask & lanes_below(lane))] = node;
                 queued += __popcll(mask);
                 wave_sync();
-                if (queued >= 64) {
-                    evaluate(true, queue[lane]);
+                if (queued >= 128) {
+                    evaluate(128);
                     wave_sync();
-                    const uint32_t rest = queued - 64;
-                    const uint32_t moved = uint32_t(lane) < rest ? queue[64 + lane] : 0;
+                    const uint32_t rest = queued - 128;
+                    const uint32_t moved = uint32_t(lane) < rest ? queue[128 + lane] : 0;
                     wave_sync();
                     if (uint32_t(lane) < rest) queue[lane] = moved;
                     queued = rest;
@@ -1831,11 +1870,17 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
                 }
             };
             if (EMIT) {
+                // The direction's bitmask (<= 128 words) is held two words per lane.
+                const uint32_t low = uint32_t(lane) < words ? record[lane] : 0u;
+                const uint32_t high = uint32_t(lane) + 64 < words ? record[lane + 64] : 0u;
                 for (uint32_t first = 0; first < info.y && first / 32 < words; first += 64) {
-                    const uint64_t set = uint64_t(record[first / 32]) | (uint64_t(record[first / 32 + 1]) << 32);
+                    const uint32_t k = first / 32;  // even
+                    const uint32_t w0 = __shfl(k < 64 ? low : high, int(k % 64));
+                    const uint32_t w1 = __shfl(k + 1 < 64 ? low : high, int((k + 1) % 64));
+                    const uint64_t set = uint64_t(w0) | (uint64_t(w1) << 32);
                     if (set) push((set >> lane) & 1u, first + lane);
                 }
-                if (queued) evaluate(uint32_t(lane) < queued, uint32_t(lane) < queued ? queue[lane] : 0);
+                if (queued) evaluate(queued);
                 wave_sync();
             } else {
                 const v3 centre = qinv(qi, branch) / bi.x;  // the target's centre in source template units
@@ -1843,16 +1888,20 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
                 const bool recorded = info.y <= 32 * min(words, SDF_MASK_WORDS);
                 for (uint32_t k = lane; k < words && k < SDF_MASK_WORDS; k += 64) bits[k] = 0;
                 wave_sync();
-                for (uint32_t first = 0; first < info.y; first += 64) {
-                    const uint32_t node = first + lane;
-                    bool keep = false;
-                    if (node < info.y) {
-                        const v3 d = xyz(p.sample[info.x + node]) - centre;
-                        keep = !(scale * dot(d, d) > bound);
+                for (uint32_t first = 0; first < info.y; first += 256) {  // four node rows in flight per lane
+                    bool keep[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; u++) {
+                        const uint32_t node = first + 64 * u + lane;
+                        keep[u] = false;
+                        if (node < info.y) {
+                            const v3 d = xyz(p.sample[info.x + node]) - centre;
+                            keep[u] = !(scale * dot(d, d) > bound);
+                        }
                     }
-                    push(keep, node);
+                    for (uint32_t u = 0; u < 4 && first + 64 * u < info.y; u++) push(keep[u], first + 64 * u + lane);
                 }
-                if (queued) evaluate(uint32_t(lane) < queued, uint32_t(lane) < queued ? queue[lane] : 0);
+                if (queued) evaluate(queued);
                 wave_sync();
                 if (!recorded) fail(p, 2);  // more nodes than the bitmask rows hold
                 for (uint32_t k = lane; k < words; k += 64) record[k] = k < SDF_MASK_WORDS ? bits[k] : 0;
