@@ -1,5 +1,6 @@
 /*
- * sand_dem_hip.h -- C ABI of the MI355X `hip` backend for the surface-mesh DEM step.
+ * sand_dem_hip.h -- C ABI of the MI355X `hip` backend for the DEM timestep
+ * (surface-mesh and SDF-grid contact geometry, translating planes, diagnostics).
  *
  * The boundary mirrors the reference's native CPU backend ABI
  * (src/sand_atlas_dem/cpu.py:288-293, generated `cpu_dispatch`):
@@ -70,7 +71,7 @@ enum sad_side { SAD_SIDE_META = 0, SAD_SIDE_SLIP, SAD_SIDE_NORMAL_AREA, SAD_SIDE
 #define SAD_OK 0
 #define SAD_E_HIP 1          /* HIP runtime error; sad_hip_last_error() names it */
 #define SAD_E_ARGUMENT 2     /* contract violation: size, null pointer, unknown kernel */
-#define SAD_E_UNSUPPORTED 3  /* configuration outside the mesh path (SDF, walls, ...) */
+#define SAD_E_UNSUPPORTED 3  /* configuration not provided (source-surface normals, limits) */
 
 /* Device workspace owned by the library (replaces the reference's collector
  * scratch object, cpu_parallel.hpp:3-25 / cpu.py:364-376).  Holds staging,
@@ -97,7 +98,9 @@ const char* sad_hip_kernel_name(uint32_t kernel);
  * (backend.py:62-83).  `stream` is a hipStream_t (NULL = default stream).
  * Supported names: beginStep advanceCell checkRebuild clearBins fillBins
  * countPairs pairScan emitPairs resetMeshContacts prepareMeshPairs
- * collectMeshFrontiers clearContactStorage resolveContacts buildHistoryHash
+ * collectMeshFrontiers collectMeshContacts (the reference's verification
+ * collector; like the CPU backend, the same deterministic collection)
+ * clearContactStorage resolveContacts buildHistoryHash
  * gather finishStep captureContactKinematics checkContactTimestep advanceWalls
  * collectWallContacts gatherWallRows refreshWallPower diagnosticRows fabricRows,
  * and for the SDF geometry (values[SAD_V_MESH_GEOMETRY] == 0) countContacts
@@ -135,6 +138,11 @@ int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buf
  * aux[id / 256] to output[id].  Unsigned wrap-around, as the GPU kernels. */
 int sad_hip_scan_kernel(const char* kernel, uint32_t n, const uint32_t* input,
                         uint32_t* output, uint32_t* aux, void* stream);
+
+/* The adapter's compiled-kernel check (solver.slang:4-10, dispatched by the
+ * backends' doctor at backend.py:103-119 and cpu.py:668-689): b[i] = a[i] * a[i] + 1
+ * for i < n, FP32. */
+int sad_hip_smoke(uint32_t n, const void* a, void* b, void* stream);
 
 /* The diagnostics reductions (diagnostic.slang, driven by Engine.diagnostics at
  * gpu.py:746-792): one total per group of 256 records, summed by the pairwise

@@ -1344,7 +1344,7 @@ extern "C" int oracle_dispatch(const char* kernel, uint32_t count, void** buffer
         else if (name == "emitPairs") each(emitPairs);
         else if (name == "resetMeshContacts") each(resetMeshContacts);
         else if (name == "prepareMeshPairs") each(prepareMeshPairs);
-        else if (name == "collectMeshFrontiers") {
+        else if (name == "collectMeshFrontiers" || name == "collectMeshContacts") {  // cpu.py:80-96: both the serial collector
             uint32_t pairs = std::min(r.control[1], r.c.pairCapacity);
             if (r.c.threads > 1 && pairs > 1 && r.control[0] == 0) return collectParallel(r, pairs, r.c.threads);
             for (uint32_t pair = 0; pair < pairs; pair++) cpuCollectMesh(r, pair);

@@ -3140,7 +3140,10 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         LAUNCH(sc, KID_reset_mesh, k_reset_mesh, dim3(blocks(c.pairCapacity)), T, s, p, w);
     } else if (name == "prepareMeshPairs") {
         LAUNCH(sc, KID_prepare_pairs, k_prepare_pairs, dim3(blocks(c.pairCapacity)), T, s, p);
-    } else if (name == "collectMeshFrontiers") {
+    } else if (name == "collectMeshFrontiers" || name == "collectMeshContacts") {
+        // collectMeshContacts is the reference's verification collector (same contact
+        // set, atomic order); the CPU backend maps both names to its serial collector
+        // (cpu.py:80-96), and so does this one, to the deterministic collection.
         LAUNCH(sc, KID_adj_clear, k_adj_clear, perGrain, T, s, p, w);
         LAUNCH(sc, KID_adj_count, k_adj_count, gridStride, T, s, p, w);
         ScanArgs adj{w.adjCount, w.adjOffset, w.sums, c.N, nullptr, 1, nullptr, nullptr};
@@ -3230,14 +3233,10 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         }
         LAUNCH(sc, KID_fabric_rows, k_fabric_rows, dim3(blocks(c.pairCapacity)), T, s, p);
     } else {
-        // Reference entry points this backend does not provide: the serial
-        // verification collector and the smoke kernel.
-        static const char* unsupported[] = {"collectMeshContacts", "smoke"};
-        for (const char* u : unsupported)
-            if (name == u) {
-                std::snprintf(g_error, sizeof(g_error), "kernel %s is not part of the hip mesh path", u);
-                return SAD_E_UNSUPPORTED;
-            }
+        if (name == "smoke") {
+            std::snprintf(g_error, sizeof(g_error), "smoke binds its own buffers: call sad_hip_smoke");
+            return SAD_E_UNSUPPORTED;
+        }
         std::snprintf(g_error, sizeof(g_error), "unknown kernel %s", name.c_str());
         return SAD_E_ARGUMENT;
     }
@@ -3636,6 +3635,20 @@ int sad_hip_scan_kernel(const char* kernel, uint32_t n, const uint32_t* input, u
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SAD_OK : hip_fail(e, kernel);
+}
+
+// The adapter's compiled-kernel check (solver.slang:4-10): b = a * a + 1.
+__global__ void k_smoke(uint32_t n, const float* a, float* b) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) b[id] = a[id] * a[id] + 1.0f;
+}
+int sad_hip_smoke(uint32_t n, const void* a, void* b, void* stream) {
+    if (n == 0) return SAD_OK;
+    if (!a || !b) return SAD_E_ARGUMENT;
+    hipLaunchKernelGGL(k_smoke, dim3(blocks(n)), dim3(256), 0, static_cast<hipStream_t>(stream), n,
+                       static_cast<const float*>(a), static_cast<float*>(b));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAD_OK : hip_fail(e, "smoke");
 }
 
 int sad_hip_reduce_kernel(const char* kernel, uint32_t n, uint32_t walls, uint32_t output_count, const void* input,

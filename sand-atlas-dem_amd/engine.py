@@ -423,15 +423,8 @@ class Engine:
     def dispatch(self, name, n, e, b):
         self.m.dispatch(name, n, e, r=b)
 
-    def forces(self, e, b, *, capture_kinematics=True):  # gpu.py:547-565, mesh path
-        if self.viscous and capture_kinematics:
-            self.dispatch("captureContactKinematics", self.n, e, b)
-        self.dispatch("clearBins", self.bin_count, e, b)
-        self.dispatch("fillBins", self.n, e, b)
-        self.dispatch("countPairs", self.n, e, b)
-        self.dispatch("pairScan", self.n, e, b)
-        self.dispatch("emitPairs", self.n, e, b)
-        if self.mesh_geometry:  # _encode_contacts (gpu.py:532-545)
+    def _encode_contacts(self, e, b):  # gpu.py:532-545
+        if self.mesh_geometry:
             self.dispatch("resetMeshContacts", self.pair_capacity, e, b)
             self.dispatch("prepareMeshPairs", self.pair_capacity, e, b)
             self.dispatch("collectMeshFrontiers", 64 * 4096, e, b)
@@ -442,6 +435,16 @@ class Engine:
             self.dispatch("emitContacts", 64 * self.pair_capacity, e, b)
         if self.wall_count:
             self.dispatch("collectWallContacts", 64 * self.n * self.wall_count, e, b)
+
+    def forces(self, e, b, *, capture_kinematics=True):  # gpu.py:547-565
+        if self.viscous and capture_kinematics:
+            self.dispatch("captureContactKinematics", self.n, e, b)
+        self.dispatch("clearBins", self.bin_count, e, b)
+        self.dispatch("fillBins", self.n, e, b)
+        self.dispatch("countPairs", self.n, e, b)
+        self.dispatch("pairScan", self.n, e, b)
+        self.dispatch("emitPairs", self.n, e, b)
+        self._encode_contacts(e, b)
         self.dispatch("clearContactStorage", max(self.n, self.hash_size), e, b)
         self.dispatch("resolveContacts", self.contact_capacity, e, b)
         self.dispatch("buildHistoryHash", self.contact_capacity, e, b)

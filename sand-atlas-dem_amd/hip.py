@@ -195,7 +195,21 @@ class HipBackend:
         if status:
             raise RuntimeError(f"{what} failed ({status}): {self.lib.sad_hip_last_error().decode()}")
 
-    def dispatch(self, name, n, encoder=None, *, r):
+    def dispatch(self, name, n, encoder=None, *, r=None, **buffers):
+        if name == "smoke":  # its own bindings (solver.slang:4-10; backend.py:106)
+            count = int(buffers.get("N", n))
+            a, b = buffers["a"], buffers["b"]
+            if min(a.size, b.size) < 4 * count:
+                raise ValueError("smoke buffers hold fewer than N floats")
+
+            def smoke():
+                self._check(self.lib.sad_hip_smoke(count, ct.c_void_p(a.ptr), ct.c_void_p(b.ptr), self.stream),
+                            "dispatch smoke")
+
+            return smoke() if encoder is None else encoder.operations.append(smoke)
+        if r is None:
+            raise TypeError("solver kernels take the Resources bindings as r=")
+
         def execute():
             ptrs, sizes = _table(r)
             values = config_values(r["c"])
@@ -329,7 +343,15 @@ class HipBackend:
         return {self.lib.sad_hip_kernel_name(k).decode(): (ms[k], int(launches[k])) for k in range(n) if launches[k]}
 
     def doctor(self):
+        """Device report with the compiled-kernel check (backend.py:103-119)."""
         props = self.torch.cuda.get_device_properties(self.torch_device)
+        x = np.linspace(-2, 2, 257, dtype=np.float32)
+        a, b = self.buffer(x), self.buffer(np.zeros_like(x))
+        self.dispatch("smoke", len(x), N=len(x), a=a, b=b)
+        result = self.read(b)
+        error = float(np.max(np.abs(result - (x * x + 1))))
+        if not np.allclose(result, x * x + 1, rtol=1e-6, atol=1e-6):
+            raise RuntimeError("HIP compiled kernel smoke check failed")
         return dict(
             format="sand-dem-doctor/1",
             backend="hip",
@@ -340,4 +362,7 @@ class HipBackend:
             abi_version=int(self.lib.sad_hip_abi_version()),
             numerics="hand-written gfx950 HIP, FP32 compensated state, CPU-order deterministic collection",
             host=platform.processor() or platform.machine(),
+            smoke_elements=len(x),
+            smoke_passed=True,
+            smoke_max_error=error,
         )

@@ -106,6 +106,7 @@ def library():
             "sad_hip_profile_read": (ct.c_int, [vp, dp, u64p, u32]),
             "sad_hip_kernel_name": (ct.c_char_p, [u32]),
             "sad_hip_scan_kernel": (ct.c_int, [ct.c_char_p, u32, vp, vp, vp, vp]),
+            "sad_hip_smoke": (ct.c_int, [u32, vp, vp, vp]),
             "sad_hip_reduce_kernel": (ct.c_int, [ct.c_char_p, u32, u32, u32, vp, ct.c_uint64, vp, ct.c_uint64, vp]),
             "sad_hip_tune": (ct.c_int, [vp, ct.c_char_p, ct.c_int64, ct.POINTER(ct.c_int64)]),
             "sad_hip_invalidate": (ct.c_int, [vp]),
@@ -139,6 +140,7 @@ EXPORTED = (
     "sad_hip_kernel_name",
     "sad_hip_scan_kernel",
     "sad_hip_reduce_kernel",
+    "sad_hip_smoke",
     "sad_hip_tune",
     "sad_hip_invalidate",
     "sad_hip_nn_configure",

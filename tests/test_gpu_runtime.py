@@ -125,3 +125,37 @@ def test_reference_scan_kernels_through_the_adapter(n):
     m.wait()
     want = (np.cumsum(values.astype(np.uint64)) - values).astype(np.uint32)
     assert np.array_equal(m.read(dst, np.uint32)[:n], want)
+
+
+def test_doctor_runs_the_compiled_kernel_check():
+    """doctor() reports the smoke kernel (solver.slang:4-10) passed, as the
+    reference's backends do (backend.py:103-119)."""
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    report = HipBackend(0).doctor()
+    assert report["backend"] == "hip" and report["smoke_passed"] and not report["fallback"]
+    assert report["smoke_elements"] == 257 and report["smoke_max_error"] <= 1e-6
+
+
+def test_serial_collector_name_matches_oracle(pkg, oracle_module):
+    """collectMeshContacts (the reference's verification collector) dispatched on
+    the HIP backend gives the oracle's contacts, forces and history exactly."""
+    from sand_atlas_dem_amd import assembly
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    class SerialEngine(pkg.Engine):
+        def _encode_contacts(self, e, b):
+            self.dispatch("resetMeshContacts", self.pair_capacity, e, b)
+            self.dispatch("collectMeshContacts", 64 * self.pair_capacity, e, b)
+
+    w = assembly.config_workload("dense", n=300, templates=4, nodes=256)
+    engines = []
+    for backend in (HipBackend(0), oracle_module.make_oracle(pkg, threads=8)):
+        e = SerialEngine(w.shapes, w.spec, w.positions, w.cell, w.ids, w.scales, backend=backend)
+        e.set_kinematics(rotation=w.rotations, velocity=w.velocities * e.time_unit / e.length_unit)
+        e.initialize()
+        engines.append(e.state())
+    a, b = engines
+    assert a["contacts"] == b["contacts"] > 1000
+    for name in ("meta", "slip", "normalArea", "force", "torque"):
+        assert np.array_equal(a[name], b[name]), name
