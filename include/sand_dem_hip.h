@@ -139,6 +139,12 @@ int sad_hip_probe(const char* kernel, const uint32_t* uniforms, void* const* buf
 int sad_hip_scan_kernel(const char* kernel, uint32_t n, const uint32_t* input,
                         uint32_t* output, uint32_t* aux, void* stream);
 
+/* normalProbe (normal.slang:12-20), the reference's probe of its normal force law
+ * normalResponse (normal.slang:3-11): input rows (depth, area, normal velocity, -)
+ * -> output rows (force, loss power, 0, 0) for i < n, FP32. */
+int sad_hip_normal_probe(uint32_t n, float stiffness, float viscosity, float weight,
+                         const void* input, void* output, void* stream);
+
 /* The adapter's compiled-kernel check (solver.slang:4-10, dispatched by the
  * backends' doctor at backend.py:103-119 and cpu.py:668-689): b[i] = a[i] * a[i] + 1
  * for i < n, FP32. */

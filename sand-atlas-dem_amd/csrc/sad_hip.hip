@@ -3637,6 +3637,31 @@ int sad_hip_scan_kernel(const char* kernel, uint32_t n, const uint32_t* input, u
     return e == hipSuccess ? SAD_OK : hip_fail(e, kernel);
 }
 
+// normalProbe (normal.slang:12-20): rows (depth, area, normal velocity, -) ->
+// (force, loss power, 0, 0) by normalResponse (normal.slang:3-11).
+__global__ void k_normal_probe(uint32_t n, float stiffness, float viscosity, float weight, const float4* in,
+                               float4* out) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const float4 data = in[id];
+    float force = 0.0f, lossPower = 0.0f;
+    if (data.x > 0.0f) {
+        const float elastic = weight * stiffness * data.y * data.x;
+        force = viscosity == 0.0f ? elastic : fmaxf(0.0f, elastic - weight * viscosity * data.y * data.z);
+        lossPower = fmaxf(0.0f, (elastic - force) * data.z);
+    }
+    out[id] = make_float4(force, lossPower, 0.0f, 0.0f);
+}
+int sad_hip_normal_probe(uint32_t n, float stiffness, float viscosity, float weight, const void* input,
+                         void* output, void* stream) {
+    if (n == 0) return SAD_OK;
+    if (!input || !output) return SAD_E_ARGUMENT;
+    hipLaunchKernelGGL(k_normal_probe, dim3(blocks(n)), dim3(256), 0, static_cast<hipStream_t>(stream), n, stiffness,
+                       viscosity, weight, static_cast<const float4*>(input), static_cast<float4*>(output));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAD_OK : hip_fail(e, "normalProbe");
+}
+
 // The adapter's compiled-kernel check (solver.slang:4-10): b = a * a + 1.
 __global__ void k_smoke(uint32_t n, const float* a, float* b) {
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;

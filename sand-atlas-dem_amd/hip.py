@@ -80,13 +80,15 @@ class Kernel:
 
     solver.slang kernels go through the library's dispatch table; scan.slang's
     blockScan/addBlocks through `sad_hip_scan_kernel`; diagnostic.slang's
-    reduceRows/reduceFabric/reduceWallRows through `sad_hip_reduce_kernel`.
+    reduceRows/reduceFabric/reduceWallRows through `sad_hip_reduce_kernel`;
+    normal.slang's normalProbe through `sad_hip_normal_probe`.
     """
 
     MODULES = {
         "solver.slang": None,
         "scan.slang": {"blockScan", "addBlocks"},
         "diagnostic.slang": {"reduceRows", "reduceFabric", "reduceWallRows"},
+        "normal.slang": {"normalProbe"},
     }
 
     def __init__(self, backend, name, module):
@@ -101,6 +103,17 @@ class Kernel:
             r = (vars or {}).get("r", kwargs.get("r"))
             return self.m.dispatch(self.name, int(thread_count[0]), command_encoder, r=r)
         n = int(kwargs["N"])
+        if self.module == "normal.slang":
+            src, dst = kwargs["input"], kwargs["output"]
+            if min(src.size, dst.size) < 16 * n:
+                raise ValueError("normalProbe buffers hold fewer than N rows")
+            args = (float(kwargs["stiffness"]), float(kwargs["viscosity"]), float(kwargs["weight"]))
+
+            def probe():
+                self.m._check(self.m.lib.sad_hip_normal_probe(n, *args, ct.c_void_p(src.ptr), ct.c_void_p(dst.ptr),
+                                                              self.m.stream), "normalProbe")
+
+            return probe() if command_encoder is None else command_encoder.operations.append(probe)
         if self.module == "diagnostic.slang":
             return self._reduce(n, thread_count, command_encoder, kwargs)
         if self.name == "blockScan":

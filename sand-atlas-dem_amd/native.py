@@ -107,6 +107,7 @@ def library():
             "sad_hip_kernel_name": (ct.c_char_p, [u32]),
             "sad_hip_scan_kernel": (ct.c_int, [ct.c_char_p, u32, vp, vp, vp, vp]),
             "sad_hip_smoke": (ct.c_int, [u32, vp, vp, vp]),
+            "sad_hip_normal_probe": (ct.c_int, [u32, ct.c_float, ct.c_float, ct.c_float, vp, vp, vp]),
             "sad_hip_reduce_kernel": (ct.c_int, [ct.c_char_p, u32, u32, u32, vp, ct.c_uint64, vp, ct.c_uint64, vp]),
             "sad_hip_tune": (ct.c_int, [vp, ct.c_char_p, ct.c_int64, ct.POINTER(ct.c_int64)]),
             "sad_hip_invalidate": (ct.c_int, [vp]),
@@ -141,6 +142,7 @@ EXPORTED = (
     "sad_hip_scan_kernel",
     "sad_hip_reduce_kernel",
     "sad_hip_smoke",
+    "sad_hip_normal_probe",
     "sad_hip_tune",
     "sad_hip_invalidate",
     "sad_hip_nn_configure",

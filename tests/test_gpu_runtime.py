@@ -159,3 +159,28 @@ def test_serial_collector_name_matches_oracle(pkg, oracle_module):
     assert a["contacts"] == b["contacts"] > 1000
     for name in ("meta", "slip", "normalArea", "force", "torque"):
         assert np.array_equal(a[name], b[name]), name
+
+
+def test_normal_probe_closed_form():
+    """normalProbe through the adapter (test_cpu_solver.py:685-709 of the
+    reference): the force of rows with positive depth is max(0, w k a d - w c a v),
+    bit-equal to the same FP32 operations on the host, zero otherwise."""
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    hip = HipBackend(0)
+    values = np.array([[1e-7, 1e-8, -0.01, 0], [1e-7, 1e-8, 0.2, 0], [-1e-7, 1e-8, 0, 0], [2e-7, 3e-8, 0.0, 0]],
+                      np.float32)
+    source, result = hip.buffer(values), hip.buffer(count=len(values), stride=16)
+    hip.kernel("normalProbe", "normal.slang").dispatch(thread_count=[len(values), 1, 1], N=len(values),
+                                                       stiffness=1e12, viscosity=1e6, weight=0.5, input=source,
+                                                       output=result)
+    got = hip.read(result, shape=(-1, 4))
+    f32 = np.float32
+    d, a, v = values[:, 0], values[:, 1], values[:, 2]
+    elastic = f32(0.5) * f32(1e12) * a * d
+    force = np.where(d > 0, np.maximum(f32(0), elastic - f32(0.5) * f32(1e6) * a * v), f32(0))
+    loss = np.where(d > 0, np.maximum(f32(0), (elastic - force) * v), f32(0))
+    np.testing.assert_array_equal(got[:, 0], force)
+    np.testing.assert_array_equal(got[:, 1], loss)
+    reference = np.where(d > 0, np.maximum(0, 0.5 * 1e12 * a * d - 0.5 * 1e6 * a * v), 0)
+    np.testing.assert_allclose(got[:, 0], reference, rtol=2e-5, atol=1e-9)
