@@ -927,11 +927,29 @@ __global__ void k_adj_fill(P p, W w) {
         w.adjList[w.adjOffset[ij.y] + atomicAdd(w.adjCursor + ij.y, 1u)] = k;
     }
 }
-__global__ void k_adj_sort(P p, W w) {
-    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+// Each grain's pair ids in ascending order (the order the gather walks).  One wave
+// per grain: up to 64 ids are placed by rank (ids are distinct) from LDS; longer
+// lists fall back to an insertion sort by one lane.
+__global__ __launch_bounds__(256) void k_adj_sort(P p, W w) {
+    __shared__ uint32_t idsMem[4][64];
+    const int lane = threadIdx.x & 63;
+    const uint32_t id = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (id >= p.c.N) return;
     uint32_t* a = w.adjList + w.adjOffset[id];
-    uint32_t n = w.adjCount[id];
+    const uint32_t n = w.adjCount[id];
+    if (n <= 64) {
+        uint32_t* ids = idsMem[threadIdx.x >> 6];
+        const uint32_t mine = uint32_t(lane) < n ? a[lane] : 0u;
+        if (uint32_t(lane) < n) ids[lane] = mine;
+        wave_sync();
+        if (uint32_t(lane) < n) {
+            uint32_t rank = 0;
+            for (uint32_t k = 0; k < n; k++) rank += ids[k] < mine ? 1u : 0u;
+            a[rank] = mine;
+        }
+        return;
+    }
+    if (lane != 0) return;
     for (uint32_t k = 1; k < n; k++) {
         uint32_t v = a[k];
         uint32_t m = k;
@@ -3149,7 +3167,7 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         ScanArgs adj{w.adjCount, w.adjOffset, w.sums, c.N, nullptr, 1, nullptr, nullptr};
         launch_scan(sc, adj, c.N, s);
         LAUNCH(sc, KID_adj_fill, k_adj_fill, gridStride, T, s, p, w);
-        LAUNCH(sc, KID_adj_sort, k_adj_sort, perGrain, T, s, p, w);
+        LAUNCH(sc, KID_adj_sort, k_adj_sort, dim3(blocks(c.N, 4)), T, s, p, w);
         int rc = sc.ensure_nodes(p.meshNodes);
         if (rc) return rc;
         LAUNCH(sc, KID_wide_nodes, k_wide_nodes, dim3(blocks(p.meshNodes)), T, s, p, w, p.meshNodes);
@@ -3211,7 +3229,7 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         ScanArgs adj{w.adjCount, w.adjOffset, w.sums, c.N, nullptr, 1, nullptr, nullptr};
         launch_scan(sc, adj, c.N, s);
         LAUNCH(sc, KID_adj_fill, k_adj_fill, gridStride, T, s, p, w);
-        LAUNCH(sc, KID_adj_sort, k_adj_sort, perGrain, T, s, p, w);
+        LAUNCH(sc, KID_adj_sort, k_adj_sort, dim3(blocks(c.N, 4)), T, s, p, w);
         LAUNCH(sc, KID_sdf_count, k_sdf_contacts<false>, dim3(blocks(c.pairCapacity, 4)), T, s, p, w);
     } else if (name == "contactScan") {
         ScanArgs a{p.contactCounts, p.contactOffsets, w.sums, c.pairCapacity, nullptr, 1, nullptr, nullptr};
