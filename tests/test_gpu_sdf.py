@@ -82,3 +82,21 @@ def test_sdf_with_planes_matches_oracle_exactly(oracle_cpu):
         a.advance(3)
         b.advance(3)
         assert_states_equal(a.state(), b.state())
+
+
+def test_sdf_contact_overflow_is_flagged_then_fits(pkg, oracle_cpu, sdf_workload):
+    """Too few contact slots raise flag 32 (contactTotal, solver.slang:568-574) and
+    write nothing out of bounds; with room, the same step equals the oracle."""
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    print("sdf overflow", flush=True)
+    hip = HipBackend(0)
+    tight = replace(sdf_workload, spec=replace(sdf_workload.spec, contacts_per_particle=4))
+    with pytest.raises(pkg.RejectedStep) as info:
+        tight.engine(hip).initialize()
+    assert info.value.flags & 32
+    roomy = replace(sdf_workload, spec=replace(sdf_workload.spec, contacts_per_particle=96))
+    a, b = roomy.engine(hip), roomy.engine(oracle_cpu)
+    a.initialize()
+    b.initialize()
+    assert_states_equal(a.state(), b.state())
