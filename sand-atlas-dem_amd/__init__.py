@@ -1,4 +1,4 @@
-"""MI355X-native `hip` backend for the surface-mesh DEM step of sand-atlas-dem.
+"""MI355X-native `hip` backend for the DEM timestep of sand-atlas-dem.
 
 Import name: `sand_atlas_dem_amd` (the directory is `sand-atlas-dem_amd/`; the repo
 root's `load_package()` helpers register it under that name).

@@ -1,10 +1,10 @@
-"""Host step driver for the surface-mesh DEM step.
+"""Host step driver for the DEM timestep (surface-mesh and SDF-grid geometry).
 
-Restates the reference `Engine` (`src/sand_atlas_dem/gpu.py:94-1092`) for the mesh
-path: SI -> normalized units, capacities, the buffer inventory in the kernels'
+Restates the reference `Engine` (`src/sand_atlas_dem/gpu.py:94-1092`): SI ->
+normalized units, capacities, the buffer inventory in the kernels'
 `Resources` layout, the per-step encoder sequence, the device error word with
-transactional rollback, capacity growth, the double-buffered friction history, and
-state/restore.  It is adapter-agnostic exactly as the reference is: it talks to a
+transactional rollback, capacity growth, the double-buffered friction history,
+state/restore and the diagnostics summary.  It is adapter-agnostic exactly as the reference is: it talks to a
 backend object through the duck-typed adapter protocol of `gpu.py` (buffer,
 dispatch, create_command_encoder/copy_buffer, submit, wait, read, read_prefix,
 write_prefix).  The product adapter is `HipBackend` (`hip.py`); when an adapter

@@ -1,7 +1,7 @@
-"""Experiment contract for the surface-mesh step (SI units).
+"""Experiment contract for the DEM timestep (SI units).
 
 Restates the fields and validation of the reference `Experiment`
-(`src/sand_atlas_dem/spec.py:12-166`) that the mesh timestep reads through
+(`src/sand_atlas_dem/spec.py:12-166`) that the timestep reads through
 `Engine` (`gpu.py:95-368, 490-527`).  Field names, defaults and error behaviour are
 kept, so a reference case JSON loads unchanged; stage/servo fields are accepted and
 validated but driving stages is the run layer's job, outside this backend.
