@@ -1805,7 +1805,8 @@ __device__ bool sdfNodeContact(const P& p, const SdfCorners& corners, v3 arm, v4
 // deterministic, so the same nodes hit with the same values), 64 node ids per step,
 // ranked by ballot.  The counts and offsets of the two directions also go to the
 // per-direction task rows the gather walks, as after a mesh collection.
-constexpr uint32_t SDF_QUEUE = 192, SDF_MASK_WORDS = 128;  // LDS per wave: queue, bitmask (4,096 nodes)
+// Exact evaluations per lane per batch; the queue holds one batch plus a row of 64.
+constexpr uint32_t SDF_PER_LANE = 3, SDF_QUEUE = 64 * SDF_PER_LANE + 64, SDF_MASK_WORDS = 128, SDF_SCREEN_ROWS = 4;  // LDS per wave: queue, bitmask (4,096 nodes)
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
     __shared__ uint32_t queueMem[4][SDF_QUEUE];
@@ -1832,17 +1833,18 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
             uint32_t* record = w.sdfHits + size_t(2 * pair + pass) * words;
             evaluated += info.y;
             // Exact evaluation of one node per active lane; hits ranked by ballot.
-            // Exact evaluation of queue entries [0, n), n <= 128, two per lane (entries
-            // lane and 64 + lane): both points' corner loads are issued before either
-            // is used.  A lane without a second entry repeats entry 0 (same flags) and
-            // discards it.  Hits are ranked by ballot in entry order.
+            // Exact evaluation of queue entries [0, n), n <= 64 * SDF_PER_LANE, several per
+            // lane (entries lane, 64 + lane, ...): every point's corner loads are issued
+            // before any is used.  A lane without an entry in a row repeats entry 0
+            // (same flags) and discards it.  Hits are ranked by ballot in entry order.
             auto evaluate = [&](uint32_t n) {
-                uint32_t node[2];
-                v3 arm[2];
-                float area[2];
-                SdfCorners corners[2];
+                uint32_t node[SDF_PER_LANE];
+                v3 arm[SDF_PER_LANE];
+                float area[SDF_PER_LANE];
+                SdfCorners corners[SDF_PER_LANE];
 #pragma unroll
-                for (uint32_t u = 0; u < 2; u++) {
+                for (uint32_t u = 0; u < SDF_PER_LANE; u++) {
+                    if (64 * u >= n) break;
                     const uint32_t e = 64 * u + lane;
                     node[u] = queue[e < n ? e : 0u];
                     const float4 sp = p.sample[info.x + node[u]];
@@ -1850,7 +1852,7 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
                     area[u] = sp.w;
                     corners[u] = sdfLoad(p, targetShape, qinv(qj, arm[u] - branch) / bj.x);
                 }
-                for (uint32_t u = 0; u < 2 && 64 * u < n; u++) {
+                for (uint32_t u = 0; u < SDF_PER_LANE && 64 * u < n; u++) {
                     float4 pd, na;
                     const bool hit = sdfNodeContact(p, corners[u], arm[u], qj, area[u], bi.x, bi.w, bj.x, bj.w, pd, na) &&
                                      64 * u + lane < n;
@@ -1869,18 +1871,18 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
                     count += __popcll(mask);
                 }
             };
-            // Node-ordered queue of candidates, evaluated 128 at a time.
+            // Node-ordered queue of candidates, evaluated 64 * SDF_PER_LANE at a time.
             uint32_t queued = 0;
             auto push = [&](bool keep, uint32_t node) {
                 const uint64_t mask = __ballot(keep);
                 if (keep) queue[queued + __popcll(mask & lanes_below(lane))] = node;
                 queued += __popcll(mask);
                 wave_sync();
-                if (queued >= 128) {
-                    evaluate(128);
+                if (queued >= 64 * SDF_PER_LANE) {
+                    evaluate(64 * SDF_PER_LANE);
                     wave_sync();
-                    const uint32_t rest = queued - 128;
-                    const uint32_t moved = uint32_t(lane) < rest ? queue[128 + lane] : 0;
+                    const uint32_t rest = queued - 64 * SDF_PER_LANE;
+                    const uint32_t moved = uint32_t(lane) < rest ? queue[64 * SDF_PER_LANE + lane] : 0;
                     wave_sync();
                     if (uint32_t(lane) < rest) queue[lane] = moved;
                     queued = rest;
@@ -1906,10 +1908,10 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
                 const bool recorded = info.y <= 32 * min(words, SDF_MASK_WORDS);
                 for (uint32_t k = lane; k < words && k < SDF_MASK_WORDS; k += 64) bits[k] = 0;
                 wave_sync();
-                for (uint32_t first = 0; first < info.y; first += 256) {  // four node rows in flight per lane
-                    bool keep[4];
+                for (uint32_t first = 0; first < info.y; first += 64 * SDF_SCREEN_ROWS) {  // rows in flight per lane
+                    bool keep[SDF_SCREEN_ROWS];
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; u++) {
+                    for (uint32_t u = 0; u < SDF_SCREEN_ROWS; u++) {
                         const uint32_t node = first + 64 * u + lane;
                         keep[u] = false;
                         if (node < info.y) {
@@ -1917,7 +1919,8 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
                             keep[u] = !(scale * dot(d, d) > bound);
                         }
                     }
-                    for (uint32_t u = 0; u < 4 && first + 64 * u < info.y; u++) push(keep[u], first + 64 * u + lane);
+                    for (uint32_t u = 0; u < SDF_SCREEN_ROWS && first + 64 * u < info.y; u++)
+                        push(keep[u], first + 64 * u + lane);
                 }
                 if (queued) evaluate(queued);
                 wave_sync();
