@@ -65,3 +65,24 @@ def test_threaded_packer_is_faster_on_a_large_library(pkg):
     pkg.geometry.pack_meshes(shapes, unit, threads=8)
     native = time.perf_counter() - t0
     assert native < serial / 2, (native, serial)
+
+
+@pytest.mark.parametrize("count", [1, 64, 65, 1024, 8193])
+def test_frontiers_partition_original_keys(pkg, count):
+    """The 32 frontier subtrees cover every node key once, each frontier row's escape
+    is its subtree root's (reference tests/test_parallel_mesh.py:17-33)."""
+    shape = pkg.geometry.box_shape([1e-3] * 3, nodes=16)
+    rng = np.random.default_rng(592)
+    shape.samples = rng.normal(size=(count, 3)) * 0.001
+    shape.areas = np.ones(count)
+    data = pkg.geometry.pack_meshes([shape], 0.001)
+    keys = []
+    for first, end, _, _ in data["meshFrontier"]:
+        if end == 0:
+            continue
+        assert end == data["meshClusterRanges"][first, 2]
+        for node in range(first, end):
+            start, size, _, _ = data["meshClusterRanges"][node]
+            keys.extend(data["meshSampleIndices"][start : start + size])
+    assert sorted(map(int, keys)) == list(range(count))
+    assert len(data["meshFrontier"]) == 32
