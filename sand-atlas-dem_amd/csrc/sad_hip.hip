@@ -127,6 +127,8 @@ struct W {
     float skipLimit;      // re-anchor once the motion bound reaches this (template units)
     float* sdfReach;      // SDF geometry, per template: squared radius beyond which the field is >= 0
     uint32_t* sdfHits;    // SDF geometry, per directed task: bitmask of its contact nodes (count pass)
+    float4* sdfLeaves;    // SDF geometry, per template 64 rows: node leaves' bounding spheres (centre, radius)
+    uint16_t* sdfOrder;   // SDF geometry, per template 4,096 node ids, leaf after leaf (64 per leaf)
     uint32_t sdfWords;    // 32-bit words per task in sdfHits (>= largest node count / 32, even)
     uint32_t* wallCount;  // plane contacts per (wall, grain) task, task = wall * N + grain
     uint32_t* wallOffset; // exclusive scan of wallCount (ids follow the mesh contacts)
@@ -1745,6 +1747,9 @@ __device__ v4 sdfFinish(const P& p, const SdfCorners& c) {
     return {grad.x, grad.y, grad.z, value};
 }
 __device__ v4 sdfSample(const P& p, uint32_t s, v3 q) { return sdfFinish(p, sdfLoad(p, s, q)); }
+// Exact evaluations per lane per batch; the queue holds one batch plus a row of 64;
+// per wave the contact-node bitmask covers 4,096 nodes.
+constexpr uint32_t SDF_PER_LANE = 3, SDF_QUEUE = 64 * SDF_PER_LANE + 64, SDF_MASK_WORDS = 128;
 // Per template, the squared radius (template units) outside which every grid cell
 // has only nonnegative corners: the largest |corner| over negative grid values, plus
 // two spacings (more than a cell diagonal, covering the rounding of the cell index).
@@ -1776,6 +1781,82 @@ __global__ __launch_bounds__(256) void k_sdf_reach(P p, W w, uint32_t* largest) 
         w.sdfReach[shape] = reach < 0.0f ? -1.0f : reach * reach;
     }
 }
+// Per template, a spatial index of its nodes for the count pass's screen: nodes
+// sorted by the Morton code of their position (10 bits per axis over the node
+// bounding box; ties by node id), cut into leaves of 64, each with a bounding sphere
+// (radius padded by 1e-5 relative).  A leaf whose sphere lies beyond the reach
+// holds no node that can be in contact.  Templates with more than 4,096 nodes are
+// refused before this runs.
+__global__ __launch_bounds__(256) void k_sdf_index(P p, W w) {
+    __shared__ uint64_t keys[32 * SDF_MASK_WORDS];
+    __shared__ float part[6][256];
+    __shared__ float box[6];
+    const uint32_t shape = blockIdx.x, t = threadIdx.x;
+    const uint4 info = p.sampleInfo[shape];
+    const uint32_t n = min(info.y, 32u * SDF_MASK_WORDS);
+    float lo3[3] = {INFINITY, INFINITY, INFINITY}, hi3[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = t; k < n; k += 256) {
+        const float4 q = p.sample[info.x + k];
+        const float c[3] = {q.x, q.y, q.z};
+        for (int a = 0; a < 3; a++) lo3[a] = fminf(lo3[a], c[a]), hi3[a] = fmaxf(hi3[a], c[a]);
+    }
+    for (int a = 0; a < 3; a++) part[a][t] = lo3[a], part[3 + a][t] = hi3[a];
+    __syncthreads();
+    if (t < 6) {
+        float v = part[t][0];
+        for (uint32_t k = 1; k < 256; k++) v = t < 3 ? fminf(v, part[t][k]) : fmaxf(v, part[t][k]);
+        box[t] = v;
+    }
+    __syncthreads();
+    uint32_t size = 1;
+    while (size < n) size <<= 1;
+    for (uint32_t k = t; k < size; k += 256) {
+        uint64_t key = ~0ull;
+        if (k < n) {
+            const float4 q = p.sample[info.x + k];
+            const float c[3] = {q.x, q.y, q.z};
+            uint32_t code = 0;
+            for (int a = 0; a < 3; a++) {
+                const float extent = box[3 + a] - box[a];
+                const float u = extent > 0.0f ? (c[a] - box[a]) / extent : 0.0f;
+                const uint32_t cell = min(1023u, uint32_t(fmaxf(u, 0.0f) * 1024.0f));
+                for (int bit = 0; bit < 10; bit++) code |= ((cell >> bit) & 1u) << (3 * bit + a);
+            }
+            key = (uint64_t(code) << 16) | k;
+        }
+        keys[k] = key;
+    }
+    __syncthreads();
+    for (uint32_t span = 2; span <= size; span <<= 1)  // bitonic sort, ascending
+        for (uint32_t stride = span >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t k = t; k < size; k += 256) {
+                const uint32_t partner = k ^ stride;
+                if (partner > k) {
+                    const bool up = (k & span) == 0;
+                    const uint64_t a = keys[k], b = keys[partner];
+                    if ((a > b) == up) { keys[k] = b; keys[partner] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    uint16_t* order = w.sdfOrder + size_t(shape) * (32 * SDF_MASK_WORDS);
+    for (uint32_t k = t; k < n; k += 256) order[k] = uint16_t(keys[k] & 0xffffu);
+    const uint32_t leaves = (n + 63) / 64;
+    if (t < leaves) {
+        v3 lo = {INFINITY, INFINITY, INFINITY}, hi = {-INFINITY, -INFINITY, -INFINITY};
+        const uint32_t end = min(n, 64 * t + 64);
+        for (uint32_t k = 64 * t; k < end; k++) {
+            const v3 q = xyz(p.sample[info.x + uint32_t(keys[k] & 0xffffu)]);
+            lo = {fminf(lo.x, q.x), fminf(lo.y, q.y), fminf(lo.z, q.z)};
+            hi = {fmaxf(hi.x, q.x), fmaxf(hi.y, q.y), fmaxf(hi.z, q.z)};
+        }
+        const v3 centre = 0.5f * (lo + hi);
+        float radius = 0.0f;
+        for (uint32_t k = 64 * t; k < end; k++)
+            radius = fmaxf(radius, length(xyz(p.sample[info.x + uint32_t(keys[k] & 0xffffu)]) - centre));
+        w.sdfLeaves[size_t(shape) * 64 + t] = make_float4(centre.x, centre.y, centre.z, radius * 1.00001f + 1e-30f);
+    }
+}
 // nodeContact (solver.slang:319-343), SDF branch, after the node's local point.
 __device__ bool sdfNodeContact(const P& p, const SdfCorners& corners, v3 arm, v4 qj, float area, float si,
                                float wi, float sj, float wj, float4& pd, float4& na) {
@@ -1805,8 +1886,6 @@ __device__ bool sdfNodeContact(const P& p, const SdfCorners& corners, v3 arm, v4
 // deterministic, so the same nodes hit with the same values), 64 node ids per step,
 // ranked by ballot.  The counts and offsets of the two directions also go to the
 // per-direction task rows the gather walks, as after a mesh collection.
-// Exact evaluations per lane per batch; the queue holds one batch plus a row of 64.
-constexpr uint32_t SDF_PER_LANE = 3, SDF_QUEUE = 64 * SDF_PER_LANE + 64, SDF_MASK_WORDS = 128, SDF_SCREEN_ROWS = 4;  // LDS per wave: queue, bitmask (4,096 nodes)
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
     __shared__ uint32_t queueMem[4][SDF_QUEUE];
@@ -1908,19 +1987,32 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
                 const bool recorded = info.y <= 32 * min(words, SDF_MASK_WORDS);
                 for (uint32_t k = lane; k < words && k < SDF_MASK_WORDS; k += 64) bits[k] = 0;
                 wave_sync();
-                for (uint32_t first = 0; first < info.y; first += 64 * SDF_SCREEN_ROWS) {  // rows in flight per lane
-                    bool keep[SDF_SCREEN_ROWS];
-#pragma unroll
-                    for (uint32_t u = 0; u < SDF_SCREEN_ROWS; u++) {
-                        const uint32_t node = first + 64 * u + lane;
-                        keep[u] = false;
-                        if (node < info.y) {
-                            const v3 d = xyz(p.sample[info.x + node]) - centre;
-                            keep[u] = !(scale * dot(d, d) > bound);
-                        }
+                // Only the count and the set of contact nodes come out of this pass, so
+                // the nodes are visited leaf by leaf: a leaf whose bounding sphere
+                // stays beyond the reach (1e-3 relative margin over the node screen)
+                // is passed over, the nodes of the others are screened one per lane.
+                const uint32_t source = p.shapeId[i];
+                const float limit = bound > 0.0f ? sqrtf(bound / scale) * 1.001f : -1.0f;
+                const uint32_t leaves = (min(info.y, 32u * SDF_MASK_WORDS) + 63) / 64;
+                bool near = false;
+                if (uint32_t(lane) < leaves) {
+                    const float4 leaf = w.sdfLeaves[size_t(source) * 64 + lane];
+                    near = !(length(xyz(leaf) - centre) - leaf.w > limit);
+                }
+                uint64_t nearLeaves = __ballot(near);
+                const uint16_t* order = w.sdfOrder + size_t(source) * (32 * SDF_MASK_WORDS);
+                while (nearLeaves) {
+                    const uint32_t leaf = uint32_t(__builtin_ctzll(nearLeaves));
+                    nearLeaves &= nearLeaves - 1;
+                    const uint32_t k = 64 * leaf + lane;
+                    uint32_t node = 0;
+                    bool keep = false;
+                    if (k < info.y) {
+                        node = order[k];
+                        const v3 d = xyz(p.sample[info.x + node]) - centre;
+                        keep = !(scale * dot(d, d) > bound);
                     }
-                    for (uint32_t u = 0; u < SDF_SCREEN_ROWS && first + 64 * u < info.y; u++)
-                        push(keep[u], first + 64 * u + lane);
+                    push(keep, node);
                 }
                 if (queued) evaluate(queued);
                 wave_sync();
@@ -2701,6 +2793,7 @@ struct Caps {
     uint32_t counters = 0, taskCount = 0, taskOffset = 0, adjCount = 0, adjOffset = 0, adjCursor = 0, adjList = 0;
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
     uint32_t stageCount = 0, keyCount = 0, keyStart = 0, wallCount = 0, wallOffset = 0, sdfReach = 0, sdfHits = 0;
+    uint32_t sdfLeaves = 0, sdfOrder = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -2741,6 +2834,9 @@ struct ScratchImpl : Scratch {
         if (p.c.meshGeometry) return SAD_OK;
         int rc = grow(w.sdfReach, caps.sdfReach, std::max(p.c.shapes, 1u), bytes);
         if (rc) return rc;
+        const uint32_t shapes = std::max(p.c.shapes, 1u);
+        if ((rc = grow(w.sdfLeaves, caps.sdfLeaves, size_t(64) * shapes, bytes))) return rc;
+        if ((rc = grow(w.sdfOrder, caps.sdfOrder, size_t(32 * SDF_MASK_WORDS) * shapes, bytes))) return rc;
         if (reachEpoch != epoch || reachShapes != p.c.shapes) {
             if (!probeWord) HIP_OK(hipMalloc(reinterpret_cast<void**>(&probeWord), 3 * sizeof(uint32_t)));
             HIP_OK(hipMemsetAsync(probeWord, 0, sizeof(uint32_t), s));
@@ -2755,6 +2851,8 @@ struct ScratchImpl : Scratch {
                 return SAD_E_UNSUPPORTED;
             }
             w.sdfWords = std::max(2u, (largest + 63) / 64 * 2);
+            hipLaunchKernelGGL(k_sdf_index, dim3(shapes), dim3(256), 0, s, p, w);
+            HIP_OK(hipGetLastError());
             reachEpoch = epoch;
             reachShapes = p.c.shapes;
         }
@@ -2925,6 +3023,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.totals);
         (void)hipFree(w.counters); (void)hipFree(w.taskCount); (void)hipFree(w.taskOffset); (void)hipFree(w.adjCount);
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums); (void)hipFree(w.sdfReach); (void)hipFree(w.sdfHits);
+        (void)hipFree(w.sdfLeaves); (void)hipFree(w.sdfOrder);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
         (void)hipFree(w.anchor); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
