@@ -1692,20 +1692,18 @@ __global__ void k_scatter(P p, W w) {
 // ---------------------------------------------------------------- SDF contacts (solver.slang:188-200, 319-343, 536-574)
 // sdf: the template grid's trilinear value and piecewise gradient, 1e20 outside the
 // grid; lerp(x, y, s) is x + s (y - x).  A corner index beyond the distance buffer
-// (inconsistent grid rows) raises flag 2 and reads as outside.
-// Split in two so a caller can issue the corner loads of several points before it
-// uses any: sdfLoad fetches the 8 corners (in-bounds reads only; the cell is taken
-// only for points inside the grid), sdfFinish forms the value and gradient.
+// (inconsistent grid rows) raises flag 2 and reads as outside.  It comes in two parts,
+// so a caller can issue the corner loads of several points before it uses any:
+// sdfLoad fetches the 8 corners (in-bounds reads only; the cell is taken only for
+// points inside the grid), sdfFinish forms the value and gradient.
 struct SdfCorners {
     float f[8];
     v3 g;
     float spacing;
     bool inside, ok;
 };
-__device__ SdfCorners sdfLoad(const P& p, uint32_t s, v3 q) {
+__device__ SdfCorners sdfLoad(const P& p, uint4 meta, float4 o, v3 q) {  // meta, o: the template's grid rows
     SdfCorners c;
-    const uint4 meta = p.grids[s];
-    const float4 o = p.gridOrigin[s];
     c.g = (q - xyz(o)) / o.w;
     c.spacing = o.w;
     c.inside = c.g.x >= 0.0f && c.g.y >= 0.0f && c.g.z >= 0.0f && !(c.g.x >= float(int64_t(meta.y) - 1)) &&
@@ -1746,7 +1744,6 @@ __device__ v4 sdfFinish(const P& p, const SdfCorners& c) {
             }
     return {grad.x, grad.y, grad.z, value};
 }
-__device__ v4 sdfSample(const P& p, uint32_t s, v3 q) { return sdfFinish(p, sdfLoad(p, s, q)); }
 // Exact evaluations per lane per batch; the queue holds one batch plus a row of 64;
 // per wave the contact-node bitmask covers 4,096 nodes.
 constexpr uint32_t SDF_PER_LANE = 3, SDF_QUEUE = 64 * SDF_PER_LANE + 64, SDF_MASK_WORDS = 128;
@@ -1888,6 +1885,9 @@ __device__ bool sdfNodeContact(const P& p, const SdfCorners& corners, v3 arm, v4
 // per-direction task rows the gather walks, as after a mesh collection.
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
+    // The emit pass evaluates only recorded contact nodes (a few per direction): one
+    // per lane keeps its registers low.
+    constexpr uint32_t PER_LANE = EMIT ? 1u : SDF_PER_LANE;
     __shared__ uint32_t queueMem[4][SDF_QUEUE];
     __shared__ uint32_t maskMem[4][SDF_MASK_WORDS];
     const int lane = threadIdx.x & 63;
@@ -1909,29 +1909,31 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
             const float4 bi = p.body[i], bj = p.body[j];
             const v4 qi = v4of(p.rotation[i]), qj = v4of(p.rotation[j]);
             const v3 branch = centerBranch(p, i, j);
+            const uint4 gridRow = p.grids[targetShape];
+            const float4 gridOrigin = p.gridOrigin[targetShape];
             uint32_t* record = w.sdfHits + size_t(2 * pair + pass) * words;
             evaluated += info.y;
             // Exact evaluation of one node per active lane; hits ranked by ballot.
-            // Exact evaluation of queue entries [0, n), n <= 64 * SDF_PER_LANE, several per
+            // Exact evaluation of queue entries [0, n), n <= 64 * PER_LANE, several per
             // lane (entries lane, 64 + lane, ...): every point's corner loads are issued
             // before any is used.  A lane without an entry in a row repeats entry 0
             // (same flags) and discards it.  Hits are ranked by ballot in entry order.
             auto evaluate = [&](uint32_t n) {
-                uint32_t node[SDF_PER_LANE];
-                v3 arm[SDF_PER_LANE];
-                float area[SDF_PER_LANE];
-                SdfCorners corners[SDF_PER_LANE];
+                uint32_t node[PER_LANE];
+                v3 arm[PER_LANE];
+                float area[PER_LANE];
+                SdfCorners corners[PER_LANE];
 #pragma unroll
-                for (uint32_t u = 0; u < SDF_PER_LANE; u++) {
+                for (uint32_t u = 0; u < PER_LANE; u++) {
                     if (64 * u >= n) break;
                     const uint32_t e = 64 * u + lane;
                     node[u] = queue[e < n ? e : 0u];
                     const float4 sp = p.sample[info.x + node[u]];
                     arm[u] = qrot(qi, xyz(sp) * bi.x);
                     area[u] = sp.w;
-                    corners[u] = sdfLoad(p, targetShape, qinv(qj, arm[u] - branch) / bj.x);
+                    corners[u] = sdfLoad(p, gridRow, gridOrigin, qinv(qj, arm[u] - branch) / bj.x);
                 }
-                for (uint32_t u = 0; u < SDF_PER_LANE && 64 * u < n; u++) {
+                for (uint32_t u = 0; u < PER_LANE && 64 * u < n; u++) {
                     float4 pd, na;
                     const bool hit = sdfNodeContact(p, corners[u], arm[u], qj, area[u], bi.x, bi.w, bj.x, bj.w, pd, na) &&
                                      64 * u + lane < n;
@@ -1950,18 +1952,18 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
                     count += __popcll(mask);
                 }
             };
-            // Node-ordered queue of candidates, evaluated 64 * SDF_PER_LANE at a time.
+            // Node-ordered queue of candidates, evaluated 64 * PER_LANE at a time.
             uint32_t queued = 0;
             auto push = [&](bool keep, uint32_t node) {
                 const uint64_t mask = __ballot(keep);
                 if (keep) queue[queued + __popcll(mask & lanes_below(lane))] = node;
                 queued += __popcll(mask);
                 wave_sync();
-                if (queued >= 64 * SDF_PER_LANE) {
-                    evaluate(64 * SDF_PER_LANE);
+                if (queued >= 64 * PER_LANE) {
+                    evaluate(64 * PER_LANE);
                     wave_sync();
-                    const uint32_t rest = queued - 64 * SDF_PER_LANE;
-                    const uint32_t moved = uint32_t(lane) < rest ? queue[64 * SDF_PER_LANE + lane] : 0;
+                    const uint32_t rest = queued - 64 * PER_LANE;
+                    const uint32_t moved = uint32_t(lane) < rest ? queue[64 * PER_LANE + lane] : 0;
                     wave_sync();
                     if (uint32_t(lane) < rest) queue[lane] = moved;
                     queued = rest;
