@@ -1746,7 +1746,7 @@ __device__ v4 sdfFinish(const P& p, const SdfCorners& c) {
 }
 // Exact evaluations per lane per batch; the queue holds one batch plus a row of 64;
 // per wave the contact-node bitmask covers 4,096 nodes.
-constexpr uint32_t SDF_PER_LANE = 3, SDF_QUEUE = 64 * SDF_PER_LANE + 64, SDF_MASK_WORDS = 128;
+constexpr uint32_t SDF_PER_LANE = 2, SDF_QUEUE = 64 * SDF_PER_LANE + 64, SDF_MASK_WORDS = 128;
 // Per template, the squared radius (template units) outside which every grid cell
 // has only nonnegative corners: the largest |corner| over negative grid values, plus
 // two spacings (more than a cell diagonal, covering the rounding of the cell index).
