@@ -100,3 +100,47 @@ def test_sdf_contact_overflow_is_flagged_then_fits(pkg, oracle_cpu, sdf_workload
     a.initialize()
     b.initialize()
     assert_states_equal(a.state(), b.state())
+
+
+def test_sdf_small_templates_match_oracle_exactly(pkg, oracle_cpu):
+    """Templates of 6, 65 and 200 nodes (one partial leaf, a leaf and one node, several
+    partial leaves) and the six-node cube pair's closed form, on the HIP path."""
+    from sand_atlas_dem_amd import geometry
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    print("sdf small templates", flush=True)
+    hip = HipBackend(0)
+    h = 1e-4
+    cube = geometry.with_sdf(geometry.box_shape([h] * 3, nodes=64), 16)
+    cube.samples = np.concatenate([np.eye(3), -np.eye(3)]) * h / 2
+    cube.areas = np.full(6, h * h)
+    spec = pkg.Experiment(particles=2, timestep=1e-9, timestep_policy="active-contact", friction=0.5,
+                          contact_geometry="sdf", max_overlap_fraction=0.1, contacts_per_particle=128)
+    engines = [pkg.Engine([cube], spec, [[0.005] * 3, [0.005 + 9.99e-5, 0.005, 0.005]], [0.01] * 3, [0, 0], [1, 1],
+                          backend=m) for m in (hip, oracle_cpu)]
+    for e in engines:
+        e.initialize()
+    a, b = (e.state() for e in engines)
+    assert_states_equal(a, b)
+    np.testing.assert_allclose(a["force"][0, :3] * engines[0].force_unit, [-1e-3, 0, 0], rtol=2e-4, atol=1e-9)
+    rng = np.random.default_rng(4)
+    v, f = geometry.icosphere(2)
+    shapes = [geometry.with_sdf(geometry.make_shape(f"s{k}", v * 3e-4 * (1 + 0.1 * k), f, nodes=n, seed=k), 20)
+              for k, n in enumerate((6, 65, 200))]
+    n = 200
+    spec = pkg.Experiment(particles=n, timestep=1e-9, contact_geometry="sdf", max_overlap_fraction=0.5,
+                          contacts_per_particle=128)
+    grid = np.stack(np.meshgrid(*[np.arange(6)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n]
+    positions = (grid + 0.5) * 5.6e-4 + rng.uniform(-2e-5, 2e-5, (n, 3))
+    ids = rng.integers(0, 3, n)
+    engines = [pkg.Engine(shapes, spec, positions, [3.36e-3] * 3, ids, [1] * n, backend=m) for m in (hip, oracle_cpu)]
+    q = rng.normal(size=(n, 4))
+    for e in engines:
+        e.set_kinematics(rotation=q / np.linalg.norm(q, axis=1)[:, None])
+        e.initialize()
+    a, b = (e.state() for e in engines)
+    assert b["contacts"] > 50
+    assert_states_equal(a, b)
+    for e in engines:
+        e.advance(3)
+    assert_states_equal(engines[0].state(), engines[1].state())
