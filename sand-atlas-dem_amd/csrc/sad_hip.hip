@@ -1893,7 +1893,9 @@ __global__ __launch_bounds__(256) void k_sdf_contacts(P p, W w) {
     const int lane = threadIdx.x & 63;
     uint32_t* queue = queueMem[threadIdx.x >> 6];
     uint32_t* bits = maskMem[threadIdx.x >> 6];
-    const uint32_t pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // The pair index is the same for the whole wave: held in a scalar register, the
+    // pair's rows (ids, bodies, poses, grid rows) come through scalar loads.
+    const uint32_t pair = uniformU(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (pair >= p.c.pairCapacity) return;
     const bool live = pair < p.control[1] && ld_volatile(p.control) == 0;
     if (EMIT && !live) return;
