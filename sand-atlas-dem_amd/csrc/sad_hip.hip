@@ -1404,7 +1404,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
     uint32_t* queue = queues[wv];
     uint32_t* nearestTri = nearestTris[wv];
     uint64_t* cand = candidates[wv];
-    uint32_t queried = 0;
+    uint32_t queried = 0, skippedNodes = 0;  // statistics, added once per wave at its exit
     // XCD-affine queues: tasks are grouped by target template; a wave drains the
     // queue of its own XCD (workgroups are dispatched round-robin over 8 XCDs)
     // and then steals from the others.
@@ -1641,11 +1641,12 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 an[3] = make_float4(__uint_as_float(f.i), __uint_as_float(f.j),
                                     __uint_as_float(ld_volatile(w.counters + 10 * LINE)), __uint_as_float(1u));
             }
-            if (skipped && lane == 0) atomicAdd(w.counters + 11 * LINE, skipped);
+            skippedNodes += skipped;
         }
         if (lane == 0) w.taskCount[task] = running;
     }
     if (lane == 0 && queried) atomicAdd(w.counters + 2 * LINE, queried);
+    if (lane == 0 && skippedNodes) atomicAdd(w.counters + 11 * LINE, skippedNodes);
 }
 
 // The plain kernel keeps <= 96 VGPRs and 29 KB of LDS per workgroup so five
@@ -2165,10 +2166,18 @@ __global__ void k_clear_storage(P p) {
     for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x)
         if (id < p.c.hashSize) p.hash[id] = DEM_EMPTY;
 }
+// Sliding contacts are counted into control[6] once per wave (ballot + one atomic):
+// per-contact atomics on that one word serialise in L2.
+__device__ __forceinline__ void countSliding(const P& p, bool sliding) {
+    const uint64_t hits = __ballot(sliding), active = __ballot(true);
+    if (hits && int(threadIdx.x & 63) == __ffsll((unsigned long long)active) - 1)
+        atomicAdd(p.control + 6, uint32_t(__popcll(hits)));
+}
 __global__ void k_resolve(P p) {
     if (ld_volatile(p.control) != 0) return;
     uint32_t count = min(p.control[2], p.c.contactCapacity);
     for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < count; id += gridDim.x * blockDim.x) {
+        bool slid = false;
         uint4 key = p.meta[id];
         uint32_t i = key.x, j = key.y;
         float4 na = p.normalArea[id];
@@ -2227,36 +2236,37 @@ __global__ void k_resolve(P p) {
             if (!isfinite(forceEnergy.x) || !isfinite(forceEnergy.y) || !isfinite(forceEnergy.z) ||
                 !isfinite(forceEnergy.w) || !isfinite(lossPower))
                 fail(p, 512);
-            else if (sliding)
-                atomicAdd(p.control + 6, 1u);
-            continue;
+            else
+                slid = sliding;
+        } else {
+            v3 branch = centerBranch(p, i, j);
+            v3 translation = xyz(p.velocity[i]) - xyz(p.velocity[j]);
+            if (viscosity > 0) translation = xyz(p.forceVelocity[i]) - xyz(p.forceVelocity[j]);
+            v3 rv = translation - xyz(p.cell[1]) * branch + cross(wi, arm) - cross(wj, arm - branch);
+            elastic = elastic + (rv - n * dot(rv, n)) * p.c.dt;
+            v3 trial = elastic;
+            float area = na.w, depth = pdv.w;
+            float fn = 0, lossPower = 0;
+            float normalVelocity = dot(rv, n);
+            if (depth > 0) {  // normalResponse (normal.slang:3-11), weight 0.5
+                float el = 0.5f * p.c.kn * area * depth;
+                fn = viscosity == 0 ? el : fmaxf(0.0f, el - 0.5f * viscosity * area * normalVelocity);
+                lossPower = fmaxf(0.0f, (el - fn) * normalVelocity);
+            }
+            p.normalLossPower[id] = lossPower;
+            v3 ft = -0.5f * p.c.kt * area * elastic;
+            float magnitude = length(ft);
+            if (magnitude > p.c.mu * fn && magnitude > 0) {
+                ft = ft * (p.c.mu * fn / magnitude);
+                elastic = -ft / (0.5f * p.c.kt * area);
+                slid = true;
+            }
+            p.slip[id] = f4(elastic, fmaxf(0.0f, -dot(ft, trial - elastic)));
+            float4 cf = f4(fn * n + ft, 0.25f * area * (p.c.kn * depth * depth + p.c.kt * dot(elastic, elastic)));
+            p.contactForce[id] = cf;
+            if (!isfinite(cf.x) || !isfinite(cf.y) || !isfinite(cf.z) || !isfinite(cf.w) || !isfinite(lossPower)) fail(p, 512);
         }
-        v3 branch = centerBranch(p, i, j);
-        v3 translation = xyz(p.velocity[i]) - xyz(p.velocity[j]);
-        if (viscosity > 0) translation = xyz(p.forceVelocity[i]) - xyz(p.forceVelocity[j]);
-        v3 rv = translation - xyz(p.cell[1]) * branch + cross(wi, arm) - cross(wj, arm - branch);
-        elastic = elastic + (rv - n * dot(rv, n)) * p.c.dt;
-        v3 trial = elastic;
-        float area = na.w, depth = pdv.w;
-        float fn = 0, lossPower = 0;
-        float normalVelocity = dot(rv, n);
-        if (depth > 0) {  // normalResponse (normal.slang:3-11), weight 0.5
-            float el = 0.5f * p.c.kn * area * depth;
-            fn = viscosity == 0 ? el : fmaxf(0.0f, el - 0.5f * viscosity * area * normalVelocity);
-            lossPower = fmaxf(0.0f, (el - fn) * normalVelocity);
-        }
-        p.normalLossPower[id] = lossPower;
-        v3 ft = -0.5f * p.c.kt * area * elastic;
-        float magnitude = length(ft);
-        if (magnitude > p.c.mu * fn && magnitude > 0) {
-            ft = ft * (p.c.mu * fn / magnitude);
-            elastic = -ft / (0.5f * p.c.kt * area);
-            atomicAdd(p.control + 6, 1u);
-        }
-        p.slip[id] = f4(elastic, fmaxf(0.0f, -dot(ft, trial - elastic)));
-        float4 cf = f4(fn * n + ft, 0.25f * area * (p.c.kn * depth * depth + p.c.kt * dot(elastic, elastic)));
-        p.contactForce[id] = cf;
-        if (!isfinite(cf.x) || !isfinite(cf.y) || !isfinite(cf.z) || !isfinite(cf.w) || !isfinite(lossPower)) fail(p, 512);
+        countSliding(p, slid);
     }
 }
 __global__ void k_build_hash(P p) {
