@@ -144,3 +144,48 @@ def test_sdf_small_templates_match_oracle_exactly(pkg, oracle_cpu):
     for e in engines:
         e.advance(3)
     assert_states_equal(engines[0].state(), engines[1].state())
+
+
+def test_sdf_restart_matches_oracle_and_continuous_run(oracle_cpu, sdf_workload):
+    """restore() on the SDF geometry (gpu.py:1016-1092): HIP and oracle restores agree
+    exactly; against the continuous run the reference's own gate applies
+    (test_cpu_solver.py:212-238: rtol 2e-6, atol 1e-9)."""
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    print("sdf restart", flush=True)
+    w = sdf_workload
+    runs = {}
+    for name, m in (("hip", HipBackend(0)), ("cpu", oracle_cpu)):
+        e = w.engine(m)
+        e.initialize()
+        e.advance(3)
+        state = e.state()
+        e.advance(3)
+        r = w.engine(m)
+        r.restore(state, 3)
+        r.advance(3)
+        runs[name] = (e.state(), r.state())
+    assert_states_equal(runs["hip"][1], runs["cpu"][1])
+    continuous, restored = runs["hip"]
+    assert np.array_equal(continuous["meta"], restored["meta"])
+    for name in ("position", "velocity", "rotation", "momentum", "force", "torque", "slip", "accumulated"):
+        np.testing.assert_allclose(restored[name], continuous[name], rtol=2e-6, atol=1e-9)
+
+
+def test_sdf_layer_matches_oracle_exactly(oracle_cpu):
+    """SDF grains between z planes in an x-y periodic cell (per-axis periodicity)."""
+    from sand_atlas_dem_amd import assembly, geometry
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    print("sdf layer", flush=True)
+    w = assembly.config_workload("layer", n=300, templates=4, nodes=256)
+    w = replace(w, shapes=[geometry.with_sdf(s, 24) for s in w.shapes], spec=replace(w.spec, contact_geometry="sdf"))
+    a, b = w.engine(HipBackend(0)), w.engine(oracle_cpu)
+    a.initialize()
+    b.initialize()
+    assert b.state()["contacts"] > 100
+    assert_states_equal(a.state(), b.state())
+    for _ in range(3):
+        a.advance(3, rate=w.rate)
+        b.advance(3, rate=w.rate)
+        assert_states_equal(a.state(), b.state())
