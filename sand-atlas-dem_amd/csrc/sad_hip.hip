@@ -1170,6 +1170,9 @@ __device__ bool cullBatch(const P& p, const W& w, const TaskFrame& f, const uint
 // distance and feature equal the traversal's) and stages hits by wave rank: the same
 // contacts, values and ranks as querying each batch to completion.
 constexpr uint32_t NO_TRIANGLE = 0xffffffffu;
+// Widening of the warm-start bound, in template units of distance: about 30 ulps
+// of a coordinate of magnitude 1, far above the closest point's rounding.
+constexpr float WARM_SLACK = 4e-6f;
 __device__ __forceinline__ v3 queryPoint(const TaskFrame& f, float4 sp, v3& arm) {
     arm = qrot(f.qi, xyz(sp) * f.s_i);
     return qinv(f.qj, arm - f.branch) / f.s_j;
@@ -1206,6 +1209,10 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
     // never pruned (their boxes are no farther than the minimum), and the record
     // test still starts from 1e30, so the winner -- the first triangle in visit
     // order at the minimum distance -- is the one the plain traversal finds.
+    // In FP32 a box's distance can exceed the computed distance of a triangle it
+    // holds (the closest point carries a few ulps of rounding), so the bound is
+    // widened by WARM_SLACK in distance: (sqrt(d) + WARM_SLACK)^2.  A looser bound
+    // only visits more boxes; it never changes the winner.
     const uint32_t triFirst = warmTri ? w.triBase[3 * f.sj] : 0u, triCount = warmTri ? w.triBase[3 * f.sj + 1] : 0u;
     auto start = [&]() {
         v3 arm;
@@ -1223,7 +1230,8 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
                 uint32_t feature;
                 const float4* t = p.meshTriangles + 3 * size_t(triFirst + last - 1);
                 const v3 x = closestTriangle(q, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
-                prune = dot(q - x, q - x);
+                const float reach = sqrtf(dot(q - x, q - x)) + WARM_SLACK;
+                prune = reach * reach;
             }
         }
     };
