@@ -9,6 +9,7 @@ these libraries or packings; its own assemblies come from `initial_assembly`
 """
 
 from dataclasses import dataclass
+from pathlib import Path
 
 import numpy as np
 
@@ -272,7 +273,30 @@ def conservative_timestep(shapes, ids, scales, spec):
     return float(np.floor(dt / 10**exponent * 100) / 100 * 10**exponent)
 
 
-def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, steps=None, resolution=None):
+RELAXED = Path(__file__).resolve().parent / "data"
+
+
+def relaxed_path(config, n, seed):
+    return RELAXED / f"relaxed_{config}_{n}_{seed}.npz"
+
+
+def with_relaxed_state(w, path):
+    """The workload with the poses of a relaxed fixture (written by `relax.py`) and the
+    reference's default overlap gate (max_overlap_fraction 0.01, `spec.py:44`)."""
+    from dataclasses import replace
+
+    from .relax import identity
+
+    data = np.load(path, allow_pickle=False)
+    if str(data["identity"]) != identity(w):
+        raise ValueError(f"{path.name} was relaxed from a different assembly")
+    spec = replace(w.spec, max_overlap_fraction=0.01)
+    return replace(w, spec=spec, positions=data["positions"].astype(np.float64), cell=data["cell"].astype(np.float64),
+                   rotations=data["rotations"].astype(np.float64), velocities=np.zeros_like(w.velocities))
+
+
+def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, steps=None, resolution=None,
+                    relaxed=False):
     """The named benchmark configurations (sizes overridable for tests).
 
     small  : 1,000 icospheres (subdiv 2, sigma 0.1), 512 nodes, phi 0.30, v ~ N(0, 0.01 m/s)
@@ -288,7 +312,18 @@ def config_workload(config, *, seed=2024, n=None, templates=None, nodes=None, st
              top plane moving down at 0.5 layer heights per second
     sdf    : the dense assembly with the reference's default contact geometry, SDF
              grids (resolution 64: 64 points across the largest extent, 3 of padding)
+
+    relaxed=True replaces the sphere-packing poses by the committed relaxed state of
+    the same assembly (largest overlap below 1% of a diameter) and raises if there is
+    none for these sizes and seed.
     """
+    if relaxed:
+        w = config_workload(config, seed=seed, n=n, templates=templates, nodes=nodes, steps=steps,
+                            resolution=resolution)
+        path = relaxed_path(config, len(w.ids), seed)
+        if not path.exists():
+            raise FileNotFoundError(f"no relaxed state {path.name}; write it with relax.py on a GPU")
+        return with_relaxed_state(w, path)
     rng = streams(seed)
     d0 = 3e-4
     base = dict(friction=0.5, density=2650.0, tangential_stiffness=0.5)
