@@ -1,15 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark: grain-steps/s of the surface-mesh DEM step on one MI355X.
 
-Workload (default `dense`): 10,000 grains from a 64-template library of perturbed
-icospheres (subdivision 2-4, 320/1,280/5,120 triangles, lognormal(0, 0.15) radial
-noise), 2,048 area-weighted surface nodes per template, solids fraction 0.60 on a
-relaxed sphere packing, isotropic affine compression at 0.5/s, Coulomb mu 0.5,
-FP32 state, synthetic and seeded.  A step is one full explicit timestep (neighbour
-check/rebuild, node->triangle BVH contact collection, friction-history resolve,
-force/torque gather, two half kicks).  The timed region is `Engine.advance(K)` with
-inputs already resident in HBM, bracketed by device synchronisation, as the
-reference's benchmark times `advance` (src/sand_atlas_dem/benchmark.py:244-269).
+Headline workload (default `--config uneven`): 50,000 grains from a 424-template
+library of sand-like spherical-harmonic grains (320-5,120 triangles, log-uniform;
+256-4,096 area-weighted surface nodes in proportion to area), solids fraction 0.60
+on a periodic sphere packing, Coulomb mu 0.5, FP32 state, synthetic and seeded.
+Further workloads (`--extra`, default the 10,000-grain dense library) are measured
+the same way and reported under "extra".  A step is one full explicit timestep
+(neighbour check/rebuild, node->triangle BVH contact collection, friction-history
+resolve, force/torque gather, two half kicks).  The timed region is
+`Engine.advance(K)` with inputs already resident in HBM, bracketed by device
+synchronisation, as the reference's benchmark times `advance`
+(src/sand_atlas_dem/benchmark.py:244-269).
+
+For each workload: the plain HIP path (hipGraph-captured steps, autotuned launch
+shape), a per-kernel HIP-event profile with the dominant kernel's roofline, and the
+learned cull trained on the GPU, audited and timed from the same state.  For the
+headline workload also the C++ oracle on the host cores (multi-thread and single
+thread) with a parity check against the HIP path over the same steps.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -17,6 +25,7 @@ Prints one JSON line on stdout; progress goes to stderr.
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 from pathlib import Path
@@ -132,15 +141,222 @@ def nn_leg(args, w, e, hip, start, step0, plain_state, log_):
     return out
 
 
-def measured_traffic(kernel, config):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary (FETCH_SIZE and
-    WRITE_SIZE, each scaled by its 1 GiB copy calibration), if one exists."""
+def _profile_record(kernel, config, kind):
     names = {"dense": "dense10k", "uneven": "uneven50k"}
-    path = ROOT / "profiles" / f"{kernel}_{names.get(config, config)}_traffic.json"
+    path = ROOT / "profiles" / f"{kernel}_{names.get(config, config)}_{kind}.json"
     if not path.exists():
         return None
     data = json.loads(path.read_text())
-    return data["traffic_bytes"] if data.get("kernel") == kernel and data.get("workload") == config else None
+    data["source"] = str(path.relative_to(ROOT))
+    return data if data.get("kernel") == kernel and data.get("workload") == config else None
+
+
+def measured_traffic(kernel, config):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by its 1 GiB copy calibration), if one exists."""
+    data = _profile_record(kernel, config, "traffic")
+    return data["traffic_bytes"] if data else None
+
+
+def measured_flops(kernel, config):
+    """FP32 operations per launch from the committed rocprofv3 PMC summary (VALU FP32
+    add, multiply, FMA and transcendental wave instructions, times the active lanes
+    per VALU instruction), if one exists."""
+    data = _profile_record(kernel, config, "valu")
+    if not data:
+        return None
+    return {"flops_per_launch": data["flops_per_launch"], "active_lanes": data["active_lanes"],
+            "peak_tflops": 157.3, "source": data["source"]}
+
+
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_legs(args, w, state, step0, hip_rate, log_):
+    """The oracle on the host cores: `cpu_steps` steps of the same workload from the
+    GPU's state on min(16, cores) threads, checked against the HIP path over the same
+    steps (the parity leg); then one step on a single thread, of the same workload or,
+    for large ones, of a smaller assembly drawn from the same library."""
+    import numpy as np
+    from oracle import oracle
+    from sand_atlas_dem_amd import assembly
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    pkg = load_package()
+    cores = host_cores()
+    threads = args.cpu_threads or min(16, cores)
+    cpu = w.engine(oracle.make_oracle(pkg, threads=threads))
+    cpu.restore(state, step0)
+    ref = w.engine(HipBackend(0, graph=True))
+    ref.restore(state, step0)
+    t0 = time.perf_counter()
+    cpu.advance(args.cpu_steps, rate=hip_rate)
+    cpu_s = time.perf_counter() - t0
+    ref.advance(args.cpu_steps, rate=hip_rate)
+    a, b = ref.state(), cpu.state()
+    errs = {}
+    for name in ("force", "torque", "position", "rotation"):
+        x, y = a[name][:, :3].astype(np.float64), b[name][:, :3].astype(np.float64)
+        errs[name] = float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-30))
+    parity = {"contacts_equal": bool(a["contacts"] == b["contacts"] and np.array_equal(a["meta"], b["meta"])),
+              "states_identical": all(np.array_equal(a[k], b[k]) for k in ("force", "torque", "position", "rotation",
+                                                                              "velocity", "slip")),
+              "max_rel_err": errs, "steps": args.cpu_steps, "contacts": int(a["contacts"])}
+    n = len(w.ids)
+    many = {"value": n * args.cpu_steps / cpu_s, "threads": threads, "steps": args.cpu_steps, "grains": n}
+    log_(f"[bench] cpu oracle on {threads} threads: {many['value']:.4g} grain-steps/s; parity {parity}")
+    single_n = n if n <= args.cpu_single_max else args.cpu_single_max
+    if single_n == n:
+        one_w, one_state, one_step = w, state, step0
+    else:
+        one_w = assembly.config_workload(w.name, seed=args.seed, n=single_n)
+        one_state, one_step = None, 0
+    one = one_w.engine(oracle.make_oracle(pkg, threads=1))
+    if one_state is None:
+        one.initialize()
+    else:
+        one.restore(one_state, one_step)
+    t0 = time.perf_counter()
+    one.advance(1, rate=hip_rate)
+    one_s = time.perf_counter() - t0
+    single = {"value": single_n / one_s, "threads": 1, "steps": 1, "grains": single_n}
+    log_(f"[bench] cpu oracle on 1 thread: {single['value']:.4g} grain-steps/s ({single_n} grains)")
+    baseline = {
+        "value": many["value"],
+        "unit": "grain-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{args.cpu_steps} steps of the same {n}-grain workload from the GPU's state: the C++ oracle "
+                  f"(serial kernels, order-preserving pair-parallel collector) on {threads} threads",
+        "cpu_model": cpu_model(),
+        "host_cores": cores,
+        "single_thread": dict(single, sample=(
+            f"1 step of the same workload on 1 thread" if single_n == n else
+            f"1 step of a {single_n}-grain assembly from the same template library and solids fraction "
+            f"on 1 thread (the full {n}-grain step would take minutes)")),
+    }
+    return baseline, parity
+
+
+def measure(args, config, log_, *, primary):
+    """Everything measured on one workload; returns a block of the JSON line."""
+    import numpy as np
+    import torch
+    from sand_atlas_dem_amd import assembly, autotune
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    t0 = time.perf_counter()
+    n_override = args.grains if primary else None
+    w = assembly.config_workload(config, seed=args.seed, n=n_override)
+    log_(f"[bench] workload {config}: {len(w.ids)} grains, {len(w.shapes)} templates, "
+         f"dt {w.spec.timestep:.3g} s, generated in {time.perf_counter() - t0:.1f} s")
+    hip = HipBackend(0, graph=not args.no_graph)
+    t0 = time.perf_counter()
+    e = w.engine(hip)
+    e.initialize()
+    log_(f"[bench] engine + initial forces {time.perf_counter() - t0:.1f} s")
+    e.advance(args.warmup, rate=w.rate)
+    hip.wait()
+    tuned = None
+    sdf = not e.mesh_geometry
+    if not args.no_autotune and not sdf:
+        tuned = autotune.autotune(e, rate=w.rate, steps=6, log=log_)
+        log_(f"[bench] autotuned launch shape {tuned['best']} ({tuned['ms_per_step']:.3f} ms/step)")
+    start, step0 = e.state(), e.step
+    e.restore(start, step0)
+    e.advance(1, rate=w.rate)  # capture the graphs outside the timed region
+    e.restore(start, step0)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.advance(args.steps, rate=w.rate)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n = e.n
+    value = n * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log_(f"[bench] {config}: timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} grain-steps/s")
+    plain_state = e.state()
+
+    # Per-kernel HIP-event profile over a separate window (same stream as the launches).
+    before = hip.stats()
+    hip.profile(True)
+    e.advance(args.profile_steps, rate=w.rate)
+    hip.wait()
+    prof = hip.profile_read()
+    hip.profile(False)
+    after = hip.stats()
+    steps_p = after["collections"] - before["collections"]
+    pd = (after["directed_pairs"] - before["directed_pairs"]) / steps_p
+    q = (after["queried_nodes"] - before["queried_nodes"]) / steps_p
+    c = (after["contacts"] - before["contacts"]) / steps_p
+    skipped = (after["skipped_nodes"] - before["skipped_nodes"]) / steps_p
+    sdf_static = sum(16 * len(s.samples) + 4 * s.sdf.size for s in w.shapes) if sdf else 0
+    kernels = {}
+    total_ms = sum(v[0] for v in prof.values())
+    for name, (ms, launches) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        avg = ms / launches
+        b = kernel_bytes(name, n, pd, q, c, e.hash_size, sdf_static)
+        kernels[name] = {"avg_ms": round(avg, 5), "share": round(ms / total_ms, 4),
+                         **({"gbps": round(b / (avg * 1e-3) / 1e9, 2)} if b else {})}
+    dominant = max(prof, key=lambda k: prof[k][0])
+    dom_ms = prof[dominant][0] / prof[dominant][1]
+    dom_bytes = kernel_bytes(dominant, n, pd, q, c, e.hash_size, sdf_static) or 0
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    step_bytes = 296 * n + (752 * pd + sdf_static if sdf else 92 * pd + 20 * q) + 240 * c
+    log_(f"[bench] dominant kernel {dominant}: {dom_ms:.3f} ms/launch; P_d {pd:.0f} Q {q:.0f} C {c:.0f} per step")
+    flops = measured_flops(dominant, config)
+    block = {
+        "workload": f"{config}: {WORKLOADS[config]}",
+        "grains": n,
+        "value": value,
+        "ms_per_step": ms_per_step,
+        "launch_shape": tuned["best"] if tuned else None,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved,
+            "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / (PEAK_HBM / 1e9),
+            "traffic": measured_traffic(dominant, config),
+            "launch_ms": dom_ms,
+        },
+        "step_roofline": {
+            "bytes_per_step": step_bytes,
+            "achieved_gbps": step_bytes / (ms_per_step * 1e-3) / 1e9,
+            "frac": step_bytes / (ms_per_step * 1e-3) / PEAK_HBM,
+            "per_step": {"directed_pairs": pd, "queried_nodes": q, "contacts": c, "skipped_nodes": skipped,
+                         "contacts_per_grain": c / n},
+        },
+        "fp32": None if flops is None else dict(flops, achieved_tflops=flops["flops_per_launch"] / (dom_ms * 1e-3) / 1e12),
+        "kernels": kernels,
+    }
+    if primary and not args.no_cpu_baseline:
+        block["cpu_baseline"], block["parity"] = cpu_legs(args, w, plain_state, step0 + args.steps, w.rate, log_)
+    nn = None if args.no_nn or sdf else nn_leg(args, w, e, hip, start, step0, plain_state, log_)
+    if nn and nn.get("value"):
+        nn["speedup_vs_plain"] = nn["value"] / value
+    block["nn_culled"] = nn
+    if e.wall_count:  # wall reactions and work after the timed run (SI units)
+        e.restore(plain_state, step0 + args.steps)
+        block["walls"] = [{"normal": x["normal"], "offset_m": x["offset"], "reaction_N": list(x["reaction"]),
+                           "work_J": x["work"], "contacts": x["active_surface_contacts"]} for x in e.walls()]
+    return block
 
 
 def main():
@@ -148,11 +364,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="dense", choices=sorted(WORKLOADS))
+    ap.add_argument("--config", default="uneven", choices=sorted(WORKLOADS))
+    ap.add_argument("--grains", type=int, default=None, help="override the headline workload's grain count")
+    ap.add_argument("--extra", default="dense", help="comma-separated further workloads measured after the "
+                                                     "headline (plain + NN-culled), or 'none'")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps in the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="oracle steps in the multi-thread CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--cpu-single-max", type=int, default=5000,
+                    help="largest assembly the single-thread CPU leg steps (a smaller one is drawn beyond)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one instead of a captured step")
     ap.add_argument("--no-autotune", action="store_true")
@@ -178,154 +399,38 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
-
-    pkg = load_package()
-    from sand_atlas_dem_amd import assembly, autotune
-    from sand_atlas_dem_amd.hip import HipBackend
-    import numpy as np
-    import torch
-
-    t0 = time.perf_counter()
-    w = assembly.config_workload(args.config, seed=args.seed)
-    log(f"[bench] workload {args.config}: {len(w.ids)} grains, {len(w.shapes)} templates, "
-        f"dt {w.spec.timestep:.3g} s, generated in {time.perf_counter() - t0:.1f} s")
-    hip = HipBackend(0, graph=not args.no_graph)
-    t0 = time.perf_counter()
-    e = w.engine(hip)
-    e.initialize()
-    log(f"[bench] engine + initial forces {time.perf_counter() - t0:.1f} s")
-    e.advance(args.warmup, rate=w.rate)
-    hip.wait()
-    log(f"[bench] warmup {args.warmup} steps done")
-    tuned = None
-    sdf = not e.mesh_geometry  # no collection launch shape to tune, no learned cull
-    if not args.no_autotune and not sdf:
-        tuned = autotune.autotune(e, rate=w.rate, steps=6, log=log)
-        log(f"[bench] autotuned launch shape {tuned['best']} ({tuned['ms_per_step']:.3f} ms/step)")
-    start, step0 = e.state(), e.step
-    e.restore(start, step0)
-    e.advance(1, rate=w.rate)  # capture the graphs outside the timed region
-    e.restore(start, step0)
-
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e.advance(args.steps, rate=w.rate)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    n = e.n
-    value = n * args.steps / elapsed
-    ms_per_step = 1e3 * elapsed / args.steps
-    log(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} grain-steps/s")
-    plain_state = e.state()
-
-    # Per-kernel HIP-event profile over a separate window (same stream as the launches).
-    before = hip.stats()
-    hip.profile(True)
-    e.advance(args.profile_steps, rate=w.rate)
-    hip.wait()
-    prof = hip.profile_read()
-    hip.profile(False)
-    after = hip.stats()
-    steps_p = after["collections"] - before["collections"]
-    pd = (after["directed_pairs"] - before["directed_pairs"]) / steps_p
-    q = (after["queried_nodes"] - before["queried_nodes"]) / steps_p
-    c = (after["contacts"] - before["contacts"]) / steps_p
-    sdf_static = sum(16 * len(s.samples) + 4 * s.sdf.size for s in w.shapes) if sdf else 0
-    kernels = {}
-    for name, (ms, launches) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
-        avg = ms / launches
-        b = kernel_bytes(name, n, pd, q, c, e.hash_size, sdf_static)
-        kernels[name] = {
-            "avg_ms": round(avg, 5),
-            "share": round(ms / sum(v[0] for v in prof.values()), 4),
-            **({"gbps": round(b / (avg * 1e-3) / 1e9, 2)} if b else {}),
-        }
-    dominant = max(prof, key=lambda k: prof[k][0])
-    dom_ms = prof[dominant][0] / prof[dominant][1]
-    dom_bytes = kernel_bytes(dominant, n, pd, q, c, e.hash_size, sdf_static) or 0
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    step_bytes = 296 * n + (752 * pd + sdf_static if sdf else 92 * pd + 20 * q) + 240 * c
-    log(f"[bench] dominant kernel {dominant}: {dom_ms:.3f} ms/launch; P_d {pd:.0f} Q {q:.0f} C {c:.0f} per step")
-
-    cpu_baseline = None
-    parity = None
-    if not args.no_cpu_baseline:
-        from oracle import oracle
-
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        state = plain_state
-        cpu = w.engine(oracle.make_oracle(pkg, threads=threads))
-        cpu.restore(state, step0 + args.steps)
-        ref = w.engine(hip)
-        ref.restore(state, step0 + args.steps)
-        t0 = time.perf_counter()
-        cpu.advance(args.cpu_steps, rate=w.rate)
-        cpu_s = time.perf_counter() - t0
-        ref.advance(args.cpu_steps, rate=w.rate)
-        a, b = ref.state(), cpu.state()
-        errs = {}
-        for name in ("force", "torque", "position", "rotation"):
-            x, y = a[name][:, :3].astype(np.float64), b[name][:, :3].astype(np.float64)
-            errs[name] = float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-30))
-        parity = {
-            "contacts_equal": bool(a["contacts"] == b["contacts"] and np.array_equal(a["meta"], b["meta"])),
-            "max_rel_err": errs,
-            "steps": args.cpu_steps,
-        }
-        cpu_baseline = {
-            "value": n * args.cpu_steps / cpu_s,
-            "unit": "grain-steps/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{args.cpu_steps} step(s) of the same {n}-grain workload from the GPU's state, C++ oracle "
-                      f"(serial kernels, pair-parallel collector on {threads} threads)",
-        }
-        log(f"[bench] cpu baseline {cpu_baseline['value']:.4g} grain-steps/s on {threads} threads; parity {parity}")
-
-    nn = None if args.no_nn or sdf else nn_leg(args, w, e, hip, start, step0, plain_state, log)
-    if nn and nn.get("value"):
-        nn["speedup_vs_plain"] = nn["value"] / value
-
+    load_package()
+    head = measure(args, args.config, log, primary=True)
+    extras = {}
+    for config in [c for c in args.extra.split(",") if c and c != "none" and c != args.config]:
+        extras[config] = measure(args, config, log, primary=False)
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "grain-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded templates and packing)",
-        "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "grains": n,
+        "config": {"workload": head["workload"], "grains": head["grains"],
                    "path": "plain HIP" + (" (hipGraph-captured steps)" if not args.no_graph else ""),
-                   "launch_shape": tuned["best"] if tuned else None, "seed": args.seed},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dominant,
-            "achieved": achieved,
-            "peak": PEAK_HBM / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / (PEAK_HBM / 1e9),
-            "traffic": measured_traffic(dominant, args.config),
-        },
-        "step_roofline": {
-            "bytes_per_step": step_bytes,
-            "achieved_gbps": step_bytes / (ms_per_step * 1e-3) / 1e9,
-            "frac": step_bytes / (ms_per_step * 1e-3) / PEAK_HBM,
-            "per_step": {"directed_pairs": pd, "queried_nodes": q, "contacts": c},
-        },
-        "kernels": kernels,
-        "cpu_baseline": cpu_baseline,
-        "parity": parity,
-        "nn_culled": nn,
+                   "launch_shape": head["launch_shape"], "seed": args.seed},
+        "roofline": head["roofline"],
+        "cpu_baseline": head.get("cpu_baseline"),
+        "parity": head.get("parity"),
+        "nn_culled": head["nn_culled"],
+        "step_roofline": head["step_roofline"],
+        "fp32": head["fp32"],
+        "kernels": head["kernels"],
+        "extra": extras,
     }
-    if e.wall_count:  # wall reactions and work after the timed run (SI units)
-        e.restore(plain_state, step0 + args.steps)
-        out["walls"] = [{"normal": x["normal"], "offset_m": x["offset"], "reaction_N": list(x["reaction"]),
-                         "work_J": x["work"], "contacts": x["active_surface_contacts"]} for x in e.walls()]
+    if "walls" in head:
+        out["walls"] = head["walls"]
     print(json.dumps(out), flush=True)
 
 

@@ -43,7 +43,7 @@ def identity(w):
 
 
 def relax(w, backend, *, target=0.008, damping=0.05, chunk=250, max_steps=60_000, rest=0.01, start_phi=None,
-          compress_steps=20_000, log=print):
+          compress_steps=20_000, settle_steps=0, log=print):
     """Relax workload `w`; returns (positions m, rotations, cell m, report).
 
     damping: velocity loss per step (drag = damping / dt).  rest: stop only once the
@@ -51,6 +51,8 @@ def relax(w, backend, *, target=0.008, damping=0.05, chunk=250, max_steps=60_000
     first expand the assembly affinely to that solids fraction, relax it there, then
     compress it back to the workload's cell over `compress_steps` steps at a constant
     isotropic log rate (the reference's strain stage, `cell[1]`), relaxing throughout.
+    settle_steps: before compressing, hold the expanded assembly for at most this many
+    steps, or until its largest overlap is below `target`.
     """
     from dataclasses import replace
 
@@ -74,11 +76,13 @@ def relax(w, backend, *, target=0.008, damping=0.05, chunk=250, max_steps=60_000
     steps = 0
     compress = 0 if stretch == 1.0 else compress_steps
     rate = np.full(3, -np.log(stretch) / max(compress, 1) / spec.timestep)
-    phase = "compress" if compress else "hold"
+    phase = "settle" if compress and settle_steps else ("compress" if compress else "hold")
+    compressed = 0
     while steps < max_steps:
-        n = chunk if phase == "hold" else min(chunk, compress - steps)
+        n = min(chunk, compress - compressed) if phase == "compress" else chunk
         e.advance(n, drag=drag, wall_velocities=walls, rate=rate if phase == "compress" else (0, 0, 0))
         steps += n
+        compressed += n if phase == "compress" else 0
         d = e.diagnostics()
         speed = float(np.abs(e.read("velocity")[: e.n, :3]).max()) * e.length_unit / e.time_unit
         row = dict(step=steps, phase=phase, max_overlap=d["max_overlap_fraction"],
@@ -88,8 +92,12 @@ def relax(w, backend, *, target=0.008, damping=0.05, chunk=250, max_steps=60_000
                    seconds=time.perf_counter() - t0)
         history.append(row)
         log(json.dumps(row))
+        if phase == "settle":
+            if d["max_overlap_fraction"] < target or steps >= settle_steps:
+                phase = "compress"
+            continue
         if phase == "compress":
-            if steps >= compress:
+            if compressed >= compress:
                 phase = "hold"
             continue
         if d["max_overlap_fraction"] < target and speed < rest:
@@ -136,6 +144,7 @@ def main():
     ap.add_argument("--max-steps", type=int, default=60_000)
     ap.add_argument("--start-phi", type=float, default=None)
     ap.add_argument("--compress-steps", type=int, default=20_000)
+    ap.add_argument("--settle-steps", type=int, default=0)
     args = ap.parse_args()
     t0 = time.perf_counter()
     w = assembly.config_workload(args.config, seed=args.seed, n=args.n, relaxed=False)
@@ -143,7 +152,8 @@ def main():
     backend = HipBackend(0, graph=True)
     positions, rotations, cell, report = package.relax(
         w, backend, target=args.target, damping=args.damping, chunk=args.chunk, max_steps=args.max_steps,
-        start_phi=args.start_phi, compress_steps=args.compress_steps, log=lambda s: print(s, flush=True))
+        start_phi=args.start_phi, compress_steps=args.compress_steps, settle_steps=args.settle_steps,
+        log=lambda s: print(s, flush=True))
     package.save(args.out, w, positions, rotations, cell, report)
     print(json.dumps({k: v for k, v in report.items() if k != "history"}), flush=True)
 

@@ -10,6 +10,13 @@ Sources (data files the reference holds, read with numpy's non-pickling loader):
   evidence/mesh-face-normal-probe-v2/baseline.npz
       52 points around a 100 um cube with analytic distance and normal, and the
       production shader's output.
+  evidence/{brick-mesh-query,adaptive-mesh-query,adaptive-breadth-query}-probe-v1/
+      shape-{0,1}-cap-*.npz
+      Three further sets of 2,820 normalized FP32 points per shape (cell boundaries,
+      ties, external points, shape offsets) around the same box and prism; `expected`
+      = independent FP64 signed distance in metres, `baseline` = the production
+      meshProbe output.  The breadth set was taken on a twice-subdivided box of the
+      same extents (same distances).
 Run once with the reference tree mounted; the fixtures are committed.
 """
 
@@ -33,6 +40,12 @@ def main():
         expected_normal=d["expected_normal"],
         actual=d["actual"],
     )
+    for probe, tag, cap in (("brick-mesh-query-probe-v1", "brick", 96), ("adaptive-mesh-query-probe-v1", "adaptive", 24),
+                            ("adaptive-breadth-query-probe-v1", "breadth", 24)):
+        for sid, shape in ((0, "box"), (1, "concave")):
+            d = np.load(REFERENCE / probe / f"shape-{sid}-cap-{cap}.npz")
+            np.savez_compressed(HERE / f"{tag}_probe_{shape}.npz", points=d["points"], expected=d["expected"],
+                                baseline=d["baseline"])
 
 
 if __name__ == "__main__":

@@ -45,6 +45,23 @@ def test_mesh_distance_against_reference_golden_vectors(pkg, cpu, sid, name):
     assert np.all(np.abs(out[:, :3] - native[:, :3]).max(1) <= bound)
 
 
+@pytest.mark.parametrize("tag", ["brick", "adaptive", "breadth"])
+@pytest.mark.parametrize("sid,shape", [(0, "box"), (1, "concave")])
+def test_mesh_distance_against_reference_edge_case_vectors(pkg, cpu, tag, sid, shape):
+    """evidence/{brick,adaptive,breadth}-*-probe-v1: 2,820 points per shape at cell
+    boundaries, ties, external points and offsets; FP64 truth and the reference's
+    production query, both to the reference's own 2e-9 m gate."""
+    d = np.load(GOLDEN / f"{tag}_probe_{shape}.npz")
+    e = probe_library(pkg, cpu)
+    pts = np.pad(d["points"], ((0, 0), (0, 1))).astype(np.float32)
+    out = cpu.probe("meshProbe", [len(pts), sid], e.bindings(), pts, np.zeros((len(pts), 4), np.float32))
+    l = e.length_unit
+    assert np.abs(out[:, 3].astype(float) * l - d["expected"]).max() < 2e-9
+    assert np.abs(out[:, 3].astype(float) - d["baseline"][:, 3]).max() * l < 2e-9
+    resolved = np.abs(d["baseline"][:, 3]) * l > 2e-9  # off the surface by more than the gate
+    assert np.array_equal(np.sign(out[resolved, 3]), np.sign(d["baseline"][resolved, 3]))
+
+
 def test_box_face_normals_match_analytic_where_resolvable(pkg, cpu):
     """evidence/mesh-face-normal-probe-v2: 100 um cube, analytic distance and normal."""
     d = np.load(GOLDEN / "face_normal_probe_box.npz")
