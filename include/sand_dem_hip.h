@@ -123,6 +123,21 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers,
                     void* const* sides, const uint64_t* side_bytes,
                     void* scratch, void* stream, int graph);
 
+/* The timesteps of sad_hip_advance without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * library workspace is carved from a file-scope static device array of 64 MiB per
+ * GPU instead of the scratch object; the caller still passes every Resources buffer
+ * and both history sides as for sad_hip_advance.  Mesh geometry only, plain launches
+ * (no captured graph, temporal skip or learned cull); results equal sad_hip_advance's.
+ * An assembly whose workspace exceeds the array (about 448 bytes per contact slot
+ * plus per-pair and per-grain rows) returns SAD_E_UNSUPPORTED before any launch,
+ * with sad_hip_last_error() giving the bytes needed.  All calls on one GPU share its
+ * array, so calls that overlap on one GPU race. */
+int sad_hip_advance_static(uint32_t steps, uint32_t active, void* const* buffers,
+                           const uint64_t* bytes, const double* values,
+                           void* const* sides, const uint64_t* side_bytes, void* stream);
+
 /* Probes over the reference's query functions (solver.slang:253-257, 306-318,
  * 418-424): meshProbe (float4 points -> float4 normal,signed distance),
  * meshCullProbe (float4 points -> uint), meshClusterProbe (src,dst -> uint per

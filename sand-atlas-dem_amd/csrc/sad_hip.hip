@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -3608,6 +3609,187 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const
         return SAD_OK;
     } catch (...) {
         std::snprintf(g_error, sizeof(g_error), "exception in advance");
+        return SAD_E_HIP;
+    }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free stepping
+// sad_hip_advance_static: the timesteps of sad_hip_advance with the library
+// workspace carved from one file-scope device array instead of hipMalloc'd
+// buffers, for callers that must not allocate (graph capture, serving loops).
+// Plain launches only: no captured graph, no temporal skip, no learned cull.
+// Each GPU holds its own copy of the array; all calls on one GPU share it, so
+// calls that overlap on one GPU race.  Every call clears the counters it needs.
+namespace {
+
+constexpr size_t STATIC_WORKSPACE_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char g_static_workspace[STATIC_WORKSPACE_BYTES];
+constexpr int STATIC_DEVICES = 64;
+
+__global__ void k_fill_words(uint32_t* words, uint32_t count, uint32_t value) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x)
+        words[k] = value;
+}
+
+struct StaticDevice {
+    unsigned char* base = nullptr;  // this device's address of g_static_workspace
+    ScratchImpl* context = nullptr; // launch settings; its workspace points into base
+};
+StaticDevice g_static[STATIC_DEVICES];
+std::mutex g_static_lock;
+
+// Per device, with that device current: the array's device address (the host-side
+// name is only a handle) and a launch context that owns no device memory.
+int static_device(StaticDevice*& out) {
+    int device = -1;
+    HIP_OK(hipGetDevice(&device));
+    if (device < 0 || device >= STATIC_DEVICES) {
+        std::snprintf(g_error, sizeof(g_error), "device %d beyond the %d the static workspace serves", device,
+                      STATIC_DEVICES);
+        return SAD_E_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> guard(g_static_lock);
+    StaticDevice& d = g_static[device];
+    if (!d.base) {
+        void* address = nullptr;
+        HIP_OK(hipGetSymbolAddress(&address, HIP_SYMBOL(g_static_workspace)));
+        if (!address) {
+            std::snprintf(g_error, sizeof(g_error), "no device address for the static workspace");
+            return SAD_E_HIP;
+        }
+        d.base = static_cast<unsigned char*>(address);
+    }
+    if (!d.context) {
+        auto* context = new ScratchImpl();  // host object; never destroyed (its workspace is static)
+        context->device = device;
+        HIP_OK(hipDeviceGetAttribute(&context->cus, hipDeviceAttributeMultiprocessorCount, device));
+        d.context = context;
+    }
+    out = &d;
+    return SAD_OK;
+}
+
+// Lay the workspace out in the static array: the zero-initialised head (counters,
+// staged counts, totals, queue counters, plane-contact counts) first, then the rest.
+// Returns the bytes used, or 0 with g_error set when the array is too small.
+size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& head) {
+    const Cfg& c = p.c;
+    W& w = sc.w;
+    Caps& caps = sc.caps;
+    size_t used = 0;
+    auto take = [&](auto*& ptr, size_t count, uint32_t& cap) {
+        using T = std::remove_reference_t<decltype(*ptr)>;
+        count = std::max<size_t>(count, 1);
+        ptr = reinterpret_cast<T*>(base + used);
+        used += (count * sizeof(T) + 255) & ~size_t(255);
+        cap = uint32_t(std::min<size_t>(count, 0xffffffffu));
+    };
+    uint32_t unused = 0;
+    const size_t tasks = size_t(2) * c.pairCapacity, wallTasks = size_t(c.wallCount) * c.N;
+    const uint32_t keys = bucketShape(p.templates).keys();
+    take(w.counters, 16 * LINE, caps.counters);
+    take(w.stageCount, 8 * LINE, caps.stageCount);
+    take(w.totals, 8, unused);
+    take(w.xcd, 24 * LINE, caps.xcd);
+    take(w.wallCount, wallTasks, caps.wallCount);
+    head = used;
+    take(w.wallOffset, wallTasks, caps.wallOffset);
+    take(w.taskCount, tasks, caps.taskCount);
+    take(w.taskOffset, tasks + 1, caps.taskOffset);
+    take(w.adjCount, c.N, caps.adjCount);
+    take(w.adjOffset, c.N, caps.adjOffset);
+    take(w.adjCursor, c.N, caps.adjCursor);
+    take(w.adjList, tasks, caps.adjList);
+    take(w.xcdTasks, tasks, caps.xcdTasks);
+    take(w.keyCount, keys, caps.keyCount);
+    take(w.keyStart, keys, caps.keyStart);
+    const size_t widest = std::max<size_t>(std::max<size_t>(tasks + 1, c.N), std::max<size_t>(wallTasks, keys));
+    take(w.sums, (widest + SCAN_TILE - 1) / SCAN_TILE + 1, caps.sums);
+    take(w.wide, 4 * size_t(std::max(p.meshNodes, 1u)), caps.wide);
+    take(w.stageKey, size_t(8) * c.contactCapacity, caps.stageKey);
+    take(w.stageMeta, size_t(8) * c.contactCapacity, caps.stageMeta);
+    take(w.stagePd, size_t(8) * c.contactCapacity, caps.stagePd);
+    take(w.stageNa, size_t(8) * c.contactCapacity, caps.stageNa);
+    if (used > STATIC_WORKSPACE_BYTES) {
+        std::snprintf(g_error, sizeof(g_error),
+                      "allocation-free stepping needs %zu bytes of workspace for this assembly; the static "
+                      "workspace holds %zu (N %u, pair capacity %u, contact capacity %u)",
+                      used, STATIC_WORKSPACE_BYTES, c.N, c.pairCapacity, c.contactCapacity);
+        return 0;
+    }
+    w.stageCap = c.contactCapacity;
+    w.taskCap = uint32_t(tasks);
+    w.nCap = c.N;
+    w.sumsCap = caps.sums;
+    w.wallCap = uint32_t(std::max<size_t>(wallTasks, 1));
+    w.skip = 0;
+    w.warm = 0;
+    w.nnEnabled = 0;
+    w.nnAudit = 0;
+    w.nnTemplates = 0;
+    w.anchor = nullptr;
+    w.nodeDist = nullptr;
+    w.nodeTri = nullptr;
+    w.leafDist = nullptr;
+    w.triBase = nullptr;
+    return used;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sad_hip_advance_static(uint32_t steps, uint32_t active, void* const* buffers, const uint64_t* bytes,
+                           const double* values, void* const* sides, const uint64_t* side_bytes, void* stream) {
+    if (!buffers || !bytes || !values || !sides || !side_bytes) return SAD_E_ARGUMENT;
+    try {
+        StaticDevice* d = nullptr;
+        int rc = static_device(d);
+        if (rc) return rc;
+        ScratchImpl& sc = *d->context;
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        void* table[SAD_BUFFER_COUNT];
+        uint64_t sizes[SAD_BUFFER_COUNT];
+        std::memcpy(table, buffers, sizeof(table));
+        std::memcpy(sizes, bytes, sizeof(sizes));
+        double vals[SAD_VALUE_COUNT];
+        std::memcpy(vals, values, sizeof(vals));
+        for (uint32_t k = 0; k < steps; k++) {
+            const uint32_t side = (active + k) & 1, old = 1 - side;
+            const int slotsNew[4] = {SAD_META, SAD_SLIP, SAD_NORMAL_AREA, SAD_HASH};
+            const int slotsOld[4] = {SAD_OLD_META, SAD_OLD_SLIP, SAD_OLD_NORMAL, SAD_OLD_HASH};
+            for (int q = 0; q < SAD_SIDE_COUNT; q++) {
+                table[slotsNew[q]] = sides[side * SAD_SIDE_COUNT + q];
+                sizes[slotsNew[q]] = side_bytes[side * SAD_SIDE_COUNT + q];
+                table[slotsOld[q]] = sides[old * SAD_SIDE_COUNT + q];
+                sizes[slotsOld[q]] = side_bytes[old * SAD_SIDE_COUNT + q];
+            }
+            if (k > 0) vals[SAD_V_HISTORY_STEP] = 1;
+            P p{};
+            if ((rc = bind(p, table, sizes, vals))) return rc;
+            if (!p.c.meshGeometry) {
+                std::snprintf(g_error, sizeof(g_error), "allocation-free stepping covers the mesh geometry only");
+                return SAD_E_UNSUPPORTED;
+            }
+            if (p.templates > MAX_TEMPLATES) {
+                std::snprintf(g_error, sizeof(g_error), "more than %u templates", MAX_TEMPLATES);
+                return SAD_E_UNSUPPORTED;
+            }
+            size_t head = 0;
+            if (!carve_static(sc, d->base, p, head)) return SAD_E_UNSUPPORTED;
+            if (k == 0) {  // what a previous call left must not leak into this one
+                const uint32_t words = uint32_t(head / 4);
+                hipLaunchKernelGGL(k_fill_words, dim3(std::min<uint32_t>(blocks(words), 1024u)), dim3(256), 0, s,
+                                   reinterpret_cast<uint32_t*>(d->base), words, 0u);
+                HIP_OK(hipGetLastError());
+            }
+            if ((rc = launch_step(p, sc, s))) return rc;
+        }
+        return SAD_OK;
+    } catch (...) {
+        std::snprintf(g_error, sizeof(g_error), "exception in advance_static");
         return SAD_E_HIP;
     }
 }

@@ -101,6 +101,7 @@ def library():
             "sad_hip_scratch_stats": (ct.c_int, [vp, u64p]),
             "sad_hip_dispatch": (ct.c_int, [ct.c_char_p, u32, vpp, u64p, dp, vp, vp]),
             "sad_hip_advance": (ct.c_int, [u32, u32, vpp, u64p, dp, vpp, u64p, vp, vp, ct.c_int]),
+            "sad_hip_advance_static": (ct.c_int, [u32, u32, vpp, u64p, dp, vpp, u64p, vp]),
             "sad_hip_probe": (ct.c_int, [ct.c_char_p, ct.POINTER(u32), vpp, u64p, dp, vp, vp, vp]),
             "sad_hip_profile": (ct.c_int, [vp, ct.c_int]),
             "sad_hip_profile_read": (ct.c_int, [vp, dp, u64p, u32]),
@@ -135,6 +136,7 @@ EXPORTED = (
     "sad_hip_scratch_stats",
     "sad_hip_dispatch",
     "sad_hip_advance",
+    "sad_hip_advance_static",
     "sad_hip_probe",
     "sad_hip_profile",
     "sad_hip_profile_read",
