@@ -17,6 +17,13 @@ from . import native
 from .engine import BUFFER_ORDER, config_values
 
 SIDE_ORDER = ("meta", "slip", "normalArea", "hash")
+# Buffers whose contents the collection stage's temporal-skip anchors depend on
+# (template geometry, node samples, body scales, shape ids).  Poses are not among
+# them: the anchors bound the motion of the poses themselves.
+GEOMETRY = ("shapeId", "body", "sampleInfo", "sample", "meshRoots", "meshHullInfo", "meshPlanes", "meshMin",
+            "meshMax", "meshLinks", "meshTriangles", "meshNormals", "meshFrontier", "meshClusterInfo",
+            "meshClusterRanges", "meshClusterMin", "meshClusterMax", "meshSampleIndices", "grids", "gridOrigin",
+            "distance")
 
 
 def _torch():
@@ -33,7 +40,6 @@ class Buffer:
     def __init__(self, backend, array=None, *, count=None, stride=4):
         torch = backend.torch
         self.backend = backend
-        backend.invalidate()
         if array is not None:
             host = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
             self.tensor = torch.from_numpy(host.copy()).to(backend.torch_device)
@@ -41,6 +47,7 @@ class Buffer:
             self.tensor = torch.zeros(max(4, count * stride), dtype=torch.uint8, device=backend.torch_device)
         self.size = int(self.tensor.numel())
         self.ptr = int(self.tensor.data_ptr())
+        backend.uploaded(self)  # a new allocation may reuse a freed geometry buffer's address
 
     def copy_from_numpy(self, array):
         data = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
@@ -48,7 +55,7 @@ class Buffer:
             raise ValueError("HIP buffer upload size mismatch")
         import torch
 
-        self.backend.invalidate()
+        self.backend.uploaded(self)
         self.tensor.copy_(torch.from_numpy(data.copy()))
 
     def to_numpy(self):
@@ -169,13 +176,25 @@ class HipBackend:
         self.buffer_type = Buffer
         self.kernels = {}
         # Temporal node skip: identical results, fewer exact queries between pose
-        # changes.  Every upload through this adapter drops the anchors, so stored
-        # distances never outlive the geometry they were measured on.
+        # changes.  A new buffer, or an upload into a buffer that a launch then binds
+        # as geometry, drops the anchors, so stored distances never outlive the
+        # geometry they were measured on.
+        self.dirty = set()
         self.tune("temporal_skip", int(bool(temporal_skip)))
 
     def invalidate(self):
-        """Forget the collection stage's temporal-skip anchors (after any upload)."""
+        """Forget the collection stage's temporal-skip anchors."""
         self._check(self.lib.sad_hip_invalidate(self.scratch), "invalidate")
+
+    def uploaded(self, buffer):
+        """Note a host upload; it drops the anchors if a launch binds it as geometry."""
+        self.dirty.add(buffer.ptr)
+
+    def _settle(self, bindings):
+        if self.dirty:
+            if any(name in bindings and bindings[name].ptr in self.dirty for name in GEOMETRY):
+                self.invalidate()
+            self.dirty.clear()
 
     def __del__(self):
         lib, scratch = getattr(self, "lib", None), getattr(self, "scratch", None)
@@ -224,6 +243,7 @@ class HipBackend:
             raise TypeError("solver kernels take the Resources bindings as r=")
 
         def execute():
+            self._settle(r)
             ptrs, sizes = _table(r)
             values = config_values(r["c"])
             self._check(
@@ -243,6 +263,7 @@ class HipBackend:
         """Whole timesteps in one native call (per-step order of gpu.py:714-726)."""
 
         def execute():
+            self._settle(bindings)
             ptrs, sizes = _table(bindings)
             values = config_values(bindings["c"])
             sides = [history_buffers[s][name] for s in (0, 1) for name in SIDE_ORDER]
@@ -263,6 +284,7 @@ class HipBackend:
         torch = self.torch
         src = None if data is None else torch.from_numpy(np.ascontiguousarray(data).copy()).to(self.torch_device)
         dst = torch.zeros(out.nbytes, dtype=torch.uint8, device=self.torch_device)
+        self._settle(bindings)
         ptrs, sizes = _table(bindings)
         values = config_values(bindings["c"])
         u = (ct.c_uint32 * 4)(*(list(uniforms) + [0] * (4 - len(uniforms))))
@@ -278,6 +300,7 @@ class HipBackend:
 
     def probe_device(self, name, uniforms, bindings, src, dst):
         """Query probe on device tensors (no host round trip)."""
+        self._settle(bindings)
         ptrs, sizes = _table(bindings)
         values = config_values(bindings["c"])
         u = (ct.c_uint32 * 4)(*(list(uniforms) + [0] * (4 - len(uniforms))))
@@ -331,7 +354,7 @@ class HipBackend:
         if data.size > buffer.size:
             raise ValueError("HIP buffer prefix out of bounds")
         if data.size:
-            self.invalidate()
+            self.uploaded(buffer)
             buffer.tensor[: data.size].copy_(self.torch.from_numpy(data.copy()))
 
     def stats(self):

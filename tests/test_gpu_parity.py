@@ -202,3 +202,18 @@ def test_batched_hull_tests_equal_reference(pkg, hip):
             assert np.array_equal(ref, fast), (src, dst)
             checked += int(ref.sum())
     assert checked > 0
+
+
+@pytest.mark.parametrize("seed,scales", [(531, [1.0, 0.7]), (532, [4.0, 0.2]), (533, [0.2, 4.0])])
+def test_pair_bounds_match_oracle(pkg, hip, oracle_cpu, seed, scales):
+    """meshPairBoundsProbe and the pair mask of prepareMeshPairs bit-equal to the oracle
+    (which is pinned by FP64 corner enclosure, tests/test_oracle.py)."""
+    import test_oracle as t
+
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=(2, 4))
+    q = (q / np.linalg.norm(q, axis=1)[:, None]).astype(np.float32)
+    out = [t.prepared_pair_bounds(t.pair_bounds_engine(pkg, m, scales), q) for m in (hip, oracle_cpu)]
+    assert out[0][0] == out[1][0] == 3 and out[0][1] == out[1][1]
+    for a, b in zip(out[0][2], out[1][2]):
+        assert same(a, b)

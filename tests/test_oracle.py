@@ -236,3 +236,57 @@ def test_parallel_collector_equals_serial(pkg, oracle_module):
         states.append(e.state())
     for name in ("meta", "slip", "normalArea", "force", "torque", "position", "rotation"):
         assert np.array_equal(states[0][name], states[1][name]), name
+
+
+def pair_bounds_engine(pkg, backend, scales):
+    """Two 100 x 70 x 40 um boxes (1,024 nodes) at one point, the reference's fixture of
+    test_parallel_mesh.py:135-200."""
+    shape = pkg.geometry.box_shape([1e-4, 7e-5, 4e-5], nodes=1024)
+    spec = pkg.Experiment(periodic=False, particles=2, contact_geometry="mesh", timestep=1e-10, pairs_per_particle=4)
+    return pkg.Engine([shape], spec, [[0.003] * 3, [0.003] * 3], [0.01] * 3, [0, 0], scales, backend=backend)
+
+
+def prepared_pair_bounds(e, q):
+    """Rotations q, one candidate pair (0, 1), prepareMeshPairs; then every cluster's
+    hoisted bounds in both directions (meshPairBoundsProbe)."""
+    e.buffers["rotation"].copy_from_numpy(q.astype(np.float32))
+    control = np.zeros(16, np.uint32)
+    control[1] = 1
+    e.buffers["control"].copy_from_numpy(control)
+    e.m.write_prefix(e.buffers["pairs"], np.array([[0, 1]], np.uint32))
+    e.m.dispatch("prepareMeshPairs", e.pair_capacity, r=e.bindings(dt=0))
+    e.m.device.wait()
+    mask = int(e.m.read(e.buffers["meshPairMask"], np.uint32)[0])
+    count = e.buffers["meshClusterRanges"].size // 16
+    bounds = [e.m.probe("meshPairBoundsProbe", [count, direction, 0], e.bindings(dt=0), None,
+                        np.zeros((count, 2, 4), np.float32)) for direction in (0, 1)]
+    return mask, count, bounds
+
+
+@pytest.mark.parametrize("seed,scales", [(531, [1.0, 0.7]), (532, [4.0, 0.2]), (533, [0.2, 4.0])])
+def test_pair_bounds_enclose_exact_fp64_corners(pkg, cpu, seed, scales):
+    """meshPairBounds (solver.slang:404-417): each cluster box, carried into the other
+    grain's frame by the hoisted affine map and padded, contains the FP64 images of
+    all eight corners (the reference's gate, test_parallel_mesh.py:135-200)."""
+    e = pair_bounds_engine(pkg, cpu, scales)
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=(2, 4))
+    q = (q / np.linalg.norm(q, axis=1)[:, None]).astype(np.float32)
+    mask, count, bounds = prepared_pair_bounds(e, q)
+    assert mask == 3
+    low = e.read("meshClusterMin")[:count, :3].astype(float)
+    high = e.read("meshClusterMax")[:count, :3].astype(float)
+    corners = np.stack([np.where(np.array([c & 1, c & 2, c & 4], bool), high, low) for c in range(8)], axis=1)
+
+    def rotate(r, v):
+        return v + 2 * np.cross(r[:3], np.cross(r[:3], v) + r[3] * v)
+
+    for direction, box in enumerate(bounds):
+        src, dst = (0, 1) if direction == 0 else (1, 0)
+        conj = q[dst].astype(float) * [-1, -1, -1, 1]
+        exact = rotate(conj, rotate(q[src].astype(float), corners * float(e.body[src, 0]))) / float(e.body[dst, 0])
+        lo, hi = box[:, 0, None, :3].astype(float), box[:, 1, None, :3].astype(float)
+        assert np.all(exact >= lo) and np.all(exact <= hi)
+        # and tight: no wider than the exact image's extent plus 1e-3 of the coordinates' size
+        slack = 1e-3 * (1 + np.abs(exact).max())
+        assert np.all((hi - lo)[:, 0] <= (exact.max(1) - exact.min(1)) + slack)
