@@ -180,7 +180,8 @@ int sad_hip_reduce_kernel(const char* kernel, uint32_t n, uint32_t walls, uint32
 /* Tunables of the collection stage (no reference counterpart; none changes any
  * result): "collect_blocks_per_cu" (1..32 workgroups per CU in the persistent
  * launch), "collect_pop" (1..16 tasks taken per queue atomic), "collect_queue"
- * (128 or 256 queued nodes per wavefront), "temporal_skip" (0/1, default 0: pass
+ * (128 or 256 queued nodes per wavefront), "collect_leaf_batch" (1..64, default 32:
+ * the exact queries stop a descent pass once this many lanes hold a leaf to test), "temporal_skip" (0/1, default 0: pass
  * over nodes whose signed distance at an anchor pose exceeds a rigorous bound on
  * their motion since), "skip_limit_micro" (2000..250000: re-anchor a directed pair
  * once its motion bound reaches this many 1e-6 template units).  value < 0 only

@@ -22,6 +22,7 @@ SPACE = {
     "collect_blocks_per_cu": (2, 3, 4, 6, 8),
     "collect_pop": (1, 2, 4),
     "collect_queue": (128, 256),
+    "collect_leaf_batch": (8, 16, 32, 64),
 }
 
 

@@ -140,6 +140,7 @@ struct W {
     unsigned long long* nnStats;  // [0] culled, [LINE/2] audited culled nodes, [LINE] audit misses
     uint32_t stageCap, taskCap, nCap, sumsCap, wallCap;
     uint32_t pop;         // tasks taken per queue atomic (tunable)
+    uint32_t leafBatch;   // exact queries: lanes holding a leaf that end a descent pass (tunable)
 };
 
 constexpr int WAVE = 64;
@@ -931,23 +932,26 @@ __global__ void k_adj_fill(P p, W w) {
     }
 }
 // Each grain's pair ids in ascending order (the order the gather walks).  One wave
-// per grain: up to 64 ids are placed by rank (ids are distinct) from LDS; longer
-// lists fall back to an insertion sort by one lane.
+// per grain: the ids are staged in LDS and each is placed by its rank (ids are
+// distinct), 64 at a time, so a list of n ids costs n^2/64 LDS reads per lane.
+// Lists longer than ADJ_SORT_LDS (none in practice: a grain has tens of candidate
+// pairs) fall back to an insertion sort by one lane.
+constexpr uint32_t ADJ_SORT_LDS = 1024;
 __global__ __launch_bounds__(256) void k_adj_sort(P p, W w) {
-    __shared__ uint32_t idsMem[4][64];
+    __shared__ uint32_t idsMem[4][ADJ_SORT_LDS];
     const int lane = threadIdx.x & 63;
     const uint32_t id = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (id >= p.c.N) return;
     uint32_t* a = w.adjList + w.adjOffset[id];
     const uint32_t n = w.adjCount[id];
-    if (n <= 64) {
+    if (n <= ADJ_SORT_LDS) {
         uint32_t* ids = idsMem[threadIdx.x >> 6];
-        const uint32_t mine = uint32_t(lane) < n ? a[lane] : 0u;
-        if (uint32_t(lane) < n) ids[lane] = mine;
+        for (uint32_t k = lane; k < n; k += WAVE) ids[k] = a[k];
         wave_sync();
-        if (uint32_t(lane) < n) {
+        for (uint32_t k = lane; k < n; k += WAVE) {
+            const uint32_t mine = ids[k];
             uint32_t rank = 0;
-            for (uint32_t k = 0; k < n; k++) rank += ids[k] < mine ? 1u : 0u;
+            for (uint32_t m = 0; m < n; m++) rank += ids[m] < mine ? 1u : 0u;
             a[rank] = mine;
         }
         return;
@@ -1238,49 +1242,68 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
     };
     if (active) start();
     uint32_t next = WAVE;  // first unassigned queue entry (wave-uniform)
+    // Each lane walks its own stack in its own order; the wave alternates between
+    // descending (lanes pop and test internal nodes until they hold a leaf or their
+    // stack is empty) and testing the held leaves' triangles together, so the cheap
+    // box tests and the costly triangle tests no longer diverge within one pass.  A
+    // lane stops descending while it holds a leaf and resumes only after that leaf's
+    // triangles updated its bound: its pops, pruning decisions and record updates
+    // happen in exactly the order of the one-entry-per-pass walk.
+    const uint32_t leafBatch = w.leafBatch ? w.leafBatch : uint32_t(WAVE);
+    bool hasLeaf = false;
+    uint32_t leafRef = 0;
     while (__ballot(active) != 0) {
-        if (active) {
-            --top;
-            const uint32_t ref = sref[top * WAVE];
-            const float bound = fminf(best, prune);
-            if (!(dload(sdist + top * WAVE) > bound)) {
-                if ((ref & LEAF) == 0) {
-                    const float4* node = wide + 4 * size_t(ref);
-                    float4 w0 = node[0], w1 = node[1], w2 = node[2], w3 = node[3];
-                    float da = boxDistance(xyz(w0), xyz(w1), q), db = boxDistance(xyz(w2), xyz(w3), q);
-                    uint32_t a = __float_as_uint(w0.w), b = __float_as_uint(w1.w);
-                    if (da < db) { uint32_t t = a; a = b; b = t; float u = da; da = db; db = u; }
-                    if (top + 2 > WSTACK) {
-                        fail(p, 1024);
-                        top = 0;
-                        bestTri = NO_TRIANGLE;
+        for (;;) {
+            const bool descend = active && !hasLeaf && top > 0;
+            if (__ballot(descend) == 0 || uint32_t(__popcll(__ballot(hasLeaf))) >= leafBatch) break;
+            if (descend) {
+                --top;
+                const uint32_t ref = sref[top * WAVE];
+                const float bound = fminf(best, prune);
+                if (!(dload(sdist + top * WAVE) > bound)) {
+                    if ((ref & LEAF) == 0) {
+                        const float4* node = wide + 4 * size_t(ref);
+                        float4 w0 = node[0], w1 = node[1], w2 = node[2], w3 = node[3];
+                        float da = boxDistance(xyz(w0), xyz(w1), q), db = boxDistance(xyz(w2), xyz(w3), q);
+                        uint32_t a = __float_as_uint(w0.w), b = __float_as_uint(w1.w);
+                        if (da < db) { uint32_t t = a; a = b; b = t; float u = da; da = db; db = u; }
+                        if (top + 2 > WSTACK) {
+                            fail(p, 1024);
+                            top = 0;
+                            bestTri = NO_TRIANGLE;
+                        } else {
+                            if (da <= bound) { sref[top * WAVE] = a; dstore(sdist + top * WAVE, da); top++; }
+                            if (db <= bound) { sref[top * WAVE] = b; dstore(sdist + top * WAVE, db); top++; }
+                        }
                     } else {
-                        if (da <= bound) { sref[top * WAVE] = a; dstore(sdist + top * WAVE, da); top++; }
-                        if (db <= bound) { sref[top * WAVE] = b; dstore(sdist + top * WAVE, db); top++; }
-                    }
-                } else {
-                    uint32_t first = ref & 0x0fffffffu, count = ((ref >> 28) & 3u) + 1;
-                    for (uint32_t c = 0; c < count; c++) {
-                        uint32_t triangle = first + c, feature;
-                        const float4* t = p.meshTriangles + 3 * size_t(triangle);
-                        const v3 a = xyz(t[0]);
-                        // A triangle whose plane is farther than the current bound
-                        // cannot reach the minimum (the distance to a triangle is at
-                        // least the distance to its plane; 1e-6 absolute and 1e-4
-                        // relative cover the FP32 normal and both evaluations), so
-                        // skipping it cannot change which triangle wins.
-                        const float h = fabsf(dot(q - a, xyz(p.meshNormals[7 * size_t(triangle)]))) - 1e-6f;
-                        if (h > 0.0f && h * h > 1.0001f * fminf(best, prune)) continue;
-                        v3 x = closestTriangle(q, a, xyz(t[1]), xyz(t[2]), feature);
-                        float distance = dot(q - x, q - x);
-                        if (distance < best) { best = distance; bestTri = triangle; }
+                        hasLeaf = true;
+                        leafRef = ref;
                     }
                 }
             }
-            if (top == 0) {
-                nearestTri[k] = bestTri;
-                active = false;
+        }
+        if (hasLeaf) {
+            const uint32_t first = leafRef & 0x0fffffffu, count = ((leafRef >> 28) & 3u) + 1;
+            for (uint32_t c = 0; c < count; c++) {
+                uint32_t triangle = first + c, feature;
+                const float4* t = p.meshTriangles + 3 * size_t(triangle);
+                const v3 a = xyz(t[0]);
+                // A triangle whose plane is farther than the current bound cannot
+                // reach the minimum (the distance to a triangle is at least the
+                // distance to its plane; 1e-6 absolute and 1e-4 relative cover the
+                // FP32 normal and both evaluations), so skipping it cannot change
+                // which triangle wins.
+                const float h = fabsf(dot(q - a, xyz(p.meshNormals[7 * size_t(triangle)]))) - 1e-6f;
+                if (h > 0.0f && h * h > 1.0001f * fminf(best, prune)) continue;
+                v3 x = closestTriangle(q, a, xyz(t[1]), xyz(t[2]), feature);
+                float distance = dot(q - x, q - x);
+                if (distance < best) { best = distance; bestTri = triangle; }
             }
+            hasLeaf = false;
+        }
+        if (active && top == 0) {
+            nearestTri[k] = bestTri;
+            active = false;
         }
         const uint64_t idle = __ballot(!active);
         if (next < qn && idle != 0) {
@@ -2825,6 +2848,7 @@ struct ScratchImpl : Scratch {
     int blocksPerCU = SAD_COLLECT_BLOCKS_PER_CU;
     uint32_t pop = POP;
     int queue = QCAP;
+    uint32_t leafBatch = 32;
     // Temporal node skip (sad_hip_tune "temporal_skip", "skip_limit_micro"): off by
     // default; the caller that enables it calls sad_hip_invalidate whenever it
     // rewrites template geometry, body scales or shape ids in place.
@@ -3310,6 +3334,7 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         LAUNCH(sc, KID_bucket_fill, k_bucket_fill, gridStride, T, s, p, w);
         const dim3 persistent(std::max(1, sc.cus) * sc.blocksPerCU);
         w.pop = sc.pop;
+        w.leafBatch = sc.leafBatch;
         if (w.nnEnabled) {
             if (sc.queue == 128)
                 LAUNCH(sc, KID_collect, (k_collect_nn<128>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
@@ -3600,6 +3625,7 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const
             if ((rc = sc->sync_epoch(s))) return rc;
             if (graph && !sc->profiling) {
                 sc->w.pop = sc->pop;
+                sc->w.leafBatch = sc->leafBatch;
                 rc = graph_step(p, *sc, graph_key(table, sizes, vals, *sc), s);
             } else {
                 rc = launch_step(p, *sc, s);
@@ -3816,6 +3842,12 @@ int sad_hip_tune(void* scratch, const char* name, int64_t value, int64_t* previo
         if (value >= 0) {
             if (value != 128 && value != 256) return SAD_E_ARGUMENT;
             sc->queue = int(value);
+        }
+    } else if (key == "collect_leaf_batch") {
+        old = sc->leafBatch;
+        if (value >= 0) {
+            if (value < 1 || value > 64) return SAD_E_ARGUMENT;
+            sc->leafBatch = uint32_t(value);
         }
     } else if (key == "temporal_skip") {
         old = sc->skip;

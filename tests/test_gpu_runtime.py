@@ -47,8 +47,10 @@ def test_graph_captured_steps_match_eager(dense):
 
 
 @pytest.mark.parametrize("setting", [
-    {"collect_blocks_per_cu": 2, "collect_pop": 4, "collect_queue": 128, "skip_limit_micro": 2000},
-    {"collect_blocks_per_cu": 8, "collect_pop": 2, "collect_queue": 256, "skip_limit_micro": 250000},
+    {"collect_blocks_per_cu": 2, "collect_pop": 4, "collect_queue": 128, "skip_limit_micro": 2000,
+     "collect_leaf_batch": 1},
+    {"collect_blocks_per_cu": 8, "collect_pop": 2, "collect_queue": 256, "skip_limit_micro": 250000,
+     "collect_leaf_batch": 64},
 ])
 def test_launch_shapes_do_not_change_results(dense, setting):
     from sand_atlas_dem_amd.hip import HipBackend
@@ -65,7 +67,8 @@ def test_tune_rejects_bad_values(pkg):
     from sand_atlas_dem_amd.hip import HipBackend
 
     m = HipBackend(0)
-    for name, value in (("collect_queue", 64), ("collect_pop", 0), ("collect_blocks_per_cu", 99), ("nope", 1)):
+    for name, value in (("collect_queue", 64), ("collect_pop", 0), ("collect_blocks_per_cu", 99),
+                        ("collect_leaf_batch", 0), ("collect_leaf_batch", 65), ("nope", 1)):
         with pytest.raises(RuntimeError):
             m.tune(name, value)
 
