@@ -305,6 +305,7 @@ def measure(args, config, log_, *, primary):
     q = (after["queried_nodes"] - before["queried_nodes"]) / steps_p
     c = (after["contacts"] - before["contacts"]) / steps_p
     skipped = (after["skipped_nodes"] - before["skipped_nodes"]) / steps_p
+    unchanged = (after["unchanged_winners"] - before["unchanged_winners"]) / steps_p
     sdf_static = sum(16 * len(s.samples) + 4 * s.sdf.size for s in w.shapes) if sdf else 0
     kernels = {}
     total_ms = sum(v[0] for v in prof.values())
@@ -318,7 +319,8 @@ def measure(args, config, log_, *, primary):
     dom_bytes = kernel_bytes(dominant, n, pd, q, c, e.hash_size, sdf_static) or 0
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     step_bytes = 296 * n + (752 * pd + sdf_static if sdf else 92 * pd + 20 * q) + 240 * c
-    log_(f"[bench] dominant kernel {dominant}: {dom_ms:.3f} ms/launch; P_d {pd:.0f} Q {q:.0f} C {c:.0f} per step")
+    log_(f"[bench] dominant kernel {dominant}: {dom_ms:.3f} ms/launch; P_d {pd:.0f} Q {q:.0f} C {c:.0f} per step, "
+         f"{unchanged:.0f} queries kept their nearest triangle without a traversal")
     flops = measured_flops(dominant, config)
     block = {
         "workload": f"{config}: {WORKLOADS[config]}",
@@ -341,7 +343,7 @@ def measure(args, config, log_, *, primary):
             "achieved_gbps": step_bytes / (ms_per_step * 1e-3) / 1e9,
             "frac": step_bytes / (ms_per_step * 1e-3) / PEAK_HBM,
             "per_step": {"directed_pairs": pd, "queried_nodes": q, "contacts": c, "skipped_nodes": skipped,
-                         "contacts_per_grain": c / n},
+                         "unchanged_winners": unchanged, "contacts_per_grain": c / n},
         },
         "fp32": None if flops is None else dict(flops, achieved_tflops=flops["flops_per_launch"] / (dom_ms * 1e-3) / 1e12),
         "kernels": kernels,

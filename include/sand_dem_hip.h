@@ -79,9 +79,10 @@ enum sad_side { SAD_SIDE_META = 0, SAD_SIDE_SLIP, SAD_SIDE_NORMAL_AREA, SAD_SIDE
  * demand, never shrinks.  Returns NULL on failure. */
 void* sad_hip_scratch_create(void);
 void sad_hip_scratch_destroy(void* scratch);
-/* stats[0..8]: cumulative steps (collections), directed candidate pairs, exactly
+/* stats[0..9]: cumulative steps (collections), directed candidate pairs, exactly
  * queried nodes and contacts since creation; workspace bytes; learned-cull culled,
- * audited and missed nodes; nodes passed over by the temporal skip. */
+ * audited and missed nodes; nodes passed over by the temporal skip; queried nodes
+ * whose nearest triangle was proven unchanged without a traversal. */
 int sad_hip_scratch_stats(void* scratch, uint64_t* stats);
 
 /* Per-kernel HIP-event timing on the launch stream (no reference counterpart;
@@ -181,7 +182,10 @@ int sad_hip_reduce_kernel(const char* kernel, uint32_t n, uint32_t walls, uint32
  * result): "collect_blocks_per_cu" (1..32 workgroups per CU in the persistent
  * launch), "collect_pop" (1..16 tasks taken per queue atomic), "collect_queue"
  * (128 or 256 queued nodes per wavefront), "collect_leaf_batch" (1..64, default 32:
- * the exact queries stop a descent pass once this many lanes hold a leaf to test), "temporal_skip" (0/1, default 0: pass
+ * the exact queries stop a descent pass once this many lanes hold a leaf to test),
+ * "gap_shortcut" (0/1, default 1, with "temporal_skip": a query whose node moved less
+ * than half its stored nearest-triangle gap since the previous collection takes the
+ * previous nearest triangle without a traversal), "temporal_skip" (0/1, default 0: pass
  * over nodes whose signed distance at an anchor pose exceeds a rigorous bound on
  * their motion since), "skip_limit_micro" (2000..250000: re-anchor a directed pair
  * once its motion bound reaches this many 1e-6 template units).  value < 0 only

@@ -141,6 +141,13 @@ struct W {
     uint32_t stageCap, taskCap, nCap, sumsCap, wallCap;
     uint32_t pop;         // tasks taken per queue atomic (tunable)
     uint32_t leafBatch;   // exact queries: lanes holding a leaf that end a descent pass (tunable)
+    // Unchanged-winner shortcut (with the temporal skip and the warm start): per task
+    // the frame of its previous collection (qi, qj, (branch, generation), (i, j,
+    // epoch, 1)); per node slot (generation << 16) | gap, the gap in GAP_QUANTUM
+    // units: a lower bound, at the last query, of how much nearer its nearest
+    // triangle was than any other.
+    float4* lastFrame;    // 4 rows per task
+    uint32_t* nodeGap;    // nodeStride words per task
 };
 
 constexpr int WAVE = 64;
@@ -971,6 +978,7 @@ __global__ void k_reset_mesh(P p, W w) {  // resetMeshContacts (solver.slang:397
     if (id == 0) {
         w.counters[2 * LINE] = 0;
         w.counters[11 * LINE] = 0;
+        w.counters[12 * LINE] = 0;
         for (int q = 0; q < 8; q++) w.stageCount[q * LINE] = 0;
     }
     if (ld_volatile(p.control) != 0) return;
@@ -1178,6 +1186,30 @@ constexpr uint32_t NO_TRIANGLE = 0xffffffffu;
 // Widening of the warm-start bound, in template units of distance: about 30 ulps
 // of a coordinate of magnitude 1, far above the closest point's rounding.
 constexpr float WARM_SLACK = 4e-6f;
+// Unchanged-winner shortcut.  A node's query found nearest triangle W at the point
+// x0 and a lower bound L on every other triangle's squared distance (from the
+// triangles it tested, the boxes it pruned and the triangles whose plane it
+// skipped).  In exact arithmetic every other triangle was at least
+// G = sqrt(L) - sqrt(d_W) - 2 GAP_EVAL farther, GAP_EVAL covering the FP32 error of
+// one distance evaluation.  Distances are 1-Lipschitz, so after the node's point
+// has moved by at most m the gap is at least G - 2m; while that stays above
+// 2 GAP_EVAL, W is the unique FP32 minimum at the new point, which is the winner the
+// traversal would return, so the query is just closestTriangle(W) and the stored
+// gap becomes G - 2m.  m comes from the task's previous frame (the same bound as the
+// temporal skip's, with an absolute term for the rounding of the frame differences).
+constexpr float GAP_EVAL = WARM_SLACK;
+constexpr float GAP_QUANTUM = 1.0f / 1048576.0f;  // 2^-20 template units
+constexpr float GAP_MOTION_ABS = 2e-6f;
+struct GapCtx {
+    uint32_t* row;     // this task's per-slot words, or null when the shortcut is off
+    float g0, g1;      // motion since the task's previous collection: g0 + g1 |p|
+    uint32_t genPrev;  // generation written by that collection (0: none usable)
+    uint32_t genNew;   // generation this collection writes
+};
+__device__ __forceinline__ uint32_t gapWord(uint32_t gen, float gap) {
+    const float units = gap > 0.0f ? fminf(gap / GAP_QUANTUM, 65535.0f) : 0.0f;
+    return (gen << 16) | uint32_t(units);  // truncation: rounds the gap down
+}
 __device__ __forceinline__ v3 queryPoint(const TaskFrame& f, float4 sp, v3& arm) {
     arm = qrot(f.qi, xyz(sp) * f.s_i);
     return qinv(f.qj, arm - f.branch) / f.s_j;
@@ -1198,14 +1230,15 @@ __device__ __forceinline__ int8_t quantizeDistance(float d) {
 template <typename D>
 __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
                            const uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
-                           uint32_t& running, int lane, int8_t* refreshDist, uint16_t* warmTri) {
+                           uint32_t& running, int lane, int8_t* refreshDist, uint16_t* warmTri,
+                           const GapCtx& gc, uint32_t& shortcuts) {
     if (qn == 0) return;
     const uint32_t root = p.meshRoots[f.sj].x;
     const uint32_t rootRef = nodeRef(p, root);
     const v3 rootLo = xyz(p.meshMin[root]), rootHi = xyz(p.meshMax[root]);
     const float4* __restrict__ wide = w.wide;
-    uint32_t k = lane, top = 0, bestTri = NO_TRIANGLE;
-    float best = 1e30f, prune = 1e30f;
+    uint32_t k = lane, top = 0, bestTri = NO_TRIANGLE, slotNow = 0, cut = 0;
+    float best = 1e30f, prune = 1e30f, second = 1e30f, kept = -1.0f;
     v3 q = {0, 0, 0};
     bool active = k < qn;
     // Warm start: the squared distance to the node's nearest triangle of an earlier
@@ -1222,16 +1255,34 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
     auto start = [&]() {
         v3 arm;
         const uint32_t node = queue[k] & 0xffffu, slot = queue[k] >> 16;
-        q = queryPoint(f, p.sample[f.sampleBase + node], arm);
+        const float4 sp = p.sample[f.sampleBase + node];
+        q = queryPoint(f, sp, arm);
         sref[0] = rootRef;
         dstore(sdist, boxDistance(rootLo, rootHi, q));
         top = 1;
         best = 1e30f;
         prune = 1e30f;
+        second = 1e30f;
+        kept = -1.0f;
+        slotNow = slot;
         bestTri = NO_TRIANGLE;
         if (warmTri) {
             const uint32_t last = warmTri[slot];
             if (last != 0u && last <= triCount) {
+                if (gc.row && gc.genPrev != 0u) {
+                    const uint32_t word = gc.row[slot];
+                    if ((word >> 16) == gc.genPrev) {
+                        const float left = float(word & 0xffffu) * GAP_QUANTUM -
+                                           2.0f * (gc.g0 + gc.g1 * length(xyz(sp)));
+                        if (left > 2.0f * GAP_EVAL) {  // the winner cannot have changed
+                            bestTri = triFirst + last - 1;
+                            kept = left;
+                            top = 0;
+                            cut++;
+                            return;
+                        }
+                    }
+                }
                 uint32_t feature;
                 const float4* t = p.meshTriangles + 3 * size_t(triFirst + last - 1);
                 const v3 x = closestTriangle(q, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
@@ -1260,7 +1311,8 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
                 --top;
                 const uint32_t ref = sref[top * WAVE];
                 const float bound = fminf(best, prune);
-                if (!(dload(sdist + top * WAVE) > bound)) {
+                const float popped = dload(sdist + top * WAVE);
+                if (!(popped > bound)) {
                     if ((ref & LEAF) == 0) {
                         const float4* node = wide + 4 * size_t(ref);
                         float4 w0 = node[0], w1 = node[1], w2 = node[2], w3 = node[3];
@@ -1273,12 +1325,16 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
                             bestTri = NO_TRIANGLE;
                         } else {
                             if (da <= bound) { sref[top * WAVE] = a; dstore(sdist + top * WAVE, da); top++; }
+                            else second = fminf(second, da);
                             if (db <= bound) { sref[top * WAVE] = b; dstore(sdist + top * WAVE, db); top++; }
+                            else second = fminf(second, db);
                         }
                     } else {
                         hasLeaf = true;
                         leafRef = ref;
                     }
+                } else {
+                    second = fminf(second, popped);  // a pruned subtree: a lower bound for all its triangles
                 }
             }
         }
@@ -1294,15 +1350,34 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
                 // FP32 normal and both evaluations), so skipping it cannot change
                 // which triangle wins.
                 const float h = fabsf(dot(q - a, xyz(p.meshNormals[7 * size_t(triangle)]))) - 1e-6f;
-                if (h > 0.0f && h * h > 1.0001f * fminf(best, prune)) continue;
+                if (h > 0.0f && h * h > 1.0001f * fminf(best, prune)) {
+                    second = fminf(second, h * h);
+                    continue;
+                }
                 v3 x = closestTriangle(q, a, xyz(t[1]), xyz(t[2]), feature);
                 float distance = dot(q - x, q - x);
-                if (distance < best) { best = distance; bestTri = triangle; }
+                if (distance < best) {
+                    second = fminf(second, best);
+                    best = distance;
+                    bestTri = triangle;
+                } else {
+                    second = fminf(second, distance);
+                }
             }
             hasLeaf = false;
         }
         if (active && top == 0) {
             nearestTri[k] = bestTri;
+            if (gc.row) {
+                // The gap this query leaves for the next collection: kept from the
+                // shortcut, or measured by the traversal (0 when there is none).
+                float gap = kept;
+                if (kept < 0.0f)
+                    gap = bestTri != NO_TRIANGLE && second < 1e30f
+                              ? sqrtf(second) - sqrtf(best) - 2.0f * GAP_EVAL - GAP_MOTION_ABS
+                              : 0.0f;
+                gc.row[slotNow] = gapWord(gc.genNew, gap);
+            }
             active = false;
         }
         const uint64_t idle = __ballot(!active);
@@ -1316,6 +1391,8 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
             next += __popcll(idle);
         }
     }
+    for (int d = 1; d < WAVE; d <<= 1) cut += __shfl_xor(cut, d);
+    shortcuts += cut;
     wave_sync();
     for (uint32_t b = 0; b < qn; b += WAVE) {
         const uint32_t kk = b + lane;
@@ -1403,7 +1480,7 @@ template <bool NN, typename D>
 __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
                            uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
                            uint32_t& running, int lane, uint16_t* nnA, uint16_t* nnB, int8_t* refreshDist,
-                           uint16_t* warmTri) {
+                           uint16_t* warmTri, const GapCtx& gc, uint32_t& shortcuts) {
     // Templates whose margin is infinite (calibration found no safe cull) skip the network.
     if constexpr (NN) {
         if (f.sj < w.nnTemplates && w.nnF[size_t(f.sj) * nn::FSTRIDE + nn::FMARGIN] < 1e30f) {
@@ -1411,7 +1488,8 @@ __device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
             wave_sync();
         }
     }
-    queryQueue(p, w, f, task, stageQueue, queue, nearestTri, qn, sref, sdist, running, lane, refreshDist, warmTri);
+    queryQueue(p, w, f, task, stageQueue, queue, nearestTri, qn, sref, sdist, running, lane, refreshDist, warmTri, gc,
+               shortcuts);
 }
 
 // One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
@@ -1436,7 +1514,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
     uint32_t* queue = queues[wv];
     uint32_t* nearestTri = nearestTris[wv];
     uint64_t* cand = candidates[wv];
-    uint32_t queried = 0, skippedNodes = 0;  // statistics, added once per wave at its exit
+    uint32_t queried = 0, skippedNodes = 0, shortcutNodes = 0;  // statistics, added once per wave at its exit
     // XCD-affine queues: tasks are grouped by target template; a wave drains the
     // queue of its own XCD (workgroups are dispatched round-robin over 8 XCDs)
     // and then steals from the others.
@@ -1524,6 +1602,35 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     for (uint32_t k = lane; k < words; k += WAVE) d32[k] = 0u;
                     __threadfence_block();
                 }
+            }
+            // Unchanged-winner shortcut: the motion bound since this task's previous
+            // collection and the generations that tag each node's stored gap.
+            // Generations run 1..65535 per task; on wrap the task's words are
+            // cleared, so a stale word can never carry the expected generation.
+            GapCtx gc{nullptr, 0.0f, 0.0f, 0u, 0u};
+            if (warmTri && w.nodeGap) {
+                const float4* lf = w.lastFrame + 4 * size_t(task);
+                const float4 l0 = lf[0], l1 = lf[1], l2 = lf[2], l3 = lf[3];
+                uint32_t* row = w.nodeGap + size_t(task) * w.nodeStride;
+                const uint32_t genPrev = __float_as_uint(l2.w) & 0xffffu;
+                uint32_t genNew = genPrev + 1u;
+                if (genNew > 0xffffu) {
+                    const uint32_t nodes = p.sampleInfo[f.si].y;
+                    for (uint32_t k = lane; k < nodes; k += WAVE) row[k] = 0u;
+                    __threadfence_block();
+                    genNew = 1u;
+                }
+                const bool valid = __float_as_uint(l3.w) == 1u && __float_as_uint(l3.x) == f.i &&
+                                   __float_as_uint(l3.y) == f.j &&
+                                   __float_as_uint(l3.z) == ld_volatile(w.counters + 10 * LINE) && genNew != 1u;
+                const float dqi = sqrtf(dot4(f.qi - v4of(l0), f.qi - v4of(l0)));
+                const float dqj = sqrtf(dot4(f.qj - v4of(l1), f.qj - v4of(l1)));
+                const float db = length(f.branch - xyz(l2));
+                gc.row = uniformPtr(row);
+                gc.g0 = uniformF(1.001f * (2.0f * dqj * length(f.branch) + db) / f.s_j + GAP_MOTION_ABS);
+                gc.g1 = uniformF(1.001f * (2.0f * dqj + 2.0f * dqi) * f.s_i / f.s_j);
+                gc.genPrev = uniformU(valid ? genPrev : 0u);
+                gc.genNew = uniformU(genNew);
             }
             // Rejection plane: largest violation at the source centre, first in
             // plane order on ties (strict >, initial -1e30), as a wave reduction
@@ -1636,7 +1743,8 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     uint32_t n = __popcll(m);
                     if (qn + n > QN) {
                         wave_sync();
-                        flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist, warmTri);
+                        flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist, warmTri, gc,
+                                          shortcutNodes);
                         queried += qn;
                         qn = 0;
                         wave_sync();
@@ -1646,7 +1754,8 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 }
             }
             wave_sync();
-            flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist, warmTri);
+            flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist, warmTri, gc,
+                                          shortcutNodes);
             queried += qn;
             wave_sync();
             if (refreshDist) {
@@ -1665,6 +1774,14 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     }
                 }
             }
+            if (gc.row && lane == 0) {  // this collection's frame, for the next one's motion bound
+                float4* lf = w.lastFrame + 4 * size_t(task);
+                lf[0] = f4(f.qi);
+                lf[1] = f4(f.qj);
+                lf[2] = f4(f.branch, __uint_as_float(gc.genNew));
+                lf[3] = make_float4(__uint_as_float(f.i), __uint_as_float(f.j),
+                                    __uint_as_float(ld_volatile(w.counters + 10 * LINE)), __uint_as_float(1u));
+            }
             if (refreshDist && lane == 0) {
                 float4* an = w.anchor + 4 * size_t(task);
                 an[0] = f4(f.qi);
@@ -1679,6 +1796,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
     }
     if (lane == 0 && queried) atomicAdd(w.counters + 2 * LINE, queried);
     if (lane == 0 && skippedNodes) atomicAdd(w.counters + 11 * LINE, skippedNodes);
+    if (lane == 0 && shortcutNodes) atomicAdd(w.counters + 12 * LINE, shortcutNodes);
 }
 
 // The plain kernel keeps <= 96 VGPRs and 29 KB of LDS per workgroup so five
@@ -1704,6 +1822,7 @@ __global__ void k_contact_total(P p, W w) {
         atomicAdd(w.totals + 2, (unsigned long long)w.counters[2 * LINE]);
         atomicAdd(w.totals + 3, (unsigned long long)total);
         atomicAdd(w.totals + 4, (unsigned long long)w.counters[11 * LINE]);
+        atomicAdd(w.totals + 5, (unsigned long long)w.counters[12 * LINE]);
     }
 }
 __global__ void k_scatter(P p, W w) {
@@ -2864,6 +2983,12 @@ struct ScratchImpl : Scratch {
     uint32_t* triBase = nullptr;
     uint32_t triBaseCap = 0, maxTriangles = 0, maxClusters = 0;
     size_t leafDistCap = 0;
+    // Unchanged-winner shortcut rows (sad_hip_tune "gap_shortcut", default on).
+    int gapShortcut = 1;
+    uint32_t* gapRows = nullptr;
+    float4* lastFrames = nullptr;
+    size_t gapCap = 0;
+    uint32_t lastFrameCap = 0;
     const void* triBaseFor = nullptr;
     // Captured timesteps keyed by everything a step's launches bake in.
     struct Graph { std::vector<uint64_t> key; hipGraphExec_t exec; };
@@ -2985,6 +3110,8 @@ struct ScratchImpl : Scratch {
         if (key != lastTable) { lastTable.swap(key); epoch++; }
         w.skip = 0;
         w.warm = 0;
+        w.nodeGap = nullptr;
+        w.lastFrame = nullptr;
         if (!skip || p.c.meshGeometry == 0) return SAD_OK;  // the SDF step has no node queries to skip
         if (maxNodesFor != p.sampleInfo || maxNodesShapes != p.c.shapes || triBaseFor != p.meshLinks) {
             if (!probeWord) HIP_OK(hipMalloc(reinterpret_cast<void**>(&probeWord), 3 * sizeof(uint32_t)));
@@ -3012,6 +3139,9 @@ struct ScratchImpl : Scratch {
         const size_t tasks = std::max<size_t>(w.taskCap, 1);
         const size_t bytesNeeded = tasks * stride;
         if (3 * bytesNeeded > (size_t(64) << 30)) return SAD_OK;  // more than 64 GiB of rows: leave it off
+        // The unchanged-winner shortcut's rows (4 bytes per node slot) come on top,
+        // up to 96 GiB more; beyond that the shortcut alone stays off.
+        const bool gaps = maxTriangles <= 65534u && 4 * bytesNeeded <= (size_t(96) << 30);
         if (anchorCap < 4 * tasks) {
             if (w.anchor) { (void)hipFree(w.anchor); bytes -= size_t(anchorCap) * sizeof(float4); }
             w.anchor = nullptr;
@@ -3047,6 +3177,25 @@ struct ScratchImpl : Scratch {
             epoch++;
         }
         w.clusterStride = clusterStride;
+        if (gaps && gapShortcut && (gapCap < 4 * bytesNeeded || lastFrameCap < 4 * tasks)) {
+            if (gapRows) { (void)hipFree(gapRows); bytes -= gapCap; }
+            if (lastFrames) { (void)hipFree(lastFrames); bytes -= size_t(lastFrameCap) * sizeof(float4); }
+            gapRows = nullptr;
+            lastFrames = nullptr;
+            gapCap = 0;
+            lastFrameCap = 0;
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&gapRows), 4 * bytesNeeded));
+            HIP_OK(hipMemset(gapRows, 0, 4 * bytesNeeded));  // generation 0: never a usable gap
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&lastFrames), 4 * tasks * sizeof(float4)));
+            HIP_OK(hipMemset(lastFrames, 0, 4 * tasks * sizeof(float4)));
+            gapCap = 4 * bytesNeeded;
+            lastFrameCap = uint32_t(4 * tasks);
+            bytes += gapCap + size_t(lastFrameCap) * sizeof(float4);
+        }
+        // A null row pointer turns the shortcut off in the kernel.
+        const bool gapsOn = gaps && gapShortcut && gapRows && gapCap >= 4 * bytesNeeded;
+        w.nodeGap = gapsOn ? gapRows : nullptr;
+        w.lastFrame = gapsOn ? lastFrames : nullptr;
         w.triBase = triBase;
         w.warm = maxTriangles <= 65534u ? 1u : 0u;
         w.skipLimit = float(skipLimitMicro) * 1e-6f;
@@ -3074,7 +3223,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
         (void)hipFree(w.anchor); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
-        (void)hipFree(triBase); (void)hipFree(w.leafDist);
+        (void)hipFree(triBase); (void)hipFree(w.leafDist); (void)hipFree(gapRows); (void)hipFree(lastFrames);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
 };
@@ -3446,6 +3595,7 @@ int sad_hip_scratch_stats(void* scratch, uint64_t* stats) {
     }
     for (int k = 0; k < 4; k++) stats[k] = totals[k];
     stats[8] = totals[4];
+    stats[9] = totals[5];
     stats[4] = sc->bytes;
     unsigned long long nnv[3] = {0, 0, 0};
     if (sc->w.nnStats) {
@@ -3760,6 +3910,8 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     w.nodeTri = nullptr;
     w.leafDist = nullptr;
     w.triBase = nullptr;
+    w.nodeGap = nullptr;
+    w.lastFrame = nullptr;
     return used;
 }
 
@@ -3848,6 +4000,12 @@ int sad_hip_tune(void* scratch, const char* name, int64_t value, int64_t* previo
         if (value >= 0) {
             if (value < 1 || value > 64) return SAD_E_ARGUMENT;
             sc->leafBatch = uint32_t(value);
+        }
+    } else if (key == "gap_shortcut") {
+        old = sc->gapShortcut;
+        if (value >= 0) {
+            if (value > 1) return SAD_E_ARGUMENT;
+            sc->gapShortcut = int(value);
         }
     } else if (key == "temporal_skip") {
         old = sc->skip;

@@ -81,3 +81,33 @@ def test_engines_sharing_a_workspace_do_not_share_anchors():
         for _ in range(3):
             alone.advance(3, rate=w.rate)
         _same(e.state(), alone.state())
+
+
+@pytest.mark.parametrize("config", ["dense", "uneven", "walls"])
+def test_unchanged_winner_shortcut_leaves_results_unchanged(config):
+    """A queued node whose point moved less than half its stored nearest-triangle gap
+    since the previous collection takes that triangle without a traversal; the
+    states equal those of full traversals, over many one- and multi-step calls
+    (poses uploaded in between by restore()), while most queries take the shortcut."""
+    from sand_atlas_dem_amd import assembly
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    w = assembly.config_workload(config, n=300, templates=6, nodes=512 if config != "uneven" else None)
+    full = HipBackend(0, graph=True)
+    full.tune("gap_shortcut", 0)
+    cut = HipBackend(0, graph=True)
+    assert cut.tune("gap_shortcut") == 1
+    a, b = w.engine(full), w.engine(cut)
+    a.initialize()
+    b.initialize()
+    for k, steps in enumerate((1, 1, 4, 1, 6, 2, 10)):
+        a.advance(steps, rate=w.rate)
+        b.advance(steps, rate=w.rate)
+        _same(a.state(), b.state())
+        if k == 3:  # rewind both to a checkpoint: the stored gaps must stay sound
+            state, step = a.state(), a.step
+            a.restore(state, step)
+            b.restore(state, step)
+    s = cut.stats()
+    assert full.stats()["unchanged_winners"] == 0
+    assert s["unchanged_winners"] > 0.3 * s["queried_nodes"], s
