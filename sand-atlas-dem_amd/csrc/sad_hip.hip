@@ -1227,6 +1227,71 @@ __device__ __forceinline__ int8_t quantizeDistance(float d) {
     if (!(d == d)) return 0;
     return int8_t(fminf(fmaxf(d * 512.0f, -127.0f), 127.0f));
 }
+// Wave-cooperative hull tests.  Under the temporal skip a leaf or node loop leaves
+// only a few lanes with a hull test to run, and a passing test walks every plane of
+// its group (50-200 on smooth grains) one lane at a time.  These forms run one
+// test with the whole wave: the eight box corners are transformed one per lane and
+// reduced across each group of eight lanes, and the planes are tested 64 at a time.
+// Every value is formed by the one-lane forms' FP32 expressions (a min/max
+// reduction is exact in any order once NaN corners are replaced by the one-lane
+// form's initial +-1e30), so the verdict -- no plane value above zero, inside the
+// root box -- is the same bit for bit.  A failing test returns the largest plane
+// excess instead of the first, an equally valid bound on the distance.
+constexpr uint32_t COOP_LEAVES = 8;  // at most this many leaves per 64-leaf block: cooperative
+constexpr uint32_t COOP_NODES = 8;   // at most this many hull-tested nodes per leaf: cooperative
+__device__ __forceinline__ float waveMax(float v) {
+    for (int d = 1; d < WAVE; d <<= 1) v = fmaxf(v, __shfl_xor(v, d));
+    return v;
+}
+__device__ __forceinline__ float broadcastF(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+// meshClusterOutsideFast of one wave-uniform cluster, all lanes active.
+__device__ float meshClusterOutsideWave(const P& p, v4 qs, v4 qd, float ss, float sd, v3 branch, uint32_t dstShape,
+                                        uint32_t cluster, int lane) {
+    const v3 sl = xyz(p.meshClusterMin[cluster]), sh = xyz(p.meshClusterMax[cluster]);
+    const uint32_t corner = uint32_t(lane) & 7u;
+    v3 q = {(corner & 1) ? sh.x : sl.x, (corner & 2) ? sh.y : sl.y, (corner & 4) ? sh.z : sl.z};
+    q = qinv(qd, qrot(qs, q * ss) - branch) / sd;
+    v3 low = fmin3(mk3(1e30f, 1e30f, 1e30f), q), high = fmax3(mk3(-1e30f, -1e30f, -1e30f), q);
+    for (int d = 1; d < 8; d <<= 1) {
+        low = fmin3(low, mk3(__shfl_xor(low.x, d), __shfl_xor(low.y, d), __shfl_xor(low.z, d)));
+        high = fmax3(high, mk3(__shfl_xor(high.x, d), __shfl_xor(high.y, d), __shfl_xor(high.z, d)));
+    }
+    float pad = 64.0f * DEM_EPS * (1.0f + maxabs(low) + maxabs(high) + length(branch) / sd);
+    low = low - mk3(pad, pad, pad);
+    high = high + mk3(pad, pad, pad);
+    uint32_t root = p.meshRoots[dstShape].x;
+    v3 tl = xyz(p.meshMin[root]), th = xyz(p.meshMax[root]);
+    if (high.x < tl.x || high.y < tl.y || high.z < tl.z || low.x > th.x || low.y > th.y || low.z > th.z)
+        return fmaxf(fmaxf(fmaxf(tl.x - high.x, low.x - th.x), fmaxf(tl.y - high.y, low.y - th.y)),
+                     fmaxf(tl.z - high.z, low.z - th.z));
+    v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
+    v3 direction = (center - 0.5f * (tl + th)) / (th - tl);
+    const uint4 info = p.meshHullInfo[6 * dstShape + __builtin_amdgcn_readfirstlane(dominantGroup(direction))];
+    float worst = 0.0f;
+    for (uint32_t k = lane; k < info.y; k += WAVE) {
+        const float4 pl = p.meshPlanes[info.x + k];
+        worst = fmaxf(worst, dot(xyz(pl), center) + pl.w - dot(fabs3(xyz(pl)), half));
+    }
+    return waveMax(worst);
+}
+// meshOutsideFast of one wave-uniform point, all lanes active.
+__device__ float meshOutsideWave(const P& p, uint32_t shape, v3 q, int lane) {
+    uint32_t root = p.meshRoots[shape].x;
+    v3 low = xyz(p.meshMin[root]), high = xyz(p.meshMax[root]);
+    if (q.x < low.x || q.y < low.y || q.z < low.z || q.x > high.x || q.y > high.y || q.z > high.z)
+        return fmaxf(fmaxf(fmaxf(low.x - q.x, q.x - high.x), fmaxf(low.y - q.y, q.y - high.y)),
+                     fmaxf(low.z - q.z, q.z - high.z));
+    v3 direction = (q - 0.5f * (low + high)) / (high - low);
+    const uint4 info = p.meshHullInfo[6 * shape + __builtin_amdgcn_readfirstlane(dominantGroup(direction))];
+    float worst = 0.0f;
+    for (uint32_t k = lane; k < info.y; k += WAVE) {
+        const float4 pl = p.meshPlanes[info.x + k];
+        worst = fmaxf(worst, dot(xyz(pl), q) + pl.w);
+    }
+    return waveMax(worst);
+}
 template <typename D>
 __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
                            const uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
@@ -1673,7 +1738,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             uint32_t leafSkipped = 0;  // per lane, summed below
             for (uint32_t base = 0; base < info.y; base += WAVE) {
                 const uint32_t c = base + lane;
-                bool ok = false;
+                bool test = false;
                 if (c < info.y) {
                     const uint4 range = p.meshClusterRanges[info.x + c];
                     if (range.y > 0) {
@@ -1684,15 +1749,31 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                             farLeaf = float(leafDist[c]) * SKIP_QUANTUM > c0 + c1 * radius + SKIP_MARGIN;
                             if (farLeaf) leafSkipped += range.y;
                         }
-                        if (!farLeaf) {
-                            const float out =
-                                meshClusterOutsideFast(p, f.qi, f.qj, f.s_i, f.s_j, f.branch, f.sj, info.x + c);
-                            ok = !(out > 0.0f);
-                            if (refreshDist && !ok) leafDist[c] = quantizeDistance(out);
+                        test = !farLeaf;
+                    }
+                }
+                uint64_t tests = __ballot(test), m = 0;
+                if (uint32_t(__popcll(tests)) > COOP_LEAVES) {
+                    bool ok = false;
+                    if (test) {
+                        const float out = meshClusterOutsideFast(p, f.qi, f.qj, f.s_i, f.s_j, f.branch, f.sj, info.x + c);
+                        ok = !(out > 0.0f);
+                        if (refreshDist && !ok) leafDist[c] = quantizeDistance(out);
+                    }
+                    m = __ballot(ok);
+                } else {
+                    while (tests != 0) {
+                        const uint32_t rel = base + uint32_t(__builtin_ctzll(tests));
+                        tests &= tests - 1;
+                        const float out =
+                            meshClusterOutsideWave(p, f.qi, f.qj, f.s_i, f.s_j, f.branch, f.sj, info.x + rel, lane);
+                        if (!(out > 0.0f)) {
+                            m |= 1ull << (rel - base);
+                        } else if (refreshDist && lane == 0) {
+                            leafDist[rel] = quantizeDistance(out);
                         }
                     }
                 }
-                const uint64_t m = __ballot(ok);
                 if (lane == 0) cand[base / WAVE] = m;
             }
             if (skipMode) {
@@ -1709,10 +1790,11 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     const uint32_t rel = word * WAVE + uint32_t(__builtin_ctzll(bits));
                     bits &= bits - 1;
                     const uint4 range = p.meshClusterRanges[info.x + rel];
-                    bool pass_ = false;
+                    bool pass_ = false, test = false;
                     uint32_t node = 0;
                     bool far = false;
                     uint32_t slot = 0;
+                    v3 local = mk3(0.0f, 0.0f, 0.0f);
                     if (uint32_t(lane) < range.y) {
                         node = p.meshSampleIndices[range.x + lane];
                         if (w.skip) slot = range.x + lane - keysBase;
@@ -1728,14 +1810,35 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                             pass_ = true;
                         } else if (!far) {
                             v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
-                            v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
+                            local = qinv(f.qj, arm - f.branch) / f.s_j;
                             float planePad = 64.0f * DEM_EPS * (1.0f + maxabs(local));
-                            // The rejection plane and hull tests; a node they reject
-                            // keeps their evidence as its distance lower bound.
+                            // The rejection plane, then the hull test; a node they
+                            // reject keeps their evidence as its distance lower bound.
                             const float rejection = dot(xyz(f.plane), local) + f.plane.w;
-                            const float outside = rejection > planePad ? rejection : meshOutsideFast(p, f.sj, local);
+                            if (rejection > planePad) {
+                                if (refreshDist) refreshDist[slot] = quantizeDistance(rejection);
+                            } else {
+                                test = true;
+                            }
+                        }
+                    }
+                    uint64_t tests = __ballot(test);
+                    if (uint32_t(__popcll(tests)) > COOP_NODES) {
+                        if (test) {
+                            const float outside = meshOutsideFast(p, f.sj, local);
                             pass_ = !(outside > 0.0f);
                             if (refreshDist && !pass_) refreshDist[slot] = quantizeDistance(outside);
+                        }
+                    } else {
+                        while (tests != 0) {
+                            const int src = __builtin_ctzll(tests);
+                            tests &= tests - 1;
+                            const v3 q = mk3(broadcastF(local.x, src), broadcastF(local.y, src), broadcastF(local.z, src));
+                            const float outside = meshOutsideWave(p, f.sj, q, lane);
+                            if (lane == src) {
+                                pass_ = !(outside > 0.0f);
+                                if (refreshDist && !pass_) refreshDist[slot] = quantizeDistance(outside);
+                            }
                         }
                     }
                     if (skipMode) skipped += __popcll(__ballot(far));
