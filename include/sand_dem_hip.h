@@ -183,9 +183,14 @@ int sad_hip_reduce_kernel(const char* kernel, uint32_t n, uint32_t walls, uint32
  * launch), "collect_pop" (1..16 tasks taken per queue atomic), "collect_queue"
  * (128 or 256 queued nodes per wavefront), "collect_leaf_batch" (1..64, default 32:
  * the exact queries stop a descent pass once this many lanes hold a leaf to test),
- * "gap_shortcut" (0/1, default 1, with "temporal_skip": a query whose node moved less
- * than half its stored nearest-triangle gap since the previous collection takes the
- * previous nearest triangle without a traversal), "temporal_skip" (0/1, default 0: pass
+ * "collect_batch_tasks" (1..16, default 16; at most 8 with the 128-node queue: tasks
+ * whose queued nodes a wavefront queries together), "collect_batch_fill" (1..256,
+ * default 96: queued traversals that end such a batch early), "gap_shortcut" (0/1,
+ * default 1, with "temporal_skip": a query whose node moved less than half its stored
+ * nearest-triangle gap since the previous collection takes the previous nearest
+ * triangle without a traversal), "gap_reach_micro" (0..100000, default 0: a traversal
+ * also visits what lies within this many 1e-6 template units beyond its bound, so the
+ * gap it stores is the true one up to that reach), "temporal_skip" (0/1, default 0: pass
  * over nodes whose signed distance at an anchor pose exceeds a rigorous bound on
  * their motion since), "skip_limit_micro" (2000..250000: re-anchor a directed pair
  * once its motion bound reaches this many 1e-6 template units).  value < 0 only

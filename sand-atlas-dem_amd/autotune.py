@@ -1,10 +1,12 @@
 """Launch-shape autotuner for the collection stage (no reference counterpart).
 
-The collection kernel is a persistent work-queue launch whose shape has three
-knobs (`sad_hip_tune`): workgroups per CU, tasks taken per queue atomic, and the
-per-wavefront query-queue capacity.  None of them changes any result (tasks are
-ranked by id, hits by their running count within a task), so the tuner is free to
-pick whichever is fastest on the workload at hand.  The temporal-skip re-anchor
+The collection kernel is a persistent work-queue launch whose shape has several
+knobs (`sad_hip_tune`): workgroups per CU, tasks taken per queue atomic, the
+per-wavefront query-queue capacity, the leaf batch of the exact queries, the tasks
+and traversals a wavefront queries together, and the gap reach of a traversal.
+None of them changes any result (tasks are ranked by id, hits by their running
+count within a task, and every query returns the same winner), so the tuner is
+free to pick whichever is fastest on the workload at hand.  The temporal-skip re-anchor
 distance is left at its default: its cost shows over hundreds of steps, not in a
 few timed ones.  It times a few real steps per
 candidate from one saved state (coordinate descent over the knobs, defaults
@@ -23,6 +25,9 @@ SPACE = {
     "collect_pop": (1, 2, 4),
     "collect_queue": (128, 256),
     "collect_leaf_batch": (8, 16, 32, 64),
+    "collect_batch_tasks": (8, 16),
+    "collect_batch_fill": (48, 96, 160),
+    "gap_reach_micro": (0, 300, 1000),
 }
 
 

@@ -141,6 +141,9 @@ struct W {
     uint32_t stageCap, taskCap, nCap, sumsCap, wallCap;
     uint32_t pop;         // tasks taken per queue atomic (tunable)
     uint32_t leafBatch;   // exact queries: lanes holding a leaf that end a descent pass (tunable)
+    uint32_t batchTasks;  // exact queries: tasks queried together per wave, 1..BATCH_MAX (tunable)
+    uint32_t batchFill;   // exact queries: traversals queued that end a batch early (tunable)
+    float gapReach;       // exact queries: extra search distance that measures the winner's gap (tunable)
     // Unchanged-winner shortcut (with the temporal skip and the warm start): per task
     // the frame of its previous collection (qi, qj, (branch, generation), (i, j,
     // epoch, 1)); per node slot (generation << 16) | gap, the gap in GAP_QUANTUM
@@ -1170,19 +1173,6 @@ __device__ bool cullBatch(const P& p, const W& w, const TaskFrame& f, const uint
     return lane < 32 ? verdict[0] : second;
 }
 
-// Exact queries of queue[0..qn), ranked in queue order, in two phases.
-//
-// Phase 1 finds every queued node's nearest triangle with lane refill: each lane
-// runs one BVH traversal (meshDistanceWide's exact visit order, one stack entry per
-// wave iteration) and, when it finishes, takes the next unqueried node, so lanes do
-// not idle while the wave's slowest traversal runs (per-query visit counts differ
-// by ~3x).  The winner's triangle index goes to nearest[k].
-//
-// Phase 2 takes the queue 64 nodes at a time in order, rebuilds each query's result
-// from its nearest triangle (closestTriangle is deterministic, so the point, squared
-// distance and feature equal the traversal's) and stages hits by wave rank: the same
-// contacts, values and ranks as querying each batch to completion.
-constexpr uint32_t NO_TRIANGLE = 0xffffffffu;
 // Widening of the warm-start bound, in template units of distance: about 30 ulps
 // of a coordinate of magnitude 1, far above the closest point's rounding.
 constexpr float WARM_SLACK = 4e-6f;
@@ -1195,16 +1185,20 @@ constexpr float WARM_SLACK = 4e-6f;
 // has moved by at most m the gap is at least G - 2m; while that stays above
 // 2 GAP_EVAL, W is the unique FP32 minimum at the new point, which is the winner the
 // traversal would return, so the query is just closestTriangle(W) and the stored
-// gap becomes G - 2m.  m comes from the task's previous frame (the same bound as the
-// temporal skip's, with an absolute term for the rounding of the frame differences).
+// gap becomes G - 2m.  The points are the FP32 query points themselves: the task's
+// frame of its previous collection is kept, so the node's previous point is
+// recomputed bit for bit from it, and m is their computed distance widened by
+// 1e-4 relative and GAP_MOTION_ABS (1 + |x|), far above the rounding of the
+// difference and of the length.
 constexpr float GAP_EVAL = WARM_SLACK;
 constexpr float GAP_QUANTUM = 1.0f / 1048576.0f;  // 2^-20 template units
 constexpr float GAP_MOTION_ABS = 2e-6f;
 struct GapCtx {
-    uint32_t* row;     // this task's per-slot words, or null when the shortcut is off
-    float g0, g1;      // motion since the task's previous collection: g0 + g1 |p|
-    uint32_t genPrev;  // generation written by that collection (0: none usable)
-    uint32_t genNew;   // generation this collection writes
+    uint32_t* row;       // this task's per-slot words, or null when the shortcut is off
+    v4 qiPrev, qjPrev;   // the frame of the task's previous collection
+    v3 branchPrev;
+    uint32_t genPrev;    // generation written by that collection (0: none usable)
+    uint32_t genNew;     // generation this collection writes
 };
 __device__ __forceinline__ uint32_t gapWord(uint32_t gen, float gap) {
     const float units = gap > 0.0f ? fminf(gap / GAP_QUANTUM, 65535.0f) : 0.0f;
@@ -1292,20 +1286,97 @@ __device__ float meshOutsideWave(const P& p, uint32_t shape, v3 q, int lane) {
     }
     return waveMax(worst);
 }
-template <typename D>
-__device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
-                           const uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
-                           uint32_t& running, int lane, int8_t* refreshDist, uint16_t* warmTri,
-                           const GapCtx& gc, uint32_t& shortcuts) {
-    if (qn == 0) return;
+// A wave's query batch.  Under the temporal skip a task queues about 10-16 nodes,
+// so querying one task at a time would leave most of the wave's 64 lanes idle.
+// The wave instead filters several tasks in turn, appending each one's queued
+// nodes to its queue, and queries them together: each queue entry carries the
+// batch slot of its task (held in nearestTri[] until the query overwrites it with
+// the winner), and each lane reads its task's frame from the slot's record.  Hits
+// are then staged task by task, each task's in its queue order, with the task's
+// own running count, so every contact, value and rank is the same as querying
+// each task alone.
+constexpr uint32_t BATCH_MAX = 16;  // with the 256-node queue; 8 with the 128-node queue (LDS)
+struct BatchRec {  // 112 bytes in LDS per task of the batch
+    uint32_t task, root, rootRef, sampleBase;
+    uint32_t triFirst, triCount, gens, end;  // gens = genPrev | genNew << 16; end = queue end | refresh << 16
+    float4 qi, qj;
+    float4 branchSi;  // branch, s_i
+    float4 tail;      // s_j, min(wi, wj), unused, unused
+    uint32_t i, j, unused0, unused1;
+};
+static_assert(sizeof(BatchRec) == 112, "batch record layout");
+
+__device__ __forceinline__ void batchPut(BatchRec* r, const P& p, const W& w, const TaskFrame& f, uint32_t task,
+                                         bool refresh, const GapCtx& gc, int lane) {
     const uint32_t root = p.meshRoots[f.sj].x;
     const uint32_t rootRef = nodeRef(p, root);
-    const v3 rootLo = xyz(p.meshMin[root]), rootHi = xyz(p.meshMax[root]);
+    const uint32_t triFirst = w.warm ? w.triBase[3 * f.sj] : 0u, triCount = w.warm ? w.triBase[3 * f.sj + 1] : 0u;
+    if (lane == 0) {
+        r->task = task; r->root = root; r->rootRef = rootRef; r->sampleBase = f.sampleBase;
+        r->triFirst = triFirst; r->triCount = triCount;
+        r->gens = gc.genPrev | (gc.genNew << 16);
+        r->end = refresh ? 1u << 16 : 0u;
+        r->qi = f4(f.qi); r->qj = f4(f.qj);
+        r->branchSi = f4(f.branch, f.s_i);
+        r->tail = make_float4(f.s_j, fminf(f.wi, f.wj), 0.0f, 0.0f);
+        r->i = f.i; r->j = f.j;
+    }
+}
+
+// Exact queries of queue[0..qn), the entries of batch slots 0..nb-1 in slot order,
+// in two phases.
+//
+// Phase 1 finds every queued node's nearest triangle with lane refill: each lane
+// runs one BVH traversal (meshDistanceWide's exact visit order) for the node of its
+// entry, in its task's frame, and when it finishes takes the next unqueried entry,
+// so lanes do not idle while the wave's slowest traversal runs (per-query visit
+// counts differ by ~3x).  The winner's triangle index goes to nearestTri[k].
+//
+// Phase 2 takes each slot's entries 64 at a time in order, rebuilds each query's
+// result from its nearest triangle (closestTriangle is deterministic, so the point,
+// squared distance and feature equal the traversal's) and stages hits by wave rank:
+// the same contacts, values and ranks as querying each task to completion.  Every
+// slot's hit count goes to taskCount, except the last slot's when its task is still
+// being filtered (`lastOpen`): its count is returned and carried into the next batch.
+constexpr uint32_t NO_TRIANGLE = 0xffffffffu;
+
+// Queue entries carry their state in nearestTri[]: PENDING | batch slot until the
+// entry's traversal ends, then the winner (a triangle index < 2^28, or NO_TRIANGLE).
+// Entries whose winner the unchanged-winner shortcut proved hold it from the start.
+constexpr uint32_t PENDING = 0x40000000u;
+__device__ __forceinline__ bool isPending(uint32_t v) { return (v & 0xc0000000u) == PENDING; }
+// Position of the r-th set bit (from 0) of m; m holds more than r set bits.
+__device__ __forceinline__ uint32_t selectBit(uint64_t m, uint32_t r) {
+    uint32_t pos = 0, c = __popc(uint32_t(m));
+    if (r >= c) { r -= c; m >>= 32; pos += 32; }
+    c = __popc(uint32_t(m) & 0xffffu);
+    if (r >= c) { r -= c; m >>= 16; pos += 16; }
+    c = __popc(uint32_t(m) & 0xffu);
+    if (r >= c) { r -= c; m >>= 8; pos += 8; }
+    c = __popc(uint32_t(m) & 0xfu);
+    if (r >= c) { r -= c; m >>= 4; pos += 4; }
+    c = __popc(uint32_t(m) & 0x3u);
+    if (r >= c) { r -= c; m >>= 2; pos += 2; }
+    if (r >= (uint32_t(m) & 1u)) pos += 1;
+    return pos;
+}
+template <typename D>
+__device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, uint32_t nb, uint32_t stageQueue,
+                               const uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
+                               uint32_t carry, bool lastOpen, int lane) {
     const float4* __restrict__ wide = w.wide;
-    uint32_t k = lane, top = 0, bestTri = NO_TRIANGLE, slotNow = 0, cut = 0;
-    float best = 1e30f, prune = 1e30f, second = 1e30f, kept = -1.0f;
+    const bool gapOn = w.warm && w.nodeGap;
+    uint32_t k = 0, top = 0, bestTri = NO_TRIANGLE, slotNow = 0, taskNow = 0, genNow = 0;
+    float best = 1e30f, prune = 1e30f, second = 1e30f;
+    // Gap reach G: the traversal also visits what lies within G of its bound, so a
+    // recorded gap is at least min(G, the true gap) instead of the distance to the
+    // nearest pruned box; `ext` = (sqrt(best) + G)^2 replaces best in the bound.
+    // Visiting more never changes the winner (the extra triangles lie farther than
+    // the minimum, and the record test stays strict).
+    const float reachG = gapOn ? w.gapReach : 0.0f;
+    float ext = 1e30f;
     v3 q = {0, 0, 0};
-    bool active = k < qn;
+    bool active = false;
     // Warm start: the squared distance to the node's nearest triangle of an earlier
     // step is an upper bound on this query's minimum, so subtrees whose box lies
     // strictly farther can be pruned from the start.  Triangles at the minimum are
@@ -1316,48 +1387,60 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
     // holds (the closest point carries a few ulps of rounding), so the bound is
     // widened by WARM_SLACK in distance: (sqrt(d) + WARM_SLACK)^2.  A looser bound
     // only visits more boxes; it never changes the winner.
-    const uint32_t triFirst = warmTri ? w.triBase[3 * f.sj] : 0u, triCount = warmTri ? w.triBase[3 * f.sj + 1] : 0u;
     auto start = [&]() {
-        v3 arm;
-        const uint32_t node = queue[k] & 0xffffu, slot = queue[k] >> 16;
-        const float4 sp = p.sample[f.sampleBase + node];
-        q = queryPoint(f, sp, arm);
-        sref[0] = rootRef;
-        dstore(sdist, boxDistance(rootLo, rootHi, q));
+        const uint32_t entry = queue[k];
+        const BatchRec& r = batch[nearestTri[k] & 0xffffu];
+        const uint32_t node = entry & 0xffffu, slot = entry >> 16;
+        const uint4 h0 = make_uint4(r.task, r.root, r.rootRef, r.sampleBase);
+        const uint4 h1 = make_uint4(r.triFirst, r.triCount, r.gens, r.end);
+        const float4 qi = r.qi, qj = r.qj, bs = r.branchSi, tail = r.tail;
+        const float4 sp = p.sample[h0.w + node];
+        // queryPoint in the task's frame
+        const v3 arm = qrot(v4of(qi), xyz(sp) * bs.w);
+        q = qinv(v4of(qj), arm - xyz(bs)) / tail.x;
+        sref[0] = h0.z;
+        dstore(sdist, boxDistance(xyz(p.meshMin[h0.y]), xyz(p.meshMax[h0.y]), q));
         top = 1;
         best = 1e30f;
         prune = 1e30f;
         second = 1e30f;
-        kept = -1.0f;
+        ext = 1e30f;
         slotNow = slot;
+        taskNow = h0.x;
+        genNow = h1.z >> 16;
         bestTri = NO_TRIANGLE;
-        if (warmTri) {
-            const uint32_t last = warmTri[slot];
-            if (last != 0u && last <= triCount) {
-                if (gc.row && gc.genPrev != 0u) {
-                    const uint32_t word = gc.row[slot];
-                    if ((word >> 16) == gc.genPrev) {
-                        const float left = float(word & 0xffffu) * GAP_QUANTUM -
-                                           2.0f * (gc.g0 + gc.g1 * length(xyz(sp)));
-                        if (left > 2.0f * GAP_EVAL) {  // the winner cannot have changed
-                            bestTri = triFirst + last - 1;
-                            kept = left;
-                            top = 0;
-                            cut++;
-                            return;
-                        }
-                    }
-                }
+        if (w.warm) {
+            const uint32_t last = w.nodeTri[size_t(h0.x) * w.nodeStride + slot];
+            if (last != 0u && last <= h1.y) {
                 uint32_t feature;
-                const float4* t = p.meshTriangles + 3 * size_t(triFirst + last - 1);
+                const float4* t = p.meshTriangles + 3 * size_t(h1.x + last - 1);
                 const v3 x = closestTriangle(q, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
-                const float reach = sqrtf(dot(q - x, q - x)) + WARM_SLACK;
+                const float reach = sqrtf(dot(q - x, q - x)) + WARM_SLACK + reachG;
                 prune = reach * reach;
             }
         }
     };
-    if (active) start();
-    uint32_t next = WAVE;  // first unassigned queue entry (wave-uniform)
+    // Idle lanes take the next pending entries in queue order.
+    uint32_t next = 0;  // first queue entry not yet scanned (wave-uniform)
+    auto refill = [&]() {
+        for (;;) {
+            const uint64_t idle = __ballot(!active);
+            if (idle == 0 || next >= qn) break;
+            const uint32_t kk = next + lane;
+            const uint64_t pend = __ballot(kk < qn && isPending(nearestTri[kk]));
+            const uint32_t nIdle = __popcll(idle), nPend = __popcll(pend);
+            if (!active) {
+                const uint32_t rank = __popcll(idle & lanes_below(lane));
+                if (rank < nPend) {
+                    k = next + selectBit(pend, rank);
+                    active = true;
+                    start();
+                }
+            }
+            next += nPend <= nIdle ? uint32_t(WAVE) : selectBit(pend, nIdle - 1) + 1;
+        }
+    };
+    refill();
     // Each lane walks its own stack in its own order; the wave alternates between
     // descending (lanes pop and test internal nodes until they hold a leaf or their
     // stack is empty) and testing the held leaves' triangles together, so the cheap
@@ -1375,7 +1458,7 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
             if (descend) {
                 --top;
                 const uint32_t ref = sref[top * WAVE];
-                const float bound = fminf(best, prune);
+                const float bound = fminf(ext, prune);
                 const float popped = dload(sdist + top * WAVE);
                 if (!(popped > bound)) {
                     if ((ref & LEAF) == 0) {
@@ -1415,7 +1498,7 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
                 // FP32 normal and both evaluations), so skipping it cannot change
                 // which triangle wins.
                 const float h = fabsf(dot(q - a, xyz(p.meshNormals[7 * size_t(triangle)]))) - 1e-6f;
-                if (h > 0.0f && h * h > 1.0001f * fminf(best, prune)) {
+                if (h > 0.0f && h * h > 1.0001f * fminf(ext, prune)) {
                     second = fminf(second, h * h);
                     continue;
                 }
@@ -1425,6 +1508,12 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
                     second = fminf(second, best);
                     best = distance;
                     bestTri = triangle;
+                    if (reachG > 0.0f) {
+                        const float e = sqrtf(best) + reachG;
+                        ext = e * e;
+                    } else {
+                        ext = best;
+                    }
                 } else {
                     second = fminf(second, distance);
                 }
@@ -1433,76 +1522,88 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
         }
         if (active && top == 0) {
             nearestTri[k] = bestTri;
-            if (gc.row) {
-                // The gap this query leaves for the next collection: kept from the
-                // shortcut, or measured by the traversal (0 when there is none).
-                float gap = kept;
-                if (kept < 0.0f)
-                    gap = bestTri != NO_TRIANGLE && second < 1e30f
-                              ? sqrtf(second) - sqrtf(best) - 2.0f * GAP_EVAL - GAP_MOTION_ABS
-                              : 0.0f;
-                gc.row[slotNow] = gapWord(gc.genNew, gap);
+            if (gapOn) {
+                // The gap this query leaves for the next collection (0 when there is none).
+                const float gap = bestTri != NO_TRIANGLE && second < 1e30f
+                                      ? sqrtf(second) - sqrtf(best) - 2.0f * GAP_EVAL - GAP_MOTION_ABS
+                                      : 0.0f;
+                w.nodeGap[size_t(taskNow) * w.nodeStride + slotNow] = gapWord(genNow, gap);
             }
             active = false;
         }
-        const uint64_t idle = __ballot(!active);
-        if (next < qn && idle != 0) {
-            const uint32_t slot = next + __popcll(idle & lanes_below(lane));
-            if (!active && slot < qn) {
-                k = slot;
-                active = true;
-                start();
-            }
-            next += __popcll(idle);
-        }
+        refill();
     }
-    for (int d = 1; d < WAVE; d <<= 1) cut += __shfl_xor(cut, d);
-    shortcuts += cut;
     wave_sync();
-    for (uint32_t b = 0; b < qn; b += WAVE) {
-        const uint32_t kk = b + lane;
-        bool hit = false;
-        uint32_t node = 0;
-        float4 pd = make_float4(0, 0, 0, 0), na = make_float4(0, 0, 0, 0);
-        if (kk < qn) {
-            node = queue[kk] & 0xffffu;
-            const uint32_t slot = queue[kk] >> 16;
-            const float4 sp = p.sample[f.sampleBase + node];
-            v3 arm;
-            const v3 x = queryPoint(f, sp, arm);
-            // meshDistanceWide's result from the nearest triangle.
-            const uint32_t triangle = nearestTri[kk];
-            float bestD = 1e30f;
-            v3 nearest = {0, 0, 0}, pseudo = {0, 0, 0};
-            if (triangle != NO_TRIANGLE) {
-                uint32_t feature;
-                const float4* t = p.meshTriangles + 3 * size_t(triangle);
-                nearest = closestTriangle(x, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
-                bestD = dot(x - nearest, x - nearest);
-                pseudo = xyz(p.meshNormals[7 * triangle + feature]);
-            }
-            const float distance = sqrtf(bestD);
-            const float side = dot(x - nearest, pseudo) < 0 ? -1.0f : 1.0f;
-            const v3 normal = distance > 1e-12f ? side * (x - nearest) / distance : pseudo;
-            if (refreshDist) refreshDist[slot] = quantizeDistance(side * distance);
-            if (warmTri && triangle != NO_TRIANGLE) warmTri[slot] = uint16_t(triangle - triFirst + 1);
-            // The rest of queryNode.
-            const float depth = -(side * distance) * f.s_j;
-            pd = f4(arm, depth);
-            if (depth > 0) {
-                const v3 grad = qrot(f.qj, normal);
-                const float len = length(grad);
-                if (len < 1e-7f || !isfinite(depth)) {
-                    fail(p, 2);
-                } else {
-                    na = f4(grad / len, sp.w * f.s_i * f.s_i);
-                    if (depth > p.c.maxOverlap * fminf(f.wi, f.wj)) fail(p, 4);
-                    hit = true;
+    uint32_t from = 0, open = 0;
+    for (uint32_t s = 0; s < nb; s++) {
+        // The slot's frame, read uniformly (scalar registers).
+        const BatchRec& r = batch[s];
+        TaskFrame f;
+        const uint32_t task = uniformU(r.task), end = uniformU(r.end);
+        f.i = r.i; f.j = r.j; f.sampleBase = r.sampleBase;
+        f.qi = v4of(r.qi); f.qj = v4of(r.qj);
+        const float4 bs = r.branchSi, tail = r.tail;
+        f.branch = xyz(bs); f.s_i = bs.w; f.s_j = tail.x; f.wi = tail.y; f.wj = tail.y;
+        f.si = 0; f.sj = 0; f.plane = make_float4(0, 0, 0, 0);
+        uniformFrame(f);
+        const uint32_t triFirst = uniformU(r.triFirst);
+        const uint32_t to = end & 0xffffu;
+        const size_t row = size_t(task) * w.nodeStride;
+        int8_t* refreshDist = (end >> 16) ? w.nodeDist + row : nullptr;
+        uint16_t* warmTri = w.warm ? w.nodeTri + row : nullptr;
+        uint32_t running = s == 0 ? carry : 0u;
+        for (uint32_t b = from; b < to; b += WAVE) {
+            const uint32_t kk = b + lane;
+            bool hit = false;
+            uint32_t node = 0;
+            float4 pd = make_float4(0, 0, 0, 0), na = make_float4(0, 0, 0, 0);
+            if (kk < to) {
+                node = queue[kk] & 0xffffu;
+                const uint32_t slot = queue[kk] >> 16;
+                const float4 sp = p.sample[f.sampleBase + node];
+                v3 arm;
+                const v3 x = queryPoint(f, sp, arm);
+                // meshDistanceWide's result from the nearest triangle.
+                const uint32_t triangle = nearestTri[kk];
+                float bestD = 1e30f;
+                v3 nearest = {0, 0, 0}, pseudo = {0, 0, 0};
+                if (triangle != NO_TRIANGLE) {
+                    uint32_t feature;
+                    const float4* t = p.meshTriangles + 3 * size_t(triangle);
+                    nearest = closestTriangle(x, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
+                    bestD = dot(x - nearest, x - nearest);
+                    pseudo = xyz(p.meshNormals[7 * triangle + feature]);
+                }
+                const float distance = sqrtf(bestD);
+                const float side = dot(x - nearest, pseudo) < 0 ? -1.0f : 1.0f;
+                const v3 normal = distance > 1e-12f ? side * (x - nearest) / distance : pseudo;
+                if (refreshDist) refreshDist[slot] = quantizeDistance(side * distance);
+                if (warmTri && triangle != NO_TRIANGLE) warmTri[slot] = uint16_t(triangle - triFirst + 1);
+                // The rest of queryNode.
+                const float depth = -(side * distance) * f.s_j;
+                pd = f4(arm, depth);
+                if (depth > 0) {
+                    const v3 grad = qrot(f.qj, normal);
+                    const float len = length(grad);
+                    if (len < 1e-7f || !isfinite(depth)) {
+                        fail(p, 2);
+                    } else {
+                        na = f4(grad / len, sp.w * f.s_i * f.s_i);
+                        if (depth > p.c.maxOverlap * fminf(f.wi, f.wj)) fail(p, 4);
+                        hit = true;
+                    }
                 }
             }
+            stageHits(p, w, f, task, stageQueue, hit, node, pd, na, running, lane);
         }
-        stageHits(p, w, f, task, stageQueue, hit, node, pd, na, running, lane);
+        from = to;
+        if (s + 1 < nb || !lastOpen) {
+            if (lane == 0) w.taskCount[task] = running;
+        } else {
+            open = running;
+        }
     }
+    return open;
 }
 
 // Learned cull of the queue: drops nodes whose predicted signed distance exceeds
@@ -1510,13 +1611,14 @@ __device__ void queryQueue(const P& p, const W& w, const TaskFrame& f, uint32_t 
 // are those of the unculled walk).  Audit mode keeps every node, queries it and
 // counts culled nodes that were hits.  Returns the surviving queue length.
 __device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
-                              uint32_t* queue, uint32_t qn, uint32_t* sref, float* sdist, uint32_t& running, int lane,
-                              uint16_t* nnA, uint16_t* nnB) {
+                              uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, float* sdist,
+                              uint32_t& running, int lane, uint16_t* nnA, uint16_t* nnB) {
     uint32_t kept = 0, culledTotal = 0, missTotal = 0;
     for (uint32_t q = 0; q < qn; q += WAVE) {
         uint32_t k = q + lane;
         bool culled = cullBatch(p, w, f, queue, q, qn, nnA, nnB, lane);
         const uint32_t word = k < qn ? queue[k] : 0u, node = word & 0xffffu;
+        const uint32_t state = k < qn ? nearestTri[k] : 0u;
         culledTotal += __popcll(__ballot(culled));
         if (w.nnAudit) {
             bool hit = false;
@@ -1528,7 +1630,11 @@ __device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32
             bool keep = k < qn && !culled;
             uint64_t m = __ballot(keep);
             wave_sync();
-            if (keep) queue[kept + __popcll(m & lanes_below(lane))] = word;
+            if (keep) {
+                const uint32_t at = kept + __popcll(m & lanes_below(lane));
+                queue[at] = word;
+                nearestTri[at] = state;
+            }
             kept += __popcll(m);
             wave_sync();
         }
@@ -1542,19 +1648,22 @@ __device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32
 }
 
 template <bool NN, typename D>
-__device__ void flushQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
-                           uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
-                           uint32_t& running, int lane, uint16_t* nnA, uint16_t* nnB, int8_t* refreshDist,
-                           uint16_t* warmTri, const GapCtx& gc, uint32_t& shortcuts) {
-    // Templates whose margin is infinite (calibration found no safe cull) skip the network.
+__device__ uint32_t flushBatch(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
+                               uint32_t* queue, uint32_t* nearestTri, BatchRec* batch, uint32_t nb, uint32_t qn,
+                               uint32_t* sref, D* sdist, uint32_t carry, bool lastOpen, int lane, uint16_t* nnA,
+                               uint16_t* nnB) {
+    // The learned cull batches one task at a time (the network is the target
+    // template's): nb == 1 and f is that task's frame.  Templates whose margin is
+    // infinite (calibration found no safe cull) skip the network.
     if constexpr (NN) {
         if (f.sj < w.nnTemplates && w.nnF[size_t(f.sj) * nn::FSTRIDE + nn::FMARGIN] < 1e30f) {
-            qn = cullQueue(p, w, f, task, stageQueue, queue, qn, sref, sdist, running, lane, nnA, nnB);
+            qn = cullQueue(p, w, f, task, stageQueue, queue, nearestTri, qn, sref, sdist, carry, lane, nnA, nnB);
+            wave_sync();
+            if (lane == 0) batch[0].end = (batch[0].end & 0xffff0000u) | qn;
             wave_sync();
         }
     }
-    queryQueue(p, w, f, task, stageQueue, queue, nearestTri, qn, sref, sdist, running, lane, refreshDist, warmTri, gc,
-               shortcuts);
+    return queryBatch(p, w, batch, nb, stageQueue, queue, nearestTri, qn, sref, sdist, carry, lastOpen, lane);
 }
 
 // One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
@@ -1571,6 +1680,9 @@ __device__ __forceinline__ void collectBody(P p, W w) {
     __shared__ uint32_t queues[COLLECT_WAVES][QN];
     __shared__ uint32_t nearestTris[COLLECT_WAVES][QN];
     __shared__ uint64_t candidates[COLLECT_WAVES][MAX_CLUSTERS / 64];
+    // The learned cull batches one task; the 128-node queue leaves LDS for 8 records.
+    constexpr uint32_t batchCap = NN ? 1u : (QN >= 256 ? BATCH_MAX : 8u);
+    __shared__ BatchRec batches[COLLECT_WAVES][batchCap];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t* sref = &stackRef[wv][lane];
     D* sdist = &stackDist[wv][lane];
@@ -1579,7 +1691,13 @@ __device__ __forceinline__ void collectBody(P p, W w) {
     uint32_t* queue = queues[wv];
     uint32_t* nearestTri = nearestTris[wv];
     uint64_t* cand = candidates[wv];
+    BatchRec* batch = batches[wv];
     uint32_t queried = 0, skippedNodes = 0, shortcutNodes = 0;  // statistics, added once per wave at its exit
+    // The query batch: nb tasks with qn queued nodes; carry = hits already staged
+    // for the batch's first task when an earlier batch ended inside it.
+    uint32_t nb = 0, qn = 0, carry = 0, pending = 0;  // pending: queued nodes that need a traversal
+    const uint32_t batchLimit = max(1u, min(w.batchTasks, batchCap));
+    TaskFrame f{};
     // XCD-affine queues: tasks are grouped by target template; a wave drains the
     // queue of its own XCD (workgroups are dispatched round-robin over 8 XCDs)
     // and then steals from the others.
@@ -1602,13 +1720,19 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 tried++;
             }
         }
-        if (left == 0) break;
+        if (left == 0) {
+            if (nb > 0) {
+                wave_sync();
+                flushBatch<NN, D>(p, w, f, task, home, queue, nearestTri, batch, nb, qn, sref, sdist, carry, false,
+                                  lane, nnA, nnB);
+                queried += qn;
+            }
+            break;
+        }
         task = w.xcdTasks[qoff[from] + next];
         next++;
         left--;
-        uint32_t running = 0;
         if (ld_volatile(p.control) == 0) {
-            TaskFrame f;
             uint2 ij = p.pairs[task >> 1];
             uint32_t pass = task & 1;
             f.i = pass == 0 ? ij.x : ij.y;
@@ -1672,7 +1796,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             // collection and the generations that tag each node's stored gap.
             // Generations run 1..65535 per task; on wrap the task's words are
             // cleared, so a stale word can never carry the expected generation.
-            GapCtx gc{nullptr, 0.0f, 0.0f, 0u, 0u};
+            GapCtx gc{nullptr, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0}, 0u, 0u};
             if (warmTri && w.nodeGap) {
                 const float4* lf = w.lastFrame + 4 * size_t(task);
                 const float4 l0 = lf[0], l1 = lf[1], l2 = lf[2], l3 = lf[3];
@@ -1688,12 +1812,10 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 const bool valid = __float_as_uint(l3.w) == 1u && __float_as_uint(l3.x) == f.i &&
                                    __float_as_uint(l3.y) == f.j &&
                                    __float_as_uint(l3.z) == ld_volatile(w.counters + 10 * LINE) && genNew != 1u;
-                const float dqi = sqrtf(dot4(f.qi - v4of(l0), f.qi - v4of(l0)));
-                const float dqj = sqrtf(dot4(f.qj - v4of(l1), f.qj - v4of(l1)));
-                const float db = length(f.branch - xyz(l2));
                 gc.row = uniformPtr(row);
-                gc.g0 = uniformF(1.001f * (2.0f * dqj * length(f.branch) + db) / f.s_j + GAP_MOTION_ABS);
-                gc.g1 = uniformF(1.001f * (2.0f * dqj + 2.0f * dqi) * f.s_i / f.s_j);
+                gc.qiPrev = {uniformF(l0.x), uniformF(l0.y), uniformF(l0.z), uniformF(l0.w)};
+                gc.qjPrev = {uniformF(l1.x), uniformF(l1.y), uniformF(l1.z), uniformF(l1.w)};
+                gc.branchPrev = {uniformF(l2.x), uniformF(l2.y), uniformF(l2.z)};
                 gc.genPrev = uniformU(valid ? genPrev : 0u);
                 gc.genNew = uniformU(genNew);
             }
@@ -1725,6 +1847,12 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 f.plane = i0 == 0xffffffffu ? make_float4(0, 0, 0, 0) : p.meshPlanes[i0];
             }
             uniformFrame(f);
+            const bool refresh = refreshDist != nullptr;
+            const bool gapTask = gc.row != nullptr && gc.genPrev != 0u;
+            const uint32_t triFirst = gapTask ? uniformU(w.triBase[3 * f.sj]) : 0u;
+            const uint32_t triCount = gapTask ? uniformU(w.triBase[3 * f.sj + 1]) : 0u;
+            batchPut(batch + nb, p, w, f, task, refresh, gc, lane);
+            nb++;
             // Leaf culls, lane-parallel.  Internal clusters are not tested: a leaf's
             // own test is a valid filter, and a leaf the preorder walk would prune
             // with an ancestor holds only nodes outside the target, so the hits and
@@ -1782,7 +1910,6 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             }
             wave_sync();
             // Candidate leaves in preorder feed the query queue.
-            uint32_t qn = 0;
             const uint32_t words = (info.y + WAVE - 1) / WAVE;
             for (uint32_t word = 0; word < words; word++) {
                 uint64_t bits = cand[word];
@@ -1795,10 +1922,11 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     bool far = false;
                     uint32_t slot = 0;
                     v3 local = mk3(0.0f, 0.0f, 0.0f);
+                    float4 sp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     if (uint32_t(lane) < range.y) {
                         node = p.meshSampleIndices[range.x + lane];
                         if (w.skip) slot = range.x + lane - keysBase;
-                        float4 sp = p.sample[f.sampleBase + node];
+                        sp = p.sample[f.sampleBase + node];
                         bool inside = false;
                         if (skipMode) {
                             const float stored = float(dist[slot]) * SKIP_QUANTUM;
@@ -1842,24 +1970,69 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                         }
                     }
                     if (skipMode) skipped += __popcll(__ballot(far));
+                    // Unchanged-winner shortcut: while the node's stored gap less
+                    // twice its motion since its last query stays above 2 GAP_EVAL,
+                    // its nearest triangle is the same, so it needs no traversal and
+                    // its gap shrinks by twice that motion.
+                    uint32_t done = 0u;  // 1 + the proven nearest triangle, 0: traverse
+                    if (pass_ && gapTask) {
+                        const uint32_t last = warmTri[slot];
+                        if (last != 0u && last <= triCount) {
+                            const uint32_t word = gc.row[slot];
+                            if ((word >> 16) == gc.genPrev) {
+                                const v3 a = xyz(sp) * f.s_i;
+                                const v3 now = qinv(f.qj, qrot(f.qi, a) - f.branch) / f.s_j;  // queryPoint
+                                const v3 then = qinv(gc.qjPrev, qrot(gc.qiPrev, a) - gc.branchPrev) / f.s_j;
+                                const float moved =
+                                    1.0001f * length(now - then) + GAP_MOTION_ABS * (1.0f + maxabs(now));
+                                const float left = float(word & 0xffffu) * GAP_QUANTUM - 2.0f * moved;
+                                if (left > 2.0f * GAP_EVAL) {
+                                    gc.row[slot] = gapWord(gc.genNew, left);
+                                    done = triFirst + last;
+                                }
+                            }
+                        }
+                    }
+                    if (gapTask) shortcutNodes += __popcll(__ballot(done != 0u));
                     uint64_t m = __ballot(pass_);
                     uint32_t n = __popcll(m);
-                    if (qn + n > QN) {
+                    if (qn + n > QN) {  // the batch ends inside this task
                         wave_sync();
-                        flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist, warmTri, gc,
-                                          shortcutNodes);
+                        if (lane == 0) batch[nb - 1].end = qn | (refresh ? 1u << 16 : 0u);
+                        wave_sync();
+                        carry = flushBatch<NN, D>(p, w, f, task, home, queue, nearestTri, batch, nb, qn, sref, sdist,
+                                                  carry, true, lane, nnA, nnB);
                         queried += qn;
                         qn = 0;
+                        pending = 0;
+                        wave_sync();
+                        batchPut(batch, p, w, f, task, refresh, gc, lane);
+                        nb = 1;
                         wave_sync();
                     }
-                    if (pass_) queue[qn + __popcll(m & lanes_below(lane))] = node | (slot << 16);
+                    if (pass_) {
+                        const uint32_t at = qn + __popcll(m & lanes_below(lane));
+                        queue[at] = node | (slot << 16);
+                        nearestTri[at] = done != 0u ? done - 1 : PENDING | (nb - 1);
+                    }
                     qn += n;
+                    pending += __popcll(__ballot(pass_ && done == 0u));
                 }
             }
             wave_sync();
-            flushQueue<NN, D>(p, w, f, task, home, queue, nearestTri, qn, sref, sdist, running, lane, nnA, nnB, refreshDist, warmTri, gc,
-                                          shortcutNodes);
-            queried += qn;
+            if (lane == 0) batch[nb - 1].end = qn | (refresh ? 1u << 16 : 0u);
+            // A re-anchoring task records its leaves' bounds from its queries' results
+            // below, so its batch ends with it.
+            if (refresh || nb >= batchLimit || pending >= w.batchFill) {
+                wave_sync();
+                flushBatch<NN, D>(p, w, f, task, home, queue, nearestTri, batch, nb, qn, sref, sdist, carry, false,
+                                  lane, nnA, nnB);
+                queried += qn;
+                qn = 0;
+                nb = 0;
+                carry = 0;
+                pending = 0;
+            }
             wave_sync();
             if (refreshDist) {
                 // Each candidate leaf's bound: the least of its nodes' stored values.
@@ -1894,8 +2067,9 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                                     __uint_as_float(ld_volatile(w.counters + 10 * LINE)), __uint_as_float(1u));
             }
             skippedNodes += skipped;
+        } else if (lane == 0) {
+            w.taskCount[task] = 0u;
         }
-        if (lane == 0) w.taskCount[task] = running;
     }
     if (lane == 0 && queried) atomicAdd(w.counters + 2 * LINE, queried);
     if (lane == 0 && skippedNodes) atomicAdd(w.counters + 11 * LINE, skippedNodes);
@@ -3071,6 +3245,9 @@ struct ScratchImpl : Scratch {
     uint32_t pop = POP;
     int queue = QCAP;
     uint32_t leafBatch = 32;
+    uint32_t batchTasks = 16;
+    uint32_t batchFill = 96;
+    int gapReachMicro = 0;
     // Temporal node skip (sad_hip_tune "temporal_skip", "skip_limit_micro"): off by
     // default; the caller that enables it calls sad_hip_invalidate whenever it
     // rewrites template geometry, body scales or shape ids in place.
@@ -3587,6 +3764,9 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         const dim3 persistent(std::max(1, sc.cus) * sc.blocksPerCU);
         w.pop = sc.pop;
         w.leafBatch = sc.leafBatch;
+        w.batchTasks = sc.batchTasks;
+        w.batchFill = sc.batchFill;
+        w.gapReach = float(sc.gapReachMicro) * 1e-6f;
         if (w.nnEnabled) {
             if (sc.queue == 128)
                 LAUNCH(sc, KID_collect, (k_collect_nn<128>), persistent, dim3(COLLECT_WAVES * WAVE), s, p, w);
@@ -3879,6 +4059,9 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const
             if (graph && !sc->profiling) {
                 sc->w.pop = sc->pop;
                 sc->w.leafBatch = sc->leafBatch;
+                sc->w.batchTasks = sc->batchTasks;
+                sc->w.batchFill = sc->batchFill;
+                sc->w.gapReach = float(sc->gapReachMicro) * 1e-6f;
                 rc = graph_step(p, *sc, graph_key(table, sizes, vals, *sc), s);
             } else {
                 rc = launch_step(p, *sc, s);
@@ -4103,6 +4286,24 @@ int sad_hip_tune(void* scratch, const char* name, int64_t value, int64_t* previo
         if (value >= 0) {
             if (value < 1 || value > 64) return SAD_E_ARGUMENT;
             sc->leafBatch = uint32_t(value);
+        }
+    } else if (key == "collect_batch_tasks") {
+        old = sc->batchTasks;
+        if (value >= 0) {
+            if (value < 1 || value > int64_t(BATCH_MAX)) return SAD_E_ARGUMENT;
+            sc->batchTasks = uint32_t(value);
+        }
+    } else if (key == "collect_batch_fill") {
+        old = sc->batchFill;
+        if (value >= 0) {
+            if (value < 1 || value > QCAP) return SAD_E_ARGUMENT;
+            sc->batchFill = uint32_t(value);
+        }
+    } else if (key == "gap_reach_micro") {
+        old = sc->gapReachMicro;
+        if (value >= 0) {
+            if (value > 100000) return SAD_E_ARGUMENT;
+            sc->gapReachMicro = int(value);
         }
     } else if (key == "gap_shortcut") {
         old = sc->gapShortcut;
