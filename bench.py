@@ -252,6 +252,41 @@ def cpu_legs(args, w, state, step0, hip_rate, log_):
     return baseline, parity
 
 
+def curve(args, log_):
+    """Step time over a long run: the temporal skip and the unchanged-winner shortcut
+    keep state from step to step, so their gain can drift as the packing evolves."""
+    import torch
+    from sand_atlas_dem_amd import assembly, autotune
+    from sand_atlas_dem_amd.hip import HipBackend
+
+    w = assembly.config_workload(args.config, seed=args.seed, n=args.grains)
+    hip = HipBackend(0, graph=not args.no_graph)
+    e = w.engine(hip)
+    e.initialize()
+    e.advance(args.warmup, rate=w.rate)
+    if not args.no_autotune:
+        tuned = autotune.autotune(e, rate=w.rate, steps=6, log=log_)
+        log_(f"[curve] autotuned launch shape {tuned['best']}")
+    done = 0
+    while done < args.curve:
+        k = min(args.curve_window, args.curve - done)
+        before = hip.stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.advance(k, rate=w.rate)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / k
+        after = hip.stats()
+        done += k
+        d = {name: (after[name] - before[name]) / k for name in after}
+        print(json.dumps({"config": args.config, "grains": e.n, "steps_done": done, "ms_per_step": round(ms, 4),
+                          "contacts_per_step": round(d["contacts"]), "queried_per_step": round(d["queried_nodes"]),
+                          "shortcut_fraction": round(d["unchanged_winners"] / max(d["queried_nodes"], 1), 4),
+                          "skipped_per_step": round(d["skipped_nodes"]),
+                          "anchored_per_step": round(d["anchored_tasks"]),
+                          "directed_pairs": round(d["directed_pairs"])}), flush=True)
+
+
 def measure(args, config, log_, *, primary):
     """Everything measured on one workload; returns a block of the JSON line."""
     import numpy as np
@@ -384,6 +419,10 @@ def main():
     ap.add_argument("--nn-batch", type=int, default=16384)
     ap.add_argument("--nn-iters", type=int, default=4000)
     ap.add_argument("--audit-steps", type=int, default=5)
+    ap.add_argument("--curve", type=int, default=0,
+                    help="instead of the bench line: run this many steps of --config after the autotuned "
+                         "warmup and print ms/step per --curve-window steps (one JSON line each)")
+    ap.add_argument("--curve-window", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -402,6 +441,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     load_package()
+    if args.curve:
+        curve(args, log)
+        return
     head = measure(args, args.config, log, primary=True)
     extras = {}
     for config in [c for c in args.extra.split(",") if c and c != "none" and c != args.config]:

@@ -79,10 +79,11 @@ enum sad_side { SAD_SIDE_META = 0, SAD_SIDE_SLIP, SAD_SIDE_NORMAL_AREA, SAD_SIDE
  * demand, never shrinks.  Returns NULL on failure. */
 void* sad_hip_scratch_create(void);
 void sad_hip_scratch_destroy(void* scratch);
-/* stats[0..9]: cumulative steps (collections), directed candidate pairs, exactly
- * queried nodes and contacts since creation; workspace bytes; learned-cull culled,
- * audited and missed nodes; nodes passed over by the temporal skip; queried nodes
- * whose nearest triangle was proven unchanged without a traversal. */
+/* stats[0..10] (the caller passes 11 words): cumulative steps (collections),
+ * directed candidate pairs, exactly queried nodes and contacts since creation;
+ * workspace bytes; learned-cull culled, audited and missed nodes; nodes passed over
+ * by the temporal skip; queried nodes whose nearest triangle was proven unchanged
+ * without a traversal; directed-pair tasks that (re)took a temporal-skip anchor. */
 int sad_hip_scratch_stats(void* scratch, uint64_t* stats);
 
 /* Per-kernel HIP-event timing on the launch stream (no reference counterpart;
@@ -193,7 +194,9 @@ int sad_hip_reduce_kernel(const char* kernel, uint32_t n, uint32_t walls, uint32
  * gap it stores is the true one up to that reach), "temporal_skip" (0/1, default 0: pass
  * over nodes whose signed distance at an anchor pose exceeds a rigorous bound on
  * their motion since), "skip_limit_micro" (2000..250000: re-anchor a directed pair
- * once its motion bound reaches this many 1e-6 template units).  value < 0 only
+ * once its motion bound reaches this many 1e-6 template units), "skip_rebase" (0/1,
+ * default 0: re-anchor an expired pair by moving its stored bounds toward zero by its
+ * motion bound instead of recomputing them).  value < 0 only
  * reads; *previous (if non-null) receives the value before the call. */
 int sad_hip_tune(void* scratch, const char* name, int64_t value, int64_t* previous);
 

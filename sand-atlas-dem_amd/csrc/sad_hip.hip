@@ -126,6 +126,7 @@ struct W {
     uint32_t nodeStride;  // bytes per task (>= nodes of any template, multiple of 64)
     uint32_t skip;        // 1 = on
     float skipLimit;      // re-anchor once the motion bound reaches this (template units)
+    uint32_t rebase;      // 1 = an expired anchor is moved by its motion bound, not recomputed
     float* sdfReach;      // SDF geometry, per template: squared radius beyond which the field is >= 0
     uint32_t* sdfHits;    // SDF geometry, per directed task: bitmask of its contact nodes (count pass)
     float4* sdfLeaves;    // SDF geometry, per template 64 rows: node leaves' bounding spheres (centre, radius)
@@ -151,6 +152,25 @@ struct W {
     // triangle was than any other.
     float4* lastFrame;    // 4 rows per task
     uint32_t* nodeGap;    // nodeStride words per task
+    // Persistent task rows (with the temporal skip).  The per-task rows above are
+    // indexed by row, not by task: pair q's tasks use rows 2 pairRow[q] and
+    // 2 pairRow[q] + 1.  A pair keeps its row pair across neighbour-list rebuilds,
+    // through a hash of its key (i, j); rows of pairs that left the list are freed
+    // and new pairs take free ones, so only new pairs start without anchors, warm
+    // triangles or gaps.
+    uint32_t* pairRow;    // per listed pair: its row pair
+    unsigned long long* rowKeys;  // hash slots: (i << 32) | j of a listed pair, or EMPTY_KEY
+    uint32_t* rowOfSlot;  // hash slots: that pair's row pair
+    uint32_t* rowMark;    // per row pair: the assignment generation that kept it
+    unsigned long long* rowOwner;  // per row pair: key of the pair it was assigned to
+    uint32_t* rowNeed;    // per pair: 1 when it needs a free row pair (scan input)
+    uint32_t* rowNeedRank;
+    uint32_t* rowFree;    // per row pair: 1 when free (scan input)
+    uint32_t* rowFreeRank;
+    uint32_t* rowState;   // [0] rows assigned once, [3] generation, [4] assigning in this step,
+                          // [5] a task found its row pair owned by another pair (stale map)
+    uint32_t rowHashMask; // hash slots - 1 (a power of two)
+    uint32_t rowPairs;    // row pairs (= pair capacity)
 };
 
 constexpr int WAVE = 64;
@@ -982,6 +1002,7 @@ __global__ void k_reset_mesh(P p, W w) {  // resetMeshContacts (solver.slang:397
         w.counters[2 * LINE] = 0;
         w.counters[11 * LINE] = 0;
         w.counters[12 * LINE] = 0;
+        w.counters[13 * LINE] = 0;
         for (int q = 0; q < 8; q++) w.stageCount[q * LINE] = 0;
     }
     if (ld_volatile(p.control) != 0) return;
@@ -1068,6 +1089,100 @@ __global__ void k_bucket_fill(P p, W w) {
     const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks; t += gridDim.x * blockDim.x)
         w.xcdTasks[atomicAdd(w.keyStart + taskKey(p, t, b), 1u)] = t;
+}
+
+// ---------------------------------------------------------------- persistent task rows
+// Assigned when the neighbour list was rebuilt in this step (or never yet): every
+// listed pair looks its key up in the hash of the previous list and keeps that
+// pair's row pair; the row pairs no listed pair kept are free, and the k-th new pair
+// (in list order) takes the k-th free row pair; then the hash is rebuilt from the
+// new list.  Which rows a pair gets never changes a result: a task uses its row
+// pair only while that row pair's owner key is its own pair (a pair list restored
+// from a checkpoint may not match the map; such a task runs without the skip and
+// flags the map for re-assignment at the next collection), and every row is checked
+// against its anchor's key (pair and epoch) before it is trusted.
+constexpr unsigned long long EMPTY_KEY = ~0ull;
+__device__ __forceinline__ uint32_t rowHash(unsigned long long key) {
+    key ^= key >> 33;
+    key *= 0xff51afd7ed558ccdull;
+    key ^= key >> 33;
+    key *= 0xc4ceb9fe1a85ec53ull;
+    key ^= key >> 33;
+    return uint32_t(key);
+}
+__device__ __forceinline__ unsigned long long pairKey(uint2 ij) {
+    return (static_cast<unsigned long long>(ij.x) << 32) | ij.y;
+}
+__global__ void k_rows_lookup(P p, W w) {
+    const bool assign = p.control[4] != 0 || ld_volatile(w.rowState) == 0u || ld_volatile(w.rowState + 5) != 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) w.rowState[4] = assign ? 1u : 0u;
+    if (!assign) return;
+    const bool listed = ld_volatile(w.rowState) != 0u;  // the hash holds the previous list
+    const uint32_t gen = w.rowState[3] + 1u;
+    const uint32_t pairs = min(p.control[1], p.c.pairCapacity);
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < w.rowPairs; q += gridDim.x * blockDim.x) {
+        uint32_t need = q < pairs ? 1u : 0u;
+        if (need && listed) {
+            const unsigned long long key = pairKey(p.pairs[q]);
+            uint32_t h = rowHash(key) & w.rowHashMask;
+            for (uint32_t probe = 0; probe <= w.rowHashMask; probe++) {
+                const unsigned long long k = w.rowKeys[h];
+                if (k == key) {
+                    const uint32_t r = w.rowOfSlot[h];
+                    w.pairRow[q] = r;
+                    w.rowMark[r] = gen;
+                    need = 0u;
+                    break;
+                }
+                if (k == EMPTY_KEY) break;
+                h = (h + 1u) & w.rowHashMask;
+            }
+        }
+        w.rowNeed[q] = need;
+    }
+}
+__global__ void k_rows_free(P p, W w) {
+    if (w.rowState[4] == 0u) return;
+    const uint32_t gen = w.rowState[3] + 1u, slots = w.rowHashMask + 1u;
+    const uint32_t n = max(w.rowPairs, slots);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        if (k < w.rowPairs) w.rowFree[k] = w.rowMark[k] != gen ? 1u : 0u;
+        if (k < slots) w.rowKeys[k] = EMPTY_KEY;  // every lookup is done: the hash is rebuilt below
+    }
+}
+__global__ void k_rows_assign(P p, W w) {
+    if (w.rowState[4] == 0u) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        w.rowState[3] = w.rowState[3] + 1u;
+        w.rowState[0] = 1u;
+        w.rowState[5] = 0u;
+    }
+    const uint32_t pairs = min(p.control[1], p.c.pairCapacity);
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < pairs; q += gridDim.x * blockDim.x) {
+        uint32_t r = w.pairRow[q];
+        if (w.rowNeed[q]) {
+            // The rank-th free row pair: the last row pair whose exclusive free rank is
+            // <= rank (free row pairs outnumber new pairs: there are pairCapacity of them).
+            const uint32_t rank = w.rowNeedRank[q];
+            uint32_t lo = 0, hi = w.rowPairs;  // first index whose rank exceeds `rank`, in (0, rowPairs]
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (w.rowFreeRank[mid] > rank) hi = mid; else lo = mid + 1;
+            }
+            r = lo - 1u;
+            w.pairRow[q] = r;
+        }
+        const unsigned long long key = pairKey(p.pairs[q]);
+        w.rowOwner[r] = key;
+        uint32_t h = rowHash(key) & w.rowHashMask;
+        for (uint32_t probe = 0; probe <= w.rowHashMask; probe++) {
+            if (atomicCAS(w.rowKeys + h, EMPTY_KEY, key) == EMPTY_KEY) {
+                w.rowOfSlot[h] = r;
+                break;
+            }
+            h = (h + 1u) & w.rowHashMask;
+        }
+    }
 }
 
 struct TaskFrame {
@@ -1217,6 +1332,20 @@ constexpr float SKIP_QUANTUM = 1.0f / 512.0f;
 constexpr float SKIP_MARGIN = 1e-3f;
 // Rounded toward zero: a lower bound of a positive distance, an upper bound of a
 // negative one; NaN stores 0 (neither far nor inside).
+// Four stored bounds moved toward zero by `shrink` quanta (clamped at zero).  Leaf
+// bounds are read only as lower bounds of positive distances: their negative values
+// are kept.
+__device__ __forceinline__ uint32_t shrinkBounds(uint32_t word, uint32_t shrink, bool positiveOnly) {
+    uint32_t out = 0;
+    const int s = int(min(shrink, 255u));
+    for (int b = 0; b < 4; b++) {
+        int v = int(int8_t(uint8_t(word >> (8 * b))));
+        if (v > 0) v = max(0, v - s);
+        else if (v < 0 && !positiveOnly) v = min(0, v + s);
+        out |= uint32_t(uint8_t(int8_t(v))) << (8 * b);
+    }
+    return out;
+}
 __device__ __forceinline__ int8_t quantizeDistance(float d) {
     if (!(d == d)) return 0;
     return int8_t(fminf(fmaxf(d * 512.0f, -127.0f), 127.0f));
@@ -1302,12 +1431,13 @@ struct BatchRec {  // 112 bytes in LDS per task of the batch
     float4 qi, qj;
     float4 branchSi;  // branch, s_i
     float4 tail;      // s_j, min(wi, wj), unused, unused
-    uint32_t i, j, unused0, unused1;
+    uint32_t i, j, row, unused;  // row: the task's row in the persistent per-task rows, or NO_ROW
 };
+constexpr uint32_t NO_ROW = 0xffffffffu;
 static_assert(sizeof(BatchRec) == 112, "batch record layout");
 
 __device__ __forceinline__ void batchPut(BatchRec* r, const P& p, const W& w, const TaskFrame& f, uint32_t task,
-                                         bool refresh, const GapCtx& gc, int lane) {
+                                         uint32_t row, bool refresh, const GapCtx& gc, int lane) {
     const uint32_t root = p.meshRoots[f.sj].x;
     const uint32_t rootRef = nodeRef(p, root);
     const uint32_t triFirst = w.warm ? w.triBase[3 * f.sj] : 0u, triCount = w.warm ? w.triBase[3 * f.sj + 1] : 0u;
@@ -1319,7 +1449,7 @@ __device__ __forceinline__ void batchPut(BatchRec* r, const P& p, const W& w, co
         r->qi = f4(f.qi); r->qj = f4(f.qj);
         r->branchSi = f4(f.branch, f.s_i);
         r->tail = make_float4(f.s_j, fminf(f.wi, f.wj), 0.0f, 0.0f);
-        r->i = f.i; r->j = f.j;
+        r->i = f.i; r->j = f.j; r->row = row;
     }
 }
 
@@ -1366,7 +1496,7 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
                                uint32_t carry, bool lastOpen, int lane) {
     const float4* __restrict__ wide = w.wide;
     const bool gapOn = w.warm && w.nodeGap;
-    uint32_t k = 0, top = 0, bestTri = NO_TRIANGLE, slotNow = 0, taskNow = 0, genNow = 0;
+    uint32_t k = 0, top = 0, bestTri = NO_TRIANGLE, slotNow = 0, rowNow = 0, genNow = 0;
     float best = 1e30f, prune = 1e30f, second = 1e30f;
     // Gap reach G: the traversal also visits what lies within G of its bound, so a
     // recorded gap is at least min(G, the true gap) instead of the distance to the
@@ -1391,7 +1521,7 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
         const uint32_t entry = queue[k];
         const BatchRec& r = batch[nearestTri[k] & 0xffffu];
         const uint32_t node = entry & 0xffffu, slot = entry >> 16;
-        const uint4 h0 = make_uint4(r.task, r.root, r.rootRef, r.sampleBase);
+        const uint4 h0 = make_uint4(r.row, r.root, r.rootRef, r.sampleBase);
         const uint4 h1 = make_uint4(r.triFirst, r.triCount, r.gens, r.end);
         const float4 qi = r.qi, qj = r.qj, bs = r.branchSi, tail = r.tail;
         const float4 sp = p.sample[h0.w + node];
@@ -1406,10 +1536,10 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
         second = 1e30f;
         ext = 1e30f;
         slotNow = slot;
-        taskNow = h0.x;
+        rowNow = h0.x;
         genNow = h1.z >> 16;
         bestTri = NO_TRIANGLE;
-        if (w.warm) {
+        if (w.warm && h0.x != NO_ROW) {
             const uint32_t last = w.nodeTri[size_t(h0.x) * w.nodeStride + slot];
             if (last != 0u && last <= h1.y) {
                 uint32_t feature;
@@ -1522,12 +1652,12 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
         }
         if (active && top == 0) {
             nearestTri[k] = bestTri;
-            if (gapOn) {
+            if (gapOn && rowNow != NO_ROW) {
                 // The gap this query leaves for the next collection (0 when there is none).
                 const float gap = bestTri != NO_TRIANGLE && second < 1e30f
                                       ? sqrtf(second) - sqrtf(best) - 2.0f * GAP_EVAL - GAP_MOTION_ABS
                                       : 0.0f;
-                w.nodeGap[size_t(taskNow) * w.nodeStride + slotNow] = gapWord(genNow, gap);
+                w.nodeGap[size_t(rowNow) * w.nodeStride + slotNow] = gapWord(genNow, gap);
             }
             active = false;
         }
@@ -1548,9 +1678,10 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
         uniformFrame(f);
         const uint32_t triFirst = uniformU(r.triFirst);
         const uint32_t to = end & 0xffffu;
-        const size_t row = size_t(task) * w.nodeStride;
-        int8_t* refreshDist = (end >> 16) ? w.nodeDist + row : nullptr;
-        uint16_t* warmTri = w.warm ? w.nodeTri + row : nullptr;
+        const uint32_t rowIndex = uniformU(r.row);
+        const size_t row = size_t(rowIndex) * w.nodeStride;
+        int8_t* refreshDist = (end >> 16) && rowIndex != NO_ROW ? w.nodeDist + row : nullptr;
+        uint16_t* warmTri = w.warm && rowIndex != NO_ROW ? w.nodeTri + row : nullptr;
         uint32_t running = s == 0 ? carry : 0u;
         for (uint32_t b = from; b < to; b += WAVE) {
             const uint32_t kk = b + lane;
@@ -1760,15 +1891,25 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             // A task without a valid anchor, or whose bound has grown past
             // skipLimit, tests and queries every node and re-anchors; the rejection
             // plane of the anchor (any hull plane is a valid filter) is reused.
-            bool skipMode = false;
+            bool skipMode = false, rebased = false;
             float c0 = 0.0f, c1 = 0.0f;
             int8_t* dist = nullptr;
             int8_t* refreshDist = nullptr;
             uint32_t skipped = 0, anchoredPlane = 0xffffffffu;
-            uint16_t* warmTri = w.warm ? uniformPtr(w.nodeTri + size_t(task) * w.nodeStride) : nullptr;
+            // The task's persistent rows, used only while their owner is this pair.
+            uint32_t rowTask = task;
+            bool rows = false;
             if (w.skip) {
-                dist = w.nodeDist + size_t(task) * w.nodeStride;
-                const float4* an = w.anchor + 4 * size_t(task);
+                const uint32_t r = w.pairRow[task >> 1];
+                rows = w.rowOwner[r] == pairKey(ij);
+                if (!rows && lane == 0) w.rowState[5] = 1u;
+                rowTask = uniformU(2u * r + (task & 1u));
+                rows = uniformU(rows ? 1u : 0u) != 0u;
+            }
+            uint16_t* warmTri = w.warm && rows ? uniformPtr(w.nodeTri + size_t(rowTask) * w.nodeStride) : nullptr;
+            if (rows) {
+                dist = w.nodeDist + size_t(rowTask) * w.nodeStride;
+                const float4* an = w.anchor + 4 * size_t(rowTask);
                 const float4 a0 = an[0], a1 = an[1], a2 = an[2], a3 = an[3];
                 const bool valid = __float_as_uint(a3.w) == 1u && __float_as_uint(a3.x) == f.i &&
                                    __float_as_uint(a3.y) == f.j && __float_as_uint(a3.z) == ld_volatile(w.counters + 10 * LINE);
@@ -1777,13 +1918,35 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 const float db = length(f.branch - xyz(a2));
                 c0 = 1.001f * (2.0f * dqj * length(f.branch) + db) / f.s_j;
                 c1 = 1.001f * (2.0f * dqj + 2.0f * dqi) * f.s_i / f.s_j;
-                skipMode = valid && c0 + c1 * (p.body[f.i].z / f.s_i) + SKIP_MARGIN < w.skipLimit;
+                const float reachAll = c0 + c1 * (p.body[f.i].z / f.s_i);  // bound for every node of the grain
+                skipMode = valid && reachAll + SKIP_MARGIN < w.skipLimit;
+                rebased = valid && !skipMode && w.rebase != 0u && reachAll < 0.25f;
                 c0 = uniformF(c0);
                 c1 = uniformF(c1);
                 skipMode = uniformU(skipMode ? 1u : 0u) != 0u;
+                rebased = uniformU(rebased ? 1u : 0u) != 0u;
                 dist = uniformPtr(dist);
                 if (skipMode) {
                     anchoredPlane = uniformU(__float_as_uint(a2.w));
+                } else if (rebased) {
+                    // The anchor has expired: re-anchor at the current poses by moving
+                    // every stored bound toward zero by the motion since the old anchor
+                    // (rounded up to whole quanta).  A positive value stays a lower bound
+                    // of its node's distance and a negative one an upper bound, so they
+                    // hold at the new anchor with zero motion; this collection then
+                    // overwrites the value of every node and leaf its tests reach.
+                    const uint32_t shrink = uint32_t(ceilf(uniformF(reachAll) / SKIP_QUANTUM)) + 1u;
+                    const uint32_t words = (p.sampleInfo[f.si].y + 3) / 4;
+                    uint32_t* d32 = reinterpret_cast<uint32_t*>(dist);
+                    for (uint32_t k = lane; k < words; k += WAVE) d32[k] = shrinkBounds(d32[k], shrink, false);
+                    uint32_t* l32 = reinterpret_cast<uint32_t*>(w.leafDist + size_t(rowTask) * w.clusterStride);
+                    const uint32_t leafWords = (p.meshClusterInfo[f.si].y + 3) / 4;
+                    for (uint32_t k = lane; k < leafWords; k += WAVE) l32[k] = shrinkBounds(l32[k], shrink, true);
+                    __threadfence_block();
+                    refreshDist = dist;
+                    skipMode = true;
+                    c0 = 0.0f;
+                    c1 = 0.0f;
                 } else {
                     refreshDist = dist;
                     const uint32_t words = (p.sampleInfo[f.si].y + 3) / 4;
@@ -1798,9 +1961,9 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             // cleared, so a stale word can never carry the expected generation.
             GapCtx gc{nullptr, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0}, 0u, 0u};
             if (warmTri && w.nodeGap) {
-                const float4* lf = w.lastFrame + 4 * size_t(task);
+                const float4* lf = w.lastFrame + 4 * size_t(rowTask);
                 const float4 l0 = lf[0], l1 = lf[1], l2 = lf[2], l3 = lf[3];
-                uint32_t* row = w.nodeGap + size_t(task) * w.nodeStride;
+                uint32_t* row = w.nodeGap + size_t(rowTask) * w.nodeStride;
                 const uint32_t genPrev = __float_as_uint(l2.w) & 0xffffu;
                 uint32_t genNew = genPrev + 1u;
                 if (genNew > 0xffffu) {
@@ -1822,7 +1985,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             // Rejection plane: largest violation at the source centre, first in
             // plane order on ties (strict >, initial -1e30), as a wave reduction
             // over (value, sequence) pairs.
-            if (skipMode) {
+            if (skipMode && !rebased) {
                 f.plane = anchoredPlane == 0xffffffffu ? make_float4(0, 0, 0, 0) : p.meshPlanes[anchoredPlane];
             } else {
                 v3 centre = -qinv(f.qj, f.branch) / f.s_j;
@@ -1851,7 +2014,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             const bool gapTask = gc.row != nullptr && gc.genPrev != 0u;
             const uint32_t triFirst = gapTask ? uniformU(w.triBase[3 * f.sj]) : 0u;
             const uint32_t triCount = gapTask ? uniformU(w.triBase[3 * f.sj + 1]) : 0u;
-            batchPut(batch + nb, p, w, f, task, refresh, gc, lane);
+            batchPut(batch + nb, p, w, f, task, rows ? rowTask : NO_ROW, refresh, gc, lane);
             nb++;
             // Leaf culls, lane-parallel.  Internal clusters are not tested: a leaf's
             // own test is a valid filter, and a leaf the preorder walk would prune
@@ -1861,8 +2024,8 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             // motion bound of its box is passed over untested; a re-anchoring task
             // records the cull's evidence for each rejected leaf.
             const uint4 info = p.meshClusterInfo[f.si];
-            int8_t* leafDist = w.skip ? uniformPtr(w.leafDist + size_t(task) * w.clusterStride) : nullptr;
-            const uint32_t keysBase = w.skip ? uniformU(w.triBase[3 * f.si + 2]) : 0u;
+            int8_t* leafDist = rows ? uniformPtr(w.leafDist + size_t(rowTask) * w.clusterStride) : nullptr;
+            const uint32_t keysBase = rows ? uniformU(w.triBase[3 * f.si + 2]) : 0u;
             uint32_t leafSkipped = 0;  // per lane, summed below
             for (uint32_t base = 0; base < info.y; base += WAVE) {
                 const uint32_t c = base + lane;
@@ -1925,7 +2088,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     float4 sp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     if (uint32_t(lane) < range.y) {
                         node = p.meshSampleIndices[range.x + lane];
-                        if (w.skip) slot = range.x + lane - keysBase;
+                        if (rows) slot = range.x + lane - keysBase;
                         sp = p.sample[f.sampleBase + node];
                         bool inside = false;
                         if (skipMode) {
@@ -2006,7 +2169,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                         qn = 0;
                         pending = 0;
                         wave_sync();
-                        batchPut(batch, p, w, f, task, refresh, gc, lane);
+                        batchPut(batch, p, w, f, task, rows ? rowTask : NO_ROW, refresh, gc, lane);
                         nb = 1;
                         wave_sync();
                     }
@@ -2051,7 +2214,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 }
             }
             if (gc.row && lane == 0) {  // this collection's frame, for the next one's motion bound
-                float4* lf = w.lastFrame + 4 * size_t(task);
+                float4* lf = w.lastFrame + 4 * size_t(rowTask);
                 lf[0] = f4(f.qi);
                 lf[1] = f4(f.qj);
                 lf[2] = f4(f.branch, __uint_as_float(gc.genNew));
@@ -2059,7 +2222,8 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                                     __uint_as_float(ld_volatile(w.counters + 10 * LINE)), __uint_as_float(1u));
             }
             if (refreshDist && lane == 0) {
-                float4* an = w.anchor + 4 * size_t(task);
+                atomicAdd(w.counters + 13 * LINE, 1u);
+                float4* an = w.anchor + 4 * size_t(rowTask);
                 an[0] = f4(f.qi);
                 an[1] = f4(f.qj);
                 an[2] = f4(f.branch, __uint_as_float(anchoredPlane));
@@ -2100,6 +2264,7 @@ __global__ void k_contact_total(P p, W w) {
         atomicAdd(w.totals + 3, (unsigned long long)total);
         atomicAdd(w.totals + 4, (unsigned long long)w.counters[11 * LINE]);
         atomicAdd(w.totals + 5, (unsigned long long)w.counters[12 * LINE]);
+        atomicAdd(w.totals + 6, (unsigned long long)w.counters[13 * LINE]);
     }
 }
 __global__ void k_scatter(P p, W w) {
@@ -3181,7 +3346,7 @@ thread_local char g_error[256] = "no error";
     X(build_hash) X(gather) X(finish_step) X(capture_kinematics) X(check_timestep) \
     X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_offsets) X(bucket_fill) \
     X(advance_walls) X(wall_count) X(wall_emit) X(wall_total) X(gather_wall_rows) X(refresh_wall_power) \
-    X(diagnostic_rows) X(fabric_rows) X(sdf_count) X(sdf_total) X(sdf_emit)
+    X(diagnostic_rows) X(fabric_rows) X(sdf_count) X(sdf_total) X(sdf_emit) X(rows)
 enum KernelId {
 #define SAD_ID(name) KID_##name,
     SAD_KERNELS(SAD_ID)
@@ -3253,6 +3418,7 @@ struct ScratchImpl : Scratch {
     // rewrites template geometry, body scales or shape ids in place.
     int skip = 0;
     int skipLimitMicro = 6000;
+    int rebase = 0;
     uint32_t epoch = 1, epochWritten = 0;  // anchors carry the epoch they were taken in
     std::vector<uint64_t> lastTable;        // binding table of the previous call
     const void* maxNodesFor = nullptr;
@@ -3269,6 +3435,7 @@ struct ScratchImpl : Scratch {
     float4* lastFrames = nullptr;
     size_t gapCap = 0;
     uint32_t lastFrameCap = 0;
+    uint32_t rowMapPairs = 0;  // pair capacity of the persistent-row map
     const void* triBaseFor = nullptr;
     // Captured timesteps keyed by everything a step's launches bake in.
     struct Graph { std::vector<uint64_t> key; hipGraphExec_t exec; };
@@ -3446,6 +3613,32 @@ struct ScratchImpl : Scratch {
             epoch++;  // stored distances are gone: no anchor may be trusted
         }
         w.nodeStride = stride;
+        // The persistent-row map (pair capacity = tasks / 2): a new map starts with
+        // no pair listed, so the next collection assigns every row.
+        const uint32_t pairCap = uint32_t(tasks / 2);
+        if (rowMapPairs < pairCap) {
+            free_row_map();
+            uint32_t slots = 1;
+            while (slots < 2 * pairCap) slots <<= 1;
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.pairRow), size_t(pairCap) * sizeof(uint32_t)));
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.rowMark), size_t(pairCap) * sizeof(uint32_t)));
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.rowNeed), size_t(pairCap) * sizeof(uint32_t)));
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.rowNeedRank), size_t(pairCap) * sizeof(uint32_t)));
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.rowFree), size_t(pairCap) * sizeof(uint32_t)));
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.rowFreeRank), size_t(pairCap) * sizeof(uint32_t)));
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.rowKeys), size_t(slots) * sizeof(unsigned long long)));
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.rowOfSlot), size_t(slots) * sizeof(uint32_t)));
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.rowOwner), size_t(pairCap) * sizeof(unsigned long long)));
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.rowState), 8 * sizeof(uint32_t)));
+            HIP_OK(hipMemset(w.pairRow, 0, size_t(pairCap) * sizeof(uint32_t)));
+            HIP_OK(hipMemset(w.rowMark, 0, size_t(pairCap) * sizeof(uint32_t)));
+            HIP_OK(hipMemset(w.rowOwner, 0xff, size_t(pairCap) * sizeof(unsigned long long)));  // no owner
+            HIP_OK(hipMemset(w.rowState, 0, 8 * sizeof(uint32_t)));
+            rowMapPairs = pairCap;
+            w.rowPairs = pairCap;
+            w.rowHashMask = slots - 1;
+            bytes += size_t(pairCap) * 32 + size_t(slots) * 12 + 32;
+        }
         const uint32_t clusterStride = (std::max(maxClusters, 1u) + 63) / 64 * 64;
         if (leafDistCap < tasks * clusterStride || w.clusterStride != clusterStride) {
             if (w.leafDist) { (void)hipFree(w.leafDist); bytes -= leafDistCap; }
@@ -3479,6 +3672,7 @@ struct ScratchImpl : Scratch {
         w.triBase = triBase;
         w.warm = maxTriangles <= 65534u ? 1u : 0u;
         w.skipLimit = float(skipLimitMicro) * 1e-6f;
+        w.rebase = uint32_t(rebase);
         w.skip = 1;
         return SAD_OK;
     }
@@ -3491,7 +3685,20 @@ struct ScratchImpl : Scratch {
         epochWritten = epoch;
         return SAD_OK;
     }
+    void free_row_map() {
+        if (!rowMapPairs) return;
+        (void)hipFree(w.pairRow); (void)hipFree(w.rowMark); (void)hipFree(w.rowNeed); (void)hipFree(w.rowNeedRank);
+        (void)hipFree(w.rowFree); (void)hipFree(w.rowFreeRank); (void)hipFree(w.rowKeys); (void)hipFree(w.rowOfSlot);
+        (void)hipFree(w.rowState); (void)hipFree(w.rowOwner);
+        bytes -= size_t(rowMapPairs) * 32 + size_t(w.rowHashMask + 1) * 12 + 32;
+        w.pairRow = w.rowMark = w.rowNeed = w.rowNeedRank = w.rowFree = w.rowFreeRank = w.rowOfSlot = w.rowState = nullptr;
+        w.rowKeys = w.rowOwner = nullptr;
+        rowMapPairs = 0;
+        w.rowPairs = 0;
+        w.rowHashMask = 0;
+    }
     ~ScratchImpl() {
+        free_row_map();
         drop_graphs();
         if (captureStream) (void)hipStreamDestroy(captureStream);
         for (auto& m : pending) { pool.push_back(m.start); pool.push_back(m.stop); }
@@ -3761,6 +3968,15 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         launch_scan(sc, keyScan, keys, s);
         LAUNCH(sc, KID_bucket_offsets, k_bucket_offsets, dim3(1), dim3(64), s, p, w);
         LAUNCH(sc, KID_bucket_fill, k_bucket_fill, gridStride, T, s, p, w);
+        if (w.skip) {  // persistent task rows (assigned when the list was rebuilt)
+            LAUNCH(sc, KID_rows, k_rows_lookup, gridStride, T, s, p, w);
+            ScanArgs need{w.rowNeed, w.rowNeedRank, w.sums, w.rowPairs, nullptr, 1, w.rowState + 4, nullptr};
+            launch_scan(sc, need, w.rowPairs, s);
+            LAUNCH(sc, KID_rows, k_rows_free, gridStride, T, s, p, w);
+            ScanArgs fr{w.rowFree, w.rowFreeRank, w.sums, w.rowPairs, nullptr, 1, w.rowState + 4, nullptr};
+            launch_scan(sc, fr, w.rowPairs, s);
+            LAUNCH(sc, KID_rows, k_rows_assign, gridStride, T, s, p, w);
+        }
         const dim3 persistent(std::max(1, sc.cus) * sc.blocksPerCU);
         w.pop = sc.pop;
         w.leafBatch = sc.leafBatch;
@@ -3879,6 +4095,7 @@ int sad_hip_scratch_stats(void* scratch, uint64_t* stats) {
     for (int k = 0; k < 4; k++) stats[k] = totals[k];
     stats[8] = totals[4];
     stats[9] = totals[5];
+    stats[10] = totals[6];
     stats[4] = sc->bytes;
     unsigned long long nnv[3] = {0, 0, 0};
     if (sc->w.nnStats) {
@@ -4304,6 +4521,12 @@ int sad_hip_tune(void* scratch, const char* name, int64_t value, int64_t* previo
         if (value >= 0) {
             if (value > 100000) return SAD_E_ARGUMENT;
             sc->gapReachMicro = int(value);
+        }
+    } else if (key == "skip_rebase") {
+        old = sc->rebase;
+        if (value >= 0) {
+            if (value > 1) return SAD_E_ARGUMENT;
+            sc->rebase = int(value);
         }
     } else if (key == "gap_shortcut") {
         old = sc->gapShortcut;

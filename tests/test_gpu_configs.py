@@ -167,16 +167,25 @@ def test_neighbour_rebuilds_with_skip_and_warm_start_match_oracle(pkg, oracle_cp
     a.initialize()
     b.initialize()
     rebuilds = 0
+    anchored_at_rebuilds = []  # anchors taken by the collection right after each rebuild
     for step in range(120):  # one step per call: the anchors persist across calls
+        before = hip.stats()
         a.advance(1, rate=rate)
         b.advance(1, rate=rate)
+        after = hip.stats()
         rebuilt = int(a.read("control", np.uint32, 1)[4])
         if rebuilt or step % 20 == 19:
             assert_same(a, b, f"step {step + 1}")
             assert np.array_equal(a.read("referencePosition"), b.read("referencePosition"))
+        if rebuilt and step > 0:
+            tasks = after["directed_pairs"] - before["directed_pairs"]
+            anchored_at_rebuilds.append((after["anchored_tasks"] - before["anchored_tasks"]) / tasks)
         rebuilds += rebuilt
     assert rebuilds >= 2, rebuilds
     assert hip.stats()["skipped_nodes"] > 0
+    # A rebuilt list keeps each surviving pair's rows: only new pairs (and pairs at
+    # their re-anchor distance) take anchors, not the whole list.
+    assert max(anchored_at_rebuilds) < 0.25, anchored_at_rebuilds
 
 
 def test_small_configuration_full_size_1000_steps(pkg, oracle_cpu):

@@ -51,7 +51,8 @@ def test_graph_captured_steps_match_eager(dense):
      "collect_leaf_batch": 1, "collect_batch_tasks": 1, "collect_batch_fill": 1},
     {"collect_blocks_per_cu": 8, "collect_pop": 2, "collect_queue": 256, "skip_limit_micro": 250000,
      "collect_leaf_batch": 64, "collect_batch_tasks": 16, "collect_batch_fill": 256},
-    {"collect_queue": 128, "collect_batch_tasks": 3, "collect_batch_fill": 17},
+    {"collect_queue": 128, "collect_batch_tasks": 3, "collect_batch_fill": 17, "skip_rebase": 1,
+     "skip_limit_micro": 2000},
 ])
 def test_launch_shapes_do_not_change_results(dense, setting):
     from sand_atlas_dem_amd.hip import HipBackend
@@ -70,7 +71,7 @@ def test_tune_rejects_bad_values(pkg):
     m = HipBackend(0)
     for name, value in (("collect_queue", 64), ("collect_pop", 0), ("collect_blocks_per_cu", 99),
                         ("collect_leaf_batch", 0), ("collect_leaf_batch", 65), ("collect_batch_tasks", 0),
-                        ("collect_batch_tasks", 17), ("collect_batch_fill", 0), ("collect_batch_fill", 257),
+                        ("collect_batch_tasks", 17), ("collect_batch_fill", 0), ("collect_batch_fill", 257), ("skip_rebase", 2),
                         ("nope", 1)):
         with pytest.raises(RuntimeError):
             m.tune(name, value)

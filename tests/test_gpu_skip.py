@@ -47,8 +47,11 @@ def test_skip_leaves_results_unchanged(config):
     assert s["queried_nodes"] < plain.stats()["queried_nodes"]
 
 
-@pytest.mark.parametrize("limit", [2000, 250000])
-def test_skip_limit_does_not_change_results(limit):
+@pytest.mark.parametrize("limit,rebase", [(2000, 0), (250000, 0), (2000, 1)])
+def test_skip_limit_does_not_change_results(limit, rebase):
+    """Tight and loose re-anchor limits; with `skip_rebase` an expired anchor is moved
+    by its motion bound instead of being recomputed (re-anchoring happens every few
+    steps at the tight limit)."""
     from sand_atlas_dem_amd import assembly
     from sand_atlas_dem_amd.hip import HipBackend
 
@@ -56,8 +59,10 @@ def test_skip_limit_does_not_change_results(limit):
     plain = HipBackend(0, temporal_skip=False)
     fast = HipBackend(0, temporal_skip=True)
     fast.tune("skip_limit_micro", limit)
-    for a, b in zip(_run(w, plain, 5), _run(w, fast, 5)):
+    fast.tune("skip_rebase", rebase)
+    for a, b in zip(_run(w, plain, 5, chunks=6), _run(w, fast, 5, chunks=6)):
         _same(a, b)
+    assert fast.stats()["skipped_nodes"] > 0
 
 
 def test_engines_sharing_a_workspace_do_not_share_anchors():
