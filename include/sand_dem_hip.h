@@ -79,11 +79,13 @@ enum sad_side { SAD_SIDE_META = 0, SAD_SIDE_SLIP, SAD_SIDE_NORMAL_AREA, SAD_SIDE
  * demand, never shrinks.  Returns NULL on failure. */
 void* sad_hip_scratch_create(void);
 void sad_hip_scratch_destroy(void* scratch);
-/* stats[0..10] (the caller passes 11 words): cumulative steps (collections),
+/* stats[0..11] (the caller passes 12 words): cumulative steps (collections),
  * directed candidate pairs, exactly queried nodes and contacts since creation;
  * workspace bytes; learned-cull culled, audited and missed nodes; nodes passed over
  * by the temporal skip; queried nodes whose nearest triangle was proven unchanged
- * without a traversal; directed-pair tasks that (re)took a temporal-skip anchor. */
+ * without a traversal; directed-pair tasks that (re)took a temporal-skip anchor;
+ * mesh collections that rebuilt the per-grain pair adjacency and task buckets
+ * (the others reused them: the pair list and templates were unchanged). */
 int sad_hip_scratch_stats(void* scratch, uint64_t* stats);
 
 /* Per-kernel HIP-event timing on the launch stream (no reference counterpart;

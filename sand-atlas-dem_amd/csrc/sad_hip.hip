@@ -44,6 +44,8 @@ struct P {
     uint32_t meshNodes;  // BVH nodes over all templates (rows of meshLinks)
     uint32_t templates;  // rows of meshRoots
     uint32_t planes;     // rows of meshPlanes
+    uint32_t triangles;  // meshTriangles rows / 3
+    uint32_t sampleKeys; // rows of meshSampleIndices
     uint32_t* control;
     float4 *cell, *position, *positionRemainder, *velocityRemainder, *momentumRemainder;
     float4 *rotationRemainder, *referencePositionRemainder, *cellLog, *accumulatedRemainder;
@@ -105,6 +107,13 @@ struct W {
     float4* stageNa;      // C
     unsigned long long* totals;  // steps, directed pairs, queried nodes, contacts (cumulative)
     float4* wide;         // 4 rows per BVH node: both child boxes + child references
+    // Per-step copies of template geometry in the order the collection reads it:
+    // triangle rows with the face normal in their w components (vertex k's row
+    // carries normal component k), and the surface nodes in sample-tree key order
+    // (key k holds node meshSampleIndices[k] of its template), so a leaf's nodes
+    // are contiguous rows.
+    float4* triPacked;    // 3 rows per triangle
+    float4* sampleKeyed;  // 1 row per sample key
     uint32_t* xcd;        // per queue q: count at [q*LINE], fill cursor at [(8+q)*LINE], pop cursor at [(16+q)*LINE]
     uint32_t* stageCount; // staged hits per queue at [q*LINE]; queue q stages into [q*stageCap, (q+1)*stageCap)
     uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue, ascending target template within a queue
@@ -171,6 +180,15 @@ struct W {
                           // [5] a task found its row pair owned by another pair (stale map)
     uint32_t rowHashMask; // hash slots - 1 (a power of two)
     uint32_t rowPairs;    // row pairs (= pair capacity)
+    // Persistent pair-derived structures: the per-grain pair adjacency and the task
+    // buckets depend only on the pair list and the endpoints' templates, so they are
+    // rebuilt only when those differ from the ones they were built from (a neighbour
+    // list rebuild, a host upload).  builtKey[k] = (i, j, shapeId[i], shapeId[j]) of
+    // pair k when they were built; the header counters[DERIVED_HEAD * LINE + 0..3]
+    // holds (pair count, N, templates, builtGen) and counters[DERIVED_FLAG * LINE]
+    // is 1 while this step rebuilds them.  Null builtKey: rebuilt every step.
+    uint4* builtKey;      // pair capacity
+    uint32_t builtGen;    // host generation of the derived arrays' allocations
 };
 
 constexpr int WAVE = 64;
@@ -539,6 +557,29 @@ __global__ void k_wide_nodes(P p, W w, uint32_t nodes) {
         w.wide[4 * size_t(n) + 1] = make_float4(ha.x, ha.y, ha.z, __uint_as_float(nodeRef(p, b)));
         w.wide[4 * size_t(n) + 2] = make_float4(lb.x, lb.y, lb.z, 0.0f);
         w.wide[4 * size_t(n) + 3] = make_float4(hb.x, hb.y, hb.z, 0.0f);
+    }
+}
+__global__ void k_pack_triangles(P p, W w) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < p.triangles; t += gridDim.x * blockDim.x) {
+        const float4 n = p.meshNormals[7 * size_t(t)];
+        const float4 a = p.meshTriangles[3 * size_t(t)], b = p.meshTriangles[3 * size_t(t) + 1],
+                     c = p.meshTriangles[3 * size_t(t) + 2];
+        w.triPacked[3 * size_t(t)] = make_float4(a.x, a.y, a.z, n.x);
+        w.triPacked[3 * size_t(t) + 1] = make_float4(b.x, b.y, b.z, n.y);
+        w.triPacked[3 * size_t(t) + 2] = make_float4(c.x, c.y, c.z, n.z);
+    }
+}
+// One block per template: its leaves' keys, 64 lanes per leaf.
+__global__ __launch_bounds__(256) void k_pack_samples(P p, W w) {
+    const uint32_t t = blockIdx.x;
+    if (t >= p.c.shapes) return;
+    const uint4 info = p.meshClusterInfo[t];
+    const uint32_t base = p.sampleInfo[t].x;
+    for (uint32_t k = threadIdx.x; k < info.y * 64u; k += blockDim.x) {
+        const uint4 range = p.meshClusterRanges[info.x + k / 64];
+        const uint32_t lane = k % 64;
+        if (lane < range.y && range.x + lane < p.sampleKeys)
+            w.sampleKeyed[range.x + lane] = p.sample[base + p.meshSampleIndices[range.x + lane]];
     }
 }
 // Cached push distances on the LDS stack: exact FP32, or BF16 truncated toward
@@ -936,16 +977,47 @@ __global__ __launch_bounds__(256) void k_scan_apply(ScanArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- persistent pair-derived structures
+// The mesh collection's per-grain pair adjacency and task buckets persist between
+// steps.  Each collection first compares the pair list and the endpoints' template
+// ids with those the structures were built from (builtKey and its header); the
+// builders run only when anything differs and record what they built from.  The
+// comparison reads the list itself, so a neighbour-list rebuild, a restored
+// checkpoint or any other host write is caught without host bookkeeping.  (The
+// reference's experimental persistent endpoint slots validate cached keys through
+// the current hash instead: examples/persistent_gather_shader.py:6-71.)
+constexpr int DERIVED_FLAG = 14;  // counters line: 1 while this step rebuilds
+constexpr int DERIVED_HEAD = 15;  // counters line: pair count, N, templates, builtGen
+__global__ void k_derived_check(P p, W w) {
+    uint32_t* flag = w.counters + DERIVED_FLAG * LINE;
+    const uint32_t* head = w.counters + DERIVED_HEAD * LINE;
+    const uint32_t pairs = min(p.control[1], p.c.pairCapacity);
+    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!w.builtKey || head[0] != pairs || head[1] != p.c.N || head[2] != p.templates || head[3] != w.builtGen) {
+        if (first) *flag = 1u;
+        return;
+    }
+    bool differ = false;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < pairs; k += gridDim.x * blockDim.x) {
+        const uint2 ij = p.pairs[k];
+        const uint4 b = w.builtKey[k];
+        differ |= b.x != ij.x || b.y != ij.y || b.z != p.shapeId[ij.x] || b.w != p.shapeId[ij.y];
+    }
+    if (differ) atomicOr(flag, 1u);
+}
+// Builders run when `gate` is null (always) or *gate != 0.
+__device__ __forceinline__ bool gated(const uint32_t* gate) { return gate && ld_volatile(gate) == 0u; }
+
 // ---------------------------------------------------------------- per-grain pair adjacency
-// Pairs touching each grain in ascending pair index; rebuilt every step so the
-// workspace carries no state between steps.
-__global__ void k_adj_clear(P p, W w) {
+// Pairs touching each grain in ascending pair index.
+__global__ void k_adj_clear(P p, W w, const uint32_t* gate) {
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= p.c.N) return;
+    if (id >= p.c.N || gated(gate)) return;
     w.adjCount[id] = 0;
     w.adjCursor[id] = 0;
 }
-__global__ void k_adj_count(P p, W w) {
+__global__ void k_adj_count(P p, W w, const uint32_t* gate) {
+    if (gated(gate)) return;
     uint32_t pairs = min(p.control[1], p.c.pairCapacity);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < pairs; k += gridDim.x * blockDim.x) {
         uint2 ij = p.pairs[k];
@@ -953,12 +1025,24 @@ __global__ void k_adj_count(P p, W w) {
         atomicAdd(w.adjCount + ij.y, 1u);
     }
 }
-__global__ void k_adj_fill(P p, W w) {
+// With a gate (the mesh collection) it also records what the structures are built
+// from; without one (the SDF path rebuilds every step) it marks the record stale.
+__global__ void k_adj_fill(P p, W w, const uint32_t* gate) {
+    if (gated(gate)) return;
     uint32_t pairs = min(p.control[1], p.c.pairCapacity);
+    const bool record = gate && w.builtKey;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < pairs; k += gridDim.x * blockDim.x) {
         uint2 ij = p.pairs[k];
         w.adjList[w.adjOffset[ij.x] + atomicAdd(w.adjCursor + ij.x, 1u)] = k;
         w.adjList[w.adjOffset[ij.y] + atomicAdd(w.adjCursor + ij.y, 1u)] = k;
+        if (record) w.builtKey[k] = make_uint4(ij.x, ij.y, p.shapeId[ij.x], p.shapeId[ij.y]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t* head = w.counters + DERIVED_HEAD * LINE;
+        head[0] = record ? pairs : 0xffffffffu;
+        head[1] = p.c.N;
+        head[2] = p.templates;
+        head[3] = w.builtGen;
     }
 }
 // Each grain's pair ids in ascending order (the order the gather walks).  One wave
@@ -967,11 +1051,11 @@ __global__ void k_adj_fill(P p, W w) {
 // Lists longer than ADJ_SORT_LDS (none in practice: a grain has tens of candidate
 // pairs) fall back to an insertion sort by one lane.
 constexpr uint32_t ADJ_SORT_LDS = 1024;
-__global__ __launch_bounds__(256) void k_adj_sort(P p, W w) {
+__global__ __launch_bounds__(256) void k_adj_sort(P p, W w, const uint32_t* gate) {
     __shared__ uint32_t idsMem[4][ADJ_SORT_LDS];
     const int lane = threadIdx.x & 63;
     const uint32_t id = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (id >= p.c.N) return;
+    if (id >= p.c.N || gated(gate)) return;
     uint32_t* a = w.adjList + w.adjOffset[id];
     const uint32_t n = w.adjCount[id];
     if (n <= ADJ_SORT_LDS) {
@@ -1062,20 +1146,26 @@ __device__ __forceinline__ uint32_t taskKey(const P& p, uint32_t task, BucketSha
     uint32_t rank = (target & 7) * b.perQueue + (target >> 3);
     return rank * b.sources + (b.sources > 1 ? source : 0u);
 }
-__global__ void k_bucket_reset(P p, W w) {
+// The queues' pop cursors restart every collection; counts, fill cursors and
+// key counts only when the buckets are rebuilt.
+__global__ void k_bucket_reset(P p, W w, const uint32_t* gate) {
+    if (blockIdx.x == 0 && threadIdx.x < 8) w.xcd[(16 + threadIdx.x) * LINE] = 0;
+    if (gated(gate)) return;
     const BucketShape b = bucketShape(p.templates);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < b.keys(); k += gridDim.x * blockDim.x)
         w.keyCount[k] = 0;
-    if (blockIdx.x == 0 && threadIdx.x < 24) w.xcd[threadIdx.x * LINE] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 16) w.xcd[threadIdx.x * LINE] = 0;
 }
-__global__ void k_bucket_count(P p, W w) {
+__global__ void k_bucket_count(P p, W w, const uint32_t* gate) {
+    if (gated(gate)) return;
     const BucketShape b = bucketShape(p.templates);
     const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks; t += gridDim.x * blockDim.x)
         atomicAdd(w.keyCount + taskKey(p, t, b), 1u);
 }
 // Queue totals from the scanned key offsets (w.keyStart, exclusive).
-__global__ void k_bucket_offsets(P p, W w) {
+__global__ void k_bucket_offsets(P p, W w, const uint32_t* gate) {
+    if (gated(gate)) return;
     const BucketShape b = bucketShape(p.templates);
     if (threadIdx.x < 8) {
         const uint32_t q = threadIdx.x, span = b.perQueue * b.sources;
@@ -1084,7 +1174,8 @@ __global__ void k_bucket_offsets(P p, W w) {
         w.xcd[q * LINE] = hi - lo;
     }
 }
-__global__ void k_bucket_fill(P p, W w) {
+__global__ void k_bucket_fill(P p, W w, const uint32_t* gate) {
+    if (gated(gate)) return;
     const BucketShape b = bucketShape(p.templates);
     const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks; t += gridDim.x * blockDim.x)
@@ -1543,7 +1634,7 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
             const uint32_t last = w.nodeTri[size_t(h0.x) * w.nodeStride + slot];
             if (last != 0u && last <= h1.y) {
                 uint32_t feature;
-                const float4* t = p.meshTriangles + 3 * size_t(h1.x + last - 1);
+                const float4* t = w.triPacked + 3 * size_t(h1.x + last - 1);
                 const v3 x = closestTriangle(q, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
                 const float reach = sqrtf(dot(q - x, q - x)) + WARM_SLACK + reachG;
                 prune = reach * reach;
@@ -1620,19 +1711,20 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
             const uint32_t first = leafRef & 0x0fffffffu, count = ((leafRef >> 28) & 3u) + 1;
             for (uint32_t c = 0; c < count; c++) {
                 uint32_t triangle = first + c, feature;
-                const float4* t = p.meshTriangles + 3 * size_t(triangle);
-                const v3 a = xyz(t[0]);
+                const float4* t = w.triPacked + 3 * size_t(triangle);
+                const float4 t0 = t[0], t1 = t[1], t2 = t[2];
+                const v3 a = xyz(t0);
                 // A triangle whose plane is farther than the current bound cannot
                 // reach the minimum (the distance to a triangle is at least the
                 // distance to its plane; 1e-6 absolute and 1e-4 relative cover the
                 // FP32 normal and both evaluations), so skipping it cannot change
-                // which triangle wins.
-                const float h = fabsf(dot(q - a, xyz(p.meshNormals[7 * size_t(triangle)]))) - 1e-6f;
+                // which triangle wins.  The face normal rides in the rows' w.
+                const float h = fabsf(dot(q - a, mk3(t0.w, t1.w, t2.w))) - 1e-6f;
                 if (h > 0.0f && h * h > 1.0001f * fminf(ext, prune)) {
                     second = fminf(second, h * h);
                     continue;
                 }
-                v3 x = closestTriangle(q, a, xyz(t[1]), xyz(t[2]), feature);
+                v3 x = closestTriangle(q, a, xyz(t1), xyz(t2), feature);
                 float distance = dot(q - x, q - x);
                 if (distance < best) {
                     second = fminf(second, best);
@@ -1700,7 +1792,7 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
                 v3 nearest = {0, 0, 0}, pseudo = {0, 0, 0};
                 if (triangle != NO_TRIANGLE) {
                     uint32_t feature;
-                    const float4* t = p.meshTriangles + 3 * size_t(triangle);
+                    const float4* t = w.triPacked + 3 * size_t(triangle);
                     nearest = closestTriangle(x, xyz(t[0]), xyz(t[1]), xyz(t[2]), feature);
                     bestD = dot(x - nearest, x - nearest);
                     pseudo = xyz(p.meshNormals[7 * triangle + feature]);
@@ -2089,7 +2181,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     if (uint32_t(lane) < range.y) {
                         node = p.meshSampleIndices[range.x + lane];
                         if (rows) slot = range.x + lane - keysBase;
-                        sp = p.sample[f.sampleBase + node];
+                        sp = w.sampleKeyed[range.x + lane];  // = p.sample[f.sampleBase + node]
                         bool inside = false;
                         if (skipMode) {
                             const float stored = float(dist[slot]) * SKIP_QUANTUM;
@@ -2252,6 +2344,10 @@ __global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect_nn(P p, W w) {
 __global__ void k_contact_total(P p, W w) {
     uint32_t pairs = min(p.control[1], p.c.pairCapacity);
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id == 0) {  // this step's builders have all run
+        if (w.counters[DERIVED_FLAG * LINE]) atomicAdd(w.totals + 7, 1ull);
+        w.counters[DERIVED_FLAG * LINE] = 0u;
+    }
     if (ld_volatile(p.control) != 0) return;
     if (id < pairs) p.contactCounts[id] = w.taskCount[2 * id] + w.taskCount[2 * id + 1];
     if (id == 0) {
@@ -3346,7 +3442,8 @@ thread_local char g_error[256] = "no error";
     X(build_hash) X(gather) X(finish_step) X(capture_kinematics) X(check_timestep) \
     X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_offsets) X(bucket_fill) \
     X(advance_walls) X(wall_count) X(wall_emit) X(wall_total) X(gather_wall_rows) X(refresh_wall_power) \
-    X(diagnostic_rows) X(fabric_rows) X(sdf_count) X(sdf_total) X(sdf_emit) X(rows)
+    X(diagnostic_rows) X(fabric_rows) X(sdf_count) X(sdf_total) X(sdf_emit) X(rows) X(derived_check) \
+    X(pack_geometry)
 enum KernelId {
 #define SAD_ID(name) KID_##name,
     SAD_KERNELS(SAD_ID)
@@ -3400,7 +3497,7 @@ struct Caps {
     uint32_t counters = 0, taskCount = 0, taskOffset = 0, adjCount = 0, adjOffset = 0, adjCursor = 0, adjList = 0;
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
     uint32_t stageCount = 0, keyCount = 0, keyStart = 0, wallCount = 0, wallOffset = 0, sdfReach = 0, sdfHits = 0;
-    uint32_t sdfLeaves = 0, sdfOrder = 0;
+    uint32_t sdfLeaves = 0, sdfOrder = 0, builtKey = 0, triPacked = 0, sampleKeyed = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -3446,6 +3543,10 @@ struct ScratchImpl : Scratch {
         graphs.clear();
     }
     int ensure_nodes(uint32_t nodes) { return grow(w.wide, caps.wide, 4 * size_t(std::max(nodes, 1u)), bytes); }
+    int ensure_packed(const P& p) {
+        int rc = grow(w.triPacked, caps.triPacked, 3 * size_t(std::max(p.triangles, 1u)), bytes);
+        return rc ? rc : grow(w.sampleKeyed, caps.sampleKeyed, std::max(p.sampleKeys, 1u), bytes);
+    }
     // SDF geometry: per-template reach and the hit-bitmask rows, refreshed whenever
     // the geometry epoch changes (an upload, or new bindings); outside any capture.
     uint32_t reachEpoch = 0, reachShapes = 0;
@@ -3477,10 +3578,23 @@ struct ScratchImpl : Scratch {
         }
         return grow(w.sdfHits, caps.sdfHits, size_t(2) * std::max(p.c.pairCapacity, 1u) * w.sdfWords, bytes);
     }
+    // A new allocation of any persistent pair-derived array (adjacency, buckets, the
+    // record of what they were built from) starts a new generation: the device
+    // header no longer matches, so the next collection rebuilds them.
+    const void* derivedPtrs[9] = {};
+    void note_derived() {
+        const void* now[9] = {w.adjCount, w.adjOffset, w.adjCursor, w.adjList, w.xcd,
+                              w.xcdTasks, w.keyCount, w.keyStart, w.builtKey};
+        if (std::memcmp(now, derivedPtrs, sizeof(now)) != 0) {
+            std::memcpy(derivedPtrs, now, sizeof(now));
+            w.builtGen++;
+        }
+    }
     int ensure_keys(uint32_t templates) {
         const uint32_t keys = bucketShape(templates).keys();
         int rc = grow(w.keyCount, caps.keyCount, keys, bytes);
         if (rc || (rc = grow(w.keyStart, caps.keyStart, keys, bytes))) return rc;
+        note_derived();
         const size_t sums = (size_t(keys) + SCAN_TILE - 1) / SCAN_TILE + 1;
         if (sums > caps.sums && (rc = grow(w.sums, caps.sums, sums, bytes))) return rc;
         w.sumsCap = caps.sums;
@@ -3524,6 +3638,8 @@ struct ScratchImpl : Scratch {
         if ((rc = grow(w.stageNa, caps.stageNa, size_t(8) * c.contactCapacity, bytes))) return rc;
         if ((rc = grow(w.xcd, caps.xcd, 24 * LINE, bytes))) return rc;
         if ((rc = grow(w.xcdTasks, caps.xcdTasks, tasks, bytes))) return rc;
+        if ((rc = grow(w.builtKey, caps.builtKey, c.pairCapacity, bytes))) return rc;
+        note_derived();
 
         w.stageCap = c.contactCapacity;
         w.taskCap = uint32_t(tasks);
@@ -3708,7 +3824,8 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.adjOffset); (void)hipFree(w.adjCursor); (void)hipFree(w.adjList); (void)hipFree(w.sums); (void)hipFree(w.sdfReach); (void)hipFree(w.sdfHits);
         (void)hipFree(w.sdfLeaves); (void)hipFree(w.sdfOrder);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
-        (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
+        (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.builtKey); (void)hipFree(w.triPacked);
+        (void)hipFree(w.sampleKeyed); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
         (void)hipFree(w.anchor); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
         (void)hipFree(triBase); (void)hipFree(w.leafDist); (void)hipFree(gapRows); (void)hipFree(lastFrames);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
@@ -3797,6 +3914,8 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
     p.meshNodes = uint32_t(bytes[SAD_MESH_LINKS] / 16);
     p.templates = uint32_t(bytes[SAD_MESH_ROOTS] / 16);
     p.planes = uint32_t(bytes[SAD_MESH_PLANES] / 16);
+    p.triangles = uint32_t(bytes[SAD_MESH_TRIANGLES] / 48);
+    p.sampleKeys = uint32_t(bytes[SAD_MESH_SAMPLE_INDICES] / 4);
     p.control = slot<uint32_t>(b, SAD_CONTROL);
     p.cell = slot<float4>(b, SAD_CELL);
     p.position = slot<float4>(b, SAD_POSITION);
@@ -3947,27 +4066,31 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         // collectMeshContacts is the reference's verification collector (same contact
         // set, atomic order); the CPU backend maps both names to its serial collector
         // (cpu.py:80-96), and so does this one, to the deterministic collection.
-        LAUNCH(sc, KID_adj_clear, k_adj_clear, perGrain, T, s, p, w);
-        LAUNCH(sc, KID_adj_count, k_adj_count, gridStride, T, s, p, w);
-        ScanArgs adj{w.adjCount, w.adjOffset, w.sums, c.N, nullptr, 1, nullptr, nullptr};
+        const uint32_t* gate = w.counters + DERIVED_FLAG * LINE;
+        LAUNCH(sc, KID_derived_check, k_derived_check, gridStride, T, s, p, w);
+        LAUNCH(sc, KID_adj_clear, k_adj_clear, perGrain, T, s, p, w, gate);
+        LAUNCH(sc, KID_adj_count, k_adj_count, gridStride, T, s, p, w, gate);
+        ScanArgs adj{w.adjCount, w.adjOffset, w.sums, c.N, nullptr, 1, gate, nullptr};
         launch_scan(sc, adj, c.N, s);
-        LAUNCH(sc, KID_adj_fill, k_adj_fill, gridStride, T, s, p, w);
-        LAUNCH(sc, KID_adj_sort, k_adj_sort, dim3(blocks(c.N, 4)), T, s, p, w);
+        LAUNCH(sc, KID_adj_fill, k_adj_fill, gridStride, T, s, p, w, gate);
+        LAUNCH(sc, KID_adj_sort, k_adj_sort, dim3(blocks(c.N, 4)), T, s, p, w, gate);
         int rc = sc.ensure_nodes(p.meshNodes);
-        if (rc) return rc;
+        if (rc || (rc = sc.ensure_packed(p))) return rc;
         LAUNCH(sc, KID_wide_nodes, k_wide_nodes, dim3(blocks(p.meshNodes)), T, s, p, w, p.meshNodes);
+        LAUNCH(sc, KID_pack_geometry, k_pack_triangles, dim3(std::min(blocks(p.triangles), 4096u)), T, s, p, w);
+        LAUNCH(sc, KID_pack_geometry, k_pack_samples, dim3(std::max(c.shapes, 1u)), T, s, p, w);
         if (p.templates > MAX_TEMPLATES) {
             std::snprintf(g_error, sizeof(g_error), "more than %u templates", MAX_TEMPLATES);
             return SAD_E_UNSUPPORTED;
         }
         if ((rc = sc.ensure_keys(p.templates))) return rc;
         const uint32_t keys = bucketShape(p.templates).keys();
-        LAUNCH(sc, KID_bucket_reset, k_bucket_reset, dim3(std::min(blocks(keys), 1024u)), T, s, p, w);
-        LAUNCH(sc, KID_bucket_count, k_bucket_count, gridStride, T, s, p, w);
-        ScanArgs keyScan{w.keyCount, w.keyStart, w.sums, keys, nullptr, 1, nullptr, w.counters + 6 * LINE};
+        LAUNCH(sc, KID_bucket_reset, k_bucket_reset, dim3(std::min(blocks(keys), 1024u)), T, s, p, w, gate);
+        LAUNCH(sc, KID_bucket_count, k_bucket_count, gridStride, T, s, p, w, gate);
+        ScanArgs keyScan{w.keyCount, w.keyStart, w.sums, keys, nullptr, 1, gate, w.counters + 6 * LINE};
         launch_scan(sc, keyScan, keys, s);
-        LAUNCH(sc, KID_bucket_offsets, k_bucket_offsets, dim3(1), dim3(64), s, p, w);
-        LAUNCH(sc, KID_bucket_fill, k_bucket_fill, gridStride, T, s, p, w);
+        LAUNCH(sc, KID_bucket_offsets, k_bucket_offsets, dim3(1), dim3(64), s, p, w, gate);
+        LAUNCH(sc, KID_bucket_fill, k_bucket_fill, gridStride, T, s, p, w, gate);
         if (w.skip) {  // persistent task rows (assigned when the list was rebuilt)
             LAUNCH(sc, KID_rows, k_rows_lookup, gridStride, T, s, p, w);
             ScanArgs need{w.rowNeed, w.rowNeedRank, w.sums, w.rowPairs, nullptr, 1, w.rowState + 4, nullptr};
@@ -4022,12 +4145,12 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         if (c.wallCount) LAUNCH(sc, KID_refresh_wall_power, k_refresh_wall_power, dim3(blocks(c.wallCount * c.N)), T, s, p);
     } else if (name == "countContacts") {
         // The gather walks each grain's pairs in pair order (as after a mesh collection).
-        LAUNCH(sc, KID_adj_clear, k_adj_clear, perGrain, T, s, p, w);
-        LAUNCH(sc, KID_adj_count, k_adj_count, gridStride, T, s, p, w);
+        LAUNCH(sc, KID_adj_clear, k_adj_clear, perGrain, T, s, p, w, nullptr);
+        LAUNCH(sc, KID_adj_count, k_adj_count, gridStride, T, s, p, w, nullptr);
         ScanArgs adj{w.adjCount, w.adjOffset, w.sums, c.N, nullptr, 1, nullptr, nullptr};
         launch_scan(sc, adj, c.N, s);
-        LAUNCH(sc, KID_adj_fill, k_adj_fill, gridStride, T, s, p, w);
-        LAUNCH(sc, KID_adj_sort, k_adj_sort, dim3(blocks(c.N, 4)), T, s, p, w);
+        LAUNCH(sc, KID_adj_fill, k_adj_fill, gridStride, T, s, p, w, nullptr);
+        LAUNCH(sc, KID_adj_sort, k_adj_sort, dim3(blocks(c.N, 4)), T, s, p, w, nullptr);
         LAUNCH(sc, KID_sdf_count, k_sdf_contacts<false>, dim3(blocks(c.pairCapacity, 4)), T, s, p, w);
     } else if (name == "contactScan") {
         ScanArgs a{p.contactCounts, p.contactOffsets, w.sums, c.pairCapacity, nullptr, 1, nullptr, nullptr};
@@ -4096,6 +4219,7 @@ int sad_hip_scratch_stats(void* scratch, uint64_t* stats) {
     stats[8] = totals[4];
     stats[9] = totals[5];
     stats[10] = totals[6];
+    stats[11] = totals[7];
     stats[4] = sc->bytes;
     unsigned long long nnv[3] = {0, 0, 0};
     if (sc->w.nnStats) {
@@ -4269,6 +4393,7 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const
             if (rc) return rc;
             if ((rc = sc->ensure(p.c))) return rc;
             if ((rc = sc->ensure_nodes(p.meshNodes))) return rc;
+            if ((rc = sc->ensure_packed(p))) return rc;
             if (p.templates <= MAX_TEMPLATES && (rc = sc->ensure_keys(p.templates))) return rc;
             if ((rc = sc->ensure_skip(p, table, sizes))) return rc;
             if ((rc = sc->ensure_sdf(p, s))) return rc;  // no allocation inside a captured step
@@ -4387,6 +4512,8 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     const size_t widest = std::max<size_t>(std::max<size_t>(tasks + 1, c.N), std::max<size_t>(wallTasks, keys));
     take(w.sums, (widest + SCAN_TILE - 1) / SCAN_TILE + 1, caps.sums);
     take(w.wide, 4 * size_t(std::max(p.meshNodes, 1u)), caps.wide);
+    take(w.triPacked, 3 * size_t(p.triangles), caps.triPacked);
+    take(w.sampleKeyed, p.sampleKeys, caps.sampleKeyed);
     take(w.stageKey, size_t(8) * c.contactCapacity, caps.stageKey);
     take(w.stageMeta, size_t(8) * c.contactCapacity, caps.stageMeta);
     take(w.stagePd, size_t(8) * c.contactCapacity, caps.stagePd);
@@ -4415,6 +4542,7 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     w.triBase = nullptr;
     w.nodeGap = nullptr;
     w.lastFrame = nullptr;
+    w.builtKey = nullptr;  // the adjacency and buckets are rebuilt in every step
     return used;
 }
 

@@ -359,10 +359,11 @@ class HipBackend:
 
     def stats(self):
         """Cumulative device work counters of this backend's workspace."""
-        out = (ct.c_uint64 * 11)()
+        out = (ct.c_uint64 * 12)()
         self._check(self.lib.sad_hip_scratch_stats(self.scratch, out), "stats")
         keys = ("collections", "directed_pairs", "queried_nodes", "contacts", "scratch_bytes", "nn_culled",
-                "nn_audited", "nn_misses", "skipped_nodes", "unchanged_winners", "anchored_tasks")
+                "nn_audited", "nn_misses", "skipped_nodes", "unchanged_winners", "anchored_tasks",
+                "derived_rebuilds")
         return dict(zip(keys, (int(v) for v in out)))
 
     def profile(self, enable=True):

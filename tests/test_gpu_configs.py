@@ -168,12 +168,22 @@ def test_neighbour_rebuilds_with_skip_and_warm_start_match_oracle(pkg, oracle_cp
     b.initialize()
     rebuilds = 0
     anchored_at_rebuilds = []  # anchors taken by the collection right after each rebuild
+    derived_at_rebuilds = 0
     for step in range(120):  # one step per call: the anchors persist across calls
         before = hip.stats()
         a.advance(1, rate=rate)
         b.advance(1, rate=rate)
         after = hip.stats()
         rebuilt = int(a.read("control", np.uint32, 1)[4])
+        # The per-grain adjacency and task buckets are rebuilt only when the pair
+        # list changed (never between neighbour-list rebuilds, although every call
+        # uploads the state again).
+        derived = after["derived_rebuilds"] - before["derived_rebuilds"]
+        if step > 0:
+            assert derived <= rebuilt, (step, rebuilt, derived)
+            derived_at_rebuilds += derived
+        else:
+            early = a.state(), b.state(), a.step
         if rebuilt or step % 20 == 19:
             assert_same(a, b, f"step {step + 1}")
             assert np.array_equal(a.read("referencePosition"), b.read("referencePosition"))
@@ -182,6 +192,17 @@ def test_neighbour_rebuilds_with_skip_and_warm_start_match_oracle(pkg, oracle_cp
             anchored_at_rebuilds.append((after["anchored_tasks"] - before["anchored_tasks"]) / tasks)
         rebuilds += rebuilt
     assert rebuilds >= 2, rebuilds
+    assert derived_at_rebuilds >= 1, derived_at_rebuilds
+    # Restoring the first step's state (an older pair list) into the same workspace:
+    # the collection finds the list changed and rebuilds what derives from it.
+    before = hip.stats()
+    a.restore(early[0], early[2])
+    b.restore(early[1], early[2])
+    for k in range(3):
+        a.advance(1, rate=rate)
+        b.advance(1, rate=rate)
+        assert_same(a, b, f"step {k + 1} after restoring step 1")
+    assert hip.stats()["derived_rebuilds"] > before["derived_rebuilds"]
     assert hip.stats()["skipped_nodes"] > 0
     # A rebuilt list keeps each surviving pair's rows: only new pairs (and pairs at
     # their re-anchor distance) take anchors, not the whole list.
