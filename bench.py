@@ -127,12 +127,21 @@ def nn_leg(args, w, e, hip, start, step0, plain_state, log_):
     lc.attach(audit=False)
     e.restore(start, step0)
     hip.wait()
+    before = hip.stats()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e.advance(args.steps, rate=w.rate)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    after = hip.stats()
     lc.detach()
+    out["timed"] = {"culled_per_step": (after["nn_culled"] - before["nn_culled"]) / args.steps,
+                    "queued_per_step": (after["queried_nodes"] - before["queried_nodes"]) / args.steps,
+                    "contacts_per_step": (after["contacts"] - before["contacts"]) / args.steps}
+    # Any sound cull drops only queued nodes that are not contacts: (Q - C) / Q of the
+    # exact queries is the most it can remove.
+    t = out["timed"]
+    out["sound_cull_bound"] = max(t["queued_per_step"] - t["contacts_per_step"], 0.0) / max(t["queued_per_step"], 1.0)
     st = e.state()
     same = st["contacts"] == plain_state["contacts"] and all(
         np.array_equal(st[k], plain_state[k]) for k in ("meta", "force", "torque", "position", "rotation"))
@@ -231,9 +240,10 @@ def cpu_legs(args, w, state, step0, hip_rate, log_):
     else:
         one.restore(one_state, one_step)
     t0 = time.perf_counter()
-    one.advance(1, rate=hip_rate)
+    one.advance(args.cpu_single_steps, rate=hip_rate)
     one_s = time.perf_counter() - t0
-    single = {"value": single_n / one_s, "threads": 1, "steps": 1, "grains": single_n}
+    single = {"value": single_n * args.cpu_single_steps / one_s, "threads": 1, "steps": args.cpu_single_steps,
+              "grains": single_n}
     log_(f"[bench] cpu oracle on 1 thread: {single['value']:.4g} grain-steps/s ({single_n} grains)")
     baseline = {
         "value": many["value"],
@@ -245,9 +255,9 @@ def cpu_legs(args, w, state, step0, hip_rate, log_):
         "cpu_model": cpu_model(),
         "host_cores": cores,
         "single_thread": dict(single, sample=(
-            f"1 step of the same workload on 1 thread" if single_n == n else
-            f"1 step of a {single_n}-grain assembly from the same template library and solids fraction "
-            f"on 1 thread (the full {n}-grain step would take minutes)")),
+            f"{args.cpu_single_steps} steps of the same workload on 1 thread" if single_n == n else
+            f"{args.cpu_single_steps} steps of a {single_n}-grain assembly from the same template library and "
+            f"solids fraction on 1 thread (a full {n}-grain step takes about {n / single['value']:.0f} s there)")),
     }
     return baseline, parity
 
@@ -411,6 +421,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--cpu-single-max", type=int, default=5000,
                     help="largest assembly the single-thread CPU leg steps (a smaller one is drawn beyond)")
+    ap.add_argument("--cpu-single-steps", type=int, default=5, help="steps of the single-thread CPU leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one instead of a captured step")
     ap.add_argument("--no-autotune", action="store_true")

@@ -246,8 +246,9 @@ int sad_hip_nn_sample(const uint32_t* uniforms, void* const* buffers, const uint
                       const double* values, const void* triangles, const float* cdf,
                       const uint32_t* tri_start, const uint32_t* tri_count, const void* box,
                       void* points, float* labels, void* stream);
-/* One BF16 MFMA 16x16x16 tile (row-major A, B; FP32 D) for operand-layout tests. */
-int sad_hip_mfma_probe(const void* a, const void* b, float* d, void* stream);
+/* One BF16 MFMA 16x16xk tile, k = 16 or 32 (row-major A 16 x k, B k x 16; FP32 D
+ * 16 x 16) for operand-layout tests. */
+int sad_hip_mfma_probe(const void* a, const void* b, float* d, int k, void* stream);
 
 /* Name of the last HIP error seen by this library (static string). */
 const char* sad_hip_last_error(void);

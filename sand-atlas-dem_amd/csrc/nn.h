@@ -1,15 +1,18 @@
 // nn.h -- learned signed-distance cull: one small MLP per grain template,
 // positional encoding (6 octaves) -> 64 -> 64 -> 64 -> 1 with ReLU, evaluated with
-// BF16 MFMA (v_mfma_f32_16x16x16_bf16) and FP32 accumulation.
+// gfx950's BF16 MFMA (v_mfma_f32_16x16x32_bf16; one 16x16x16 step for the
+// 48-wide input layer's last 16 columns) and FP32 accumulation.
 //
 // No reference counterpart (the reference has only the exact BVH query and its
 // conservative hull culls).  The network predicts the signed distance of a point
 // in a template's frame; a node whose prediction exceeds the template's margin
 // (three times the largest validation error) skips the exact query.
 //
-// MFMA 16x16x16 operand layout per lane l (0..63): A[m][k] with m = l % 16 and
-// k = 4 (l / 16) + {0..3}; B[k][n] with n = l % 16 and k = 4 (l / 16) + {0..3};
-// D[m][n] with n = l % 16 and m = 4 (l / 16) + {0..3}.
+// MFMA operand layouts per lane l (0..63), D[m][n] with n = l % 16 and
+// m = 4 (l / 16) + {0..3} for both shapes:
+//   16x16x16: A[m][k] with m = l % 16, k = 4 (l / 16) + {0..3}; B[k][n] with n = l % 16
+//             and the same k;
+//   16x16x32: A[m][k] with m = l % 16, k = 8 (l / 16) + {0..7}; B[k][n] likewise.
 #pragma once
 #include "dem_math.h"
 
@@ -31,13 +34,33 @@ constexpr int PARAMS = OFF_B4 + 1;  // 10,945
 constexpr int STRIDE = 11008;        // per-template stride of master / gradient / Adam buffers
 // BF16 weight block per template (MFMA B operands; W1 zero-padded to K1 columns).
 constexpr int BW1 = 0, BW2 = H * K1, BW3 = BW2 + H * H, BSTRIDE = BW3 + H * H;
-// FP32 inference block per template: biases, output layer, input normalisation, margin.
-constexpr int FB1 = 0, FB2 = 64, FB3 = 128, FW4 = 192, FB4 = 256, FCENTER = 257, FSCALE = 260, FMARGIN = 261;
+// FP32 inference block per template: biases, output layer, input normalisation, margin
+// (label units), the signed distance in template units at or below which the network
+// cannot cull a node within its validated error ((margin - error) x label scale), and
+// the label scale (template units per label unit).
+constexpr int FB1 = 0, FB2 = 64, FB3 = 128, FW4 = 192, FB4 = 256, FCENTER = 257, FSCALE = 260, FMARGIN = 261,
+              FNEAR = 262, FLABEL = 263;
 constexpr int FSTRIDE = 264;
 }  // namespace nn
 
 typedef short nn_bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 nn_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float nn_f32x4 __attribute__((ext_vector_type(4)));
+
+// acc += A B over one 32-wide k block: a and b are lane l's 8 consecutive k values.
+DEM_D nn_f32x4 nn_mfma32(const uint16_t* a, const uint16_t* b, nn_f32x4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const nn_bf16x8*>(a),
+                                                   *reinterpret_cast<const nn_bf16x8*>(b), acc, 0, 0, 0);
+}
+// The same from 8 values gathered per lane (strided operands).
+DEM_D nn_f32x4 nn_mfma32g(const uint16_t (&a)[8], const uint16_t (&b)[8], nn_f32x4 acc) {
+    nn_bf16x8 av, bv;
+    for (int j = 0; j < 8; j++) {
+        av[j] = __builtin_bit_cast(__bf16, a[j]);
+        bv[j] = __builtin_bit_cast(__bf16, b[j]);
+    }
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+}
 
 DEM_D uint16_t nn_to_bf16(float x) {  // round to nearest even
     uint32_t u = __float_as_uint(x);
@@ -54,6 +77,7 @@ DEM_D void nn_features(v3 p, const float* f, uint16_t* row) {
     row[1] = nn_to_bf16(u[1]);
     row[2] = nn_to_bf16(u[2]);
     float scale = 3.14159265358979f;
+#pragma unroll 1
     for (int k = 0; k < nn::OCTAVES; k++) {
         for (int d = 0; d < 3; d++) {
             float a = scale * u[d];
@@ -67,15 +91,28 @@ DEM_D void nn_features(v3 p, const float* f, uint16_t* row) {
 
 // out[r][:] = relu(in[r][:] W^T + b) for RT 16-row tiles; in has KT*16 columns.
 // W is [64][KT*16] BF16 row-major in global memory; in/out are BF16 in LDS.
-template <int RT, int KT>
+// K32 selects the 16x16x32 MFMA for 32-wide k blocks (the rest in 16-wide steps).
+// The collection kernel's inference uses the 16x16x16 form only: under that
+// kernel's register pressure hipcc (ROCm 7.2) assigns an MFMA's destination to
+// registers of its A or B operand.  The 16x16x16 form gives correct results so;
+// the 16x16x32 form, whose operands span twice the registers, gave wrong
+// predictions there, while in the training and batch-inference kernels (operands
+// kept apart) it equals the 16x16x16 form bit for bit.
+template <int RT, int KT, bool K32 = true>
 DEM_D void nn_layer(const uint16_t* in, uint16_t* out, const uint16_t* __restrict__ W, const float* __restrict__ bias,
                     int lane) {
     constexpr int LD = KT * 16;
-    const int r = lane & 15, q = (lane >> 4) * 4;
-    for (int nt = 0; nt < 4; nt++) {
+    constexpr int BLOCKS32 = K32 ? KT / 2 : 0;
+    const int r = lane & 15, q = (lane >> 4) * 4, q8 = (lane >> 4) * 8;
+#pragma unroll 1
+    for (int nt = 0; nt < 4; nt++) {  // not unrolled: few registers live inside the collection kernel
         nn_f32x4 acc[RT];
         for (int mt = 0; mt < RT; mt++) acc[mt] = nn_f32x4{0, 0, 0, 0};
-        for (int kt = 0; kt < KT; kt++) {
+        for (int kb = 0; kb < BLOCKS32; kb++) {
+            const uint16_t* b = W + (nt * 16 + r) * LD + kb * 32 + q8;
+            for (int mt = 0; mt < RT; mt++) acc[mt] = nn_mfma32(in + (mt * 16 + r) * LD + kb * 32 + q8, b, acc[mt]);
+        }
+        for (int kt = 2 * BLOCKS32; kt < KT; kt++) {
             nn_bf16x4 b = *reinterpret_cast<const nn_bf16x4*>(W + (nt * 16 + r) * LD + kt * 16 + q);
             for (int mt = 0; mt < RT; mt++) {
                 nn_bf16x4 a = *reinterpret_cast<const nn_bf16x4*>(in + (mt * 16 + r) * LD + kt * 16 + q);
@@ -92,16 +129,16 @@ DEM_D void nn_layer(const uint16_t* in, uint16_t* out, const uint16_t* __restric
 // Predicted signed distance of RT*16 points whose feature rows are in `bufA`
 // (row r at bufA + r*K1); `bufB` is scratch of RT*16*64 BF16.  Lane l < RT*16
 // returns the prediction of row l.
-template <int RT>
+template <int RT, bool K32 = true>
 DEM_D float nn_forward(uint16_t* bufA, uint16_t* bufB, const uint16_t* __restrict__ wb, const float* __restrict__ f,
                        int lane) {
-    nn_layer<RT, 3>(bufA, bufB, wb + nn::BW1, f + nn::FB1, lane);
+    nn_layer<RT, 3, K32>(bufA, bufB, wb + nn::BW1, f + nn::FB1, lane);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    nn_layer<RT, 4>(bufB, bufA, wb + nn::BW2, f + nn::FB2, lane);
+    nn_layer<RT, 4, K32>(bufB, bufA, wb + nn::BW2, f + nn::FB2, lane);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    nn_layer<RT, 4>(bufA, bufB, wb + nn::BW3, f + nn::FB3, lane);
+    nn_layer<RT, 4, K32>(bufA, bufB, wb + nn::BW3, f + nn::FB3, lane);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     float y = f[nn::FB4];
@@ -118,19 +155,19 @@ DEM_D float nn_forward(uint16_t* bufA, uint16_t* bufB, const uint16_t* __restric
 // atomics.  M = outputs, N = inputs, K = rows.
 template <int LDA, int NIN>
 DEM_D void nn_weight_grad(const uint16_t* G, const uint16_t* A, float* accW, int lane) {
-    const int r = lane & 15, q = (lane >> 4) * 4;
+    const int r = lane & 15, q = (lane >> 4) * 4, q8 = (lane >> 4) * 8;
     constexpr int NT = (NIN + 15) / 16;
     for (int mt = 0; mt < 4; mt++)
         for (int nt = 0; nt < NT; nt++) {
             nn_f32x4 acc = {0, 0, 0, 0};
-            for (int kt = 0; kt < 4; kt++) {
-                nn_bf16x4 a, b;
-                for (int j = 0; j < 4; j++) {
-                    int row = kt * 16 + q + j;
-                    a[j] = short(G[row * nn::H + mt * 16 + r]);
-                    b[j] = short(A[row * LDA + nt * 16 + r]);
+            for (int kt = 0; kt < 2; kt++) {  // 64 rows = two 32-wide k blocks
+                uint16_t a[8], b[8];
+                for (int j = 0; j < 8; j++) {
+                    int row = kt * 32 + q8 + j;
+                    a[j] = G[row * nn::H + mt * 16 + r];
+                    b[j] = A[row * LDA + nt * 16 + r];
                 }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+                acc = nn_mfma32g(a, b, acc);
             }
             int in = nt * 16 + r;
             if (in < NIN)
@@ -142,17 +179,16 @@ DEM_D void nn_weight_grad(const uint16_t* G, const uint16_t* A, float* accW, int
 // BF16 rounding) is added to accB.
 DEM_D void nn_backprop(const uint16_t* G, const uint16_t* __restrict__ W, const uint16_t* Hmask, uint16_t* Gout,
                        float* accB, int lane) {
-    const int r = lane & 15, q = (lane >> 4) * 4;
+    const int r = lane & 15, q = (lane >> 4) * 4, q8 = (lane >> 4) * 8;
     for (int nt = 0; nt < 4; nt++) {
         const int col = nt * 16 + r;
         float colsum = 0.0f;
         for (int mt = 0; mt < 4; mt++) {
             nn_f32x4 acc = {0, 0, 0, 0};
-            for (int kt = 0; kt < 4; kt++) {
-                nn_bf16x4 a = *reinterpret_cast<const nn_bf16x4*>(G + (mt * 16 + r) * nn::H + kt * 16 + q);
-                nn_bf16x4 b;
-                for (int j = 0; j < 4; j++) b[j] = short(W[(kt * 16 + q + j) * nn::H + nt * 16 + r]);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            for (int kt = 0; kt < 2; kt++) {
+                uint16_t b[8];
+                for (int j = 0; j < 8; j++) b[j] = W[(kt * 32 + q8 + j) * nn::H + nt * 16 + r];
+                acc = nn_mfma32(G + (mt * 16 + r) * nn::H + kt * 32 + q8, b, acc);
             }
             for (int i = 0; i < 4; i++) {
                 int row = mt * 16 + q + i;
@@ -316,14 +352,25 @@ __global__ __launch_bounds__(64) void k_nn_infer(const float4* pts, const uint32
     float y = nn_forward<4>(A, B, wb + size_t(t) * nn::BSTRIDE, f, lane);
     out[size_t(tile) * 64 + lane] = y;
 }
-// MFMA operand-layout probe: D = A B for one 16x16x16 BF16 tile (A, B row-major).
-__global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t* A, const uint16_t* B, float* D) {
-    const int lane = threadIdx.x, r = lane & 15, q = (lane >> 4) * 4;
-    nn_bf16x4 a, b;
-    for (int j = 0; j < 4; j++) {
-        a[j] = short(A[r * 16 + q + j]);
-        b[j] = short(B[(q + j) * 16 + r]);
+// MFMA operand-layout probe: D = A B for one BF16 tile, 16x16xK with K = 16 or 32
+// (A is 16 x K and B is K x 16, both row-major).
+__global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, int k32) {
+    const int lane = threadIdx.x, r = lane & 15, q = (lane >> 4) * 4, q8 = (lane >> 4) * 8;
+    nn_f32x4 d = {0, 0, 0, 0};
+    if (k32) {
+        uint16_t a[8], b[8];
+        for (int j = 0; j < 8; j++) {
+            a[j] = A[r * 32 + q8 + j];
+            b[j] = B[(q8 + j) * 16 + r];
+        }
+        d = nn_mfma32g(a, b, d);
+    } else {
+        nn_bf16x4 a, b;
+        for (int j = 0; j < 4; j++) {
+            a[j] = short(A[r * 16 + q + j]);
+            b[j] = short(B[(q + j) * 16 + r]);
+        }
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, d, 0, 0, 0);
     }
-    nn_f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, nn_f32x4{0, 0, 0, 0}, 0, 0, 0);
     for (int i = 0; i < 4; i++) D[(q + i) * 16 + r] = d[i];
 }

@@ -203,6 +203,9 @@ constexpr int LINE = 64;
 #ifndef SAD_POP
 #define SAD_POP 1
 #endif
+#ifndef SAD_COLLECT_NN_WAVES_PER_EU
+#define SAD_COLLECT_NN_WAVES_PER_EU 4
+#endif
 #ifndef SAD_COLLECT_WAVES_PER_EU
 #define SAD_COLLECT_WAVES_PER_EU 5
 #endif
@@ -1346,39 +1349,6 @@ __device__ void stageHits(const P& p, const W& w, const TaskFrame& f, uint32_t t
     running += count;
 }
 
-// Learned cull of up to 64 queued nodes (two 32-row MFMA passes).  The BF16
-// activations use the wave's BVH-stack LDS (idle outside the exact queries).
-// Returns this lane's verdict for queue entry q + lane.
-__device__ bool cullBatch(const P& p, const W& w, const TaskFrame& f, const uint32_t* queue, uint32_t q, uint32_t qn,
-                          uint16_t* bufA, uint16_t* bufB, int lane) {
-    if (f.sj >= w.nnTemplates) return false;
-    const uint16_t* wb = w.nnW + size_t(f.sj) * nn::BSTRIDE;
-    const float* fb = w.nnF + size_t(f.sj) * nn::FSTRIDE;
-    const float margin = fb[nn::FMARGIN];
-    bool verdict[2] = {false, false};
-    for (int half = 0; half < 2; half++) {
-        if (q + half * 32 >= qn) break;
-        if (lane < 32) {
-            uint32_t k = q + half * 32 + lane;
-            uint16_t* row = bufA + lane * nn::K1;
-            if (k < qn) {
-                float4 sp = p.sample[f.sampleBase + (queue[k] & 0xffffu)];
-                v3 arm = qrot(f.qi, xyz(sp) * f.s_i);
-                v3 local = qinv(f.qj, arm - f.branch) / f.s_j;
-                nn_features(local, fb, row);
-            } else {
-                for (int c = 0; c < nn::K1; c++) row[c] = 0;
-            }
-        }
-        wave_sync();
-        float y = nn_forward<2>(bufA, bufB, wb, fb, lane);
-        verdict[half] = lane < 32 && q + half * 32 + lane < qn && y > margin;
-        wave_sync();
-    }
-    bool second = __shfl(verdict[1], lane & 31);
-    return lane < 32 ? verdict[0] : second;
-}
-
 // Widening of the warm-start bound, in template units of distance: about 30 ulps
 // of a coordinate of magnitude 1, far above the closest point's rounding.
 constexpr float WARM_SLACK = 4e-6f;
@@ -1522,7 +1492,7 @@ struct BatchRec {  // 112 bytes in LDS per task of the batch
     float4 qi, qj;
     float4 branchSi;  // branch, s_i
     float4 tail;      // s_j, min(wi, wj), unused, unused
-    uint32_t i, j, row, unused;  // row: the task's row in the persistent per-task rows, or NO_ROW
+    uint32_t i, j, row, sj;  // row: the task's row in the persistent per-task rows, or NO_ROW; sj: target template
 };
 constexpr uint32_t NO_ROW = 0xffffffffu;
 static_assert(sizeof(BatchRec) == 112, "batch record layout");
@@ -1540,7 +1510,7 @@ __device__ __forceinline__ void batchPut(BatchRec* r, const P& p, const W& w, co
         r->qi = f4(f.qi); r->qj = f4(f.qj);
         r->branchSi = f4(f.branch, f.s_i);
         r->tail = make_float4(f.s_j, fminf(f.wi, f.wj), 0.0f, 0.0f);
-        r->i = f.i; r->j = f.j; r->row = row;
+        r->i = f.i; r->j = f.j; r->row = row; r->sj = f.sj;
     }
 }
 
@@ -1566,6 +1536,10 @@ constexpr uint32_t NO_TRIANGLE = 0xffffffffu;
 // Entries whose winner the unchanged-winner shortcut proved hold it from the start.
 constexpr uint32_t PENDING = 0x40000000u;
 __device__ __forceinline__ bool isPending(uint32_t v) { return (v & 0xc0000000u) == PENDING; }
+// A pending entry the learned cull may drop (its network is evaluated before the
+// traversals): set at admission, ignored by the traversal (it reads the batch slot
+// from the low bits).
+constexpr uint32_t NN_CAND = 0x20000000u;
 // Position of the r-th set bit (from 0) of m; m holds more than r set bits.
 __device__ __forceinline__ uint32_t selectBit(uint64_t m, uint32_t r) {
     uint32_t pos = 0, c = __popc(uint32_t(m));
@@ -1581,10 +1555,10 @@ __device__ __forceinline__ uint32_t selectBit(uint64_t m, uint32_t r) {
     if (r >= (uint32_t(m) & 1u)) pos += 1;
     return pos;
 }
-template <typename D>
+template <bool NN, typename D>
 __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, uint32_t nb, uint32_t stageQueue,
                                const uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
-                               uint32_t carry, bool lastOpen, int lane) {
+                               uint32_t carry, bool lastOpen, int lane, const uint32_t* auditBits) {
     const float4* __restrict__ wide = w.wide;
     const bool gapOn = w.warm && w.nodeGap;
     uint32_t k = 0, top = 0, bestTri = NO_TRIANGLE, slotNow = 0, rowNow = 0, genNow = 0;
@@ -1777,7 +1751,7 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
         uint32_t running = s == 0 ? carry : 0u;
         for (uint32_t b = from; b < to; b += WAVE) {
             const uint32_t kk = b + lane;
-            bool hit = false;
+            bool hit = false, boundViolation = false;
             uint32_t node = 0;
             float4 pd = make_float4(0, 0, 0, 0), na = make_float4(0, 0, 0, 0);
             if (kk < to) {
@@ -1800,7 +1774,14 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
                 const float distance = sqrtf(bestD);
                 const float side = dot(x - nearest, pseudo) < 0 ? -1.0f : 1.0f;
                 const v3 normal = distance > 1e-12f ? side * (x - nearest) / distance : pseudo;
-                if (refreshDist) refreshDist[slot] = quantizeDistance(side * distance);
+                bool keepBound = false;  // an audit-marked entry keeps the network's bound (as when culled)
+                if constexpr (NN) {
+                    if (auditBits && refreshDist && ((auditBits[kk / 32] >> (kk % 32)) & 1u) != 0u) {
+                        keepBound = true;
+                        if (side * distance < float(refreshDist[slot]) * SKIP_QUANTUM) boundViolation = true;
+                    }
+                }
+                if (refreshDist && !keepBound) refreshDist[slot] = quantizeDistance(side * distance);
                 if (warmTri && triangle != NO_TRIANGLE) warmTri[slot] = uint16_t(triangle - triFirst + 1);
                 // The rest of queryNode.
                 const float depth = -(side * distance) * f.s_j;
@@ -1817,6 +1798,13 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
                     }
                 }
             }
+            if constexpr (NN) {  // audit: contacts, or distances below the network's bound, among dropped entries
+                if (auditBits) {
+                    const bool marked = kk < to && ((auditBits[kk / 32] >> (kk % 32)) & 1u) != 0u;
+                    const uint32_t missed = __popcll(__ballot(marked && (hit || boundViolation)));
+                    if (missed && lane == 0) atomicAdd(w.nnStats + LINE, (unsigned long long)missed);
+                }
+            }
             stageHits(p, w, f, task, stageQueue, hit, node, pd, na, running, lane);
         }
         from = to;
@@ -1829,64 +1817,110 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
     return open;
 }
 
-// Learned cull of the queue: drops nodes whose predicted signed distance exceeds
-// the template margin, compacting survivors in place (order kept, so hit ranks
-// are those of the unculled walk).  Audit mode keeps every node, queries it and
-// counts culled nodes that were hits.  Returns the surviving queue length.
-__device__ uint32_t cullQueue(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
-                              uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, float* sdist,
-                              uint32_t& running, int lane, uint16_t* nnA, uint16_t* nnB) {
-    uint32_t kept = 0, culledTotal = 0, missTotal = 0;
-    for (uint32_t q = 0; q < qn; q += WAVE) {
-        uint32_t k = q + lane;
-        bool culled = cullBatch(p, w, f, queue, q, qn, nnA, nnB, lane);
-        const uint32_t word = k < qn ? queue[k] : 0u, node = word & 0xffffu;
-        const uint32_t state = k < qn ? nearestTri[k] : 0u;
-        culledTotal += __popcll(__ballot(culled));
-        if (w.nnAudit) {
-            bool hit = false;
-            float4 pd = make_float4(0, 0, 0, 0), na = make_float4(0, 0, 0, 0);
-            if (k < qn) hit = queryNode(p, w, f, node, sref, sdist, pd, na);
-            missTotal += __popcll(__ballot(culled && hit));
-            stageHits(p, w, f, task, stageQueue, hit, node, pd, na, running, lane);
-        } else {
-            bool keep = k < qn && !culled;
-            uint64_t m = __ballot(keep);
-            wave_sync();
-            if (keep) {
-                const uint32_t at = kept + __popcll(m & lanes_below(lane));
-                queue[at] = word;
-                nearestTri[at] = state;
+// Learned cull of a query batch, before its traversals.  The entries admitted as
+// candidates (NN_CAND) are evaluated 16 at a time per task, in that task's frame with
+// its target template's network; an entry whose predicted signed distance exceeds
+// the template's margin is dropped, and the survivors are compacted in place in
+// order (the hit ranks are those of the unculled walk, since a dropped entry is not
+// a contact when the network is right).  In audit mode nothing is dropped: the
+// entries are marked in `auditBits`, every entry is queried and marked hits are
+// counted as misses.  Returns the surviving queue length.
+__device__ uint32_t cullEntries(const P& p, const W& w, BatchRec* batch, uint32_t nb, uint32_t* queue,
+                                uint32_t* nearestTri, uint32_t qn, uint32_t* auditBits, uint16_t* nnA,
+                                uint16_t* nnB, int lane) {
+    const bool audit = w.nnAudit != 0u;
+    if (audit)
+        for (uint32_t k = lane; k < (qn + 31) / 32; k += WAVE) auditBits[k] = 0u;
+    wave_sync();
+    uint32_t from = 0, out = 0, culledTotal = 0;
+    for (uint32_t s = 0; s < nb; s++) {
+        BatchRec& r = batch[s];
+        const uint32_t end = uniformU(r.end), to = end & 0xffffu, sj = uniformU(r.sj);
+        const bool on = sj < w.nnTemplates && w.nnF[size_t(sj) * nn::FSTRIDE + nn::FMARGIN] < 1e30f;
+        const uint16_t* wb = w.nnW + size_t(sj) * nn::BSTRIDE;
+        const float* fb = w.nnF + size_t(sj) * nn::FSTRIDE;
+        const float margin = on ? fb[nn::FMARGIN] : 0.0f;
+        // A re-anchoring task records a distance bound for each node it drops, so the
+        // temporal skip passes the node over in later steps as it would after an
+        // exact query: (min(y, 1) - margin) x label scale for a prediction y.  It
+        // fails only where the network errs by more than the margin, the same
+        // error that would drop a contact (the audit checks both).
+        int8_t* refreshDist = (end >> 16) != 0u && uniformU(r.row) != NO_ROW
+                                  ? w.nodeDist + size_t(uniformU(r.row)) * w.nodeStride
+                                  : nullptr;
+        const float labelScale = on ? fb[nn::FLABEL] : 0.0f;
+        for (uint32_t b = from; b < to; b += WAVE) {
+            const uint32_t k = b + lane;
+            const uint32_t word = k < to ? queue[k] : 0u, state = k < to ? nearestTri[k] : 0u;
+            uint64_t cand = __ballot(on && k < to && isPending(state) && (state & NN_CAND) != 0u);
+            uint64_t culled = 0;
+            while (cand != 0) {
+                const uint32_t rows = min(uint32_t(__popcll(cand)), 16u);
+                const uint32_t pos = lane < 16 && uint32_t(lane) < rows ? selectBit(cand, lane) : 0u;
+                if (lane < 16) {
+                    uint16_t* row = nnA + lane * nn::K1;
+                    if (uint32_t(lane) < rows) {  // the node's point in the task's frame (read here: few live registers)
+                        const float4 sp = p.sample[r.sampleBase + (queue[b + pos] & 0xffffu)];
+                        const float4 bs = r.branchSi;
+                        const v3 arm = qrot(v4of(r.qi), xyz(sp) * bs.w);
+                        nn_features(qinv(v4of(r.qj), arm - xyz(bs)) / r.tail.x, fb, row);
+                    } else {
+                        for (int c = 0; c < nn::K1; c++) row[c] = 0;
+                    }
+                }
+                wave_sync();
+                const float y = nn_forward<1, false>(nnA, nnB, wb, fb, lane);  // 16x16x16 MFMA here (nn_layer)
+                const bool dropMine = lane < 16 && uint32_t(lane) < rows && y > margin;
+                if (dropMine && refreshDist)
+                    refreshDist[queue[b + pos] >> 16] = quantizeDistance((fminf(y, 1.0f) - margin) * labelScale);
+                const uint64_t drop = __ballot(dropMine);
+                for (uint64_t m = drop; m != 0; m &= m - 1) culled |= 1ull << __shfl(pos, __builtin_ctzll(m));
+                for (uint32_t c = 0; c < rows; c++) cand &= cand - 1;  // the rows just evaluated
+                wave_sync();
             }
-            kept += __popcll(m);
+            culledTotal += __popcll(culled);
+            const bool mine = ((culled >> lane) & 1ull) != 0ull;
+            if (audit) {
+                if (mine) atomicOr(auditBits + k / 32, 1u << (k % 32));
+                out = to;
+            } else {
+                const bool keep = k < to && !mine;
+                const uint64_t m = __ballot(keep);
+                if (keep) {
+                    const uint32_t at = out + __popcll(m & lanes_below(lane));
+                    queue[at] = word;
+                    nearestTri[at] = state;
+                }
+                out += __popcll(m);
+            }
             wave_sync();
         }
+        if (lane == 0) r.end = (end & 0xffff0000u) | out;
+        from = to;
     }
+    wave_sync();
     if (lane == 0 && culledTotal) {
         atomicAdd(w.nnStats, (unsigned long long)culledTotal);
-        if (w.nnAudit) atomicAdd(w.nnStats + LINE / 2, (unsigned long long)culledTotal);
-        if (missTotal) atomicAdd(w.nnStats + LINE, (unsigned long long)missTotal);
+        if (audit) atomicAdd(w.nnStats + LINE / 2, (unsigned long long)culledTotal);
     }
-    return w.nnAudit ? 0u : kept;
+    return out;
 }
 
 template <bool NN, typename D>
-__device__ uint32_t flushBatch(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t stageQueue,
-                               uint32_t* queue, uint32_t* nearestTri, BatchRec* batch, uint32_t nb, uint32_t qn,
-                               uint32_t* sref, D* sdist, uint32_t carry, bool lastOpen, int lane, uint16_t* nnA,
+__device__ uint32_t flushBatch(const P& p, const W& w, uint32_t stageQueue, uint32_t* queue, uint32_t* nearestTri,
+                               BatchRec* batch, uint32_t nb, uint32_t qn, uint32_t* sref, D* sdist, uint32_t carry,
+                               bool lastOpen, int lane, uint32_t& nnPending, uint32_t* auditBits, uint16_t* nnA,
                                uint16_t* nnB) {
-    // The learned cull batches one task at a time (the network is the target
-    // template's): nb == 1 and f is that task's frame.  Templates whose margin is
-    // infinite (calibration found no safe cull) skip the network.
+    const uint32_t* audited = nullptr;
     if constexpr (NN) {
-        if (f.sj < w.nnTemplates && w.nnF[size_t(f.sj) * nn::FSTRIDE + nn::FMARGIN] < 1e30f) {
-            qn = cullQueue(p, w, f, task, stageQueue, queue, nearestTri, qn, sref, sdist, carry, lane, nnA, nnB);
-            wave_sync();
-            if (lane == 0) batch[0].end = (batch[0].end & 0xffff0000u) | qn;
-            wave_sync();
+        if (nnPending != 0u) {
+            qn = cullEntries(p, w, batch, nb, queue, nearestTri, qn, auditBits, nnA, nnB, lane);
+            if (w.nnAudit) audited = auditBits;
         }
+        nnPending = 0u;
     }
-    return queryBatch(p, w, batch, nb, stageQueue, queue, nearestTri, qn, sref, sdist, carry, lastOpen, lane);
+    return queryBatch<NN, D>(p, w, batch, nb, stageQueue, queue, nearestTri, qn, sref, sdist, carry, lastOpen, lane,
+                             audited);
 }
 
 // One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
@@ -1895,22 +1929,25 @@ __device__ uint32_t flushBatch(const P& p, const W& w, const TaskFrame& f, uint3
 // are ranked by their running count within the task, not by flush boundaries).
 template <bool NN, int QN>
 __device__ __forceinline__ void collectBody(P p, W w) {
-    // The learned-cull variant needs the FP32 distance stack: its 4 KB doubles as a
-    // BF16 activation buffer of the network.
-    using D = typename std::conditional<NN, float, uint16_t>::type;
-    __shared__ uint32_t stackRef[COLLECT_WAVES][WSTACK * WAVE];
+    using D = uint16_t;
+    // The learned-cull variant evaluates its network before a batch's traversals:
+    // the traversal stack's 4 KB then holds two 16 x 64 BF16 activation buffers.
+    __shared__ __align__(16) uint32_t stackRef[COLLECT_WAVES][WSTACK * WAVE];
     __shared__ D stackDist[COLLECT_WAVES][WSTACK * WAVE];
     __shared__ uint32_t queues[COLLECT_WAVES][QN];
     __shared__ uint32_t nearestTris[COLLECT_WAVES][QN];
     __shared__ uint64_t candidates[COLLECT_WAVES][MAX_CLUSTERS / 64];
-    // The learned cull batches one task; the 128-node queue leaves LDS for 8 records.
-    constexpr uint32_t batchCap = NN ? 1u : (QN >= 256 ? BATCH_MAX : 8u);
+    // The 128-node queue leaves LDS for 8 batch records.
+    constexpr uint32_t batchCap = QN >= 256 ? BATCH_MAX : 8u;
     __shared__ BatchRec batches[COLLECT_WAVES][batchCap];
+    __shared__ uint32_t auditWords[COLLECT_WAVES][NN ? QN / 32 : 1];  // learned-cull audit marks per entry
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t* sref = &stackRef[wv][lane];
     D* sdist = &stackDist[wv][lane];
-    uint16_t* nnA = reinterpret_cast<uint16_t*>(stackRef[wv]);   // 32 x 64 BF16 = the stack's 4 KB
-    uint16_t* nnB = reinterpret_cast<uint16_t*>(stackDist[wv]);
+    uint16_t* nnA = reinterpret_cast<uint16_t*>(stackRef[wv]);
+    uint16_t* nnB = nnA + 16 * nn::H;
+    uint32_t* auditBits = auditWords[wv];
+    uint32_t nnPending = 0;  // learned-cull candidates in the open batch
     uint32_t* queue = queues[wv];
     uint32_t* nearestTri = nearestTris[wv];
     uint64_t* cand = candidates[wv];
@@ -1946,8 +1983,8 @@ __device__ __forceinline__ void collectBody(P p, W w) {
         if (left == 0) {
             if (nb > 0) {
                 wave_sync();
-                flushBatch<NN, D>(p, w, f, task, home, queue, nearestTri, batch, nb, qn, sref, sdist, carry, false,
-                                  lane, nnA, nnB);
+                flushBatch<NN, D>(p, w, home, queue, nearestTri, batch, nb, qn, sref, sdist, carry, false, lane,
+                                  nnPending, auditBits, nnA, nnB);
                 queried += qn;
             }
             break;
@@ -2103,6 +2140,25 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             }
             uniformFrame(f);
             const bool refresh = refreshDist != nullptr;
+            // Learned cull: on for targets whose network admits a cull.  A task with
+            // persistent rows evaluates it only when it re-anchors: the nodes it drops
+            // then keep the network's distance bound, so the temporal skip passes
+            // them over until the next anchor, and the nodes it keeps were within its
+            // reach of the surface, which the motion allowed before the next anchor
+            // barely changes.  A node whose signed distance is known to be at most
+            // nnNear is never dropped (the network's validated error), so it is
+            // queried without evaluating it.
+            bool nnOn = false;
+            float nnNear = 0.0f;
+            if constexpr (NN) {
+                if (f.sj < w.nnTemplates && (!rows || refresh)) {
+                    const float* fb = w.nnF + size_t(f.sj) * nn::FSTRIDE;
+                    nnOn = fb[nn::FMARGIN] < 1e30f;
+                    nnNear = fb[nn::FNEAR];
+                }
+                nnOn = uniformU(nnOn ? 1u : 0u) != 0u;
+                nnNear = uniformF(nnNear);
+            }
             const bool gapTask = gc.row != nullptr && gc.genPrev != 0u;
             const uint32_t triFirst = gapTask ? uniformU(w.triBase[3 * f.sj]) : 0u;
             const uint32_t triCount = gapTask ? uniformU(w.triBase[3 * f.sj + 1]) : 0u;
@@ -2175,6 +2231,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     bool pass_ = false, test = false;
                     uint32_t node = 0;
                     bool far = false;
+                    float nearBound = 1e30f;  // an upper bound of the node's signed distance, when one is known
                     uint32_t slot = 0;
                     v3 local = mk3(0.0f, 0.0f, 0.0f);
                     float4 sp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -2188,6 +2245,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                             const float reach = c0 + c1 * length(xyz(sp)) + SKIP_MARGIN;
                             far = stored > reach;
                             inside = stored < -reach;
+                            if (stored < 0.0f) nearBound = stored + reach;  // stored rounds toward zero
                         }
                         if (inside) {
                             pass_ = true;
@@ -2255,8 +2313,8 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                         wave_sync();
                         if (lane == 0) batch[nb - 1].end = qn | (refresh ? 1u << 16 : 0u);
                         wave_sync();
-                        carry = flushBatch<NN, D>(p, w, f, task, home, queue, nearestTri, batch, nb, qn, sref, sdist,
-                                                  carry, true, lane, nnA, nnB);
+                        carry = flushBatch<NN, D>(p, w, home, queue, nearestTri, batch, nb, qn, sref, sdist, carry,
+                                                  true, lane, nnPending, auditBits, nnA, nnB);
                         queried += qn;
                         qn = 0;
                         pending = 0;
@@ -2265,10 +2323,17 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                         nb = 1;
                         wave_sync();
                     }
+                    bool nnCand = false;
+                    if constexpr (NN) {
+                        if (nnOn) {
+                            nnCand = pass_ && done == 0u && nearBound > nnNear;
+                            nnPending += __popcll(__ballot(nnCand));
+                        }
+                    }
                     if (pass_) {
                         const uint32_t at = qn + __popcll(m & lanes_below(lane));
                         queue[at] = node | (slot << 16);
-                        nearestTri[at] = done != 0u ? done - 1 : PENDING | (nb - 1);
+                        nearestTri[at] = done != 0u ? done - 1 : PENDING | (nnCand ? NN_CAND : 0u) | (nb - 1);
                     }
                     qn += n;
                     pending += __popcll(__ballot(pass_ && done == 0u));
@@ -2280,8 +2345,8 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             // below, so its batch ends with it.
             if (refresh || nb >= batchLimit || pending >= w.batchFill) {
                 wave_sync();
-                flushBatch<NN, D>(p, w, f, task, home, queue, nearestTri, batch, nb, qn, sref, sdist, carry, false,
-                                  lane, nnA, nnB);
+                flushBatch<NN, D>(p, w, home, queue, nearestTri, batch, nb, qn, sref, sdist, carry, false, lane,
+                                  nnPending, auditBits, nnA, nnB);
                 queried += qn;
                 qn = 0;
                 nb = 0;
@@ -2338,7 +2403,8 @@ template <int QN>
 __global__ __launch_bounds__(COLLECT_WAVES * WAVE) __attribute__((amdgpu_waves_per_eu(SAD_COLLECT_WAVES_PER_EU)))
 void k_collect(P p, W w) { collectBody<false, QN>(p, w); }
 template <int QN>
-__global__ __launch_bounds__(COLLECT_WAVES * WAVE) void k_collect_nn(P p, W w) { collectBody<true, QN>(p, w); }
+__global__ __launch_bounds__(COLLECT_WAVES * WAVE) __attribute__((amdgpu_waves_per_eu(SAD_COLLECT_NN_WAVES_PER_EU)))
+void k_collect_nn(P p, W w) { collectBody<true, QN>(p, w); }
 
 // control[2] = total contacts, per-pair counts (solver.slang:397-403 semantics).
 __global__ void k_contact_total(P p, W w) {
@@ -4894,10 +4960,10 @@ int sad_hip_nn_sample(const uint32_t* uniforms, void* const* buffers, const uint
     return e == hipSuccess ? SAD_OK : hip_fail(e, "nn_sample");
 }
 
-int sad_hip_mfma_probe(const void* a, const void* b, float* d, void* stream) {
-    if (!a || !b || !d) return SAD_E_ARGUMENT;
+int sad_hip_mfma_probe(const void* a, const void* b, float* d, int k, void* stream) {
+    if (!a || !b || !d || (k != 16 && k != 32)) return SAD_E_ARGUMENT;
     hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const uint16_t*>(a), static_cast<const uint16_t*>(b), d);
+                       static_cast<const uint16_t*>(a), static_cast<const uint16_t*>(b), d, k == 32 ? 1 : 0);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SAD_OK : hip_fail(e, "mfma_probe");
 }

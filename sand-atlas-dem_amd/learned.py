@@ -47,8 +47,8 @@ OFF_B4 = OFF_W4 + H
 PARAMS = OFF_B4 + 1
 STRIDE = 11008  # FP32 master / gradient / Adam stride per template
 BSTRIDE = 2 * H * H + H * K1  # BF16 weight block per template
-FSTRIDE = 264  # FP32 block: biases, output layer, centre, scale, margin
-FB1, FB2, FB3, FW4, FB4, FCENTER, FSCALE, FMARGIN = 0, 64, 128, 192, 256, 257, 260, 261
+FSTRIDE = 264  # FP32 block: biases, output layer, centre, scale, margin, near bound
+FB1, FB2, FB3, FW4, FB4, FCENTER, FSCALE, FMARGIN, FNEAR, FLABEL = 0, 64, 128, 192, 256, 257, 260, 261, 262, 263
 
 
 def initial_parameters(templates, rng):
@@ -248,6 +248,13 @@ class LearnedCull:
         self.enabled = self.margin < 1.0
         device_margin = np.where(self.enabled, self.margin, np.float32(np.inf)).astype(np.float32)
         self.blocks[:, FMARGIN] = self.torch.from_numpy(device_margin).to(self.dev)
+        # Within its validated error the network predicts at most d / label_scale +
+        # max_error for a node at signed distance d, so it drops none with
+        # d <= (margin - max_error) x label_scale (template units): the collection
+        # queries such nodes without evaluating the network.
+        self.near = np.where(self.enabled, (self.margin - self.max_error) * self.label_scale, 0.0).astype(np.float32)
+        self.blocks[:, FNEAR] = self.torch.from_numpy(self.near).to(self.dev)
+        self.blocks[:, FLABEL] = self.torch.from_numpy(self.label_scale.astype(np.float32)).to(self.dev)
         inside = self.valid_labels <= 0
         would_cull = pred > np.where(self.margin < 1.0, self.margin, np.inf)[:, None]
         self.validation_report = {

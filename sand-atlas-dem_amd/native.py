@@ -119,7 +119,7 @@ def library():
             "sad_hip_nn_pack": (ct.c_int, [vp, vp, vp, u32, vp]),
             "sad_hip_nn_infer": (ct.c_int, [vp, vp, u32, vp, vp, vp, vp]),
             "sad_hip_nn_sample": (ct.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
-            "sad_hip_mfma_probe": (ct.c_int, [vp, vp, vp, vp]),
+            "sad_hip_mfma_probe": (ct.c_int, [vp, vp, vp, ct.c_int, vp]),
             "sad_hip_last_error": (ct.c_char_p, []),
             "sad_hip_abi_version": (ct.c_int, []),
         }

@@ -68,18 +68,19 @@ def test_uneven_graph_steps_match_oracle(uneven, oracle_cpu):
     assert a.state()["contacts"] > 1000
 
 
-@pytest.mark.parametrize("config", ["dense", "uneven"])
-def test_learned_cull_matches_oracle(pkg, oracle_cpu, config):
+@pytest.mark.parametrize("config,skip", [("dense", False), ("uneven", False), ("dense", True), ("uneven", True)])
+def test_learned_cull_matches_oracle(pkg, oracle_cpu, config, skip):
     """Train the per-template networks on the GPU, audit the cull (every culled node
     is still queried exactly: none may be a contact), then run culled and compare
-    with the oracle's unculled run."""
+    with the oracle's unculled run.  Without the temporal skip every queued node
+    meets the network; with it (the benchmark's path) nodes known to lie within the
+    network's reach of the surface are queried without it, and culled nodes'
+    stored bounds stay conservative for the skip."""
     from sand_atlas_dem_amd import assembly, learned
     from sand_atlas_dem_amd.hip import HipBackend
 
     w = assembly.config_workload(config, n=400, templates=8)
-    # The temporal skip would pass over most of the nodes the network culls (both
-    # drop nodes far outside the target); off here, every queued node meets the cull.
-    m = HipBackend(0, temporal_skip=False)
+    m = HipBackend(0, temporal_skip=skip)
     e = w.engine(m)
     e.initialize()
     lc = learned.LearnedCull(e, points=1 << 18, batch=8192, validation=8192, seed=2)
@@ -88,6 +89,10 @@ def test_learned_cull_matches_oracle(pkg, oracle_cpu, config):
     assert lc.validation_report["validation_misses"] == 0
     state, step = e.state(), e.step
     e.restore(state, step)
+    # Under the skip, anchored tasks pass over the far nodes the network would
+    # drop; dropping the anchors makes the first collection re-anchor every task,
+    # so the network meets every queued node there.
+    m.invalidate()
     assert lc.attach(audit=True), "no template admits a cull"
     before = m.stats()
     e.advance(4, rate=w.rate)
@@ -96,12 +101,15 @@ def test_learned_cull_matches_oracle(pkg, oracle_cpu, config):
     assert after["nn_misses"] - before["nn_misses"] == 0
     assert audited > 0
     e.restore(state, step)
+    m.invalidate()
     lc.attach(audit=False)
     before = m.stats()
     e.advance(4, rate=w.rate)
     after = m.stats()
     lc.detach()
     assert after["nn_culled"] - before["nn_culled"] > 0
+    if skip:
+        assert after["skipped_nodes"] > before["skipped_nodes"]
     b = w.engine(oracle_cpu)
     b.restore(state, step)
     b.advance(4, rate=w.rate)

@@ -35,16 +35,18 @@ def _random_master(rng, templates):
     return master
 
 
-def test_mfma_operand_layout(hip):
-    """D = A B for one 16x16x16 BF16 tile, exact on small integers."""
+@pytest.mark.parametrize("k", [16, 32])
+def test_mfma_operand_layout(hip, k):
+    """D = A B for one 16x16xk BF16 tile (k = 16, and gfx950's k = 32), exact on small
+    integers: pins the per-lane operand layouts the network's layers use."""
     torch = hip.torch
     rng = np.random.default_rng(1)
-    A = rng.integers(-8, 9, (16, 16)).astype(np.float32)
-    B = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    A = rng.integers(-8, 9, (16, k)).astype(np.float32)
+    B = rng.integers(-8, 9, (k, 16)).astype(np.float32)
     a = torch.from_numpy(nn.bf16_bits(A).view(np.int16)).to(hip.torch_device)
     b = torch.from_numpy(nn.bf16_bits(B).view(np.int16)).to(hip.torch_device)
     d = torch.zeros((16, 16), dtype=torch.float32, device=hip.torch_device)
-    hip._check(hip.lib.sad_hip_mfma_probe(_p(a), _p(b), _p(d), hip.stream), "mfma probe")
+    hip._check(hip.lib.sad_hip_mfma_probe(_p(a), _p(b), _p(d), k, hip.stream), "mfma probe")
     assert np.array_equal(d.cpu().numpy(), A @ B)
 
 
@@ -189,15 +191,16 @@ def test_learned_cull_gain_on_concave_grains(pkg):
         e.advance(3, rate=w.rate)
         after = m.stats()
         lc.detach()
-        queried = after["queried_nodes"] - before["queried_nodes"]
-        return e.state(), queried, after
+        return e.state(), {k: after[k] - before[k] for k in after}
 
-    plain, queried, _ = run("plain")
-    audit, _, s = run("audit")
-    assert s["nn_audited"] == s["nn_culled"] > 0
-    assert s["nn_misses"] == 0
-    culled, queued, s = run("culled")
-    exact = queued - s["nn_culled"]
+    plain, d = run("plain")
+    queried = d["queried_nodes"]
+    audit, d = run("audit")
+    assert d["nn_audited"] == d["nn_culled"] > 0
+    assert d["nn_misses"] == 0
+    culled, d = run("culled")
+    queued = d["queried_nodes"]
+    exact = queued - d["nn_culled"]  # queued nodes that still took the exact query
     assert queued == queried
     assert exact <= 0.9 * queried, (exact, queried)
     for other in (audit, culled):
