@@ -2921,16 +2921,23 @@ __global__ void k_clear_storage(P p) {
     for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x)
         if (id < p.c.hashSize) p.hash[id] = DEM_EMPTY;
 }
-// Sliding contacts are counted into control[6] once per wave (ballot + one atomic):
-// per-contact atomics on that one word serialise in L2.
-__device__ __forceinline__ void countSliding(const P& p, bool sliding) {
-    const uint64_t hits = __ballot(sliding), active = __ballot(true);
-    if (hits && int(threadIdx.x & 63) == __ffsll((unsigned long long)active) - 1)
-        atomicAdd(p.control + 6, uint32_t(__popcll(hits)));
+// Sliding contacts are counted into control[6] once per workgroup: each lane counts
+// its own contacts, the workgroup sums them in LDS, and one atomic per workgroup
+// adds the total.  Atomics on that one word serialise in L2, so one per wave (tens
+// of thousands per launch once a fifth of the contacts slide) tripled the kernel.
+__device__ __forceinline__ void addSliding(const P& p, uint32_t mine, uint32_t* total) {
+    if (threadIdx.x == 0) *total = 0u;
+    __syncthreads();
+    for (int d = 32; d > 0; d >>= 1) mine += __shfl_xor(mine, d);
+    if ((threadIdx.x & 63) == 0 && mine) atomicAdd(total, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && *total) atomicAdd(p.control + 6, *total);
 }
-__global__ void k_resolve(P p) {
-    if (ld_volatile(p.control) != 0) return;
-    uint32_t count = min(p.control[2], p.c.contactCapacity);
+__global__ __launch_bounds__(256) void k_resolve(P p) {
+    __shared__ uint32_t slidingTotal;
+    // Every lane reaches the workgroup sum below, also after an error flag.
+    const uint32_t count = ld_volatile(p.control) != 0 ? 0u : min(p.control[2], p.c.contactCapacity);
+    uint32_t slidingMine = 0;
     for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < count; id += gridDim.x * blockDim.x) {
         bool slid = false;
         uint4 key = p.meta[id];
@@ -3021,8 +3028,9 @@ __global__ void k_resolve(P p) {
             p.contactForce[id] = cf;
             if (!isfinite(cf.x) || !isfinite(cf.y) || !isfinite(cf.z) || !isfinite(cf.w) || !isfinite(lossPower)) fail(p, 512);
         }
-        countSliding(p, slid);
+        slidingMine += slid ? 1u : 0u;
     }
+    addSliding(p, slidingMine, &slidingTotal);
 }
 __global__ void k_build_hash(P p) {
     if (ld_volatile(p.control) != 0) return;
