@@ -154,6 +154,7 @@ struct W {
     uint32_t batchTasks;  // exact queries: tasks queried together per wave, 1..BATCH_MAX (tunable)
     uint32_t batchFill;   // exact queries: traversals queued that end a batch early (tunable)
     float gapReach;       // exact queries: extra search distance that measures the winner's gap (tunable)
+    uint32_t stageChunk;  // staging slots a wave reserves per atomic (0: the hits of each call)
     // Unchanged-winner shortcut (with the temporal skip and the warm start): per task
     // the frame of its previous collection (qi, qj, (branch, generation), (i, j,
     // epoch, 1)); per node slot (generation << 16) | gap, the gap in GAP_QUANTUM
@@ -1324,18 +1325,33 @@ __device__ bool queryNode(const P& p, const W& w, const TaskFrame& f, uint32_t n
     return true;
 }
 
-// Rank-ordered staging of one batch of hits.
+// Rank-ordered staging of one batch of hits.  A wave reserves its staging slots
+// w.stageChunk at a time (0: exactly the hits of each call), so its queue's counter
+// takes one atomic per chunk instead of one per call: atomics on one word serialise.
+// [resNext, resEnd) is the wave's unused reservation (wave-uniform); the slots it
+// still holds when the wave exits are marked unused (task NO_TASK) for k_scatter.
+constexpr uint32_t NO_TASK = 0xffffffffu;
+constexpr uint32_t STAGE_CHUNK = 256;
 __device__ void stageHits(const P& p, const W& w, const TaskFrame& f, uint32_t task, uint32_t queue, bool hit,
-                          uint32_t node, float4 pd, float4 na, uint32_t& running, int lane) {
+                          uint32_t node, float4 pd, float4 na, uint32_t& running, int lane, uint32_t& resNext,
+                          uint32_t& resEnd) {
     uint64_t mask = __ballot(hit);
     uint32_t count = __popcll(mask);
     if (count == 0) return;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(w.stageCount + queue * LINE, count);
-    base = __shfl(base, 0);
+    const uint32_t avail = resEnd - resNext, first = resNext;  // ranks below avail use the current chunk
+    uint32_t more = 0;
+    if (count > avail) {
+        const uint32_t need = count - avail, take = max(need, w.stageChunk);
+        if (lane == 0) more = atomicAdd(w.stageCount + queue * LINE, take);
+        more = uniformU(__shfl(more, 0));
+        resNext = more + need;
+        resEnd = more + take;
+    } else {
+        resNext += count;
+    }
     if (hit) {
         uint32_t rank = __popcll(mask & lanes_below(lane));
-        uint32_t local = base + rank;
+        uint32_t local = rank < avail ? first + rank : more + (rank - avail);
         size_t pos = size_t(queue) * w.stageCap + local;
         if (local < w.stageCap) {
             w.stageKey[pos] = make_uint2(task, running + rank);
@@ -1558,7 +1574,8 @@ __device__ __forceinline__ uint32_t selectBit(uint64_t m, uint32_t r) {
 template <bool NN, typename D>
 __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, uint32_t nb, uint32_t stageQueue,
                                const uint32_t* queue, uint32_t* nearestTri, uint32_t qn, uint32_t* sref, D* sdist,
-                               uint32_t carry, bool lastOpen, int lane, const uint32_t* auditBits) {
+                               uint32_t carry, bool lastOpen, int lane, const uint32_t* auditBits,
+                               uint32_t& resNext, uint32_t& resEnd) {
     const float4* __restrict__ wide = w.wide;
     const bool gapOn = w.warm && w.nodeGap;
     uint32_t k = 0, top = 0, bestTri = NO_TRIANGLE, slotNow = 0, rowNow = 0, genNow = 0;
@@ -1805,7 +1822,7 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
                     if (missed && lane == 0) atomicAdd(w.nnStats + LINE, (unsigned long long)missed);
                 }
             }
-            stageHits(p, w, f, task, stageQueue, hit, node, pd, na, running, lane);
+            stageHits(p, w, f, task, stageQueue, hit, node, pd, na, running, lane, resNext, resEnd);
         }
         from = to;
         if (s + 1 < nb || !lastOpen) {
@@ -1910,7 +1927,7 @@ template <bool NN, typename D>
 __device__ uint32_t flushBatch(const P& p, const W& w, uint32_t stageQueue, uint32_t* queue, uint32_t* nearestTri,
                                BatchRec* batch, uint32_t nb, uint32_t qn, uint32_t* sref, D* sdist, uint32_t carry,
                                bool lastOpen, int lane, uint32_t& nnPending, uint32_t* auditBits, uint16_t* nnA,
-                               uint16_t* nnB) {
+                               uint16_t* nnB, uint32_t& resNext, uint32_t& resEnd) {
     const uint32_t* audited = nullptr;
     if constexpr (NN) {
         if (nnPending != 0u) {
@@ -1920,7 +1937,7 @@ __device__ uint32_t flushBatch(const P& p, const W& w, uint32_t stageQueue, uint
         nnPending = 0u;
     }
     return queryBatch<NN, D>(p, w, batch, nb, stageQueue, queue, nearestTri, qn, sref, sdist, carry, lastOpen, lane,
-                             audited);
+                             audited, resNext, resEnd);
 }
 
 // One directed pair per wave (cpuCollectMesh, cpu_scheduling.slang:53-91).
@@ -1948,6 +1965,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
     uint16_t* nnB = nnA + 16 * nn::H;
     uint32_t* auditBits = auditWords[wv];
     uint32_t nnPending = 0;  // learned-cull candidates in the open batch
+    uint32_t resNext = 0, resEnd = 0;  // this wave's unused staging reservation
     uint32_t* queue = queues[wv];
     uint32_t* nearestTri = nearestTris[wv];
     uint64_t* cand = candidates[wv];
@@ -1984,7 +2002,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             if (nb > 0) {
                 wave_sync();
                 flushBatch<NN, D>(p, w, home, queue, nearestTri, batch, nb, qn, sref, sdist, carry, false, lane,
-                                  nnPending, auditBits, nnA, nnB);
+                                  nnPending, auditBits, nnA, nnB, resNext, resEnd);
                 queried += qn;
             }
             break;
@@ -2314,7 +2332,8 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                         if (lane == 0) batch[nb - 1].end = qn | (refresh ? 1u << 16 : 0u);
                         wave_sync();
                         carry = flushBatch<NN, D>(p, w, home, queue, nearestTri, batch, nb, qn, sref, sdist, carry,
-                                                  true, lane, nnPending, auditBits, nnA, nnB);
+                                                  true, lane, nnPending, auditBits, nnA, nnB, resNext,
+                                                  resEnd);
                         queried += qn;
                         qn = 0;
                         pending = 0;
@@ -2346,7 +2365,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             if (refresh || nb >= batchLimit || pending >= w.batchFill) {
                 wave_sync();
                 flushBatch<NN, D>(p, w, home, queue, nearestTri, batch, nb, qn, sref, sdist, carry, false, lane,
-                                  nnPending, auditBits, nnA, nnB);
+                                  nnPending, auditBits, nnA, nnB, resNext, resEnd);
                 queried += qn;
                 qn = 0;
                 nb = 0;
@@ -2392,6 +2411,8 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             w.taskCount[task] = 0u;
         }
     }
+    for (uint32_t k = resNext + lane; k < min(resEnd, w.stageCap); k += WAVE)  // reserved, never used
+        w.stageKey[size_t(home) * w.stageCap + k] = make_uint2(NO_TASK, 0u);
     if (lane == 0 && queried) atomicAdd(w.counters + 2 * LINE, queried);
     if (lane == 0 && skippedNodes) atomicAdd(w.counters + 11 * LINE, skippedNodes);
     if (lane == 0 && shortcutNodes) atomicAdd(w.counters + 12 * LINE, shortcutNodes);
@@ -2436,6 +2457,7 @@ __global__ void k_scatter(P p, W w) {
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < staged; r += gridDim.x * blockDim.x) {
         size_t k = size_t(queue) * w.stageCap + r;
         uint2 key = w.stageKey[k];
+        if (key.x == NO_TASK) continue;  // a slot its wave reserved but did not use
         uint32_t id = w.taskOffset[key.x] + key.y;
         if (id < p.c.contactCapacity) {
             p.meta[id] = w.stageMeta[k];
@@ -4113,6 +4135,14 @@ int launch_scan(Scratch& sc, const ScanArgs& a, uint32_t cap, hipStream_t s) {
     return SAD_OK;
 }
 
+// Staging slots a collection wave reserves per atomic.  Reserved-but-unused slots
+// are at most STAGE_CHUNK - 1 per wave: chunks only where that leaves the queues'
+// capacity ample.
+uint32_t stage_chunk(const Cfg& c, const ScratchImpl& sc) {
+    const uint64_t waves = uint64_t(std::max(1, sc.cus)) * sc.blocksPerCU * COLLECT_WAVES;
+    return uint64_t(c.contactCapacity) >= 2 * waves * STAGE_CHUNK ? STAGE_CHUNK : 0u;
+}
+
 // Reference kernels by name (gpu.py:547-565 / solver.slang entry points).
 int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
     const Cfg& c = p.c;
@@ -4175,6 +4205,7 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
             LAUNCH(sc, KID_rows, k_rows_assign, gridStride, T, s, p, w);
         }
         const dim3 persistent(std::max(1, sc.cus) * sc.blocksPerCU);
+        w.stageChunk = stage_chunk(c, sc);
         w.pop = sc.pop;
         w.leafBatch = sc.leafBatch;
         w.batchTasks = sc.batchTasks;
@@ -4473,6 +4504,7 @@ int sad_hip_advance(uint32_t steps, uint32_t active, void* const* buffers, const
             if ((rc = sc->ensure_sdf(p, s))) return rc;  // no allocation inside a captured step
             if ((rc = sc->sync_epoch(s))) return rc;
             if (graph && !sc->profiling) {
+                sc->w.stageChunk = stage_chunk(p.c, *sc);
                 sc->w.pop = sc->pop;
                 sc->w.leafBatch = sc->leafBatch;
                 sc->w.batchTasks = sc->batchTasks;
