@@ -217,6 +217,32 @@ constexpr int POP = SAD_POP;          // default tasks taken per queue atomic
 
 __device__ __forceinline__ void fail(const P& p, uint32_t bit) { atomicOr(p.control, bit); }
 __device__ __forceinline__ uint32_t ld_volatile(const uint32_t* a) { return __atomic_load_n(a, __ATOMIC_RELAXED); }
+// Streaming stores and loads (the non-temporal hint): data written once and read once
+// by a later kernel, kept from displacing the reused template geometry in L2.
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t nt_u4 __attribute__((ext_vector_type(4)));
+typedef uint32_t nt_u2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_stream(float4* a, float4 v) {
+    __builtin_nontemporal_store(nt_f4{v.x, v.y, v.z, v.w}, reinterpret_cast<nt_f4*>(a));
+}
+__device__ __forceinline__ void st_stream(uint4* a, uint4 v) {
+    __builtin_nontemporal_store(nt_u4{v.x, v.y, v.z, v.w}, reinterpret_cast<nt_u4*>(a));
+}
+__device__ __forceinline__ void st_stream(uint2* a, uint2 v) {
+    __builtin_nontemporal_store(nt_u2{v.x, v.y}, reinterpret_cast<nt_u2*>(a));
+}
+__device__ __forceinline__ float4 ld_stream(const float4* a) {
+    const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(a));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 ld_stream(const uint4* a) {
+    const nt_u4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_u4*>(a));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 ld_stream(const uint2* a) {
+    const nt_u2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_u2*>(a));
+    return make_uint2(v.x, v.y);
+}
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1354,10 +1380,10 @@ __device__ void stageHits(const P& p, const W& w, const TaskFrame& f, uint32_t t
         uint32_t local = rank < avail ? first + rank : more + (rank - avail);
         size_t pos = size_t(queue) * w.stageCap + local;
         if (local < w.stageCap) {
-            w.stageKey[pos] = make_uint2(task, running + rank);
-            w.stageMeta[pos] = make_uint4(f.i, f.j, node, 0);
-            w.stagePd[pos] = pd;
-            w.stageNa[pos] = na;
+            st_stream(w.stageKey + pos, make_uint2(task, running + rank));
+            st_stream(w.stageMeta + pos, make_uint4(f.i, f.j, node, 0));
+            st_stream(w.stagePd + pos, pd);
+            st_stream(w.stageNa + pos, na);
         } else {
             fail(p, 32);
         }
@@ -2456,13 +2482,13 @@ __global__ void k_scatter(P p, W w) {
     uint32_t staged = min(w.stageCount[queue * LINE], w.stageCap);
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < staged; r += gridDim.x * blockDim.x) {
         size_t k = size_t(queue) * w.stageCap + r;
-        uint2 key = w.stageKey[k];
+        uint2 key = ld_stream(w.stageKey + k);
         if (key.x == NO_TASK) continue;  // a slot its wave reserved but did not use
         uint32_t id = w.taskOffset[key.x] + key.y;
         if (id < p.c.contactCapacity) {
-            p.meta[id] = w.stageMeta[k];
-            p.pointDepth[id] = w.stagePd[k];
-            p.normalArea[id] = w.stageNa[k];
+            p.meta[id] = ld_stream(w.stageMeta + k);
+            p.pointDepth[id] = ld_stream(w.stagePd + k);
+            p.normalArea[id] = ld_stream(w.stageNa + k);
         }
     }
 }
