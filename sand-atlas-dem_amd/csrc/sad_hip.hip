@@ -1700,10 +1700,15 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
                 const float popped = dload(sdist + top * WAVE);
                 if (!(popped > bound)) {
                     if ((ref & LEAF) == 0) {
+                        // The node's rows, and the next node's: in the depth-first layout
+                        // the left child is node ref + 1, so its rows (unused when it is a
+                        // leaf) are read in the same round trip.
                         const float4* node = wide + 4 * size_t(ref);
                         float4 w0 = node[0], w1 = node[1], w2 = node[2], w3 = node[3];
+                        const float4 v0 = node[4], v1 = node[5], v2 = node[6], v3 = node[7];
                         float da = boxDistance(xyz(w0), xyz(w1), q), db = boxDistance(xyz(w2), xyz(w3), q);
                         uint32_t a = __float_as_uint(w0.w), b = __float_as_uint(w1.w);
+                        const bool leftNear = da < db;
                         if (da < db) { uint32_t t = a; a = b; b = t; float u = da; da = db; db = u; }
                         if (top + 2 > WSTACK) {
                             fail(p, 1024);
@@ -1712,8 +1717,34 @@ __device__ uint32_t queryBatch(const P& p, const W& w, const BatchRec* batch, ui
                         } else {
                             if (da <= bound) { sref[top * WAVE] = a; dstore(sdist + top * WAVE, da); top++; }
                             else second = fminf(second, da);
-                            if (db <= bound) { sref[top * WAVE] = b; dstore(sdist + top * WAVE, db); top++; }
-                            else second = fminf(second, db);
+                            if (!(db <= bound)) {
+                                second = fminf(second, db);
+                            } else if ((b & LEAF) != 0) {
+                                // The near child is popped next and passes the same bound
+                                // (no triangle is tested in between): hold its leaf now.
+                                hasLeaf = true;
+                                leafRef = b;
+                            } else if (leftNear) {
+                                // The near child is the left one, whose rows are in hand:
+                                // pop it in this pass, exactly as the next pass would.
+                                float ea = boxDistance(xyz(v0), xyz(v1), q), eb = boxDistance(xyz(v2), xyz(v3), q);
+                                uint32_t c = __float_as_uint(v0.w), d = __float_as_uint(v1.w);
+                                if (ea < eb) { uint32_t t = c; c = d; d = t; float u = ea; ea = eb; eb = u; }
+                                if (top + 2 > WSTACK) {
+                                    fail(p, 1024);
+                                    top = 0;
+                                    bestTri = NO_TRIANGLE;
+                                } else {
+                                    if (ea <= bound) { sref[top * WAVE] = c; dstore(sdist + top * WAVE, ea); top++; }
+                                    else second = fminf(second, ea);
+                                    if (eb <= bound) { sref[top * WAVE] = d; dstore(sdist + top * WAVE, eb); top++; }
+                                    else second = fminf(second, eb);
+                                }
+                            } else {
+                                sref[top * WAVE] = b;
+                                dstore(sdist + top * WAVE, db);
+                                top++;
+                            }
                         }
                     } else {
                         hasLeaf = true;
@@ -2056,11 +2087,13 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             //   (2 |dq_j| (|a| + |b|) + 2 |dq_i| |a| + |db|) / s_j = c0 + c1 |p|.
             // The exact signed distance is 1-Lipschitz in x.  A node whose anchor
             // distance exceeds that bound (plus SKIP_MARGIN) is still outside the
-            // target: not a contact, passed over.  One whose anchor distance is below
-            // minus the bound is still inside the target, at least that far from the
-            // hull (which contains the mesh), so the rejection plane and hull tests
-            // would pass it: it is queued without them.  Either way the queue holds
-            // every contact in walk order and the hits and their ranks are unchanged.
+            // target: not a contact, passed over.  One whose anchor distance is
+            // negative was inside the target: it is queued without the rejection plane
+            // and hull tests (below minus the bound it is still inside, so they would
+            // pass it; otherwise they are only filters, and a node they would have
+            // rejected costs one exact query that finds it outside).  Either way the
+            // queue holds every contact in walk order and the hits and their ranks are
+            // unchanged.
             // A task without a valid anchor, or whose bound has grown past
             // skipLimit, tests and queries every node and re-anchors; the rejection
             // plane of the anchor (any hull plane is a valid filter) is reused.
@@ -2221,7 +2254,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             uint32_t leafSkipped = 0;  // per lane, summed below
             for (uint32_t base = 0; base < info.y; base += WAVE) {
                 const uint32_t c = base + lane;
-                bool test = false;
+                bool test = false, nearLeaf = false;
                 if (c < info.y) {
                     const uint4 range = p.meshClusterRanges[info.x + c];
                     if (range.y > 0) {
@@ -2231,11 +2264,14 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                             const float radius = length(fmax3(fabs3(lo), fabs3(hi)));
                             farLeaf = float(leafDist[c]) * SKIP_QUANTUM > c0 + c1 * radius + SKIP_MARGIN;
                             if (farLeaf) leafSkipped += range.y;
+                            // A leaf with a node inside the target at the anchor is a
+                            // candidate without its cull test (the test only filters).
+                            nearLeaf = !farLeaf && leafDist[c] < 0;
                         }
-                        test = !farLeaf;
+                        test = !farLeaf && !nearLeaf;
                     }
                 }
-                uint64_t tests = __ballot(test), m = 0;
+                uint64_t tests = __ballot(test), m = __ballot(nearLeaf);
                 if (uint32_t(__popcll(tests)) > COOP_LEAVES) {
                     bool ok = false;
                     if (test) {
@@ -2243,7 +2279,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                         ok = !(out > 0.0f);
                         if (refreshDist && !ok) leafDist[c] = quantizeDistance(out);
                     }
-                    m = __ballot(ok);
+                    m |= __ballot(ok);
                 } else {
                     while (tests != 0) {
                         const uint32_t rel = base + uint32_t(__builtin_ctzll(tests));
@@ -2288,7 +2324,10 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                             const float stored = float(dist[slot]) * SKIP_QUANTUM;
                             const float reach = c0 + c1 * length(xyz(sp)) + SKIP_MARGIN;
                             far = stored > reach;
-                            inside = stored < -reach;
+                            // Inside the target at the anchor: queued without the plane
+                            // and hull tests, which only filter (a node that has since
+                            // left the hull costs one exact query, never a contact).
+                            inside = stored < 0.0f;
                             if (stored < 0.0f) nearBound = stored + reach;  // stored rounds toward zero
                         }
                         if (inside) {
