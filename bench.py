@@ -150,6 +150,38 @@ def nn_leg(args, w, e, hip, start, step0, plain_state, log_):
     return out
 
 
+def servo_leg(args, w, e, state, step, log_):
+    """Config 5's loading on the layer: a triaxial stage holding the x-y stress of the
+    layer at 20 kPa with the reference's servo (gain 1,000, filter time 10 us, rates up
+    to 1,000/s as in its drained-triaxial case) while the top plane closes at 0.5 layer
+    heights/s; diagnostics every 50 steps (the reference default).  Timed from the plain
+    run's start state over the same step count, diagnostics and servo included."""
+    import numpy as np
+    import torch
+    from sand_atlas_dem_amd.servo import Stages
+
+    stage = dict(kind="triaxial", steps=10 ** 9, pressure=2e4, gain=1000.0, filter_time=1e-5, max_rate=1000.0,
+                 rate=0.5, stop_strain=0.1)
+    interval = 50
+    e.restore(state, step)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run = Stages(e, [stage], interval=interval)
+    run.run(max_chunks=max(1, args.steps // interval))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    steps = e.step - step
+    last = run.rows[-1]
+    out = {"pressure_pa": 2e4, "interval": interval, "steps": steps, "ms_per_step": 1e3 * elapsed / steps,
+           "value": e.n * steps / elapsed,
+           "last_rate": last["rate"], "layer_stress_diag_pa": np.diag(last["layer_stress"]).tolist(),
+           "layer_axial_strain": last["layer_axial_strain"],
+           "wall_reactions_N": [x["reaction"] for x in last["walls"]], "wall_work_J": last["wall_work"]}
+    log_(f"[bench] layer servo: {out['ms_per_step']:.3f} ms/step with diagnostics every {interval} steps, "
+         f"rates {last['rate']}, layer stress {out['layer_stress_diag_pa']}")
+    return out
+
+
 def _profile_record(kernel, config, kind):
     names = {"dense": "dense10k", "uneven": "uneven50k"}
     path = ROOT / "profiles" / f"{kernel}_{names.get(config, config)}_{kind}.json"
@@ -399,6 +431,8 @@ def measure(args, config, log_, *, primary):
     if nn and nn.get("value"):
         nn["speedup_vs_plain"] = nn["value"] / value
     block["nn_culled"] = nn
+    if config == "layer":
+        block["servo"] = servo_leg(args, w, e, start, step0, log_)
     if e.wall_count:  # wall reactions and work after the timed run (SI units)
         e.restore(plain_state, step0 + args.steps)
         block["walls"] = [{"normal": x["normal"], "offset_m": x["offset"], "reaction_N": list(x["reaction"]),

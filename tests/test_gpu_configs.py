@@ -3,8 +3,8 @@ against the oracle on identical seeded inputs (needs an MI355X).
 
 * batched uneven meshes (320-5,120 triangles, 256-4,096 nodes per template) through
   the persistent work queues and hipGraph-captured steps;
-* the learned cull on the dense and uneven workloads: audited (no culled contact),
-  then the culled run equal to the oracle's unculled run;
+* the learned cull on the dense, uneven and layer (planes across z) workloads: audited
+  (no culled contact), then the culled run equal to the oracle's unculled run;
 * the reference Engine's kernel-by-kernel dispatch (no native step encoding), mesh
   and SDF geometry;
 * neighbour-list rebuilds while the temporal skip and warm-started queries are on;
@@ -68,7 +68,8 @@ def test_uneven_graph_steps_match_oracle(uneven, oracle_cpu):
     assert a.state()["contacts"] > 1000
 
 
-@pytest.mark.parametrize("config,skip", [("dense", False), ("uneven", False), ("dense", True), ("uneven", True)])
+@pytest.mark.parametrize("config,skip", [("dense", False), ("uneven", False), ("dense", True), ("uneven", True),
+                                         ("layer", True)])
 def test_learned_cull_matches_oracle(pkg, oracle_cpu, config, skip):
     """Train the per-template networks on the GPU, audit the cull (every culled node
     is still queried exactly: none may be a contact), then run culled and compare

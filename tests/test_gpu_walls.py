@@ -159,3 +159,32 @@ def test_layer_with_per_axis_periodicity_matches_oracle(hip, oracle_cpu):
         a.advance(2, rate=rate)
         b.advance(2, rate=rate)
         assert_states_equal(a.state(), b.state())
+
+
+def test_layer_servo_matches_oracle(hip, oracle_cpu):
+    """Config 5's loading on the layer: a triaxial stage holding the x-y stress at the
+    confining pressure (the reference's servo, simulation.py:138-161) while the top plane
+    closes.  The HIP run and the oracle's take the same rates from the same diagnostics
+    rows and reach the same state."""
+    from sand_atlas_dem_amd import assembly
+    from sand_atlas_dem_amd.servo import Stages
+
+    w = assembly.config_workload("layer", n=300, templates=6, nodes=256)
+    planes = [dict(p, offset=p["offset"] + 0.04 * 3e-4) for p in w.spec.planes]
+    w = replace(w, spec=replace(w.spec, planes=planes))
+    stage = dict(kind="triaxial", steps=1000, pressure=2e4, gain=1000.0, filter_time=1e-5, max_rate=1000.0,
+                 rate=0.5, stop_strain=0.1)
+    runs = []
+    for backend in (hip, oracle_cpu):
+        e = w.engine(backend)
+        e.initialize()
+        run = Stages(e, [stage], interval=3)
+        assert run.run(max_chunks=4) == "paused"
+        runs.append((e, run.rows))
+    (a, rows_a), (b, rows_b) = runs
+    for ra, rb in zip(rows_a, rows_b):
+        assert ra.get("rate") == rb.get("rate")
+        assert ra["stress"] == rb["stress"] and ra["layer_stress"] == rb["layer_stress"]
+        assert ra["walls"] == rb["walls"]
+    assert all(r["rate"][2] == 0.0 and r["rate"][0] != 0.0 for r in rows_a[1:])
+    assert_states_equal(a.state(), b.state())
