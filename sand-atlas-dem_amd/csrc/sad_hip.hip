@@ -1464,7 +1464,7 @@ __device__ __forceinline__ int8_t quantizeDistance(float d) {
 // root box -- is the same bit for bit.  A failing test returns the largest plane
 // excess instead of the first, an equally valid bound on the distance.
 constexpr uint32_t COOP_LEAVES = 8;  // at most this many leaves per 64-leaf block: cooperative
-constexpr uint32_t COOP_NODES = 8;   // at most this many hull-tested nodes per leaf: cooperative
+constexpr uint32_t COOP_NODES = 16;  // at most this many hull-tested nodes per leaf: cooperative
 __device__ __forceinline__ float waveMax(float v) {
     for (int d = 1; d < WAVE; d <<= 1) v = fmaxf(v, __shfl_xor(v, d));
     return v;
@@ -1517,6 +1517,169 @@ __device__ float meshOutsideWave(const P& p, uint32_t shape, v3 q, int lane) {
         worst = fmaxf(worst, dot(xyz(pl), q) + pl.w);
     }
     return waveMax(worst);
+}
+// The target template's root box and hull-plane groups, written to the wave's LDS
+// record once per task: a hull test then reads only its group's planes from global
+// memory instead of a chain of root, box and group loads.  The group's (first,
+// count) is indexed by the per-lane group.  mid() and ext() are the one-lane
+// form's 0.5 (low + high) and high - low (same bits).
+struct HullCtx {
+    v3 low, high;
+    uint2 range[6];
+    __device__ __forceinline__ v3 mid() const { return 0.5f * (low + high); }
+    __device__ __forceinline__ v3 ext() const { return high - low; }
+};
+__device__ __forceinline__ void hullCtx(HullCtx* h, const P& p, uint32_t shape, int lane) {
+    if (lane < 6) {
+        const uint4 info = p.meshHullInfo[6 * shape + lane];
+        h->range[lane] = make_uint2(info.x, info.y);
+    } else if (lane == 6) {
+        const uint32_t root = p.meshRoots[shape].x;
+        h->low = xyz(p.meshMin[root]);
+        h->high = xyz(p.meshMax[root]);
+    }
+    wave_sync();
+}
+__device__ __forceinline__ bool outsideRoot(const HullCtx& h, v3 q) {
+    return q.x < h.low.x || q.y < h.low.y || q.z < h.low.z || q.x > h.high.x || q.y > h.high.y || q.z > h.high.z;
+}
+__device__ __forceinline__ float rootExcess(const HullCtx& h, v3 q) {
+    return fmaxf(fmaxf(fmaxf(h.low.x - q.x, q.x - h.high.x), fmaxf(h.low.y - q.y, q.y - h.high.y)),
+                 fmaxf(h.low.z - q.z, q.z - h.high.z));
+}
+// meshOutsideFast with the task's hull context.
+__device__ __forceinline__ float nodeOutside(const P& p, const HullCtx& h, v3 q) {
+    if (outsideRoot(h, q)) return rootExcess(h, q);
+    const uint2 r = h.range[dominantGroup((q - h.mid()) / h.ext())];
+    return r.y == 0 ? 0.0f : planesExcess(p.meshPlanes, r.x, r.y, q);
+}
+// Hull tests of the lanes in `tests` (at most 16) together: the tested points are
+// spread over groups of 64 / 2^ceil(log2 n) lanes, each group evaluates its point's
+// planes (or root-box excess) and reduces the largest, and every tested lane gets
+// its own point's value back.  The verdict (no plane value above zero, inside the
+// root box) is the one-lane form's bit for bit; a failing point reports its largest
+// excess rather than the first batch's, an equally valid distance bound.
+__device__ __forceinline__ float nodesOutsideWave(const P& p, const HullCtx& h, uint64_t tests, v3 q, int lane) {
+    const uint32_t n = uint32_t(__popcll(tests));
+    const uint32_t shift = n <= 1 ? 6u : n <= 2 ? 5u : n <= 4 ? 4u : n <= 8 ? 3u : 2u;
+    const uint32_t span = 1u << shift, group = uint32_t(lane) >> shift, sub = uint32_t(lane) & (span - 1u);
+    int src = -1;
+    uint64_t bits = tests;
+    for (uint32_t k = 0; bits != 0; k++) {
+        const int s = __builtin_ctzll(bits);
+        bits &= bits - 1;
+        if (group == k) src = s;
+    }
+    const int from = src < 0 ? lane : src;
+    const v3 x = mk3(__shfl(q.x, from), __shfl(q.y, from), __shfl(q.z, from));
+    float worst = 0.0f;
+    if (src >= 0) {
+        if (outsideRoot(h, x)) {
+            worst = rootExcess(h, x);
+        } else {
+            const uint2 r = h.range[dominantGroup((x - h.mid()) / h.ext())];
+            for (uint32_t k = sub; k < r.y; k += span) {
+                const float4 pl = p.meshPlanes[r.x + k];
+                worst = fmaxf(worst, dot(xyz(pl), x) + pl.w);
+            }
+        }
+    }
+    for (uint32_t d = 1; d < span; d <<= 1) worst = fmaxf(worst, __shfl_xor(worst, int(d)));
+    const uint32_t rank = uint32_t(__popcll(tests & lanes_below(lane)));
+    return __shfl(worst, int(rank << shift));
+}
+// The transformed, padded box of a source cluster in the target frame
+// (meshClusterOutsideFast's corner loop, same expressions).
+__device__ __forceinline__ void clusterBox(v3 sl, v3 sh, v4 qs, v4 qd, float ss, float sd, v3 branch, v3& low,
+                                           v3& high) {
+    low = {1e30f, 1e30f, 1e30f};
+    high = {-1e30f, -1e30f, -1e30f};
+    for (uint32_t corner = 0; corner < 8; corner++) {
+        v3 q = {(corner & 1) ? sh.x : sl.x, (corner & 2) ? sh.y : sl.y, (corner & 4) ? sh.z : sl.z};
+        q = qinv(qd, qrot(qs, q * ss) - branch) / sd;
+        low = fmin3(low, q);
+        high = fmax3(high, q);
+    }
+}
+__device__ __forceinline__ bool boxOutsideRoot(const HullCtx& h, v3 low, v3 high) {
+    return high.x < h.low.x || high.y < h.low.y || high.z < h.low.z || low.x > h.high.x || low.y > h.high.y ||
+           low.z > h.high.z;
+}
+__device__ __forceinline__ float boxRootExcess(const HullCtx& h, v3 low, v3 high) {
+    return fmaxf(fmaxf(fmaxf(h.low.x - high.x, low.x - h.high.x), fmaxf(h.low.y - high.y, low.y - h.high.y)),
+                 fmaxf(h.low.z - high.z, low.z - h.high.z));
+}
+__device__ __forceinline__ float clusterPad(v3 low, v3 high, v3 branch, float sd) {
+    return 64.0f * DEM_EPS * (1.0f + maxabs(low) + maxabs(high) + length(branch) / sd);
+}
+// meshClusterOutsideFast with the task's hull context.
+__device__ __forceinline__ float clusterOutside(const P& p, const HullCtx& h, v4 qs, v4 qd, float ss, float sd, v3 branch,
+                                uint32_t cluster) {
+    v3 low, high;
+    clusterBox(xyz(p.meshClusterMin[cluster]), xyz(p.meshClusterMax[cluster]), qs, qd, ss, sd, branch, low, high);
+    const float pad = clusterPad(low, high, branch, sd);
+    low = low - mk3(pad, pad, pad);
+    high = high + mk3(pad, pad, pad);
+    if (boxOutsideRoot(h, low, high)) return boxRootExcess(h, low, high);
+    const v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
+    const uint2 r = h.range[dominantGroup((center - h.mid()) / h.ext())];
+    for (uint32_t k = 0; k < r.y; k += PLANE_BATCH) {
+        float4 pl[PLANE_BATCH];
+#pragma unroll
+        for (uint32_t j = 0; j < PLANE_BATCH; j++) pl[j] = p.meshPlanes[r.x + min(k + j, r.y - 1)];
+        float worst = 0.0f;
+#pragma unroll
+        for (uint32_t j = 0; j < PLANE_BATCH; j++)
+            worst = fmaxf(worst, dot(xyz(pl[j]), center) + pl[j].w - dot(fabs3(xyz(pl[j])), half));
+        if (worst > 0) return worst;
+    }
+    return 0.0f;
+}
+// Leaf tests of the lanes in `tests` (at most 8) together, as nodesOutsideWave: each
+// leaf gets 64 / 2^ceil(log2 n) lanes, eight of which transform its box corners
+// (reduced as meshClusterOutsideWave does) before the group tests its planes.
+// Lane c of `tests` is cluster clusterBase + c.
+__device__ __forceinline__ float leavesOutsideWave(const P& p, const HullCtx& h, uint64_t tests, uint32_t clusterBase, v4 qs,
+                                   v4 qd, float ss, float sd, v3 branch, int lane) {
+    const uint32_t n = uint32_t(__popcll(tests));
+    const uint32_t shift = n <= 1 ? 6u : n <= 2 ? 5u : n <= 4 ? 4u : 3u;
+    const uint32_t span = 1u << shift, group = uint32_t(lane) >> shift, sub = uint32_t(lane) & (span - 1u);
+    int src = -1;
+    uint64_t bits = tests;
+    for (uint32_t k = 0; bits != 0; k++) {
+        const int s = __builtin_ctzll(bits);
+        bits &= bits - 1;
+        if (group == k) src = s;
+    }
+    const uint32_t cluster = clusterBase + uint32_t(src < 0 ? 0 : src);
+    const v3 sl = xyz(p.meshClusterMin[cluster]), sh = xyz(p.meshClusterMax[cluster]);
+    const uint32_t corner = sub & 7u;
+    v3 q = {(corner & 1) ? sh.x : sl.x, (corner & 2) ? sh.y : sl.y, (corner & 4) ? sh.z : sl.z};
+    q = qinv(qd, qrot(qs, q * ss) - branch) / sd;
+    v3 low = fmin3(mk3(1e30f, 1e30f, 1e30f), q), high = fmax3(mk3(-1e30f, -1e30f, -1e30f), q);
+    for (int d = 1; d < 8; d <<= 1) {
+        low = fmin3(low, mk3(__shfl_xor(low.x, d), __shfl_xor(low.y, d), __shfl_xor(low.z, d)));
+        high = fmax3(high, mk3(__shfl_xor(high.x, d), __shfl_xor(high.y, d), __shfl_xor(high.z, d)));
+    }
+    const float pad = clusterPad(low, high, branch, sd);
+    low = low - mk3(pad, pad, pad);
+    high = high + mk3(pad, pad, pad);
+    float worst = 0.0f;
+    if (src >= 0) {
+        if (boxOutsideRoot(h, low, high)) {
+            worst = boxRootExcess(h, low, high);
+        } else {
+            const v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
+            const uint2 r = h.range[dominantGroup((center - h.mid()) / h.ext())];
+            for (uint32_t k = sub; k < r.y; k += span) {
+                const float4 pl = p.meshPlanes[r.x + k];
+                worst = fmaxf(worst, dot(xyz(pl), center) + pl.w - dot(fabs3(xyz(pl)), half));
+            }
+        }
+    }
+    for (uint32_t d = 1; d < span; d <<= 1) worst = fmaxf(worst, __shfl_xor(worst, int(d)));
+    const uint32_t rank = uint32_t(__popcll(tests & lanes_below(lane)));
+    return __shfl(worst, int(rank << shift));
 }
 // A wave's query batch.  Under the temporal skip a task queues about 10-16 nodes,
 // so querying one task at a time would leave most of the wave's 64 lanes idle.
@@ -2015,6 +2178,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
     constexpr uint32_t batchCap = QN >= 256 ? BATCH_MAX : 8u;
     __shared__ BatchRec batches[COLLECT_WAVES][batchCap];
     __shared__ uint32_t auditWords[COLLECT_WAVES][NN ? QN / 32 : 1];  // learned-cull audit marks per entry
+    __shared__ HullCtx hulls[COLLECT_WAVES];                          // the task's target root box and hull groups
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t* sref = &stackRef[wv][lane];
     D* sdist = &stackDist[wv][lane];
@@ -2216,6 +2380,8 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 f.plane = i0 == 0xffffffffu ? make_float4(0, 0, 0, 0) : p.meshPlanes[i0];
             }
             uniformFrame(f);
+            hullCtx(&hulls[wv], p, f.sj, lane);
+            const HullCtx& hc = hulls[wv];
             const bool refresh = refreshDist != nullptr;
             // Learned cull: on for targets whose network admits a cull.  A task with
             // persistent rows evaluates it only when it re-anchors: the nodes it drops
@@ -2271,27 +2437,17 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                         test = !farLeaf && !nearLeaf;
                     }
                 }
-                uint64_t tests = __ballot(test), m = __ballot(nearLeaf);
-                if (uint32_t(__popcll(tests)) > COOP_LEAVES) {
-                    bool ok = false;
-                    if (test) {
-                        const float out = meshClusterOutsideFast(p, f.qi, f.qj, f.s_i, f.s_j, f.branch, f.sj, info.x + c);
-                        ok = !(out > 0.0f);
-                        if (refreshDist && !ok) leafDist[c] = quantizeDistance(out);
-                    }
+                const uint64_t tests = __ballot(test);
+                uint64_t m = __ballot(nearLeaf);
+                if (tests != 0) {
+                    float out = 0.0f;
+                    if (uint32_t(__popcll(tests)) <= COOP_LEAVES)
+                        out = leavesOutsideWave(p, hc, tests, info.x + base, f.qi, f.qj, f.s_i, f.s_j, f.branch, lane);
+                    else if (test)
+                        out = clusterOutside(p, hc, f.qi, f.qj, f.s_i, f.s_j, f.branch, info.x + c);
+                    const bool ok = test && !(out > 0.0f);
+                    if (test && !ok && refreshDist) leafDist[c] = quantizeDistance(out);
                     m |= __ballot(ok);
-                } else {
-                    while (tests != 0) {
-                        const uint32_t rel = base + uint32_t(__builtin_ctzll(tests));
-                        tests &= tests - 1;
-                        const float out =
-                            meshClusterOutsideWave(p, f.qi, f.qj, f.s_i, f.s_j, f.branch, f.sj, info.x + rel, lane);
-                        if (!(out > 0.0f)) {
-                            m |= 1ull << (rel - base);
-                        } else if (refreshDist && lane == 0) {
-                            leafDist[rel] = quantizeDistance(out);
-                        }
-                    }
                 }
                 if (lane == 0) cand[base / WAVE] = m;
             }
@@ -2346,23 +2502,16 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                             }
                         }
                     }
-                    uint64_t tests = __ballot(test);
-                    if (uint32_t(__popcll(tests)) > COOP_NODES) {
+                    const uint64_t tests = __ballot(test);
+                    if (tests != 0) {
+                        float outside = 0.0f;
+                        if (uint32_t(__popcll(tests)) <= COOP_NODES)
+                            outside = nodesOutsideWave(p, hc, tests, local, lane);
+                        else if (test)
+                            outside = nodeOutside(p, hc, local);
                         if (test) {
-                            const float outside = meshOutsideFast(p, f.sj, local);
                             pass_ = !(outside > 0.0f);
                             if (refreshDist && !pass_) refreshDist[slot] = quantizeDistance(outside);
-                        }
-                    } else {
-                        while (tests != 0) {
-                            const int src = __builtin_ctzll(tests);
-                            tests &= tests - 1;
-                            const v3 q = mk3(broadcastF(local.x, src), broadcastF(local.y, src), broadcastF(local.z, src));
-                            const float outside = meshOutsideWave(p, f.sj, q, lane);
-                            if (lane == src) {
-                                pass_ = !(outside > 0.0f);
-                                if (refreshDist && !pass_) refreshDist[slot] = quantizeDistance(outside);
-                            }
                         }
                     }
                     if (skipMode) skipped += __popcll(__ballot(far));
