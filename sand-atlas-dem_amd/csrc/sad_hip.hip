@@ -46,6 +46,7 @@ struct P {
     uint32_t planes;     // rows of meshPlanes
     uint32_t triangles;  // meshTriangles rows / 3
     uint32_t sampleKeys; // rows of meshSampleIndices
+    uint32_t clusters;   // rows of meshClusterRanges
     uint32_t* control;
     float4 *cell, *position, *positionRemainder, *velocityRemainder, *momentumRemainder;
     float4 *rotationRemainder, *referencePositionRemainder, *cellLog, *accumulatedRemainder;
@@ -114,6 +115,8 @@ struct W {
     // are contiguous rows.
     float4* triPacked;    // 3 rows per triangle
     float4* sampleKeyed;  // 1 row per sample key
+    uint4* clusterPacked; // per sample-tree cluster: first key, node count, |box corner| bound (float bits),
+                          // first key less the template's first key (the node's row slot base)
     uint32_t* xcd;        // per queue q: count at [q*LINE], fill cursor at [(8+q)*LINE], pop cursor at [(16+q)*LINE]
     uint32_t* stageCount; // staged hits per queue at [q*LINE]; queue q stages into [q*stageCap, (q+1)*stageCap)
     uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue, ascending target template within a queue
@@ -599,12 +602,22 @@ __global__ void k_pack_triangles(P p, W w) {
         w.triPacked[3 * size_t(t) + 2] = make_float4(c.x, c.y, c.z, n.z);
     }
 }
-// One block per template: its leaves' keys, 64 lanes per leaf.
+// One block per template: its clusters' packed rows, then its leaves' keys, 64 lanes
+// per leaf.  A cluster's bound on |p| over its box is the skip test's expression.
 __global__ __launch_bounds__(256) void k_pack_samples(P p, W w) {
     const uint32_t t = blockIdx.x;
     if (t >= p.c.shapes) return;
     const uint4 info = p.meshClusterInfo[t];
     const uint32_t base = p.sampleInfo[t].x;
+    uint32_t first = info.x;  // the template's first leaf (leaves are laid out in preorder)
+    while (first + 1 < info.x + info.y && p.meshClusterRanges[first].y == 0) first++;
+    const uint32_t keysBase = info.y ? p.meshClusterRanges[first].x : 0u;
+    for (uint32_t c = info.x + threadIdx.x; c < info.x + info.y && c < p.clusters; c += blockDim.x) {
+        const uint4 range = p.meshClusterRanges[c];
+        const v3 lo = xyz(p.meshClusterMin[c]), hi = xyz(p.meshClusterMax[c]);
+        const float radius = length(fmax3(fabs3(lo), fabs3(hi)));
+        w.clusterPacked[c] = make_uint4(range.x, range.y, __float_as_uint(radius), range.y ? range.x - keysBase : 0u);
+    }
     for (uint32_t k = threadIdx.x; k < info.y * 64u; k += blockDim.x) {
         const uint4 range = p.meshClusterRanges[info.x + k / 64];
         const uint32_t lane = k % 64;
@@ -2416,20 +2429,18 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             // records the cull's evidence for each rejected leaf.
             const uint4 info = p.meshClusterInfo[f.si];
             int8_t* leafDist = rows ? uniformPtr(w.leafDist + size_t(rowTask) * w.clusterStride) : nullptr;
-            const uint32_t keysBase = rows ? uniformU(w.triBase[3 * f.si + 2]) : 0u;
             uint32_t leafSkipped = 0;  // per lane, summed below
             for (uint32_t base = 0; base < info.y; base += WAVE) {
                 const uint32_t c = base + lane;
                 bool test = false, nearLeaf = false;
                 if (c < info.y) {
-                    const uint4 range = p.meshClusterRanges[info.x + c];
-                    if (range.y > 0) {
+                    const uint4 cp = w.clusterPacked[info.x + c];
+                    if (cp.y > 0) {
                         bool farLeaf = false;
                         if (skipMode) {
-                            const v3 lo = xyz(p.meshClusterMin[info.x + c]), hi = xyz(p.meshClusterMax[info.x + c]);
-                            const float radius = length(fmax3(fabs3(lo), fabs3(hi)));
+                            const float radius = __uint_as_float(cp.z);
                             farLeaf = float(leafDist[c]) * SKIP_QUANTUM > c0 + c1 * radius + SKIP_MARGIN;
-                            if (farLeaf) leafSkipped += range.y;
+                            if (farLeaf) leafSkipped += cp.y;
                             // A leaf with a node inside the target at the anchor is a
                             // candidate without its cull test (the test only filters).
                             nearLeaf = !farLeaf && leafDist[c] < 0;
@@ -2463,7 +2474,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 while (bits != 0) {
                     const uint32_t rel = word * WAVE + uint32_t(__builtin_ctzll(bits));
                     bits &= bits - 1;
-                    const uint4 range = p.meshClusterRanges[info.x + rel];
+                    const uint4 range = w.clusterPacked[info.x + rel];
                     bool pass_ = false, test = false;
                     uint32_t node = 0;
                     bool far = false;
@@ -2473,7 +2484,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     float4 sp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     if (uint32_t(lane) < range.y) {
                         node = p.meshSampleIndices[range.x + lane];
-                        if (rows) slot = range.x + lane - keysBase;
+                        if (rows) slot = range.w + lane;
                         sp = w.sampleKeyed[range.x + lane];  // = p.sample[f.sampleBase + node]
                         bool inside = false;
                         if (skipMode) {
@@ -2595,9 +2606,9 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     while (bits != 0) {
                         const uint32_t rel = word * WAVE + uint32_t(__builtin_ctzll(bits));
                         bits &= bits - 1;
-                        const uint4 range = p.meshClusterRanges[info.x + rel];
+                        const uint4 range = w.clusterPacked[info.x + rel];
                         int v = 127;
-                        if (uint32_t(lane) < range.y) v = dist[range.x + lane - keysBase];
+                        if (uint32_t(lane) < range.y) v = dist[range.w + lane];
                         for (int d = 1; d < WAVE; d <<= 1) v = min(v, __shfl_xor(v, d));
                         if (lane == 0) leafDist[rel] = int8_t(v);
                     }
@@ -3807,7 +3818,7 @@ struct Caps {
     uint32_t counters = 0, taskCount = 0, taskOffset = 0, adjCount = 0, adjOffset = 0, adjCursor = 0, adjList = 0;
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
     uint32_t stageCount = 0, keyCount = 0, keyStart = 0, wallCount = 0, wallOffset = 0, sdfReach = 0, sdfHits = 0;
-    uint32_t sdfLeaves = 0, sdfOrder = 0, builtKey = 0, triPacked = 0, sampleKeyed = 0;
+    uint32_t sdfLeaves = 0, sdfOrder = 0, builtKey = 0, triPacked = 0, sampleKeyed = 0, clusterPacked = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -3855,7 +3866,8 @@ struct ScratchImpl : Scratch {
     int ensure_nodes(uint32_t nodes) { return grow(w.wide, caps.wide, 4 * size_t(std::max(nodes, 1u)), bytes); }
     int ensure_packed(const P& p) {
         int rc = grow(w.triPacked, caps.triPacked, 3 * size_t(std::max(p.triangles, 1u)), bytes);
-        return rc ? rc : grow(w.sampleKeyed, caps.sampleKeyed, std::max(p.sampleKeys, 1u), bytes);
+        if (rc || (rc = grow(w.sampleKeyed, caps.sampleKeyed, std::max(p.sampleKeys, 1u), bytes))) return rc;
+        return grow(w.clusterPacked, caps.clusterPacked, std::max(p.clusters, 1u), bytes);
     }
     // SDF geometry: per-template reach and the hit-bitmask rows, refreshed whenever
     // the geometry epoch changes (an upload, or new bindings); outside any capture.
@@ -4135,7 +4147,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.sdfLeaves); (void)hipFree(w.sdfOrder);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.builtKey); (void)hipFree(w.triPacked);
-        (void)hipFree(w.sampleKeyed); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
+        (void)hipFree(w.sampleKeyed); (void)hipFree(w.clusterPacked); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
         (void)hipFree(w.anchor); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
         (void)hipFree(triBase); (void)hipFree(w.leafDist); (void)hipFree(gapRows); (void)hipFree(lastFrames);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
@@ -4226,6 +4238,7 @@ int bind(P& p, void* const* b, const uint64_t* bytes, const double* v) {
     p.planes = uint32_t(bytes[SAD_MESH_PLANES] / 16);
     p.triangles = uint32_t(bytes[SAD_MESH_TRIANGLES] / 48);
     p.sampleKeys = uint32_t(bytes[SAD_MESH_SAMPLE_INDICES] / 4);
+    p.clusters = uint32_t(bytes[SAD_MESH_CLUSTER_RANGES] / 16);
     p.control = slot<uint32_t>(b, SAD_CONTROL);
     p.cell = slot<float4>(b, SAD_CELL);
     p.position = slot<float4>(b, SAD_POSITION);
@@ -4834,6 +4847,7 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     take(w.wide, 4 * size_t(std::max(p.meshNodes, 1u)), caps.wide);
     take(w.triPacked, 3 * size_t(p.triangles), caps.triPacked);
     take(w.sampleKeyed, p.sampleKeys, caps.sampleKeyed);
+    take(w.clusterPacked, p.clusters, caps.clusterPacked);
     take(w.stageKey, size_t(8) * c.contactCapacity, caps.stageKey);
     take(w.stageMeta, size_t(8) * c.contactCapacity, caps.stageMeta);
     take(w.stagePd, size_t(8) * c.contactCapacity, caps.stagePd);
