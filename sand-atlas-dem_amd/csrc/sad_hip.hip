@@ -117,6 +117,7 @@ struct W {
     float4* sampleKeyed;  // 1 row per sample key
     uint4* clusterPacked; // per sample-tree cluster: first key, node count, |box corner| bound (float bits),
                           // first key less the template's first key (the node's row slot base)
+    uint4* templateRec;   // TREC rows per template (see k_pack_samples)
     uint32_t* xcd;        // per queue q: count at [q*LINE], fill cursor at [(8+q)*LINE], pop cursor at [(16+q)*LINE]
     uint32_t* stageCount; // staged hits per queue at [q*LINE]; queue q stages into [q*stageCap, (q+1)*stageCap)
     uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue, ascending target template within a queue
@@ -199,6 +200,7 @@ constexpr int WAVE = 64;
 constexpr int COLLECT_WAVES = 4;
 constexpr int QCAP = 256;             // per-wave query queue (node keys)
 constexpr int MAX_CLUSTERS = 1024;    // per source template
+constexpr uint32_t TREC = 8;          // rows of a template record
 constexpr int STACK = 32;             // BVH traversal stack (reference depth limit)
 constexpr int SCAN_TILE = 1024;       // 256 threads x 4
 // Device counters that many waves update atomically each sit on their own 256-byte
@@ -602,8 +604,13 @@ __global__ void k_pack_triangles(P p, W w) {
         w.triPacked[3 * size_t(t) + 2] = make_float4(c.x, c.y, c.z, n.z);
     }
 }
-// One block per template: its clusters' packed rows, then its leaves' keys, 64 lanes
-// per leaf.  A cluster's bound on |p| over its box is the skip test's expression.
+// One block per template: its record, its clusters' packed rows, then its leaves'
+// keys, 64 lanes per leaf.  A cluster's bound on |p| over its box is the skip
+// test's expression.  The record holds what a collection task reads of its target
+// template, one row per lane of a single load:
+//   0  root box low (xyz), root node (w bits)   1  root box high (xyz), root node ref (w bits)
+//   2-4  hull groups 0-5 as (first, count) pairs
+//   5  first triangle and triangle count of the BVH (0, 0 without warm starts), first key
 __global__ __launch_bounds__(256) void k_pack_samples(P p, W w) {
     const uint32_t t = blockIdx.x;
     if (t >= p.c.shapes) return;
@@ -612,6 +619,24 @@ __global__ __launch_bounds__(256) void k_pack_samples(P p, W w) {
     uint32_t first = info.x;  // the template's first leaf (leaves are laid out in preorder)
     while (first + 1 < info.x + info.y && p.meshClusterRanges[first].y == 0) first++;
     const uint32_t keysBase = info.y ? p.meshClusterRanges[first].x : 0u;
+    if (threadIdx.x < TREC && t < p.templates) {
+        const uint32_t row = threadIdx.x, root = p.meshRoots[t].x;
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        if (row == 0) {
+            const float4 lo = p.meshMin[root];
+            r = make_uint4(__float_as_uint(lo.x), __float_as_uint(lo.y), __float_as_uint(lo.z), root);
+        } else if (row == 1) {
+            const float4 hi = p.meshMax[root];
+            r = make_uint4(__float_as_uint(hi.x), __float_as_uint(hi.y), __float_as_uint(hi.z), nodeRef(p, root));
+        } else if (row < 5) {
+            const uint4 a = p.meshHullInfo[6 * t + 2 * (row - 2)], b = p.meshHullInfo[6 * t + 2 * (row - 2) + 1];
+            r = make_uint4(a.x, a.y, b.x, b.y);
+        } else if (row == 5) {
+            r = w.warm ? make_uint4(w.triBase[3 * t], w.triBase[3 * t + 1], keysBase, 0u)
+                       : make_uint4(0u, 0u, keysBase, 0u);
+        }
+        w.templateRec[TREC * size_t(t) + row] = r;
+    }
     for (uint32_t c = info.x + threadIdx.x; c < info.x + info.y && c < p.clusters; c += blockDim.x) {
         const uint4 range = p.meshClusterRanges[c];
         const v3 lo = xyz(p.meshClusterMin[c]), hi = xyz(p.meshClusterMax[c]);
@@ -1542,14 +1567,15 @@ struct HullCtx {
     __device__ __forceinline__ v3 mid() const { return 0.5f * (low + high); }
     __device__ __forceinline__ v3 ext() const { return high - low; }
 };
-__device__ __forceinline__ void hullCtx(HullCtx* h, const P& p, uint32_t shape, int lane) {
-    if (lane < 6) {
-        const uint4 info = p.meshHullInfo[6 * shape + lane];
-        h->range[lane] = make_uint2(info.x, info.y);
-    } else if (lane == 6) {
-        const uint32_t root = p.meshRoots[shape].x;
-        h->low = xyz(p.meshMin[root]);
-        h->high = xyz(p.meshMax[root]);
+// From the target's template record, row `lane` of which is `rec`.
+__device__ __forceinline__ void hullCtx(HullCtx* h, uint4 rec, int lane) {
+    if (lane < 2) {
+        const v3 corner = mk3(__uint_as_float(rec.x), __uint_as_float(rec.y), __uint_as_float(rec.z));
+        if (lane == 0) h->low = corner;
+        else h->high = corner;
+    } else if (lane < 5) {
+        h->range[2 * (lane - 2)] = make_uint2(rec.x, rec.y);
+        h->range[2 * (lane - 2) + 1] = make_uint2(rec.z, rec.w);
     }
     wave_sync();
 }
@@ -1715,11 +1741,14 @@ struct BatchRec {  // 112 bytes in LDS per task of the batch
 constexpr uint32_t NO_ROW = 0xffffffffu;
 static_assert(sizeof(BatchRec) == 112, "batch record layout");
 
-__device__ __forceinline__ void batchPut(BatchRec* r, const P& p, const W& w, const TaskFrame& f, uint32_t task,
+// The target's root node, its reference, and its first triangle and triangle count
+// (0, 0 without warm starts), from its template record.
+struct TargetRoot {
+    uint32_t root, rootRef, triFirst, triCount;
+};
+__device__ __forceinline__ void batchPut(BatchRec* r, const TaskFrame& f, const TargetRoot& tg, uint32_t task,
                                          uint32_t row, bool refresh, const GapCtx& gc, int lane) {
-    const uint32_t root = p.meshRoots[f.sj].x;
-    const uint32_t rootRef = nodeRef(p, root);
-    const uint32_t triFirst = w.warm ? w.triBase[3 * f.sj] : 0u, triCount = w.warm ? w.triBase[3 * f.sj + 1] : 0u;
+    const uint32_t root = tg.root, rootRef = tg.rootRef, triFirst = tg.triFirst, triCount = tg.triCount;
     if (lane == 0) {
         r->task = task; r->root = root; r->rootRef = rootRef; r->sampleBase = f.sampleBase;
         r->triFirst = triFirst; r->triCount = triCount;
@@ -2251,6 +2280,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             f.j = pass == 0 ? ij.y : ij.x;
             f.si = p.shapeId[f.i];
             f.sj = p.shapeId[f.j];
+            const uint4 trec = w.templateRec[TREC * size_t(f.sj) + (uint32_t(lane) & (TREC - 1))];
             f.sampleBase = p.sampleInfo[f.si].x;
             f.qi = v4of(p.rotation[f.i]);
             f.qj = v4of(p.rotation[f.j]);
@@ -2393,8 +2423,10 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 f.plane = i0 == 0xffffffffu ? make_float4(0, 0, 0, 0) : p.meshPlanes[i0];
             }
             uniformFrame(f);
-            hullCtx(&hulls[wv], p, f.sj, lane);
+            hullCtx(&hulls[wv], trec, lane);
             const HullCtx& hc = hulls[wv];
+            const TargetRoot tg{uniformU(__shfl(trec.w, 0)), uniformU(__shfl(trec.w, 1)),
+                                w.warm ? uniformU(__shfl(trec.x, 5)) : 0u, w.warm ? uniformU(__shfl(trec.y, 5)) : 0u};
             const bool refresh = refreshDist != nullptr;
             // Learned cull: on for targets whose network admits a cull.  A task with
             // persistent rows evaluates it only when it re-anchors: the nodes it drops
@@ -2416,9 +2448,9 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 nnNear = uniformF(nnNear);
             }
             const bool gapTask = gc.row != nullptr && gc.genPrev != 0u;
-            const uint32_t triFirst = gapTask ? uniformU(w.triBase[3 * f.sj]) : 0u;
-            const uint32_t triCount = gapTask ? uniformU(w.triBase[3 * f.sj + 1]) : 0u;
-            batchPut(batch + nb, p, w, f, task, rows ? rowTask : NO_ROW, refresh, gc, lane);
+            const uint32_t triFirst = gapTask ? tg.triFirst : 0u;
+            const uint32_t triCount = gapTask ? tg.triCount : 0u;
+            batchPut(batch + nb, f, tg, task, rows ? rowTask : NO_ROW, refresh, gc, lane);
             nb++;
             // Leaf culls, lane-parallel.  Internal clusters are not tested: a leaf's
             // own test is a valid filter, and a leaf the preorder walk would prune
@@ -2563,7 +2595,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                         qn = 0;
                         pending = 0;
                         wave_sync();
-                        batchPut(batch, p, w, f, task, rows ? rowTask : NO_ROW, refresh, gc, lane);
+                        batchPut(batch, f, tg, task, rows ? rowTask : NO_ROW, refresh, gc, lane);
                         nb = 1;
                         wave_sync();
                     }
@@ -3819,6 +3851,7 @@ struct Caps {
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
     uint32_t stageCount = 0, keyCount = 0, keyStart = 0, wallCount = 0, wallOffset = 0, sdfReach = 0, sdfHits = 0;
     uint32_t sdfLeaves = 0, sdfOrder = 0, builtKey = 0, triPacked = 0, sampleKeyed = 0, clusterPacked = 0;
+    uint32_t templateRec = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -3867,7 +3900,8 @@ struct ScratchImpl : Scratch {
     int ensure_packed(const P& p) {
         int rc = grow(w.triPacked, caps.triPacked, 3 * size_t(std::max(p.triangles, 1u)), bytes);
         if (rc || (rc = grow(w.sampleKeyed, caps.sampleKeyed, std::max(p.sampleKeys, 1u), bytes))) return rc;
-        return grow(w.clusterPacked, caps.clusterPacked, std::max(p.clusters, 1u), bytes);
+        if ((rc = grow(w.clusterPacked, caps.clusterPacked, std::max(p.clusters, 1u), bytes))) return rc;
+        return grow(w.templateRec, caps.templateRec, TREC * size_t(std::max(p.c.shapes, 1u)), bytes);
     }
     // SDF geometry: per-template reach and the hit-bitmask rows, refreshed whenever
     // the geometry epoch changes (an upload, or new bindings); outside any capture.
@@ -4147,7 +4181,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.sdfLeaves); (void)hipFree(w.sdfOrder);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.builtKey); (void)hipFree(w.triPacked);
-        (void)hipFree(w.sampleKeyed); (void)hipFree(w.clusterPacked); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
+        (void)hipFree(w.sampleKeyed); (void)hipFree(w.clusterPacked); (void)hipFree(w.templateRec); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
         (void)hipFree(w.anchor); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
         (void)hipFree(triBase); (void)hipFree(w.leafDist); (void)hipFree(gapRows); (void)hipFree(lastFrames);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
@@ -4848,6 +4882,7 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     take(w.triPacked, 3 * size_t(p.triangles), caps.triPacked);
     take(w.sampleKeyed, p.sampleKeys, caps.sampleKeyed);
     take(w.clusterPacked, p.clusters, caps.clusterPacked);
+    take(w.templateRec, TREC * size_t(p.c.shapes), caps.templateRec);
     take(w.stageKey, size_t(8) * c.contactCapacity, caps.stageKey);
     take(w.stageMeta, size_t(8) * c.contactCapacity, caps.stageMeta);
     take(w.stagePd, size_t(8) * c.contactCapacity, caps.stagePd);
