@@ -213,7 +213,7 @@ constexpr int LINE = 64;
 #define SAD_COLLECT_NN_WAVES_PER_EU 4
 #endif
 #ifndef SAD_COLLECT_WAVES_PER_EU
-#define SAD_COLLECT_WAVES_PER_EU 5
+#define SAD_COLLECT_WAVES_PER_EU 4
 #endif
 #ifndef SAD_COLLECT_BLOCKS_PER_CU
 #define SAD_COLLECT_BLOCKS_PER_CU 6
@@ -611,7 +611,11 @@ __global__ void k_pack_triangles(P p, W w) {
 //   0  root box low (xyz), root node (w bits)   1  root box high (xyz), root node ref (w bits)
 //   2-4  hull groups 0-5 as (first, count) pairs
 //   5  first triangle and triangle count of the BVH (0, 0 without warm starts), first key
+//   6  the largest corner-radius bound over its leaves (float bits)
 __global__ __launch_bounds__(256) void k_pack_samples(P p, W w) {
+    __shared__ uint32_t widest;  // float bits of a nonnegative value: ordered as unsigned
+    if (threadIdx.x == 0) widest = 0u;
+    __syncthreads();
     const uint32_t t = blockIdx.x;
     if (t >= p.c.shapes) return;
     const uint4 info = p.meshClusterInfo[t];
@@ -619,7 +623,7 @@ __global__ __launch_bounds__(256) void k_pack_samples(P p, W w) {
     uint32_t first = info.x;  // the template's first leaf (leaves are laid out in preorder)
     while (first + 1 < info.x + info.y && p.meshClusterRanges[first].y == 0) first++;
     const uint32_t keysBase = info.y ? p.meshClusterRanges[first].x : 0u;
-    if (threadIdx.x < TREC && t < p.templates) {
+    if (threadIdx.x < 6 && t < p.templates) {
         const uint32_t row = threadIdx.x, root = p.meshRoots[t].x;
         uint4 r = make_uint4(0u, 0u, 0u, 0u);
         if (row == 0) {
@@ -642,6 +646,12 @@ __global__ __launch_bounds__(256) void k_pack_samples(P p, W w) {
         const v3 lo = xyz(p.meshClusterMin[c]), hi = xyz(p.meshClusterMax[c]);
         const float radius = length(fmax3(fabs3(lo), fabs3(hi)));
         w.clusterPacked[c] = make_uint4(range.x, range.y, __float_as_uint(radius), range.y ? range.x - keysBase : 0u);
+        if (range.y > 0) atomicMax(&widest, __float_as_uint(radius));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && t < p.templates) {
+        w.templateRec[TREC * size_t(t) + 6] = make_uint4(widest, 0u, 0u, 0u);
+        w.templateRec[TREC * size_t(t) + 7] = make_uint4(0u, 0u, 0u, 0u);
     }
     for (uint32_t k = threadIdx.x; k < info.y * 64u; k += blockDim.x) {
         const uint4 range = p.meshClusterRanges[info.x + k / 64];
@@ -2309,6 +2319,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             int8_t* dist = nullptr;
             int8_t* refreshDist = nullptr;
             uint32_t skipped = 0, anchoredPlane = 0xffffffffu;
+            bool allFar = false;
             // The task's persistent rows, used only while their owner is this pair.
             uint32_t rowTask = task;
             bool rows = false;
@@ -2324,7 +2335,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 dist = w.nodeDist + size_t(rowTask) * w.nodeStride;
                 const float4* an = w.anchor + 4 * size_t(rowTask);
                 const float4 a0 = an[0], a1 = an[1], a2 = an[2], a3 = an[3];
-                const bool valid = __float_as_uint(a3.w) == 1u && __float_as_uint(a3.x) == f.i &&
+                const bool valid = (__float_as_uint(a3.w) & 0xffu) == 1u && __float_as_uint(a3.x) == f.i &&
                                    __float_as_uint(a3.y) == f.j && __float_as_uint(a3.z) == ld_volatile(w.counters + 10 * LINE);
                 const float dqi = sqrtf(dot4(f.qi - v4of(a0), f.qi - v4of(a0)));
                 const float dqj = sqrtf(dot4(f.qj - v4of(a1), f.qj - v4of(a1)));
@@ -2337,6 +2348,14 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 c0 = uniformF(c0);
                 c1 = uniformF(c1);
                 skipMode = uniformU(skipMode ? 1u : 0u) != 0u;
+                if (skipMode) {
+                    // The anchor's least leaf bound against the motion bound of the
+                    // source's widest leaf: the leaf test below would pass over every leaf.
+                    const float widest = __uint_as_float(uniformU(w.templateRec[TREC * size_t(f.si) + 6].x));
+                    const int leafMin = int(int8_t(uint8_t(__float_as_uint(a3.w) >> 8)));
+                    allFar = float(leafMin) * SKIP_QUANTUM > c0 + c1 * widest + SKIP_MARGIN;
+                    allFar = uniformU(allFar ? 1u : 0u) != 0u;
+                }
                 rebased = uniformU(rebased ? 1u : 0u) != 0u;
                 dist = uniformPtr(dist);
                 if (skipMode) {
@@ -2367,6 +2386,11 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     for (uint32_t k = lane; k < words; k += WAVE) d32[k] = 0u;
                     __threadfence_block();
                 }
+            }
+            if (allFar) {  // no node can be a contact: nothing to queue
+                if (lane == 0) w.taskCount[task] = 0u;
+                skippedNodes += uniformU(p.sampleInfo[f.si].y);
+                continue;
             }
             // Unchanged-winner shortcut: the motion bound since this task's previous
             // collection and the generations that tag each node's stored gap.
@@ -2654,14 +2678,23 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 lf[3] = make_float4(__uint_as_float(f.i), __uint_as_float(f.j),
                                     __uint_as_float(ld_volatile(w.counters + 10 * LINE)), __uint_as_float(1u));
             }
-            if (refreshDist && lane == 0) {
-                atomicAdd(w.counters + 13 * LINE, 1u);
-                float4* an = w.anchor + 4 * size_t(rowTask);
-                an[0] = f4(f.qi);
-                an[1] = f4(f.qj);
-                an[2] = f4(f.branch, __uint_as_float(anchoredPlane));
-                an[3] = make_float4(__uint_as_float(f.i), __uint_as_float(f.j),
-                                    __uint_as_float(ld_volatile(w.counters + 10 * LINE)), __uint_as_float(1u));
+            if (refreshDist) {
+                // The task's least leaf bound, for the all-far test of later collections.
+                __threadfence_block();
+                int leafMin = 127;
+                for (uint32_t c = lane; c < info.y; c += WAVE)
+                    if (w.clusterPacked[info.x + c].y > 0) leafMin = min(leafMin, int(leafDist[c]));
+                for (int d = 1; d < WAVE; d <<= 1) leafMin = min(leafMin, __shfl_xor(leafMin, d));
+                if (lane == 0) {
+                    atomicAdd(w.counters + 13 * LINE, 1u);
+                    float4* an = w.anchor + 4 * size_t(rowTask);
+                    an[0] = f4(f.qi);
+                    an[1] = f4(f.qj);
+                    an[2] = f4(f.branch, __uint_as_float(anchoredPlane));
+                    an[3] = make_float4(__uint_as_float(f.i), __uint_as_float(f.j),
+                                        __uint_as_float(ld_volatile(w.counters + 10 * LINE)),
+                                        __uint_as_float(1u | (uint32_t(uint8_t(int8_t(leafMin))) << 8)));
+                }
             }
             skippedNodes += skipped;
         } else if (lane == 0) {
