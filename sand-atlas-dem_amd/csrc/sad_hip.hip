@@ -118,6 +118,7 @@ struct W {
     uint4* clusterPacked; // per sample-tree cluster: first key, node count, |box corner| bound (float bits),
                           // first key less the template's first key (the node's row slot base)
     uint4* templateRec;   // TREC rows per template (see k_pack_samples)
+    uint32_t* taskFar;    // per task: the source's node count when k_task_far passes it over whole, else 0
     uint32_t* xcd;        // per queue q: count at [q*LINE], fill cursor at [(8+q)*LINE], pop cursor at [(16+q)*LINE]
     uint32_t* stageCount; // staged hits per queue at [q*LINE]; queue q stages into [q*stageCap, (q+1)*stageCap)
     uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue, ascending target template within a queue
@@ -1481,6 +1482,56 @@ __device__ __forceinline__ v3 queryPoint(const TaskFrame& f, float4 sp, v3& arm)
 // these magnitudes).
 constexpr float SKIP_QUANTUM = 1.0f / 512.0f;
 constexpr float SKIP_MARGIN = 1e-3f;
+// The temporal skip's motion bound since a task's anchor (rows a0, a1, a2: q_i, q_j,
+// branch): a node p of the source has moved at most c0 + c1 |p| in the target frame,
+// and every node of the grain (bounding radius / s_i) at most `all`.
+__device__ __forceinline__ void motionBound(v4 qi, v4 qj, v3 branch, float s_i, float s_j, float radius, float4 a0,
+                                            float4 a1, float4 a2, float& c0, float& c1, float& all) {
+    const float dqi = sqrtf(dot4(qi - v4of(a0), qi - v4of(a0)));
+    const float dqj = sqrtf(dot4(qj - v4of(a1), qj - v4of(a1)));
+    const float db = length(branch - xyz(a2));
+    c0 = 1.001f * (2.0f * dqj * length(branch) + db) / s_j;
+    c1 = 1.001f * (2.0f * dqj + 2.0f * dqi) * s_i / s_j;
+    all = c0 + c1 * (radius / s_i);
+}
+// The anchor's least leaf bound (bits 8-15 of row 3's w) against the motion bound of
+// the source's widest leaf: true when the leaf test would pass over every leaf.
+__device__ __forceinline__ bool allLeavesFar(float4 a3, float c0, float c1, float widest) {
+    const int leafMin = int(int8_t(uint8_t(__float_as_uint(a3.w) >> 8)));
+    return float(leafMin) * SKIP_QUANTUM > c0 + c1 * widest + SKIP_MARGIN;
+}
+// Before the collection, one thread per task: the tasks it can pass over whole (their
+// anchor valid, the skip on, every leaf far), with the collection's own inputs and
+// FP32 expressions.  About half the tasks of a dense packing; the collection then
+// spends one load on each of them instead of its wave-wide setup.
+__global__ void k_task_far(P p, W w) {
+    const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
+    const uint32_t epoch = ld_volatile(w.counters + 10 * LINE);
+    for (uint32_t task = blockIdx.x * blockDim.x + threadIdx.x; task < tasks; task += gridDim.x * blockDim.x) {
+        uint32_t far = 0u;
+        const uint2 ij = p.pairs[task >> 1];
+        const uint32_t r = w.pairRow[task >> 1];
+        if (w.rowOwner[r] == pairKey(ij)) {
+            const uint32_t i = (task & 1) ? ij.y : ij.x, j = (task & 1) ? ij.x : ij.y;
+            const float4* an = w.anchor + 4 * size_t(2u * r + (task & 1u));
+            const float4 a0 = an[0], a1 = an[1], a2 = an[2], a3 = an[3];
+            const bool valid = (__float_as_uint(a3.w) & 0xffu) == 1u && __float_as_uint(a3.x) == i &&
+                               __float_as_uint(a3.y) == j && __float_as_uint(a3.z) == epoch;
+            if (valid) {
+                const float4 bi = p.body[i], bj = p.body[j];
+                float c0, c1, all;
+                motionBound(v4of(p.rotation[i]), v4of(p.rotation[j]), centerBranch(p, i, j), bi.x, bj.x, bi.z, a0, a1,
+                            a2, c0, c1, all);
+                if (all + SKIP_MARGIN < w.skipLimit) {
+                    const uint32_t si = p.shapeId[i];
+                    const float widest = __uint_as_float(w.templateRec[TREC * size_t(si) + 6].x);
+                    if (allLeavesFar(a3, c0, c1, widest)) far = p.sampleInfo[si].y;
+                }
+            }
+        }
+        w.taskFar[task] = far;
+    }
+}
 // Rounded toward zero: a lower bound of a positive distance, an upper bound of a
 // negative one; NaN stores 0 (neither far nor inside).
 // Four stored bounds moved toward zero by `shrink` quanta (clamped at zero).  Leaf
@@ -2283,6 +2334,14 @@ __device__ __forceinline__ void collectBody(P p, W w) {
         task = w.xcdTasks[qoff[from] + next];
         next++;
         left--;
+        if (w.skip) {  // passed over whole by k_task_far: no node can be a contact
+            const uint32_t far = uniformU(w.taskFar[task]);
+            if (far != 0u) {
+                if (lane == 0) w.taskCount[task] = 0u;
+                skippedNodes += far;
+                continue;
+            }
+        }
         if (ld_volatile(p.control) == 0) {
             uint2 ij = p.pairs[task >> 1];
             uint32_t pass = task & 1;
@@ -2319,7 +2378,6 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             int8_t* dist = nullptr;
             int8_t* refreshDist = nullptr;
             uint32_t skipped = 0, anchoredPlane = 0xffffffffu;
-            bool allFar = false;
             // The task's persistent rows, used only while their owner is this pair.
             uint32_t rowTask = task;
             bool rows = false;
@@ -2337,25 +2395,13 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 const float4 a0 = an[0], a1 = an[1], a2 = an[2], a3 = an[3];
                 const bool valid = (__float_as_uint(a3.w) & 0xffu) == 1u && __float_as_uint(a3.x) == f.i &&
                                    __float_as_uint(a3.y) == f.j && __float_as_uint(a3.z) == ld_volatile(w.counters + 10 * LINE);
-                const float dqi = sqrtf(dot4(f.qi - v4of(a0), f.qi - v4of(a0)));
-                const float dqj = sqrtf(dot4(f.qj - v4of(a1), f.qj - v4of(a1)));
-                const float db = length(f.branch - xyz(a2));
-                c0 = 1.001f * (2.0f * dqj * length(f.branch) + db) / f.s_j;
-                c1 = 1.001f * (2.0f * dqj + 2.0f * dqi) * f.s_i / f.s_j;
-                const float reachAll = c0 + c1 * (p.body[f.i].z / f.s_i);  // bound for every node of the grain
+                float reachAll;  // bound for every node of the grain
+                motionBound(f.qi, f.qj, f.branch, f.s_i, f.s_j, p.body[f.i].z, a0, a1, a2, c0, c1, reachAll);
                 skipMode = valid && reachAll + SKIP_MARGIN < w.skipLimit;
                 rebased = valid && !skipMode && w.rebase != 0u && reachAll < 0.25f;
                 c0 = uniformF(c0);
                 c1 = uniformF(c1);
                 skipMode = uniformU(skipMode ? 1u : 0u) != 0u;
-                if (skipMode) {
-                    // The anchor's least leaf bound against the motion bound of the
-                    // source's widest leaf: the leaf test below would pass over every leaf.
-                    const float widest = __uint_as_float(uniformU(w.templateRec[TREC * size_t(f.si) + 6].x));
-                    const int leafMin = int(int8_t(uint8_t(__float_as_uint(a3.w) >> 8)));
-                    allFar = float(leafMin) * SKIP_QUANTUM > c0 + c1 * widest + SKIP_MARGIN;
-                    allFar = uniformU(allFar ? 1u : 0u) != 0u;
-                }
                 rebased = uniformU(rebased ? 1u : 0u) != 0u;
                 dist = uniformPtr(dist);
                 if (skipMode) {
@@ -2386,11 +2432,6 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     for (uint32_t k = lane; k < words; k += WAVE) d32[k] = 0u;
                     __threadfence_block();
                 }
-            }
-            if (allFar) {  // no node can be a contact: nothing to queue
-                if (lane == 0) w.taskCount[task] = 0u;
-                skippedNodes += uniformU(p.sampleInfo[f.si].y);
-                continue;
             }
             // Unchanged-winner shortcut: the motion bound since this task's previous
             // collection and the generations that tag each node's stored gap.
@@ -3829,7 +3870,7 @@ thread_local char g_error[256] = "no error";
     X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_offsets) X(bucket_fill) \
     X(advance_walls) X(wall_count) X(wall_emit) X(wall_total) X(gather_wall_rows) X(refresh_wall_power) \
     X(diagnostic_rows) X(fabric_rows) X(sdf_count) X(sdf_total) X(sdf_emit) X(rows) X(derived_check) \
-    X(pack_geometry)
+    X(pack_geometry) X(task_far)
 enum KernelId {
 #define SAD_ID(name) KID_##name,
     SAD_KERNELS(SAD_ID)
@@ -3901,14 +3942,14 @@ struct ScratchImpl : Scratch {
     // default; the caller that enables it calls sad_hip_invalidate whenever it
     // rewrites template geometry, body scales or shape ids in place.
     int skip = 0;
-    int skipLimitMicro = 6000;
+    int skipLimitMicro = 12000;  // re-anchor distance: best of 3000-20000 over 2,000-step runs (dense, uneven)
     int rebase = 0;
     uint32_t epoch = 1, epochWritten = 0;  // anchors carry the epoch they were taken in
     std::vector<uint64_t> lastTable;        // binding table of the previous call
     const void* maxNodesFor = nullptr;
     uint32_t maxNodesShapes = 0, maxNodes = 0;
     size_t nodeDistCap = 0;
-    uint32_t anchorCap = 0;
+    uint32_t anchorCap = 0, taskFarCap = 0;
     uint32_t* probeWord = nullptr;
     uint32_t* triBase = nullptr;
     uint32_t triBaseCap = 0, maxTriangles = 0, maxClusters = 0;
@@ -4102,6 +4143,12 @@ struct ScratchImpl : Scratch {
             HIP_OK(hipMemset(w.anchor, 0, 4 * tasks * sizeof(float4)));
             anchorCap = uint32_t(4 * tasks);
             bytes += size_t(anchorCap) * sizeof(float4);
+            if (w.taskFar) { (void)hipFree(w.taskFar); bytes -= size_t(taskFarCap) * sizeof(uint32_t); }
+            w.taskFar = nullptr;
+            taskFarCap = 0;
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.taskFar), tasks * sizeof(uint32_t)));
+            taskFarCap = uint32_t(tasks);
+            bytes += size_t(taskFarCap) * sizeof(uint32_t);
         }
         if (nodeDistCap < bytesNeeded || w.nodeStride != stride) {
             if (w.nodeDist) { (void)hipFree(w.nodeDist); bytes -= nodeDistCap; }
@@ -4215,7 +4262,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.builtKey); (void)hipFree(w.triPacked);
         (void)hipFree(w.sampleKeyed); (void)hipFree(w.clusterPacked); (void)hipFree(w.templateRec); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
-        (void)hipFree(w.anchor); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
+        (void)hipFree(w.anchor); (void)hipFree(w.taskFar); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
         (void)hipFree(triBase); (void)hipFree(w.leafDist); (void)hipFree(gapRows); (void)hipFree(lastFrames);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
@@ -4497,6 +4544,7 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
             ScanArgs fr{w.rowFree, w.rowFreeRank, w.sums, w.rowPairs, nullptr, 1, w.rowState + 4, nullptr};
             launch_scan(sc, fr, w.rowPairs, s);
             LAUNCH(sc, KID_rows, k_rows_assign, gridStride, T, s, p, w);
+            LAUNCH(sc, KID_task_far, k_task_far, dim3(blocks(2 * c.pairCapacity)), T, s, p, w);
         }
         const dim3 persistent(std::max(1, sc.cus) * sc.blocksPerCU);
         w.stageChunk = stage_chunk(c, sc);
@@ -4938,6 +4986,7 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     w.nnAudit = 0;
     w.nnTemplates = 0;
     w.anchor = nullptr;
+    w.taskFar = nullptr;
     w.nodeDist = nullptr;
     w.nodeTri = nullptr;
     w.leafDist = nullptr;
