@@ -137,6 +137,7 @@ struct W {
     uint32_t clusterStride;
     const uint32_t* triBase;  // per template: first triangle, triangle count, first sample key
     uint32_t warm;        // 1 = warm-start the exact queries from nodeTri
+    uint32_t planeCache;  // 1 = nodeTri also caches a hull-test node's separating plane (tagged)
     uint32_t nodeStride;  // bytes per task (>= nodes of any template, multiple of 64)
     uint32_t skip;        // 1 = on
     float skipLimit;      // re-anchor once the motion bound reaches this (template units)
@@ -1622,6 +1623,13 @@ __device__ float meshOutsideWave(const P& p, uint32_t shape, v3 q, int lane) {
 // memory instead of a chain of root, box and group loads.  The group's (first,
 // count) is indexed by the per-lane group.  mid() and ext() are the one-lane
 // form's 0.5 (low + high) and high - low (same bits).
+// A node's cached separating plane, kept in its warm-start slot (nodeTri) with the
+// top bit set: 3 bits of hull group, 12 of index in the group.  NO_PLANE: none.
+constexpr uint32_t NO_PLANE = 0u;
+constexpr uint32_t PLANE_TAG = 0x8000u;
+__device__ __forceinline__ uint32_t planeCode(uint32_t group, uint32_t k) {
+    return k < 4096u ? PLANE_TAG | (group << 12) | k : NO_PLANE;
+}
 struct HullCtx {
     v3 low, high;
     uint2 range[6];
@@ -1647,11 +1655,27 @@ __device__ __forceinline__ float rootExcess(const HullCtx& h, v3 q) {
     return fmaxf(fmaxf(fmaxf(h.low.x - q.x, q.x - h.high.x), fmaxf(h.low.y - q.y, q.y - h.high.y)),
                  fmaxf(h.low.z - q.z, q.z - h.high.z));
 }
-// meshOutsideFast with the task's hull context.
-__device__ __forceinline__ float nodeOutside(const P& p, const HullCtx& h, v3 q) {
+// meshOutsideFast with the task's hull context; `code` names the plane of a rejection
+// (the largest of the first batch that has one).
+__device__ __forceinline__ float nodeOutside(const P& p, const HullCtx& h, v3 q, uint32_t& code) {
+    code = NO_PLANE;
     if (outsideRoot(h, q)) return rootExcess(h, q);
-    const uint2 r = h.range[dominantGroup((q - h.mid()) / h.ext())];
-    return r.y == 0 ? 0.0f : planesExcess(p.meshPlanes, r.x, r.y, q);
+    const uint32_t g = dominantGroup((q - h.mid()) / h.ext());
+    const uint2 r = h.range[g];
+    for (uint32_t k = 0; k < r.y; k += PLANE_BATCH) {
+        float worst = 0.0f;
+        uint32_t best = NO_PLANE;
+#pragma unroll
+        for (uint32_t j = 0; j < PLANE_BATCH; j++) {
+            const uint32_t kk = min(k + j, r.y - 1);
+            const float4 pl = p.meshPlanes[r.x + kk];
+            const float e = dot(xyz(pl), q) + pl.w;
+            if (e > worst) { worst = e; best = planeCode(g, kk); }
+            worst = fmaxf(worst, e);
+        }
+        if (worst > 0) { code = best; return worst; }
+    }
+    return 0.0f;
 }
 // Hull tests of the lanes in `tests` (at most 16) together: the tested points are
 // spread over groups of 64 / 2^ceil(log2 n) lanes, each group evaluates its point's
@@ -1659,7 +1683,8 @@ __device__ __forceinline__ float nodeOutside(const P& p, const HullCtx& h, v3 q)
 // its own point's value back.  The verdict (no plane value above zero, inside the
 // root box) is the one-lane form's bit for bit; a failing point reports its largest
 // excess rather than the first batch's, an equally valid distance bound.
-__device__ __forceinline__ float nodesOutsideWave(const P& p, const HullCtx& h, uint64_t tests, v3 q, int lane) {
+__device__ __forceinline__ float nodesOutsideWave(const P& p, const HullCtx& h, uint64_t tests, v3 q, int lane,
+                                                  uint32_t& code) {
     const uint32_t n = uint32_t(__popcll(tests));
     const uint32_t shift = n <= 1 ? 6u : n <= 2 ? 5u : n <= 4 ? 4u : n <= 8 ? 3u : 2u;
     const uint32_t span = 1u << shift, group = uint32_t(lane) >> shift, sub = uint32_t(lane) & (span - 1u);
@@ -1673,19 +1698,29 @@ __device__ __forceinline__ float nodesOutsideWave(const P& p, const HullCtx& h, 
     const int from = src < 0 ? lane : src;
     const v3 x = mk3(__shfl(q.x, from), __shfl(q.y, from), __shfl(q.z, from));
     float worst = 0.0f;
+    uint32_t best = NO_PLANE;
     if (src >= 0) {
         if (outsideRoot(h, x)) {
             worst = rootExcess(h, x);
         } else {
-            const uint2 r = h.range[dominantGroup((x - h.mid()) / h.ext())];
+            const uint32_t g = dominantGroup((x - h.mid()) / h.ext());
+            const uint2 r = h.range[g];
             for (uint32_t k = sub; k < r.y; k += span) {
                 const float4 pl = p.meshPlanes[r.x + k];
-                worst = fmaxf(worst, dot(xyz(pl), x) + pl.w);
+                const float e = dot(xyz(pl), x) + pl.w;
+                if (e > worst) { worst = e; best = planeCode(g, k); }
+                worst = fmaxf(worst, e);
             }
         }
     }
-    for (uint32_t d = 1; d < span; d <<= 1) worst = fmaxf(worst, __shfl_xor(worst, int(d)));
+    for (uint32_t d = 1; d < span; d <<= 1) {
+        const float ow = __shfl_xor(worst, int(d));
+        const uint32_t ob = __shfl_xor(best, int(d));
+        if (ow > worst) best = ob;
+        worst = fmaxf(worst, ow);
+    }
     const uint32_t rank = uint32_t(__popcll(tests & lanes_below(lane)));
+    code = __shfl(best, int(rank << shift));
     return __shfl(worst, int(rank << shift));
 }
 // The transformed, padded box of a source cluster in the target frame
@@ -2576,13 +2611,14 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     uint32_t node = 0;
                     bool far = false;
                     float nearBound = 1e30f;  // an upper bound of the node's signed distance, when one is known
-                    uint32_t slot = 0;
+                    uint32_t slot = 0, cached = 0;  // cached: the slot's warm-start word
                     v3 local = mk3(0.0f, 0.0f, 0.0f);
                     float4 sp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     if (uint32_t(lane) < range.y) {
                         node = p.meshSampleIndices[range.x + lane];
                         if (rows) slot = range.w + lane;
                         sp = w.sampleKeyed[range.x + lane];  // = p.sample[f.sampleBase + node]
+                        if (warmTri) cached = warmTri[slot];
                         bool inside = false;
                         if (skipMode) {
                             const float stored = float(dist[slot]) * SKIP_QUANTUM;
@@ -2610,16 +2646,37 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                             }
                         }
                     }
+                    // The node's cached separating plane: a hull plane of the target that it
+                    // lies more than SKIP_MARGIN outside of keeps it out of every contact (its
+                    // distance to the mesh exceeds that, far above the FP32 error of a
+                    // query), so the group test, itself only a filter, is not needed.
+                    if (test && (cached & PLANE_TAG) != 0u && w.planeCache) {
+                        const uint32_t g = (cached >> 12) & 7u, k = cached & 0xfffu;
+                        if (g < 6u) {
+                            const uint2 r = hc.range[g];
+                            if (k < r.y) {
+                                const float4 pl = p.meshPlanes[r.x + k];
+                                const float e = dot(xyz(pl), local) + pl.w;
+                                if (e > SKIP_MARGIN) {
+                                    test = false;
+                                    if (refreshDist) refreshDist[slot] = quantizeDistance(e);
+                                }
+                            }
+                        }
+                    }
                     const uint64_t tests = __ballot(test);
                     if (tests != 0) {
                         float outside = 0.0f;
+                        uint32_t code = NO_PLANE;
                         if (uint32_t(__popcll(tests)) <= COOP_NODES)
-                            outside = nodesOutsideWave(p, hc, tests, local, lane);
+                            outside = nodesOutsideWave(p, hc, tests, local, lane, code);
                         else if (test)
-                            outside = nodeOutside(p, hc, local);
+                            outside = nodeOutside(p, hc, local, code);
                         if (test) {
                             pass_ = !(outside > 0.0f);
                             if (refreshDist && !pass_) refreshDist[slot] = quantizeDistance(outside);
+                            if (!pass_ && warmTri && w.planeCache && code != NO_PLANE && code != cached)
+                                warmTri[slot] = uint16_t(code);
                         }
                     }
                     if (skipMode) skipped += __popcll(__ballot(far));
@@ -2629,7 +2686,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     // its gap shrinks by twice that motion.
                     uint32_t done = 0u;  // 1 + the proven nearest triangle, 0: traverse
                     if (pass_ && gapTask) {
-                        const uint32_t last = warmTri[slot];
+                        const uint32_t last = cached;
                         if (last != 0u && last <= triCount) {
                             const uint32_t word = gc.row[slot];
                             if ((word >> 16) == gc.genPrev) {
@@ -4103,6 +4160,7 @@ struct ScratchImpl : Scratch {
         if (key != lastTable) { lastTable.swap(key); epoch++; }
         w.skip = 0;
         w.warm = 0;
+        w.planeCache = 0;
         w.nodeGap = nullptr;
         w.lastFrame = nullptr;
         if (!skip || p.c.meshGeometry == 0) return SAD_OK;  // the SDF step has no node queries to skip
@@ -4223,6 +4281,7 @@ struct ScratchImpl : Scratch {
         w.lastFrame = gapsOn ? lastFrames : nullptr;
         w.triBase = triBase;
         w.warm = maxTriangles <= 65534u ? 1u : 0u;
+        w.planeCache = maxTriangles < PLANE_TAG ? 1u : 0u;  // a tag never reads as a triangle
         w.skipLimit = float(skipLimitMicro) * 1e-6f;
         w.rebase = uint32_t(rebase);
         w.skip = 1;
@@ -4982,6 +5041,7 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     w.wallCap = uint32_t(std::max<size_t>(wallTasks, 1));
     w.skip = 0;
     w.warm = 0;
+    w.planeCache = 0;
     w.nnEnabled = 0;
     w.nnAudit = 0;
     w.nnTemplates = 0;
