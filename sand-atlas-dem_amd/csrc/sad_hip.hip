@@ -3576,41 +3576,66 @@ __global__ __launch_bounds__(256) void k_gather(P p, W w) {
             if (fill == GCHUNK) { process(fill); fill = 0; }
         }
     }
-    const uint32_t first = w.adjOffset[id];
-    for (uint32_t k = w.adjCount[id]; k-- > 0;) {
-        uint32_t pair = w.adjList[first + k];
-        uint2 ij = p.pairs[pair];
-        for (int pass = 1; pass >= 0; pass--) {
-            uint32_t task = 2 * pair + pass;
-            uint32_t cnt = w.taskCount[task];
-            if (cnt == 0) continue;
-            uint32_t src = pass == 0 ? ij.x : ij.y, dst = pass == 0 ? ij.y : ij.x;
-            float source = id == src ? 1.0f : 0.0f;
-            v3 branch = centerBranch(p, src, dst);
-            float minW = fminf(p.body[src].w, p.body[dst].w);
-            uint32_t off = w.taskOffset[task];
-            for (uint32_t done = 0; done < cnt;) {
-                uint32_t m = min(cnt - done, uint32_t(GCHUNK - fill));
-                if (uint32_t(lane) < m) {
-                    uint32_t contact = off + cnt - 1 - (done + lane);
-                    float4 cf = p.contactForce[contact];
-                    float4 pdv = p.pointDepth[contact];
-                    int e = fill + lane;
-                    st[G_FX * GSTRIDE + e] = cf.x; st[G_FY * GSTRIDE + e] = cf.y; st[G_FZ * GSTRIDE + e] = cf.z;
-                    st[G_E * GSTRIDE + e] = cf.w;
-                    st[G_AX * GSTRIDE + e] = pdv.x; st[G_AY * GSTRIDE + e] = pdv.y; st[G_AZ * GSTRIDE + e] = pdv.z;
-                    st[G_DEPTH * GSTRIDE + e] = pdv.w;
-                    st[G_SLIP * GSTRIDE + e] = p.slip[contact].w;
-                    st[G_NLP * GSTRIDE + e] = p.normalLossPower[contact];
-                    st[G_AREA * GSTRIDE + e] = p.c.activeTimestep ? p.normalArea[contact].w : 0.0f;
-                    st[G_SRC * GSTRIDE + e] = source;
-                    st[G_BX * GSTRIDE + e] = branch.x; st[G_BY * GSTRIDE + e] = branch.y; st[G_BZ * GSTRIDE + e] = branch.z;
-                    st[G_MINW * GSTRIDE + e] = minW;
-                }
-                fill += int(m);
-                done += m;
-                if (fill == GCHUNK) { process(fill); fill = 0; }
+    // The grain's pair endpoints: the tasks of its adjacency list in list order (pairs
+    // descending, pass 1 then pass 0), 64 per round, one lane each, so a round reads
+    // every task's count, offset, branch and radii together; then each lane of a chunk
+    // finds its endpoint's task by a search over the running counts.  The endpoint
+    // sequence, and so every sum, is the one of a task-by-task walk.
+    const uint32_t first = w.adjOffset[id], tasksHere = 2 * w.adjCount[id];
+    for (uint32_t s0 = 0; s0 < tasksHere; s0 += WAVE) {
+        const uint32_t s = s0 + uint32_t(lane);
+        uint32_t cnt = 0, off = 0;
+        float source = 0.0f, minW = 0.0f;
+        v3 branch = {0.0f, 0.0f, 0.0f};
+        if (s < tasksHere) {
+            const uint32_t k = tasksHere / 2 - 1 - s / 2, pass = 1 - (s & 1);
+            const uint32_t pair = w.adjList[first + k];
+            const uint2 ij = p.pairs[pair];
+            const uint32_t task = 2 * pair + pass;
+            cnt = w.taskCount[task];
+            if (cnt != 0) {
+                const uint32_t src = pass == 0 ? ij.x : ij.y, dst = pass == 0 ? ij.y : ij.x;
+                source = id == src ? 1.0f : 0.0f;
+                branch = centerBranch(p, src, dst);
+                minW = fminf(p.body[src].w, p.body[dst].w);
+                off = w.taskOffset[task];
             }
+        }
+        uint32_t incl = cnt;  // running count through this lane's task
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const uint32_t o = __shfl_up(incl, d);
+            if (lane >= d) incl += o;
+        }
+        const uint32_t total = __shfl(incl, WAVE - 1);
+        for (uint32_t e0 = 0; e0 < total;) {
+            const uint32_t m = min(total - e0, uint32_t(GCHUNK - fill));
+            const uint32_t e = e0 + uint32_t(lane);
+            uint32_t t = 0;  // the first task lane whose running count exceeds e
+            for (uint32_t step = WAVE / 2; step > 0; step >>= 1)
+                if (__shfl(incl, int(t + step - 1)) <= e) t += step;
+            const uint32_t tIncl = __shfl(incl, int(t)), tCnt = __shfl(cnt, int(t)), tOff = __shfl(off, int(t));
+            const float tSource = __shfl(source, int(t)), tMinW = __shfl(minW, int(t));
+            const v3 tBranch = mk3(__shfl(branch.x, int(t)), __shfl(branch.y, int(t)), __shfl(branch.z, int(t)));
+            if (uint32_t(lane) < m) {
+                const uint32_t i = e - (tIncl - tCnt);  // the endpoint's place in its task, from its first
+                const uint32_t contact = tOff + tCnt - 1 - i;
+                float4 cf = p.contactForce[contact];
+                float4 pdv = p.pointDepth[contact];
+                int at = fill + lane;
+                st[G_FX * GSTRIDE + at] = cf.x; st[G_FY * GSTRIDE + at] = cf.y; st[G_FZ * GSTRIDE + at] = cf.z;
+                st[G_E * GSTRIDE + at] = cf.w;
+                st[G_AX * GSTRIDE + at] = pdv.x; st[G_AY * GSTRIDE + at] = pdv.y; st[G_AZ * GSTRIDE + at] = pdv.z;
+                st[G_DEPTH * GSTRIDE + at] = pdv.w;
+                st[G_SLIP * GSTRIDE + at] = p.slip[contact].w;
+                st[G_NLP * GSTRIDE + at] = p.normalLossPower[contact];
+                st[G_AREA * GSTRIDE + at] = p.c.activeTimestep ? p.normalArea[contact].w : 0.0f;
+                st[G_SRC * GSTRIDE + at] = tSource;
+                st[G_BX * GSTRIDE + at] = tBranch.x; st[G_BY * GSTRIDE + at] = tBranch.y; st[G_BZ * GSTRIDE + at] = tBranch.z;
+                st[G_MINW * GSTRIDE + at] = tMinW;
+            }
+            fill += int(m);
+            e0 += m;
+            if (fill == GCHUNK) { process(fill); fill = 0; }
         }
     }
     if (fill) process(fill);
