@@ -119,6 +119,7 @@ struct W {
                           // first key less the template's first key (the node's row slot base)
     uint4* templateRec;   // TREC rows per template (see k_pack_samples)
     uint32_t* taskFar;    // per task: the source's node count when k_task_far passes it over whole, else 0
+    float4* taskFrame;    // TFRAME rows per task, written by k_task_far for the tasks it does not pass over
     uint32_t* xcd;        // per queue q: count at [q*LINE], fill cursor at [(8+q)*LINE], pop cursor at [(16+q)*LINE]
     uint32_t* stageCount; // staged hits per queue at [q*LINE]; queue q stages into [q*stageCap, (q+1)*stageCap)
     uint32_t* xcdTasks;   // 2P task ids grouped by XCD queue, ascending target template within a queue
@@ -203,6 +204,7 @@ constexpr int COLLECT_WAVES = 4;
 constexpr int QCAP = 256;             // per-wave query queue (node keys)
 constexpr int MAX_CLUSTERS = 1024;    // per source template
 constexpr uint32_t TREC = 8;          // rows of a template record
+constexpr uint32_t TFRAME = 7;        // rows of a task frame record (k_task_far)
 constexpr int STACK = 32;             // BVH traversal stack (reference depth limit)
 constexpr int SCAN_TILE = 1024;       // 256 threads x 4
 // Device counters that many waves update atomically each sit on their own 256-byte
@@ -1504,7 +1506,13 @@ __device__ __forceinline__ bool allLeavesFar(float4 a3, float c0, float c1, floa
 // Before the collection, one thread per task: the tasks it can pass over whole (their
 // anchor valid, the skip on, every leaf far), with the collection's own inputs and
 // FP32 expressions.  About half the tasks of a dense packing; the collection then
-// spends one load on each of them instead of its wave-wide setup.
+// spends one load on each of them instead of its wave-wide setup.  For every other
+// task it writes the frame the collection would compute (one record instead of a
+// chain of dependent loads):
+//   0  i, j, source and target template        1  q_i            2  q_j
+//   3  branch (xyz), s_i                        4  s_j, d_i, d_j, first sample of the source (bits)
+//   5  row task (bits), flags (1 rows, 2 skip, 4 re-anchor by shrinking), c0, c1
+//   6  motion bound of every node, anchored rejection plane (bits)
 __global__ void k_task_far(P p, W w) {
     const uint32_t tasks = 2 * min(p.control[1], p.c.pairCapacity);
     const uint32_t epoch = ld_volatile(w.counters + 10 * LINE);
@@ -1512,25 +1520,43 @@ __global__ void k_task_far(P p, W w) {
         uint32_t far = 0u;
         const uint2 ij = p.pairs[task >> 1];
         const uint32_t r = w.pairRow[task >> 1];
-        if (w.rowOwner[r] == pairKey(ij)) {
-            const uint32_t i = (task & 1) ? ij.y : ij.x, j = (task & 1) ? ij.x : ij.y;
-            const float4* an = w.anchor + 4 * size_t(2u * r + (task & 1u));
+        const uint32_t i = (task & 1) ? ij.y : ij.x, j = (task & 1) ? ij.x : ij.y;
+        const uint32_t si = p.shapeId[i], sj = p.shapeId[j];
+        const float4 bi = p.body[i], bj = p.body[j];
+        const v4 qi = v4of(p.rotation[i]), qj = v4of(p.rotation[j]);
+        const v3 branch = centerBranch(p, i, j);
+        const bool rows = w.rowOwner[r] == pairKey(ij);
+        const uint32_t rowTask = 2u * r + (task & 1u);
+        float c0 = 0.0f, c1 = 0.0f, all = 0.0f;
+        uint32_t flags = rows ? 1u : 0u, plane = 0xffffffffu;
+        if (rows) {
+            const float4* an = w.anchor + 4 * size_t(rowTask);
             const float4 a0 = an[0], a1 = an[1], a2 = an[2], a3 = an[3];
             const bool valid = (__float_as_uint(a3.w) & 0xffu) == 1u && __float_as_uint(a3.x) == i &&
                                __float_as_uint(a3.y) == j && __float_as_uint(a3.z) == epoch;
-            if (valid) {
-                const float4 bi = p.body[i], bj = p.body[j];
-                float c0, c1, all;
-                motionBound(v4of(p.rotation[i]), v4of(p.rotation[j]), centerBranch(p, i, j), bi.x, bj.x, bi.z, a0, a1,
-                            a2, c0, c1, all);
-                if (all + SKIP_MARGIN < w.skipLimit) {
-                    const uint32_t si = p.shapeId[i];
-                    const float widest = __uint_as_float(w.templateRec[TREC * size_t(si) + 6].x);
-                    if (allLeavesFar(a3, c0, c1, widest)) far = p.sampleInfo[si].y;
-                }
+            motionBound(qi, qj, branch, bi.x, bj.x, bi.z, a0, a1, a2, c0, c1, all);
+            const bool skipMode = valid && all + SKIP_MARGIN < w.skipLimit;
+            const bool rebased = valid && !skipMode && w.rebase != 0u && all < 0.25f;
+            flags |= (skipMode ? 2u : 0u) | (rebased ? 4u : 0u);
+            plane = __float_as_uint(a2.w);
+            if (skipMode) {
+                const float widest = __uint_as_float(w.templateRec[TREC * size_t(si) + 6].x);
+                if (allLeavesFar(a3, c0, c1, widest)) far = p.sampleInfo[si].y;
             }
+        } else {
+            w.rowState[5] = 1u;  // a listed pair without its row pair: re-assign at the next collection
         }
         w.taskFar[task] = far;
+        if (far == 0u) {
+            float4* fr = w.taskFrame + TFRAME * size_t(task);
+            fr[0] = make_float4(__uint_as_float(i), __uint_as_float(j), __uint_as_float(si), __uint_as_float(sj));
+            fr[1] = f4(qi);
+            fr[2] = f4(qj);
+            fr[3] = f4(branch, bi.x);
+            fr[4] = make_float4(bj.x, bi.w, bj.w, __uint_as_float(p.sampleInfo[si].x));
+            fr[5] = make_float4(__uint_as_float(rowTask), __uint_as_float(flags), c0, c1);
+            fr[6] = make_float4(all, __uint_as_float(plane), 0.0f, 0.0f);
+        }
     }
 }
 // Rounded toward zero: a lower bound of a positive distance, an upper bound of a
@@ -2378,19 +2404,34 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             }
         }
         if (ld_volatile(p.control) == 0) {
-            uint2 ij = p.pairs[task >> 1];
-            uint32_t pass = task & 1;
-            f.i = pass == 0 ? ij.x : ij.y;
-            f.j = pass == 0 ? ij.y : ij.x;
-            f.si = p.shapeId[f.i];
-            f.sj = p.shapeId[f.j];
+            // The task's frame and row decision: k_task_far's record under the skip
+            // (one row per lane), else from the grains' rows.
+            float4 fr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (w.skip) {
+                fr = w.taskFrame[TFRAME * size_t(task) + min(uint32_t(lane), TFRAME - 1)];
+                auto U = [&](float v, int l) { return uint32_t(__builtin_amdgcn_readlane(__float_as_int(v), l)); };
+                auto F = [&](float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); };
+                f.i = U(fr.x, 0); f.j = U(fr.y, 0); f.si = U(fr.z, 0); f.sj = U(fr.w, 0);
+                f.qi = {F(fr.x, 1), F(fr.y, 1), F(fr.z, 1), F(fr.w, 1)};
+                f.qj = {F(fr.x, 2), F(fr.y, 2), F(fr.z, 2), F(fr.w, 2)};
+                f.branch = {F(fr.x, 3), F(fr.y, 3), F(fr.z, 3)};
+                f.s_i = F(fr.w, 3);
+                f.s_j = F(fr.x, 4); f.wi = F(fr.y, 4); f.wj = F(fr.z, 4); f.sampleBase = U(fr.w, 4);
+            } else {
+                const uint2 ij = p.pairs[task >> 1];
+                const uint32_t pass = task & 1;
+                f.i = pass == 0 ? ij.x : ij.y;
+                f.j = pass == 0 ? ij.y : ij.x;
+                f.si = p.shapeId[f.i];
+                f.sj = p.shapeId[f.j];
+                f.sampleBase = p.sampleInfo[f.si].x;
+                f.qi = v4of(p.rotation[f.i]);
+                f.qj = v4of(p.rotation[f.j]);
+                const float4 bi = p.body[f.i], bj = p.body[f.j];
+                f.s_i = bi.x; f.s_j = bj.x; f.wi = bi.w; f.wj = bj.w;
+                f.branch = centerBranch(p, f.i, f.j);
+            }
             const uint4 trec = w.templateRec[TREC * size_t(f.sj) + (uint32_t(lane) & (TREC - 1))];
-            f.sampleBase = p.sampleInfo[f.si].x;
-            f.qi = v4of(p.rotation[f.i]);
-            f.qj = v4of(p.rotation[f.j]);
-            float4 bi = p.body[f.i], bj = p.body[f.j];
-            f.s_i = bi.x; f.s_j = bj.x; f.wi = bi.w; f.wj = bj.w;
-            f.branch = centerBranch(p, f.i, f.j);
             // Temporal skip.  The node's target-frame point is
             //   x = R_j^T (R_i a - b) / s_j,   a = s_i p,
             // and for unit quaternions |R(q) v - R(q') v| <= 2 |q - q'| |v|, so since
@@ -2416,31 +2457,22 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             // The task's persistent rows, used only while their owner is this pair.
             uint32_t rowTask = task;
             bool rows = false;
-            if (w.skip) {
-                const uint32_t r = w.pairRow[task >> 1];
-                rows = w.rowOwner[r] == pairKey(ij);
-                if (!rows && lane == 0) w.rowState[5] = 1u;
-                rowTask = uniformU(2u * r + (task & 1u));
-                rows = uniformU(rows ? 1u : 0u) != 0u;
+            uint32_t flags = 0u;
+            if (w.skip) {  // rows exist only under the skip
+                rowTask = uniformU(__builtin_amdgcn_readlane(__float_as_int(fr.x), 5));
+                flags = uniformU(__builtin_amdgcn_readlane(__float_as_int(fr.y), 5));
+                rows = (flags & 1u) != 0u;
             }
             uint16_t* warmTri = w.warm && rows ? uniformPtr(w.nodeTri + size_t(rowTask) * w.nodeStride) : nullptr;
             if (rows) {
-                dist = w.nodeDist + size_t(rowTask) * w.nodeStride;
-                const float4* an = w.anchor + 4 * size_t(rowTask);
-                const float4 a0 = an[0], a1 = an[1], a2 = an[2], a3 = an[3];
-                const bool valid = (__float_as_uint(a3.w) & 0xffu) == 1u && __float_as_uint(a3.x) == f.i &&
-                                   __float_as_uint(a3.y) == f.j && __float_as_uint(a3.z) == ld_volatile(w.counters + 10 * LINE);
-                float reachAll;  // bound for every node of the grain
-                motionBound(f.qi, f.qj, f.branch, f.s_i, f.s_j, p.body[f.i].z, a0, a1, a2, c0, c1, reachAll);
-                skipMode = valid && reachAll + SKIP_MARGIN < w.skipLimit;
-                rebased = valid && !skipMode && w.rebase != 0u && reachAll < 0.25f;
-                c0 = uniformF(c0);
-                c1 = uniformF(c1);
-                skipMode = uniformU(skipMode ? 1u : 0u) != 0u;
-                rebased = uniformU(rebased ? 1u : 0u) != 0u;
-                dist = uniformPtr(dist);
+                dist = uniformPtr(w.nodeDist + size_t(rowTask) * w.nodeStride);
+                c0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fr.z), 5));
+                c1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fr.w), 5));
+                const float reachAll = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fr.x), 6));
+                skipMode = (flags & 2u) != 0u;
+                rebased = (flags & 4u) != 0u;
                 if (skipMode) {
-                    anchoredPlane = uniformU(__float_as_uint(a2.w));
+                    anchoredPlane = uniformU(__builtin_amdgcn_readlane(__float_as_int(fr.y), 6));
                 } else if (rebased) {
                     // The anchor has expired: re-anchor at the current poses by moving
                     // every stored bound toward zero by the motion since the old anchor
@@ -4032,6 +4064,7 @@ struct ScratchImpl : Scratch {
     uint32_t maxNodesShapes = 0, maxNodes = 0;
     size_t nodeDistCap = 0;
     uint32_t anchorCap = 0, taskFarCap = 0;
+    size_t taskFrameCap = 0;
     uint32_t* probeWord = nullptr;
     uint32_t* triBase = nullptr;
     uint32_t triBaseCap = 0, maxTriangles = 0, maxClusters = 0;
@@ -4232,6 +4265,12 @@ struct ScratchImpl : Scratch {
             HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.taskFar), tasks * sizeof(uint32_t)));
             taskFarCap = uint32_t(tasks);
             bytes += size_t(taskFarCap) * sizeof(uint32_t);
+            if (w.taskFrame) { (void)hipFree(w.taskFrame); bytes -= size_t(taskFrameCap) * sizeof(float4); }
+            w.taskFrame = nullptr;
+            taskFrameCap = 0;
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.taskFrame), TFRAME * tasks * sizeof(float4)));
+            taskFrameCap = TFRAME * tasks;
+            bytes += taskFrameCap * sizeof(float4);
         }
         if (nodeDistCap < bytesNeeded || w.nodeStride != stride) {
             if (w.nodeDist) { (void)hipFree(w.nodeDist); bytes -= nodeDistCap; }
@@ -4346,7 +4385,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.builtKey); (void)hipFree(w.triPacked);
         (void)hipFree(w.sampleKeyed); (void)hipFree(w.clusterPacked); (void)hipFree(w.templateRec); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
-        (void)hipFree(w.anchor); (void)hipFree(w.taskFar); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
+        (void)hipFree(w.anchor); (void)hipFree(w.taskFar); (void)hipFree(w.taskFrame); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
         (void)hipFree(triBase); (void)hipFree(w.leafDist); (void)hipFree(gapRows); (void)hipFree(lastFrames);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
@@ -5072,6 +5111,7 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     w.nnTemplates = 0;
     w.anchor = nullptr;
     w.taskFar = nullptr;
+    w.taskFrame = nullptr;
     w.nodeDist = nullptr;
     w.nodeTri = nullptr;
     w.leafDist = nullptr;
