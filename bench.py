@@ -307,7 +307,7 @@ def curve(args, log_):
     e.initialize()
     e.advance(args.warmup, rate=w.rate)
     if not args.no_autotune:
-        tuned = autotune.autotune(e, rate=w.rate, steps=6, log=log_)
+        tuned = autotune.autotune(e, rate=w.rate, steps=10, log=log_)
         log_(f"[curve] autotuned launch shape {tuned['best']}")
     done = 0
     while done < args.curve:
@@ -351,7 +351,7 @@ def measure(args, config, log_, *, primary):
     tuned = None
     sdf = not e.mesh_geometry
     if not args.no_autotune and not sdf:
-        tuned = autotune.autotune(e, rate=w.rate, steps=6, log=log_)
+        tuned = autotune.autotune(e, rate=w.rate, steps=10, log=log_)
         log_(f"[bench] autotuned launch shape {tuned['best']} ({tuned['ms_per_step']:.3f} ms/step)")
     start, step0 = e.state(), e.step
     e.restore(start, step0)

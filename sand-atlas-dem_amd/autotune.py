@@ -68,6 +68,11 @@ def autotune(engine, *, rate=(0, 0, 0), steps=3, space=SPACE, cache=None, log=No
     def measure(setting):
         apply(m, setting)
         engine.restore(state, step)
+        # Every trial starts from the same temporal-skip state: no anchors, so the first
+        # (untimed) step anchors every pair at the restored poses.
+        invalidate = getattr(m, "invalidate", None)
+        if invalidate:
+            invalidate()
         engine.advance(1, rate=rate)  # first step after a change may capture a graph
         m.wait()
         t0 = time.perf_counter()
