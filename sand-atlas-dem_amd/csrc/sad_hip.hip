@@ -2594,11 +2594,13 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             const uint4 info = p.meshClusterInfo[f.si];
             int8_t* leafDist = rows ? uniformPtr(w.leafDist + size_t(rowTask) * w.clusterStride) : nullptr;
             uint32_t leafSkipped = 0;  // per lane, summed below
+            uint4 cp0 = make_uint4(0u, 0u, 0u, 0u);  // lane c: cluster c's packed row (the first 64 clusters)
             for (uint32_t base = 0; base < info.y; base += WAVE) {
                 const uint32_t c = base + lane;
                 bool test = false, nearLeaf = false;
                 if (c < info.y) {
                     const uint4 cp = w.clusterPacked[info.x + c];
+                    if (base == 0) cp0 = cp;
                     if (cp.y > 0) {
                         bool farLeaf = false;
                         if (skipMode) {
@@ -2638,12 +2640,18 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                 while (bits != 0) {
                     const uint32_t rel = word * WAVE + uint32_t(__builtin_ctzll(bits));
                     bits &= bits - 1;
-                    const uint4 range = w.clusterPacked[info.x + rel];
+                    const uint4 range =
+                        rel < uint32_t(WAVE)
+                            ? make_uint4(uint32_t(__builtin_amdgcn_readlane(int(cp0.x), int(rel))),
+                                         uint32_t(__builtin_amdgcn_readlane(int(cp0.y), int(rel))), 0u,
+                                         uint32_t(__builtin_amdgcn_readlane(int(cp0.w), int(rel))))
+                            : w.clusterPacked[info.x + rel];
                     bool pass_ = false, test = false;
                     uint32_t node = 0;
                     bool far = false;
                     float nearBound = 1e30f;  // an upper bound of the node's signed distance, when one is known
                     uint32_t slot = 0, cached = 0;  // cached: the slot's warm-start word
+                    uint32_t gapWordNow = 0;        // the slot's gap word, read ahead of the tests
                     v3 local = mk3(0.0f, 0.0f, 0.0f);
                     float4 sp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     if (uint32_t(lane) < range.y) {
@@ -2662,6 +2670,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                             inside = stored < 0.0f;
                             if (stored < 0.0f) nearBound = stored + reach;  // stored rounds toward zero
                         }
+                        if (gapTask && !far) gapWordNow = gc.row[slot];
                         if (inside) {
                             pass_ = true;
                         } else if (!far) {
@@ -2720,7 +2729,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                     if (pass_ && gapTask) {
                         const uint32_t last = cached;
                         if (last != 0u && last <= triCount) {
-                            const uint32_t word = gc.row[slot];
+                            const uint32_t word = gapWordNow;
                             if ((word >> 16) == gc.genPrev) {
                                 const v3 a = xyz(sp) * f.s_i;
                                 const v3 now = qinv(f.qj, qrot(f.qi, a) - f.branch) / f.s_j;  // queryPoint
