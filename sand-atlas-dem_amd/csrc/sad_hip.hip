@@ -98,6 +98,7 @@ struct W {
     uint32_t* taskCount;  // 2P
     uint32_t* taskOffset; // 2P + 1
     uint32_t* adjCount;   // N
+    float4* omegaRow;     // N: each grain's angular velocity for the resolve (omega(), computed once)
     uint32_t* adjOffset;  // N
     uint32_t* adjCursor;  // N
     uint32_t* adjList;    // 2P
@@ -3384,7 +3385,13 @@ __device__ __forceinline__ void addSliding(const P& p, uint32_t mine, uint32_t* 
     __syncthreads();
     if (threadIdx.x == 0 && *total) atomicAdd(p.control + 6, *total);
 }
-__global__ __launch_bounds__(256) void k_resolve(P p) {
+// Each grain's omega() for the resolve, which would otherwise form it for both grains
+// of every contact (same function, same inputs, same bits).
+__global__ void k_omega_rows(P p, W w) {
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < p.c.N; id += gridDim.x * blockDim.x)
+        w.omegaRow[id] = f4(omega(p, id), 0.0f);
+}
+__global__ __launch_bounds__(256) void k_resolve(P p, W w) {
     __shared__ uint32_t slidingTotal;
     // Every lane reaches the workgroup sum below, also after an error flag.
     const uint32_t count = ld_volatile(p.control) != 0 ? 0u : min(p.control[2], p.c.contactCapacity);
@@ -3397,9 +3404,9 @@ __global__ __launch_bounds__(256) void k_resolve(P p) {
         v3 n = xyz(na);
         const bool wall = j >= p.c.N;
         float viscosity = wall ? p.wallMaterial[j - p.c.N].w : p.c.normalViscosity;
-        v3 wi = viscosity > 0 ? xyz(p.forceOmega[i]) : omega(p, i);
+        v3 wi = xyz(viscosity > 0 ? p.forceOmega[i] : w.omegaRow[i]);
         v3 wj = {0, 0, 0};
-        if (!wall) wj = viscosity > 0 ? xyz(p.forceOmega[j]) : omega(p, j);
+        if (!wall) wj = xyz(viscosity > 0 ? p.forceOmega[j] : w.omegaRow[j]);
         v3 elastic = {0, 0, 0};
         if (p.c.historyStep != 0) {
             uint32_t slot = hashKey(p.c.hashSize, key);
@@ -4048,7 +4055,7 @@ struct Caps {
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
     uint32_t stageCount = 0, keyCount = 0, keyStart = 0, wallCount = 0, wallOffset = 0, sdfReach = 0, sdfHits = 0;
     uint32_t sdfLeaves = 0, sdfOrder = 0, builtKey = 0, triPacked = 0, sampleKeyed = 0, clusterPacked = 0;
-    uint32_t templateRec = 0;
+    uint32_t templateRec = 0, omegaRow = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -4170,6 +4177,7 @@ struct ScratchImpl : Scratch {
         if ((rc = grow(w.taskCount, caps.taskCount, tasks, bytes))) return rc;
         if ((rc = grow(w.taskOffset, caps.taskOffset, tasks + 1, bytes))) return rc;
         if ((rc = grow(w.adjCount, caps.adjCount, c.N, bytes))) return rc;
+        if ((rc = grow(w.omegaRow, caps.omegaRow, c.N, bytes))) return rc;
         if ((rc = grow(w.adjOffset, caps.adjOffset, c.N, bytes))) return rc;
         if ((rc = grow(w.adjCursor, caps.adjCursor, c.N, bytes))) return rc;
         if ((rc = grow(w.adjList, caps.adjList, tasks, bytes))) return rc;
@@ -4393,7 +4401,7 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.sdfLeaves); (void)hipFree(w.sdfOrder);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.builtKey); (void)hipFree(w.triPacked);
-        (void)hipFree(w.sampleKeyed); (void)hipFree(w.clusterPacked); (void)hipFree(w.templateRec); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
+        (void)hipFree(w.sampleKeyed); (void)hipFree(w.clusterPacked); (void)hipFree(w.templateRec); (void)hipFree(w.omegaRow); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
         (void)hipFree(w.anchor); (void)hipFree(w.taskFar); (void)hipFree(w.taskFrame); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
         (void)hipFree(triBase); (void)hipFree(w.leafDist); (void)hipFree(gapRows); (void)hipFree(lastFrames);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
@@ -4701,7 +4709,10 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         LAUNCH(sc, KID_contact_total, k_contact_total, dim3(blocks(c.pairCapacity)), T, s, p, w);
         LAUNCH(sc, KID_scatter, k_scatter, dim3(gridStride.x / 8 + 1, 8), T, s, p, w);
     } else if (name == "clearContactStorage") LAUNCH(sc, KID_clear_storage, k_clear_storage, gridStride, T, s, p);
-    else if (name == "resolveContacts") LAUNCH(sc, KID_resolve, k_resolve, gridStride, T, s, p);
+    else if (name == "resolveContacts") {
+        LAUNCH(sc, KID_resolve, k_omega_rows, perGrain, T, s, p, w);
+        LAUNCH(sc, KID_resolve, k_resolve, gridStride, T, s, p, w);
+    }
     else if (name == "buildHistoryHash") LAUNCH(sc, KID_build_hash, k_build_hash, gridStride, T, s, p);
     else if (name == "gather") LAUNCH(sc, KID_gather, k_gather, dim3(blocks(c.N, 4)), T, s, p, w);
     else if (name == "finishStep") LAUNCH(sc, KID_finish_step, k_finish_step, perGrain, T, s, p);
@@ -5083,6 +5094,7 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     take(w.taskCount, tasks, caps.taskCount);
     take(w.taskOffset, tasks + 1, caps.taskOffset);
     take(w.adjCount, c.N, caps.adjCount);
+    take(w.omegaRow, c.N, caps.omegaRow);
     take(w.adjOffset, c.N, caps.adjOffset);
     take(w.adjCursor, c.N, caps.adjCursor);
     take(w.adjList, tasks, caps.adjList);
