@@ -4000,7 +4000,7 @@ thread_local char g_error[256] = "no error";
     X(wide_nodes) X(bucket_reset) X(bucket_count) X(bucket_offsets) X(bucket_fill) \
     X(advance_walls) X(wall_count) X(wall_emit) X(wall_total) X(gather_wall_rows) X(refresh_wall_power) \
     X(diagnostic_rows) X(fabric_rows) X(sdf_count) X(sdf_total) X(sdf_emit) X(rows) X(derived_check) \
-    X(pack_geometry) X(task_far)
+    X(pack_geometry) X(task_far) X(omega_rows)
 enum KernelId {
 #define SAD_ID(name) KID_##name,
     SAD_KERNELS(SAD_ID)
@@ -4710,7 +4710,7 @@ int run_kernel(const std::string& name, P& p, ScratchImpl& sc, hipStream_t s) {
         LAUNCH(sc, KID_scatter, k_scatter, dim3(gridStride.x / 8 + 1, 8), T, s, p, w);
     } else if (name == "clearContactStorage") LAUNCH(sc, KID_clear_storage, k_clear_storage, gridStride, T, s, p);
     else if (name == "resolveContacts") {
-        LAUNCH(sc, KID_resolve, k_omega_rows, perGrain, T, s, p, w);
+        LAUNCH(sc, KID_omega_rows, k_omega_rows, perGrain, T, s, p, w);
         LAUNCH(sc, KID_resolve, k_resolve, gridStride, T, s, p, w);
     }
     else if (name == "buildHistoryHash") LAUNCH(sc, KID_build_hash, k_build_hash, gridStride, T, s, p);
