@@ -134,6 +134,8 @@ struct W {
                           // its template, so a leaf's nodes are contiguous)
     uint16_t* nodeTri;    // nodeStride entries per task (by slot): 1 + the node's last nearest triangle
                           // (relative to its template's first), 0 = none
+    uint16_t* leafPlane;  // clusterStride words per task: a culled leaf's separating plane (tagged), or 0
+    float4* clusterSphere;  // per sample-tree cluster: box centre (xyz) and half diagonal, source frame
     int8_t* leafDist;     // clusterStride bytes per task: per sample-tree leaf, a lower bound of
                           // its nodes' signed distances at the anchor poses
     uint32_t clusterStride;
@@ -651,6 +653,7 @@ __global__ __launch_bounds__(256) void k_pack_samples(P p, W w) {
         const v3 lo = xyz(p.meshClusterMin[c]), hi = xyz(p.meshClusterMax[c]);
         const float radius = length(fmax3(fabs3(lo), fabs3(hi)));
         w.clusterPacked[c] = make_uint4(range.x, range.y, __float_as_uint(radius), range.y ? range.x - keysBase : 0u);
+        w.clusterSphere[c] = f4(0.5f * (lo + hi), 0.5f * length(hi - lo));
         if (range.y > 0) atomicMax(&widest, __float_as_uint(radius));
     }
     __syncthreads();
@@ -1776,7 +1779,8 @@ __device__ __forceinline__ float clusterPad(v3 low, v3 high, v3 branch, float sd
 }
 // meshClusterOutsideFast with the task's hull context.
 __device__ __forceinline__ float clusterOutside(const P& p, const HullCtx& h, v4 qs, v4 qd, float ss, float sd, v3 branch,
-                                uint32_t cluster) {
+                                uint32_t cluster, uint32_t& code) {
+    code = NO_PLANE;
     v3 low, high;
     clusterBox(xyz(p.meshClusterMin[cluster]), xyz(p.meshClusterMax[cluster]), qs, qd, ss, sd, branch, low, high);
     const float pad = clusterPad(low, high, branch, sd);
@@ -1784,16 +1788,21 @@ __device__ __forceinline__ float clusterOutside(const P& p, const HullCtx& h, v4
     high = high + mk3(pad, pad, pad);
     if (boxOutsideRoot(h, low, high)) return boxRootExcess(h, low, high);
     const v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
-    const uint2 r = h.range[dominantGroup((center - h.mid()) / h.ext())];
+    const uint32_t g = dominantGroup((center - h.mid()) / h.ext());
+    const uint2 r = h.range[g];
     for (uint32_t k = 0; k < r.y; k += PLANE_BATCH) {
         float4 pl[PLANE_BATCH];
 #pragma unroll
         for (uint32_t j = 0; j < PLANE_BATCH; j++) pl[j] = p.meshPlanes[r.x + min(k + j, r.y - 1)];
         float worst = 0.0f;
+        uint32_t best = NO_PLANE;
 #pragma unroll
-        for (uint32_t j = 0; j < PLANE_BATCH; j++)
-            worst = fmaxf(worst, dot(xyz(pl[j]), center) + pl[j].w - dot(fabs3(xyz(pl[j])), half));
-        if (worst > 0) return worst;
+        for (uint32_t j = 0; j < PLANE_BATCH; j++) {
+            const float e = dot(xyz(pl[j]), center) + pl[j].w - dot(fabs3(xyz(pl[j])), half);
+            if (e > worst) { worst = e; best = planeCode(g, min(k + j, r.y - 1)); }
+            worst = fmaxf(worst, e);
+        }
+        if (worst > 0) { code = best; return worst; }
     }
     return 0.0f;
 }
@@ -1802,7 +1811,7 @@ __device__ __forceinline__ float clusterOutside(const P& p, const HullCtx& h, v4
 // (reduced as meshClusterOutsideWave does) before the group tests its planes.
 // Lane c of `tests` is cluster clusterBase + c.
 __device__ __forceinline__ float leavesOutsideWave(const P& p, const HullCtx& h, uint64_t tests, uint32_t clusterBase, v4 qs,
-                                   v4 qd, float ss, float sd, v3 branch, int lane) {
+                                   v4 qd, float ss, float sd, v3 branch, int lane, uint32_t& code) {
     const uint32_t n = uint32_t(__popcll(tests));
     const uint32_t shift = n <= 1 ? 6u : n <= 2 ? 5u : n <= 4 ? 4u : 3u;
     const uint32_t span = 1u << shift, group = uint32_t(lane) >> shift, sub = uint32_t(lane) & (span - 1u);
@@ -1827,20 +1836,30 @@ __device__ __forceinline__ float leavesOutsideWave(const P& p, const HullCtx& h,
     low = low - mk3(pad, pad, pad);
     high = high + mk3(pad, pad, pad);
     float worst = 0.0f;
+    uint32_t best = NO_PLANE;
     if (src >= 0) {
         if (boxOutsideRoot(h, low, high)) {
             worst = boxRootExcess(h, low, high);
         } else {
             const v3 center = 0.5f * (low + high), half = 0.5f * (high - low);
-            const uint2 r = h.range[dominantGroup((center - h.mid()) / h.ext())];
+            const uint32_t g = dominantGroup((center - h.mid()) / h.ext());
+            const uint2 r = h.range[g];
             for (uint32_t k = sub; k < r.y; k += span) {
                 const float4 pl = p.meshPlanes[r.x + k];
-                worst = fmaxf(worst, dot(xyz(pl), center) + pl.w - dot(fabs3(xyz(pl)), half));
+                const float e = dot(xyz(pl), center) + pl.w - dot(fabs3(xyz(pl)), half);
+                if (e > worst) { worst = e; best = planeCode(g, k); }
+                worst = fmaxf(worst, e);
             }
         }
     }
-    for (uint32_t d = 1; d < span; d <<= 1) worst = fmaxf(worst, __shfl_xor(worst, int(d)));
+    for (uint32_t d = 1; d < span; d <<= 1) {
+        const float ow = __shfl_xor(worst, int(d));
+        const uint32_t ob = __shfl_xor(best, int(d));
+        if (ow > worst) best = ob;
+        worst = fmaxf(worst, ow);
+    }
     const uint32_t rank = uint32_t(__popcll(tests & lanes_below(lane)));
+    code = __shfl(best, int(rank << shift));
     return __shfl(worst, int(rank << shift));
 }
 // A wave's query batch.  Under the temporal skip a task queues about 10-16 nodes,
@@ -2594,6 +2613,7 @@ __device__ __forceinline__ void collectBody(P p, W w) {
             // records the cull's evidence for each rejected leaf.
             const uint4 info = p.meshClusterInfo[f.si];
             int8_t* leafDist = rows ? uniformPtr(w.leafDist + size_t(rowTask) * w.clusterStride) : nullptr;
+            uint16_t* leafPlane = rows && w.leafPlane ? uniformPtr(w.leafPlane + size_t(rowTask) * w.clusterStride) : nullptr;
             uint32_t leafSkipped = 0;  // per lane, summed below
             uint4 cp0 = make_uint4(0u, 0u, 0u, 0u);  // lane c: cluster c's packed row (the first 64 clusters)
             for (uint32_t base = 0; base < info.y; base += WAVE) {
@@ -2613,18 +2633,37 @@ __device__ __forceinline__ void collectBody(P p, W w) {
                             nearLeaf = !farLeaf && leafDist[c] < 0;
                         }
                         test = !farLeaf && !nearLeaf;
+                        // The leaf's cached separating plane: its box lies more than the
+                        // skip margin outside that hull plane (bounded through the box's
+                        // circumscribed sphere), so none of its nodes can be a contact.
+                        if (test && leafPlane) {
+                            const uint32_t cached = leafPlane[c];
+                            const uint32_t g = (cached >> 12) & 7u, k = cached & 0xfffu;
+                            if ((cached & PLANE_TAG) != 0u && g < 6u && k < hc.range[g].y) {
+                                const float4 pl = p.meshPlanes[hc.range[g].x + k];
+                                const float4 sph = w.clusterSphere[info.x + c];
+                                const v3 cc = qinv(f.qj, qrot(f.qi, xyz(sph) * f.s_i) - f.branch) / f.s_j;
+                                const float e = dot(xyz(pl), cc) + pl.w - (sph.w * f.s_i / f.s_j * 1.0001f + 1e-6f);
+                                if (e > SKIP_MARGIN) {
+                                    test = false;
+                                    if (refreshDist) leafDist[c] = quantizeDistance(e);
+                                }
+                            }
+                        }
                     }
                 }
                 const uint64_t tests = __ballot(test);
                 uint64_t m = __ballot(nearLeaf);
                 if (tests != 0) {
                     float out = 0.0f;
+                    uint32_t code = NO_PLANE;
                     if (uint32_t(__popcll(tests)) <= COOP_LEAVES)
-                        out = leavesOutsideWave(p, hc, tests, info.x + base, f.qi, f.qj, f.s_i, f.s_j, f.branch, lane);
+                        out = leavesOutsideWave(p, hc, tests, info.x + base, f.qi, f.qj, f.s_i, f.s_j, f.branch, lane, code);
                     else if (test)
-                        out = clusterOutside(p, hc, f.qi, f.qj, f.s_i, f.s_j, f.branch, info.x + c);
+                        out = clusterOutside(p, hc, f.qi, f.qj, f.s_i, f.s_j, f.branch, info.x + c, code);
                     const bool ok = test && !(out > 0.0f);
                     if (test && !ok && refreshDist) leafDist[c] = quantizeDistance(out);
+                    if (test && !ok && leafPlane && code != NO_PLANE) leafPlane[c] = uint16_t(code);
                     m |= __ballot(ok);
                 }
                 if (lane == 0) cand[base / WAVE] = m;
@@ -4055,7 +4094,7 @@ struct Caps {
     uint32_t sums = 0, stageKey = 0, stageMeta = 0, stagePd = 0, stageNa = 0, xcd = 0, xcdTasks = 0, wide = 0;
     uint32_t stageCount = 0, keyCount = 0, keyStart = 0, wallCount = 0, wallOffset = 0, sdfReach = 0, sdfHits = 0;
     uint32_t sdfLeaves = 0, sdfOrder = 0, builtKey = 0, triPacked = 0, sampleKeyed = 0, clusterPacked = 0;
-    uint32_t templateRec = 0, omegaRow = 0;
+    uint32_t templateRec = 0, omegaRow = 0, clusterSphere = 0;
 };
 
 struct ScratchImpl : Scratch {
@@ -4084,7 +4123,7 @@ struct ScratchImpl : Scratch {
     uint32_t* probeWord = nullptr;
     uint32_t* triBase = nullptr;
     uint32_t triBaseCap = 0, maxTriangles = 0, maxClusters = 0;
-    size_t leafDistCap = 0;
+    size_t leafDistCap = 0, leafPlaneCap = 0;
     // Unchanged-winner shortcut rows (sad_hip_tune "gap_shortcut", default on).
     int gapShortcut = 1;
     uint32_t* gapRows = nullptr;
@@ -4106,6 +4145,7 @@ struct ScratchImpl : Scratch {
         int rc = grow(w.triPacked, caps.triPacked, 3 * size_t(std::max(p.triangles, 1u)), bytes);
         if (rc || (rc = grow(w.sampleKeyed, caps.sampleKeyed, std::max(p.sampleKeys, 1u), bytes))) return rc;
         if ((rc = grow(w.clusterPacked, caps.clusterPacked, std::max(p.clusters, 1u), bytes))) return rc;
+        if ((rc = grow(w.clusterSphere, caps.clusterSphere, std::max(p.clusters, 1u), bytes))) return rc;
         return grow(w.templateRec, caps.templateRec, TREC * size_t(std::max(p.c.shapes, 1u)), bytes);
     }
     // SDF geometry: per-template reach and the hit-bitmask rows, refreshed whenever
@@ -4338,6 +4378,13 @@ struct ScratchImpl : Scratch {
             HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.leafDist), tasks * clusterStride));
             leafDistCap = tasks * clusterStride;
             bytes += leafDistCap;
+            if (w.leafPlane) { (void)hipFree(w.leafPlane); bytes -= 2 * leafPlaneCap; }
+            w.leafPlane = nullptr;
+            leafPlaneCap = 0;
+            HIP_OK(hipMalloc(reinterpret_cast<void**>(&w.leafPlane), 2 * tasks * clusterStride));
+            HIP_OK(hipMemset(w.leafPlane, 0, 2 * tasks * clusterStride));
+            leafPlaneCap = tasks * clusterStride;
+            bytes += 2 * leafPlaneCap;
             epoch++;
         }
         w.clusterStride = clusterStride;
@@ -4401,9 +4448,9 @@ struct ScratchImpl : Scratch {
         (void)hipFree(w.sdfLeaves); (void)hipFree(w.sdfOrder);
         (void)hipFree(w.xcd); (void)hipFree(w.xcdTasks); (void)hipFree(w.wide); (void)hipFree(w.stageCount); (void)hipFree(w.nnStats);
         (void)hipFree(w.keyCount); (void)hipFree(w.keyStart); (void)hipFree(w.builtKey); (void)hipFree(w.triPacked);
-        (void)hipFree(w.sampleKeyed); (void)hipFree(w.clusterPacked); (void)hipFree(w.templateRec); (void)hipFree(w.omegaRow); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
+        (void)hipFree(w.sampleKeyed); (void)hipFree(w.clusterPacked); (void)hipFree(w.clusterSphere); (void)hipFree(w.templateRec); (void)hipFree(w.omegaRow); (void)hipFree(w.wallCount); (void)hipFree(w.wallOffset);
         (void)hipFree(w.anchor); (void)hipFree(w.taskFar); (void)hipFree(w.taskFrame); (void)hipFree(w.nodeDist); (void)hipFree(w.nodeTri); (void)hipFree(probeWord);
-        (void)hipFree(triBase); (void)hipFree(w.leafDist); (void)hipFree(gapRows); (void)hipFree(lastFrames);
+        (void)hipFree(triBase); (void)hipFree(w.leafDist); (void)hipFree(w.leafPlane); (void)hipFree(gapRows); (void)hipFree(lastFrames);
         (void)hipFree(w.stageKey); (void)hipFree(w.stageMeta); (void)hipFree(w.stagePd); (void)hipFree(w.stageNa);
     }
 };
@@ -5107,6 +5154,7 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     take(w.triPacked, 3 * size_t(p.triangles), caps.triPacked);
     take(w.sampleKeyed, p.sampleKeys, caps.sampleKeyed);
     take(w.clusterPacked, p.clusters, caps.clusterPacked);
+    take(w.clusterSphere, p.clusters, caps.clusterSphere);
     take(w.templateRec, TREC * size_t(p.c.shapes), caps.templateRec);
     take(w.stageKey, size_t(8) * c.contactCapacity, caps.stageKey);
     take(w.stageMeta, size_t(8) * c.contactCapacity, caps.stageMeta);
@@ -5136,6 +5184,7 @@ size_t carve_static(ScratchImpl& sc, unsigned char* base, const P& p, size_t& he
     w.nodeDist = nullptr;
     w.nodeTri = nullptr;
     w.leafDist = nullptr;
+    w.leafPlane = nullptr;
     w.triBase = nullptr;
     w.nodeGap = nullptr;
     w.lastFrame = nullptr;
