@@ -62,7 +62,8 @@ def load_package():
 
 
 def kernel_bytes(name, n, pd, q, c, hash_size, sdf_static=0):
-    """Algorithmic HBM bytes per launch (per-unit figures in DESIGN.md)."""
+    """Algorithmic HBM bytes per launch: per-unit figures (bytes per grain, directed pair, queued
+    node and contact) times the launch's units."""
     table = {
         "k_collect": 92 * pd + 20 * q + 48 * c,
         "k_resolve": 128 * c,

@@ -1,7 +1,7 @@
 """HIP backend vs the oracle on identical seeded inputs (needs an MI355X).
 
-Gates: neighbour lists, contact key sets and order, and
-history mapping bit-exact; FP32 state, forces and torques equal to the oracle.
+Gates: neighbour lists, contact key sets and order, and history mapping
+bit-exact; FP32 state, forces and torques equal to the oracle.
 The kernels reproduce the oracle's FP32 operation order, so the comparisons below
 are exact (value equality, +0 == -0); where a comparison is made at a tolerance it
 is written next to it.
