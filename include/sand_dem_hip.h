@@ -191,7 +191,7 @@ int sad_hip_reduce_kernel(const char* kernel, uint32_t n, uint32_t walls, uint32
  * default 96: queued traversals that end such a batch early), "gap_shortcut" (0/1,
  * default 1, with "temporal_skip": a query whose node moved less than half its stored
  * nearest-triangle gap since the previous collection takes the previous nearest
- * triangle without a traversal), "gap_reach_micro" (0..100000, default 0: a traversal
+ * triangle without a traversal), "gap_reach_micro" (0..100000, default 300: a traversal
  * also visits what lies within this many 1e-6 template units beyond its bound, so the
  * gap it stores is the true one up to that reach), "temporal_skip" (0/1, default 0: pass
  * over nodes whose signed distance at an anchor pose exceeds a rigorous bound on

@@ -4106,7 +4106,7 @@ struct ScratchImpl : Scratch {
     uint32_t leafBatch = 32;
     uint32_t batchTasks = 16;
     uint32_t batchFill = 96;
-    int gapReachMicro = 0;
+    int gapReachMicro = 300;  // best of 0-10000 over 1,000-step runs at dense; 1000 is best at uneven
     // Temporal node skip (sad_hip_tune "temporal_skip", "skip_limit_micro"): off by
     // default; the caller that enables it calls sad_hip_invalidate whenever it
     // rewrites template geometry, body scales or shape ids in place.
