@@ -195,7 +195,7 @@ int sad_hip_reduce_kernel(const char* kernel, uint32_t n, uint32_t walls, uint32
  * also visits what lies within this many 1e-6 template units beyond its bound, so the
  * gap it stores is the true one up to that reach), "temporal_skip" (0/1, default 0: pass
  * over nodes whose signed distance at an anchor pose exceeds a rigorous bound on
- * their motion since), "skip_limit_micro" (2000..250000, default 12000: re-anchor a
+ * their motion since), "skip_limit_micro" (2000..250000, default 20000: re-anchor a
  * directed pair once its motion bound reaches this many 1e-6 template units), "skip_rebase" (0/1,
  * default 0: re-anchor an expired pair by moving its stored bounds toward zero by its
  * motion bound instead of recomputing them).  value < 0 only

@@ -4111,7 +4111,7 @@ struct ScratchImpl : Scratch {
     // default; the caller that enables it calls sad_hip_invalidate whenever it
     // rewrites template geometry, body scales or shape ids in place.
     int skip = 0;
-    int skipLimitMicro = 12000;  // re-anchor distance: best of 3000-20000 over 2,000-step runs (dense, uneven)
+    int skipLimitMicro = 20000;  // re-anchor distance: best of 2000-40000 over 2,000-step runs at uneven
     int rebase = 0;
     uint32_t epoch = 1, epochWritten = 0;  // anchors carry the epoch they were taken in
     std::vector<uint64_t> lastTable;        // binding table of the previous call
