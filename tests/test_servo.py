@@ -179,3 +179,19 @@ def test_servo_drives_the_oracle_layer(pkg, oracle_module):
         assert np.allclose(r["cell_log_rate"], rate, rtol=1e-6)  # the rates the step ran at (FP32)
         assert r["layer_gap"] < before["layer_gap"]
         assert len(r["walls"]) == 2
+
+
+def test_wall_and_strain_stages_pass_their_settings_to_the_step():
+    """A wall stage hands its plane velocities to every interval of its steps and runs no
+    cell strain; a strain stage runs its fixed rates; the last interval is cut to the
+    stage's remaining steps."""
+    e = LinearCell(layer=True)
+    wall = dict(kind="wall", steps=25, wall_velocities=[[0, 0, 0], [0, 0, -1e-3]], drag=2.0)
+    strain = dict(kind="strain", steps=10, rate=[0.5, -0.5, 0.0])
+    run = Stages(e, [wall, strain])
+    assert run.run() == "complete"
+    assert [c[0] for c in e.calls] == [10, 10, 5, 10]
+    for c in e.calls[:3]:
+        assert c[1] == (0.0, 0.0, 0.0) and c[2] == 2.0 and c[3] == wall["wall_velocities"]
+    assert e.calls[3][1] == (0.5, -0.5, 0.0) and e.calls[3][3] is None
+    assert [r["stage"] for r in run.rows[1:]] == [0, 0, 0, 1]
